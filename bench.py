@@ -1,0 +1,195 @@
+"""Benchmark: solved right-hand sides per second at relative residual 1e-8.
+
+One step = one pass of the hot path for one right-hand side on one GPU:
+assemble the dense Burton–Miller matrix for the wavenumber (far, near and
+self kernels), add the plane-wave right-hand side, and run GMRES to a
+recomputed relative residual of 1e-8 — all with the mesh and its
+frequency-invariant near-pair routes already resident in HBM (the
+workspace is built once per mesh, as the reference's resident CUDA workspace
+is; its build time is reported as ``setup_ms``).
+
+Default workload: rigid sphere, icosphere with 20480 panels, k = 10,
+beta = i/k, one plane wave along +z, GMRES(100), rtol 1e-8, FP64.
+
+Printed: one JSON line with the metric, the per-kernel roofline of the
+dominant kernel (timed with HIP events on the library's stream), and the CPU
+baseline (the oracle's C++ restatement of the reference algorithm timed on a
+sample of rows on this host's cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import sonium_amd as sa  # noqa: E402
+from sonium_amd.mesh import icosphere  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FLOPS_PER_EVAL = 70      # FP64 flops per kernel evaluation (FMA = 2, sqrt/div/sincos = 1)
+FLOPS_PER_COEFF = 14     # coefficient H*gamma*tau + E*beta per pair
+
+WORKLOADS = {
+    "c2": {"subdiv": 5, "k": 10.0, "restart": 100,
+           "name": "rigid sphere icosphere 20480 panels, k=10, beta=i/k, 1 plane wave +z, GMRES(100) rtol 1e-8, complex128"},
+    "c1": {"subdiv": 3, "k": 2.0, "restart": 50,
+           "name": "rigid sphere icosphere 1280 panels, k=2, beta=i/k, 1 plane wave +z, GMRES(50) rtol 1e-8, complex128"},
+}
+
+
+def cpu_baseline(m, k, beta, spmv_count, rows, threads):
+    """Oracle (C++ restatement of the reference CPU path) on `rows` rows; extrapolated to N."""
+    import oracle
+
+    n = m.num_elements
+    rows = min(rows, n)
+    t0 = time.perf_counter()
+    A_rows, _ = oracle.build_tbem_system(m.nodes, m.triangles, k, beta, rows=(0, rows), threads=threads)
+    t_asm = time.perf_counter() - t0
+    x = np.ones(n, dtype=np.complex128)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.zgemv(A_rows, x, threads=threads)
+    t_mv = (time.perf_counter() - t0) / reps
+    scale = n / rows
+    t_solve = spmv_count * t_mv * scale
+    t_total = t_asm * scale + t_solve
+    return {
+        "value": 1.0 / t_total,
+        "unit": "RHS/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle assembly of rows [0,{rows}) of {n} and {reps} matvecs of that row block, "
+                   f"extrapolated x{scale:.1f}; solve = {spmv_count} matvecs (the GPU run's count); "
+                   f"assembly {t_asm * scale:.1f} s + solve {t_solve:.1f} s per RHS"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--cpu-rows", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU row sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+    wl = WORKLOADS[args.workload]
+    ctx = sa.Context(0)
+    m = icosphere(1.0, wl["subdiv"])
+    n = m.num_elements
+    k = wl["k"]
+    ph = sa.PhysicsParams.from_wavenumber(k)
+    beta = ph.burton_miller_beta()
+    t0 = time.perf_counter()
+    tm = sa.TbemMesh(ctx, m)
+    ws = sa.TbemAssemblyWorkspace(ctx, tm)
+    ctx.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    st = ws.stats
+    system = sa.TbemSystem(ctx, n)
+    op = sa.DenseMatrixOperator(system)
+    x = sa.DeviceVector(ctx, n)
+    inc = sa.IncidentField.plane_wave_z()
+    cfg = sa.GmresConfig(max_iterations=100, restart=wl["restart"], tolerance=1e-8)
+    _, _, rhs_ptr = system.device_pointers()
+
+    def step():
+        ws.assemble(ph, beta, system)
+        inc.add_to_system(system, tm, ph, beta, sign=1.0)
+        return sa.gmres_device(op, rhs_ptr, x, cfg)
+
+    for _ in range(args.warmup):
+        sol = step()
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sols = [step() for _ in range(args.steps)]
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    sol = sols[-1]
+    if not all(s.converged and s.residual <= 1e-8 for s in sols):
+        raise SystemExit(f"bench: a timed solve did not converge: {[(s.status, s.residual) for s in sols]}")
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.steps / elapsed
+    stats = {name: ctx.kernel_stats(name) for name in sa._ffi.KERNEL_CLASSES}
+
+    far_launches, far_ms = stats["assemble_far"]
+    mv_launches, mv_ms = stats["matvec"]
+    far_pairs = n * n - n - st["near_pairs"]
+    far_flops = far_pairs * (13 * FLOPS_PER_EVAL + FLOPS_PER_COEFF)
+    far_avg_s = far_ms / max(far_launches, 1) / 1e3
+    mv_bytes = 16 * n * n + 32 * n
+    mv_avg_s = mv_ms / max(mv_launches, 1) / 1e3
+    kernels = {
+        "assemble_far": {"launches": far_launches, "avg_ms": far_avg_s * 1e3,
+                         "tflops": far_flops / far_avg_s / 1e12, "flops_per_launch": far_flops},
+        "matvec": {"launches": mv_launches, "avg_ms": mv_avg_s * 1e3,
+                   "gbs": mv_bytes / mv_avg_s / 1e9, "bytes_per_launch": mv_bytes},
+        "assemble_near_ms": stats["assemble_near"][1] / max(stats["assemble_near"][0], 1),
+        "assemble_self_ms": stats["assemble_self"][1] / max(stats["assemble_self"][0], 1),
+        "krylov_vector_ms_total": stats["krylov_vector"][1] / args.steps,
+        "rhs_ms": stats["rhs"][1] / max(stats["rhs"][0], 1),
+    }
+    if mv_ms >= far_ms:
+        achieved = mv_bytes / mv_avg_s / 1e9
+        roofline = {"kernel": "zgemv_n_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        achieved = far_flops / far_avg_s / 1e12
+        roofline = {"kernel": "far_kernel", "bound": "fp64-valu", "achieved": achieved,
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                    "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        cpu = cpu_baseline(m, k, beta, sol.spmv_count, args.cpu_rows, threads)
+
+    line = {
+        "metric": "solved RHS/s at rel. residual 1e-8 (plain FP64 on GPU)",
+        "value": value,
+        "unit": "RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated icosphere mesh, analytic plane wave)",
+        "config": {"workload": wl["name"], "panels": n, "k": k, "form": "plain FP64",
+                   "parallelism": "single GPU"},
+        "solve": {"iterations": sol.iterations, "restarts": sol.restarts, "spmv_count": sol.spmv_count,
+                  "residual": sol.residual},
+        "setup_ms": setup_ms,
+        "routes": st,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,275 @@
+/*
+ * sbh.h — C ABI of sonium_amd: the dense Burton–Miller Helmholtz BEM path
+ * (constant-panel collocation, Tri3) on one AMD Instinct MI355X (gfx950).
+ *
+ * Every entry point is a drop-in for one reference interface; each
+ * declaration cites the reference item it replaces (paths relative to the
+ * reference repository root).  Conventions:
+ *   - complex numbers are interleaved {re, im} doubles (sbh_c64), the layout
+ *     of num_complex::Complex64 (#[repr(C)]);
+ *   - matrices are row-major; a block of k right-hand sides is an n x k
+ *     row-major array, one right-hand side per column
+ *     (crates/sonium-solvers/crates/sonium-solvers-core/src/traits.rs:2989-3000);
+ *   - every function returns an sbh_status; on failure sbh_last_error()
+ *     returns a message for the calling thread.  No function aborts;
+ *   - the caller owns host arrays; the library owns device objects, which are
+ *     released by the matching *_destroy call;
+ *   - pointers tagged SBH_MEM_DEVICE are HIP device pointers on the
+ *     context's device, SBH_MEM_HOST ordinary host memory.
+ */
+#ifndef SBH_H
+#define SBH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double re, im;
+} sbh_c64;
+
+typedef enum {
+    SBH_OK = 0,
+    SBH_ERR_INVALID_ARGUMENT = 1, /* CudaTbemAssemblyError::InvalidInput, tbem.rs:3693-3718 */
+    SBH_ERR_UNSUPPORTED = 2,      /* CudaTbemAssemblyError::UnsupportedElement */
+    SBH_ERR_BACKEND = 3,          /* CudaTbemAssemblyError::Backend / BackendError, sonium-compute/src/error.rs:42-102 */
+    SBH_ERR_OUT_OF_MEMORY = 4,
+    SBH_ERR_NUMERICAL = 5         /* SolverStatus::NumericalError, gmres.rs:252-275 */
+} sbh_status;
+
+typedef enum { SBH_MEM_HOST = 0, SBH_MEM_DEVICE = 1 } sbh_memory;
+
+typedef struct sbh_context sbh_context;
+typedef struct sbh_mesh sbh_mesh;
+typedef struct sbh_workspace sbh_workspace;
+typedef struct sbh_system sbh_system;
+typedef struct sbh_operator sbh_operator;
+typedef struct sbh_precond sbh_precond;
+typedef struct sbh_vector sbh_vector;
+
+/* Thread-local message of the last failed call on this thread. */
+const char* sbh_last_error(void);
+/* Library version string. */
+const char* sbh_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Physics parameters — mirrors PhysicsParams
+ * (crates/sonium-bem/src/core/types.rs:190-218, 306-487) under
+ * ACOUSTIC_CONVENTION (crates/sonium-bem/src/core/conventions.rs:308-313). */
+typedef struct {
+    double frequency;
+    double speed_of_sound;
+    double density;
+    double wave_number;
+    double omega;
+    double wave_length;
+    double harmonic_factor; /* spatial sign, +1: G = exp(ikr)/(4 pi r) */
+    double pressure_factor; /* density * omega * harmonic_factor */
+    double tau;             /* +1 exterior, -1 interior */
+} sbh_physics;
+
+/* PhysicsParams::new_with_convention (types.rs:190-218). */
+sbh_status sbh_physics_new(double frequency, double speed_of_sound, double density, int is_internal,
+                           sbh_physics* out);
+/* Same parameters from an exact wavenumber (frequency = k c / 2 pi); lets a
+ * caller fix k bit-for-bit, as the reference's tests do by choosing f. */
+sbh_status sbh_physics_from_wavenumber(double wave_number, double speed_of_sound, double density,
+                                       int is_internal, sbh_physics* out);
+/* PhysicsParams::burton_miller_beta_scaled (types.rs:418-424); scale 1 is
+ * burton_miller_beta (types.rs:314-320). */
+sbh_c64 sbh_burton_miller_beta_scaled(const sbh_physics* physics, double scale);
+
+/* ---------------------------------------------------------------------------
+ * Context: one HIP device, one stream, optional kernel timing. */
+sbh_status sbh_context_create(int device, sbh_context** out);
+void sbh_context_destroy(sbh_context* ctx);
+sbh_status sbh_context_synchronize(sbh_context* ctx);
+/* Kernel timing with HIP events on the context stream (off by default). */
+sbh_status sbh_context_set_profiling(sbh_context* ctx, int enabled);
+typedef enum {
+    SBH_KERNEL_ASSEMBLE_FAR = 0,
+    SBH_KERNEL_ASSEMBLE_NEAR = 1,
+    SBH_KERNEL_ASSEMBLE_SELF = 2,
+    SBH_KERNEL_RHS = 3,
+    SBH_KERNEL_MATVEC = 4,
+    SBH_KERNEL_KRYLOV_VECTOR = 5,
+    SBH_KERNEL_BLOCK = 6,
+    SBH_KERNEL_CLASS_COUNT = 7
+} sbh_kernel_class;
+/* Synchronizes, then reports launches and summed event time of one class
+ * since the last reset. */
+sbh_status sbh_context_kernel_stats(sbh_context* ctx, int kernel_class, int64_t* launches,
+                                    double* total_ms);
+sbh_status sbh_context_reset_stats(sbh_context* ctx);
+
+/* Device-resident complex vector (or n x k block) owned by the library, so
+ * callers can keep right-hand sides and solutions in HBM between calls. */
+sbh_status sbh_vector_create(sbh_context* ctx, int64_t length, sbh_vector** out);
+void sbh_vector_destroy(sbh_vector* v);
+int64_t sbh_vector_length(const sbh_vector* v);
+sbh_c64* sbh_vector_data(sbh_vector* v); /* device pointer */
+sbh_status sbh_vector_upload(sbh_vector* v, const sbh_c64* host, int64_t offset, int64_t count);
+sbh_status sbh_vector_download(const sbh_vector* v, sbh_c64* host, int64_t offset, int64_t count);
+
+/* ---------------------------------------------------------------------------
+ * Mesh: nodes + Tri3 elements + boundary conditions, host description of
+ * `elements: &[Element], nodes: &[[f64;3]]` (types.rs:582-719).  Element
+ * centre, area and outward normal are derived as compute_element_geometry
+ * does (crates/sonium-bem/src/core/mesh/generators.rs:779-870).  DOF numbering
+ * is the element order (dof_addresses = [i], as prepare_elements assigns for a
+ * rigid body, bem_solver.rs:4564-4570).
+ *   bc_type[i]   0 = Velocity, 1 = Pressure (get_bc_type_and_value, assembly/common.rs:273-307); NULL = all 0
+ *   bc_value[i]  the panel's (average) BC value; NULL = all zero (rigid)
+ *   admittance[i], has_admittance[i]  optional Robin admittance; NULL = none */
+sbh_status sbh_mesh_create(sbh_context* ctx, const double* nodes, int64_t num_nodes,
+                           const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
+                           const sbh_c64* bc_value, const sbh_c64* admittance,
+                           const uint8_t* has_admittance, sbh_mesh** out);
+void sbh_mesh_destroy(sbh_mesh* mesh);
+int64_t sbh_mesh_num_elements(const sbh_mesh* mesh);
+/* Host copies of centres, outward normals (N x 3) and areas (N); any may be NULL. */
+sbh_status sbh_mesh_geometry(const sbh_mesh* mesh, double* centers, double* normals, double* areas);
+
+/* ---------------------------------------------------------------------------
+ * Resident assembly workspace — replaces CudaTbemAssemblyWorkspace::new
+ * (crates/sonium-bem/src/core/assembly/tbem.rs:3779-4022): uploads the
+ * geometry once and builds the frequency-invariant routes (near pairs and
+ * their generate_subelements trees, singular.rs:1372-1528).  route_threads
+ * host threads build the routes (0 = hardware concurrency). */
+sbh_status sbh_workspace_create(sbh_context* ctx, const sbh_mesh* mesh, int route_threads,
+                                sbh_workspace** out);
+void sbh_workspace_destroy(sbh_workspace* ws);
+/* Route statistics: near pairs, subdivision leaves, host build time. */
+sbh_status sbh_workspace_stats(const sbh_workspace* ws, int64_t* near_pairs, int64_t* leaves,
+                               double* build_ms);
+
+/* Dense system on the device: row-major N x N matrix + N right-hand side
+ * (TbemSystem, tbem.rs:134-141). */
+sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** out);
+void sbh_system_destroy(sbh_system* sys);
+int64_t sbh_system_num_dofs(const sbh_system* sys);
+/* Device pointers of the matrix (leading dimension in *lda, in elements) and rhs. */
+sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t* lda,
+                                      sbh_c64** rhs);
+/* Copies rows [row_begin, row_end) of the matrix (dense, N per row) and the
+ * rhs to host memory; either destination may be NULL. */
+sbh_status sbh_system_download(sbh_system* sys, int64_t row_begin, int64_t row_end,
+                               sbh_c64* matrix_rows, sbh_c64* rhs);
+/* Overwrites the rhs from host or device memory. */
+sbh_status sbh_system_set_rhs(sbh_system* sys, const sbh_c64* rhs, int memory);
+
+/* CudaTbemAssemblyWorkspace::assemble (tbem.rs:4230-4389) /
+ * build_tbem_system_with_beta (tbem.rs:1196-1203 → 1284-1466): assembles
+ * A and the BC part of b for one wavenumber into sys.
+ *   A_ij = H_ij*gamma*tau + E_ij*beta (field BC 0), -(G_ij*gamma*tau + Ht_ij*beta) (BC 1),
+ *   A_ii += -gamma/2 (BC 0) or +beta*tau/2 (BC 1), plus admittance terms. */
+sbh_status sbh_workspace_assemble(sbh_workspace* ws, const sbh_physics* physics, sbh_c64 beta,
+                                  sbh_system* sys);
+/* build_tbem_system (tbem.rs:877-883): beta = burton_miller_beta(). */
+sbh_status sbh_build_tbem_system(sbh_context* ctx, const sbh_mesh* mesh,
+                                 const sbh_physics* physics, sbh_system* sys);
+
+/* Plane-wave incident right-hand side:
+ * IncidentField::compute_rhs_with_beta (crates/sonium-bem/src/core/incident.rs:807-834)
+ * added with the formulation sign as add_incident_field_rhs_stabilized does
+ * (bem_solver.rs:5005-5034):  rhs_col += sign * -(gamma p_inc + beta tau dp_inc/dn).
+ * directions: num_rhs x 3 unit vectors; amplitudes: num_rhs complex.
+ * rhs: N x num_rhs row-major (column c = direction c), device or host.
+ * accumulate = 0 overwrites instead of adding. */
+sbh_status sbh_incident_plane_wave_rhs(sbh_context* ctx, const sbh_mesh* mesh,
+                                       const sbh_physics* physics, sbh_c64 beta, double sign,
+                                       const double* directions, const sbh_c64* amplitudes,
+                                       int64_t num_rhs, sbh_c64* rhs, int memory, int accumulate);
+
+/* ---------------------------------------------------------------------------
+ * LinearOperator<Complex64> over a device-resident dense matrix —
+ * DenseMatrixOperator (bem_solver.rs:6548-6593) / DenseOperator
+ * (crates/sonium-bem/src/core/solver/fmm_interface.rs:26-387) with the trait
+ * methods of traits.rs:2949-3290.  The operator borrows sys; sys must outlive it. */
+typedef enum { SBH_ADJOINT_UNKNOWN = 0, SBH_ADJOINT_UNSUPPORTED = 1, SBH_ADJOINT_EXACT = 2 } sbh_adjoint_capability;
+sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out);
+void sbh_operator_destroy(sbh_operator* op);
+int64_t sbh_operator_num_rows(const sbh_operator* op);
+int64_t sbh_operator_num_cols(const sbh_operator* op);
+/* transpose_capability / hermitian_capability (traits.rs:3128-3137). */
+int sbh_operator_transpose_capability(const sbh_operator* op);
+int sbh_operator_hermitian_capability(const sbh_operator* op);
+/* apply_into (traits.rs:2987): y = A x.  Checked like try_apply
+ * (traits.rs:2964-2986): non-finite input is rejected with
+ * SBH_ERR_NUMERICAL before any device work when x is host memory. */
+sbh_status sbh_operator_apply(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory);
+/* apply_transpose_into (traits.rs:3145): y = A^T x. */
+sbh_status sbh_operator_apply_transpose(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory);
+/* apply_hermitian (traits.rs:3174): y = A^H x. */
+sbh_status sbh_operator_apply_hermitian(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory);
+/* apply_block (traits.rs:3000-3017; DenseMatrixOperator::apply_block →
+ * DenseMatrix::matmul_into, dense.rs:450-488): Y = A X with X, Y n x k row-major. */
+sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k,
+                                    int memory);
+
+/* ---------------------------------------------------------------------------
+ * Preconditioner<Complex64> (traits.rs:3645-3780): identity
+ * (IdentityPreconditioner) and the inverse diagonal of a dense operator. */
+sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out);
+sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out);
+void sbh_precond_destroy(sbh_precond* p);
+sbh_status sbh_precond_apply(sbh_precond* p, const sbh_c64* r, sbh_c64* y, int memory);
+
+/* ---------------------------------------------------------------------------
+ * Restarted GMRES — GmresConfig / GmresSolution
+ * (crates/sonium-solvers/crates/sonium-solvers-iterative/src/gmres.rs:26-130),
+ * gmres / gmres_with_guess / gmres_preconditioned(_with_guess)
+ * (gmres.rs:286-297, 564-626) → gmres_core (:631-1057) +
+ * finalize_gmres_solution (:159-250).  Krylov basis and vectors live on the
+ * device; the Hessenberg/Givens recurrence runs on the host in FP64. */
+typedef struct {
+    int64_t max_iterations; /* restart cycles */
+    int64_t restart;
+    double tolerance;
+} sbh_gmres_config;
+
+typedef enum {
+    SBH_SOLVER_CONVERGED = 0,
+    SBH_SOLVER_MAX_ITERATIONS = 1,
+    SBH_SOLVER_BREAKDOWN = 2,
+    SBH_SOLVER_NUMERICAL_ERROR = 3
+} sbh_solver_status;
+
+typedef struct {
+    int64_t iterations; /* total Arnoldi steps (matrix-vector products in the cycles) */
+    int64_t restarts;
+    double residual;           /* recomputed ||b - A x|| / ||b|| */
+    double estimated_residual; /* projected residual of the last step */
+    double backward_error;     /* ||b - A x|| / max(||A x|| + ||b||, min_positive) */
+    int32_t converged;
+    int32_t status;            /* sbh_solver_status */
+    int64_t spmv_count;
+    int64_t reorthogonalizations;
+} sbh_gmres_solution;
+
+/* b, x0 (may be NULL = zero guess) and x use `memory`; precond may be NULL
+ * (identity). */
+sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, const sbh_c64* x0,
+                     const sbh_gmres_config* config, sbh_c64* x, int memory,
+                     sbh_gmres_solution* solution);
+
+/* ---------------------------------------------------------------------------
+ * Test hooks (no GPU work): the host route builder's view of one pair, so
+ * tests can compare it with the CPU oracle's generate_subelements.
+ * leaves_out: up to 110 x 6 doubles (xi,eta of 3 vertices); returns the
+ * leaf count in *count (1 leaf with the full element means "far"). */
+sbh_status sbh_debug_subelements(const double* source_point, const double* element_coords,
+                                 double element_area, double* leaves_out, int* count);
+
+/* Test hook: the device sin/cos used by the assembly kernels, evaluated at
+ * host-supplied points (for accuracy checks against the C library). */
+sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBH_H */

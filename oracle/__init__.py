@@ -1,0 +1,212 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes front end to ``oracle/_build/liboracle.so`` (a C++ restatement of the
+reference's assembly, right-hand side and GMRES, see tbem_oracle.cpp) and the
+Mie series (``oracle.mie``).  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py may import this package; the product package
+``sonium_amd`` never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_gauss_legendre.argtypes = [ctypes.c_int, _dp, _dp]
+        L.oracle_triangle_quadrature.argtypes = [ctypes.c_int, _dp]
+        L.oracle_triangle_quadrature.restype = ctypes.c_int
+        L.oracle_element_geometry.argtypes = [_dp, _i64p, ctypes.c_int64, _dp, _dp, _dp]
+        L.oracle_generate_subelements.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        L.oracle_generate_subelements.restype = ctypes.c_int
+        L.oracle_regular_integration.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_double, _dp]
+        L.oracle_singular_integration.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.POINTER(ctypes.c_int), _dp]
+        L.oracle_build_tbem_system.argtypes = [
+            _dp, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), _dp, _dp,
+            ctypes.POINTER(ctypes.c_uint8), _dp, ctypes.c_double, ctypes.c_double,
+            ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.oracle_build_tbem_system.restype = ctypes.c_int
+        L.oracle_plane_wave_rhs.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        L.oracle_zgemv.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.oracle_gmres.argtypes = [_dp, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_gmres.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def default_threads() -> int:
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def gauss_legendre(order: int):
+    x = np.zeros(order)
+    w = np.zeros(order)
+    lib().oracle_gauss_legendre(order, _p(x), _p(w))
+    return x, w
+
+
+def triangle_quadrature(order: int) -> np.ndarray:
+    out = np.zeros((max(order, 4) ** 2 + 13, 3))
+    n = lib().oracle_triangle_quadrature(order, _p(out))
+    return out[:n].copy()
+
+
+def element_geometry(nodes, tris):
+    nodes = _f64(nodes)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    n = tris.shape[0]
+    c = np.zeros((n, 3))
+    nn = np.zeros((n, 3))
+    a = np.zeros(n)
+    lib().oracle_element_geometry(_p(nodes), tris.ctypes.data_as(_i64p), n, _p(c), _p(nn), _p(a))
+    return c, nn, a
+
+
+def generate_subelements(src, coords, area):
+    out = np.zeros((110, 8))
+    n = lib().oracle_generate_subelements(_p(_f64(src, 3)), _p(_f64(coords, 9)), float(area), _p(out))
+    return out[:n].copy()
+
+
+def _integrals(v):
+    return {"g": complex(v[0], v[1]), "h": complex(v[2], v[3]), "ht": complex(v[4], v[5]),
+            "e": complex(v[6], v[7])}
+
+
+def regular_integration(src, srcn, coords, area, k, harmonic_factor=1.0):
+    out = np.zeros(8)
+    lib().oracle_regular_integration(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)),
+                                     float(area), float(k), float(harmonic_factor), _p(out))
+    return _integrals(out)
+
+
+def singular_integration(src, srcn, coords, k, params=None, harmonic_factor=1.0):
+    out = np.zeros(8)
+    pp = None
+    if params is not None:
+        arr = (ctypes.c_int * 4)(*params)
+        pp = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int))
+    lib().oracle_singular_integration(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)),
+                                      float(k), float(harmonic_factor), pp, _p(out))
+    return _integrals(out)
+
+
+def build_tbem_system(nodes, tris, k, beta, *, bc_type=None, bc_value=None, admittance=None,
+                      rows=None, threads=None, density=1.21, speed_of_sound=343.0, tau=1.0,
+                      harmonic_factor=1.0):
+    """Rows of the dense Burton–Miller matrix and BC right-hand side.
+
+    Mirrors build_tbem_system_with_beta_parallel (tbem.rs:1284-1466) for a
+    Tri3 mesh with identity DOF numbering.  ``admittance`` is a complex array
+    with NaN marking panels without admittance.
+    """
+    nodes = _f64(nodes)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    n = tris.shape[0]
+    r0, r1 = (0, n) if rows is None else rows
+    omega = k * speed_of_sound
+    phys = np.array([k, harmonic_factor, density * omega * harmonic_factor, tau])
+    bt = None if bc_type is None else np.ascontiguousarray(bc_type, dtype=np.int32)
+    bv = None if bc_value is None else np.ascontiguousarray(np.asarray(bc_value, dtype=np.complex128).view(np.float64))
+    ad = has = None
+    if admittance is not None:
+        adc = np.asarray(admittance, dtype=np.complex128)
+        has = np.ascontiguousarray(~np.isnan(adc.real), dtype=np.uint8)
+        ad = np.ascontiguousarray(np.nan_to_num(adc).view(np.float64))
+    mat = np.zeros((r1 - r0, n), dtype=np.complex128)
+    rhs = np.zeros(r1 - r0, dtype=np.complex128)
+    beta = complex(beta)
+    rc = lib().oracle_build_tbem_system(
+        _p(nodes), tris.ctypes.data_as(_i64p), n,
+        None if bt is None else bt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        None if bv is None else _p(bv), None if ad is None else _p(ad),
+        None if has is None else has.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+        _p(phys), beta.real, beta.imag, r0, r1, _p(mat.view(np.float64)), _p(rhs.view(np.float64)),
+        threads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_build_tbem_system rejected its inputs")
+    return mat, rhs
+
+
+def plane_wave_rhs(centers, normals, k, beta, direction=(0.0, 0.0, 1.0), amplitude=1.0, sign=1.0,
+                   tau=1.0, harmonic_factor=1.0, rhs=None):
+    centers = _f64(centers)
+    normals = _f64(normals)
+    n = centers.shape[0]
+    out = np.zeros(n, dtype=np.complex128) if rhs is None else rhs
+    d = _f64(direction, 3)
+    amp = complex(amplitude)
+    beta = complex(beta)
+    lib().oracle_plane_wave_rhs(_p(centers), _p(normals), n, float(k), float(harmonic_factor), float(tau),
+                                beta.real, beta.imag, float(sign), _p(d), amp.real, amp.imag,
+                                _p(out.view(np.float64)))
+    return out
+
+
+def zgemv(A, x, threads=None):
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.zeros(A.shape[0], dtype=np.complex128)
+    lib().oracle_zgemv(_p(A.view(np.float64)), A.shape[0], A.shape[1], _p(x.view(np.float64)),
+                       _p(y.view(np.float64)), threads or default_threads())
+    return y
+
+
+STATUS = {0: "converged", 1: "max_iterations", 2: "breakdown", 3: "numerical_error"}
+
+
+def gmres(A, b, x0=None, *, restart=50, max_iterations=100, tol=1e-8, jacobi=False, threads=None):
+    """Restated gmres()/gmres_preconditioned_with_guess (gmres.rs:286, 608)."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    n = b.shape[0]
+    x = np.zeros(n, dtype=np.complex128)
+    st = np.zeros(8)
+    x0p = None
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.complex128)
+        x0p = _p(x0.view(np.float64))
+    lib().oracle_gmres(_p(A.view(np.float64)), n, _p(b.view(np.float64)), x0p, restart,
+                       max_iterations, tol, 1 if jacobi else 0, threads or default_threads(),
+                       _p(x.view(np.float64)), _p(st))
+    stats = {"iterations": int(st[0]), "restarts": int(st[1]), "residual": st[2],
+             "estimated_residual": st[3], "backward_error": st[4], "converged": bool(st[5]),
+             "status": STATUS[int(st[6])], "spmv_count": int(st[7])}
+    return x, stats

@@ -1,0 +1,607 @@
+"""sonium_amd — dense Burton–Miller Helmholtz BEM on AMD Instinct MI355X.
+
+Host-side mirror of the reference's interface for this path, over the C ABI
+of libsbh (include/sbh.h):
+
+* ``PhysicsParams`` — crates/sonium-bem/src/core/types.rs:190-218, 306-487
+* ``TbemAssemblyWorkspace`` (``new``/``assemble``) — CudaTbemAssemblyWorkspace,
+  crates/sonium-bem/src/core/assembly/tbem.rs:3779-4389
+* ``build_tbem_system`` / ``build_tbem_system_with_beta`` — tbem.rs:877, 1196
+* ``TbemSystem`` — tbem.rs:134-141 (device-resident matrix and rhs)
+* ``IncidentField`` — crates/sonium-bem/src/core/incident.rs:165-198, 807-834
+* ``DenseMatrixOperator`` — LinearOperator<Complex64>,
+  crates/sonium-solvers/crates/sonium-solvers-core/src/traits.rs:2949-3290
+* ``IdentityPreconditioner`` / ``JacobiPreconditioner`` — traits.rs:3645-3780
+* ``GmresConfig`` / ``GmresSolution`` / ``gmres`` and variants —
+  crates/sonium-solvers/crates/sonium-solvers-iterative/src/gmres.rs:26-130, 286, 564-626
+
+All compute runs in the HIP library; a missing library raises on first use.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+from . import mesh as mesh  # noqa: F401  (re-export the mesh generators)
+from .mesh import Mesh
+
+MEM_HOST = _ffi.MEM_HOST
+MEM_DEVICE = _ffi.MEM_DEVICE
+
+STATUS_NAMES = {0: "ok", 1: "invalid_argument", 2: "unsupported", 3: "backend", 4: "out_of_memory",
+                5: "numerical"}
+SOLVER_STATUS = {0: "converged", 1: "max_iterations", 2: "breakdown", 3: "numerical_error"}
+
+
+class SbhError(RuntimeError):
+    """A failed C ABI call; ``status`` is the sbh_status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def _lib():
+    return _ffi.load()
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _lib().sbh_last_error()
+        raise SbhError(status, msg.decode() if msg else "")
+
+
+def _cptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c128(a, n=None) -> np.ndarray:
+    arr = np.ascontiguousarray(a, dtype=np.complex128)
+    if n is not None and arr.size != n:
+        raise ValueError(f"expected {n} complex values, got {arr.size}")
+    return arr
+
+
+class PhysicsParams:
+    """Physical parameters of one frequency (types.rs:190-218)."""
+
+    def __init__(self, frequency: float, speed_of_sound: float = 343.0, density: float = 1.21,
+                 is_internal: bool = False, *, _raw: Optional[_ffi.physics_t] = None):
+        if _raw is not None:
+            self._p = _raw
+            return
+        self._p = _ffi.physics_t()
+        _check(_lib().sbh_physics_new(frequency, speed_of_sound, density, int(is_internal),
+                                      ctypes.byref(self._p)))
+
+    @classmethod
+    def from_wavenumber(cls, wave_number: float, speed_of_sound: float = 343.0, density: float = 1.21,
+                        is_internal: bool = False) -> "PhysicsParams":
+        p = _ffi.physics_t()
+        _check(_lib().sbh_physics_from_wavenumber(wave_number, speed_of_sound, density, int(is_internal),
+                                                  ctypes.byref(p)))
+        return cls(0.0, _raw=p)
+
+    def __getattr__(self, name):
+        if name in dict(_ffi.physics_t._fields_):
+            return getattr(self._p, name)
+        raise AttributeError(name)
+
+    def burton_miller_beta_scaled(self, scale: float) -> complex:
+        b = _lib().sbh_burton_miller_beta_scaled(ctypes.byref(self._p), scale)
+        return complex(b.re, b.im)
+
+    def burton_miller_beta(self) -> complex:
+        return self.burton_miller_beta_scaled(1.0)
+
+    def gamma(self) -> float:
+        return 1.0
+
+
+class Context:
+    """One HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_context_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        _check(_lib().sbh_context_synchronize(self._h))
+
+    def set_profiling(self, enabled: bool):
+        _check(_lib().sbh_context_set_profiling(self._h, int(enabled)))
+
+    def reset_stats(self):
+        _check(_lib().sbh_context_reset_stats(self._h))
+
+    def kernel_stats(self, name: str):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _check(_lib().sbh_context_kernel_stats(self._h, _ffi.KERNEL_CLASSES[name], ctypes.byref(n),
+                                               ctypes.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceVector:
+    """Library-owned device vector of complex128 (an n x k block when used as one)."""
+
+    def __init__(self, ctx: Context, length: int):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_vector_create(ctx.handle, int(length), ctypes.byref(h)))
+        self._h = h
+        self.length = int(length)
+
+    @property
+    def ptr(self):
+        return _lib().sbh_vector_data(self._h)
+
+    def upload(self, values, offset: int = 0):
+        v = _c128(values).ravel()
+        _check(_lib().sbh_vector_upload(self._h, _cptr(v), offset, v.size))
+
+    def download(self, offset: int = 0, count: Optional[int] = None) -> np.ndarray:
+        count = self.length - offset if count is None else count
+        out = np.empty(count, dtype=np.complex128)
+        _check(_lib().sbh_vector_download(self._h, _cptr(out), offset, count))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_vector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TbemMesh:
+    """Device-side mesh + boundary conditions (types.rs:582-719).
+
+    ``bc_type`` 0 = Velocity, 1 = Pressure; ``bc_value`` the panel BC value
+    (zero = rigid); ``admittance`` complex per panel, NaN where absent.
+    """
+
+    def __init__(self, ctx: Context, mesh: Mesh, bc_type=None, bc_value=None, admittance=None):
+        self.ctx = ctx
+        self.mesh = mesh
+        n = mesh.num_elements
+        self.num_elements = n
+        bt = None if bc_type is None else np.ascontiguousarray(bc_type, dtype=np.int32).reshape(n)
+        bv = None if bc_value is None else _c128(bc_value, n)
+        ad = has = None
+        if admittance is not None:
+            a = _c128(admittance, n)
+            has = np.ascontiguousarray(~np.isnan(a.real), dtype=np.uint8)
+            ad = np.nan_to_num(a)
+        self._keep = (bt, bv, ad, has)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_mesh_create(
+            ctx.handle, _cptr(mesh.nodes), mesh.nodes.shape[0], _cptr(mesh.triangles), n,
+            None if bt is None else _cptr(bt), None if bv is None else _cptr(bv),
+            None if ad is None else _cptr(ad), None if has is None else _cptr(has), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def geometry(self):
+        n = self.num_elements
+        c = np.empty((n, 3))
+        nn = np.empty((n, 3))
+        a = np.empty(n)
+        _check(_lib().sbh_mesh_geometry(self._h, _cptr(c), _cptr(nn), _cptr(a)))
+        return c, nn, a
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TbemSystem:
+    """Device-resident dense system (tbem.rs:134-141): row-major N x N matrix and rhs."""
+
+    def __init__(self, ctx: Context, num_dofs: int):
+        self.ctx = ctx
+        self.num_dofs = int(num_dofs)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_system_create(ctx.handle, self.num_dofs, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_pointers(self):
+        a = ctypes.c_void_p()
+        lda = ctypes.c_int64()
+        r = ctypes.c_void_p()
+        _check(_lib().sbh_system_device_pointers(self._h, ctypes.byref(a), ctypes.byref(lda), ctypes.byref(r)))
+        return a.value, int(lda.value), r.value
+
+    def matrix_rows(self, row_begin: int, row_end: int) -> np.ndarray:
+        out = np.empty((row_end - row_begin, self.num_dofs), dtype=np.complex128)
+        _check(_lib().sbh_system_download(self._h, row_begin, row_end, _cptr(out), None))
+        return out
+
+    @property
+    def matrix(self) -> np.ndarray:
+        return self.matrix_rows(0, self.num_dofs)
+
+    @property
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.num_dofs, dtype=np.complex128)
+        _check(_lib().sbh_system_download(self._h, 0, 0, None, _cptr(out)))
+        return out
+
+    def set_rhs(self, values):
+        v = _c128(values, self.num_dofs)
+        _check(_lib().sbh_system_set_rhs(self._h, _cptr(v), MEM_HOST))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TbemAssemblyWorkspace:
+    """Resident assembly workspace: geometry + near-pair routes built once, assemble per frequency."""
+
+    def __init__(self, ctx: Context, mesh: TbemMesh, route_threads: int = 0):
+        self.ctx = ctx
+        self.mesh = mesh
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_workspace_create(ctx.handle, mesh.handle, route_threads, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def stats(self):
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _check(_lib().sbh_workspace_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
+        return {"near_pairs": int(a.value), "leaves": int(b.value), "build_ms": float(ms.value)}
+
+    def assemble(self, physics: PhysicsParams, beta: Optional[complex] = None,
+                 system: Optional[TbemSystem] = None) -> TbemSystem:
+        if beta is None:
+            beta = physics.burton_miller_beta()
+        if system is None:
+            system = TbemSystem(self.ctx, self.mesh.num_elements)
+        b = _ffi.c64(complex(beta).real, complex(beta).imag)
+        _check(_lib().sbh_workspace_assemble(self._h, ctypes.byref(physics._p), b, system.handle))
+        return system
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_workspace_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_tbem_system_with_beta(ctx: Context, mesh: TbemMesh, physics: PhysicsParams, beta: complex) -> TbemSystem:
+    """tbem.rs:1196-1203 (→ build_tbem_system_with_beta_parallel, :1284-1466)."""
+    ws = TbemAssemblyWorkspace(ctx, mesh)
+    try:
+        sys = ws.assemble(physics, beta)
+        ctx.synchronize()
+        return sys
+    finally:
+        ws.close()
+
+
+def build_tbem_system(ctx: Context, mesh: TbemMesh, physics: PhysicsParams) -> TbemSystem:
+    """tbem.rs:877-883: beta = physics.burton_miller_beta()."""
+    return build_tbem_system_with_beta(ctx, mesh, physics, physics.burton_miller_beta())
+
+
+class IncidentField:
+    """Plane-wave incident field (incident.rs:165-198)."""
+
+    def __init__(self, directions, amplitudes=None):
+        d = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 3)
+        self.directions = d
+        amps = np.ones(d.shape[0], dtype=np.complex128) if amplitudes is None else _c128(amplitudes, d.shape[0])
+        self.amplitudes = amps
+
+    @classmethod
+    def plane_wave_z(cls) -> "IncidentField":
+        return cls([[0.0, 0.0, 1.0]])
+
+    @classmethod
+    def plane_wave(cls, direction, amplitude: float = 1.0) -> "IncidentField":
+        return cls([direction], [amplitude])
+
+    @property
+    def num_rhs(self) -> int:
+        return int(self.directions.shape[0])
+
+    def compute_rhs_with_beta(self, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
+                              sign: float = 1.0) -> np.ndarray:
+        """sign * -(gamma p_inc + beta tau dp_inc/dn) at the panel centres, N x num_rhs."""
+        out = np.zeros((mesh.num_elements, self.num_rhs), dtype=np.complex128)
+        self._launch(mesh, physics, beta, sign, _cptr(out), MEM_HOST, accumulate=False)
+        return out if self.num_rhs > 1 else out[:, 0]
+
+    def add_to_system(self, system: TbemSystem, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
+                      sign: float = 1.0):
+        """add_incident_field_rhs_stabilized (bem_solver.rs:5005-5034) on the device rhs."""
+        if self.num_rhs != 1:
+            raise ValueError("a TbemSystem holds one right-hand side")
+        _, _, rhs_ptr = system.device_pointers()
+        self._launch(mesh, physics, beta, sign, ctypes.c_void_p(rhs_ptr), MEM_DEVICE, accumulate=True)
+
+    def write_device(self, vec: DeviceVector, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
+                     sign: float = 1.0, accumulate: bool = False):
+        self._launch(mesh, physics, beta, sign, ctypes.c_void_p(vec.ptr), MEM_DEVICE, accumulate)
+
+    def _launch(self, mesh, physics, beta, sign, ptr, memory, accumulate):
+        b = _ffi.c64(complex(beta).real, complex(beta).imag)
+        _check(_lib().sbh_incident_plane_wave_rhs(
+            mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), b, float(sign), _cptr(self.directions),
+            _cptr(self.amplitudes), self.num_rhs, ptr, memory, int(accumulate)))
+
+
+AdjointCapability = {0: "Unknown", 1: "Unsupported", 2: "Exact"}
+
+
+class DenseMatrixOperator:
+    """LinearOperator<Complex64> over a device-resident TbemSystem (traits.rs:2949-3290)."""
+
+    def __init__(self, system: TbemSystem):
+        self.system = system
+        self.ctx = system.ctx
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_dense_operator_create(system.handle, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_rows(self) -> int:
+        return int(_lib().sbh_operator_num_rows(self._h))
+
+    def num_cols(self) -> int:
+        return int(_lib().sbh_operator_num_cols(self._h))
+
+    def transpose_capability(self) -> str:
+        return AdjointCapability[_lib().sbh_operator_transpose_capability(self._h)]
+
+    def hermitian_capability(self) -> str:
+        return AdjointCapability[_lib().sbh_operator_hermitian_capability(self._h)]
+
+    def _apply(self, fn, x):
+        xv = _c128(x, self.num_cols())
+        y = np.empty(self.num_rows(), dtype=np.complex128)
+        _check(fn(self._h, _cptr(xv), _cptr(y), MEM_HOST))
+        return y
+
+    def apply(self, x) -> np.ndarray:
+        return self._apply(_lib().sbh_operator_apply, x)
+
+    def apply_into(self, x, y: np.ndarray) -> None:
+        y[...] = self.apply(x)
+
+    def apply_transpose(self, x) -> np.ndarray:
+        return self._apply(_lib().sbh_operator_apply_transpose, x)
+
+    def apply_hermitian(self, x) -> np.ndarray:
+        return self._apply(_lib().sbh_operator_apply_hermitian, x)
+
+    def apply_block(self, X) -> np.ndarray:
+        Xa = np.ascontiguousarray(X, dtype=np.complex128)
+        if Xa.ndim != 2 or Xa.shape[0] != self.num_cols():
+            raise ValueError("X must be n x k")
+        Y = np.empty((self.num_rows(), Xa.shape[1]), dtype=np.complex128)
+        _check(_lib().sbh_operator_apply_block(self._h, _cptr(Xa), _cptr(Y), Xa.shape[1], MEM_HOST))
+        return Y
+
+    def apply_device(self, x: DeviceVector, y: DeviceVector) -> None:
+        _check(_lib().sbh_operator_apply(self._h, ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr), MEM_DEVICE))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_operator_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Precond:
+    def apply(self, r) -> np.ndarray:
+        rv = _c128(r, self.n)
+        y = np.empty(self.n, dtype=np.complex128)
+        _check(_lib().sbh_precond_apply(self._h, _cptr(rv), _cptr(y), MEM_HOST))
+        return y
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_precond_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IdentityPreconditioner(_Precond):
+    def __init__(self, ctx: Context, n: int):
+        self.n = int(n)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_precond_identity(ctx.handle, self.n, ctypes.byref(h)))
+        self._h = h
+
+
+class JacobiPreconditioner(_Precond):
+    """Inverse-diagonal left preconditioner of a dense operator."""
+
+    def __init__(self, op: DenseMatrixOperator):
+        self.n = op.num_rows()
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_precond_jacobi(op.handle, ctypes.byref(h)))
+        self._h = h
+
+
+@dataclass
+class GmresConfig:
+    """gmres.rs:26-71; defaults of GmresConfig<f64>::default()."""
+
+    max_iterations: int = 100
+    restart: int = 30
+    tolerance: float = 1e-6
+
+    @classmethod
+    def for_small_problems(cls) -> "GmresConfig":
+        return cls(max_iterations=50, restart=50, tolerance=1e-8)
+
+    @classmethod
+    def with_restart(cls, restart: int) -> "GmresConfig":
+        return cls(restart=restart)
+
+    def _c(self):
+        return _ffi.gmres_config_t(int(self.max_iterations), int(self.restart), float(self.tolerance))
+
+
+@dataclass
+class GmresSolution:
+    """gmres.rs:108-130."""
+
+    x: Optional[np.ndarray]
+    iterations: int
+    restarts: int
+    residual: float
+    estimated_residual: float
+    backward_error: float
+    converged: bool
+    status: str
+    spmv_count: int = 0
+    reorthogonalizations: int = 0
+    extra: dict = field(default_factory=dict)
+
+
+def _solution(raw: _ffi.gmres_solution_t, x) -> GmresSolution:
+    return GmresSolution(x=x, iterations=int(raw.iterations), restarts=int(raw.restarts),
+                         residual=float(raw.residual), estimated_residual=float(raw.estimated_residual),
+                         backward_error=float(raw.backward_error), converged=bool(raw.converged),
+                         status=SOLVER_STATUS[int(raw.status)], spmv_count=int(raw.spmv_count),
+                         reorthogonalizations=int(raw.reorthogonalizations))
+
+
+def gmres_preconditioned_with_guess(op: DenseMatrixOperator, precond, b, x0, config: GmresConfig) -> GmresSolution:
+    """gmres.rs:608-626 (host vectors)."""
+    n = op.num_rows()
+    bv = _c128(b, n)
+    x0v = None if x0 is None else _c128(x0, n)
+    x = np.zeros(n, dtype=np.complex128)
+    raw = _ffi.gmres_solution_t()
+    cfg = config._c()
+    _check(_lib().sbh_gmres(op.handle, None if precond is None else precond._h, _cptr(bv),
+                            None if x0v is None else _cptr(x0v), ctypes.byref(cfg), _cptr(x), MEM_HOST,
+                            ctypes.byref(raw)))
+    return _solution(raw, x)
+
+
+def gmres(op: DenseMatrixOperator, b, config: GmresConfig) -> GmresSolution:
+    """gmres.rs:286-297."""
+    return gmres_preconditioned_with_guess(op, None, b, None, config)
+
+
+def gmres_with_guess(op: DenseMatrixOperator, b, x0, config: GmresConfig) -> GmresSolution:
+    """gmres.rs:564-580."""
+    return gmres_preconditioned_with_guess(op, None, b, x0, config)
+
+
+def gmres_preconditioned(op: DenseMatrixOperator, precond, b, config: GmresConfig) -> GmresSolution:
+    """gmres.rs:585-600."""
+    return gmres_preconditioned_with_guess(op, precond, b, None, config)
+
+
+def gmres_device(op: DenseMatrixOperator, b_ptr: int, x: DeviceVector, config: GmresConfig,
+                 precond=None, x0: Optional[DeviceVector] = None) -> GmresSolution:
+    """GMRES on device-resident b (device pointer) and x; the solution stays in HBM."""
+    raw = _ffi.gmres_solution_t()
+    cfg = config._c()
+    _check(_lib().sbh_gmres(op.handle, None if precond is None else precond._h, ctypes.c_void_p(b_ptr),
+                            None if x0 is None else ctypes.c_void_p(x0.ptr), ctypes.byref(cfg),
+                            ctypes.c_void_p(x.ptr), MEM_DEVICE, ctypes.byref(raw)))
+    return _solution(raw, None)
+
+
+def device_sincos(ctx: Context, x) -> tuple:
+    """The assembly kernels' sin/cos evaluated on the device (test hook)."""
+    xv = np.ascontiguousarray(x, dtype=np.float64)
+    s = np.empty_like(xv)
+    c = np.empty_like(xv)
+    _check(_lib().sbh_debug_device_sincos(ctx.handle, _cptr(xv), xv.size, _cptr(s), _cptr(c)))
+    return s, c
+
+
+def debug_subelements(source_point, element_coords, element_area: float) -> np.ndarray:
+    """The host route builder's subdivision of one pair (test hook, no GPU)."""
+    src = np.ascontiguousarray(source_point, dtype=np.float64).reshape(3)
+    coords = np.ascontiguousarray(element_coords, dtype=np.float64).reshape(9)
+    out = np.zeros((110, 6))
+    cnt = ctypes.c_int()
+    _check(_lib().sbh_debug_subelements(_cptr(src), _cptr(coords), float(element_area), _cptr(out),
+                                        ctypes.byref(cnt)))
+    return out[: cnt.value].copy()
+
+
+__all__ = [
+    "Context", "DeviceVector", "PhysicsParams", "TbemMesh", "TbemSystem", "TbemAssemblyWorkspace",
+    "build_tbem_system", "build_tbem_system_with_beta", "IncidentField", "DenseMatrixOperator",
+    "IdentityPreconditioner", "JacobiPreconditioner", "GmresConfig", "GmresSolution", "gmres",
+    "gmres_with_guess", "gmres_preconditioned", "gmres_preconditioned_with_guess", "gmres_device",
+    "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE",
+]

@@ -1,0 +1,134 @@
+"""ctypes bindings of libsbh.so (the C ABI declared in include/sbh.h).
+
+The library is loaded from sonium_amd/_lib/libsbh.so.  There is no fallback:
+if the HIP library is missing, importing the bindings raises immediately.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsbh.so")
+
+MEM_HOST = 0
+MEM_DEVICE = 1
+
+KERNEL_CLASSES = {
+    "assemble_far": 0,
+    "assemble_near": 1,
+    "assemble_self": 2,
+    "rhs": 3,
+    "matvec": 4,
+    "krylov_vector": 5,
+    "block": 6,
+}
+
+
+class c64(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class physics_t(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_double) for name in (
+        "frequency", "speed_of_sound", "density", "wave_number", "omega", "wave_length",
+        "harmonic_factor", "pressure_factor", "tau")]
+
+
+class gmres_config_t(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64),
+                ("tolerance", ctypes.c_double)]
+
+
+class gmres_solution_t(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("restarts", ctypes.c_int64),
+                ("residual", ctypes.c_double), ("estimated_residual", ctypes.c_double),
+                ("backward_error", ctypes.c_double), ("converged", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("spmv_count", ctypes.c_int64),
+                ("reorthogonalizations", ctypes.c_int64)]
+
+
+_vp = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "sbh_last_error": (ctypes.c_char_p, []),
+    "sbh_version": (ctypes.c_char_p, []),
+    "sbh_physics_new": (_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _int,
+                               ctypes.POINTER(physics_t)]),
+    "sbh_physics_from_wavenumber": (_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _int,
+                                           ctypes.POINTER(physics_t)]),
+    "sbh_burton_miller_beta_scaled": (c64, [ctypes.POINTER(physics_t), ctypes.c_double]),
+    "sbh_context_create": (_int, [_int, _pp]),
+    "sbh_context_destroy": (None, [_vp]),
+    "sbh_context_synchronize": (_int, [_vp]),
+    "sbh_context_set_profiling": (_int, [_vp, _int]),
+    "sbh_context_kernel_stats": (_int, [_vp, _int, ctypes.POINTER(_i64), _dp]),
+    "sbh_context_reset_stats": (_int, [_vp]),
+    "sbh_vector_create": (_int, [_vp, _i64, _pp]),
+    "sbh_vector_destroy": (None, [_vp]),
+    "sbh_vector_length": (_i64, [_vp]),
+    "sbh_vector_data": (_vp, [_vp]),
+    "sbh_vector_upload": (_int, [_vp, _vp, _i64, _i64]),
+    "sbh_vector_download": (_int, [_vp, _vp, _i64, _i64]),
+    "sbh_mesh_create": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _pp]),
+    "sbh_mesh_destroy": (None, [_vp]),
+    "sbh_mesh_num_elements": (_i64, [_vp]),
+    "sbh_mesh_geometry": (_int, [_vp, _vp, _vp, _vp]),
+    "sbh_workspace_create": (_int, [_vp, _vp, _int, _pp]),
+    "sbh_workspace_destroy": (None, [_vp]),
+    "sbh_workspace_stats": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _dp]),
+    "sbh_system_create": (_int, [_vp, _i64, _pp]),
+    "sbh_system_destroy": (None, [_vp]),
+    "sbh_system_num_dofs": (_i64, [_vp]),
+    "sbh_system_device_pointers": (_int, [_vp, _pp, ctypes.POINTER(_i64), _pp]),
+    "sbh_system_download": (_int, [_vp, _i64, _i64, _vp, _vp]),
+    "sbh_system_set_rhs": (_int, [_vp, _vp, _int]),
+    "sbh_workspace_assemble": (_int, [_vp, ctypes.POINTER(physics_t), c64, _vp]),
+    "sbh_build_tbem_system": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp]),
+    "sbh_incident_plane_wave_rhs": (_int, [_vp, _vp, ctypes.POINTER(physics_t), c64, ctypes.c_double,
+                                           _vp, _vp, _i64, _vp, _int, _int]),
+    "sbh_dense_operator_create": (_int, [_vp, _pp]),
+    "sbh_operator_destroy": (None, [_vp]),
+    "sbh_operator_num_rows": (_i64, [_vp]),
+    "sbh_operator_num_cols": (_i64, [_vp]),
+    "sbh_operator_transpose_capability": (_int, [_vp]),
+    "sbh_operator_hermitian_capability": (_int, [_vp]),
+    "sbh_operator_apply": (_int, [_vp, _vp, _vp, _int]),
+    "sbh_operator_apply_transpose": (_int, [_vp, _vp, _vp, _int]),
+    "sbh_operator_apply_hermitian": (_int, [_vp, _vp, _vp, _int]),
+    "sbh_operator_apply_block": (_int, [_vp, _vp, _vp, _i64, _int]),
+    "sbh_precond_identity": (_int, [_vp, _i64, _pp]),
+    "sbh_precond_jacobi": (_int, [_vp, _pp]),
+    "sbh_precond_destroy": (None, [_vp]),
+    "sbh_precond_apply": (_int, [_vp, _vp, _vp, _int]),
+    "sbh_gmres": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(gmres_config_t), _vp, _int,
+                         ctypes.POINTER(gmres_solution_t)]),
+    "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
+    "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libsbh.so; raises OSError (loudly) when the HIP build is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(
+            f"sonium_amd: HIP library not built ({LIB_PATH} missing); run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C sonium_amd/csrc`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,782 @@
+// C ABI of libsbh (declarations and reference citations in include/sbh.h).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "internal.hpp"
+
+namespace sbh {
+sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
+                     const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol);
+void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+sbh_status guard(F&& f) {
+    try {
+        f();
+        return SBH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return SBH_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SBH_ERR_BACKEND;
+    }
+}
+
+inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
+
+// Device copy of a host or device input vector.
+struct DeviceInput {
+    DeviceArray<double2> owned;
+    const double2* ptr = nullptr;
+    DeviceInput(sbh_context* ctx, const sbh_c64* p, size_t n, int memory) {
+        if (memory == SBH_MEM_DEVICE) {
+            ptr = reinterpret_cast<const double2*>(p);
+        } else {
+            owned.allocate(n);
+            if (n) SBH_HIP(hipMemcpyAsync(owned.ptr, p, n * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+            ptr = owned.ptr;
+        }
+    }
+};
+
+struct DeviceOutput {
+    DeviceArray<double2> owned;
+    double2* ptr = nullptr;
+    sbh_c64* host = nullptr;
+    size_t n = 0;
+    sbh_context* ctx;
+    DeviceOutput(sbh_context* c, sbh_c64* p, size_t nn, int memory) : n(nn), ctx(c) {
+        if (memory == SBH_MEM_DEVICE) {
+            ptr = reinterpret_cast<double2*>(p);
+        } else {
+            owned.allocate(n);
+            ptr = owned.ptr;
+            host = p;
+        }
+    }
+    void finish() {
+        if (host && n) {
+            SBH_HIP(hipMemcpyAsync(host, ptr, n * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+};
+
+void check_memory(int memory) {
+    SBH_REQUIRE(memory == SBH_MEM_HOST || memory == SBH_MEM_DEVICE, "memory must be SBH_MEM_HOST or SBH_MEM_DEVICE");
+}
+
+bool all_finite(const sbh_c64* p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i].re) || !std::isfinite(p[i].im)) return false;
+    return true;
+}
+
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void context_retain(sbh_context* ctx) { ctx->refs.fetch_add(1); }
+
+void context_release(sbh_context* ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& t : ctx->timing)
+        for (auto& e : t.events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // namespace sbh
+
+using namespace sbh;
+
+double* sbh_context::pinned_rw(size_t count) {
+    if (pinned_count < count) {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_count = 0;
+        SBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), count * sizeof(double), hipHostMallocDefault));
+        pinned_count = count;
+    }
+    return pinned;
+}
+
+double2* sbh_context::device_scratch(size_t count) {
+    if (scratch_count < count) {
+        SBH_HIP(hipStreamSynchronize(stream));
+        if (scratch) (void)hipFree(scratch);
+        scratch = nullptr;
+        scratch_count = 0;
+        SBH_HIP(hipMalloc(reinterpret_cast<void**>(&scratch), count * sizeof(double2)));
+        scratch_count = count;
+    }
+    return scratch;
+}
+
+void sbh_context::begin_timed(int cls, hipEvent_t* start) {
+    *start = nullptr;
+    if (!profiling) return;
+    SBH_HIP(hipEventCreate(start));
+    SBH_HIP(hipEventRecord(*start, stream));
+}
+
+void sbh_context::end_timed(int cls, hipEvent_t start) {
+    if (!profiling || !start) return;
+    hipEvent_t stop;
+    SBH_HIP(hipEventCreate(&stop));
+    SBH_HIP(hipEventRecord(stop, stream));
+    std::lock_guard<std::mutex> lk(mu);
+    timing[cls].events.emplace_back(start, stop);
+    timing[cls].launches += 1;
+}
+
+extern "C" {
+
+const char* sbh_last_error(void) { return g_last_error.c_str(); }
+
+const char* sbh_version(void) { return "sonium_amd-sbh 0.1 (gfx950)"; }
+
+sbh_status sbh_physics_new(double frequency, double speed_of_sound, double density, int is_internal,
+                           sbh_physics* out) {
+    return guard([&] {
+        SBH_REQUIRE(out, "out is null");
+        SBH_REQUIRE(std::isfinite(frequency) && frequency > 0.0, "frequency must be positive and finite");
+        SBH_REQUIRE(std::isfinite(speed_of_sound) && speed_of_sound > 0.0, "speed of sound must be positive");
+        SBH_REQUIRE(std::isfinite(density) && density > 0.0, "density must be positive");
+        const double PI = 3.14159265358979323846;
+        sbh_physics p;
+        p.frequency = frequency;
+        p.speed_of_sound = speed_of_sound;
+        p.density = density;
+        p.omega = 2.0 * PI * frequency;
+        p.wave_number = p.omega / speed_of_sound;
+        p.wave_length = speed_of_sound / frequency;
+        p.harmonic_factor = 1.0;
+        p.pressure_factor = density * p.omega * p.harmonic_factor;
+        p.tau = is_internal ? -1.0 : 1.0;
+        *out = p;
+    });
+}
+
+sbh_status sbh_physics_from_wavenumber(double wave_number, double speed_of_sound, double density,
+                                       int is_internal, sbh_physics* out) {
+    return guard([&] {
+        SBH_REQUIRE(out, "out is null");
+        SBH_REQUIRE(std::isfinite(wave_number) && wave_number > 0.0, "wave number must be positive and finite");
+        SBH_REQUIRE(std::isfinite(speed_of_sound) && speed_of_sound > 0.0, "speed of sound must be positive");
+        SBH_REQUIRE(std::isfinite(density) && density > 0.0, "density must be positive");
+        const double PI = 3.14159265358979323846;
+        sbh_physics p;
+        p.speed_of_sound = speed_of_sound;
+        p.density = density;
+        p.wave_number = wave_number;
+        p.omega = wave_number * speed_of_sound;
+        p.frequency = p.omega / (2.0 * PI);
+        p.wave_length = 2.0 * PI / wave_number;
+        p.harmonic_factor = 1.0;
+        p.pressure_factor = density * p.omega * p.harmonic_factor;
+        p.tau = is_internal ? -1.0 : 1.0;
+        *out = p;
+    });
+}
+
+sbh_c64 sbh_burton_miller_beta_scaled(const sbh_physics* ph, double scale) {
+    if (!ph || !(ph->tau > 0.0)) return sbh_c64{0.0, 0.0};
+    return sbh_c64{0.0, ph->harmonic_factor * scale / ph->wave_number};
+}
+
+sbh_status sbh_context_create(int device, sbh_context** out) {
+    return guard([&] {
+        SBH_REQUIRE(out, "out is null");
+        *out = nullptr;
+        int count = 0;
+        SBH_HIP(hipGetDeviceCount(&count));
+        SBH_REQUIRE(device >= 0 && device < count, "device index out of range");
+        std::unique_ptr<sbh_context> ctx(new sbh_context());
+        ctx->device = device;
+        SBH_HIP(hipSetDevice(device));
+        SBH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->pinned_rw(1 << 16);
+        upload_quadrature_tables(ctx.get());
+        *out = ctx.release();
+    });
+}
+
+void sbh_context_destroy(sbh_context* ctx) {
+    if (ctx) context_release(ctx);
+}
+
+sbh_status sbh_context_synchronize(sbh_context* ctx) {
+    return guard([&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        use_device(ctx);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+sbh_status sbh_context_set_profiling(sbh_context* ctx, int enabled) {
+    return guard([&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        ctx->profiling = enabled != 0;
+    });
+}
+
+sbh_status sbh_context_kernel_stats(sbh_context* ctx, int cls, int64_t* launches, double* total_ms) {
+    return guard([&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        SBH_REQUIRE(cls >= 0 && cls < SBH_KERNEL_CLASS_COUNT, "kernel class out of range");
+        use_device(ctx);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        KernelTiming& t = ctx->timing[cls];
+        for (auto& e : t.events) {
+            float ms = 0.0f;
+            SBH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            t.total_ms += ms;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        t.events.clear();
+        if (launches) *launches = t.launches;
+        if (total_ms) *total_ms = t.total_ms;
+    });
+}
+
+sbh_status sbh_context_reset_stats(sbh_context* ctx) {
+    return guard([&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        use_device(ctx);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (auto& t : ctx->timing) {
+            for (auto& e : t.events) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+            t.events.clear();
+            t.launches = 0;
+            t.total_ms = 0.0;
+        }
+    });
+}
+
+sbh_status sbh_vector_create(sbh_context* ctx, int64_t length, sbh_vector** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && out && length >= 1, "invalid vector arguments");
+        use_device(ctx);
+        std::unique_ptr<sbh_vector> v(new sbh_vector());
+        v->ctx = ctx;
+        v->length = length;
+        v->d.allocate((size_t)length);
+        SBH_HIP(hipMemsetAsync(v->d.ptr, 0, v->d.bytes(), ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        context_retain(ctx);
+        *out = v.release();
+    });
+}
+
+void sbh_vector_destroy(sbh_vector* v) {
+    if (!v) return;
+    sbh_context* ctx = v->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete v;
+    context_release(ctx);
+}
+
+int64_t sbh_vector_length(const sbh_vector* v) { return v ? v->length : 0; }
+
+sbh_c64* sbh_vector_data(sbh_vector* v) { return v ? reinterpret_cast<sbh_c64*>(v->d.ptr) : nullptr; }
+
+sbh_status sbh_vector_upload(sbh_vector* v, const sbh_c64* host, int64_t offset, int64_t count) {
+    return guard([&] {
+        SBH_REQUIRE(v && host, "vector/host is null");
+        SBH_REQUIRE(offset >= 0 && count >= 0 && offset + count <= v->length, "range out of bounds");
+        use_device(v->ctx);
+        SBH_HIP(hipMemcpyAsync(v->d.ptr + offset, host, sizeof(double2) * count, hipMemcpyHostToDevice,
+                               v->ctx->stream));
+        SBH_HIP(hipStreamSynchronize(v->ctx->stream));
+    });
+}
+
+sbh_status sbh_vector_download(const sbh_vector* v, sbh_c64* host, int64_t offset, int64_t count) {
+    return guard([&] {
+        SBH_REQUIRE(v && host, "vector/host is null");
+        SBH_REQUIRE(offset >= 0 && count >= 0 && offset + count <= v->length, "range out of bounds");
+        use_device(v->ctx);
+        SBH_HIP(hipMemcpyAsync(host, v->d.ptr + offset, sizeof(double2) * count, hipMemcpyDeviceToHost,
+                               v->ctx->stream));
+        SBH_HIP(hipStreamSynchronize(v->ctx->stream));
+    });
+}
+
+sbh_status sbh_mesh_create(sbh_context* ctx, const double* nodes, int64_t num_nodes,
+                           const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
+                           const sbh_c64* bc_value, const sbh_c64* admittance,
+                           const uint8_t* has_admittance, sbh_mesh** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && out, "ctx/out is null");
+        *out = nullptr;
+        SBH_REQUIRE(num_nodes >= 3 && nodes, "mesh needs at least 3 nodes");
+        SBH_REQUIRE(num_elements >= 1 && triangles, "mesh needs at least one element");
+        SBH_REQUIRE(num_elements < (int64_t)1 << 31, "too many elements");
+        for (int64_t i = 0; i < 3 * num_nodes; ++i)
+            SBH_REQUIRE(std::isfinite(nodes[i]), "node coordinates must be finite");
+        for (int64_t i = 0; i < 3 * num_elements; ++i)
+            SBH_REQUIRE(triangles[i] >= 0 && triangles[i] < num_nodes, "triangle node index out of range");
+        use_device(ctx);
+        std::unique_ptr<sbh_mesh> m(new sbh_mesh());
+        m->ctx = ctx;
+        m->n = num_elements;
+        m->num_nodes = num_nodes;
+        m->nodes.assign(nodes, nodes + 3 * num_nodes);
+        m->tris.assign(triangles, triangles + 3 * num_elements);
+        build_mesh_geometry(m.get());
+        m->bc.assign(num_elements, PanelBc{});
+        bool general = false;
+        for (int64_t e = 0; e < num_elements; ++e) {
+            PanelBc& b = m->bc[e];
+            b.type = bc_type ? bc_type[e] : 0;
+            SBH_REQUIRE(b.type == 0 || b.type == 1, "bc_type must be 0 (velocity) or 1 (pressure)");
+            b.value[0] = bc_value ? bc_value[e].re : 0.0;
+            b.value[1] = bc_value ? bc_value[e].im : 0.0;
+            SBH_REQUIRE(std::isfinite(b.value[0]) && std::isfinite(b.value[1]), "BC values must be finite");
+            b.has_adm = (admittance && has_admittance && has_admittance[e]) ? 1 : 0;
+            if (b.has_adm) {
+                b.adm[0] = admittance[e].re;
+                b.adm[1] = admittance[e].im;
+                SBH_REQUIRE(std::isfinite(b.adm[0]) && std::isfinite(b.adm[1]), "admittance must be finite");
+            }
+            // has_nonzero_bc (tbem.rs:1240-1242)
+            if (std::hypot(b.value[0], b.value[1]) > 1e-15) m->nonzero_bc.push_back((int32_t)e);
+            if (b.type != 0 || b.has_adm) general = true;
+        }
+        if (!m->nonzero_bc.empty()) general = true;
+        m->general = general;
+        const double FOUR_PI = 12.566370614359172;
+        std::vector<FarPanel> far(num_elements);
+        std::vector<Source> src(num_elements);
+        for (int64_t e = 0; e < num_elements; ++e) {
+            const PanelGeom& g = m->geom[e];
+            FarPanel& f = far[e];
+            for (int d = 0; d < 3; ++d) {
+                f.v0[d] = g.c[d];
+                f.e1[d] = g.c[3 + d] - g.c[d];
+                f.e2[d] = g.c[6 + d] - g.c[d];
+                f.nf[d] = g.nf[d];
+                src[e].c[d] = m->centers[3 * e + d];
+                src[e].n[d] = m->normals[3 * e + d];
+            }
+            f.wj = 0.5 * g.jac / FOUR_PI;
+            f.pad = 0.0;
+        }
+        m->d_far.allocate(num_elements);
+        m->d_geom.allocate(num_elements);
+        m->d_src.allocate(num_elements);
+        m->d_bc.allocate(num_elements);
+        SBH_HIP(hipMemcpy(m->d_far.ptr, far.data(), m->d_far.bytes(), hipMemcpyHostToDevice));
+        SBH_HIP(hipMemcpy(m->d_geom.ptr, m->geom.data(), m->d_geom.bytes(), hipMemcpyHostToDevice));
+        SBH_HIP(hipMemcpy(m->d_src.ptr, src.data(), m->d_src.bytes(), hipMemcpyHostToDevice));
+        SBH_HIP(hipMemcpy(m->d_bc.ptr, m->bc.data(), m->d_bc.bytes(), hipMemcpyHostToDevice));
+        if (!m->nonzero_bc.empty()) {
+            m->d_nonzero_bc.allocate(m->nonzero_bc.size());
+            SBH_HIP(hipMemcpy(m->d_nonzero_bc.ptr, m->nonzero_bc.data(), m->d_nonzero_bc.bytes(),
+                              hipMemcpyHostToDevice));
+        }
+        context_retain(ctx);
+        *out = m.release();
+    });
+}
+
+void sbh_mesh_destroy(sbh_mesh* mesh) {
+    if (!mesh) return;
+    sbh_context* ctx = mesh->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete mesh;
+    context_release(ctx);
+}
+
+int64_t sbh_mesh_num_elements(const sbh_mesh* mesh) { return mesh ? mesh->n : 0; }
+
+sbh_status sbh_mesh_geometry(const sbh_mesh* m, double* centers, double* normals, double* areas) {
+    return guard([&] {
+        SBH_REQUIRE(m, "mesh is null");
+        if (centers) std::memcpy(centers, m->centers.data(), sizeof(double) * 3 * m->n);
+        if (normals) std::memcpy(normals, m->normals.data(), sizeof(double) * 3 * m->n);
+        if (areas) std::memcpy(areas, m->areas.data(), sizeof(double) * m->n);
+    });
+}
+
+sbh_status sbh_workspace_create(sbh_context* ctx, const sbh_mesh* mesh, int route_threads,
+                                sbh_workspace** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && mesh && out, "ctx/mesh/out is null");
+        SBH_REQUIRE(mesh->ctx == ctx, "mesh belongs to another context");
+        *out = nullptr;
+        use_device(ctx);
+        auto t0 = std::chrono::steady_clock::now();
+        RouteTable rt = build_routes(mesh, route_threads);
+        std::unique_ptr<sbh_workspace> ws(new sbh_workspace());
+        ws->ctx = ctx;
+        ws->mesh = mesh;
+        ws->near_pairs = (int64_t)rt.pairs.size();
+        ws->leaves = (int64_t)rt.leaves.size();
+        if (ws->near_pairs) {
+            ws->d_pairs.allocate(rt.pairs.size());
+            SBH_HIP(hipMemcpy(ws->d_pairs.ptr, rt.pairs.data(), ws->d_pairs.bytes(), hipMemcpyHostToDevice));
+        }
+        if (ws->leaves) {
+            ws->d_leaves.allocate(rt.leaves.size());
+            SBH_HIP(hipMemcpy(ws->d_leaves.ptr, rt.leaves.data(), ws->d_leaves.bytes(), hipMemcpyHostToDevice));
+        }
+        ws->d_row_ptr.allocate(rt.row_ptr.size());
+        SBH_HIP(hipMemcpy(ws->d_row_ptr.ptr, rt.row_ptr.data(), ws->d_row_ptr.bytes(), hipMemcpyHostToDevice));
+        ws->build_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        context_retain(ctx);
+        *out = ws.release();
+    });
+}
+
+void sbh_workspace_destroy(sbh_workspace* ws) {
+    if (!ws) return;
+    sbh_context* ctx = ws->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ws;
+    context_release(ctx);
+}
+
+sbh_status sbh_workspace_stats(const sbh_workspace* ws, int64_t* near_pairs, int64_t* leaves,
+                               double* build_ms) {
+    return guard([&] {
+        SBH_REQUIRE(ws, "workspace is null");
+        if (near_pairs) *near_pairs = ws->near_pairs;
+        if (leaves) *leaves = ws->leaves;
+        if (build_ms) *build_ms = ws->build_ms;
+    });
+}
+
+sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && out, "ctx/out is null");
+        SBH_REQUIRE(num_dofs >= 1, "num_dofs must be positive");
+        *out = nullptr;
+        use_device(ctx);
+        std::unique_ptr<sbh_system> s(new sbh_system());
+        s->ctx = ctx;
+        s->n = num_dofs;
+        s->lda = num_dofs;
+        s->d_A.allocate((size_t)num_dofs * (size_t)num_dofs);
+        s->d_rhs.allocate((size_t)num_dofs);
+        SBH_HIP(hipMemsetAsync(s->d_rhs.ptr, 0, s->d_rhs.bytes(), ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        context_retain(ctx);
+        *out = s.release();
+    });
+}
+
+void sbh_system_destroy(sbh_system* sys) {
+    if (!sys) return;
+    sbh_context* ctx = sys->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete sys;
+    context_release(ctx);
+}
+
+int64_t sbh_system_num_dofs(const sbh_system* sys) { return sys ? sys->n : 0; }
+
+sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t* lda, sbh_c64** rhs) {
+    return guard([&] {
+        SBH_REQUIRE(sys, "system is null");
+        if (matrix) *matrix = reinterpret_cast<sbh_c64*>(sys->d_A.ptr);
+        if (lda) *lda = sys->lda;
+        if (rhs) *rhs = reinterpret_cast<sbh_c64*>(sys->d_rhs.ptr);
+    });
+}
+
+sbh_status sbh_system_download(sbh_system* sys, int64_t row_begin, int64_t row_end, sbh_c64* rows,
+                               sbh_c64* rhs) {
+    return guard([&] {
+        SBH_REQUIRE(sys, "system is null");
+        SBH_REQUIRE(row_begin >= 0 && row_begin <= row_end && row_end <= sys->n, "row range out of bounds");
+        use_device(sys->ctx);
+        const int64_t nr = row_end - row_begin;
+        if (rows && nr > 0)
+            SBH_HIP(hipMemcpy2DAsync(rows, sizeof(double2) * sys->n, sys->d_A.ptr + row_begin * sys->lda,
+                                     sizeof(double2) * sys->lda, sizeof(double2) * sys->n, (size_t)nr,
+                                     hipMemcpyDeviceToHost, sys->ctx->stream));
+        if (rhs)
+            SBH_HIP(hipMemcpyAsync(rhs, sys->d_rhs.ptr, sizeof(double2) * sys->n, hipMemcpyDeviceToHost,
+                                   sys->ctx->stream));
+        SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
+    });
+}
+
+sbh_status sbh_system_set_rhs(sbh_system* sys, const sbh_c64* rhs, int memory) {
+    return guard([&] {
+        SBH_REQUIRE(sys && rhs, "system/rhs is null");
+        check_memory(memory);
+        use_device(sys->ctx);
+        SBH_HIP(hipMemcpyAsync(sys->d_rhs.ptr, rhs, sizeof(double2) * sys->n,
+                               memory == SBH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               sys->ctx->stream));
+        SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
+    });
+}
+
+sbh_status sbh_workspace_assemble(sbh_workspace* ws, const sbh_physics* ph, sbh_c64 beta, sbh_system* sys) {
+    return guard([&] {
+        SBH_REQUIRE(ws && ph && sys, "workspace/physics/system is null");
+        SBH_REQUIRE(sys->ctx == ws->ctx, "system belongs to another context");
+        SBH_REQUIRE(sys->n == ws->mesh->n, "system size does not match the mesh");
+        SBH_REQUIRE(std::isfinite(ph->wave_number) && ph->wave_number > 0.0, "wave number must be positive");
+        SBH_REQUIRE(std::isfinite(beta.re) && std::isfinite(beta.im), "beta must be finite");
+        use_device(ws->ctx);
+        launch_assembly(ws, *ph, beta, sys);
+    });
+}
+
+sbh_status sbh_build_tbem_system(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph,
+                                 sbh_system* sys) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && mesh && ph && sys, "null argument");
+        sbh_workspace* ws = nullptr;
+        sbh_status st = sbh_workspace_create(ctx, mesh, 0, &ws);
+        if (st != SBH_OK) throw Error(st, g_last_error);
+        std::unique_ptr<sbh_workspace, void (*)(sbh_workspace*)> hold(ws, sbh_workspace_destroy);
+        st = sbh_workspace_assemble(ws, ph, sbh_burton_miller_beta_scaled(ph, 1.0), sys);
+        if (st != SBH_OK) throw Error(st, g_last_error);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+sbh_status sbh_incident_plane_wave_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph,
+                                       sbh_c64 beta, double sign, const double* directions,
+                                       const sbh_c64* amplitudes, int64_t num_rhs, sbh_c64* rhs,
+                                       int memory, int accumulate) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && mesh && ph && directions && amplitudes && rhs, "null argument");
+        SBH_REQUIRE(num_rhs >= 1, "num_rhs must be positive");
+        check_memory(memory);
+        use_device(ctx);
+        std::vector<double> dirs(3 * num_rhs);
+        for (int64_t c = 0; c < num_rhs; ++c) {
+            // IncidentField::plane_wave (incident.rs:185-198) normalizes
+            const double* d = directions + 3 * c;
+            double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            SBH_REQUIRE(std::isfinite(len), "directions must be finite");
+            if (len > 1e-10) {
+                dirs[3 * c] = d[0] / len;
+                dirs[3 * c + 1] = d[1] / len;
+                dirs[3 * c + 2] = d[2] / len;
+            } else {
+                dirs[3 * c] = 0.0;
+                dirs[3 * c + 1] = 0.0;
+                dirs[3 * c + 2] = -1.0;
+            }
+        }
+        DeviceArray<double> d_dirs(3 * num_rhs);
+        DeviceArray<double2> d_amps(num_rhs);
+        SBH_HIP(hipMemcpyAsync(d_dirs.ptr, dirs.data(), d_dirs.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        SBH_HIP(hipMemcpyAsync(d_amps.ptr, amplitudes, d_amps.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        const size_t total = (size_t)mesh->n * (size_t)num_rhs;
+        if (memory == SBH_MEM_DEVICE) {
+            launch_incident_rhs(ctx, mesh, *ph, beta, sign, d_dirs.ptr, d_amps.ptr, num_rhs,
+                                reinterpret_cast<double2*>(rhs), accumulate != 0);
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+            DeviceArray<double2> tmp(total);
+            if (accumulate)
+                SBH_HIP(hipMemcpyAsync(tmp.ptr, rhs, tmp.bytes(), hipMemcpyHostToDevice, ctx->stream));
+            launch_incident_rhs(ctx, mesh, *ph, beta, sign, d_dirs.ptr, d_amps.ptr, num_rhs, tmp.ptr,
+                                accumulate != 0);
+            SBH_HIP(hipMemcpyAsync(rhs, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out) {
+    return guard([&] {
+        SBH_REQUIRE(sys && out, "system/out is null");
+        sbh_operator* op = new sbh_operator();
+        op->ctx = sys->ctx;
+        op->sys = sys;
+        context_retain(op->ctx);
+        *out = op;
+    });
+}
+
+void sbh_operator_destroy(sbh_operator* op) {
+    if (!op) return;
+    sbh_context* ctx = op->ctx;
+    delete op;
+    context_release(ctx);
+}
+int64_t sbh_operator_num_rows(const sbh_operator* op) { return op ? op->sys->n : 0; }
+int64_t sbh_operator_num_cols(const sbh_operator* op) { return op ? op->sys->n : 0; }
+int sbh_operator_transpose_capability(const sbh_operator* op) { return op ? SBH_ADJOINT_EXACT : SBH_ADJOINT_UNKNOWN; }
+int sbh_operator_hermitian_capability(const sbh_operator* op) { return op ? SBH_ADJOINT_EXACT : SBH_ADJOINT_UNKNOWN; }
+
+static sbh_status operator_apply(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory, int mode) {
+    return guard([&] {
+        SBH_REQUIRE(op && x && y, "operator/x/y is null");
+        check_memory(memory);
+        const int64_t n = op->sys->n;
+        if (memory == SBH_MEM_HOST && !all_finite(x, (size_t)n))
+            throw Error(SBH_ERR_NUMERICAL, "operator input contains non-finite values");
+        use_device(op->ctx);
+        DeviceInput in(op->ctx, x, (size_t)n, memory);
+        DeviceOutput outv(op->ctx, y, (size_t)n, memory);
+        const sbh_system* s = op->sys;
+        if (mode == 0)
+            zgemv_n(op->ctx, s->d_A.ptr, s->lda, n, n, in.ptr, outv.ptr);
+        else
+            zgemv_t(op->ctx, s->d_A.ptr, s->lda, n, n, in.ptr, outv.ptr, mode == 2);
+        outv.finish();
+    });
+}
+
+sbh_status sbh_operator_apply(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory) {
+    return operator_apply(op, x, y, memory, 0);
+}
+sbh_status sbh_operator_apply_transpose(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory) {
+    return operator_apply(op, x, y, memory, 1);
+}
+sbh_status sbh_operator_apply_hermitian(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory) {
+    return operator_apply(op, x, y, memory, 2);
+}
+
+sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory) {
+    return guard([&] {
+        SBH_REQUIRE(op && X && Y, "operator/X/Y is null");
+        SBH_REQUIRE(k >= 1, "block width must be positive");
+        check_memory(memory);
+        const int64_t n = op->sys->n;
+        if (memory == SBH_MEM_HOST && !all_finite(X, (size_t)(n * k)))
+            throw Error(SBH_ERR_NUMERICAL, "operator input contains non-finite values");
+        use_device(op->ctx);
+        DeviceInput in(op->ctx, X, (size_t)(n * k), memory);
+        DeviceOutput outv(op->ctx, Y, (size_t)(n * k), memory);
+        zgemm_block(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, in.ptr, outv.ptr, k);
+        outv.finish();
+    });
+}
+
+sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && out && n >= 1, "invalid identity preconditioner arguments");
+        sbh_precond* p = new sbh_precond();
+        p->ctx = ctx;
+        p->n = n;
+        p->kind = 0;
+        context_retain(ctx);
+        *out = p;
+    });
+}
+
+sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out) {
+    return guard([&] {
+        SBH_REQUIRE(op && out, "operator/out is null");
+        use_device(op->ctx);
+        std::unique_ptr<sbh_precond> p(new sbh_precond());
+        p->ctx = op->ctx;
+        p->n = op->sys->n;
+        p->kind = 1;
+        p->d_dinv.allocate(p->n);
+        extract_inverse_diagonal(op->ctx, op->sys->d_A.ptr, op->sys->lda, p->n, p->d_dinv.ptr);
+        SBH_HIP(hipStreamSynchronize(op->ctx->stream));
+        context_retain(op->ctx);
+        *out = p.release();
+    });
+}
+
+void sbh_precond_destroy(sbh_precond* p) {
+    if (!p) return;
+    sbh_context* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete p;
+    context_release(ctx);
+}
+
+sbh_status sbh_precond_apply(sbh_precond* p, const sbh_c64* r, sbh_c64* y, int memory) {
+    return guard([&] {
+        SBH_REQUIRE(p && r && y, "preconditioner/r/y is null");
+        check_memory(memory);
+        use_device(p->ctx);
+        DeviceInput in(p->ctx, r, (size_t)p->n, memory);
+        DeviceOutput outv(p->ctx, y, (size_t)p->n, memory);
+        if (p->kind == 0)
+            SBH_HIP(hipMemcpyAsync(outv.ptr, in.ptr, sizeof(double2) * p->n, hipMemcpyDeviceToDevice, p->ctx->stream));
+        else
+            diag_scale(p->ctx, p->d_dinv.ptr, in.ptr, outv.ptr, p->n);
+        outv.finish();
+    });
+}
+
+sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, const sbh_c64* x0,
+                     const sbh_gmres_config* config, sbh_c64* x, int memory,
+                     sbh_gmres_solution* solution) {
+    sbh_status st = SBH_OK;
+    sbh_status g = guard([&] {
+        SBH_REQUIRE(op && b && config && x && solution, "null argument");
+        check_memory(memory);
+        if (precond) SBH_REQUIRE(precond->n == op->sys->n, "preconditioner size does not match the operator");
+        const int64_t n = op->sys->n;
+        use_device(op->ctx);
+        DeviceInput db(op->ctx, b, (size_t)n, memory);
+        std::unique_ptr<DeviceInput> dx0;
+        if (x0) dx0.reset(new DeviceInput(op->ctx, x0, (size_t)n, memory));
+        DeviceOutput dx(op->ctx, x, (size_t)n, memory);
+        st = run_gmres(op, precond, db.ptr, dx0 ? dx0->ptr : nullptr, *config, dx.ptr, solution);
+        dx.finish();
+    });
+    return g != SBH_OK ? g : st;
+}
+
+sbh_status sbh_debug_subelements(const double* source_point, const double* element_coords,
+                                 double element_area, double* leaves_out, int* count) {
+    return guard([&] {
+        SBH_REQUIRE(source_point && element_coords && leaves_out && count, "null argument");
+        *count = subelements_for_pair(source_point, element_coords, element_area, leaves_out);
+    });
+}
+
+sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && x && s && c && n >= 0, "null argument");
+        use_device(ctx);
+        DeviceArray<double> dx(n), ds(n), dc(n);
+        if (n) SBH_HIP(hipMemcpy(dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        launch_sincos_probe(ctx, dx.ptr, n, ds.ptr, dc.ptr);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        if (n) {
+            SBH_HIP(hipMemcpy(s, ds.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(c, dc.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,341 @@
+// Restarted GMRES over device-resident vectors.
+//
+// Follows gmres_core (crates/sonium-solvers/crates/sonium-solvers-iterative/
+// src/gmres.rs:631-1057) with the convergence check at every inner step and
+// no periodic exact-residual checks (GmresConfig::default), then
+// finalize_gmres_solution (gmres.rs:159-250).  Left preconditioning,
+// classical Gram–Schmidt with one DGKS reorthogonalisation pass when the
+// projected norm drops below 0.707 of the unprojected one (gmres.rs:799-829),
+// complex Givens rotations (gmres_common.rs:91-105) and back substitution
+// (gmres_common.rs:108-127).
+//
+// The Krylov basis, the iterate and every vector operation stay on the
+// device; only the (j+1) Gram–Schmidt coefficients and norms come back to the
+// host each step, where the small Hessenberg recurrence runs in FP64.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace sbh {
+namespace {
+
+using cplx = std::complex<double>;
+
+// num-complex arithmetic in the reference's order.
+inline cplx cm(cplx a, cplx b) {
+    return cplx(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+inline cplx cinv(cplx a) {
+    double n = a.real() * a.real() + a.imag() * a.imag();
+    return cplx(a.real() / n, -a.imag() / n);
+}
+inline double cnorm(cplx a) { return std::hypot(a.real(), a.imag()); }
+
+struct Krylov {
+    sbh_context* ctx;
+    sbh_operator* op;
+    sbh_precond* M;
+    int64_t n;
+    DeviceArray<double2> part;   // reduction partials
+    DeviceArray<double2> coeff;  // (m+1) coefficients
+    double* host = nullptr;      // pinned
+    int64_t spmv = 0;
+
+    Krylov(sbh_context* c, sbh_operator* o, sbh_precond* p, int64_t nn, int m)
+        : ctx(c), op(o), M(p), n(nn) {
+        int chunks = reduction_chunks(n);
+        size_t need = std::max<size_t>((size_t)(m + 2) * chunks, (size_t)((n + 255) / 256)) + 8;
+        part.allocate(need);
+        coeff.allocate((size_t)m + 2);
+        host = ctx->pinned_rw(need * 2);
+    }
+
+    void apply(const double2* x, double2* y) {
+        const sbh_system* s = op->sys;
+        zgemv_n(ctx, s->d_A.ptr, s->lda, s->n, s->n, x, y);
+        spmv += 1;
+    }
+    void precond(const double2* r, double2* y) {
+        if (!M || M->kind == 0) {
+            if (r != y) SBH_HIP(hipMemcpyAsync(y, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            diag_scale(ctx, M->d_dinv.ptr, r, y, n);
+        }
+    }
+    double fetch_sum(int count) {
+        SBH_HIP(hipMemcpyAsync(host, part.ptr, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        double s = 0.0;
+        for (int i = 0; i < count; ++i) s += host[i];
+        return s;
+    }
+    double norm(const double2* x) {
+        int nb = norm_partials(ctx, x, n, reinterpret_cast<double*>(part.ptr));
+        return std::sqrt(fetch_sum(nb));
+    }
+    // r = b - r, returns ||r||
+    double residual(const double2* b, double2* r) {
+        int nb = residual_and_norm(ctx, b, r, n, reinterpret_cast<double*>(part.ptr));
+        return std::sqrt(fetch_sum(nb));
+    }
+    // coefficients <v_i, w> (i <= j) and ||w||
+    double dots(const double2* V, int nvec, const double2* w, std::vector<cplx>& out) {
+        int chunks = dots_and_norm_partials(ctx, V, n, nvec, w, n, part.ptr);
+        size_t cnt = (size_t)(nvec + 1) * chunks;
+        SBH_HIP(hipMemcpyAsync(host, part.ptr, sizeof(double2) * cnt, hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        out.assign(nvec, cplx(0.0, 0.0));
+        for (int v = 0; v < nvec; ++v) {
+            double sr = 0.0, si = 0.0;
+            for (int c = 0; c < chunks; ++c) {
+                sr += host[2 * ((size_t)v * chunks + c)];
+                si += host[2 * ((size_t)v * chunks + c) + 1];
+            }
+            out[v] = cplx(sr, si);
+        }
+        double nn = 0.0;
+        for (int c = 0; c < chunks; ++c) nn += host[2 * ((size_t)nvec * chunks + c)];
+        return std::sqrt(nn);
+    }
+    void upload_coeffs(const std::vector<cplx>& c, int count) {
+        for (int i = 0; i < count; ++i) {
+            host[2 * i] = c[i].real();
+            host[2 * i + 1] = c[i].imag();
+        }
+        SBH_HIP(hipMemcpyAsync(coeff.ptr, host, sizeof(double2) * count, hipMemcpyHostToDevice, ctx->stream));
+    }
+    // w -= sum c_i v_i ; returns ||w||
+    double project(const double2* V, int nvec, double2* w) {
+        int nb = project_and_norm(ctx, V, n, nvec, coeff.ptr, w, n, reinterpret_cast<double*>(part.ptr));
+        return std::sqrt(fetch_sum(nb));
+    }
+};
+
+}  // namespace
+
+sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
+                     const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol) {
+    sbh_context* ctx = op->ctx;
+    const int64_t n = op->sys->n;
+    *sol = sbh_gmres_solution{};
+    auto finish = [&](int64_t it, int64_t rs, double res, double est, double bw, bool conv, int status) {
+        sol->iterations = it;
+        sol->restarts = rs;
+        sol->residual = res;
+        sol->estimated_residual = est;
+        sol->backward_error = bw;
+        sol->converged = conv ? 1 : 0;
+        sol->status = status;
+    };
+    const int m_cfg = (int)std::min<int64_t>(cfg.restart, std::max<int64_t>(n, 1));
+    const int m = std::max(1, m_cfg);
+    Krylov K(ctx, op, M, n, m);
+    DeviceArray<double2> V((size_t)(m + 1) * n), w(n), opb(n);
+    // Initial guess.
+    if (d_x0)
+        SBH_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    else
+        SBH_HIP(hipMemsetAsync(d_x, 0, sizeof(double2) * n, ctx->stream));
+
+    // invalid_gmres_input (gmres.rs:252-275)
+    double bnorm_check = K.norm(d_b);
+    double xnorm_check = d_x0 ? K.norm(d_x0) : 0.0;
+    if (cfg.max_iterations <= 0 || cfg.restart <= 0 || !std::isfinite(cfg.tolerance) ||
+        cfg.tolerance <= 0.0 || !std::isfinite(bnorm_check) || !std::isfinite(xnorm_check)) {
+        finish(0, 0, INFINITY, INFINITY, INFINITY, false, SBH_SOLVER_NUMERICAL_ERROR);
+        sol->spmv_count = K.spmv;
+        return SBH_OK;
+    }
+    const double tol = cfg.tolerance;
+    const double tol_threshold = 1e-15;
+    const double raw_b = bnorm_check;
+    const bool rhs_zero = raw_b < tol_threshold;
+    const double true_b_norm = rhs_zero ? 1.0 : raw_b;
+    K.precond(d_b, w.ptr);
+    const double raw_pb = (!M || M->kind == 0) ? raw_b : K.norm(w.ptr);
+    int64_t total = 0, restarts = 0, reorth = 0;
+    auto done = [&]() {
+        sol->spmv_count = K.spmv;
+        sol->reorthogonalizations = reorth;
+        return SBH_OK;
+    };
+    if (!rhs_zero && raw_pb < tol_threshold) {
+        finish(0, 0, 1.0, 1.0, 1.0, false, SBH_SOLVER_BREAKDOWN);
+        return done();
+    }
+    const double pb_norm = rhs_zero ? 1.0 : raw_pb;
+
+    std::vector<cplx> H((size_t)(m + 1) * m), g(m + 1), cs, sn, coeffs, re;
+    auto h = [&](int i, int j) -> cplx& { return H[(size_t)i * m + j]; };
+    auto vec = [&](int i) { return V.ptr + (size_t)i * n; };
+
+    auto exact_rel = [&]() {
+        K.apply(d_x, opb.ptr);
+        double rn = K.residual(d_b, opb.ptr);
+        return rn / std::max(true_b_norm, 2.2250738585072014e-308);
+    };
+    auto solve_update = [&](int kk) {
+        std::vector<cplx> y(kk, cplx(0.0, 0.0));
+        for (int i = kk - 1; i >= 0; --i) {
+            cplx sum = g[i];
+            for (int j = i + 1; j < kk; ++j) sum -= cm(h(i, j), y[j]);
+            if (cnorm(h(i, i)) > 1e-30) y[i] = cm(sum, cinv(h(i, i)));
+        }
+        K.upload_coeffs(y, kk);
+        basis_update(ctx, V.ptr, n, kk, K.coeff.ptr, d_x, n);
+    };
+
+    bool returned = false;
+    for (int64_t outer = 0; outer < cfg.max_iterations && !returned; ++outer) {
+        K.apply(d_x, opb.ptr);
+        double rn = K.residual(d_b, opb.ptr);
+        double true_rel = rn / true_b_norm;
+        double beta;
+        if (!M || M->kind == 0) {
+            beta = rn;
+            SBH_HIP(hipMemcpyAsync(w.ptr, opb.ptr, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            K.precond(opb.ptr, w.ptr);
+            beta = K.norm(w.ptr);
+        }
+        double rel = beta / pb_norm;
+        if (rel < tol && true_rel < tol) {
+            finish(total, restarts, true_rel, rel, true_rel, true, SBH_SOLVER_CONVERGED);
+            returned = true;
+            break;
+        }
+        if (!std::isfinite(beta) || beta == 0.0) {
+            finish(total, restarts, true_rel, rel, true_rel, false, SBH_SOLVER_BREAKDOWN);
+            returned = true;
+            break;
+        }
+        scale_copy(ctx, w.ptr, 1.0 / beta, vec(0), n);
+        std::fill(H.begin(), H.end(), cplx(0.0, 0.0));
+        cs.clear();
+        sn.clear();
+        std::fill(g.begin(), g.end(), cplx(0.0, 0.0));
+        g[0] = cplx(beta, 0.0);
+        bool inner_converged = false;
+        bool restart_now = false;
+        for (int j = 0; j < m; ++j) {
+            total += 1;
+            if (!M || M->kind == 0) {
+                K.apply(vec(j), w.ptr);
+            } else {
+                K.apply(vec(j), opb.ptr);
+                K.precond(opb.ptr, w.ptr);
+            }
+            double wnb = K.dots(V.ptr, j + 1, w.ptr, coeffs);
+            for (int i = 0; i <= j; ++i) h(i, j) = coeffs[i];
+            K.upload_coeffs(coeffs, j + 1);
+            double wn = K.project(V.ptr, j + 1, w.ptr);
+            if (wn < 0.707 * wnb) {
+                reorth += 1;
+                K.dots(V.ptr, j + 1, w.ptr, re);
+                for (int i = 0; i <= j; ++i) h(i, j) += re[i];
+                K.upload_coeffs(re, j + 1);
+                wn = K.project(V.ptr, j + 1, w.ptr);
+            }
+            h(j + 1, j) = cplx(wn, 0.0);
+            if (wn < 1e-20) {
+                inner_converged = true;
+            } else {
+                scale_copy(ctx, w.ptr, 1.0 / wn, vec(j + 1), n);
+            }
+            for (int i = 0; i < j; ++i) {
+                cplx temp = cm(std::conj(cs[i]), h(i, j)) + cm(std::conj(sn[i]), h(i + 1, j));
+                h(i + 1, j) = (cplx(0.0, 0.0) - cm(sn[i], h(i, j))) + cm(cs[i], h(i + 1, j));
+                h(i, j) = temp;
+            }
+            // givens_rotation (gmres_common.rs:91-105)
+            cplx a = h(j, j), b = h(j + 1, j), c, s;
+            if (cnorm(b) < 1e-30) {
+                c = cplx(1.0, 0.0);
+                s = cplx(0.0, 0.0);
+            } else if (cnorm(a) < 1e-30) {
+                c = cplx(0.0, 0.0);
+                s = cplx(1.0, 0.0);
+            } else {
+                double r = std::sqrt((a.real() * a.real() + a.imag() * a.imag()) +
+                                     (b.real() * b.real() + b.imag() * b.imag()));
+                c = a * (1.0 / r);
+                s = b * (1.0 / r);
+            }
+            cs.push_back(c);
+            sn.push_back(s);
+            h(j, j) = cm(std::conj(c), h(j, j)) + cm(std::conj(s), h(j + 1, j));
+            h(j + 1, j) = cplx(0.0, 0.0);
+            cplx temp = cm(std::conj(c), g[j]) + cm(std::conj(s), g[j + 1]);
+            g[j + 1] = (cplx(0.0, 0.0) - cm(s, g[j])) + cm(c, g[j + 1]);
+            g[j] = temp;
+            double abs_res = cnorm(g[j + 1]);
+            double rrel = abs_res / pb_norm;
+            bool converged = rrel < tol || (!inner_converged && abs_res < 1e-20);
+            if (converged) {
+                solve_update(j + 1);
+                double t = exact_rel();
+                if (t >= tol && t >= 1e-20) {
+                    restarts += 1;
+                    restart_now = true;
+                    break;
+                }
+                finish(total, restarts, t, rrel, t, true, SBH_SOLVER_CONVERGED);
+                returned = true;
+                break;
+            }
+            if (inner_converged) {
+                solve_update(j + 1);
+                double t = exact_rel();
+                finish(total, restarts, t, rrel, t, t < tol, t < tol ? SBH_SOLVER_CONVERGED : SBH_SOLVER_BREAKDOWN);
+                returned = true;
+                break;
+            }
+        }
+        if (returned || restart_now) continue;
+        solve_update(m);
+        double t = exact_rel();
+        if (t < tol) {
+            finish(total, restarts, t, t, t, true, SBH_SOLVER_CONVERGED);
+            returned = true;
+            break;
+        }
+        restarts += 1;
+    }
+    if (!returned) {
+        double t = exact_rel();
+        finish(total, restarts, t, t, t, false, SBH_SOLVER_MAX_ITERATIONS);
+    }
+    // finalize_gmres_solution (gmres.rs:159-250)
+    if (sol->status == SBH_SOLVER_NUMERICAL_ERROR) return done();
+    if (!std::isfinite(sol->estimated_residual) || sol->estimated_residual < 0.0) {
+        finish(sol->iterations, sol->restarts, INFINITY, sol->estimated_residual, INFINITY, false,
+               SBH_SOLVER_NUMERICAL_ERROR);
+        return done();
+    }
+    K.apply(d_x, opb.ptr);
+    double ax_norm = K.norm(opb.ptr);
+    double xn = K.norm(d_x);
+    double rn = K.residual(d_b, opb.ptr);
+    if (!std::isfinite(xn) || !std::isfinite(ax_norm) || !std::isfinite(rn)) {
+        finish(sol->iterations, sol->restarts, INFINITY, sol->estimated_residual, INFINITY, false,
+               SBH_SOLVER_NUMERICAL_ERROR);
+        return done();
+    }
+    double true_res = raw_b > 0.0 ? rn / raw_b : rn;
+    double denom = std::max(ax_norm + raw_b, 2.2250738585072014e-308);
+    sol->residual = true_res;
+    sol->backward_error = rn / denom;
+    bool conv = true_res <= tol;
+    sol->converged = conv ? 1 : 0;
+    if (conv)
+        sol->status = SBH_SOLVER_CONVERGED;
+    else if (sol->status == SBH_SOLVER_CONVERGED)
+        sol->status = SBH_SOLVER_MAX_ITERATIONS;
+    return done();
+}
+
+}  // namespace sbh

@@ -1,0 +1,259 @@
+// Internal definitions shared by the host sources and HIP kernels of libsbh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sbh.h"
+
+namespace sbh {
+
+struct Error : std::runtime_error {
+    sbh_status code;
+    Error(sbh_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define SBH_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t sbh_e_ = (expr);                                                           \
+        if (sbh_e_ != hipSuccess) {                                                           \
+            sbh_status sbh_c_ =                                                               \
+                sbh_e_ == hipErrorOutOfMemory ? SBH_ERR_OUT_OF_MEMORY : SBH_ERR_BACKEND;      \
+            throw ::sbh::Error(sbh_c_, std::string(#expr) + ": " + hipGetErrorString(sbh_e_)); \
+        }                                                                                     \
+    } while (0)
+
+#define SBH_REQUIRE(cond, msg)                                              \
+    do {                                                                    \
+        if (!(cond)) throw ::sbh::Error(SBH_ERR_INVALID_ARGUMENT, (msg));   \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DeviceArray {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DeviceArray() = default;
+    explicit DeviceArray(size_t n) { allocate(n); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DeviceArray& operator=(DeviceArray&& o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        return *this;
+    }
+    ~DeviceArray() { release(); }
+    void allocate(size_t n) {
+        release();
+        if (n == 0) return;
+        SBH_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)));
+        count = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+// Per-panel data for the far-field kernel: vertex 0, the two edge vectors
+// (exactly v1-v0 and v2-v0, the reference's dx/ds and dx/dt), the
+// vertex-order unit normal, and 0.5*J/(4*pi) so that a TR13 weight w gives
+// wga/(4*pi) = w * wj.
+struct FarPanel {
+    double v0[3];
+    double e1[3];
+    double e2[3];
+    double nf[3];
+    double wj;
+    double pad;
+};
+
+// Exact geometry for the near-pair and self-panel kernels, which evaluate in
+// the reference's operation order.
+struct PanelGeom {
+    double c[9];    // node coordinates, row-major (node, xyz)
+    double nf[3];   // vertex-order unit normal (compute_parameters)
+    double jac;     // |(v1-v0) x (v2-v0)|
+    double area;    // element area (compute_element_geometry)
+    double size;    // estimate_element_size: max(sqrt(area), 1e-10)
+    double pad;
+};
+
+// Collocation point and outward element normal of a source panel.
+struct Source {
+    double c[3];
+    double n[3];
+};
+
+// Boundary condition of a field panel (GENERAL assembly mode only).
+struct PanelBc {
+    int32_t type;       // 0 velocity, 1 pressure
+    int32_t has_adm;
+    double value[2];
+    double adm[2];
+};
+
+// One near pair: leaves [begin, begin+count) of the leaf-code array.
+struct NearPair {
+    int32_t i, j;
+    int32_t count;
+    int32_t pad;
+    int64_t begin;
+};
+
+// A subdivision leaf is stored as its path from the root triangle: bits
+// [2*l, 2*l+1] hold the child index chosen at level l+1 (0,1,2: corner
+// children [v_c, m_c, m_(c-1)], 3: centre [m0, m1, m2]); bits 58..63 hold the
+// depth.  Decoding reproduces the reference's midpoint arithmetic exactly.
+constexpr int LEAF_DEPTH_SHIFT = 58;
+
+struct KernelTiming {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace sbh
+
+// Every object created on a context holds a reference to it, so objects may be
+// destroyed in any order relative to their context.
+struct sbh_context {
+    std::atomic<int> refs{1};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    std::mutex mu;
+    sbh::KernelTiming timing[SBH_KERNEL_CLASS_COUNT];
+    // device scratch for partial sums of transposed products, grown on demand
+    double2* scratch = nullptr;
+    size_t scratch_count = 0;
+    double2* device_scratch(size_t count);
+    // small pinned staging buffer for reductions
+    double* pinned = nullptr;
+    size_t pinned_count = 0;
+    double* pinned_rw(size_t count);
+    void begin_timed(int cls, hipEvent_t* start);
+    void end_timed(int cls, hipEvent_t start);
+};
+
+struct sbh_mesh {
+    sbh_context* ctx = nullptr;
+    int64_t n = 0;
+    int64_t num_nodes = 0;
+    std::vector<double> nodes;      // V x 3
+    std::vector<int64_t> tris;      // N x 3
+    std::vector<double> centers;    // N x 3 (collocation points)
+    std::vector<double> normals;    // N x 3 (outward element normals)
+    std::vector<double> areas;      // N
+    std::vector<sbh::PanelGeom> geom;
+    std::vector<sbh::PanelBc> bc;
+    double diameter = 0.0;          // bounding-box diagonal of the nodes
+    bool general = false;           // any pressure BC, admittance or nonzero BC value
+    std::vector<int32_t> nonzero_bc;  // panels whose BC value is nonzero (tbem.rs:1240-1242)
+    sbh::DeviceArray<sbh::FarPanel> d_far;
+    sbh::DeviceArray<sbh::PanelGeom> d_geom;
+    sbh::DeviceArray<sbh::Source> d_src;
+    sbh::DeviceArray<sbh::PanelBc> d_bc;
+    sbh::DeviceArray<int32_t> d_nonzero_bc;
+};
+
+struct sbh_workspace {
+    sbh_context* ctx = nullptr;
+    const sbh_mesh* mesh = nullptr;
+    int64_t near_pairs = 0;
+    int64_t leaves = 0;
+    double build_ms = 0.0;
+    sbh::DeviceArray<sbh::NearPair> d_pairs;
+    sbh::DeviceArray<uint64_t> d_leaves;
+    sbh::DeviceArray<int64_t> d_row_ptr;  // CSR of near pairs by source row
+};
+
+struct sbh_system {
+    sbh_context* ctx = nullptr;
+    int64_t n = 0;
+    int64_t lda = 0;
+    sbh::DeviceArray<double2> d_A;
+    sbh::DeviceArray<double2> d_rhs;
+};
+
+struct sbh_vector {
+    sbh_context* ctx = nullptr;
+    int64_t length = 0;
+    sbh::DeviceArray<double2> d;
+};
+
+struct sbh_operator {
+    sbh_context* ctx = nullptr;
+    sbh_system* sys = nullptr;
+};
+
+struct sbh_precond {
+    sbh_context* ctx = nullptr;
+    int64_t n = 0;
+    int kind = 0;  // 0 identity, 1 inverse diagonal
+    sbh::DeviceArray<double2> d_dinv;
+};
+
+namespace sbh {
+
+void context_retain(sbh_context* ctx);
+void context_release(sbh_context* ctx);
+
+// Host route builder (mesh_routes.cpp).
+void build_mesh_geometry(sbh_mesh* mesh);
+struct RouteTable {
+    std::vector<NearPair> pairs;
+    std::vector<uint64_t> leaves;
+    std::vector<int64_t> row_ptr;
+};
+RouteTable build_routes(const sbh_mesh* mesh, int threads);
+int subelements_for_pair(const double* src, const double* coords, double area, double* leaves_out);
+void gauss_legendre_rule(int order, double* x, double* w);
+
+// Assembly launchers (tbem_kernels.hip).
+void upload_quadrature_tables(sbh_context* ctx);
+void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_system* sys);
+void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,
+                         sbh_c64 beta, double sign, const double* d_dirs, const double2* d_amps,
+                         int64_t num_rhs, double2* d_rhs, bool accumulate);
+
+// Dense BLAS launchers (blas_kernels.hip).
+void zgemv_n(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols,
+             const double2* x, double2* y);
+void zgemv_t(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols,
+             const double2* x, double2* y, bool conjugate);
+void zgemm_block(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X,
+                 double2* Y, int64_t k);
+// Partial sums for dot products <v_i, w> (i < nvec) and ||w||^2 in one launch;
+// out holds (nvec + 1) x chunks complex partials (the last row is the norm).
+int dots_and_norm_partials(sbh_context* ctx, const double2* V, int64_t ldv, int nvec,
+                           const double2* w, int64_t n, double2* out);
+// w -= sum_i c_i v_i; writes ||w||^2 partials into norm_out (chunks).
+int project_and_norm(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* c,
+                     double2* w, int64_t n, double* norm_out);
+// x += sum_i c_i v_i
+void basis_update(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* c,
+                  double2* x, int64_t n);
+// y = alpha * x (alpha real)
+void scale_copy(sbh_context* ctx, const double2* x, double alpha, double2* y, int64_t n);
+// r = b - r ; norm partials of r into norm_out
+int residual_and_norm(sbh_context* ctx, const double2* b, double2* r, int64_t n, double* norm_out);
+// y = d .* x (elementwise complex)
+void diag_scale(sbh_context* ctx, const double2* d, const double2* x, double2* y, int64_t n);
+// d = 1 / diag(A)
+void extract_inverse_diagonal(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, double2* d);
+// partials of ||x||^2
+int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out);
+int reduction_chunks(int64_t n);
+
+}  // namespace sbh

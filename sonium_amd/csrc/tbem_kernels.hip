@@ -1,0 +1,721 @@
+// Dense Burton–Miller assembly on gfx950 (FP64).
+//
+// Four kernels fill the N x N row-major complex matrix of
+// build_tbem_system_with_beta_parallel (crates/sonium-bem/src/core/assembly/
+// tbem.rs:1284-1466) for one wavenumber:
+//
+//   far_kernel   every off-diagonal pair with the 13-point rule on the whole
+//                field panel (the unsubdivided leaf of generate_subelements,
+//                regular.rs:124-321).  One lane per field panel (column),
+//                rows streamed through the workgroup: the lane keeps its
+//                panel in registers, the collocation point and normal are
+//                uniform (scalar loads), and 64 lanes store 1 KiB of a row.
+//                This is the FP64-VALU-bound kernel: 13 kernel evaluations
+//                per pair, each one rsqrt, one sincos (Cody–Waite reduction +
+//                Taylor polynomials) and ~75 FMA-class operations.
+//   near_kernel  overwrites the near pairs (level-0 ratio < 3) with the sum
+//                over their subdivision leaves; one wave per pair, lanes over
+//                (leaf, point), evaluated in the reference's operation order.
+//   self_kernel  overwrites the diagonal: Stokes edge integral of the
+//                hypersingular kernel plus Duffy-collapsed Gauss–Legendre on
+//                the 3 x nsec2 sub-triangles (singular.rs:297-578), the
+//                QuadratureParams::for_ka tier (singular.rs:51-85) chosen per
+//                panel from k*size, and the free terms (tbem.rs:1330-1358).
+//   bc_rhs_kernel  only when some field panel carries a nonzero BC value:
+//                the sum over such columns of tbem.rs:1419-1435.
+//
+// plane_wave_kernel adds the incident-field right-hand side
+// (incident.rs:807-834 with bem_solver.rs:5005-5034).
+
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace sbh {
+namespace {
+
+constexpr double FOUR_PI = 12.566370614359172;  // 4.0 * PI, exact product
+
+// TR13 (gauss.rs:526-540); weights are used times 0.5 (gauss.rs:117-130).
+__constant__ double kTrS[13] = {0.333333333333333, 0.260345966079040, 0.260345966079040,
+                                0.479308067841920, 0.065130102902216, 0.065130102902216,
+                                0.869739794195568, 0.638444188569810, 0.048690315425316,
+                                0.638444188569810, 0.312865496004874, 0.048690315425316,
+                                0.312865496004874};
+__constant__ double kTrT[13] = {0.333333333333333, 0.260345966079040, 0.479308067841920,
+                                0.260345966079040, 0.065130102902216, 0.869739794195568,
+                                0.065130102902216, 0.048690315425316, 0.638444188569810,
+                                0.312865496004874, 0.638444188569810, 0.312865496004874,
+                                0.048690315425316};
+__constant__ double kTrW[13] = {-0.149570044467682, 0.175615257433208, 0.175615257433208,
+                                0.175615257433208, 0.053347235608838, 0.053347235608838,
+                                0.053347235608838, 0.077113760890257, 0.077113760890257,
+                                0.077113760890257, 0.077113760890257, 0.077113760890257,
+                                0.077113760890257};
+
+// Local vertex and mid-side coordinates (singular.rs:119-124 CSI6/ETA6).
+__constant__ double kCsi6[6] = {0.0, 1.0, 0.0, 0.5, 0.5, 0.0};
+__constant__ double kEta6[6] = {0.0, 0.0, 1.0, 0.0, 0.5, 0.5};
+
+// Gauss–Legendre nodes/weights for orders 1..8 (row = order), uploaded once
+// per device from the host restatement of gauss.rs:14-103.
+__constant__ double kGlX[9][8];
+__constant__ double kGlW[9][8];
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// sin and cos of x for |x| < 2^20.  Cody & Waite (1980) two-constant
+// reduction by pi/2 with FMA, then Taylor polynomials of sin (to r^17) and
+// cos (to r^18) on |r| <= pi/4, whose truncation error is below 1e-19.
+__device__ __forceinline__ void sincos_red(double x, double* sp, double* cp) {
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double PIO2_HI = 1.5707963267948966;
+    constexpr double PIO2_LO = 6.123233995736766e-17;
+    double q = __builtin_rint(x * TWO_OVER_PI);
+    double r = __builtin_fma(-q, PIO2_HI, x);
+    r = __builtin_fma(-q, PIO2_LO, r);
+    double r2 = r * r;
+    constexpr double S3 = -1.0 / 6.0, S5 = 1.0 / 120.0, S7 = -1.0 / 5040.0, S9 = 1.0 / 362880.0,
+                     S11 = -1.0 / 39916800.0, S13 = 1.0 / 6227020800.0,
+                     S15 = -1.0 / 1307674368000.0, S17 = 1.0 / 355687428096000.0;
+    constexpr double C4 = 1.0 / 24.0, C6 = -1.0 / 720.0, C8 = 1.0 / 40320.0, C10 = -1.0 / 3628800.0,
+                     C12 = 1.0 / 479001600.0, C14 = -1.0 / 87178291200.0,
+                     C16 = 1.0 / 20922789888000.0, C18 = -1.0 / 6402373705728000.0;
+    double ps = __builtin_fma(r2, S17, S15);
+    ps = __builtin_fma(r2, ps, S13);
+    ps = __builtin_fma(r2, ps, S11);
+    ps = __builtin_fma(r2, ps, S9);
+    ps = __builtin_fma(r2, ps, S7);
+    ps = __builtin_fma(r2, ps, S5);
+    ps = __builtin_fma(r2, ps, S3);
+    double sr = __builtin_fma(r * r2, ps, r);
+    double pc = __builtin_fma(r2, C18, C16);
+    pc = __builtin_fma(r2, pc, C14);
+    pc = __builtin_fma(r2, pc, C12);
+    pc = __builtin_fma(r2, pc, C10);
+    pc = __builtin_fma(r2, pc, C8);
+    pc = __builtin_fma(r2, pc, C6);
+    pc = __builtin_fma(r2, pc, C4);
+    double cr = __builtin_fma(r2 * r2, pc, __builtin_fma(-0.5, r2, 1.0));
+    int iq = (int)q;
+    double s = (iq & 1) ? cr : sr;
+    double c = (iq & 1) ? sr : cr;
+    *sp = (iq & 2) ? -s : s;
+    *cp = ((iq + 1) & 2) ? -c : c;
+}
+
+// 1/sqrt(d2) to ~1 ulp: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double d2) {
+    double y = __builtin_amdgcn_rsq(d2);
+    double hd = 0.5 * d2;
+    y = y * __builtin_fma(-hd * y, y, 1.5);
+    y = y * __builtin_fma(-hd * y, y, 1.5);
+    return y;
+}
+
+struct Acc {
+    double2 g, h, ht, e;
+};
+
+__device__ __forceinline__ Acc acc_zero() {
+    Acc a;
+    a.g = a.h = a.ht = a.e = make_double2(0.0, 0.0);
+    return a;
+}
+
+// tbem.rs:1404-1417: matrix coefficient of a field panel for its BC.
+__device__ __forceinline__ double2 coefficient(const Acc& r, const PanelBc& bc, double2 gamma,
+                                               double2 tau, double2 beta, double2 iwr) {
+    double2 c = make_double2(0.0, 0.0);
+    if (bc.type == 0) {
+        c = cadd(cmul(cmul(r.h, gamma), tau), cmul(r.e, beta));
+    } else if (bc.type == 1) {
+        double2 t = cadd(cmul(cmul(r.g, gamma), tau), cmul(r.ht, beta));
+        c = make_double2(-t.x, -t.y);
+    }
+    if (bc.type == 0 && bc.has_adm) {
+        double2 t = cadd(cmul(cmul(r.g, gamma), tau), cmul(r.ht, beta));
+        c = csub(c, cmul(cmul(t, make_double2(bc.adm[0], bc.adm[1])), iwr));
+    }
+    return c;
+}
+
+// tbem.rs:1419-1435: BC right-hand-side contribution of a field panel.
+__device__ __forceinline__ double2 bc_contribution(const Acc& r, const PanelBc& bc, double2 gamma,
+                                                   double2 tau, double2 beta, double2 iwr) {
+    double2 v = make_double2(bc.value[0], bc.value[1]);
+    if (bc.type == 0) return cmul(cmul(cadd(cmul(cmul(r.g, gamma), tau), cmul(r.ht, beta)), v), iwr);
+    if (bc.type == 1) return cmul(cadd(cmul(cmul(r.h, gamma), tau), cmul(r.e, beta)), v);
+    return make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void wave_sum_acc(Acc& a) {
+    a.g.x = wave_sum(a.g.x); a.g.y = wave_sum(a.g.y);
+    a.h.x = wave_sum(a.h.x); a.h.y = wave_sum(a.h.y);
+    a.ht.x = wave_sum(a.ht.x); a.ht.y = wave_sum(a.ht.y);
+    a.e.x = wave_sum(a.e.x); a.e.y = wave_sum(a.e.y);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-order evaluations (no contraction): one quadrature point of the
+// regular kernel, regular.rs:214-281.
+#pragma clang fp contract(off)
+
+__device__ inline void ref_point(const double pos[3], double wga, const double src[3],
+                                 const double srcn[3], const double nf[3], double wavruim, double k2,
+                                 Acc& r) {
+    double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
+    double dis = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (!(dis > 1e-15)) return;  // normalize() returns 0, then `dis < 1e-15` skips
+    double u0 = d0 / dis, u1 = d1 / dis, u2 = d2 / dis;
+    double re1 = wavruim * dis;
+    double re2 = wga / (FOUR_PI * dis);
+    double s, c;
+    sincos_red(re1, &s, &c);
+    double2 zg = make_double2(c * re2, s * re2);
+    double2 z1 = make_double2(-1.0 / dis, wavruim);
+    double2 zb = cmul(zg, z1);
+    double re1_h = u0 * nf[0] + u1 * nf[1] + u2 * nf[2];
+    double re2_h = -(u0 * srcn[0] + u1 * srcn[1] + u2 * srcn[2]);
+    double rq = re1_h * re2_h;
+    double nxny = srcn[0] * nf[0] + srcn[1] * nf[1] + srcn[2] * nf[2];
+    double dq = dis * dis;
+    double2 zef = make_double2((3.0 / dq - k2) * rq + nxny / dq, -wavruim / dis * (3.0 * rq + nxny));
+    r.g = cadd(r.g, zg);
+    r.h = cadd(r.h, cscale(zb, re1_h));
+    r.ht = cadd(r.ht, cscale(zb, re2_h));
+    r.e = cadd(r.e, cmul(zg, zef));
+}
+
+// Shape-function position of local (s, t) (regular.rs:682-690).
+__device__ inline void shape_pos(const double* c, double s, double t, double p[3]) {
+    double N0 = 1.0 - s - t;
+    p[0] = 0.0 + N0 * c[0] + s * c[3] + t * c[6];
+    p[1] = 0.0 + N0 * c[1] + s * c[4] + t * c[7];
+    p[2] = 0.0 + N0 * c[2] + s * c[5] + t * c[8];
+}
+
+// Vertices of the leaf with path `code` (midpoint arithmetic is exact).
+__device__ inline void decode_leaf(uint64_t code, double xi[3], double et[3]) {
+    xi[0] = 0.0; xi[1] = 1.0; xi[2] = 0.0;
+    et[0] = 0.0; et[1] = 0.0; et[2] = 1.0;
+    int depth = (int)(code >> LEAF_DEPTH_SHIFT);
+    for (int l = 0; l < depth; ++l) {
+        int ch = (int)((code >> (2 * l)) & 3u);
+        double mx[3] = {(xi[0] + xi[1]) / 2.0, (xi[1] + xi[2]) / 2.0, (xi[2] + xi[0]) / 2.0};
+        double me[3] = {(et[0] + et[1]) / 2.0, (et[1] + et[2]) / 2.0, (et[2] + et[0]) / 2.0};
+        if (ch == 3) {
+            xi[0] = mx[0]; xi[1] = mx[1]; xi[2] = mx[2];
+            et[0] = me[0]; et[1] = me[1]; et[2] = me[2];
+        } else {
+            int cm = (ch + 2) % 3;
+            double x0 = xi[ch], e0 = et[ch];
+            xi[0] = x0; xi[1] = mx[ch]; xi[2] = mx[cm];
+            et[0] = e0; et[1] = me[ch]; et[2] = me[cm];
+        }
+    }
+}
+
+// Point q of a subdivided leaf (regular.rs:164-196): affine map of the TR13
+// point into the leaf, weight times |det|, then compute_parameters.
+__device__ inline void near_point(const PanelGeom& G, uint64_t code, int q, const double src[3],
+                                  const double srcn[3], double wavruim, double k2, Acc& r) {
+    double xi[3], et[3];
+    decode_leaf(code, xi, et);
+    double cs = kTrS[q], ec = kTrT[q], wg = kTrW[q] * 0.5;
+    double l0 = 1.0 - cs - ec;
+    double xio = xi[0] * l0 + xi[1] * cs + xi[2] * ec;
+    double eto = et[0] * l0 + et[1] * cs + et[2] * ec;
+    double dx1 = xi[1] - xi[0], dy1 = et[1] - et[0];
+    double dx2 = xi[2] - xi[0], dy2 = et[2] - et[0];
+    double det = fabs(dx1 * dy2 - dx2 * dy1);
+    double weih2 = wg * det;
+    double pos[3];
+    shape_pos(G.c, xio, eto, pos);
+    ref_point(pos, weih2 * G.jac, src, srcn, G.nf, wavruim, k2, r);
+}
+
+// Full TR13 on the unsubdivided panel in reference order (bc_rhs_kernel).
+__device__ inline void full_point(const PanelGeom& G, int q, const double src[3], const double srcn[3],
+                                  double wavruim, double k2, Acc& r) {
+    double pos[3];
+    shape_pos(G.c, kTrS[q], kTrT[q], pos);
+    ref_point(pos, (kTrW[q] * 0.5) * G.jac, src, srcn, G.nf, wavruim, k2, r);
+}
+
+struct SelfTier {
+    int ngpo1, ngausin, nsec1, nsec2;
+};
+
+// singular.rs:51-85 QuadratureParams::for_ka
+__device__ inline SelfTier tier_for(double ka) {
+    if (ka < 0.3) return {3, 4, 4, 2};
+    if (ka < 1.0) return {4, 5, 6, 2};
+    if (ka < 2.0) return {5, 6, 8, 3};
+    return {6, 7, 10, 4};
+}
+
+__device__ inline int self_samples(const SelfTier& t) {
+    return 3 * t.nsec1 * t.ngpo1 + 3 * t.nsec2 * t.ngausin * t.ngausin;
+}
+
+// Sample s of the self-panel rule (singular.rs:322-410 edge part, :412-578
+// bulk part).  Edge samples add to E only; bulk samples to G, H, Ht and
+// E += k^2 G (n_x . n_y).  The sub-triangle loop repeats the second
+// sub-triangle for every isec >= 1, as the reference does.
+__device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s, const double src[3],
+                                   const double srcn[3], double wavruim, double k2, Acc& r) {
+    const double* c = G.c;
+    const int per_edge = t.nsec1 * t.ngpo1;
+    const int nedge = 3 * per_edge;
+    if (s < nedge) {
+        int ieg = s / per_edge;
+        int rem = s - ieg * per_edge;
+        int isec = rem / t.ngpo1;
+        int ig = rem - isec * t.ngpo1;
+        int ig1 = (ieg + 1) % 3;
+        double dpoi[3];
+        double leneg = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            dpoi[i] = c[3 * ig1 + i] - c[3 * ieg + i];
+            leneg += dpoi[i] * dpoi[i];
+        }
+        leneg = sqrt(leneg);
+        double dpoo[3] = {dpoi[0] / leneg, dpoi[1] / leneg, dpoi[2] / leneg};
+        double leneg_scaled = leneg / (2.0 * (double)t.nsec1);
+        double delsec = 2.0 / (double)t.nsec1;
+        double secmid = -1.0 - delsec / 2.0;
+        for (int k = 0; k <= isec; ++k) secmid += delsec;
+        double sga = secmid + kGlX[t.ngpo1][ig] / (double)t.nsec1;
+        double wga = kGlW[t.ngpo1][ig] * leneg_scaled;
+        double diff[3];
+        for (int i = 0; i < 3; ++i) {
+            double crd = c[3 * ieg + i] + dpoi[i] * (sga + 1.0) / 2.0;
+            diff[i] = crd - src[i];
+        }
+        double dis = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+        if (!(dis > 1e-15)) return;
+        double u[3] = {diff[0] / dis, diff[1] / dis, diff[2] / dis};
+        double re1 = wavruim * dis;
+        double re2 = FOUR_PI * dis;
+        double sn, cs;
+        sincos_red(re1, &sn, &cs);
+        double2 zg = make_double2(cs / re2, sn / re2);
+        double2 z1 = make_double2(-1.0 / dis, wavruim);
+        double2 zgf = cmul(zg, z1);
+        double2 zd0 = cscale(zgf, u[0]), zd1 = cscale(zgf, u[1]), zd2 = cscale(zgf, u[2]);
+        double2 w0 = csub(cscale(zd1, dpoo[2]), cscale(zd2, dpoo[1]));
+        double2 w1 = csub(cscale(zd2, dpoo[0]), cscale(zd0, dpoo[2]));
+        double2 w2 = csub(cscale(zd0, dpoo[1]), cscale(zd1, dpoo[0]));
+        double2 dot = cadd(cadd(cscale(w0, srcn[0]), cscale(w1, srcn[1])), cscale(w2, srcn[2]));
+        r.e = cadd(r.e, cscale(dot, wga));
+        return;
+    }
+    s -= nedge;
+    const int ng2 = t.ngausin * t.ngausin;
+    const int per_edge_b = t.nsec2 * ng2;
+    int ieg = s / per_edge_b;
+    int rem = s - ieg * per_edge_b;
+    int isec = rem / ng2;
+    int rem2 = rem - isec * ng2;
+    int i = rem2 / t.ngausin;
+    int j = rem2 - i * t.ngausin;
+    int ig1 = (ieg + 1) % 3;
+    int ig2 = ieg + 3;
+    const double* CSI6 = kCsi6;
+    const double* ETA6 = kEta6;
+    double aresub = 1.0 / 24.0 / (double)t.nsec2;
+    double ss0 = 1.0 / 3.0, ts0 = 1.0 / 3.0, ss1, ss2, ts1, ts2;
+    if (isec == 0) {
+        ss1 = CSI6[ieg]; ss2 = CSI6[ig2]; ts1 = ETA6[ieg]; ts2 = ETA6[ig2];
+    } else {
+        ss1 = CSI6[ig2]; ss2 = CSI6[ig1]; ts1 = ETA6[ig2]; ts2 = ETA6[ig1];
+    }
+    double wei = kGlW[t.ngausin][i] * kGlW[t.ngausin][j];
+    double sga = kGlX[t.ngausin][i], tga = kGlX[t.ngausin][j];
+    double sgg = 0.5 * (1.0 - sga) * ss0 + 0.25 * (1.0 + sga) * ((1.0 - tga) * ss1 + (1.0 + tga) * ss2);
+    double tgg = 0.5 * (1.0 - sga) * ts0 + 0.25 * (1.0 + sga) * ((1.0 - tga) * ts1 + (1.0 + tga) * ts2);
+    double pos[3];
+    shape_pos(c, sgg, tgg, pos);
+    double wga = wei * (1.0 + sga) * aresub * G.jac;
+    double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
+    double dis = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (!(dis > 1e-15)) return;
+    double u0 = d0 / dis, u1 = d1 / dis, u2 = d2 / dis;
+    double re1 = wavruim * dis;
+    double re2 = wga / (FOUR_PI * dis);
+    double sn, cs;
+    sincos_red(re1, &sn, &cs);
+    double2 zg = make_double2(cs * re2, sn * re2);
+    double2 z1 = make_double2(-1.0 / dis, wavruim);
+    double2 zb = cmul(zg, z1);
+    double re1_h = u0 * G.nf[0] + u1 * G.nf[1] + u2 * G.nf[2];
+    double re2_h = -(u0 * srcn[0] + u1 * srcn[1] + u2 * srcn[2]);
+    r.g = cadd(r.g, zg);
+    r.h = cadd(r.h, cscale(zb, re1_h));
+    r.ht = cadd(r.ht, cscale(zb, re2_h));
+    double nd = srcn[0] * G.nf[0] + srcn[1] * G.nf[1] + srcn[2] * G.nf[2];
+    r.e = cadd(r.e, cscale(cscale(zg, k2), nd));
+}
+
+#pragma clang fp contract(fast)
+
+// ---------------------------------------------------------------------------
+// Far kernel.  MODE 0: every field panel has a velocity BC without
+// admittance (rigid or vibrating body), so only H and E are needed and
+// A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
+template <int MODE>
+__global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ panels,
+                                                  const Source* __restrict__ sources,
+                                                  const PanelBc* __restrict__ bcs, int64_t n,
+                                                  int64_t lda, int rows_per_block, double k,
+                                                  double tau_r, double2 beta, double2 iwr,
+                                                  double2* __restrict__ A) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < n;
+    const int64_t jj = active ? j : n - 1;
+    const FarPanel P = panels[jj];
+    PanelBc bc;
+    if (MODE == 1) bc = bcs[jj];
+    const double k2 = k * k;
+    const int64_t i0 = (int64_t)blockIdx.y * rows_per_block;
+    const int64_t i1 = min(n, i0 + rows_per_block);
+    for (int64_t i = i0; i < i1; ++i) {
+        const Source src = sources[i];
+        const double rel0 = P.v0[0] - src.c[0];
+        const double rel1 = P.v0[1] - src.c[1];
+        const double rel2 = P.v0[2] - src.c[2];
+        const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
+        double hr = 0.0, hi = 0.0, er = 0.0, ei = 0.0, gr = 0.0, gi = 0.0, tr = 0.0, ti = 0.0;
+        // TR13 constants are uniform: scalar loads from constant memory.
+#pragma unroll 2
+        for (int q = 0; q < 13; ++q) {
+            const double sq = kTrS[q], tq = kTrT[q];
+            const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
+            const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
+            const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+            // The diagonal is overwritten by self_kernel; keep it finite.
+            const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);
+            const double inv = rsqrt_nr(d2);
+            const double dis = d2 * inv;
+            const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;     // r^.n_y
+            const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;  // -r^.n_x
+            double sn, cs;
+            sincos_red(k * dis, &sn, &cs);
+            const double re2 = (kTrW[q] * P.wj) * inv;
+            const double zgr = cs * re2, zgi = sn * re2;
+            // zb = zg * (-1/r + i k)
+            const double zbr = -zgr * inv - zgi * k;
+            const double zbi = zgr * k - zgi * inv;
+            hr = __builtin_fma(zbr, pn, hr);
+            hi = __builtin_fma(zbi, pn, hi);
+            const double inv2 = inv * inv;
+            const double rq = pn * qn;
+            const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
+            const double efi = -k * inv * (3.0 * rq + nxny);
+            er += zgr * efr - zgi * efi;
+            ei += zgr * efi + zgi * efr;
+            if (MODE == 1) {
+                gr += zgr;
+                gi += zgi;
+                tr = __builtin_fma(zbr, qn, tr);
+                ti = __builtin_fma(zbi, qn, ti);
+            }
+        }
+        double2 out;
+        if (MODE == 0) {
+            // (H * gamma) * tau + E * beta with gamma = 1
+            out.x = hr * tau_r + (er * beta.x - ei * beta.y);
+            out.y = hi * tau_r + (er * beta.y + ei * beta.x);
+        } else {
+            Acc a;
+            a.g = make_double2(gr, gi);
+            a.h = make_double2(hr, hi);
+            a.ht = make_double2(tr, ti);
+            a.e = make_double2(er, ei);
+            out = coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
+        }
+        if (active) A[i * lda + j] = out;
+    }
+}
+
+// One wave per near pair; lanes over (leaf, point).
+__global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ pairs, int64_t npairs,
+                                                   const uint64_t* __restrict__ leaves,
+                                                   const PanelGeom* __restrict__ geom,
+                                                   const Source* __restrict__ sources,
+                                                   const PanelBc* __restrict__ bcs, int general,
+                                                   int64_t lda, double k, double wavruim,
+                                                   double tau_r, double2 beta, double2 iwr,
+                                                   double2* __restrict__ A) {
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= npairs) return;
+    const NearPair np = pairs[p];
+    const PanelGeom G = geom[np.j];
+    const Source src = sources[np.i];
+    const double k2 = k * k;
+    Acc a = acc_zero();
+    const int total = np.count * 13;
+    for (int t = lane; t < total; t += 64) {
+        int leaf = t / 13;
+        int q = t - leaf * 13;
+        near_point(G, leaves[np.begin + leaf], q, src.c, src.n, wavruim, k2, a);
+    }
+    wave_sum_acc(a);
+    if (lane == 0) {
+        PanelBc bc;
+        if (general) {
+            bc = bcs[np.j];
+        } else {
+            bc.type = 0;
+            bc.has_adm = 0;
+        }
+        A[(int64_t)np.i * lda + np.j] =
+            coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
+    }
+}
+
+// One wave per panel: self integrals, free terms, BC free-term RHS.
+__global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__ geom,
+                                                   const Source* __restrict__ sources,
+                                                   const PanelBc* __restrict__ bcs, int general,
+                                                   int64_t n, int64_t lda, double k, double wavruim,
+                                                   double tau_r, double2 beta, double2 iwr,
+                                                   double2* __restrict__ A, double2* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const PanelGeom G = geom[i];
+    const Source src = sources[i];
+    const double k2 = k * k;
+    const SelfTier t = tier_for(k * G.size);
+    const int ns = self_samples(t);
+    Acc a = acc_zero();
+    for (int s = lane; s < ns; s += 64) self_sample(G, t, s, src.c, src.n, wavruim, k2, a);
+    wave_sum_acc(a);
+    if (lane == 0) {
+        PanelBc bc;
+        if (general) {
+            bc = bcs[i];
+        } else {
+            bc.type = 0;
+            bc.has_adm = 0;
+            bc.value[0] = bc.value[1] = 0.0;
+        }
+        const double2 gamma = make_double2(1.0, 0.0);
+        const double2 tau = make_double2(tau_r, 0.0);
+        double2 diag = make_double2(0.0, 0.0);
+        double2 local_rhs = make_double2(0.0, 0.0);
+        double2 avg = make_double2(bc.value[0], bc.value[1]);
+        // tbem.rs:1344-1358
+        if (bc.type == 0) {
+            diag = csub(diag, cscale(gamma, 0.5));
+            if (bc.has_adm) {
+                double2 adm = make_double2(bc.adm[0], bc.adm[1]);
+                diag = csub(diag, cscale(cmul(cmul(cmul(adm, iwr), beta), tau), 0.5));
+            }
+            local_rhs = cadd(local_rhs, cscale(cmul(cmul(cmul(avg, iwr), beta), tau), 0.5));
+        } else if (bc.type == 1) {
+            diag = cadd(diag, cscale(cmul(beta, tau), 0.5));
+            local_rhs = csub(local_rhs, cscale(cmul(avg, gamma), 0.5));
+        }
+        diag = cadd(diag, coefficient(a, bc, gamma, tau, beta, iwr));
+        A[i * lda + i] = diag;
+        rhs[i] = local_rhs;
+    }
+}
+
+// Block per row: sum over nonzero-BC columns of tbem.rs:1419-1435, in a
+// fixed order (wave w takes columns w, w+4, ...; waves summed 0..3).
+__global__ __launch_bounds__(256) void bc_rhs_kernel(const int32_t* __restrict__ cols, int64_t ncols,
+                                                     const NearPair* __restrict__ pairs,
+                                                     const int64_t* __restrict__ row_ptr,
+                                                     const uint64_t* __restrict__ leaves,
+                                                     const PanelGeom* __restrict__ geom,
+                                                     const Source* __restrict__ sources,
+                                                     const PanelBc* __restrict__ bcs, double k,
+                                                     double wavruim, double tau_r, double2 beta,
+                                                     double2 iwr, double2* __restrict__ rhs) {
+    __shared__ double part[4][2];
+    const int64_t i = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const Source src = sources[i];
+    const double k2 = k * k;
+    const double2 gamma = make_double2(1.0, 0.0);
+    const double2 tau = make_double2(tau_r, 0.0);
+    double2 sum = make_double2(0.0, 0.0);
+    const int64_t pb = row_ptr[i], pe = row_ptr[i + 1];
+    for (int64_t c = wave; c < ncols; c += 4) {
+        const int32_t j = cols[c];
+        const PanelGeom G = geom[j];
+        Acc a = acc_zero();
+        if (j == i) {
+            const SelfTier t = tier_for(k * G.size);
+            const int ns = self_samples(t);
+            for (int s = lane; s < ns; s += 64) self_sample(G, t, s, src.c, src.n, wavruim, k2, a);
+        } else {
+            int64_t lo = pb, hi = pe;  // pairs of row i are sorted by column
+            while (lo < hi) {
+                int64_t mid = (lo + hi) / 2;
+                if (pairs[mid].j < j) lo = mid + 1; else hi = mid;
+            }
+            if (lo < pe && pairs[lo].j == j) {
+                const NearPair np = pairs[lo];
+                const int total = np.count * 13;
+                for (int t = lane; t < total; t += 64) {
+                    int leaf = t / 13;
+                    near_point(G, leaves[np.begin + leaf], t - leaf * 13, src.c, src.n, wavruim, k2, a);
+                }
+            } else if (lane < 13) {
+                full_point(G, lane, src.c, src.n, wavruim, k2, a);
+            }
+        }
+        wave_sum_acc(a);
+        sum = cadd(sum, bc_contribution(a, bcs[j], gamma, tau, beta, iwr));
+    }
+    if (lane == 0) {
+        part[wave][0] = sum.x;
+        part[wave][1] = sum.y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 t = rhs[i];
+        for (int w = 0; w < 4; ++w) t = cadd(t, make_double2(part[w][0], part[w][1]));
+        rhs[i] = t;
+    }
+}
+
+// incident.rs:216-330, 807-834; bem_solver.rs:5005-5034 (reference order).
+#pragma clang fp contract(off)
+__global__ void plane_wave_kernel(const Source* __restrict__ sources, int64_t n,
+                                  const double* __restrict__ dirs, const double2* __restrict__ amps,
+                                  int64_t nrhs, double k, double hf, double tau_r, double2 beta,
+                                  double sign, int accumulate, double2* __restrict__ rhs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nrhs) return;
+    const int64_t i = t / nrhs;
+    const int64_t c = t - i * nrhs;
+    const Source s = sources[i];
+    const double* d = dirs + 3 * c;
+    double kdotx = hf * k * (d[0] * s.c[0] + d[1] * s.c[1] + d[2] * s.c[2]);
+    double kdotn = hf * k * (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]);
+    double sn, cs;
+    sincos_red(kdotx, &sn, &cs);
+    double2 p = cmul(amps[c], make_double2(cs, sn));
+    double2 dpdn = cmul(make_double2(0.0, kdotn), p);
+    double2 gp = cmul(make_double2(1.0, 0.0), p);
+    double2 bt = cmul(beta, make_double2(tau_r, 0.0));
+    double2 inc = cadd(gp, cmul(bt, dpdn));
+    inc = make_double2(-inc.x, -inc.y);
+    double2 add = cmul(inc, make_double2(sign, 0.0));
+    rhs[t] = accumulate ? cadd(rhs[t], add) : add;
+}
+
+#pragma clang fp contract(fast)
+
+__global__ void sincos_probe_kernel(const double* x, int64_t n, double* s, double* c) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) sincos_red(x[t], &s[t], &c[t]);
+}
+
+}  // namespace
+
+void upload_quadrature_tables(sbh_context* ctx) {
+    double X[9][8] = {}, W[9][8] = {};
+    for (int o = 1; o <= 8; ++o) gauss_legendre_rule(o, X[o], W[o]);
+    SBH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kGlX), X, sizeof(X)));
+    SBH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kGlW), W, sizeof(W)));
+    (void)ctx;
+}
+
+void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, sbh_system* sys) {
+    sbh_context* ctx = ws->ctx;
+    const sbh_mesh* m = ws->mesh;
+    const int64_t n = m->n;
+    const double k = ph.wave_number;
+    const double wavruim = ph.harmonic_factor * ph.wave_number;
+    const double2 beta = make_double2(beta_c.re, beta_c.im);
+    const double2 iwr = make_double2(0.0, ph.pressure_factor);
+    if (n == 0) return;
+    // The sin/cos argument reduction is valid for |k r| < 2^20.
+    SBH_REQUIRE(std::fabs(k) * m->diameter < 5.0e5,
+                "k * mesh diameter exceeds the supported phase range (5e5 rad)");
+    if (ph.harmonic_factor != 1.0)
+        throw Error(SBH_ERR_UNSUPPORTED, "only harmonic_factor = +1 (ACOUSTIC_CONVENTION) is supported");
+    const int general = m->general ? 1 : 0;
+    int rows_per_block = 16;
+    while ((n + rows_per_block - 1) / rows_per_block > 65535) rows_per_block *= 2;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + rows_per_block - 1) / rows_per_block));
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_FAR, &ev);
+    if (general)
+        hipLaunchKernelGGL(far_kernel<1>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr);
+    else
+        hipLaunchKernelGGL(far_kernel<0>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);
+    if (ws->near_pairs > 0) {
+        ctx->begin_timed(SBH_KERNEL_ASSEMBLE_NEAR, &ev);
+        hipLaunchKernelGGL(near_kernel, dim3((unsigned)((ws->near_pairs + 3) / 4)), dim3(256), 0,
+                           ctx->stream, ws->d_pairs.ptr, ws->near_pairs, ws->d_leaves.ptr,
+                           m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, general, sys->lda, k, wavruim,
+                           ph.tau, beta, iwr, sys->d_A.ptr);
+        SBH_HIP(hipGetLastError());
+        ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
+    }
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_SELF, &ev);
+    hipLaunchKernelGGL(self_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream,
+                       m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, general, n, sys->lda, k, wavruim,
+                       ph.tau, beta, iwr, sys->d_A.ptr, sys->d_rhs.ptr);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_SELF, ev);
+    const int64_t ncols = (int64_t)m->nonzero_bc.size();
+    if (ncols > 0) {
+        ctx->begin_timed(SBH_KERNEL_RHS, &ev);
+        hipLaunchKernelGGL(bc_rhs_kernel, dim3((unsigned)n), dim3(256), 0, ctx->stream,
+                           m->d_nonzero_bc.ptr, ncols, ws->d_pairs.ptr, ws->d_row_ptr.ptr,
+                           ws->d_leaves.ptr, m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, k, wavruim,
+                           ph.tau, beta, iwr, sys->d_rhs.ptr);
+        SBH_HIP(hipGetLastError());
+        ctx->end_timed(SBH_KERNEL_RHS, ev);
+    }
+}
+
+void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, sbh_c64 beta,
+                         double sign, const double* d_dirs, const double2* d_amps, int64_t num_rhs,
+                         double2* d_rhs, bool accumulate) {
+    const int64_t total = m->n * num_rhs;
+    if (total == 0) return;
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_RHS, &ev);
+    hipLaunchKernelGGL(plane_wave_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, m->d_src.ptr, m->n, d_dirs, d_amps, num_rhs, ph.wave_number,
+                       ph.harmonic_factor, ph.tau, make_double2(beta.re, beta.im), sign,
+                       accumulate ? 1 : 0, d_rhs);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_RHS, ev);
+}
+
+void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(sincos_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, x, n, s, c);
+    SBH_HIP(hipGetLastError());
+}
+
+}  // namespace sbh

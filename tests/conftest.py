@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library on device 0)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import sonium_amd as sa
+
+    c = sa.Context(0)
+    yield c
+    c.close()
