@@ -85,7 +85,14 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
     ap.add_argument("--cpu-rows", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 3 plain device copies of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        ctx = sa.Context(0)
+        sa._check(sa._lib().sbh_debug_device_copy(ctx.handle, args.calibrate_copy, 3))
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU row sharding is described in DESIGN.md but not built)", file=sys.stderr)

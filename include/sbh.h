@@ -267,6 +267,9 @@ sbh_status sbh_debug_subelements(const double* source_point, const double* eleme
 /* Test hook: the device sin/cos used by the assembly kernels, evaluated at
  * host-supplied points (for accuracy checks against the C library). */
 sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
+/* Test hook: `reps` launches of a plain device copy of `bytes` bytes (reads
+ * `bytes`, writes `bytes`), used to calibrate HBM byte counters. */
+sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps);
 
 #ifdef __cplusplus
 }

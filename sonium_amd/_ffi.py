@@ -111,6 +111,7 @@ SIGNATURES = {
                          ctypes.POINTER(gmres_solution_t)]),
     "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
+    "sbh_debug_device_copy": (_int, [_vp, _i64, _int]),
 }
 
 _lib = None

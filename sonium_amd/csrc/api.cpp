@@ -764,6 +764,18 @@ sbh_status sbh_debug_subelements(const double* source_point, const double* eleme
     });
 }
 
+sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && bytes >= 16 && reps >= 1, "invalid copy arguments");
+        use_device(ctx);
+        const int64_t n = bytes / 16;
+        DeviceArray<double2> a(n), b(n);
+        SBH_HIP(hipMemsetAsync(a.ptr, 0, a.bytes(), ctx->stream));
+        for (int r = 0; r < reps; ++r) device_copy(ctx, a.ptr, b.ptr, n);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {
     return guard([&] {
         SBH_REQUIRE(ctx && x && s && c && n >= 0, "null argument");

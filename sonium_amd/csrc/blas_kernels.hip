@@ -277,6 +277,12 @@ __global__ void inv_diag_kernel(const double2* __restrict__ A, int64_t lda, int6
     d[k] = make_double2(a.x / m, -a.y / m);
 }
 
+// Plain copy of n complex values (used to calibrate HBM byte counters).
+__global__ void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
 inline unsigned blocks_for(int64_t n, int per = BLOCK) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -390,6 +396,11 @@ int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out) {
 
 void diag_scale(sbh_context* ctx, const double2* d, const double2* x, double2* y, int64_t n) {
     hipLaunchKernelGGL(diag_scale_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, ctx->stream, d, x, y, n);
+    SBH_HIP(hipGetLastError());
+}
+
+void device_copy(sbh_context* ctx, const double2* src, double2* dst, int64_t n) {
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, ctx->stream, src, dst, n);
     SBH_HIP(hipGetLastError());
 }
 

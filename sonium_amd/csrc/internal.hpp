@@ -252,6 +252,8 @@ int residual_and_norm(sbh_context* ctx, const double2* b, double2* r, int64_t n,
 void diag_scale(sbh_context* ctx, const double2* d, const double2* x, double2* y, int64_t n);
 // d = 1 / diag(A)
 void extract_inverse_diagonal(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, double2* d);
+// dst = src (n complex values)
+void device_copy(sbh_context* ctx, const double2* src, double2* dst, int64_t n);
 // partials of ||x||^2
 int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out);
 int reduction_chunks(int64_t n);

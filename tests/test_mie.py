@@ -1,0 +1,94 @@
+"""Pins of the Mie oracle (solutions_3d.rs:241-387, 769-820) against the
+reference's own fixtures:
+
+* crates/sonium-wave/fixtures/scattering-golden-v1.json — rigid-sphere a_n at
+  ka = 1 (20 terms) and ka = 5 (36 terms), 45 digits (committed under
+  tests/golden/);
+* crates/sonium-wave/fixtures/special-golden-wolfram-v1.json — j_n, y_n at
+  x in {0.1, pi/4, pi/2, pi, 2pi, 50} from Wolfram (tests/golden/);
+and against scipy.special (an independent implementation) at ka = 2, 10, 14,
+plus the Wronskian identity that the surface formula uses.
+"""
+
+import json
+import math
+import os
+
+import mpmath
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import mie
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def test_rigid_coefficients_match_reference_golden():
+    g = _load("scattering-golden-v1.json")
+    for case in g["sphere"]:
+        ka = float(case["ka"])
+        ref = np.array([complex(float(c["re"]), float(c["im"])) for c in case["coefficients"]])
+        got = mie.rigid_sphere_coefficients(ka, len(ref))
+        # the fixture is rounded to 45 decimal places (absolute precision 5e-46)
+        err = np.abs(got - ref)
+        assert np.all(err <= 1e-13 * np.abs(ref) + 1e-45), (ka, err.max())
+
+
+def test_spherical_bessel_match_wolfram_golden():
+    g = _load("special-golden-wolfram-v1.json")
+    for case in g["spherical"]:
+        js = [float(v) for v in case["j"]]
+        ys = [float(v) for v in case["y"]]
+        with mpmath.workdps(60):  # the fixture's x carries 50 digits (pi/2 exactly, ...)
+            x = mpmath.mpf(str(case["x"]))
+            vals = [mie._sph(n, x) for n in range(len(js))]
+        for n, ((j, y), jr, yr) in enumerate(zip(vals, js, ys)):
+            # absolute 1e-40: y_0(pi/2) is exactly 0 in the 50-digit fixture
+            assert abs(float(j) - jr) <= 1e-14 * abs(jr) + 1e-40, (case["name"], n)
+            assert abs(float(y) - yr) <= 1e-14 * abs(yr) + 1e-40, (case["name"], n)
+
+
+@pytest.mark.parametrize("ka", [2.0, 10.0, 14.0])
+def test_coefficients_against_scipy(ka):
+    n = mie.suggested_sphere_terms(ka)
+    a = mie.rigid_sphere_coefficients(ka, n)
+    nn = np.arange(n)
+    jp = special.spherical_jn(nn, ka, derivative=True)
+    yp = special.spherical_yn(nn, ka, derivative=True)
+    ref = jp / (jp + 1j * yp)
+    big = np.abs(ref) > 1e-12
+    assert np.max(np.abs(a[big] - ref[big]) / np.abs(ref[big])) < 1e-10
+
+
+@pytest.mark.parametrize("ka", [1.0, 2.0, 10.0])
+def test_surface_wronskian_form_equals_direct_series(ka):
+    theta = np.linspace(0.0, math.pi, 13)
+    direct = mie.total_field(ka, 1.0, 1.0, theta)
+    pts = np.stack([np.sin(theta), np.zeros_like(theta), np.cos(theta)], axis=1)
+    wr = mie.surface_pressure(ka, 1.0, pts)
+    assert np.max(np.abs(wr - direct)) < 1e-11 * np.max(np.abs(direct))
+
+
+def test_truncation_rule_and_convergence():
+    assert mie.suggested_sphere_terms(1.0) == 20
+    assert mie.suggested_sphere_terms(10.0) == 56
+    assert mie.suggested_sphere_terms(0.1) == 20
+    assert mie.suggested_sphere_terms(float("nan")) == 16
+    pts = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])
+    p1 = mie.surface_pressure(10.0, 1.0, pts)
+    p2 = mie.surface_pressure(10.0, 1.0, pts, num_terms=80)
+    assert np.max(np.abs(p1 - p2)) < 1e-12
+
+
+def test_rigid_sphere_low_frequency_limit():
+    """ka -> 0: the total surface pressure tends to the incident field plus the
+    rigid-sphere dipole/monopole corrections; at ka = 1e-3 it is ~1 everywhere."""
+    pts = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])
+    p = mie.surface_pressure(1e-3, 1.0, pts)
+    assert np.max(np.abs(p - 1.0)) < 5e-3
