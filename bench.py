@@ -40,6 +40,13 @@ FP64_PEAK_TFLOPS = 78.6
 FLOPS_PER_EVAL = 70      # FP64 flops per kernel evaluation (FMA = 2, sqrt/div/sincos = 1)
 FLOPS_PER_COEFF = 14     # coefficient H*gamma*tau + E*beta per pair
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (copy-calibrated; profiles/zgemv_n_kernel_and_far_kernel_c2_icosphere20480_k10.md).
+MEASURED_TRAFFIC = {
+    ("zgemv_n_kernel", 20480): 6.7134e9 + 3.2971e5,
+    ("far_kernel", 20480): 1.8935e7 + 6.7109e9,
+}
+
 WORKLOADS = {
     "c2": {"subdiv": 5, "k": 10.0, "restart": 100,
            "name": "rigid sphere icosphere 20480 panels, k=10, beta=i/k, 1 plane wave +z, GMRES(100) rtol 1e-8, complex128"},
@@ -160,12 +167,13 @@ def main():
     if mv_ms >= far_ms:
         achieved = mv_bytes / mv_avg_s / 1e9
         roofline = {"kernel": "zgemv_n_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": MEASURED_TRAFFIC.get(("zgemv_n_kernel", n))}
     else:
         achieved = far_flops / far_avg_s / 1e12
         roofline = {"kernel": "far_kernel", "bound": "fp64-valu", "achieved": achieved,
                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                    "traffic": None}
+                    "traffic": MEASURED_TRAFFIC.get(("far_kernel", n))}
 
     cpu = None
     if not args.no_cpu_baseline:
