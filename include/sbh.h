@@ -97,7 +97,8 @@ typedef enum {
     SBH_KERNEL_MATVEC = 4,
     SBH_KERNEL_KRYLOV_VECTOR = 5,
     SBH_KERNEL_BLOCK = 6,
-    SBH_KERNEL_CLASS_COUNT = 7
+    SBH_KERNEL_LOWP = 7,
+    SBH_KERNEL_CLASS_COUNT = 8
 } sbh_kernel_class;
 /* Synchronizes, then reports launches and summed event time of one class
  * since the last reset. */
@@ -255,6 +256,63 @@ typedef struct {
 sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, const sbh_c64* x0,
                      const sbh_gmres_config* config, sbh_c64* x, int memory,
                      sbh_gmres_solution* solution);
+
+/* Many right-hand sides — BemSolver::solve_many (bem_solver.rs:2032,
+ * 2175-2181) over apply_block: B, X0 (NULL = zero) and X are n x k row-major
+ * (1 <= k <= 64); solutions[k].  Each column follows gmres_core exactly;
+ * the columns share one block product per Arnoldi step. */
+sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0, int64_t k,
+                           const sbh_gmres_config* config, sbh_c64* X, int memory,
+                           sbh_gmres_solution* solutions);
+
+/* ---------------------------------------------------------------------------
+ * Mixed precision.  A complex FP16 or BF16 copy of a system matrix, every row
+ * scaled by a power of two (crates/sonium-compute/src/scaled_storage.rs:1-127
+ * is the reference's explicit-scale storage precedent); products accumulate
+ * in FP32 (MFMA for k > 1). */
+typedef struct sbh_lowp_matrix sbh_lowp_matrix;
+typedef enum { SBH_LOWP_FP16 = 0, SBH_LOWP_BF16 = 1 } sbh_lowp_format;
+/* Converts the system's current matrix; sys must outlive the copy. */
+sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix** out);
+/* Re-converts after the system was re-assembled. */
+sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp);
+void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp);
+/* Y ~= A X in low precision, X and Y n x k row-major (1 <= k <= 64). */
+sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory);
+/* Test hook: the stored halves (n x 2*ld, {re, im} per entry), the row
+ * inverse scales (n) and ld. */
+sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, double* row_inv, int64_t* ld);
+
+/* FGMRES with iterative refinement — GmresIrConfig / GmresIrStatus and
+ * gmres_ir_backend_preconditioned (crates/sonium-compute/src/gmres_ir.rs:
+ * 536-665; Carson & Higham 2018, Alg. 5.3): FP64 outer flexible GMRES
+ * (residual, Arnoldi, Givens, update) preconditioned by an inner GMRES on the
+ * low-precision copy.  restart_size <= 0 means one cycle of
+ * max_outer_iterations. */
+typedef struct {
+    int64_t max_outer_iterations; /* default 30 */
+    double outer_tolerance;       /* default 1e-10 */
+    int64_t max_inner_iterations; /* default 50 */
+    double inner_tolerance;       /* default 1e-3 */
+    int64_t restart_size;         /* default 0 (none) */
+} sbh_gmres_ir_config;
+
+typedef struct {
+    int64_t outer_iterations;
+    int64_t total_inner_iterations;
+    double estimated_residual;
+    double final_outer_residual; /* recomputed FP64 ||b - A x|| / ||b|| */
+    double backward_error;
+    int32_t converged;
+    int32_t status; /* sbh_solver_status */
+    int64_t restart_cycles;
+    int64_t fp64_products; /* block products with the FP64 matrix */
+    int64_t lowp_products; /* block products with the low-precision copy */
+} sbh_gmres_ir_status;
+
+sbh_status sbh_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const sbh_c64* B, int64_t k,
+                        const sbh_gmres_ir_config* config, sbh_c64* X, int memory,
+                        sbh_gmres_ir_status* status);
 
 /* ---------------------------------------------------------------------------
  * Test hooks (no GPU work): the host route builder's view of one pair, so

@@ -578,6 +578,137 @@ def gmres_device(op: DenseMatrixOperator, b_ptr: int, x: DeviceVector, config: G
     return _solution(raw, None)
 
 
+def gmres_block(op: DenseMatrixOperator, B, config: GmresConfig, X0=None):
+    """Many right-hand sides (BemSolver::solve_many, bem_solver.rs:2032): B is n x k (k <= 64).
+
+    Returns (X, [GmresSolution per column]); every column follows the
+    single-RHS gmres_core, the columns share one block product per step.
+    """
+    Bv = np.ascontiguousarray(B, dtype=np.complex128)
+    n = op.num_rows()
+    if Bv.ndim == 1:
+        Bv = Bv.reshape(n, 1)
+    k = Bv.shape[1]
+    X = np.zeros_like(Bv)
+    X0v = None if X0 is None else np.ascontiguousarray(X0, dtype=np.complex128).reshape(n, k)
+    raws = (_ffi.gmres_solution_t * k)()
+    cfg = config._c()
+    _check(_lib().sbh_gmres_block(op.handle, _cptr(Bv), None if X0v is None else _cptr(X0v), k,
+                                  ctypes.byref(cfg), _cptr(X), MEM_HOST, raws))
+    return X, [_solution(raws[c], X[:, c]) for c in range(k)]
+
+
+LOWP_FORMATS = {"fp16": 0, "bf16": 1}
+
+
+class LowpMatrix:
+    """Complex FP16/BF16 copy of a TbemSystem's matrix with power-of-two row scaling."""
+
+    def __init__(self, system: TbemSystem, fmt: str = "fp16"):
+        self.system = system
+        self.format = fmt
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_lowp_matrix_create(system.handle, LOWP_FORMATS[fmt], ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def update(self):
+        _check(_lib().sbh_lowp_matrix_update(self._h))
+
+    def apply(self, X) -> np.ndarray:
+        n = self.system.num_dofs
+        Xv = np.ascontiguousarray(X, dtype=np.complex128)
+        k = 1 if Xv.ndim == 1 else Xv.shape[1]
+        Y = np.empty_like(Xv)
+        _check(_lib().sbh_lowp_matrix_apply(self._h, _cptr(Xv), _cptr(Y), k, MEM_HOST))
+        return Y
+
+    def download(self):
+        """(halves as uint16 n x 2*ld, row inverse scales, ld)."""
+        n = self.system.num_dofs
+        ld = ctypes.c_int64()
+        _check(_lib().sbh_lowp_matrix_download(self._h, None, None, ctypes.byref(ld)))
+        halves = np.empty((n, 2 * ld.value), dtype=np.uint16)
+        inv = np.empty(n)
+        _check(_lib().sbh_lowp_matrix_download(self._h, _cptr(halves), _cptr(inv), ctypes.byref(ld)))
+        return halves, inv, int(ld.value)
+
+    def dequantized(self) -> np.ndarray:
+        halves, inv, ld = self.download()
+        n = self.system.num_dofs
+        if self.format == "fp16":
+            vals = halves.view(np.float16).astype(np.float64)
+        else:
+            vals = (halves.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        z = vals[:, 0:2 * n:2] + 1j * vals[:, 1:2 * n:2]
+        return z * inv[:, None]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_lowp_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class GmresIrConfig:
+    """GmresIrConfig (crates/sonium-compute/src/gmres_ir.rs:540-580); defaults as there."""
+
+    max_outer_iterations: int = 30
+    outer_tolerance: float = 1e-10
+    max_inner_iterations: int = 50
+    inner_tolerance: float = 1e-3
+    restart_size: int = 0
+
+    def _c(self):
+        return _ffi.gmres_ir_config_t(int(self.max_outer_iterations), float(self.outer_tolerance),
+                                      int(self.max_inner_iterations), float(self.inner_tolerance),
+                                      int(self.restart_size))
+
+
+@dataclass
+class GmresIrStatus:
+    outer_iterations: int
+    total_inner_iterations: int
+    estimated_residual: float
+    final_outer_residual: float
+    backward_error: float
+    converged: bool
+    status: str
+    restart_cycles: int
+    fp64_products: int
+    lowp_products: int
+
+
+def gmres_ir(op: DenseMatrixOperator, lp: LowpMatrix, B, config: GmresIrConfig):
+    """Mixed-precision FGMRES-IR (gmres_ir.rs:665): returns (X, [GmresIrStatus per column])."""
+    Bv = np.ascontiguousarray(B, dtype=np.complex128)
+    n = op.num_rows()
+    if Bv.ndim == 1:
+        Bv = Bv.reshape(n, 1)
+    k = Bv.shape[1]
+    X = np.zeros_like(Bv)
+    raws = (_ffi.gmres_ir_status_t * k)()
+    cfg = config._c()
+    _check(_lib().sbh_gmres_ir(op.handle, lp.handle, _cptr(Bv), k, ctypes.byref(cfg), _cptr(X), MEM_HOST, raws))
+    out = []
+    for c in range(k):
+        r = raws[c]
+        out.append(GmresIrStatus(int(r.outer_iterations), int(r.total_inner_iterations), float(r.estimated_residual),
+                                 float(r.final_outer_residual), float(r.backward_error), bool(r.converged),
+                                 SOLVER_STATUS[int(r.status)], int(r.restart_cycles), int(r.fp64_products),
+                                 int(r.lowp_products)))
+    return X, out
+
+
 def device_sincos(ctx: Context, x) -> tuple:
     """The assembly kernels' sin/cos evaluated on the device (test hook)."""
     xv = np.ascontiguousarray(x, dtype=np.float64)
@@ -603,5 +734,6 @@ __all__ = [
     "build_tbem_system", "build_tbem_system_with_beta", "IncidentField", "DenseMatrixOperator",
     "IdentityPreconditioner", "JacobiPreconditioner", "GmresConfig", "GmresSolution", "gmres",
     "gmres_with_guess", "gmres_preconditioned", "gmres_preconditioned_with_guess", "gmres_device",
-    "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE",
+    "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE", "gmres_block", "LowpMatrix", "GmresIrConfig",
+    "GmresIrStatus", "gmres_ir",
 ]

@@ -23,6 +23,7 @@ KERNEL_CLASSES = {
     "matvec": 4,
     "krylov_vector": 5,
     "block": 6,
+    "lowp": 7,
 }
 
 
@@ -47,6 +48,20 @@ class gmres_solution_t(ctypes.Structure):
                 ("backward_error", ctypes.c_double), ("converged", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("spmv_count", ctypes.c_int64),
                 ("reorthogonalizations", ctypes.c_int64)]
+
+
+class gmres_ir_config_t(ctypes.Structure):
+    _fields_ = [("max_outer_iterations", ctypes.c_int64), ("outer_tolerance", ctypes.c_double),
+                ("max_inner_iterations", ctypes.c_int64), ("inner_tolerance", ctypes.c_double),
+                ("restart_size", ctypes.c_int64)]
+
+
+class gmres_ir_status_t(ctypes.Structure):
+    _fields_ = [("outer_iterations", ctypes.c_int64), ("total_inner_iterations", ctypes.c_int64),
+                ("estimated_residual", ctypes.c_double), ("final_outer_residual", ctypes.c_double),
+                ("backward_error", ctypes.c_double), ("converged", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("restart_cycles", ctypes.c_int64),
+                ("fp64_products", ctypes.c_int64), ("lowp_products", ctypes.c_int64)]
 
 
 _vp = ctypes.c_void_p
@@ -109,6 +124,15 @@ SIGNATURES = {
     "sbh_precond_apply": (_int, [_vp, _vp, _vp, _int]),
     "sbh_gmres": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(gmres_config_t), _vp, _int,
                          ctypes.POINTER(gmres_solution_t)]),
+    "sbh_gmres_block": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(gmres_config_t), _vp, _int,
+                               ctypes.POINTER(gmres_solution_t)]),
+    "sbh_lowp_matrix_create": (_int, [_vp, _int, _pp]),
+    "sbh_lowp_matrix_update": (_int, [_vp]),
+    "sbh_lowp_matrix_destroy": (None, [_vp]),
+    "sbh_lowp_matrix_apply": (_int, [_vp, _vp, _vp, _i64, _int]),
+    "sbh_lowp_matrix_download": (_int, [_vp, _vp, _vp, ctypes.POINTER(_i64)]),
+    "sbh_gmres_ir": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(gmres_ir_config_t), _vp, _int,
+                            ctypes.POINTER(gmres_ir_status_t)]),
     "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sbh_debug_device_copy": (_int, [_vp, _i64, _int]),

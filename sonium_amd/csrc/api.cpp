@@ -9,6 +9,15 @@
 #include "internal.hpp"
 
 namespace sbh {
+sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, int k, const sbh_gmres_config& cfg,
+                           sbh_gmres_solution* sols);
+sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_B, double2* d_X, int k,
+                        const sbh_gmres_ir_config& cfg, sbh_gmres_ir_status* st);
+void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
+void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+               const double2* x, float2* x32, double2* y);
+void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, double2* Y);
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
                      const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol);
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
@@ -679,7 +688,10 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
         use_device(op->ctx);
         DeviceInput in(op->ctx, X, (size_t)(n * k), memory);
         DeviceOutput outv(op->ctx, Y, (size_t)(n * k), memory);
-        zgemm_block(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, in.ptr, outv.ptr, k);
+        if (k == 1)
+            zgemv_n(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, n, in.ptr, outv.ptr);
+        else
+            zgemm_f64(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, in.ptr, k, outv.ptr);
         outv.finish();
     });
 }
@@ -754,6 +766,123 @@ sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, c
         dx.finish();
     });
     return g != SBH_OK ? g : st;
+}
+
+sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0, int64_t k,
+                           const sbh_gmres_config* config, sbh_c64* X, int memory,
+                           sbh_gmres_solution* solutions) {
+    return guard([&] {
+        SBH_REQUIRE(op && B && config && X && solutions, "null argument");
+        SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
+        check_memory(memory);
+        const int64_t n = op->sys->n;
+        use_device(op->ctx);
+        DeviceInput db(op->ctx, B, (size_t)(n * k), memory);
+        DeviceOutput dx(op->ctx, X, (size_t)(n * k), memory);
+        if (X0) {
+            DeviceInput dx0(op->ctx, X0, (size_t)(n * k), memory);
+            SBH_HIP(hipMemcpyAsync(dx.ptr, dx0.ptr, sizeof(double2) * n * k, hipMemcpyDeviceToDevice, op->ctx->stream));
+            SBH_HIP(hipStreamSynchronize(op->ctx->stream));
+        } else {
+            SBH_HIP(hipMemsetAsync(dx.ptr, 0, sizeof(double2) * n * k, op->ctx->stream));
+        }
+        sbh_status st = run_gmres_block(op, db.ptr, dx.ptr, (int)k, *config, solutions);
+        dx.finish();
+        if (st != SBH_OK) throw Error(st, "block GMRES failed");
+    });
+}
+
+sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix** out) {
+    return guard([&] {
+        SBH_REQUIRE(sys && out, "system/out is null");
+        SBH_REQUIRE(format == SBH_LOWP_FP16 || format == SBH_LOWP_BF16, "format must be FP16 or BF16");
+        use_device(sys->ctx);
+        std::unique_ptr<sbh_lowp_matrix> lp(new sbh_lowp_matrix());
+        lp->ctx = sys->ctx;
+        lp->sys = sys;
+        lp->n = sys->n;
+        lp->ld = lowp_padded_ld(sys->n);
+        lp->format = format;
+        lp->d_A.allocate((size_t)lp->n * 2 * lp->ld);
+        lp->d_row_inv.allocate((size_t)lp->n);
+        lp->d_XT.allocate((size_t)128 * 2 * lp->ld);
+        lp->d_colmax.allocate(64);
+        lp->d_colinv.allocate(64);
+        lp->d_x32.allocate((size_t)lp->ld);
+        SBH_HIP(hipMemsetAsync(lp->d_XT.ptr, 0, lp->d_XT.bytes(), sys->ctx->stream));
+        lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr);
+        SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
+        context_retain(sys->ctx);
+        *out = lp.release();
+    });
+}
+
+sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp) {
+    return guard([&] {
+        SBH_REQUIRE(lp, "matrix is null");
+        use_device(lp->ctx);
+        lowp_convert(lp->ctx, lp->sys->d_A.ptr, lp->sys->lda, lp->n, lp->ld, lp->format, lp->d_A.ptr,
+                     lp->d_row_inv.ptr);
+    });
+}
+
+void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp) {
+    if (!lp) return;
+    sbh_context* ctx = lp->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete lp;
+    context_release(ctx);
+}
+
+sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory) {
+    return guard([&] {
+        SBH_REQUIRE(lp && X && Y, "null argument");
+        SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
+        check_memory(memory);
+        use_device(lp->ctx);
+        const int64_t n = lp->n;
+        DeviceInput in(lp->ctx, X, (size_t)(n * k), memory);
+        DeviceOutput outv(lp->ctx, Y, (size_t)(n * k), memory);
+        if (k == 1)
+            lowp_gemv(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, lp->d_x32.ptr, outv.ptr);
+        else
+            lowp_gemm(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, (int)k, lp->d_XT.ptr,
+                      lp->d_colmax.ptr, lp->d_colinv.ptr, outv.ptr);
+        outv.finish();
+    });
+}
+
+sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, double* row_inv, int64_t* ld) {
+    return guard([&] {
+        SBH_REQUIRE(lp, "matrix is null");
+        use_device(lp->ctx);
+        SBH_HIP(hipStreamSynchronize(lp->ctx->stream));
+        if (halves) SBH_HIP(hipMemcpy(halves, lp->d_A.ptr, lp->d_A.bytes(), hipMemcpyDeviceToHost));
+        if (row_inv) SBH_HIP(hipMemcpy(row_inv, lp->d_row_inv.ptr, lp->d_row_inv.bytes(), hipMemcpyDeviceToHost));
+        if (ld) *ld = lp->ld;
+    });
+}
+
+sbh_status sbh_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const sbh_c64* B, int64_t k,
+                        const sbh_gmres_ir_config* config, sbh_c64* X, int memory, sbh_gmres_ir_status* status) {
+    return guard([&] {
+        SBH_REQUIRE(op && lp && B && config && X && status, "null argument");
+        SBH_REQUIRE(lp->sys == op->sys, "the low-precision copy belongs to another system");
+        SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
+        SBH_REQUIRE(config->max_outer_iterations >= 1 && config->max_inner_iterations >= 1 &&
+                        std::isfinite(config->outer_tolerance) && config->outer_tolerance > 0.0 &&
+                        std::isfinite(config->inner_tolerance) && config->inner_tolerance > 0.0,
+                    "invalid GMRES-IR configuration");
+        check_memory(memory);
+        use_device(op->ctx);
+        const int64_t n = op->sys->n;
+        DeviceInput db(op->ctx, B, (size_t)(n * k), memory);
+        DeviceOutput dx(op->ctx, X, (size_t)(n * k), memory);
+        sbh_status st = run_gmres_ir(op, lp, db.ptr, dx.ptr, (int)k, *config, status);
+        dx.finish();
+        if (st != SBH_OK) throw Error(st, "GMRES-IR failed");
+    });
 }
 
 sbh_status sbh_debug_subelements(const double* source_point, const double* element_coords,

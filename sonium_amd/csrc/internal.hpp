@@ -65,6 +65,28 @@ struct DeviceArray {
     size_t bytes() const { return count * sizeof(T); }
 };
 
+// Owning pinned host allocation.
+template <typename T>
+struct PinnedArray {
+    T* ptr = nullptr;
+    size_t count = 0;
+    PinnedArray() = default;
+    explicit PinnedArray(size_t n) { allocate(n); }
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    ~PinnedArray() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    void allocate(size_t n) {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        count = 0;
+        if (n == 0) return;
+        SBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T), hipHostMallocDefault));
+        count = n;
+    }
+};
+
 // Per-panel data for the far-field kernel: vertex 0, the two edge vectors
 // (exactly v1-v0 and v2-v0, the reference's dx/ds and dx/dt), the
 // vertex-order unit normal, and 0.5*J/(4*pi) so that a TR13 weight w gives
@@ -192,6 +214,22 @@ struct sbh_vector {
     sbh::DeviceArray<double2> d;
 };
 
+// Complex FP16/BF16 copy of a system matrix with power-of-two row scales,
+// plus scratch for its products.
+struct sbh_lowp_matrix {
+    sbh_context* ctx = nullptr;
+    sbh_system* sys = nullptr;
+    int64_t n = 0;
+    int64_t ld = 0;  // complex entries per row, multiple of 64
+    int format = 0;  // 0 FP16, 1 BF16
+    sbh::DeviceArray<uint16_t> d_A;       // n x 2*ld halves
+    sbh::DeviceArray<double> d_row_inv;   // n
+    sbh::DeviceArray<uint16_t> d_XT;      // 128 x 2*ld halves (block operand)
+    sbh::DeviceArray<unsigned> d_colmax;  // 64
+    sbh::DeviceArray<double> d_colinv;    // 64
+    sbh::DeviceArray<float2> d_x32;       // ld (vector operand)
+};
+
 struct sbh_operator {
     sbh_context* ctx = nullptr;
     sbh_system* sys = nullptr;
@@ -256,6 +294,10 @@ void extract_inverse_diagonal(sbh_context* ctx, const double2* A, int64_t lda, i
 void device_copy(sbh_context* ctx, const double2* src, double2* dst, int64_t n);
 // partials of ||x||^2
 int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out);
+// Low-precision conversion (lowp_kernels.hip)
+void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
+                  uint16_t* out, double* row_inv);
+int64_t lowp_padded_ld(int64_t n);
 int reduction_chunks(int64_t n);
 
 }  // namespace sbh

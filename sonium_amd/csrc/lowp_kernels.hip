@@ -1,0 +1,610 @@
+// Low-precision storage, block products and batched Krylov vector kernels.
+//
+// Complex FP16/BF16 storage with power-of-two row scaling (the reference's
+// explicit-scale storage precedent: crates/sonium-compute/src/scaled_storage.rs:1-127)
+// and its products with FP32 accumulation, plus the FP64 block product
+// (DenseMatrixOperator::apply_block → DenseMatrix::matmul_into, bem_solver.rs:6567-6574,
+// dense.rs:450-488) and the column-batched vector kernels that let k
+// right-hand sides share one block product per Krylov step.
+//
+// Layout.  The low-precision matrix is row-major with ld >= n complex
+// entries per row (ld a multiple of 64, zero padded); an entry is two 16-bit
+// halves {re, im}, so a row is also the real row [re0 im0 re1 im1 ...] of
+// length 2*ld.  Row i stores A_ij * 2^(T - e_i) where 2^e_i <= max_j |A_ij| <
+// 2^(e_i+1) and T = 14 (FP16: the largest scaled entry stays below 2^15) or
+// 0 (BF16 has the FP32 exponent range); row_inv[i] = 2^(e_i - T) undoes it
+// exactly.
+//
+// Complex GEMM on MFMA.  With A_r the N x 2N real view of the complex
+// matrix, Y = A X is the real product A_r * X_r where X_r (2N x 2k) holds, for
+// complex row j, the rows [xr_j, xi_j] and [-xi_j, xr_j]; so output column
+// c < k is Re Y[:, c] and column k + c is Im Y[:, c].  X_r is stored
+// transposed (XT: BN x 2 ld, BN = 2k rounded up to 32) so each lane's MFMA B
+// operand (8 consecutive k for one column) is one 16-byte load, the same as
+// the A operand (8 consecutive k of one row).  Each column of X is scaled by
+// a power of two as well.  v_mfma_f32_32x32x16_{f16,bf16}: A 32x16, B 16x32,
+// C 32x32 FP32 in 16 registers per lane.
+
+#include "internal.hpp"
+
+namespace sbh {
+namespace {
+
+constexpr int BLOCK = 256;
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+template <int FMT>
+struct Half;
+template <>
+struct Half<0> {
+    static constexpr int TOP = 14;
+    __device__ static u16 from_float(float f) {
+        _Float16 h = (_Float16)f;
+        return __builtin_bit_cast(u16, h);
+    }
+    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(_Float16, v); }
+};
+template <>
+struct Half<1> {
+    static constexpr int TOP = 0;
+    __device__ static u16 from_float(float f) {
+        __bf16 h = (__bf16)f;
+        return __builtin_bit_cast(u16, h);
+    }
+    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(__bf16, v); }
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One block per row: row max, power-of-two scale, conversion.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void lp_convert_kernel(const double2* __restrict__ A, int64_t lda,
+                                                           int64_t n, int64_t ld, u16* __restrict__ out,
+                                                           double* __restrict__ row_inv) {
+    __shared__ double sh[BLOCK / 64];
+    const int64_t i = blockIdx.x;
+    const double2* row = A + i * lda;
+    double m = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += BLOCK) {
+        double2 v = row[j];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    int e = 0;
+    if (m > 0.0 && isfinite(m)) e = ilogb(m);
+    const double scale = ldexp(1.0, Half<FMT>::TOP - e);
+    if (threadIdx.x == 0) row_inv[i] = ldexp(1.0, e - Half<FMT>::TOP);
+    u16* o = out + i * 2 * ld;
+    for (int64_t j = threadIdx.x; j < ld; j += BLOCK) {
+        u16 re = 0, im = 0;
+        if (j < n) {
+            double2 v = row[j];
+            re = Half<FMT>::from_float((float)(v.x * scale));
+            im = Half<FMT>::from_float((float)(v.y * scale));
+        }
+        o[2 * j] = re;
+        o[2 * j + 1] = im;
+    }
+}
+
+// y = diag(row_inv) * (A_lp x), x given in FP32 (ld entries, zero padded).
+// Each wave owns 4 rows; a lane loads 4 complex entries (16 bytes) per row per step.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void lp_gemv_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
+                                                        const float2* __restrict__ x,
+                                                        const double* __restrict__ row_inv,
+                                                        double2* __restrict__ y) {
+    constexpr int ROWS = 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * ROWS;
+    if (r0 >= n) return;
+    const u32x4* rows[ROWS];
+    float sr[ROWS], si[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        int64_t rr = min(r0 + r, n - 1);
+        rows[r] = reinterpret_cast<const u32x4*>(A + rr * 2 * ld);
+        sr[r] = 0.f;
+        si[r] = 0.f;
+    }
+    const int64_t quads = ld / 4;  // 4 complex entries per 16-byte load
+    const float4* xq = reinterpret_cast<const float4*>(x);
+    for (int64_t q = lane; q < quads; q += 64) {
+        float4 x01 = xq[2 * q];
+        float4 x23 = xq[2 * q + 1];
+        float xr[4] = {x01.x, x01.z, x23.x, x23.z};
+        float xi[4] = {x01.y, x01.w, x23.y, x23.w};
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            u32x4 a = __builtin_nontemporal_load(rows[r] + q);
+            unsigned w[4] = {a[0], a[1], a[2], a[3]};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float ar = Half<FMT>::to_float((u16)(w[e] & 0xffffu));
+                float ai = Half<FMT>::to_float((u16)(w[e] >> 16));
+                sr[r] = __builtin_fmaf(ar, xr[e], sr[r]);
+                sr[r] = __builtin_fmaf(-ai, xi[e], sr[r]);
+                si[r] = __builtin_fmaf(ar, xi[e], si[r]);
+                si[r] = __builtin_fmaf(ai, xr[e], si[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        sr[r] = wave_sumf(sr[r]);
+        si[r] = wave_sumf(si[r]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            if (r0 + r < n) {
+                double s = row_inv[r0 + r];
+                y[r0 + r] = make_double2((double)sr[r] * s, (double)si[r] * s);
+            }
+    }
+}
+
+// FP64 vector -> FP32 vector (ld entries, zero padded).
+__global__ void to_f32_kernel(const double2* __restrict__ x, int64_t n, int64_t ld, float2* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    double2 v = j < n ? x[j] : make_double2(0.0, 0.0);
+    out[j] = make_float2((float)v.x, (float)v.y);
+}
+
+// Column max |re|,|im| of an n x k block (atomic max on the bit pattern of a
+// non-negative float, which orders like the value).
+__global__ void colmax_kernel(const double2* __restrict__ X, int64_t n, int k, unsigned* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    double2 v = X[t];
+    float m = (float)fmax(fabs(v.x), fabs(v.y));
+    atomicMax(out + (t % k), __float_as_uint(m));
+}
+
+__global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// X (n x k complex FP64) -> XT (BN x 2*ld halves), scaled by 2^(TOP - e_c) per column.
+template <int FMT>
+__global__ void x_to_lp_kernel(const double2* __restrict__ X, int64_t n, int k, int64_t ld,
+                               const unsigned* __restrict__ colmax, u16* __restrict__ XT,
+                               double* __restrict__ col_inv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int64_t j = t / k;
+    const int c = (int)(t - j * k);
+    float m = __uint_as_float(colmax[c]);
+    int e = 0;
+    if (m > 0.f && isfinite(m)) e = ilogb(m);
+    double s = ldexp(1.0, Half<FMT>::TOP - e);
+    if (j == 0) col_inv[c] = ldexp(1.0, e - Half<FMT>::TOP);
+    double2 v = X[t];
+    u16 xr = Half<FMT>::from_float((float)(v.x * s));
+    u16 xi = Half<FMT>::from_float((float)(v.y * s));
+    u16 nxi = Half<FMT>::from_float((float)(-v.y * s));
+    u16* re_row = XT + (int64_t)c * 2 * ld;
+    u16* im_row = XT + (int64_t)(k + c) * 2 * ld;
+    re_row[2 * j] = xr;
+    re_row[2 * j + 1] = nxi;
+    im_row[2 * j] = xi;
+    im_row[2 * j + 1] = xr;
+}
+
+template <int FMT>
+__device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
+    if constexpr (FMT == 0)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+// Y (n x k complex FP64) = diag(row_inv) (A_r X_r) diag(col_inv), MFMA with FP32 accumulation.
+// Block: 4 waves x 32 rows; each wave computes 32 rows x BN columns (NT = BN/32 MFMA tiles).
+template <int FMT, int NT>
+__global__ __launch_bounds__(BLOCK) void lp_gemm_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
+                                                        const u16* __restrict__ XT, int k,
+                                                        const double* __restrict__ row_inv,
+                                                        const double* __restrict__ col_inv,
+                                                        double* __restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t row_base = (int64_t)blockIdx.x * 128 + wave * 32;
+    if (row_base >= n) return;
+    const int64_t arow = min(row_base + (lane & 31), n - 1);
+    const int kq = (lane >> 5) * 8;  // this lane's 8-element k offset within a 16-k step
+    const u32x4* ap = reinterpret_cast<const u32x4*>(A + arow * 2 * ld + kq);
+    const u32x4* bp[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        bp[t] = reinterpret_cast<const u32x4*>(XT + (int64_t)(t * 32 + (lane & 31)) * 2 * ld + kq);
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+    const int64_t steps = (2 * ld) / 16;  // 16 k per MFMA; a uint4 step is 8 halves, so 2 uint4 per 16 k
+    for (int64_t s = 0; s < steps; s += 2) {
+        u32x4 a0 = __builtin_nontemporal_load(ap + s * 2);
+        u32x4 a1 = __builtin_nontemporal_load(ap + (s + 1) * 2);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            u32x4 b0 = bp[t][s * 2];
+            u32x4 b1 = bp[t][(s + 1) * 2];
+            acc[t] = mfma32<FMT>(a0, b0, acc[t]);
+            acc[t] = mfma32<FMT>(a1, b1, acc[t]);
+        }
+    }
+    // C layout of v_mfma_f32_32x32x*: column = lane % 32,
+    // row = 8*(v/4) + 4*(lane/32) + (v%4) for accumulator register v.
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = t * 32 + (lane & 31);
+        if (col >= 2 * k) continue;
+        const int c = col < k ? col : col - k;
+        const int part = col < k ? 0 : 1;
+        const double cs = col_inv[c];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int64_t row = row_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (row < n) Y[(row * k + c) * 2 + part] = (double)acc[t][v] * row_inv[row] * cs;
+        }
+    }
+}
+
+// FP64 complex block product Y (n x k) = A (n x n, lda) X (n x k), k <= 64 per launch (column
+// offset handled by the caller).  Tile 128 rows x 64 columns; each thread 8 rows x 4 columns.
+__global__ __launch_bounds__(BLOCK) void zgemm_f64_kernel(const double2* __restrict__ A, int64_t lda, int64_t n,
+                                                          const double2* __restrict__ X, int64_t ldx, int kc,
+                                                          double2* __restrict__ Y, int64_t ldy) {
+    constexpr int BM = 128, BN = 64, BK = 16;
+    __shared__ double2 As[BK][BM + 1];
+    __shared__ double2 Xs[BK][BN];
+    const int t = threadIdx.x;
+    const int cg = t & 15;   // columns cg*4 .. +3
+    const int rg = t >> 4;   // rows rg*8 .. +7
+    const int64_t row0 = (int64_t)blockIdx.x * BM;
+    double2 acc[8][4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
+    for (int64_t k0 = 0; k0 < n; k0 += BK) {
+        // A tile: BM x BK (row-major in memory), stored transposed in LDS
+        for (int e = t; e < BM * BK; e += BLOCK) {
+            int r = e / BK, kk = e % BK;
+            int64_t gr = row0 + r, gk = k0 + kk;
+            As[kk][r] = (gr < n && gk < n) ? A[gr * lda + gk] : make_double2(0.0, 0.0);
+        }
+        for (int e = t; e < BK * BN; e += BLOCK) {
+            int kk = e / BN, c = e % BN;
+            int64_t gk = k0 + kk;
+            Xs[kk][c] = (gk < n && c < kc) ? X[gk * ldx + c] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < BK; ++kk) {
+            double2 xv[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) xv[b] = Xs[kk][cg * 4 + b];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                double2 av = As[kk][rg * 8 + a];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc[a][b].x = __builtin_fma(av.x, xv[b].x, acc[a][b].x);
+                    acc[a][b].x = __builtin_fma(-av.y, xv[b].y, acc[a][b].x);
+                    acc[a][b].y = __builtin_fma(av.x, xv[b].y, acc[a][b].y);
+                    acc[a][b].y = __builtin_fma(av.y, xv[b].x, acc[a][b].y);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        int64_t gr = row0 + rg * 8 + a;
+        if (gr >= n) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int c = cg * 4 + b;
+            if (c < kc) Y[gr * ldy + c] = acc[a][b];
+        }
+    }
+}
+
+// ---- column-batched Krylov vector kernels (n x k row-major blocks) ----
+// A block reduces over rows [chunk*len, ...) for every column: thread t owns
+// column t % k and rows sub, sub + S, ... (S = 256 / k); the per-column
+// partials are summed in a fixed order.
+
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// out[(vec * chunks + chunk) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2
+__global__ __launch_bounds__(BLOCK) void bdots_kernel(const double2* __restrict__ V, int64_t ldv, int nvec,
+                                                      const double2* __restrict__ W, int64_t n, int k,
+                                                      int64_t chunk_len, double2* __restrict__ out) {
+    __shared__ double2 sh[BLOCK];
+    const int vec = blockIdx.y;
+    const int S = BLOCK / k;
+    const int c = threadIdx.x % k;
+    const int sub = threadIdx.x / k;
+    const int64_t b = (int64_t)blockIdx.x * chunk_len;
+    const int64_t e = min(n, b + chunk_len);
+    double2 s = make_double2(0.0, 0.0);
+    if (sub < S) {
+        if (vec < nvec) {
+            const double2* v = V + (int64_t)vec * ldv;
+            for (int64_t r = b + sub; r < e; r += S) {
+                double2 p = cmulc(v[r * k + c], W[r * k + c]);
+                s.x += p.x;
+                s.y += p.y;
+            }
+        } else {
+            for (int64_t r = b + sub; r < e; r += S) {
+                double2 q = W[r * k + c];
+                s.x += q.x * q.x + q.y * q.y;
+            }
+        }
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double2 tsum = make_double2(0.0, 0.0);
+        for (int q = 0; q < S; ++q) {
+            double2 p = sh[q * k + threadIdx.x];
+            tsum.x += p.x;
+            tsum.y += p.y;
+        }
+        out[((int64_t)vec * gridDim.x + blockIdx.x) * k + threadIdx.x] = tsum;
+    }
+}
+
+// W[:,c] -= sum_i C[i*k+c] V_i[:,c]; norm partials per column into out[chunk*k + c]
+__global__ __launch_bounds__(BLOCK) void bproject_kernel(const double2* __restrict__ V, int64_t ldv, int nvec,
+                                                         const double2* __restrict__ C, double2* __restrict__ W,
+                                                         int64_t n, int k, int64_t chunk_len,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[BLOCK];
+    const int S = BLOCK / k;
+    const int c = threadIdx.x % k;
+    const int sub = threadIdx.x / k;
+    const int64_t b = (int64_t)blockIdx.x * chunk_len;
+    const int64_t e = min(n, b + chunk_len);
+    double nr = 0.0;
+    if (sub < S) {
+        for (int64_t r = b + sub; r < e; r += S) {
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < nvec; ++i) {
+                double2 p = cmul(C[i * k + c], V[(int64_t)i * ldv + r * k + c]);
+                sr += p.x;
+                si += p.y;
+            }
+            double2 w = W[r * k + c];
+            w.x -= sr;
+            w.y -= si;
+            W[r * k + c] = w;
+            nr += w.x * w.x + w.y * w.y;
+        }
+    }
+    sh[threadIdx.x] = nr;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double tsum = 0.0;
+        for (int q = 0; q < S; ++q) tsum += sh[q * k + threadIdx.x];
+        out[(int64_t)blockIdx.x * k + threadIdx.x] = tsum;
+    }
+}
+
+// R = B - R (or only the norm when B == nullptr); norm partials per column
+__global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restrict__ B, double2* __restrict__ R,
+                                                          int64_t n, int k, int64_t chunk_len,
+                                                          double* __restrict__ out) {
+    __shared__ double sh[BLOCK];
+    const int S = BLOCK / k;
+    const int c = threadIdx.x % k;
+    const int sub = threadIdx.x / k;
+    const int64_t b = (int64_t)blockIdx.x * chunk_len;
+    const int64_t e = min(n, b + chunk_len);
+    double nr = 0.0;
+    if (sub < S) {
+        for (int64_t r = b + sub; r < e; r += S) {
+            double2 v = R[r * k + c];
+            if (B) {
+                double2 bb = B[r * k + c];
+                v = make_double2(bb.x - v.x, bb.y - v.y);
+                R[r * k + c] = v;
+            }
+            nr += v.x * v.x + v.y * v.y;
+        }
+    }
+    sh[threadIdx.x] = nr;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double tsum = 0.0;
+        for (int q = 0; q < S; ++q) tsum += sh[q * k + threadIdx.x];
+        out[(int64_t)blockIdx.x * k + threadIdx.x] = tsum;
+    }
+}
+
+// X[:,c] += sum_i Yc[i*k+c] V_i[:,c]
+__global__ void bupdate_kernel(const double2* __restrict__ V, int64_t ldv, int nvec, const double2* __restrict__ Yc,
+                               double2* __restrict__ X, int64_t n, int k) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int c = (int)(t % k);
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < nvec; ++i) {
+        double2 p = cmul(Yc[i * k + c], V[(int64_t)i * ldv + t]);
+        sr += p.x;
+        si += p.y;
+    }
+    double2 x = X[t];
+    X[t] = make_double2(x.x + sr, x.y + si);
+}
+
+// Y[:,c] = alpha[c] * X[:,c]
+__global__ void bscale_kernel(const double2* __restrict__ X, const double* __restrict__ alpha,
+                              double2* __restrict__ Y, int64_t n, int k) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const double a = alpha[t % k];
+    double2 v = X[t];
+    Y[t] = make_double2(v.x * a, v.y * a);
+}
+
+__global__ void zero_kernel(double2* __restrict__ X, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) X[t] = make_double2(0.0, 0.0);
+}
+
+inline unsigned nblocks(int64_t n, int per = BLOCK) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+int64_t lowp_padded_ld(int64_t n) { return (n + 63) / 64 * 64; }
+
+void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
+                  uint16_t* out, double* row_inv) {
+    if (format == 0)
+        hipLaunchKernelGGL(lp_convert_kernel<0>, dim3((unsigned)n), dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+    else
+        hipLaunchKernelGGL(lp_convert_kernel<1>, dim3((unsigned)n), dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+    SBH_HIP(hipGetLastError());
+}
+
+void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+               const double2* x, float2* x32, double2* y) {
+    hipLaunchKernelGGL(to_f32_kernel, dim3(nblocks(ld)), dim3(BLOCK), 0, ctx->stream, x, n, ld, x32);
+    SBH_HIP(hipGetLastError());
+    const int64_t waves = (n + 3) / 4;
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
+    if (format == 0)
+        hipLaunchKernelGGL(lp_gemv_kernel<0>, dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld, n,
+                           x32, row_inv, y);
+    else
+        hipLaunchKernelGGL(lp_gemv_kernel<1>, dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld, n,
+                           x32, row_inv, y);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_LOWP, ev);
+}
+
+void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, double2* Y) {
+    // k <= 64: BN = 2k rounded up to 32
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, ctx->stream, colmax, k, 0u);
+    hipLaunchKernelGGL(colmax_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, colmax);
+    if (format == 0)
+        hipLaunchKernelGGL(x_to_lp_kernel<0>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld, colmax, XT,
+                           col_inv);
+    else
+        hipLaunchKernelGGL(x_to_lp_kernel<1>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld, colmax, XT,
+                           col_inv);
+    SBH_HIP(hipGetLastError());
+    const int nt = (2 * k + 31) / 32;
+    const unsigned grid = (unsigned)((n + 127) / 128);
+    double* Yd = reinterpret_cast<double*>(Y);
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
+#define SBH_LP_GEMM(F, NTV)                                                                                      \
+    hipLaunchKernelGGL((lp_gemm_kernel<F, NTV>), dim3(grid), dim3(BLOCK), 0, ctx->stream, A, ld, n, XT, k, row_inv, \
+                       col_inv, Yd)
+    if (format == 0) {
+        if (nt == 1) SBH_LP_GEMM(0, 1);
+        else if (nt == 2) SBH_LP_GEMM(0, 2);
+        else if (nt == 3) SBH_LP_GEMM(0, 3);
+        else SBH_LP_GEMM(0, 4);
+    } else {
+        if (nt == 1) SBH_LP_GEMM(1, 1);
+        else if (nt == 2) SBH_LP_GEMM(1, 2);
+        else if (nt == 3) SBH_LP_GEMM(1, 3);
+        else SBH_LP_GEMM(1, 4);
+    }
+#undef SBH_LP_GEMM
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_LOWP, ev);
+}
+
+void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y) {
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_BLOCK, &ev);
+    for (int64_t c0 = 0; c0 < k; c0 += 64) {
+        int kc = (int)std::min<int64_t>(64, k - c0);
+        hipLaunchKernelGGL(zgemm_f64_kernel, dim3(nblocks(n, 128)), dim3(BLOCK), 0, ctx->stream, A, lda, n, X + c0, k,
+                           kc, Y + c0, k);
+        SBH_HIP(hipGetLastError());
+    }
+    ctx->end_timed(SBH_KERNEL_BLOCK, ev);
+}
+
+int bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
+          double2* out) {
+    int chunks = reduction_chunks(n);
+    int64_t len = (n + chunks - 1) / chunks;
+    hipLaunchKernelGGL(bdots_kernel, dim3((unsigned)chunks, (unsigned)(nvec + 1)), dim3(BLOCK), 0, ctx->stream, V, ldv,
+                       nvec, W, n, k, len, out);
+    SBH_HIP(hipGetLastError());
+    return chunks;
+}
+
+int bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
+             int k, double* out) {
+    int chunks = reduction_chunks(n);
+    int64_t len = (n + chunks - 1) / chunks;
+    hipLaunchKernelGGL(bproject_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n, k,
+                       len, out);
+    SBH_HIP(hipGetLastError());
+    return chunks;
+}
+
+int bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* out) {
+    int chunks = reduction_chunks(n);
+    int64_t len = (n + chunks - 1) / chunks;
+    hipLaunchKernelGGL(bresidual_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len, out);
+    SBH_HIP(hipGetLastError());
+    return chunks;
+}
+
+void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,
+             int k) {
+    hipLaunchKernelGGL(bupdate_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, Yc, X, n, k);
+    SBH_HIP(hipGetLastError());
+}
+
+void bscale(sbh_context* ctx, const double2* X, const double* alpha, double2* Y, int64_t n, int k) {
+    hipLaunchKernelGGL(bscale_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, alpha, Y, n, k);
+    SBH_HIP(hipGetLastError());
+}
+
+void zero_fill(sbh_context* ctx, double2* X, int64_t count) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(zero_kernel, dim3(nblocks(count)), dim3(BLOCK), 0, ctx->stream, X, count);
+    SBH_HIP(hipGetLastError());
+}
+
+}  // namespace sbh

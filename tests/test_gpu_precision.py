@@ -1,0 +1,171 @@
+"""GPU tests of the many-right-hand-side and mixed-precision paths.
+
+* the complex FP16/BF16 copy: per-row scales are exact powers of two that
+  bracket the row maximum, and every entry is the FP64 entry rounded once to
+  the format (relative error <= 2^-11 for FP16, 2^-8 for BF16, plus the
+  subnormal floor of the scaled row);
+* low-precision products (GEMV for one column, MFMA GEMM for many) against
+  numpy on the dequantized operands (FP32 accumulation), and against the FP64
+  product within the format's rounding;
+* block GMRES: every column equals the single right-hand-side GMRES
+  (gmres.rs:631-1057) — same iteration count, solution to 1e-10;
+* FGMRES-IR (gmres_ir.rs:665): recomputed FP64 residual <= 1e-8 and the
+  solution within 1e-7 of the FP64 oracle (LAPACK on the oracle matrix),
+  reported per right-hand side.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+import sonium_amd as sa
+from sonium_amd.mesh import fibonacci_directions, icosphere
+
+from helpers import rel_fro, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+U = {"fp16": 2.0 ** -11, "bf16": 2.0 ** -8}
+
+
+@pytest.fixture(scope="module")
+def c1_block(ctx):
+    m = icosphere(1.0, 3)
+    k = 2.0
+    tm = sa.TbemMesh(ctx, m)
+    ph = sa.PhysicsParams.from_wavenumber(k)
+    beta = ph.burton_miller_beta()
+    ws = sa.TbemAssemblyWorkspace(ctx, tm)
+    sys = ws.assemble(ph, beta)
+    dirs = fibonacci_directions(12)
+    B = sa.IncidentField(dirs).compute_rhs_with_beta(tm, ph, beta)
+    return {"mesh": m, "tm": tm, "sys": sys, "A": sys.matrix, "B": B, "ws": ws, "beta": beta}
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_lowp_copy_rounding_and_power_of_two_scales(c1_block, fmt):
+    A = c1_block["A"]
+    lp = sa.LowpMatrix(c1_block["sys"], fmt)
+    halves, inv, ld = lp.download()
+    assert ld % 64 == 0 and ld >= A.shape[0]
+    top = 14 if fmt == "fp16" else 0
+    rowmax = np.maximum(np.abs(A.real), np.abs(A.imag)).max(axis=1)
+    e = np.log2(inv) + top
+    assert np.all(e == np.round(e))  # exact powers of two
+    assert np.all(2.0 ** e <= rowmax) and np.all(rowmax < 2.0 ** (e + 1))
+    D = lp.dequantized()
+    floor = (2.0 ** -24 if fmt == "fp16" else 0.0) * inv[:, None]  # FP16 subnormal spacing, unscaled
+    err_re = np.abs(D.real - A.real)
+    err_im = np.abs(D.imag - A.imag)
+    # one rounding to the format (plus the FP64->FP32->format double rounding, < 2^-24)
+    assert np.all(err_re <= 1.01 * U[fmt] * np.abs(A.real) + floor + 1e-300)
+    assert np.all(err_im <= 1.01 * U[fmt] * np.abs(A.imag) + floor + 1e-300)
+    # padding columns are zero
+    assert np.all(halves[:, 2 * A.shape[0]:] == 0)
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+@pytest.mark.parametrize("k", [1, 5, 17, 64])
+def test_lowp_products(c1_block, fmt, k):
+    A = c1_block["A"]
+    n = A.shape[0]
+    lp = sa.LowpMatrix(c1_block["sys"], fmt)
+    D = lp.dequantized()
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    Y = lp.apply(X if k > 1 else X[:, 0])
+    if k == 1:
+        x32 = X[:, 0].astype(np.complex64).astype(np.complex128)
+        ref = D @ x32
+        assert rel_l2(Y, ref) < 2e-6  # FP32 accumulation of the dequantized operands
+        assert rel_l2(Y, A @ X[:, 0]) < 4 * U[fmt]
+    else:
+        # the block operand is quantized per column with a power-of-two scale
+        assert rel_fro(Y, D @ X) < 4 * U[fmt]
+        assert rel_fro(Y, A @ X) < 6 * U[fmt]
+        # each column equals the single-column product of its own quantized vector
+        col = 3 % k
+        ycol = lp.apply(X[:, col].copy())
+        assert rel_l2(Y[:, col], ycol) < 4 * U[fmt]
+
+
+def test_fp64_block_apply(c1_block):
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    A = c1_block["A"]
+    rng = np.random.default_rng(4)
+    for k in (2, 7, 64, 70):
+        X = rng.standard_normal((A.shape[0], k)) + 1j * rng.standard_normal((A.shape[0], k))
+        assert rel_fro(op.apply_block(X), A @ X) < 1e-14
+
+
+def test_block_gmres_columns_equal_single_gmres(c1_block):
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    B = c1_block["B"]
+    cfg = sa.GmresConfig(max_iterations=100, restart=50, tolerance=1e-8)
+    X, sols = sa.gmres_block(op, B, cfg)
+    for c in range(B.shape[1]):
+        single = sa.gmres(op, B[:, c], cfg)
+        assert sols[c].converged and sols[c].residual <= 1e-8
+        assert sols[c].iterations == single.iterations
+        assert rel_l2(X[:, c], single.x) < 1e-10
+        xr, st = oracle.gmres(c1_block["A"], B[:, c], restart=50, max_iterations=100, tol=1e-8)
+        assert st["iterations"] == sols[c].iterations
+        assert rel_l2(X[:, c], xr) < 1e-8
+
+
+def test_block_gmres_short_restart_matches_single(c1_block):
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    B = c1_block["B"][:, :5]
+    cfg = sa.GmresConfig(max_iterations=60, restart=6, tolerance=1e-9)
+    X, sols = sa.gmres_block(op, B, cfg)
+    for c in range(B.shape[1]):
+        single = sa.gmres(op, B[:, c], cfg)
+        assert sols[c].converged == single.converged
+        assert sols[c].restarts == single.restarts
+        assert abs(sols[c].iterations - single.iterations) <= 1
+        assert rel_l2(X[:, c], single.x) < 1e-7
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_gmres_ir_reaches_fp64_accuracy(c1_block, fmt):
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    lp = sa.LowpMatrix(c1_block["sys"], fmt)
+    B = c1_block["B"]
+    cfg = sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8, max_inner_iterations=50,
+                           inner_tolerance=1e-3 if fmt == "fp16" else 1e-2)
+    X, st = sa.gmres_ir(op, lp, B, cfg)
+    A = c1_block["A"]
+    for c in range(B.shape[1]):
+        xl = np.linalg.solve(A, B[:, c])
+        err = rel_l2(X[:, c], xl)
+        print(f"{fmt} rhs {c}: outer {st[c].outer_iterations} inner {st[c].total_inner_iterations} "
+              f"residual {st[c].final_outer_residual:.2e} error vs FP64 {err:.2e}")
+        assert st[c].converged and st[c].final_outer_residual <= 1e-8
+        assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1e-8
+        assert err < 1e-7
+        assert st[c].outer_iterations <= 10
+
+
+def test_gmres_ir_single_rhs_gemv_path(c1_block):
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    lp = sa.LowpMatrix(c1_block["sys"], "fp16")
+    b = c1_block["B"][:, 0]
+    X, st = sa.gmres_ir(op, lp, b, sa.GmresIrConfig(outer_tolerance=1e-8))
+    assert st[0].converged and st[0].final_outer_residual <= 1e-8
+    assert rel_l2(X[:, 0], np.linalg.solve(c1_block["A"], b)) < 1e-7
+
+
+def test_lowp_update_after_reassembly(ctx, c1_block):
+    """LowpMatrix.update re-converts the system's matrix after a new assemble."""
+    sys = sa.TbemSystem(ctx, c1_block["mesh"].num_elements)
+    ph = sa.PhysicsParams.from_wavenumber(1.0)
+    c1_block["ws"].assemble(ph, ph.burton_miller_beta(), sys)
+    lp = sa.LowpMatrix(sys, "fp16")
+    D1 = lp.dequantized()
+    ph2 = sa.PhysicsParams.from_wavenumber(3.0)
+    c1_block["ws"].assemble(ph2, ph2.burton_miller_beta(), sys)
+    lp.update()
+    D2 = lp.dequantized()
+    A2 = sys.matrix
+    assert rel_fro(D2, A2) < U["fp16"]
+    assert rel_fro(D1, A2) > 1e-2
