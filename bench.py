@@ -33,7 +33,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 import sonium_amd as sa  # noqa: E402
-from sonium_amd.mesh import icosphere  # noqa: E402
+from sonium_amd.mesh import fibonacci_directions, fibonacci_sphere, icosphere  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -50,6 +50,7 @@ MEASURED_TRAFFIC = {
 WORKLOADS = {
     "c2": {"subdiv": 5, "k": 10.0, "restart": 100,
            "name": "rigid sphere icosphere 20480 panels, k=10, beta=i/k, 1 plane wave +z, GMRES(100) rtol 1e-8, complex128"},
+    "c3": {"name": "many right-hand sides (separate report)"},
     "c1": {"subdiv": 3, "k": 2.0, "restart": 50,
            "name": "rigid sphere icosphere 1280 panels, k=2, beta=i/k, 1 plane wave +z, GMRES(50) rtol 1e-8, complex128"},
 }
@@ -84,6 +85,103 @@ def cpu_baseline(m, k, beta, spmv_count, rows, threads):
     }
 
 
+def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
+    """Mixed-precision form of the same step: assemble FP64, convert to the
+    row-scaled complex FP16/BF16 copy, FGMRES-IR to an FP64-recomputed
+    residual of 1e-8."""
+    system = sa.TbemSystem(ctx, n)
+    op = sa.DenseMatrixOperator(system)
+    inc = sa.IncidentField.plane_wave_z()
+    cfg = sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8, max_inner_iterations=50,
+                           inner_tolerance=inner_tol)
+    ws.assemble(ph, beta, system)
+    lp = sa.LowpMatrix(system, fmt)
+
+    def step():
+        ws.assemble(ph, beta, system)
+        lp.update()
+        b = inc.compute_rhs_with_beta(tm, ph, beta)
+        return sa.gmres_ir(op, lp, b, cfg)
+
+    step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    outs = [step() for _ in range(steps)]
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    x, st = outs[-1]
+    if not all(o[1][0].converged and o[1][0].final_outer_residual <= 1e-8 for o in outs):
+        raise SystemExit(f"bench: mixed-precision solve did not converge: {st[0]}")
+    return {"form": f"{fmt} storage + FP32 accumulate, FGMRES-IR with FP64 residual/Arnoldi", "value": steps / el,
+            "ms_per_step": el / steps * 1e3, "outer_iterations": st[0].outer_iterations,
+            "inner_iterations": st[0].total_inner_iterations, "residual": st[0].final_outer_residual,
+            "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products}, x[:, 0]
+
+
+def run_c3(args):
+    """Many right-hand sides: 40960-panel Fibonacci sphere, k = 14, 64 plane waves."""
+    ctx = sa.Context(0)
+    m = fibonacci_sphere(1.0, 129, 160)
+    n = m.num_elements
+    k = 14.0
+    ph = sa.PhysicsParams.from_wavenumber(k)
+    beta = ph.burton_miller_beta()
+    tm = sa.TbemMesh(ctx, m)
+    ws = sa.TbemAssemblyWorkspace(ctx, tm)
+    system = sa.TbemSystem(ctx, n)
+    op = sa.DenseMatrixOperator(system)
+    dirs = fibonacci_directions(64)
+    inc = sa.IncidentField(dirs)
+    t0 = time.perf_counter()
+    ws.assemble(ph, beta, system)
+    ctx.synchronize()
+    t_asm = time.perf_counter() - t0
+    B = inc.compute_rhs_with_beta(tm, ph, beta)
+    res = {}
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    X64, sols = sa.gmres_block(op, B, sa.GmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
+    t_plain = time.perf_counter() - t0
+    blk = ctx.kernel_stats("block")
+    res["plain_fp64"] = {"solve_s": t_plain, "rhs_per_s_excl_assembly": 64 / t_plain,
+                         "rhs_per_s_incl_assembly": 64 / (t_plain + t_asm),
+                         "max_iterations": max(s.iterations for s in sols),
+                         "max_residual": max(s.residual for s in sols),
+                         "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
+    for fmt, itol in (("fp16", 1e-3), ("bf16", 1e-2)):
+        ctx.reset_stats()
+        t0 = time.perf_counter()
+        lp = sa.LowpMatrix(system, fmt)
+        ctx.synchronize()
+        t_conv = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        X, st = sa.gmres_ir(op, lp, B, sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8,
+                                                         max_inner_iterations=50, inner_tolerance=itol))
+        t_ir = time.perf_counter() - t0
+        lpk = ctx.kernel_stats("lowp")
+        blk = ctx.kernel_stats("block")
+        err = [float(np.linalg.norm(X[:, c] - X64[:, c]) / np.linalg.norm(X64[:, c])) for c in range(64)]
+        lp_avg = lpk[1] / max(lpk[0], 1) / 1e3
+        lp_bytes = 4 * n * n
+        lp_flops = 8 * n * n * 64
+        res[fmt + "_ir"] = {"solve_s": t_ir, "convert_s": t_conv,
+                            "rhs_per_s_excl_assembly": 64 / (t_ir + t_conv),
+                            "rhs_per_s_incl_assembly": 64 / (t_ir + t_conv + t_asm),
+                            "max_outer": max(s.outer_iterations for s in st),
+                            "max_inner": max(s.total_inner_iterations for s in st),
+                            "max_residual": max(s.final_outer_residual for s in st),
+                            "max_error_vs_fp64": max(err), "converged": all(s.converged for s in st),
+                            "lowp_gemm_avg_ms": lp_avg * 1e3, "lowp_gemm_gbs": lp_bytes / lp_avg / 1e9,
+                            "lowp_gemm_tflops": lp_flops / lp_avg / 1e12,
+                            "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
+        lp.close()
+    ctx.set_profiling(False)
+    line = {"metric": "solved RHS/s at rel. residual 1e-8 (multi-RHS)", "workload": "C3: Fibonacci sphere 40960 panels, k=14, 64 plane waves",
+            "panels": n, "assembly_s": t_asm, "results": res}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +190,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
     ap.add_argument("--cpu-rows", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-chosen-form", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 3 plain device copies of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -99,6 +198,9 @@ def main():
         ctx = sa.Context(0)
         sa._check(sa._lib().sbh_debug_device_copy(ctx.handle, args.calibrate_copy, 3))
         print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 3}))
+        return
+    if args.workload == "c3":
+        run_c3(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -175,6 +277,13 @@ def main():
                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                     "traffic": MEASURED_TRAFFIC.get(("far_kernel", n))}
 
+    chosen = None
+    if not args.no_chosen_form:
+        x64 = x.download()
+        chosen, x_ir = run_ir_leg(ctx, ws, tm, ph, beta, n, args.steps, "fp16", 1e-3)
+        chosen["error_vs_fp64"] = float(np.linalg.norm(x_ir - x64) / np.linalg.norm(x64))
+        chosen["vs_plain_fp64"] = chosen["value"] / value
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(os.cpu_count() or 1, 16))
@@ -201,6 +310,7 @@ def main():
         "routes": st,
         "kernels": kernels,
         "roofline": roofline,
+        "chosen_form": chosen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
