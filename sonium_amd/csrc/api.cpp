@@ -17,7 +17,7 @@ void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const
 void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
                const double2* x, float2* x32, double2* y);
 void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, double2* Y);
+               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y);
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
                      const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol);
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
@@ -809,6 +809,7 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         lp->d_colmax.allocate(64);
         lp->d_colinv.allocate(64);
         lp->d_x32.allocate((size_t)lp->ld);
+        lp->d_ypart.allocate((size_t)lowp_split_count() * lp->n * 128);
         SBH_HIP(hipMemsetAsync(lp->d_XT.ptr, 0, lp->d_XT.bytes(), sys->ctx->stream));
         lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr);
         SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
@@ -848,7 +849,7 @@ sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64*
             lowp_gemv(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, lp->d_x32.ptr, outv.ptr);
         else
             lowp_gemm(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, (int)k, lp->d_XT.ptr,
-                      lp->d_colmax.ptr, lp->d_colinv.ptr, outv.ptr);
+                      lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, outv.ptr);
         outv.finish();
     });
 }

@@ -36,13 +36,14 @@ int64_t lowp_padded_ld(int64_t n);
 void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
                const double2* x, float2* x32, double2* y);
 void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, double2* Y);
+               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
-int bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
-          double2* out);
-int bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
-             int k, double* out);
-int bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* out);
+void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
+           double2* scratch, double2* out);
+void bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
+              int k, double* scratch, double* out);
+void bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* scratch, double* out);
+int64_t batched_scratch_elems(int64_t n, int k, int max_vec);
 void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,
              int k);
 void bscale(sbh_context* ctx, const double2* X, const double* alpha, double2* Y, int64_t n, int k);
@@ -98,7 +99,7 @@ struct LowpBlockOperator : BlockOperator {
             lowp_gemv(ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, X, lp->d_x32.ptr, Y);
         else
             lowp_gemm(ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, X, k, lp->d_XT.ptr,
-                      lp->d_colmax.ptr, lp->d_colinv.ptr, Y);
+                      lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, Y);
         applies += 1;
     }
 };
@@ -134,13 +135,13 @@ class BatchedGmres {
         if (flexible) Z_.allocate((size_t)m * blk);
         W_.allocate(blk);
         R_.allocate(blk);
-        chunks_ = reduction_chunks(n);
-        part_.allocate((size_t)(m + 2) * chunks_ * k + 16);
+        scratch_.allocate((size_t)batched_scratch_elems(n, k, m + 1) + 16);
+        part_.allocate((size_t)(m + 2) * k + 16);
         C_.allocate((size_t)(m + 1) * k + 1);
         alpha_.allocate((size_t)k);
         // Separate pinned regions: each is rewritten by the host only after a
         // stream synchronisation has retired the copy that last used it.
-        down_.allocate(((size_t)(m + 2) * chunks_ * k + 16) * 2);
+        down_.allocate(((size_t)(m + 2) * k + 16) * 2);
         up_coef_.allocate((size_t)(m + 1) * k * 2 + 16);
         up_alpha_.allocate((size_t)k + 16);
         pinned_ = down_.ptr;
@@ -154,8 +155,7 @@ class BatchedGmres {
     int64_t n_;
     int k_, m_;
     bool flexible_;
-    int chunks_ = 1;
-    DeviceArray<double2> V_, Z_, W_, R_, part_, C_;
+    DeviceArray<double2> V_, Z_, W_, R_, part_, C_, scratch_;
     DeviceArray<double> alpha_;
     PinnedArray<double> down_, up_coef_, up_alpha_;
     double* pinned_ = nullptr;  // download region
@@ -167,46 +167,38 @@ class BatchedGmres {
         SBH_HIP(hipMemcpyAsync(pinned_, dev, sizeof(double) * doubles, hipMemcpyDeviceToHost, ctx_->stream));
         SBH_HIP(hipStreamSynchronize(ctx_->stream));
     }
-    // per-column norms from bresidual/bproject partials (chunks x k doubles)
-    void column_sums(int chunks, std::vector<double>& out) {
-        fetch(part_.ptr, (size_t)chunks * k_);
-        out.assign(k_, 0.0);
-        for (int c = 0; c < k_; ++c)
-            for (int q = 0; q < chunks; ++q) out[c] += pinned_[(size_t)q * k_ + c];
+    // k squared column norms reduced on the device into part_
+    void column_sums(std::vector<double>& out) {
+        fetch(part_.ptr, (size_t)k_);
+        out.assign(pinned_, pinned_ + k_);
     }
     std::vector<double> norms(const double2* Xd) {
-        int ch = bresidual(ctx_, nullptr, const_cast<double2*>(Xd), n_, k_, reinterpret_cast<double*>(part_.ptr));
+        bresidual(ctx_, nullptr, const_cast<double2*>(Xd), n_, k_, reinterpret_cast<double*>(scratch_.ptr),
+                  reinterpret_cast<double*>(part_.ptr));
         std::vector<double> s;
-        column_sums(ch, s);
+        column_sums(s);
         for (auto& v : s) v = std::sqrt(v);
         return s;
     }
     std::vector<double> residual_norms(const double2* B, double2* R) {  // R = B - R
-        int ch = bresidual(ctx_, B, R, n_, k_, reinterpret_cast<double*>(part_.ptr));
+        bresidual(ctx_, B, R, n_, k_, reinterpret_cast<double*>(scratch_.ptr), reinterpret_cast<double*>(part_.ptr));
         std::vector<double> s;
-        column_sums(ch, s);
+        column_sums(s);
         for (auto& v : s) v = std::sqrt(v);
         return s;
     }
     // coefficients <v_i, w> for i < nvec (per column) and ||w||
     void dots(int nvec, const double2* Wd, std::vector<std::vector<cplx>>& coef, std::vector<double>& wn) {
-        int ch = bdots(ctx_, V_.ptr, n_ * k_, nvec, Wd, n_, k_, part_.ptr);
-        fetch(part_.ptr, (size_t)(nvec + 1) * ch * k_ * 2);
+        bdots(ctx_, V_.ptr, n_ * k_, nvec, Wd, n_, k_, scratch_.ptr, part_.ptr);
+        fetch(part_.ptr, (size_t)(nvec + 1) * k_ * 2);
         coef.assign(k_, std::vector<cplx>(nvec));
         wn.assign(k_, 0.0);
         for (int c = 0; c < k_; ++c) {
             for (int v = 0; v < nvec; ++v) {
-                double sr = 0.0, si = 0.0;
-                for (int q = 0; q < ch; ++q) {
-                    size_t idx = (((size_t)v * ch + q) * k_ + c) * 2;
-                    sr += pinned_[idx];
-                    si += pinned_[idx + 1];
-                }
-                coef[c][v] = cplx(sr, si);
+                size_t idx = ((size_t)v * k_ + c) * 2;
+                coef[c][v] = cplx(pinned_[idx], pinned_[idx + 1]);
             }
-            double s = 0.0;
-            for (int q = 0; q < ch; ++q) s += pinned_[(((size_t)nvec * ch + q) * k_ + c) * 2];
-            wn[c] = std::sqrt(s);
+            wn[c] = std::sqrt(pinned_[((size_t)nvec * k_ + c) * 2]);
         }
     }
     void upload_coeffs(const std::vector<std::vector<cplx>>& coef, int nvec, const std::vector<char>& use) {
@@ -343,9 +335,9 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
                 if (cycling[c])
                     for (int i = 0; i <= j; ++i) h(c, i, j) = coef[c][i];
             upload_coeffs(coef, j + 1, cycling);
-            int ch = bproject(ctx_, V_.ptr, n_ * k_, j + 1, C_.ptr, W_.ptr, n_, k,
-                              reinterpret_cast<double*>(part_.ptr));
-            column_sums(ch, wn);
+            bproject(ctx_, V_.ptr, n_ * k_, j + 1, C_.ptr, W_.ptr, n_, k, reinterpret_cast<double*>(scratch_.ptr),
+                     reinterpret_cast<double*>(part_.ptr));
+            column_sums(wn);
             for (auto& v : wn) v = std::sqrt(v);
             std::vector<char> reo(k, 0);
             bool anyreo = false;
@@ -363,8 +355,9 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
                         for (int i = 0; i <= j; ++i) h(c, i, j) += re[c][i];
                     }
                 upload_coeffs(re, j + 1, reo);
-                ch = bproject(ctx_, V_.ptr, n_ * k_, j + 1, C_.ptr, W_.ptr, n_, k, reinterpret_cast<double*>(part_.ptr));
-                column_sums(ch, wn2);
+                bproject(ctx_, V_.ptr, n_ * k_, j + 1, C_.ptr, W_.ptr, n_, k, reinterpret_cast<double*>(scratch_.ptr),
+                         reinterpret_cast<double*>(part_.ptr));
+                column_sums(wn2);
                 for (int c = 0; c < k; ++c)
                     if (reo[c]) wn[c] = std::sqrt(wn2[c]);
             }

@@ -228,6 +228,7 @@ struct sbh_lowp_matrix {
     sbh::DeviceArray<unsigned> d_colmax;  // 64
     sbh::DeviceArray<double> d_colinv;    // 64
     sbh::DeviceArray<float2> d_x32;       // ld (vector operand)
+    sbh::DeviceArray<float> d_ypart;      // split-K partial sums, splits x n x 128
 };
 
 struct sbh_operator {
@@ -298,6 +299,7 @@ int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out);
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
                   uint16_t* out, double* row_inv);
 int64_t lowp_padded_ld(int64_t n);
+int64_t lowp_split_count();
 int reduction_chunks(int64_t n);
 
 }  // namespace sbh

@@ -8,7 +8,7 @@
 // right-hand sides share one block product per Krylov step.
 //
 // Layout.  The low-precision matrix is row-major with ld >= n complex
-// entries per row (ld a multiple of 64, zero padded); an entry is two 16-bit
+// entries per row (ld a multiple of 512, zero padded); an entry is two 16-bit
 // halves {re, im}, so a row is also the real row [re0 im0 re1 im1 ...] of
 // length 2*ld.  Row i stores A_ij * 2^(T - e_i) where 2^e_i <= max_j |A_ij| <
 // 2^(e_i+1) and T = 14 (FP16: the largest scaled entry stays below 2^15) or
@@ -171,12 +171,28 @@ __global__ void to_f32_kernel(const double2* __restrict__ x, int64_t n, int64_t 
 
 // Column max |re|,|im| of an n x k block (atomic max on the bit pattern of a
 // non-negative float, which orders like the value).
-__global__ void colmax_kernel(const double2* __restrict__ X, int64_t n, int k, unsigned* __restrict__ out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * k) return;
-    double2 v = X[t];
-    float m = (float)fmax(fabs(v.x), fabs(v.y));
-    atomicMax(out + (t % k), __float_as_uint(m));
+// Each block reduces rows [b*R, (b+1)*R) in LDS first, then one atomic per column.
+__global__ __launch_bounds__(BLOCK) void colmax_kernel(const double2* __restrict__ X, int64_t n, int k,
+                                                       int64_t rows_per_block, unsigned* __restrict__ out) {
+    __shared__ float sh[BLOCK];
+    const int S = BLOCK / k;
+    const int c = threadIdx.x % k;
+    const int sub = threadIdx.x / k;
+    const int64_t b = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t e = min(n, b + rows_per_block);
+    float m = 0.f;
+    if (sub < S)
+        for (int64_t r = b + sub; r < e; r += S) {
+            double2 v = X[r * k + c];
+            m = fmaxf(m, (float)fmax(fabs(v.x), fabs(v.y)));
+        }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        float t = 0.f;
+        for (int q = 0; q < S; ++q) t = fmaxf(t, sh[q * k + threadIdx.x]);
+        atomicMax(out + threadIdx.x, __float_as_uint(t));
+    }
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
@@ -218,57 +234,88 @@ __device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
-// Y (n x k complex FP64) = diag(row_inv) (A_r X_r) diag(col_inv), MFMA with FP32 accumulation.
-// Block: 4 waves x 32 rows; each wave computes 32 rows x BN columns (NT = BN/32 MFMA tiles).
+// Split-K MFMA product with FP32 accumulation.  Block = 4 waves x 32 rows
+// (128 rows) and one of LP_SPLIT K slabs; slab = blockIdx % LP_SPLIT, so on
+// the 8-XCD part every XCD streams one slab of the transposed block operand,
+// which stays in that XCD's L2.  The slab is staged through LDS 128 k at a
+// time (padded rows, conflict-free 16-byte reads) and shared by the 4 waves;
+// each wave streams its 32 rows of A straight into MFMA operands.  Partial
+// sums go to Ypart[slab][row][col] and lp_reduce_kernel adds the slabs in a
+// fixed order.
+constexpr int LP_SPLIT = 8;
+constexpr int LP_BK = 128;
+constexpr int LP_LDS_ROW = LP_BK + 8;
+
 template <int FMT, int NT>
 __global__ __launch_bounds__(BLOCK) void lp_gemm_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
-                                                        const u16* __restrict__ XT, int k,
-                                                        const double* __restrict__ row_inv,
-                                                        const double* __restrict__ col_inv,
-                                                        double* __restrict__ Y) {
+                                                        const u16* __restrict__ XT, int bn,
+                                                        float* __restrict__ Ypart) {
+    __shared__ __attribute__((aligned(16))) u16 Bs[NT * 32 * LP_LDS_ROW];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t row_base = (int64_t)blockIdx.x * 128 + wave * 32;
-    if (row_base >= n) return;
+    const int split = blockIdx.x % LP_SPLIT;
+    const int64_t row_base = (int64_t)(blockIdx.x / LP_SPLIT) * 128 + wave * 32;
+    const int64_t slab = (2 * ld) / LP_SPLIT;  // a multiple of LP_BK (ld is a multiple of 512)
+    const int64_t k_begin = split * slab;
     const int64_t arow = min(row_base + (lane & 31), n - 1);
-    const int kq = (lane >> 5) * 8;  // this lane's 8-element k offset within a 16-k step
+    const int kq = (lane >> 5) * 8;
     const u32x4* ap = reinterpret_cast<const u32x4*>(A + arow * 2 * ld + kq);
-    const u32x4* bp[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-        bp[t] = reinterpret_cast<const u32x4*>(XT + (int64_t)(t * 32 + (lane & 31)) * 2 * ld + kq);
     f32x16 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
-    const int64_t steps = (2 * ld) / 16;  // 16 k per MFMA; a uint4 step is 8 halves, so 2 uint4 per 16 k
-    for (int64_t s = 0; s < steps; s += 2) {
-        u32x4 a0 = __builtin_nontemporal_load(ap + s * 2);
-        u32x4 a1 = __builtin_nontemporal_load(ap + (s + 1) * 2);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            u32x4 b0 = bp[t][s * 2];
-            u32x4 b1 = bp[t][(s + 1) * 2];
-            acc[t] = mfma32<FMT>(a0, b0, acc[t]);
-            acc[t] = mfma32<FMT>(a1, b1, acc[t]);
+    for (int64_t kc = k_begin; kc < k_begin + slab; kc += LP_BK) {
+        // stage XT[0 .. NT*32)[kc .. kc+LP_BK) into LDS (16 bytes per thread per round)
+        for (int e = threadIdx.x; e < NT * 32 * (LP_BK / 8); e += BLOCK) {
+            const int r = e / (LP_BK / 8);
+            const int q = e - r * (LP_BK / 8);
+            *reinterpret_cast<u32x4*>(&Bs[r * LP_LDS_ROW + q * 8]) =
+                *reinterpret_cast<const u32x4*>(XT + (int64_t)r * 2 * ld + kc + q * 8);
         }
+        u32x4 a[LP_BK / 16];
+#pragma unroll
+        for (int s = 0; s < LP_BK / 16; ++s) a[s] = __builtin_nontemporal_load(ap + (kc + 16 * s) / 8);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < LP_BK / 16; ++s) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[(t * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
+                acc[t] = mfma32<FMT>(a[s], b, acc[t]);
+            }
+        }
+        __syncthreads();
     }
     // C layout of v_mfma_f32_32x32x*: column = lane % 32,
     // row = 8*(v/4) + 4*(lane/32) + (v%4) for accumulator register v.
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int col = t * 32 + (lane & 31);
-        if (col >= 2 * k) continue;
-        const int c = col < k ? col : col - k;
-        const int part = col < k ? 0 : 1;
-        const double cs = col_inv[c];
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int64_t row = row_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            if (row < n) Y[(row * k + c) * 2 + part] = (double)acc[t][v] * row_inv[row] * cs;
+            if (row < n) Ypart[((int64_t)split * n + row) * bn + col] = acc[t][v];
         }
     }
+}
+
+// Y[row][c] = row_inv[row] * col_inv[c] * sum_slab (Ypart[..][row][c] + i Ypart[..][row][k+c])
+__global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int k, int bn,
+                                 const double* __restrict__ row_inv, const double* __restrict__ col_inv,
+                                 double2* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int64_t row = t / k;
+    const int c = (int)(t - row * k);
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < LP_SPLIT; ++s) {
+        const float* p = Ypart + ((int64_t)s * n + row) * bn;
+        re += (double)p[c];
+        im += (double)p[k + c];
+    }
+    const double sc = row_inv[row] * col_inv[c];
+    Y[t] = make_double2(re * sc, im * sc);
 }
 
 // FP64 complex block product Y (n x k) = A (n x n, lda) X (n x k), k <= 64 per launch (column
@@ -344,7 +391,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// out[(vec * chunks + chunk) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2
+// out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2
 __global__ __launch_bounds__(BLOCK) void bdots_kernel(const double2* __restrict__ V, int64_t ldv, int nvec,
                                                       const double2* __restrict__ W, int64_t n, int k,
                                                       int64_t chunk_len, double2* __restrict__ out) {
@@ -380,7 +427,7 @@ __global__ __launch_bounds__(BLOCK) void bdots_kernel(const double2* __restrict_
             tsum.x += p.x;
             tsum.y += p.y;
         }
-        out[((int64_t)vec * gridDim.x + blockIdx.x) * k + threadIdx.x] = tsum;
+        out[((int64_t)blockIdx.x * gridDim.y + vec) * k + threadIdx.x] = tsum;
     }
 }
 
@@ -451,6 +498,15 @@ __global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restr
     }
 }
 
+// out[o] = sum_q part[q * nout + o] for o < nout (fixed order over the chunks)
+__global__ void bsum_kernel(const double* __restrict__ part, int chunks, int64_t nout, double* __restrict__ out) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout) return;
+    double s = 0.0;
+    for (int q = 0; q < chunks; ++q) s += part[(int64_t)q * nout + o];
+    out[o] = s;
+}
+
 // X[:,c] += sum_i Yc[i*k+c] V_i[:,c]
 __global__ void bupdate_kernel(const double2* __restrict__ V, int64_t ldv, int nvec, const double2* __restrict__ Yc,
                                double2* __restrict__ X, int64_t n, int k) {
@@ -486,7 +542,9 @@ inline unsigned nblocks(int64_t n, int per = BLOCK) { return (unsigned)((n + per
 
 }  // namespace
 
-int64_t lowp_padded_ld(int64_t n) { return (n + 63) / 64 * 64; }
+int64_t lowp_padded_ld(int64_t n) { return (n + 511) / 512 * 512; }
+
+int64_t lowp_split_count() { return LP_SPLIT; }
 
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
                   uint16_t* out, double* row_inv) {
@@ -515,10 +573,14 @@ void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
 }
 
 void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, double2* Y) {
-    // k <= 64: BN = 2k rounded up to 32
+               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y) {
+    // k <= 64: the real block operand has 2k columns, padded to NT*32
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, ctx->stream, colmax, k, 0u);
-    hipLaunchKernelGGL(colmax_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, colmax);
+    {
+        const int64_t rpb = std::max<int64_t>(1, 4096 / k);
+        hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)((n + rpb - 1) / rpb)), dim3(BLOCK), 0, ctx->stream, X, n, k,
+                           rpb, colmax);
+    }
     if (format == 0)
         hipLaunchKernelGGL(x_to_lp_kernel<0>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld, colmax, XT,
                            col_inv);
@@ -527,13 +589,13 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
                            col_inv);
     SBH_HIP(hipGetLastError());
     const int nt = (2 * k + 31) / 32;
-    const unsigned grid = (unsigned)((n + 127) / 128);
-    double* Yd = reinterpret_cast<double*>(Y);
+    const int bn = nt * 32;
+    const unsigned grid = (unsigned)(((n + 127) / 128) * LP_SPLIT);
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
-#define SBH_LP_GEMM(F, NTV)                                                                                      \
-    hipLaunchKernelGGL((lp_gemm_kernel<F, NTV>), dim3(grid), dim3(BLOCK), 0, ctx->stream, A, ld, n, XT, k, row_inv, \
-                       col_inv, Yd)
+#define SBH_LP_GEMM(F, NTV)                                                                                        \
+    hipLaunchKernelGGL((lp_gemm_kernel<F, NTV>), dim3(grid), dim3(BLOCK), 0, ctx->stream, reinterpret_cast<const u16*>(A), \
+                       ld, n, reinterpret_cast<const u16*>(XT), bn, Ypart)
     if (format == 0) {
         if (nt == 1) SBH_LP_GEMM(0, 1);
         else if (nt == 2) SBH_LP_GEMM(0, 2);
@@ -546,6 +608,9 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
         else SBH_LP_GEMM(1, 4);
     }
 #undef SBH_LP_GEMM
+    SBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(lp_reduce_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, Ypart, n, k, bn, row_inv,
+                       col_inv, Y);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_LOWP, ev);
 }
@@ -562,32 +627,56 @@ void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const
     ctx->end_timed(SBH_KERNEL_BLOCK, ev);
 }
 
-int bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
-          double2* out) {
-    int chunks = reduction_chunks(n);
-    int64_t len = (n + chunks - 1) / chunks;
+// Rows per block of the column-batched reductions: ~4096 elements per block.
+static int batched_chunks(int64_t n, int k, int64_t* len) {
+    const int64_t rpb = std::max<int64_t>(1, 4096 / k);
+    int64_t chunks = (n + rpb - 1) / rpb;
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 4096));
+    *len = (n + chunks - 1) / chunks;
+    return (int)chunks;
+}
+
+// Partials land in `scratch` (chunks x (nvec+1) x k complex); the reduced
+// (nvec+1) x k complex values land in `out`.
+void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
+           double2* scratch, double2* out) {
+    int64_t len;
+    const int chunks = batched_chunks(n, k, &len);
     hipLaunchKernelGGL(bdots_kernel, dim3((unsigned)chunks, (unsigned)(nvec + 1)), dim3(BLOCK), 0, ctx->stream, V, ldv,
-                       nvec, W, n, k, len, out);
+                       nvec, W, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
-    return chunks;
+    const int64_t nout = (int64_t)(nvec + 1) * k * 2;
+    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream,
+                       reinterpret_cast<const double*>(scratch), chunks, nout, reinterpret_cast<double*>(out));
+    SBH_HIP(hipGetLastError());
 }
 
-int bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
-             int k, double* out) {
-    int chunks = reduction_chunks(n);
-    int64_t len = (n + chunks - 1) / chunks;
+// W -= sum C_i V_i per column; the reduced k squared norms land in `out`.
+void bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
+              int k, double* scratch, double* out) {
+    int64_t len;
+    const int chunks = batched_chunks(n, k, &len);
     hipLaunchKernelGGL(bproject_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n, k,
-                       len, out);
+                       len, scratch);
     SBH_HIP(hipGetLastError());
-    return chunks;
+    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(k)), dim3(BLOCK), 0, ctx->stream, scratch, chunks, (int64_t)k, out);
+    SBH_HIP(hipGetLastError());
 }
 
-int bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* out) {
-    int chunks = reduction_chunks(n);
-    int64_t len = (n + chunks - 1) / chunks;
-    hipLaunchKernelGGL(bresidual_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len, out);
+// R = B - R (B may be null: norms only); the reduced k squared norms land in `out`.
+void bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* scratch, double* out) {
+    int64_t len;
+    const int chunks = batched_chunks(n, k, &len);
+    hipLaunchKernelGGL(bresidual_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
-    return chunks;
+    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(k)), dim3(BLOCK), 0, ctx->stream, scratch, chunks, (int64_t)k, out);
+    SBH_HIP(hipGetLastError());
+}
+
+int64_t batched_scratch_elems(int64_t n, int k, int max_vec) {
+    int64_t len;
+    const int chunks = batched_chunks(n, k, &len);
+    return (int64_t)chunks * (max_vec + 1) * k;  // complex elements
 }
 
 void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,
