@@ -315,6 +315,88 @@ sbh_status sbh_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const sbh_c64* B,
                         sbh_gmres_ir_status* status);
 
 /* ---------------------------------------------------------------------------
+ * Batches of independent rigid scatterers.  No reference counterpart: the
+ * reference assembles and solves one system at a time
+ * (build_tbem_system_with_beta_parallel, tbem.rs:1284-1466; solve_many,
+ * bem_solver.rs:2032, for several right-hand sides of one system).  A batch
+ * holds B meshes (velocity BC 0 on every panel, sbh_mesh_create with NULL BC
+ * arrays), each with its own wavenumber, Burton-Miller coupling and 1..16
+ * plane-wave directions, and assembles and solves all of them together:
+ *   - sbh_batch_assemble: two persistent launches fill every instance's FP64
+ *     matrix (the same entries as sbh_workspace_assemble) and its plane-wave
+ *     right-hand sides (those of sbh_incident_plane_wave_rhs, amplitude 1);
+ *   - sbh_batch_solve: all columns of all instances advance in lock step
+ *     through restarted GMRES (format SBH_BATCH_FP64) or GMRES iterative
+ *     refinement (FP16/BF16 inner Arnoldi on the row-scaled copy, FP64
+ *     residuals), with the Hessenberg/Givens work on the device so that one
+ *     restart cycle is a fixed launch sequence, captured once in a HIP graph
+ *     and replayed (use_graph = 1).  A column is converged when its
+ *     recomputed FP64 ||b - A x|| / ||b|| < tolerance. */
+typedef struct sbh_batch sbh_batch;
+
+enum { SBH_BATCH_FP64 = -1, SBH_BATCH_FP16_IR = 0, SBH_BATCH_BF16_IR = 1 };
+
+sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_mesh* const* meshes,
+                            int route_threads, sbh_batch** out);
+void sbh_batch_destroy(sbh_batch* batch);
+/* physics[b], beta[b] and num_rhs[b] (1..16) per instance; directions holds
+ * sum(num_rhs) unit vectors (3 doubles each) in instance order; sign is the
+ * incident-field sign of add_incident_field_rhs_stabilized. */
+sbh_status sbh_batch_set_problem(sbh_batch* batch, const sbh_physics* physics, const sbh_c64* beta,
+                                 const int32_t* num_rhs, const double* directions, double sign);
+sbh_status sbh_batch_assemble(sbh_batch* batch);
+/* Row-scaled complex FP16 (0) or BF16 (1) copy of every instance matrix. */
+sbh_status sbh_batch_convert(sbh_batch* batch, int format);
+
+typedef struct {
+    int32_t format;           /* SBH_BATCH_FP64, SBH_BATCH_FP16_IR, SBH_BATCH_BF16_IR */
+    int32_t use_graph;        /* capture one restart cycle in a HIP graph */
+    int64_t restart;          /* Arnoldi steps per cycle (1..200) */
+    int64_t max_cycles;       /* restart cycles (FP64) or refinement steps (IR) */
+    double tolerance;         /* on the recomputed FP64 relative residual */
+    double inner_tolerance;   /* IR: inner estimate relative to the current residual */
+} sbh_batch_solve_config;
+
+typedef struct {
+    int64_t iterations;             /* Arnoldi steps (operator products) */
+    int64_t cycles;                 /* restarts (FP64) or refinement steps (IR) after the first cycle */
+    int64_t reorthogonalizations;
+    double residual;                /* recomputed FP64 ||b - A x|| / ||b|| */
+    int32_t converged;
+    int32_t pad;
+    double completion_ms;           /* device clock: last sbh_batch_assemble start -> convergence */
+} sbh_batch_column_status;
+
+/* status has sum(num_rhs) entries, instance-major. */
+sbh_status sbh_batch_solve(sbh_batch* batch, const sbh_batch_solve_config* config,
+                           sbh_batch_column_status* status);
+/* n x num_rhs row-major blocks of one instance (host memory). */
+sbh_status sbh_batch_download_solution(sbh_batch* batch, int64_t instance, sbh_c64* x);
+sbh_status sbh_batch_download_rhs(sbh_batch* batch, int64_t instance, sbh_c64* rhs);
+sbh_status sbh_batch_download_matrix(sbh_batch* batch, int64_t instance, sbh_c64* a);
+sbh_status sbh_batch_info(const sbh_batch* batch, int64_t* num_instances, int64_t* num_columns,
+                          int64_t* total_panels, int64_t* matrix_entries, double* route_ms);
+
+/* Launch shapes: rows per assembly item, persistent workgroups per CU,
+ * elements per vector-kernel chunk. */
+typedef struct {
+    int32_t rows_per_item;   /* 1..64 */
+    int32_t blocks_per_cu;   /* 1..16 */
+    int32_t chunk_elems;     /* 256..65536 */
+    int32_t pad;
+} sbh_batch_tuning;
+
+sbh_status sbh_batch_get_tuning(const sbh_batch* batch, sbh_batch_tuning* tuning);
+sbh_status sbh_batch_set_tuning(sbh_batch* batch, const sbh_batch_tuning* tuning);
+/* Times every candidate shape on this batch with HIP events (the assembly
+ * once per (rows_per_item, blocks_per_cu) pair; the chunk size on one
+ * Gram-Schmidt pass over 16 vectors), keeps the fastest, and reports the
+ * candidates and their times (up to max_entries rows of 4 doubles:
+ * rows_per_item, blocks_per_cu, chunk_elems, ms; unused fields 0). */
+sbh_status sbh_batch_autotune(sbh_batch* batch, double* table, int max_entries, int* num_entries,
+                              sbh_batch_tuning* best);
+
+/* ---------------------------------------------------------------------------
  * Test hooks (no GPU work): the host route builder's view of one pair, so
  * tests can compare it with the CPU oracle's generate_subelements.
  * leaves_out: up to 110 x 6 doubles (xi,eta of 3 vertices); returns the

@@ -709,6 +709,131 @@ def gmres_ir(op: DenseMatrixOperator, lp: LowpMatrix, B, config: GmresIrConfig):
     return X, out
 
 
+BATCH_FORMATS = {"fp64": -1, "fp16_ir": 0, "bf16_ir": 1}
+
+
+@dataclass
+class BatchColumnStatus:
+    iterations: int
+    cycles: int
+    reorthogonalizations: int
+    residual: float
+    converged: bool
+    completion_ms: float
+
+
+class Batch:
+    """Many independent rigid scatterers assembled and solved together.
+
+    ``meshes`` are :class:`TbemMesh` objects (velocity BC 0 on every panel);
+    ``set_problem`` gives each one a wavenumber, Burton-Miller beta (default
+    i/k) and 1..16 plane-wave directions.  ``assemble`` fills every FP64
+    matrix and right-hand side in two persistent launches; ``solve`` runs all
+    columns in lock step (restarted GMRES or GMRES iterative refinement on an
+    FP16/BF16 copy) with each restart cycle replayed from one HIP graph.
+    """
+
+    def __init__(self, ctx: Context, meshes, route_threads: int = 0):
+        self.ctx = ctx
+        self.meshes = list(meshes)
+        arr = (ctypes.c_void_p * len(self.meshes))(*[m.handle.value for m in self.meshes])
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_batch_create(ctx.handle, len(self.meshes), arr, int(route_threads), ctypes.byref(h)))
+        self._h = h
+        self.num_rhs = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        b, c, p, e = (ctypes.c_int64() for _ in range(4))
+        r = ctypes.c_double()
+        _check(_lib().sbh_batch_info(self._h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(p), ctypes.byref(e),
+                                     ctypes.byref(r)))
+        return {"instances": b.value, "columns": c.value, "panels": p.value, "matrix_entries": e.value,
+                "route_ms": r.value}
+
+    def set_problem(self, wave_numbers, directions, betas=None, sign: float = 1.0):
+        nb = len(self.meshes)
+        ks = [float(k) for k in wave_numbers]
+        if len(ks) != nb or len(directions) != nb:
+            raise ValueError("one wavenumber and one direction array per instance")
+        phys = (_ffi.physics_t * nb)()
+        bet = (_ffi.c64 * nb)()
+        for i, k in enumerate(ks):
+            ph = PhysicsParams.from_wavenumber(k)
+            phys[i] = ph._p
+            bv = ph.burton_miller_beta() if betas is None else complex(betas[i])
+            bet[i] = _ffi.c64(bv.real, bv.imag)
+        dirs = [np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3) for d in directions]
+        counts = np.array([d.shape[0] for d in dirs], dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate(dirs, axis=0))
+        _check(_lib().sbh_batch_set_problem(self._h, phys, bet, _cptr(counts), _cptr(flat), float(sign)))
+        self.num_rhs = counts
+
+    def assemble(self):
+        _check(_lib().sbh_batch_assemble(self._h))
+
+    def convert(self, fmt: str):
+        _check(_lib().sbh_batch_convert(self._h, {"fp16": 0, "bf16": 1}[fmt]))
+
+    def solve(self, fmt: str = "fp64", restart: int = 100, max_cycles: int = 10, tolerance: float = 1e-8,
+              inner_tolerance: float = 1e-4, use_graph: bool = True):
+        cfg = _ffi.batch_solve_config_t(BATCH_FORMATS[fmt], 1 if use_graph else 0, int(restart), int(max_cycles),
+                                        float(tolerance), float(inner_tolerance))
+        ncol = int(self.num_rhs.sum())
+        raws = (_ffi.batch_column_status_t * ncol)()
+        _check(_lib().sbh_batch_solve(self._h, ctypes.byref(cfg), raws))
+        return [BatchColumnStatus(int(r.iterations), int(r.cycles), int(r.reorthogonalizations), float(r.residual),
+                                  bool(r.converged), float(r.completion_ms)) for r in raws]
+
+    def _download(self, fn, i, shape):
+        out = np.empty(shape, dtype=np.complex128)
+        _check(fn(self._h, int(i), _cptr(out)))
+        return out
+
+    def solution(self, i: int) -> np.ndarray:
+        return self._download(_lib().sbh_batch_download_solution, i, (self.meshes[i].num_elements, self.num_rhs[i]))
+
+    def rhs(self, i: int) -> np.ndarray:
+        return self._download(_lib().sbh_batch_download_rhs, i, (self.meshes[i].num_elements, self.num_rhs[i]))
+
+    def matrix(self, i: int) -> np.ndarray:
+        n = self.meshes[i].num_elements
+        return self._download(_lib().sbh_batch_download_matrix, i, (n, n))
+
+    def tuning(self) -> dict:
+        t = _ffi.batch_tuning_t()
+        _check(_lib().sbh_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"rows_per_item": t.rows_per_item, "blocks_per_cu": t.blocks_per_cu, "chunk_elems": t.chunk_elems}
+
+    def set_tuning(self, rows_per_item: int, blocks_per_cu: int, chunk_elems: int):
+        t = _ffi.batch_tuning_t(int(rows_per_item), int(blocks_per_cu), int(chunk_elems), 0)
+        _check(_lib().sbh_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self):
+        """Times the candidate launch shapes on this batch; keeps and returns the fastest."""
+        table = np.zeros((64, 4))
+        cnt = ctypes.c_int()
+        best = _ffi.batch_tuning_t()
+        _check(_lib().sbh_batch_autotune(self._h, _cptr(table), 64, ctypes.byref(cnt), ctypes.byref(best)))
+        rows = [dict(zip(("rows_per_item", "blocks_per_cu", "chunk_elems", "ms"), r)) for r in table[: cnt.value]]
+        return {"rows_per_item": best.rows_per_item, "blocks_per_cu": best.blocks_per_cu,
+                "chunk_elems": best.chunk_elems}, rows
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_sincos(ctx: Context, x) -> tuple:
     """The assembly kernels' sin/cos evaluated on the device (test hook)."""
     xv = np.ascontiguousarray(x, dtype=np.float64)
@@ -735,5 +860,5 @@ __all__ = [
     "IdentityPreconditioner", "JacobiPreconditioner", "GmresConfig", "GmresSolution", "gmres",
     "gmres_with_guess", "gmres_preconditioned", "gmres_preconditioned_with_guess", "gmres_device",
     "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE", "gmres_block", "LowpMatrix", "GmresIrConfig",
-    "GmresIrStatus", "gmres_ir",
+    "GmresIrStatus", "gmres_ir", "Batch", "BatchColumnStatus",
 ]

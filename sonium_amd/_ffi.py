@@ -64,6 +64,23 @@ class gmres_ir_status_t(ctypes.Structure):
                 ("fp64_products", ctypes.c_int64), ("lowp_products", ctypes.c_int64)]
 
 
+class batch_solve_config_t(ctypes.Structure):
+    _fields_ = [("format", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("restart", ctypes.c_int64),
+                ("max_cycles", ctypes.c_int64), ("tolerance", ctypes.c_double),
+                ("inner_tolerance", ctypes.c_double)]
+
+
+class batch_column_status_t(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("cycles", ctypes.c_int64),
+                ("reorthogonalizations", ctypes.c_int64), ("residual", ctypes.c_double),
+                ("converged", ctypes.c_int32), ("pad", ctypes.c_int32), ("completion_ms", ctypes.c_double)]
+
+
+class batch_tuning_t(ctypes.Structure):
+    _fields_ = [("rows_per_item", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("chunk_elems", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _pp = ctypes.POINTER(ctypes.c_void_p)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -133,6 +150,20 @@ SIGNATURES = {
     "sbh_lowp_matrix_download": (_int, [_vp, _vp, _vp, ctypes.POINTER(_i64)]),
     "sbh_gmres_ir": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(gmres_ir_config_t), _vp, _int,
                             ctypes.POINTER(gmres_ir_status_t)]),
+    "sbh_batch_create": (_int, [_vp, _i64, _vp, _int, _pp]),
+    "sbh_batch_destroy": (None, [_vp]),
+    "sbh_batch_set_problem": (_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_double]),
+    "sbh_batch_assemble": (_int, [_vp]),
+    "sbh_batch_convert": (_int, [_vp, _int]),
+    "sbh_batch_solve": (_int, [_vp, ctypes.POINTER(batch_solve_config_t), _vp]),
+    "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),
+    "sbh_batch_download_rhs": (_int, [_vp, _i64, _vp]),
+    "sbh_batch_download_matrix": (_int, [_vp, _i64, _vp]),
+    "sbh_batch_info": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                              ctypes.POINTER(_i64), _dp]),
+    "sbh_batch_get_tuning": (_int, [_vp, ctypes.POINTER(batch_tuning_t)]),
+    "sbh_batch_set_tuning": (_int, [_vp, ctypes.POINTER(batch_tuning_t)]),
+    "sbh_batch_autotune": (_int, [_vp, _dp, _int, ctypes.POINTER(_int), ctypes.POINTER(batch_tuning_t)]),
     "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sbh_debug_device_copy": (_int, [_vp, _i64, _int]),

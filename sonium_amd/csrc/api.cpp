@@ -25,23 +25,6 @@ void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s
 namespace {
 thread_local std::string g_last_error;
 
-template <typename F>
-sbh_status guard(F&& f) {
-    try {
-        f();
-        return SBH_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return SBH_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return SBH_ERR_BACKEND;
-    }
-}
-
 inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
 
 // Device copy of a host or device input vector.

@@ -1,0 +1,741 @@
+// Batches of independent rigid scatterers: packed device layout, persistent
+// assembly, lock-step restarted GMRES / GMRES iterative refinement with the
+// per-column Arnoldi bookkeeping on the device, HIP-graph capture of one
+// restart cycle, and the launch-shape autotuner.
+//
+// Per column the recurrences are those of the single-system solver
+// (gmres.cpp, after gmres_core, gmres.rs:631-1057): classical Gram–Schmidt
+// with a second pass when ||w|| < 0.707 ||w0||, complex Givens rotations
+// (gmres_common.rs:91-105), back substitution, and convergence on the
+// recomputed FP64 residual.  Differences from the single-system loop:
+// columns whose estimate converges mid-cycle stop there and have their
+// solution formed and checked at the end of the cycle, when all columns do;
+// with refinement (format FP16/BF16) a cycle is one refinement step — an
+// inner Arnoldi cycle on the low-precision copy from the current residual,
+// x += V y, then the FP64 residual (Carson & Higham 2018, GMRES-IR).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+
+#include "batch.hpp"
+
+struct sbh_batch {
+    sbh_context* ctx = nullptr;
+    int B = 0;
+    int64_t total_panels = 0;
+    double route_ms = 0.0;
+    std::vector<const sbh_mesh*> meshes;
+    std::vector<sbh::BatchInstance> inst;
+    // geometry and routes (concatenated)
+    sbh::DeviceArray<sbh::FarPanel> d_far;
+    sbh::DeviceArray<sbh::Source> d_src;
+    sbh::DeviceArray<sbh::PanelGeom> d_geom;
+    sbh::DeviceArray<sbh::NearPair> d_pairs;
+    sbh::DeviceArray<uint64_t> d_leaves;
+    sbh::DeviceArray<int64_t> d_row_ptr;
+    sbh::DeviceArray<int> d_row_inst;
+    // problem
+    bool has_problem = false;
+    int C = 0;
+    int64_t L = 0;
+    int64_t a_elems = 0;
+    int64_t lp_elems = 0;
+    int lp_format = -2;
+    sbh::DeviceArray<sbh::BatchInstance> d_inst;
+    sbh::DeviceArray<double> d_dirs;
+    sbh::DeviceArray<int2> d_items;
+    sbh::DeviceArray<unsigned> d_counter;
+    sbh::DeviceArray<int2> d_tiles;
+    sbh::DeviceArray<double2> d_A;
+    sbh::DeviceArray<uint16_t> d_Alp;
+    sbh::DeviceArray<double> d_row_inv;
+    sbh::DeviceArray<double2> d_rhs, d_x;
+    sbh::DeviceArray<int> d_col_inst, d_col_q;
+    sbh::DeviceArray<unsigned long long> d_t0;
+    std::vector<int> col_inst, col_q;
+    int nitems = 0, ntiles = 0;
+    // vector-kernel chunks (depend on the tuning)
+    sbh::DeviceArray<sbh::VecChunk> d_chunks;
+    sbh::DeviceArray<int2> d_chunk_range;
+    int nchunks = 0;
+    int chunks_for = 0;  // chunk_elems the chunk table was built for
+    // tuning
+    sbh_batch_tuning tuning{16, 4, 4096, 0};
+    int cu_count = 256;
+    // solver workspace
+    struct Solver {
+        int m = 0, MV = 0;
+        sbh::DeviceArray<double2> V, W, R, H, cs, sn, g, coef, part;
+        sbh::DeviceArray<double> wn0, wn, est, bnorm, rnorm, scale, tol_abs;
+        sbh::DeviceArray<int> active, reorth, done, jlen, inst_active, inst_reorth, inst_pending;
+        sbh::DeviceArray<int64_t> iters, cycles, reorth_count;
+        sbh::DeviceArray<unsigned long long> t_done;
+        hipGraphExec_t exec = nullptr;
+        int graph_format = -3;
+        double graph_tol = 0.0, graph_inner_tol = 0.0;  // captured as kernel arguments
+        ~Solver() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+        }
+    };
+    std::unique_ptr<Solver> solver;
+};
+
+namespace sbh {
+
+namespace {
+
+inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
+
+constexpr int kMaxCol = 16;
+
+template <typename T>
+void upload(DeviceArray<T>& d, const std::vector<T>& h) {
+    d.allocate(h.size());
+    if (!h.empty()) SBH_HIP(hipMemcpy(d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void build_chunks(sbh_batch* b) {
+    const int64_t elems = b->tuning.chunk_elems;
+    std::vector<VecChunk> chunks;
+    std::vector<int2> range(b->B);
+    for (int i = 0; i < b->B; ++i) {
+        const BatchInstance& I = b->inst[i];
+        const int64_t rows = std::max<int64_t>(1, elems / I.ncol);
+        range[i].x = (int)chunks.size();
+        for (int64_t r0 = 0; r0 < I.n; r0 += rows) {
+            VecChunk c;
+            c.inst = i;
+            c.r0 = r0;
+            c.rows = (int32_t)std::min<int64_t>(rows, I.n - r0);
+            chunks.push_back(c);
+        }
+        range[i].y = (int)chunks.size();
+    }
+    SBH_REQUIRE(chunks.size() < (size_t)INT32_MAX, "too many vector chunks");
+    upload(b->d_chunks, chunks);
+    upload(b->d_chunk_range, range);
+    b->nchunks = (int)chunks.size();
+    b->chunks_for = b->tuning.chunk_elems;
+    b->solver.reset();  // partial-sum buffers are sized by the chunk count
+}
+
+std::vector<int> size_order(const sbh_batch* b) {
+    // largest instances first so the persistent grid ends balanced
+    std::vector<int> order(b->B);
+    for (int i = 0; i < b->B; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->inst[x].n > b->inst[y].n; });
+    return order;
+}
+
+// Assembly items (instance, first row); they depend on the tuning.
+void build_items(sbh_batch* b) {
+    std::vector<int2> items;
+    const int R = b->tuning.rows_per_item;
+    for (int i : size_order(b))
+        for (int64_t r0 = 0; r0 < b->inst[i].n; r0 += R) items.push_back(make_int2(i, (int)r0));
+    SBH_REQUIRE(items.size() < (size_t)INT32_MAX, "too many assembly items");
+    upload(b->d_items, items);
+    b->nitems = (int)items.size();
+}
+
+// Product tiles (instance, first of 128 rows); fixed per problem, since a
+// captured solve graph holds their address.
+void build_tiles(sbh_batch* b) {
+    std::vector<int2> tiles;
+    for (int i : size_order(b))
+        for (int64_t r0 = 0; r0 < b->inst[i].n; r0 += 128) tiles.push_back(make_int2(i, (int)r0));
+    upload(b->d_tiles, tiles);
+    b->ntiles = (int)tiles.size();
+}
+
+RaggedLayout layout(const sbh_batch* b) {
+    RaggedLayout l;
+    l.B = b->B;
+    l.C = b->C;
+    l.nchunks = b->nchunks;
+    l.MV = b->solver->MV;
+    l.m = b->solver->m;
+    l.L = b->L;
+    l.inst = b->d_inst.ptr;
+    l.chunks = b->d_chunks.ptr;
+    l.chunk_range = b->d_chunk_range.ptr;
+    l.col_inst = b->d_col_inst.ptr;
+    l.col_q = b->d_col_q.ptr;
+    return l;
+}
+
+void ensure_solver(sbh_batch* b, int m) {
+    if (b->chunks_for != b->tuning.chunk_elems) build_chunks(b);
+    if (b->solver && b->solver->m == m) return;
+    b->solver.reset(new sbh_batch::Solver());
+    auto& s = *b->solver;
+    s.m = m;
+    s.MV = m + 2;
+    const size_t C = (size_t)b->C, L = (size_t)b->L;
+    s.V.allocate((size_t)(m + 1) * L);
+    s.W.allocate(L);
+    s.R.allocate(L);
+    s.H.allocate(C * m * (m + 1));
+    s.cs.allocate(C * m);
+    s.sn.allocate(C * m);
+    s.g.allocate(C * (m + 1));
+    s.coef.allocate(C * s.MV);
+    s.part.allocate((size_t)b->nchunks * s.MV * kMaxCol);
+    for (auto* a : {&s.wn0, &s.wn, &s.est, &s.bnorm, &s.rnorm, &s.scale, &s.tol_abs}) a->allocate(C);
+    for (auto* a : {&s.active, &s.reorth, &s.done, &s.jlen}) a->allocate(C);
+    for (auto* a : {&s.inst_active, &s.inst_reorth, &s.inst_pending}) a->allocate((size_t)b->B);
+    for (auto* a : {&s.iters, &s.cycles, &s.reorth_count}) a->allocate(C);
+    s.t_done.allocate(C);
+    // finite contents everywhere: frozen columns multiply stale basis vectors
+    SBH_HIP(hipMemsetAsync(s.V.ptr, 0, s.V.bytes(), b->ctx->stream));
+    SBH_HIP(hipMemsetAsync(s.H.ptr, 0, s.H.bytes(), b->ctx->stream));
+    SBH_HIP(hipMemsetAsync(s.reorth.ptr, 0, s.reorth.bytes(), b->ctx->stream));
+}
+
+// One restart cycle (m Arnoldi steps, solution update, FP64 residual, start
+// of the next cycle) as a fixed launch sequence.
+void cycle(sbh_batch* b, int format, double tol, double inner_tol) {
+    sbh_context* ctx = b->ctx;
+    auto& s = *b->solver;
+    const RaggedLayout lay = layout(b);
+    const int m = s.m;
+    const int refinement = format >= 0 ? 1 : 0;
+    const size_t L = (size_t)b->L;
+    auto Vj = [&](int j) { return s.V.ptr + (size_t)j * L; };
+    for (int j = 0; j < m; ++j) {
+        batch_product(ctx, format, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_active.ptr, b->d_A.ptr,
+                      b->d_Alp.ptr, b->d_row_inv.ptr, Vj(j), s.W.ptr);
+        ragged_dots(ctx, lay, s.inst_active.ptr, s.V.ptr, j + 1, s.W.ptr, s.part.ptr);
+        ragged_sum_dots(ctx, lay, s.part.ptr, j + 1, j, s.active.ptr, s.reorth.ptr, 0, s.H.ptr, s.coef.ptr,
+                        s.wn0.ptr);
+        ragged_project(ctx, lay, s.inst_active.ptr, s.V.ptr, j + 1, s.coef.ptr, s.W.ptr, s.part.ptr);
+        ragged_norm(ctx, lay, s.part.ptr, s.active.ptr, s.wn0.ptr, 0, s.wn.ptr, s.reorth.ptr, s.reorth_count.ptr);
+        ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
+                     s.inst_pending.ptr);
+        ragged_dots(ctx, lay, s.inst_reorth.ptr, s.V.ptr, j + 1, s.W.ptr, s.part.ptr);
+        ragged_sum_dots(ctx, lay, s.part.ptr, j + 1, j, s.active.ptr, s.reorth.ptr, 1, s.H.ptr, s.coef.ptr,
+                        s.wn0.ptr);
+        ragged_project(ctx, lay, s.inst_reorth.ptr, s.V.ptr, j + 1, s.coef.ptr, s.W.ptr, s.part.ptr);
+        ragged_norm(ctx, lay, s.part.ptr, s.active.ptr, s.wn0.ptr, 1, s.wn.ptr, s.reorth.ptr, s.reorth_count.ptr);
+        ragged_givens(ctx, lay, j, s.active.ptr, s.wn.ptr, s.H.ptr, s.cs.ptr, s.sn.ptr, s.g.ptr, s.tol_abs.ptr,
+                      s.est.ptr, s.jlen.ptr, s.iters.ptr, s.scale.ptr);
+        if (j + 1 < m)
+            ragged_scale(ctx, lay, s.inst_active.ptr, s.scale.ptr, s.W.ptr, Vj(j + 1));
+        ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
+                     s.inst_pending.ptr);
+    }
+    ragged_solve(ctx, lay, s.H.ptr, s.g.ptr, s.jlen.ptr, s.coef.ptr);
+    ragged_update(ctx, lay, s.inst_pending.ptr, s.V.ptr, s.jlen.ptr, s.coef.ptr, b->d_x.ptr);
+    batch_product(ctx, -1, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
+                  nullptr, b->d_x.ptr, s.R.ptr);
+    ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
+    ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 0, tol, inner_tol, refinement, s.bnorm.ptr,
+                   s.rnorm.ptr, s.done.ptr, s.active.ptr, s.jlen.ptr, s.g.ptr, s.scale.ptr, s.tol_abs.ptr,
+                   s.cycles.ptr, s.t_done.ptr);
+    ragged_scale(ctx, lay, s.inst_pending.ptr, s.scale.ptr, s.R.ptr, Vj(0));
+    ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
+                 s.inst_pending.ptr);
+}
+
+void fill_int(sbh_context* ctx, DeviceArray<int>& a, int v) {
+    std::vector<int> h(a.count, v);
+    SBH_HIP(hipMemcpyAsync(a.ptr, h.data(), a.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    SBH_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void check_problem(const sbh_batch* b) {
+    SBH_REQUIRE(b, "batch is null");
+    SBH_REQUIRE(b->has_problem, "sbh_batch_set_problem has not been called");
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    SBH_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+void run_assembly(sbh_batch* b) {
+    BatchAssemblyArgs args;
+    args.inst = b->d_inst.ptr;
+    args.items = b->d_items.ptr;
+    args.nitems = b->nitems;
+    args.rows_per_item = b->tuning.rows_per_item;
+    args.counter = b->d_counter.ptr;
+    args.far = b->d_far.ptr;
+    args.src = b->d_src.ptr;
+    args.geom = b->d_geom.ptr;
+    args.pairs = b->d_pairs.ptr;
+    args.row_ptr = b->d_row_ptr.ptr;
+    args.leaves = b->d_leaves.ptr;
+    args.dirs = b->d_dirs.ptr;
+    args.A = b->d_A.ptr;
+    args.rhs = b->d_rhs.ptr;
+    const int64_t grid = std::min<int64_t>((int64_t)b->cu_count * b->tuning.blocks_per_cu, b->nitems);
+    launch_batch_assembly(b->ctx, args, (int)std::max<int64_t>(grid, 1));
+}
+
+}  // namespace
+}  // namespace sbh
+
+using namespace sbh;
+
+extern "C" {
+
+sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_mesh* const* meshes,
+                            int route_threads, sbh_batch** out) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && out, "ctx/out is null");
+        SBH_REQUIRE(num_instances > 0 && num_instances < (1 << 24), "num_instances out of range");
+        SBH_REQUIRE(meshes, "meshes is null");
+        *out = nullptr;
+        use_device(ctx);
+        std::unique_ptr<sbh_batch> b(new sbh_batch());
+        b->ctx = ctx;
+        b->B = (int)num_instances;
+        b->meshes.assign(meshes, meshes + num_instances);
+        int64_t panels = 0;
+        for (const sbh_mesh* m : b->meshes) {
+            SBH_REQUIRE(m, "mesh is null");
+            SBH_REQUIRE(m->ctx == ctx, "mesh belongs to another context");
+            if (m->general)
+                throw Error(SBH_ERR_UNSUPPORTED, "batches take rigid meshes only (velocity BC 0 on every panel)");
+            SBH_REQUIRE(m->n > 0 && m->n < (1 << 24), "instance size out of range");
+            panels += m->n;
+        }
+        SBH_REQUIRE(panels < (int64_t)INT32_MAX, "too many panels in the batch");
+        b->total_panels = panels;
+        // routes: instances are independent, split them over host threads
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<RouteTable> rts(b->B);
+        int nthreads = route_threads > 0 ? route_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+        nthreads = std::min(nthreads, b->B);
+        {
+            std::vector<std::thread> pool;
+            std::vector<std::exception_ptr> errs(nthreads);
+            for (int t = 0; t < nthreads; ++t)
+                pool.emplace_back([&, t] {
+                    try {
+                        for (int i = t; i < b->B; i += nthreads) rts[i] = build_routes(b->meshes[i], 1);
+                    } catch (...) {
+                        errs[t] = std::current_exception();
+                    }
+                });
+            for (auto& th : pool) th.join();
+            for (auto& e : errs)
+                if (e) std::rethrow_exception(e);
+        }
+        b->route_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        // concatenate geometry (copied from each mesh's device arrays) and routes
+        b->d_far.allocate(panels);
+        b->d_src.allocate(panels);
+        b->d_geom.allocate(panels);
+        std::vector<NearPair> pairs;
+        std::vector<uint64_t> leaves;
+        std::vector<int64_t> row_ptr((size_t)panels + 1, 0);
+        std::vector<int> row_inst((size_t)panels);
+        b->inst.resize(b->B);
+        int64_t off = 0;
+        for (int i = 0; i < b->B; ++i) {
+            const sbh_mesh* m = b->meshes[i];
+            SBH_HIP(hipMemcpy(b->d_far.ptr + off, m->d_far.ptr, m->n * sizeof(FarPanel), hipMemcpyDeviceToDevice));
+            SBH_HIP(hipMemcpy(b->d_src.ptr + off, m->d_src.ptr, m->n * sizeof(Source), hipMemcpyDeviceToDevice));
+            SBH_HIP(hipMemcpy(b->d_geom.ptr + off, m->d_geom.ptr, m->n * sizeof(PanelGeom), hipMemcpyDeviceToDevice));
+            const int64_t pair_off = (int64_t)pairs.size();
+            const int64_t leaf_off = (int64_t)leaves.size();
+            for (NearPair p : rts[i].pairs) {
+                p.begin += leaf_off;
+                pairs.push_back(p);
+            }
+            leaves.insert(leaves.end(), rts[i].leaves.begin(), rts[i].leaves.end());
+            for (int64_t r = 0; r < m->n; ++r) {
+                row_ptr[off + r] = pair_off + rts[i].row_ptr[r];
+                row_inst[off + r] = i;
+            }
+            BatchInstance& I = b->inst[i];
+            std::memset(&I, 0, sizeof(I));
+            I.panel_off = off;
+            I.n = m->n;
+            off += m->n;
+            rts[i] = RouteTable();
+        }
+        row_ptr[panels] = (int64_t)pairs.size();
+        upload(b->d_pairs, pairs);
+        upload(b->d_leaves, leaves);
+        upload(b->d_row_ptr, row_ptr);
+        upload(b->d_row_inst, row_inst);
+        b->d_counter.allocate(2);
+        b->d_t0.allocate(1);
+        SBH_HIP(hipMemset(b->d_t0.ptr, 0, sizeof(unsigned long long)));
+        int cus = 0;
+        SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        b->cu_count = std::max(cus, 1);
+        context_retain(ctx);
+        *out = b.release();
+    });
+}
+
+void sbh_batch_destroy(sbh_batch* b) {
+    if (!b) return;
+    sbh_context* ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete b;
+    context_release(ctx);
+}
+
+sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const sbh_c64* beta,
+                                 const int32_t* num_rhs, const double* directions, double sign) {
+    return guard([&] {
+        SBH_REQUIRE(b && physics && beta && num_rhs && directions, "null argument");
+        use_device(b->ctx);
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
+        int64_t a_off = 0, vec_off = 0, lp_off = 0, dir_off = 0;
+        int col = 0;
+        b->col_inst.clear();
+        b->col_q.clear();
+        for (int i = 0; i < b->B; ++i) {
+            BatchInstance& I = b->inst[i];
+            const sbh_physics& ph = physics[i];
+            SBH_REQUIRE(num_rhs[i] >= 1 && num_rhs[i] <= kMaxCol, "num_rhs must be in 1..16");
+            SBH_REQUIRE(std::isfinite(ph.wave_number), "wave number is not finite");
+            if (ph.harmonic_factor != 1.0)
+                throw Error(SBH_ERR_UNSUPPORTED, "only harmonic_factor = +1 (ACOUSTIC_CONVENTION) is supported");
+            SBH_REQUIRE(std::fabs(ph.wave_number) * b->meshes[i]->diameter < 5.0e5,
+                        "k * mesh diameter exceeds the supported phase range (5e5 rad)");
+            I.k = ph.wave_number;
+            I.tau = ph.tau;
+            I.sign = sign;
+            I.beta[0] = beta[i].re;
+            I.beta[1] = beta[i].im;
+            I.ncol = num_rhs[i];
+            I.col_off = col;
+            I.dir_off = dir_off;
+            I.a_off = a_off;
+            I.vec_off = vec_off;
+            I.ld = (I.n + 63) / 64 * 64;
+            I.lp_off = lp_off;
+            for (int q = 0; q < I.ncol; ++q) {
+                b->col_inst.push_back(i);
+                b->col_q.push_back(q);
+            }
+            a_off += I.n * I.n;
+            vec_off += I.n * I.ncol;
+            lp_off += I.n * 2 * I.ld;
+            dir_off += I.ncol;
+            col += I.ncol;
+        }
+        b->C = col;
+        b->L = vec_off;
+        b->a_elems = a_off;
+        b->lp_elems = lp_off;
+        upload(b->d_inst, b->inst);
+        b->d_dirs.allocate((size_t)dir_off * 3);
+        SBH_HIP(hipMemcpy(b->d_dirs.ptr, directions, (size_t)dir_off * 3 * sizeof(double), hipMemcpyHostToDevice));
+        upload(b->d_col_inst, b->col_inst);
+        upload(b->d_col_q, b->col_q);
+        if (b->d_A.count != (size_t)a_off) b->d_A.allocate((size_t)a_off);
+        if (b->d_rhs.count != (size_t)vec_off) {
+            b->d_rhs.allocate((size_t)vec_off);
+            b->d_x.allocate((size_t)vec_off);
+        }
+        SBH_HIP(hipMemset(b->d_x.ptr, 0, b->d_x.bytes()));
+        b->lp_format = -2;
+        build_items(b);
+        build_chunks(b);  // also drops the solver workspace and its graph
+        build_tiles(b);
+        b->has_problem = true;
+    });
+}
+
+sbh_status sbh_batch_assemble(sbh_batch* b) {
+    return guard([&] {
+        check_problem(b);
+        use_device(b->ctx);
+        ragged_stamp(b->ctx, b->d_t0.ptr);
+        run_assembly(b);
+        b->lp_format = -2;  // a new matrix invalidates the low-precision copy
+    });
+}
+
+sbh_status sbh_batch_convert(sbh_batch* b, int format) {
+    return guard([&] {
+        check_problem(b);
+        SBH_REQUIRE(format == 0 || format == 1, "format must be FP16 (0) or BF16 (1)");
+        use_device(b->ctx);
+        if (b->d_Alp.count != (size_t)b->lp_elems) {
+            b->d_Alp.allocate((size_t)b->lp_elems);
+            b->d_row_inv.allocate((size_t)b->total_panels);
+        }
+        batch_lp_convert(b->ctx, format, b->total_panels, b->d_row_inst.ptr, b->d_inst.ptr, b->d_A.ptr, b->d_Alp.ptr,
+                         b->d_row_inv.ptr);
+        b->lp_format = format;
+    });
+}
+
+sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_batch_column_status* status) {
+    return guard([&] {
+        check_problem(b);
+        SBH_REQUIRE(cfg, "config is null");
+        SBH_REQUIRE(cfg->format >= -1 && cfg->format <= 1, "format must be -1, 0 or 1");
+        SBH_REQUIRE(cfg->restart >= 1 && cfg->restart <= 200, "restart must be in 1..200");
+        SBH_REQUIRE(cfg->max_cycles >= 1, "max_cycles must be >= 1");
+        SBH_REQUIRE(cfg->tolerance > 0.0 && cfg->inner_tolerance > 0.0, "tolerances must be positive");
+        if (cfg->format >= 0)
+            SBH_REQUIRE(b->lp_format == cfg->format, "sbh_batch_convert with this format has not been called");
+        sbh_context* ctx = b->ctx;
+        use_device(ctx);
+        ensure_solver(b, (int)cfg->restart);
+        auto& s = *b->solver;
+        const RaggedLayout lay = layout(b);
+        // start: x = 0, r = b, ||b||, first basis vector
+        SBH_HIP(hipMemsetAsync(b->d_x.ptr, 0, b->d_x.bytes(), ctx->stream));
+        for (auto* a : {&s.iters, &s.cycles, &s.reorth_count})
+            SBH_HIP(hipMemsetAsync(a->ptr, 0, a->bytes(), ctx->stream));
+        SBH_HIP(hipMemsetAsync(s.t_done.ptr, 0, s.t_done.bytes(), ctx->stream));
+        SBH_HIP(hipMemsetAsync(s.done.ptr, 0, s.done.bytes(), ctx->stream));
+        fill_int(ctx, s.inst_pending, 1);
+        ragged_fill(ctx, s.R.ptr, b->L);
+        ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
+        ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 1, cfg->tolerance, cfg->inner_tolerance,
+                       cfg->format >= 0 ? 1 : 0, s.bnorm.ptr, s.rnorm.ptr, s.done.ptr, s.active.ptr, s.jlen.ptr,
+                       s.g.ptr, s.scale.ptr, s.tol_abs.ptr, s.cycles.ptr, s.t_done.ptr);
+        ragged_scale(ctx, lay, s.inst_pending.ptr, s.scale.ptr, s.R.ptr, s.V.ptr);
+        ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
+                     s.inst_pending.ptr);
+        if (cfg->use_graph && (!s.exec || s.graph_format != cfg->format || s.graph_tol != cfg->tolerance ||
+                               s.graph_inner_tol != cfg->inner_tolerance)) {
+            if (s.exec) {
+                SBH_HIP(hipGraphExecDestroy(s.exec));
+                s.exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            SBH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                cycle(b, cfg->format, cfg->tolerance, cfg->inner_tolerance);
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx->stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            SBH_HIP(hipStreamEndCapture(ctx->stream, &graph));
+            hipError_t e = hipGraphInstantiate(&s.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            SBH_HIP(e);
+            s.graph_format = cfg->format;
+            s.graph_tol = cfg->tolerance;
+            s.graph_inner_tol = cfg->inner_tolerance;
+        }
+        std::vector<int> pending(b->B);
+        for (int64_t c = 0; c < cfg->max_cycles; ++c) {
+            SBH_HIP(hipMemcpyAsync(pending.data(), s.inst_pending.ptr, pending.size() * sizeof(int),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
+            if (std::none_of(pending.begin(), pending.end(), [](int v) { return v != 0; })) break;
+            if (cfg->use_graph)
+                SBH_HIP(hipGraphLaunch(s.exec, ctx->stream));
+            else
+                cycle(b, cfg->format, cfg->tolerance, cfg->inner_tolerance);
+        }
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        if (status) {
+            const size_t C = (size_t)b->C;
+            std::vector<int64_t> it(C), cy(C), ro(C);
+            std::vector<double> rn(C), bn(C);
+            std::vector<int> dn(C);
+            std::vector<unsigned long long> td(C), t0(1);
+            SBH_HIP(hipMemcpy(it.data(), s.iters.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(cy.data(), s.cycles.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(ro.data(), s.reorth_count.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(rn.data(), s.rnorm.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(bn.data(), s.bnorm.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(dn.data(), s.done.ptr, C * 4, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(td.data(), s.t_done.ptr, C * 8, hipMemcpyDeviceToHost));
+            SBH_HIP(hipMemcpy(t0.data(), b->d_t0.ptr, 8, hipMemcpyDeviceToHost));
+            int rate_khz = 0;
+            SBH_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+            for (size_t c = 0; c < C; ++c) {
+                sbh_batch_column_status& st = status[c];
+                st.iterations = it[c];
+                st.cycles = cy[c];
+                st.reorthogonalizations = ro[c];
+                st.residual = bn[c] > 0.0 ? rn[c] / bn[c] : 0.0;
+                st.converged = dn[c] && st.residual < cfg->tolerance ? 1 : 0;
+                st.pad = 0;
+                st.completion_ms = (dn[c] && rate_khz > 0 && td[c] >= t0[0])
+                                       ? (double)(td[c] - t0[0]) / (double)rate_khz
+                                       : -1.0;
+            }
+        }
+    });
+}
+
+namespace {
+sbh_status download_block(sbh_batch* b, int64_t instance, sbh_c64* out, int what) {
+    return guard([&] {
+        check_problem(b);
+        SBH_REQUIRE(out && instance >= 0 && instance < b->B, "bad instance/out");
+        use_device(b->ctx);
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
+        const BatchInstance& I = b->inst[instance];
+        const double2* src;
+        size_t count;
+        if (what == 0) {
+            src = b->d_x.ptr + I.vec_off;
+            count = (size_t)I.n * I.ncol;
+        } else if (what == 1) {
+            src = b->d_rhs.ptr + I.vec_off;
+            count = (size_t)I.n * I.ncol;
+        } else {
+            src = b->d_A.ptr + I.a_off;
+            count = (size_t)I.n * I.n;
+        }
+        SBH_HIP(hipMemcpy(out, src, count * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+}  // namespace
+
+sbh_status sbh_batch_download_solution(sbh_batch* b, int64_t instance, sbh_c64* x) {
+    return download_block(b, instance, x, 0);
+}
+sbh_status sbh_batch_download_rhs(sbh_batch* b, int64_t instance, sbh_c64* rhs) {
+    return download_block(b, instance, rhs, 1);
+}
+sbh_status sbh_batch_download_matrix(sbh_batch* b, int64_t instance, sbh_c64* a) {
+    return download_block(b, instance, a, 2);
+}
+
+sbh_status sbh_batch_info(const sbh_batch* b, int64_t* num_instances, int64_t* num_columns, int64_t* total_panels,
+                          int64_t* matrix_entries, double* route_ms) {
+    return guard([&] {
+        SBH_REQUIRE(b, "batch is null");
+        if (num_instances) *num_instances = b->B;
+        if (num_columns) *num_columns = b->C;
+        if (total_panels) *total_panels = b->total_panels;
+        if (matrix_entries) *matrix_entries = b->a_elems;
+        if (route_ms) *route_ms = b->route_ms;
+    });
+}
+
+sbh_status sbh_batch_get_tuning(const sbh_batch* b, sbh_batch_tuning* t) {
+    return guard([&] {
+        SBH_REQUIRE(b && t, "null argument");
+        *t = b->tuning;
+    });
+}
+
+sbh_status sbh_batch_set_tuning(sbh_batch* b, const sbh_batch_tuning* t) {
+    return guard([&] {
+        SBH_REQUIRE(b && t, "null argument");
+        SBH_REQUIRE(t->rows_per_item >= 1 && t->rows_per_item <= 64, "rows_per_item must be in 1..64");
+        SBH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 16, "blocks_per_cu must be in 1..16");
+        SBH_REQUIRE(t->chunk_elems >= 256 && t->chunk_elems <= 65536, "chunk_elems must be in 256..65536");
+        use_device(b->ctx);
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
+        const bool items = t->rows_per_item != b->tuning.rows_per_item;
+        b->tuning = *t;
+        b->tuning.pad = 0;
+        if (b->has_problem) {
+            if (items) build_items(b);
+            if (b->chunks_for != b->tuning.chunk_elems) build_chunks(b);
+        }
+    });
+}
+
+sbh_status sbh_batch_autotune(sbh_batch* b, double* table, int max_entries, int* num_entries,
+                              sbh_batch_tuning* best) {
+    return guard([&] {
+        check_problem(b);
+        sbh_context* ctx = b->ctx;
+        use_device(ctx);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        int rows_out = 0;
+        auto record = [&](double r, double bp, double ce, double ms) {
+            if (table && rows_out < max_entries) {
+                table[4 * rows_out + 0] = r;
+                table[4 * rows_out + 1] = bp;
+                table[4 * rows_out + 2] = ce;
+                table[4 * rows_out + 3] = ms;
+            }
+            ++rows_out;
+        };
+        hipEvent_t e0, e1;
+        SBH_HIP(hipEventCreate(&e0));
+        SBH_HIP(hipEventCreate(&e1));
+        sbh_batch_tuning t = b->tuning;
+        // assembly shape
+        double best_ms = INFINITY;
+        sbh_batch_tuning tb = t;
+        for (int rows : {4, 8, 16, 32})
+            for (int bpc : {2, 4, 8}) {
+                b->tuning.rows_per_item = rows;
+                b->tuning.blocks_per_cu = bpc;
+                build_items(b);
+                SBH_HIP(hipEventRecord(e0, ctx->stream));
+                run_assembly(b);
+                SBH_HIP(hipEventRecord(e1, ctx->stream));
+                SBH_HIP(hipEventSynchronize(e1));
+                const double ms = elapsed_ms(e0, e1);
+                record(rows, bpc, 0, ms);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    tb.rows_per_item = rows;
+                    tb.blocks_per_cu = bpc;
+                }
+            }
+        b->tuning.rows_per_item = tb.rows_per_item;
+        b->tuning.blocks_per_cu = tb.blocks_per_cu;
+        build_items(b);
+        // chunk size: one Gram-Schmidt pass (dots + projection) over 16 basis vectors
+        best_ms = INFINITY;
+        const int nvec = 16;
+        DeviceArray<double2> V((size_t)(nvec + 1) * b->L);
+        SBH_HIP(hipMemsetAsync(V.ptr, 0, V.bytes(), ctx->stream));
+        DeviceArray<int> flag((size_t)b->B);
+        fill_int(ctx, flag, 1);
+        for (int ce : {2048, 4096, 8192, 16384}) {
+            b->tuning.chunk_elems = ce;
+            build_chunks(b);
+            DeviceArray<double2> part((size_t)b->nchunks * (nvec + 2) * kMaxCol);
+            DeviceArray<double2> coef((size_t)b->C * (nvec + 2));
+            SBH_HIP(hipMemsetAsync(coef.ptr, 0, coef.bytes(), ctx->stream));
+            RaggedLayout lay;
+            lay.B = b->B;
+            lay.C = b->C;
+            lay.nchunks = b->nchunks;
+            lay.MV = nvec + 2;
+            lay.m = nvec;
+            lay.L = b->L;
+            lay.inst = b->d_inst.ptr;
+            lay.chunks = b->d_chunks.ptr;
+            lay.chunk_range = b->d_chunk_range.ptr;
+            lay.col_inst = b->d_col_inst.ptr;
+            lay.col_q = b->d_col_q.ptr;
+            double2* W = V.ptr + (size_t)nvec * b->L;
+            SBH_HIP(hipEventRecord(e0, ctx->stream));
+            for (int rep = 0; rep < 3; ++rep) {
+                ragged_dots(ctx, lay, flag.ptr, V.ptr, nvec, W, part.ptr);
+                ragged_project(ctx, lay, flag.ptr, V.ptr, nvec, coef.ptr, W, part.ptr);
+            }
+            SBH_HIP(hipEventRecord(e1, ctx->stream));
+            SBH_HIP(hipEventSynchronize(e1));
+            const double ms = elapsed_ms(e0, e1) / 3.0;
+            record(0, 0, ce, ms);
+            if (ms < best_ms) {
+                best_ms = ms;
+                tb.chunk_elems = ce;
+            }
+        }
+        SBH_HIP(hipEventDestroy(e0));
+        SBH_HIP(hipEventDestroy(e1));
+        b->tuning.chunk_elems = tb.chunk_elems;
+        build_chunks(b);
+        if (num_entries) *num_entries = rows_out;
+        if (best) *best = b->tuning;
+    });
+}
+
+}  // extern "C"

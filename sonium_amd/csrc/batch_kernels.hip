@@ -1,0 +1,719 @@
+// Kernels of the batched solver: many independent instances (each an n x n
+// complex system with ncol <= 16 right-hand sides) advanced in lock step.
+//
+// Layout.  Instance b's vectors are n x ncol row-major blocks at vec_off of a
+// packed vector of length L = sum n*ncol; global column c = col_off + q.
+// Vector kernels run over chunks (instance, rows [r0, r0 + rows)) of about
+// 4096 elements; a thread owns local column t % ncol.  Per-column partial sums
+// go to part[(chunk * MV + slot) * 16 + q] and are added per column over the
+// instance's chunks in chunk order, so every reduction has a fixed order.
+//
+// Products.  Grouped tiles (instance, 128 rows): the FP64 product uses
+// v_mfma_f64_16x16x4_f64 and the FP16/BF16 product v_mfma_f32_32x32x16 with
+// FP32 accumulation; both use the real formulation of the complex product
+// (A as an n x 2n real matrix, the block operand's rows [Re, Im] and
+// [-Im, Re] formed while it is copied into LDS).
+//
+// Arnoldi state per column: Hessenberg column j at H[(c * m + j) * (m + 1)],
+// rotations cs/sn[c * m + i], g[c * (m + 1) + i].  The small per-column
+// work (Givens rotations, back substitution, restart) runs on the device, so
+// a whole restart cycle is a fixed sequence of launches that can be captured
+// in one graph.  Kernels skip instances whose flag is clear.
+
+#include "batch.hpp"
+
+namespace sbh {
+namespace {
+
+#include "lowp_common.hpp"
+
+constexpr int MAXCOL = 16;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ double2 cinv(double2 a) {
+    const double n = a.x * a.x + a.y * a.y;
+    return make_double2(a.x / n, -a.y / n);
+}
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+
+// ---- grouped products ----
+
+constexpr int GP_BM = 128;
+constexpr int GLP_ROW = 128 + 8;  // LDS row (halves) of the transposed block operand
+
+// Y_b = diag(row_inv) (A_lp,b X_b) on the instance's 128-row tile.  X holds
+// Krylov basis vectors (unit 2-norm per column), so its entries are converted
+// to the format without a scale; the row-scaled A keeps |A_ij| < 2^15.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void glp_gemm_kernel(const int2* __restrict__ tiles,
+                                                         const BatchInstance* __restrict__ inst,
+                                                         const int* __restrict__ flag, const u16* __restrict__ Alp,
+                                                         const double* __restrict__ row_inv,
+                                                         const double2* __restrict__ X, double2* __restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) u16 Bs[2 * MAXCOL * GLP_ROW];
+    const int2 tile = tiles[blockIdx.x];
+    if (!flag[tile.x]) return;  // uniform over the block
+    const BatchInstance& I = inst[tile.x];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t n = I.n, ld = I.ld;
+    const int kb = I.ncol;
+    const int64_t row_base = (int64_t)tile.y + wave * 32;
+    const int64_t arow = min(row_base + (lane & 31), n - 1);
+    const int kq = (lane >> 5) * 8;
+    const u32x4* ap = reinterpret_cast<const u32x4*>(Alp + I.lp_off + arow * 2 * ld + kq);
+    const double2* Xb = X + I.vec_off;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    for (int64_t kc = 0; kc < 2 * ld; kc += 128) {
+        // complex rows j0 .. j0+63 of X_b -> real k rows kc .. kc+127 of the operand,
+        // stored per real output column: column 2c is Re, 2c+1 is Im of column c.
+        const int64_t j0 = kc / 2;
+        for (int e = threadIdx.x; e < 64 * MAXCOL; e += BLOCK) {
+            const int jl = e / MAXCOL, c = e % MAXCOL;
+            const int64_t j = j0 + jl;
+            const double2 v = (j < n && c < kb) ? Xb[j * kb + c] : make_double2(0.0, 0.0);
+            const unsigned xr = Half<FMT>::from_float((float)v.x);
+            const unsigned xi = Half<FMT>::from_float((float)v.y);
+            const unsigned nxi = Half<FMT>::from_float((float)-v.y);
+            *reinterpret_cast<unsigned*>(&Bs[(2 * c) * GLP_ROW + 2 * jl]) = xr | (nxi << 16);
+            *reinterpret_cast<unsigned*>(&Bs[(2 * c + 1) * GLP_ROW + 2 * jl]) = xi | (xr << 16);
+        }
+        u32x4 a[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a[s] = __builtin_nontemporal_load(ap + (kc + 16 * s) / 8);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[(lane & 31) * GLP_ROW + s * 16 + kq]);
+            acc = mfma32<FMT>(a[s], b, acc);
+        }
+        __syncthreads();
+    }
+    // accumulator layout: column lane % 32, row 8*(v/4) + 4*(lane/32) + v%4
+    const int col = lane & 31;
+    const int c = col >> 1;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const float mine = acc[v];
+        const float other = __shfl_xor(mine, 1, 64);
+        const int64_t row = row_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if ((col & 1) == 0 && c < kb && row < n) {
+            const double s = row_inv[I.panel_off + row];
+            Y[I.vec_off + row * kb + c] = make_double2((double)mine * s, (double)other * s);
+        }
+    }
+}
+
+// FP64: Y_b = A_b X_b on the instance's 128-row tile, 16 complex columns
+// (32 real) per tile; wave w owns rows 32w..32w+31 as 2 x 2 tiles of 16 x 16.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int G64_BKC = 8;           // complex columns of A per stage
+constexpr int G64_AROW = GP_BM + 2;  // LDS row of the transposed A stage
+
+__global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict__ tiles,
+                                                         const BatchInstance* __restrict__ inst,
+                                                         const int* __restrict__ flag, const double2* __restrict__ A,
+                                                         const double2* __restrict__ X, double2* __restrict__ Y) {
+    __shared__ double As[2 * G64_BKC][G64_AROW];
+    __shared__ __attribute__((aligned(16))) double Xs[2 * G64_BKC][2 * MAXCOL];
+    const int2 tile = tiles[blockIdx.x];
+    if (!flag[tile.x]) return;
+    const BatchInstance& I = inst[tile.x];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int64_t n = I.n;
+    const int kb = I.ncol;
+    const int64_t row0 = tile.y;
+    const double2* Ab = A + I.a_off;
+    const double2* Xb = X + I.vec_off;
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = 0; k0 < n; k0 += G64_BKC) {
+#pragma unroll
+        for (int q = 0; q < (GP_BM * G64_BKC) / BLOCK; ++q) {
+            const int e = t + q * BLOCK;
+            const int r = e / G64_BKC, cc = e % G64_BKC;
+            const int64_t gr = row0 + r, gk = k0 + cc;
+            const double2 v = (gr < n && gk < n) ? Ab[gr * n + gk] : make_double2(0.0, 0.0);
+            As[2 * cc][r] = v.x;
+            As[2 * cc + 1][r] = v.y;
+        }
+        if (t < G64_BKC * MAXCOL) {
+            const int j = t / MAXCOL, c = t % MAXCOL;
+            const int64_t gk = k0 + j;
+            const double2 v = (gk < n && c < kb) ? Xb[gk * kb + c] : make_double2(0.0, 0.0);
+            *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = v;
+            *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-v.y, v.x);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < (2 * G64_BKC) / 4; ++ks) {
+            const int kr = 4 * ks + (lane >> 4);
+            double av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = As[kr][32 * w + 16 * a + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = Xs[kr][16 * b + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // 16 x 16 tile: lane l, register v -> row 4*(l/16) + v, real column l % 16
+    const int col = lane & 15;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t gr = row0 + 32 * w + 16 * a + 4 * (lane >> 4) + v;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const double mine = acc[a][b][v];
+                const double other = __shfl_xor(mine, 1, 64);
+                const int c = (16 * b + col) >> 1;
+                if ((col & 1) == 0 && c < kb && gr < n) Y[I.vec_off + gr * kb + c] = make_double2(mine, other);
+            }
+        }
+}
+
+// Row-scaled FP16/BF16 copy of every instance matrix: one block per global row.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void glp_convert_kernel(const int* __restrict__ row_inst,
+                                                            const BatchInstance* __restrict__ inst,
+                                                            const double2* __restrict__ A, u16* __restrict__ Alp,
+                                                            double* __restrict__ row_inv) {
+    const int64_t g = blockIdx.x;
+    const BatchInstance& I = inst[row_inst[g]];
+    const int64_t r = g - I.panel_off;
+    convert_row<FMT>(A + I.a_off + r * I.n, I.n, I.ld, Alp + I.lp_off + r * 2 * I.ld, row_inv + g);
+}
+
+// ---- ragged vector kernels ----
+
+// slot s < nvec: sum conj(V_s) W per column; slot nvec: ||W||^2.
+__global__ __launch_bounds__(BLOCK) void rdots_kernel(const VecChunk* __restrict__ chunks,
+                                                      const BatchInstance* __restrict__ inst,
+                                                      const int* __restrict__ flag, const double2* __restrict__ V,
+                                                      int64_t L, int nvec, const double2* __restrict__ W,
+                                                      double2* __restrict__ part, int MV) {
+    __shared__ double2 sh[BLOCK];
+    const VecChunk ch = chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    const BatchInstance& I = inst[ch.inst];
+    const int kb = I.ncol;
+    const int S = BLOCK / kb;
+    const int c = threadIdx.x % kb;
+    const int sub = threadIdx.x / kb;
+    const int slot = blockIdx.y;
+    const int64_t base = I.vec_off + ch.r0 * kb + c;
+    double2 s = make_double2(0.0, 0.0);
+    if (sub < S) {
+        if (slot < nvec) {
+            const double2* v = V + (int64_t)slot * L;
+            for (int r = sub; r < ch.rows; r += S) {
+                const double2 p = cmulc(v[base + (int64_t)r * kb], W[base + (int64_t)r * kb]);
+                s.x += p.x;
+                s.y += p.y;
+            }
+        } else {
+            for (int r = sub; r < ch.rows; r += S) {
+                const double2 q = W[base + (int64_t)r * kb];
+                s.x += q.x * q.x + q.y * q.y;
+            }
+        }
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < kb) {
+        double2 tsum = make_double2(0.0, 0.0);
+        for (int q = 0; q < S; ++q) {
+            const double2 p = sh[q * kb + threadIdx.x];
+            tsum.x += p.x;
+            tsum.y += p.y;
+        }
+        part[((int64_t)blockIdx.x * MV + slot) * MAXCOL + threadIdx.x] = tsum;
+    }
+}
+
+// Column sums of the dot partials (thread per (column, slot)).  pass 0:
+// coef = h(:, j) = the sums, wn0 = ||W|| before projection.  pass 1 (second
+// Gram–Schmidt pass): only columns flagged for reorthogonalisation take the
+// sums (added to h); the others get coef = 0.
+__global__ void rsum_dots_kernel(int C, const int* __restrict__ col_inst, const int* __restrict__ col_q,
+                                 const int2* __restrict__ chunk_range, const double2* __restrict__ part, int MV,
+                                 int nvec, int j, int m, const int* __restrict__ active,
+                                 const int* __restrict__ reorth, int pass, double2* __restrict__ H,
+                                 double2* __restrict__ coef, double* __restrict__ wn0) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)C * (nvec + 1)) return;
+    const int c = (int)(t / (nvec + 1));
+    const int slot = (int)(t % (nvec + 1));
+    const bool take = pass == 0 ? active[c] != 0 : reorth[c] != 0;
+    double2 s = make_double2(0.0, 0.0);
+    if (take) {
+        const int2 r = chunk_range[col_inst[c]];
+        const int q = col_q[c];
+        for (int ch = r.x; ch < r.y; ++ch) {
+            const double2 p = part[((int64_t)ch * MV + slot) * MAXCOL + q];
+            s.x += p.x;
+            s.y += p.y;
+        }
+    }
+    if (slot < nvec) {
+        coef[(int64_t)c * MV + slot] = s;
+        if (take) {
+            double2& h = H[((int64_t)c * m + j) * (m + 1) + slot];
+            h = pass == 0 ? s : cadd(h, s);
+        }
+    } else if (pass == 0 && take) {
+        wn0[c] = sqrt(s.x);
+    }
+}
+
+// W -= sum_{s < nvec} coef[c][s] V_s per column; ||W||^2 partials into slot 0.
+__global__ __launch_bounds__(BLOCK) void rproject_kernel(const VecChunk* __restrict__ chunks,
+                                                         const BatchInstance* __restrict__ inst,
+                                                         const int* __restrict__ flag, const double2* __restrict__ V,
+                                                         int64_t L, int nvec, const double2* __restrict__ coef, int MV,
+                                                         double2* __restrict__ W, double2* __restrict__ part) {
+    __shared__ double sh[BLOCK];
+    const VecChunk ch = chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    const BatchInstance& I = inst[ch.inst];
+    const int kb = I.ncol;
+    const int S = BLOCK / kb;
+    const int c = threadIdx.x % kb;
+    const int sub = threadIdx.x / kb;
+    const int64_t base = I.vec_off + ch.r0 * kb + c;
+    const double2* cf = coef + (int64_t)(I.col_off + c) * MV;
+    double nr = 0.0;
+    if (sub < S) {
+        for (int r = sub; r < ch.rows; r += S) {
+            const int64_t idx = base + (int64_t)r * kb;
+            double sr = 0.0, si = 0.0;
+            for (int s = 0; s < nvec; ++s) {
+                const double2 p = cmul(cf[s], V[(int64_t)s * L + idx]);
+                sr += p.x;
+                si += p.y;
+            }
+            double2 w = W[idx];
+            w.x -= sr;
+            w.y -= si;
+            W[idx] = w;
+            nr += w.x * w.x + w.y * w.y;
+        }
+    }
+    sh[threadIdx.x] = nr;
+    __syncthreads();
+    if (threadIdx.x < kb) {
+        double tsum = 0.0;
+        for (int q = 0; q < S; ++q) tsum += sh[q * kb + threadIdx.x];
+        part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(tsum, 0.0);
+    }
+}
+
+// ||W|| per column from slot 0 of the partials; pass 0 also flags the
+// columns whose norm dropped below 0.707 of the norm before projection
+// (the reorthogonalisation test of the single-system GMRES).
+__global__ void rnorm_kernel(int C, const int* __restrict__ col_inst, const int* __restrict__ col_q,
+                             const int2* __restrict__ chunk_range, const double2* __restrict__ part, int MV,
+                             const int* __restrict__ active, const double* __restrict__ wn0, int pass,
+                             double* __restrict__ wn, int* __restrict__ reorth, int64_t* __restrict__ reorth_count) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    if (!active[c]) {
+        if (pass == 0) reorth[c] = 0;
+        return;
+    }
+    const int2 r = chunk_range[col_inst[c]];
+    const int q = col_q[c];
+    double s = 0.0;
+    for (int ch = r.x; ch < r.y; ++ch) s += part[((int64_t)ch * MV) * MAXCOL + q].x;
+    const double v = sqrt(s);
+    wn[c] = v;
+    if (pass == 0) {
+        const int f = v < 0.707 * wn0[c] ? 1 : 0;
+        reorth[c] = f;
+        reorth_count[c] += f;
+    }
+}
+
+// Instance flags: any column active / flagged / not yet converged.
+__global__ void rflags_kernel(int B, const BatchInstance* __restrict__ inst, const int* __restrict__ active,
+                              const int* __restrict__ reorth, const int* __restrict__ done, int* __restrict__ inst_active,
+                              int* __restrict__ inst_reorth, int* __restrict__ inst_pending) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const BatchInstance& I = inst[b];
+    int a = 0, r = 0, p = 0;
+    for (int q = 0; q < I.ncol; ++q) {
+        a |= active[I.col_off + q];
+        r |= reorth[I.col_off + q] & active[I.col_off + q];
+        p |= !done[I.col_off + q];
+    }
+    inst_active[b] = a;
+    inst_reorth[b] = r;
+    inst_pending[b] = p;
+}
+
+// Step j of the per-column Arnoldi recurrence after the projection: store
+// h(j+1, j) = ||w||, apply the previous rotations to column j, form and apply
+// the new rotation (gmres_common.rs:91-105), update g and the residual
+// estimate.  A column freezes (jlen = j + 1) when the estimate drops below
+// its threshold, when ||w|| < 1e-20 (happy breakdown) or at the end of the
+// cycle.  scale = 1/||w|| for the next basis vector (0 once frozen).
+__global__ void rgivens_kernel(int C, int j, int m, int* __restrict__ active, const double* __restrict__ wn,
+                               double2* __restrict__ H, double2* __restrict__ cs, double2* __restrict__ sn,
+                               double2* __restrict__ g, const double* __restrict__ tol_abs, double* __restrict__ est,
+                               int* __restrict__ jlen, int64_t* __restrict__ iters, double* __restrict__ scale) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    if (!active[c]) {
+        scale[c] = 0.0;
+        return;
+    }
+    double2* h = H + ((int64_t)c * m + j) * (m + 1);
+    double2* C_ = cs + (int64_t)c * m;
+    double2* S_ = sn + (int64_t)c * m;
+    double2* gc = g + (int64_t)c * (m + 1);
+    const double w = wn[c];
+    h[j + 1] = make_double2(w, 0.0);
+    const bool breakdown = w < 1e-20;
+    for (int i = 0; i < j; ++i) {
+        const double2 temp = cadd(cmul(conj2(C_[i]), h[i]), cmul(conj2(S_[i]), h[i + 1]));
+        h[i + 1] = cadd(csub(make_double2(0.0, 0.0), cmul(S_[i], h[i])), cmul(C_[i], h[i + 1]));
+        h[i] = temp;
+    }
+    const double2 a = h[j], bb = h[j + 1];
+    double2 c_, s_;
+    if (cabs2(bb) < 1e-30) {
+        c_ = make_double2(1.0, 0.0);
+        s_ = make_double2(0.0, 0.0);
+    } else if (cabs2(a) < 1e-30) {
+        c_ = make_double2(0.0, 0.0);
+        s_ = make_double2(1.0, 0.0);
+    } else {
+        const double r = sqrt((a.x * a.x + a.y * a.y) + (bb.x * bb.x + bb.y * bb.y));
+        c_ = make_double2(a.x * (1.0 / r), a.y * (1.0 / r));
+        s_ = make_double2(bb.x * (1.0 / r), bb.y * (1.0 / r));
+    }
+    C_[j] = c_;
+    S_[j] = s_;
+    h[j] = cadd(cmul(conj2(c_), h[j]), cmul(conj2(s_), h[j + 1]));
+    h[j + 1] = make_double2(0.0, 0.0);
+    const double2 temp = cadd(cmul(conj2(c_), gc[j]), cmul(conj2(s_), gc[j + 1]));
+    gc[j + 1] = cadd(csub(make_double2(0.0, 0.0), cmul(s_, gc[j])), cmul(c_, gc[j + 1]));
+    gc[j] = temp;
+    const double e = cabs2(gc[j + 1]);
+    est[c] = e;
+    iters[c] += 1;
+    const bool stop = e < tol_abs[c] || (!breakdown && e < 1e-20) || breakdown || j + 1 == m;
+    if (stop) {
+        active[c] = 0;
+        jlen[c] = j + 1;
+        scale[c] = 0.0;
+    } else {
+        scale[c] = 1.0 / w;
+    }
+}
+
+// Back substitution y = R^{-1} g over the column's jlen steps (gmres.rs
+// solve_upper_triangular order); y goes to coef[c][0..jlen), zeros after.
+__global__ void rsolve_kernel(int C, int m, int MV, const double2* __restrict__ H, const double2* __restrict__ g,
+                              const int* __restrict__ jlen, double2* __restrict__ coef) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const int kk = jlen[c];
+    double2* y = coef + (int64_t)c * MV;
+    for (int i = 0; i < MV; ++i) y[i] = make_double2(0.0, 0.0);
+    const double2* gc = g + (int64_t)c * (m + 1);
+    for (int i = kk - 1; i >= 0; --i) {
+        double2 sum = gc[i];
+        for (int jj = i + 1; jj < kk; ++jj) sum = csub(sum, cmul(H[((int64_t)c * m + jj) * (m + 1) + i], y[jj]));
+        const double2 d = H[((int64_t)c * m + i) * (m + 1) + i];
+        if (cabs2(d) > 1e-30) y[i] = cmul(sum, cinv(d));
+    }
+}
+
+// X += sum_{s < jlen} coef[c][s] V_s for the columns of pending instances.
+__global__ __launch_bounds__(BLOCK) void rupdate_kernel(const VecChunk* __restrict__ chunks,
+                                                        const BatchInstance* __restrict__ inst,
+                                                        const int* __restrict__ flag, const double2* __restrict__ V,
+                                                        int64_t L, const int* __restrict__ jlen,
+                                                        const double2* __restrict__ coef, int MV,
+                                                        double2* __restrict__ X) {
+    const VecChunk ch = chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    const BatchInstance& I = inst[ch.inst];
+    const int kb = I.ncol;
+    const int S = BLOCK / kb;
+    const int c = threadIdx.x % kb;
+    const int sub = threadIdx.x / kb;
+    if (sub >= S) return;
+    const int nv = jlen[I.col_off + c];
+    const double2* cf = coef + (int64_t)(I.col_off + c) * MV;
+    const int64_t base = I.vec_off + ch.r0 * kb + c;
+    for (int r = sub; r < ch.rows; r += S) {
+        const int64_t idx = base + (int64_t)r * kb;
+        double2 x = X[idx];
+        for (int s = 0; s < nv; ++s) {
+            const double2 p = cmul(cf[s], V[(int64_t)s * L + idx]);
+            x.x += p.x;
+            x.y += p.y;
+        }
+        X[idx] = x;
+    }
+}
+
+// R = B - R per column (R holds A X on entry); ||R||^2 partials into slot 0.
+__global__ __launch_bounds__(BLOCK) void rresid_kernel(const VecChunk* __restrict__ chunks,
+                                                       const BatchInstance* __restrict__ inst,
+                                                       const int* __restrict__ flag, const double2* __restrict__ B,
+                                                       double2* __restrict__ R, double2* __restrict__ part, int MV) {
+    __shared__ double sh[BLOCK];
+    const VecChunk ch = chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    const BatchInstance& I = inst[ch.inst];
+    const int kb = I.ncol;
+    const int S = BLOCK / kb;
+    const int c = threadIdx.x % kb;
+    const int sub = threadIdx.x / kb;
+    const int64_t base = I.vec_off + ch.r0 * kb + c;
+    double nr = 0.0;
+    if (sub < S) {
+        for (int r = sub; r < ch.rows; r += S) {
+            const int64_t idx = base + (int64_t)r * kb;
+            const double2 b = B[idx];
+            const double2 ax = R[idx];
+            const double2 v = make_double2(b.x - ax.x, b.y - ax.y);
+            R[idx] = v;
+            nr += v.x * v.x + v.y * v.y;
+        }
+    }
+    sh[threadIdx.x] = nr;
+    __syncthreads();
+    if (threadIdx.x < kb) {
+        double tsum = 0.0;
+        for (int q = 0; q < S; ++q) tsum += sh[q * kb + threadIdx.x];
+        part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(tsum, 0.0);
+    }
+}
+
+// After a residual: ||r|| per column, convergence test ||r|| / ||b|| < tol
+// (with init, ||b|| is this norm), completion time stamp, and the start of
+// the next cycle: g = (||r||, 0, ...), scale = 1/||r||, threshold of the
+// Arnoldi estimate = inner_tol * ||r|| (refinement) or tol * ||b|| (plain).
+__global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, const int* __restrict__ col_q,
+                                const int2* __restrict__ chunk_range, const double2* __restrict__ part, int MV,
+                                const int* __restrict__ pending_in, int init, double tol, double inner_tol,
+                                int refinement, double* __restrict__ bnorm, double* __restrict__ rnorm,
+                                int* __restrict__ done, int* __restrict__ active, int* __restrict__ jlen,
+                                double2* __restrict__ g, double* __restrict__ scale, double* __restrict__ tol_abs,
+                                int64_t* __restrict__ cycles, unsigned long long* __restrict__ t_done) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    jlen[c] = 0;
+    if (done[c] || !pending_in[col_inst[c]]) {
+        active[c] = 0;
+        scale[c] = 0.0;
+        return;
+    }
+    const int2 r = chunk_range[col_inst[c]];
+    const int q = col_q[c];
+    double s = 0.0;
+    for (int ch = r.x; ch < r.y; ++ch) s += part[((int64_t)ch * MV) * MAXCOL + q].x;
+    const double rn = sqrt(s);
+    if (init) bnorm[c] = rn;
+    rnorm[c] = rn;
+    const double bn = bnorm[c];
+    const bool conv = bn == 0.0 || rn / bn < tol;
+    if (conv || !(rn > 0.0) || !isfinite(rn)) {
+        done[c] = 1;
+        active[c] = 0;
+        scale[c] = 0.0;
+        t_done[c] = wall_clock64();
+        return;
+    }
+    if (!init) cycles[c] += 1;
+    double2* gc = g + (int64_t)c * (m + 1);
+    gc[0] = make_double2(rn, 0.0);
+    for (int i = 1; i <= m; ++i) gc[i] = make_double2(0.0, 0.0);
+    scale[c] = 1.0 / rn;
+    tol_abs[c] = refinement ? inner_tol * rn : tol * bn;
+    active[c] = 1;
+}
+
+// dst = scale[c] * src per column (instances with the flag set).
+__global__ __launch_bounds__(BLOCK) void rscale_kernel(const VecChunk* __restrict__ chunks,
+                                                       const BatchInstance* __restrict__ inst,
+                                                       const int* __restrict__ flag, const double* __restrict__ scale,
+                                                       const double2* __restrict__ src, double2* __restrict__ dst) {
+    const VecChunk ch = chunks[blockIdx.x];
+    if (!flag[ch.inst]) return;
+    const BatchInstance& I = inst[ch.inst];
+    const int kb = I.ncol;
+    const int S = BLOCK / kb;
+    const int c = threadIdx.x % kb;
+    const int sub = threadIdx.x / kb;
+    if (sub >= S) return;
+    const double sc = scale[I.col_off + c];
+    const int64_t base = I.vec_off + ch.r0 * kb + c;
+    for (int r = sub; r < ch.rows; r += S) {
+        const int64_t idx = base + (int64_t)r * kb;
+        const double2 v = src[idx];
+        dst[idx] = make_double2(v.x * sc, v.y * sc);
+    }
+}
+
+__global__ void rfill_kernel(double2* __restrict__ p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = make_double2(0.0, 0.0);
+}
+
+__global__ void rstamp_kernel(unsigned long long* __restrict__ out) { *out = wall_clock64(); }
+
+inline unsigned nb(int64_t n, int per = BLOCK) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+// ---- launchers (declared in batch.hpp) ----
+
+void batch_lp_convert(sbh_context* ctx, int format, int64_t rows, const int* row_inst, const BatchInstance* inst,
+                      const double2* A, uint16_t* Alp, double* row_inv) {
+    if (rows == 0) return;
+    if (format == 0)
+        hipLaunchKernelGGL(glp_convert_kernel<0>, dim3((unsigned)rows), dim3(BLOCK), 0, ctx->stream, row_inst, inst, A,
+                           Alp, row_inv);
+    else
+        hipLaunchKernelGGL(glp_convert_kernel<1>, dim3((unsigned)rows), dim3(BLOCK), 0, ctx->stream, row_inst, inst, A,
+                           Alp, row_inv);
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_product(sbh_context* ctx, int format, int ntiles, const int2* tiles, const BatchInstance* inst,
+                   const int* flag, const double2* A, const uint16_t* Alp, const double* row_inv, const double2* X,
+                   double2* Y) {
+    if (ntiles == 0) return;
+    if (format < 0)
+        hipLaunchKernelGGL(g64_gemm_kernel, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag, A,
+                           X, Y);
+    else if (format == 0)
+        hipLaunchKernelGGL(glp_gemm_kernel<0>, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag,
+                           Alp, row_inv, X, Y);
+    else
+        hipLaunchKernelGGL(glp_gemm_kernel<1>, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag,
+                           Alp, row_inv, X, Y);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_dots(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* V, int nvec,
+                 const double2* W, double2* part) {
+    hipLaunchKernelGGL(rdots_kernel, dim3((unsigned)lay.nchunks, (unsigned)(nvec + 1)), dim3(BLOCK), 0, ctx->stream,
+                       lay.chunks, lay.inst, flag, V, lay.L, nvec, W, part, lay.MV);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_sum_dots(sbh_context* ctx, const RaggedLayout& lay, const double2* part, int nvec, int j,
+                     const int* active, const int* reorth, int pass, double2* H, double2* coef, double* wn0) {
+    hipLaunchKernelGGL(rsum_dots_kernel, dim3(nb((int64_t)lay.C * (nvec + 1))), dim3(BLOCK), 0, ctx->stream, lay.C,
+                       lay.col_inst, lay.col_q, lay.chunk_range, part, lay.MV, nvec, j, lay.m, active, reorth, pass, H,
+                       coef, wn0);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_project(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* V, int nvec,
+                    const double2* coef, double2* W, double2* part) {
+    hipLaunchKernelGGL(rproject_kernel, dim3((unsigned)lay.nchunks), dim3(BLOCK), 0, ctx->stream, lay.chunks, lay.inst,
+                       flag, V, lay.L, nvec, coef, lay.MV, W, part);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_norm(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* active, const double* wn0,
+                 int pass, double* wn, int* reorth, int64_t* reorth_count) {
+    hipLaunchKernelGGL(rnorm_kernel, dim3(nb(lay.C)), dim3(BLOCK), 0, ctx->stream, lay.C, lay.col_inst, lay.col_q,
+                       lay.chunk_range, part, lay.MV, active, wn0, pass, wn, reorth, reorth_count);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_flags(sbh_context* ctx, const RaggedLayout& lay, const int* active, const int* reorth, const int* done,
+                  int* inst_active, int* inst_reorth, int* inst_pending) {
+    hipLaunchKernelGGL(rflags_kernel, dim3(nb(lay.B)), dim3(BLOCK), 0, ctx->stream, lay.B, lay.inst, active, reorth,
+                       done, inst_active, inst_reorth, inst_pending);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_givens(sbh_context* ctx, const RaggedLayout& lay, int j, int* active, const double* wn, double2* H,
+                   double2* cs, double2* sn, double2* g, const double* tol_abs, double* est, int* jlen, int64_t* iters,
+                   double* scale) {
+    hipLaunchKernelGGL(rgivens_kernel, dim3(nb(lay.C)), dim3(BLOCK), 0, ctx->stream, lay.C, j, lay.m, active, wn, H,
+                       cs, sn, g, tol_abs, est, jlen, iters, scale);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_solve(sbh_context* ctx, const RaggedLayout& lay, const double2* H, const double2* g, const int* jlen,
+                  double2* coef) {
+    hipLaunchKernelGGL(rsolve_kernel, dim3(nb(lay.C, 64)), dim3(64), 0, ctx->stream, lay.C, lay.m, lay.MV, H, g, jlen,
+                       coef);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_update(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* V, const int* jlen,
+                   const double2* coef, double2* X) {
+    hipLaunchKernelGGL(rupdate_kernel, dim3((unsigned)lay.nchunks), dim3(BLOCK), 0, ctx->stream, lay.chunks, lay.inst,
+                       flag, V, lay.L, jlen, coef, lay.MV, X);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_residual(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* B, double2* R,
+                     double2* part) {
+    hipLaunchKernelGGL(rresid_kernel, dim3((unsigned)lay.nchunks), dim3(BLOCK), 0, ctx->stream, lay.chunks, lay.inst,
+                       flag, B, R, part, lay.MV);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_restart(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* pending_in, int init,
+                    double tol, double inner_tol, int refinement, double* bnorm, double* rnorm, int* done, int* active,
+                    int* jlen, double2* g, double* scale, double* tol_abs, int64_t* cycles,
+                    unsigned long long* t_done) {
+    hipLaunchKernelGGL(rrestart_kernel, dim3(nb(lay.C)), dim3(BLOCK), 0, ctx->stream, lay.C, lay.m, lay.col_inst,
+                       lay.col_q, lay.chunk_range, part, lay.MV, pending_in, init, tol, inner_tol, refinement, bnorm,
+                       rnorm, done, active, jlen, g, scale, tol_abs, cycles, t_done);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_scale(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double* scale, const double2* src,
+                  double2* dst) {
+    hipLaunchKernelGGL(rscale_kernel, dim3((unsigned)lay.nchunks), dim3(BLOCK), 0, ctx->stream, lay.chunks, lay.inst,
+                       flag, scale, src, dst);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_fill(sbh_context* ctx, double2* p, int64_t n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(rfill_kernel, dim3(nb(n)), dim3(BLOCK), 0, ctx->stream, p, n);
+    SBH_HIP(hipGetLastError());
+}
+
+void ragged_stamp(sbh_context* ctx, unsigned long long* out) {
+    hipLaunchKernelGGL(rstamp_kernel, dim3(1), dim3(1), 0, ctx->stream, out);
+    SBH_HIP(hipGetLastError());
+}
+
+}  // namespace sbh

@@ -21,6 +21,24 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& msg);
 
+// Runs f, turning exceptions into a status code and the thread's last error.
+template <typename F>
+sbh_status guard(F&& f) {
+    try {
+        f();
+        return SBH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return SBH_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SBH_ERR_BACKEND;
+    }
+}
+
 #define SBH_HIP(expr)                                                                         \
     do {                                                                                      \
         hipError_t sbh_e_ = (expr);                                                           \
@@ -131,6 +149,28 @@ struct NearPair {
     int32_t count;
     int32_t pad;
     int64_t begin;
+};
+
+// One instance of a batch of independent rigid scatterers.  Panel arrays of
+// all instances are concatenated (instance panels at panel_off); its FP64
+// matrix is n x n row-major at a_off; its right-hand sides and solutions are
+// n x ncol row-major blocks at vec_off of the packed vectors; its complex
+// FP16/BF16 copy has rows of ld complex entries at lp_off halves.
+struct BatchInstance {
+    int64_t panel_off;
+    int64_t n;
+    int64_t a_off;
+    int64_t vec_off;
+    int64_t lp_off;
+    int64_t ld;
+    int64_t dir_off;  // first direction (3 doubles each) in the direction array
+    int32_t ncol;
+    int32_t col_off;  // first global column
+    double k;
+    double tau;
+    double sign;
+    double beta[2];
+    double pad2;
 };
 
 // A subdivision leaf is stored as its path from the root triangle: bits
@@ -265,6 +305,27 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh
 void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,
                          sbh_c64 beta, double sign, const double* d_dirs, const double2* d_amps,
                          int64_t num_rhs, double2* d_rhs, bool accumulate);
+
+// Persistent assembly of a batch (tbem_kernels.hip): items are (instance,
+// first row) pairs; each writes the far entries, near pairs, diagonal and
+// plane-wave right-hand sides of rows_per_item rows.  counter holds 2 words.
+struct BatchAssemblyArgs {
+    const BatchInstance* inst;
+    const int2* items;
+    int nitems;
+    int rows_per_item;
+    unsigned* counter;
+    const FarPanel* far;
+    const Source* src;
+    const PanelGeom* geom;
+    const NearPair* pairs;
+    const int64_t* row_ptr;  // global rows -> global pair index
+    const uint64_t* leaves;
+    const double* dirs;
+    double2* A;
+    double2* rhs;
+};
+void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int grid);
 
 // Dense BLAS launchers (blas_kernels.hip).
 void zgemv_n(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols,

@@ -1,0 +1,85 @@
+// Complex FP16/BF16 storage helpers shared by the single-system and batched
+// low-precision kernels (device code, included inside namespace sbh::{anon}).
+//
+// Row scaling: row i is stored as A_ij * 2^(T - e_i) with 2^e_i <= max_j
+// max(|Re A_ij|, |Im A_ij|) < 2^(e_i + 1); T = 14 for FP16 (the largest scaled
+// entry stays below 2^15) and 0 for BF16 (FP32 exponent range).  row_inv
+// holds 2^(e_i - T), which undoes the scaling exactly.
+#pragma once
+
+constexpr int BLOCK = 256;
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+template <int FMT>
+struct Half;
+template <>
+struct Half<0> {
+    static constexpr int TOP = 14;
+    __device__ static u16 from_float(float f) {
+        _Float16 h = (_Float16)f;
+        return __builtin_bit_cast(u16, h);
+    }
+    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(_Float16, v); }
+};
+template <>
+struct Half<1> {
+    static constexpr int TOP = 0;
+    __device__ static u16 from_float(float f) {
+        __bf16 h = (__bf16)f;
+        return __builtin_bit_cast(u16, h);
+    }
+    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(__bf16, v); }
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// One block of BLOCK threads converts one row: row maximum, power-of-two
+// scale, rounding of the scaled FP64 values through FP32 to the format, zero
+// padding up to ld complex entries.
+template <int FMT>
+__device__ __forceinline__ void convert_row(const double2* __restrict__ row, int64_t n, int64_t ld,
+                                            u16* __restrict__ o, double* __restrict__ inv_out) {
+    __shared__ double sh[BLOCK / 64];
+    double m = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += BLOCK) {
+        double2 v = row[j];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    int e = 0;
+    if (m > 0.0 && isfinite(m)) e = ilogb(m);
+    const double scale = ldexp(1.0, Half<FMT>::TOP - e);
+    if (threadIdx.x == 0) *inv_out = ldexp(1.0, e - Half<FMT>::TOP);
+    for (int64_t j = threadIdx.x; j < ld; j += BLOCK) {
+        u16 re = 0, im = 0;
+        if (j < n) {
+            double2 v = row[j];
+            re = Half<FMT>::from_float((float)(v.x * scale));
+            im = Half<FMT>::from_float((float)(v.y * scale));
+        }
+        o[2 * j] = re;
+        o[2 * j + 1] = im;
+    }
+}
+
+// v_mfma_f32_32x32x16_{f16,bf16}: A 32x16 (lane l: row l%32, k 8*(l/32)..+7),
+// B 16x32 (lane l: column l%32, the same k), C 32x32 FP32 in 16 registers.
+template <int FMT>
+__device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
+    if constexpr (FMT == 0)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}

@@ -30,78 +30,21 @@
 namespace sbh {
 namespace {
 
-constexpr int BLOCK = 256;
+#include "lowp_common.hpp"
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16;
-
-template <int FMT>
-struct Half;
-template <>
-struct Half<0> {
-    static constexpr int TOP = 14;
-    __device__ static u16 from_float(float f) {
-        _Float16 h = (_Float16)f;
-        return __builtin_bit_cast(u16, h);
-    }
-    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(_Float16, v); }
-};
-template <>
-struct Half<1> {
-    static constexpr int TOP = 0;
-    __device__ static u16 from_float(float f) {
-        __bf16 h = (__bf16)f;
-        return __builtin_bit_cast(u16, h);
-    }
-    __device__ static float to_float(u16 v) { return (float)__builtin_bit_cast(__bf16, v); }
-};
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
 __device__ __forceinline__ float wave_sumf(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
-// One block per row: row max, power-of-two scale, conversion.
+// One block per row.
 template <int FMT>
 __global__ __launch_bounds__(BLOCK) void lp_convert_kernel(const double2* __restrict__ A, int64_t lda,
                                                            int64_t n, int64_t ld, u16* __restrict__ out,
                                                            double* __restrict__ row_inv) {
-    __shared__ double sh[BLOCK / 64];
     const int64_t i = blockIdx.x;
-    const double2* row = A + i * lda;
-    double m = 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += BLOCK) {
-        double2 v = row[j];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
-    }
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-    __syncthreads();
-    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-    int e = 0;
-    if (m > 0.0 && isfinite(m)) e = ilogb(m);
-    const double scale = ldexp(1.0, Half<FMT>::TOP - e);
-    if (threadIdx.x == 0) row_inv[i] = ldexp(1.0, e - Half<FMT>::TOP);
-    u16* o = out + i * 2 * ld;
-    for (int64_t j = threadIdx.x; j < ld; j += BLOCK) {
-        u16 re = 0, im = 0;
-        if (j < n) {
-            double2 v = row[j];
-            re = Half<FMT>::from_float((float)(v.x * scale));
-            im = Half<FMT>::from_float((float)(v.y * scale));
-        }
-        o[2 * j] = re;
-        o[2 * j + 1] = im;
-    }
+    convert_row<FMT>(A + i * lda, n, ld, out + i * 2 * ld, row_inv + i);
 }
 
 // y = diag(row_inv) * (A_lp x), x given in FP32 (ld entries, zero padded).
@@ -226,14 +169,6 @@ __global__ void x_to_lp_kernel(const double2* __restrict__ X, int64_t n, int k, 
     im_row[2 * j + 1] = xr;
 }
 
-template <int FMT>
-__device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
-    if constexpr (FMT == 0)
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-}
-
 // Split-K MFMA product with FP32 accumulation.  Block = 4 waves x 32 rows
 // (128 rows) and one of LP_SPLIT K slabs; slab = blockIdx % LP_SPLIT, so on
 // the 8-XCD part every XCD streams one slab of the transposed block operand,
@@ -320,63 +255,111 @@ __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int
 
 // FP64 complex block product Y (n x k) = A (n x n, lda) X (n x k), k <= 64 per launch (column
 // offset handled by the caller).  Tile 128 rows x 64 columns; each thread 8 rows x 4 columns.
+// FP64 complex block product Y = A X (k <= 64 columns per launch) on
+// v_mfma_f64_16x16x4_f64, in the real formulation used by the low-precision
+// GEMM: the complex row-major A (n x lda) is a real n x 2n matrix, and the
+// real operand Xt (2n x 2kc) has rows
+//   Xt[2j]   = (Re X[j,0], Im X[j,0], Re X[j,1], ...)     (X's row j as stored)
+//   Xt[2j+1] = (-Im X[j,0], Re X[j,0], -Im X[j,1], ...)
+// so that Y[r] = A~[r] Xt comes out in complex interleaved order. Xt is never
+// stored: its rows are formed while X is copied into LDS.
+//
+// Block: 4 waves over BM = 128 rows x 16*NT real columns; wave w owns rows
+// 32w..32w+31 (2 x NT tiles of 16 x 16). Stages of 8 complex (16 real) columns
+// of A go through LDS, transposed so that MFMA operand reads are contiguous.
+// The K range is split in ZG_SPLIT slices (slice = blockIdx % ZG_SPLIT, one per
+// XCD); partials go to Ypart[slice][row][16*NT] and zgemm_reduce_kernel sums
+// them in slice order.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int ZG_SPLIT = 8;
+constexpr int ZG_BM = 128;
+constexpr int ZG_BKC = 8;            // complex columns of A per stage
+constexpr int ZG_AROW = ZG_BM + 2;   // LDS row of the transposed A stage (doubles)
+
+template <int NT>
 __global__ __launch_bounds__(BLOCK) void zgemm_f64_kernel(const double2* __restrict__ A, int64_t lda, int64_t n,
                                                           const double2* __restrict__ X, int64_t ldx, int kc,
-                                                          double2* __restrict__ Y, int64_t ldy) {
-    constexpr int BM = 128, BN = 64, BK = 16;
-    __shared__ double2 As[BK][BM + 1];
-    __shared__ double2 Xs[BK][BN];
+                                                          int64_t kslice, double* __restrict__ Ypart) {
+    constexpr int NR = 16 * NT;  // real output columns
+    __shared__ double As[2 * ZG_BKC][ZG_AROW];
+    __shared__ __attribute__((aligned(16))) double Xs[2 * ZG_BKC][NR];
     const int t = threadIdx.x;
-    const int cg = t & 15;   // columns cg*4 .. +3
-    const int rg = t >> 4;   // rows rg*8 .. +7
-    const int64_t row0 = (int64_t)blockIdx.x * BM;
-    double2 acc[8][4];
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int split = blockIdx.x % ZG_SPLIT;
+    const int64_t row0 = (int64_t)(blockIdx.x / ZG_SPLIT) * ZG_BM;
+    const int64_t kbeg = (int64_t)split * kslice;
+    const int64_t kend = min(n, kbeg + kslice);
+    f64x4 acc[2][NT];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-    for (int64_t k0 = 0; k0 < n; k0 += BK) {
-        // A tile: BM x BK (row-major in memory), stored transposed in LDS
-        for (int e = t; e < BM * BK; e += BLOCK) {
-            int r = e / BK, kk = e % BK;
-            int64_t gr = row0 + r, gk = k0 + kk;
-            As[kk][r] = (gr < n && gk < n) ? A[gr * lda + gk] : make_double2(0.0, 0.0);
+        for (int b = 0; b < NT; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = kbeg; k0 < kend; k0 += ZG_BKC) {
+        // A stage: 128 rows x 8 complex, 4 per thread; 8 consecutive threads read one row's 128 B
+#pragma unroll
+        for (int q = 0; q < (ZG_BM * ZG_BKC) / BLOCK; ++q) {
+            const int e = t + q * BLOCK;
+            const int r = e / ZG_BKC, cc = e % ZG_BKC;
+            const int64_t gr = row0 + r, gk = k0 + cc;
+            double2 v = (gr < n && gk < kend) ? A[gr * lda + gk] : make_double2(0.0, 0.0);
+            As[2 * cc][r] = v.x;
+            As[2 * cc + 1][r] = v.y;
         }
-        for (int e = t; e < BK * BN; e += BLOCK) {
-            int kk = e / BN, c = e % BN;
-            int64_t gk = k0 + kk;
-            Xs[kk][c] = (gk < n && c < kc) ? X[gk * ldx + c] : make_double2(0.0, 0.0);
+        // Xt stage: 8 complex rows of X -> 16 real rows of Xt
+        for (int e = t; e < ZG_BKC * 8 * NT; e += BLOCK) {
+            const int j = e / (8 * NT), c = e % (8 * NT);
+            const int64_t gk = k0 + j;
+            double2 v = (gk < kend && c < kc) ? X[gk * ldx + c] : make_double2(0.0, 0.0);
+            *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = v;
+            *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-v.y, v.x);
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < BK; ++kk) {
-            double2 xv[4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) xv[b] = Xs[kk][cg * 4 + b];
+        for (int ks = 0; ks < (2 * ZG_BKC) / 4; ++ks) {
+            const int kr = 4 * ks + (lane >> 4);
+            double av[2], bv[NT];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                double2 av = As[kk][rg * 8 + a];
+            for (int a = 0; a < 2; ++a) av[a] = As[kr][32 * w + 16 * a + (lane & 15)];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    acc[a][b].x = __builtin_fma(av.x, xv[b].x, acc[a][b].x);
-                    acc[a][b].x = __builtin_fma(-av.y, xv[b].y, acc[a][b].x);
-                    acc[a][b].y = __builtin_fma(av.x, xv[b].y, acc[a][b].y);
-                    acc[a][b].y = __builtin_fma(av.y, xv[b].x, acc[a][b].y);
-                }
+            for (int b = 0; b < NT; ++b) bv[b] = Xs[kr][16 * b + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout of the 16x16 tile: lane l, register v -> row 4*(l/16) + v, column l % 16
+    double* out = Ypart + (size_t)split * n * NR;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t gr = row0 + 32 * w + 16 * a + 4 * (lane >> 4) + v;
+            if (gr < n) {
+#pragma unroll
+                for (int b = 0; b < NT; ++b) out[gr * NR + 16 * b + (lane & 15)] = acc[a][b][v];
             }
         }
-        __syncthreads();
-    }
+}
+
+// Y[r, c] = sum over slices of the partials (complex columns c < kc), slice order fixed
+__global__ void zgemm_reduce_kernel(const double* __restrict__ Ypart, int64_t n, int nr, int kc,
+                                    double2* __restrict__ Y, int64_t ldy) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * kc) return;
+    const int64_t r = e / kc;
+    const int c = (int)(e % kc);
+    double sr = 0.0, si = 0.0;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        int64_t gr = row0 + rg * 8 + a;
-        if (gr >= n) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            int c = cg * 4 + b;
-            if (c < kc) Y[gr * ldy + c] = acc[a][b];
-        }
+    for (int s = 0; s < ZG_SPLIT; ++s) {
+        const double2 p = *reinterpret_cast<const double2*>(Ypart + ((size_t)s * n + r) * nr + 2 * c);
+        sr += p.x;
+        si += p.y;
     }
+    Y[r * ldy + c] = make_double2(sr, si);
 }
 
 // ---- column-batched Krylov vector kernels (n x k row-major blocks) ----
@@ -618,10 +601,26 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y) {
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_BLOCK, &ev);
+    const int64_t kslice = ((n + ZG_SPLIT - 1) / ZG_SPLIT + ZG_BKC - 1) / ZG_BKC * ZG_BKC;
+    const int64_t row_tiles = (n + ZG_BM - 1) / ZG_BM;
+    SBH_REQUIRE(row_tiles * ZG_SPLIT < ((int64_t)1 << 31), "block product too large");
     for (int64_t c0 = 0; c0 < k; c0 += 64) {
-        int kc = (int)std::min<int64_t>(64, k - c0);
-        hipLaunchKernelGGL(zgemm_f64_kernel, dim3(nblocks(n, 128)), dim3(BLOCK), 0, ctx->stream, A, lda, n, X + c0, k,
-                           kc, Y + c0, k);
+        const int kc = (int)std::min<int64_t>(64, k - c0);
+        const int nt = kc <= 8 ? 1 : kc <= 16 ? 2 : kc <= 32 ? 4 : 8;
+        const int nr = 16 * nt;
+        double* part = reinterpret_cast<double*>(ctx->device_scratch((size_t)ZG_SPLIT * n * nr / 2));
+        const dim3 grid((unsigned)(row_tiles * ZG_SPLIT));
+#define SBH_ZG(NT_)                                                                                           \
+    hipLaunchKernelGGL(zgemm_f64_kernel<NT_>, grid, dim3(BLOCK), 0, ctx->stream, A, lda, n, X + c0, k, kc, kslice, \
+                       part)
+        if (nt == 1) SBH_ZG(1);
+        else if (nt == 2) SBH_ZG(2);
+        else if (nt == 4) SBH_ZG(4);
+        else SBH_ZG(8);
+#undef SBH_ZG
+        SBH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(nblocks(n * kc)), dim3(BLOCK), 0, ctx->stream, part, n, nr, kc,
+                           Y + c0, k);
         SBH_HIP(hipGetLastError());
     }
     ctx->end_timed(SBH_KERNEL_BLOCK, ev);

@@ -375,6 +375,64 @@ __device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s,
 // Far kernel.  MODE 0: every field panel has a velocity BC without
 // admittance (rigid or vibrating body), so only H and E are needed and
 // A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
+// 13-point sums of one (collocation point, field panel) pair.  MODE 0 needs only
+// H and E; MODE 1 also G and Ht.
+template <int MODE>
+__device__ __forceinline__ void far_sums(const FarPanel& P, const Source& src, double k, double k2, double& hr,
+                                         double& hi, double& er, double& ei, double& gr, double& gi, double& tr,
+                                         double& ti) {
+    const double rel0 = P.v0[0] - src.c[0];
+    const double rel1 = P.v0[1] - src.c[1];
+    const double rel2 = P.v0[2] - src.c[2];
+    const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
+    hr = hi = er = ei = gr = gi = tr = ti = 0.0;
+    // TR13 constants are uniform: scalar loads from constant memory.
+#pragma unroll 2
+    for (int q = 0; q < 13; ++q) {
+        const double sq = kTrS[q], tq = kTrT[q];
+        const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
+        const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
+        const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+        // The diagonal is overwritten by the self-panel rule; keep it finite.
+        const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);
+        const double inv = rsqrt_nr(d2);
+        const double dis = d2 * inv;
+        const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;     // r^.n_y
+        const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;  // -r^.n_x
+        double sn, cs;
+        sincos_red(k * dis, &sn, &cs);
+        const double re2 = (kTrW[q] * P.wj) * inv;
+        const double zgr = cs * re2, zgi = sn * re2;
+        // zb = zg * (-1/r + i k)
+        const double zbr = -zgr * inv - zgi * k;
+        const double zbi = zgr * k - zgi * inv;
+        hr = __builtin_fma(zbr, pn, hr);
+        hi = __builtin_fma(zbi, pn, hi);
+        const double inv2 = inv * inv;
+        const double rq = pn * qn;
+        const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
+        const double efi = -k * inv * (3.0 * rq + nxny);
+        er += zgr * efr - zgi * efi;
+        ei += zgr * efi + zgi * efr;
+        if (MODE == 1) {
+            gr += zgr;
+            gi += zgi;
+            tr = __builtin_fma(zbr, qn, tr);
+            ti = __builtin_fma(zbi, qn, ti);
+        }
+    }
+}
+
+// Rigid/vibrating coefficient (H * gamma) * tau + E * beta with gamma = 1.
+__device__ __forceinline__ double2 velocity_coefficient(double hr, double hi, double er, double ei, double tau_r,
+                                                        double2 beta) {
+    return make_double2(hr * tau_r + (er * beta.x - ei * beta.y), hi * tau_r + (er * beta.y + ei * beta.x));
+}
+
+// ---------------------------------------------------------------------------
+// Far kernel.  MODE 0: every field panel has a velocity BC without
+// admittance (rigid or vibrating body), so only H and E are needed and
+// A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
 template <int MODE>
 __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ panels,
                                                   const Source* __restrict__ sources,
@@ -393,51 +451,11 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
     const int64_t i1 = min(n, i0 + rows_per_block);
     for (int64_t i = i0; i < i1; ++i) {
         const Source src = sources[i];
-        const double rel0 = P.v0[0] - src.c[0];
-        const double rel1 = P.v0[1] - src.c[1];
-        const double rel2 = P.v0[2] - src.c[2];
-        const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
-        double hr = 0.0, hi = 0.0, er = 0.0, ei = 0.0, gr = 0.0, gi = 0.0, tr = 0.0, ti = 0.0;
-        // TR13 constants are uniform: scalar loads from constant memory.
-#pragma unroll 2
-        for (int q = 0; q < 13; ++q) {
-            const double sq = kTrS[q], tq = kTrT[q];
-            const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
-            const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
-            const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
-            // The diagonal is overwritten by self_kernel; keep it finite.
-            const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);
-            const double inv = rsqrt_nr(d2);
-            const double dis = d2 * inv;
-            const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;     // r^.n_y
-            const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;  // -r^.n_x
-            double sn, cs;
-            sincos_red(k * dis, &sn, &cs);
-            const double re2 = (kTrW[q] * P.wj) * inv;
-            const double zgr = cs * re2, zgi = sn * re2;
-            // zb = zg * (-1/r + i k)
-            const double zbr = -zgr * inv - zgi * k;
-            const double zbi = zgr * k - zgi * inv;
-            hr = __builtin_fma(zbr, pn, hr);
-            hi = __builtin_fma(zbi, pn, hi);
-            const double inv2 = inv * inv;
-            const double rq = pn * qn;
-            const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
-            const double efi = -k * inv * (3.0 * rq + nxny);
-            er += zgr * efr - zgi * efi;
-            ei += zgr * efi + zgi * efr;
-            if (MODE == 1) {
-                gr += zgr;
-                gi += zgi;
-                tr = __builtin_fma(zbr, qn, tr);
-                ti = __builtin_fma(zbi, qn, ti);
-            }
-        }
+        double hr, hi, er, ei, gr, gi, tr, ti;
+        far_sums<MODE>(P, src, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
         double2 out;
         if (MODE == 0) {
-            // (H * gamma) * tau + E * beta with gamma = 1
-            out.x = hr * tau_r + (er * beta.x - ei * beta.y);
-            out.y = hi * tau_r + (er * beta.y + ei * beta.x);
+            out = velocity_coefficient(hr, hi, er, ei, tau_r, beta);
         } else {
             Acc a;
             a.g = make_double2(gr, gi);
@@ -448,6 +466,66 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
         }
         if (active) A[i * lda + j] = out;
     }
+}
+
+// Coefficient of one near pair, summed over its leaves by one wave (lanes
+// over (leaf, point)); every lane returns the total.
+__device__ __forceinline__ double2 near_entry(const NearPair& np, const uint64_t* __restrict__ leaves,
+                                              const PanelGeom& G, const Source& src, const PanelBc& bc,
+                                              double k, double wavruim, double tau_r, double2 beta, double2 iwr,
+                                              int lane) {
+    const double k2 = k * k;
+    Acc a = acc_zero();
+    const int total = np.count * 13;
+    for (int t = lane; t < total; t += 64) {
+        int leaf = t / 13;
+        int q = t - leaf * 13;
+        near_point(G, leaves[np.begin + leaf], q, src.c, src.n, wavruim, k2, a);
+    }
+    wave_sum_acc(a);
+    return coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
+}
+
+// Diagonal entry and BC free-term right-hand side of panel G (self integral
+// summed by one wave, tbem.rs:1330-1358); every lane returns the totals.
+// G refers to global memory: the sample loop indexes its nodes by edge.
+__device__ __forceinline__ void self_entry(const PanelGeom& G, const Source& src, const PanelBc& bc, double k,
+                                           double wavruim, double tau_r, double2 beta, double2 iwr, int lane,
+                                           double2* diag_out, double2* rhs_out) {
+    const double k2 = k * k;
+    const SelfTier t = tier_for(k * G.size);
+    const int ns = self_samples(t);
+    Acc a = acc_zero();
+    for (int s = lane; s < ns; s += 64) self_sample(G, t, s, src.c, src.n, wavruim, k2, a);
+    wave_sum_acc(a);
+    const double2 gamma = make_double2(1.0, 0.0);
+    const double2 tau = make_double2(tau_r, 0.0);
+    double2 diag = make_double2(0.0, 0.0);
+    double2 local_rhs = make_double2(0.0, 0.0);
+    double2 avg = make_double2(bc.value[0], bc.value[1]);
+    // tbem.rs:1344-1358
+    if (bc.type == 0) {
+        diag = csub(diag, cscale(gamma, 0.5));
+        if (bc.has_adm) {
+            double2 adm = make_double2(bc.adm[0], bc.adm[1]);
+            diag = csub(diag, cscale(cmul(cmul(cmul(adm, iwr), beta), tau), 0.5));
+        }
+        local_rhs = cadd(local_rhs, cscale(cmul(cmul(cmul(avg, iwr), beta), tau), 0.5));
+    } else if (bc.type == 1) {
+        diag = cadd(diag, cscale(cmul(beta, tau), 0.5));
+        local_rhs = csub(local_rhs, cscale(cmul(avg, gamma), 0.5));
+    }
+    *diag_out = cadd(diag, coefficient(a, bc, gamma, tau, beta, iwr));
+    *rhs_out = local_rhs;
+}
+
+__device__ __forceinline__ PanelBc rigid_bc() {
+    PanelBc bc;
+    bc.type = 0;
+    bc.has_adm = 0;
+    bc.value[0] = bc.value[1] = 0.0;
+    bc.adm[0] = bc.adm[1] = 0.0;
+    return bc;
 }
 
 // One wave per near pair; lanes over (leaf, point).
@@ -463,28 +541,9 @@ __global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ 
     const int lane = threadIdx.x & 63;
     if (p >= npairs) return;
     const NearPair np = pairs[p];
-    const PanelGeom G = geom[np.j];
-    const Source src = sources[np.i];
-    const double k2 = k * k;
-    Acc a = acc_zero();
-    const int total = np.count * 13;
-    for (int t = lane; t < total; t += 64) {
-        int leaf = t / 13;
-        int q = t - leaf * 13;
-        near_point(G, leaves[np.begin + leaf], q, src.c, src.n, wavruim, k2, a);
-    }
-    wave_sum_acc(a);
-    if (lane == 0) {
-        PanelBc bc;
-        if (general) {
-            bc = bcs[np.j];
-        } else {
-            bc.type = 0;
-            bc.has_adm = 0;
-        }
-        A[(int64_t)np.i * lda + np.j] =
-            coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
-    }
+    const PanelBc bc = general ? bcs[np.j] : rigid_bc();
+    const double2 v = near_entry(np, leaves, geom[np.j], sources[np.i], bc, k, wavruim, tau_r, beta, iwr, lane);
+    if (lane == 0) A[(int64_t)np.i * lda + np.j] = v;
 }
 
 // One wave per panel: self integrals, free terms, BC free-term RHS.
@@ -497,41 +556,10 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= n) return;
-    const PanelGeom G = geom[i];
-    const Source src = sources[i];
-    const double k2 = k * k;
-    const SelfTier t = tier_for(k * G.size);
-    const int ns = self_samples(t);
-    Acc a = acc_zero();
-    for (int s = lane; s < ns; s += 64) self_sample(G, t, s, src.c, src.n, wavruim, k2, a);
-    wave_sum_acc(a);
+    const PanelBc bc = general ? bcs[i] : rigid_bc();
+    double2 diag, local_rhs;
+    self_entry(geom[i], sources[i], bc, k, wavruim, tau_r, beta, iwr, lane, &diag, &local_rhs);
     if (lane == 0) {
-        PanelBc bc;
-        if (general) {
-            bc = bcs[i];
-        } else {
-            bc.type = 0;
-            bc.has_adm = 0;
-            bc.value[0] = bc.value[1] = 0.0;
-        }
-        const double2 gamma = make_double2(1.0, 0.0);
-        const double2 tau = make_double2(tau_r, 0.0);
-        double2 diag = make_double2(0.0, 0.0);
-        double2 local_rhs = make_double2(0.0, 0.0);
-        double2 avg = make_double2(bc.value[0], bc.value[1]);
-        // tbem.rs:1344-1358
-        if (bc.type == 0) {
-            diag = csub(diag, cscale(gamma, 0.5));
-            if (bc.has_adm) {
-                double2 adm = make_double2(bc.adm[0], bc.adm[1]);
-                diag = csub(diag, cscale(cmul(cmul(cmul(adm, iwr), beta), tau), 0.5));
-            }
-            local_rhs = cadd(local_rhs, cscale(cmul(cmul(cmul(avg, iwr), beta), tau), 0.5));
-        } else if (bc.type == 1) {
-            diag = cadd(diag, cscale(cmul(beta, tau), 0.5));
-            local_rhs = csub(local_rhs, cscale(cmul(avg, gamma), 0.5));
-        }
-        diag = cadd(diag, coefficient(a, bc, gamma, tau, beta, iwr));
         A[i * lda + i] = diag;
         rhs[i] = local_rhs;
     }
@@ -600,6 +628,21 @@ __global__ __launch_bounds__(256) void bc_rhs_kernel(const int32_t* __restrict__
 
 // incident.rs:216-330, 807-834; bem_solver.rs:5005-5034 (reference order).
 #pragma clang fp contract(off)
+__device__ __forceinline__ double2 plane_wave_value(const Source& s, const double* d, double2 amp, double k,
+                                                    double hf, double tau_r, double2 beta, double sign) {
+    double kdotx = hf * k * (d[0] * s.c[0] + d[1] * s.c[1] + d[2] * s.c[2]);
+    double kdotn = hf * k * (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]);
+    double sn, cs;
+    sincos_red(kdotx, &sn, &cs);
+    double2 p = cmul(amp, make_double2(cs, sn));
+    double2 dpdn = cmul(make_double2(0.0, kdotn), p);
+    double2 gp = cmul(make_double2(1.0, 0.0), p);
+    double2 bt = cmul(beta, make_double2(tau_r, 0.0));
+    double2 inc = cadd(gp, cmul(bt, dpdn));
+    inc = make_double2(-inc.x, -inc.y);
+    return cmul(inc, make_double2(sign, 0.0));
+}
+
 __global__ void plane_wave_kernel(const Source* __restrict__ sources, int64_t n,
                                   const double* __restrict__ dirs, const double2* __restrict__ amps,
                                   int64_t nrhs, double k, double hf, double tau_r, double2 beta,
@@ -608,23 +651,82 @@ __global__ void plane_wave_kernel(const Source* __restrict__ sources, int64_t n,
     if (t >= n * nrhs) return;
     const int64_t i = t / nrhs;
     const int64_t c = t - i * nrhs;
-    const Source s = sources[i];
-    const double* d = dirs + 3 * c;
-    double kdotx = hf * k * (d[0] * s.c[0] + d[1] * s.c[1] + d[2] * s.c[2]);
-    double kdotn = hf * k * (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]);
-    double sn, cs;
-    sincos_red(kdotx, &sn, &cs);
-    double2 p = cmul(amps[c], make_double2(cs, sn));
-    double2 dpdn = cmul(make_double2(0.0, kdotn), p);
-    double2 gp = cmul(make_double2(1.0, 0.0), p);
-    double2 bt = cmul(beta, make_double2(tau_r, 0.0));
-    double2 inc = cadd(gp, cmul(bt, dpdn));
-    inc = make_double2(-inc.x, -inc.y);
-    double2 add = cmul(inc, make_double2(sign, 0.0));
+    const double2 add = plane_wave_value(sources[i], dirs + 3 * c, amps[c], k, hf, tau_r, beta, sign);
     rhs[t] = accumulate ? cadd(rhs[t], add) : add;
 }
 
 #pragma clang fp contract(fast)
+
+// Batch of rigid instances, two persistent launches over the same items
+// (instance, first row); the host orders items by decreasing instance size
+// and each workgroup takes items from a counter until none are left.
+// PHASE 0 writes the far entries of the item's rows (lane per column) and
+// their plane-wave right-hand sides; PHASE 1, launched after it, overwrites
+// the near pairs and the diagonal of those rows, which is the overwrite order
+// of the single-system launch.  The phases are separate kernels because the
+// near/self code needs about twice the registers of the far loop.
+template <int PHASE>
+__global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs args) {
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(args.counter + PHASE, 1u);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= args.nitems) break;
+        const int2 item = args.items[it];
+        const BatchInstance& I = args.inst[item.x];
+        const int64_t n = I.n;
+        const int64_t r0 = item.y;
+        const int64_t r1 = min(n, r0 + (int64_t)args.rows_per_item);
+        const Source* src = args.src + I.panel_off;
+        double2* A = args.A + I.a_off;
+        const double k = I.k;
+        const double2 beta = make_double2(I.beta[0], I.beta[1]);
+        if (PHASE == 0) {
+            const FarPanel* far = args.far + I.panel_off;
+            const double k2 = k * k;
+            for (int64_t jc = 0; jc < n; jc += 256) {
+                const int64_t j = jc + threadIdx.x;
+                const bool active = j < n;
+                const FarPanel P = far[active ? j : n - 1];
+                for (int64_t i = r0; i < r1; ++i) {
+                    double hr, hi, er, ei, gr, gi, tr, ti;
+                    far_sums<0>(P, src[i], k, k2, hr, hi, er, ei, gr, gi, tr, ti);
+                    if (active) A[i * n + j] = velocity_coefficient(hr, hi, er, ei, I.tau, beta);
+                }
+            }
+            const int64_t nr = (r1 - r0) * I.ncol;
+            for (int64_t t = threadIdx.x; t < nr; t += 256) {
+                const int64_t i = r0 + t / I.ncol;
+                const int c = (int)(t % I.ncol);
+                args.rhs[I.vec_off + i * I.ncol + c] = plane_wave_value(
+                    src[i], args.dirs + 3 * (I.dir_off + c), make_double2(1.0, 0.0), k, 1.0, I.tau, beta, I.sign);
+            }
+        } else {
+            const PanelGeom* geom = args.geom + I.panel_off;
+            const PanelBc bc = rigid_bc();
+            const double2 iwr = make_double2(0.0, 0.0);  // no admittance on rigid panels
+            const int64_t pb = args.row_ptr[I.panel_off + r0];
+            const int64_t pe = args.row_ptr[I.panel_off + r1];
+            for (int64_t p = pb + wave; p < pe; p += 4) {
+                const NearPair np = args.pairs[p];
+                const double2 v =
+                    near_entry(np, args.leaves, geom[np.j], src[np.i], bc, k, k, I.tau, beta, iwr, lane);
+                if (lane == 0) A[(int64_t)np.i * n + np.j] = v;
+            }
+            for (int64_t i = r0 + wave; i < r1; i += 4) {
+                double2 diag, unused;
+                self_entry(geom[i], src[i], bc, k, k, I.tau, beta, iwr, lane, &diag, &unused);
+                if (lane == 0) A[i * n + i] = diag;
+            }
+        }
+    }
+}
+
+__global__ void reset_counters_kernel(unsigned* counter) { counter[threadIdx.x] = 0u; }
 
 __global__ void sincos_probe_kernel(const double* x, int64_t n, double* s, double* c) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -709,6 +811,21 @@ void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* m, const sbh_physics&
                        accumulate ? 1 : 0, d_rhs);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_RHS, ev);
+}
+
+void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int grid) {
+    if (args.nitems == 0) return;
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_FAR, &ev);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(2), 0, ctx->stream, args.counter);
+    SBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(batch_assemble_kernel<0>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_NEAR, &ev);
+    hipLaunchKernelGGL(batch_assemble_kernel<1>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
 }
 
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {

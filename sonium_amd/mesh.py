@@ -180,3 +180,32 @@ def fibonacci_directions(count: int) -> np.ndarray:
     z = 1.0 - (2.0 * i + 1.0) / count
     r = np.sqrt(np.maximum(1.0 - z * z, 0.0))
     return np.stack([r * np.cos(golden * i), r * np.sin(golden * i), z], axis=1)
+
+
+def uniform_directions(rng: np.random.Generator, count: int) -> np.ndarray:
+    """count unit vectors uniform on the sphere (normalised Gaussian triples)."""
+    d = rng.standard_normal((count, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def random_ellipsoid_batch(count: int = 256, seed: int = 20261202, levels=(2, 3, 4), k_range=(1.0, 8.0),
+                           max_directions: int = 16):
+    """Seeded batch of rigid ellipsoid scatterers of uneven sizes.
+
+    Per instance, drawn in this order from ``numpy.random.Generator(PCG64(seed))``:
+    semi-axes U[0.5, 1.5]^3, icosphere level from ``levels`` (320, 1280, 5120
+    panels for 2, 3, 4), wavenumber U[k_range], number of plane-wave
+    directions U{1..max_directions}, then the directions (uniform on the
+    sphere).  Returns a list of dicts with keys mesh, semi_axes, level, k,
+    directions.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        axes = rng.uniform(0.5, 1.5, size=3)
+        level = int(rng.choice(np.asarray(levels)))
+        k = float(rng.uniform(k_range[0], k_range[1]))
+        ndir = int(rng.integers(1, max_directions + 1))
+        dirs = uniform_directions(rng, ndir)
+        out.append({"mesh": ellipsoid(axes, level), "semi_axes": axes, "level": level, "k": k, "directions": dirs})
+    return out
