@@ -397,6 +397,37 @@ sbh_status sbh_batch_autotune(sbh_batch* batch, double* table, int max_entries, 
                               sbh_batch_tuning* best);
 
 /* ---------------------------------------------------------------------------
+ * Learned warm start.  No reference counterpart; the reference's warm start
+ * is the previous frequency's solution (frequency_sweep.rs:718-730) passed to
+ * gmres_with_guess.  A 4-layer MLP (64 features -> width -> width -> width ->
+ * 2, SiLU) with BF16 weights and FP32 accumulation maps one panel's raw
+ * input {c[3], n[3], d[3], k} (centroid, outward normal, incident direction,
+ * wavenumber; 10 floats) to the incident-phase-free density
+ * u = x exp(-i k d.c) as {Re, Im}; training minimises sum |u_pred - u|^2 / (2M)
+ * with Adam on FP32 master weights.  Parameters are one FP32 vector: per layer
+ * the row-major weights (out x in) followed by the bias (out). */
+typedef struct sbh_mlp sbh_mlp;
+
+sbh_status sbh_mlp_create(sbh_context* ctx, int32_t width, uint64_t seed, sbh_mlp** out);
+void sbh_mlp_destroy(sbh_mlp* net);
+int64_t sbh_mlp_num_params(const sbh_mlp* net);
+sbh_status sbh_mlp_get_params(sbh_mlp* net, float* params);
+sbh_status sbh_mlp_set_params(sbh_mlp* net, const float* params);
+/* Loss and parameter gradients on M samples (M a multiple of 256; host arrays
+ * M x 10 inputs, M x 2 targets; grads in the parameter layout). */
+sbh_status sbh_mlp_loss_and_gradients(sbh_mlp* net, const float* inputs, const float* targets, int64_t M,
+                                      double* loss, float* grads);
+/* Training set (host arrays count x 10 and count x 2), copied to the device. */
+sbh_status sbh_mlp_set_dataset(sbh_mlp* net, const float* inputs, const float* targets, int64_t count);
+/* steps Adam steps on batches drawn from the training set (cosine learning
+ * rate from lr_max to lr_min); losses (optional, steps entries) receive the
+ * batch loss before each update. */
+sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_max, double lr_min, uint64_t seed,
+                         double* losses);
+/* Predictions (M x 2 floats) for M raw inputs (M x 10 floats). */
+sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory);
+
+/* ---------------------------------------------------------------------------
  * Test hooks (no GPU work): the host route builder's view of one pair, so
  * tests can compare it with the CPU oracle's generate_subelements.
  * leaves_out: up to 110 x 6 doubles (xi,eta of 3 vertices); returns the

@@ -1,0 +1,123 @@
+"""Learned warm start for frequency sweeps.
+
+The network (HIP kernels in csrc/mlp_kernels.hip) maps one panel sample
+{centroid, outward normal, incident direction, k} to the incident-phase-free
+surface density u = x exp(-i k d.c); the GMRES initial guess is
+x0 = u exp(i k d.c).  The reference's own warm start is the previous
+frequency's solution (frequency_sweep.rs:718-730), passed to gmres_with_guess
+(gmres.rs:564-626); the zero-guess solve stays the fallback: a guess whose
+residual is not below ||b|| is replaced by zero, and every solve is judged by
+the recomputed FP64 residual.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+from . import _check, _cptr, _lib, Context, DenseMatrixOperator, GmresConfig, gmres_with_guess, gmres
+
+RAW = 10
+
+
+def panel_inputs(centers, normals, k: float, direction) -> np.ndarray:
+    """(N, 10) float32 raw inputs of one solve (one incident direction)."""
+    n = centers.shape[0]
+    out = np.empty((n, RAW), dtype=np.float32)
+    out[:, 0:3] = centers
+    out[:, 3:6] = normals
+    out[:, 6:9] = np.asarray(direction, dtype=np.float64).reshape(1, 3)
+    out[:, 9] = k
+    return out
+
+
+def incident_phase(centers, k: float, direction) -> np.ndarray:
+    return np.exp(1j * k * (centers @ np.asarray(direction, dtype=np.float64).reshape(3)))
+
+
+def density_to_target(x, centers, k: float, direction) -> np.ndarray:
+    """u = x exp(-i k d.c) as (N, 2) float32."""
+    u = np.asarray(x) * np.conj(incident_phase(centers, k, direction))
+    return np.stack([u.real, u.imag], axis=1).astype(np.float32)
+
+
+def target_to_density(y, centers, k: float, direction) -> np.ndarray:
+    u = np.asarray(y[:, 0], dtype=np.float64) + 1j * np.asarray(y[:, 1], dtype=np.float64)
+    return u * incident_phase(centers, k, direction)
+
+
+class WarmStartMLP:
+    """4-layer MLP (64 -> width -> width -> width -> 2), BF16 weights, FP32 accumulation."""
+
+    def __init__(self, ctx: Context, width: int = 512, seed: int = 1):
+        self.ctx = ctx
+        self.width = int(width)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_mlp_create(ctx.handle, self.width, int(seed), ctypes.byref(h)))
+        self._h = h
+        self.num_params = int(_lib().sbh_mlp_num_params(h))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params, dtype=np.float32)
+        _check(_lib().sbh_mlp_get_params(self._h, _cptr(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        v = np.ascontiguousarray(p, dtype=np.float32).reshape(self.num_params)
+        _check(_lib().sbh_mlp_set_params(self._h, _cptr(v)))
+
+    def loss_and_gradients(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(_lib().sbh_mlp_loss_and_gradients(self._h, _cptr(x), _cptr(t), x.shape[0], ctypes.byref(loss),
+                                                 _cptr(g)))
+        return loss.value, g
+
+    def set_dataset(self, inputs, targets) -> None:
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        _check(_lib().sbh_mlp_set_dataset(self._h, _cptr(x), _cptr(t), x.shape[0]))
+
+    def train(self, steps: int, batch: int = 16384, lr_max: float = 2e-3, lr_min: float = 5e-5,
+              seed: int = 1) -> np.ndarray:
+        losses = np.zeros(int(steps))
+        _check(_lib().sbh_mlp_train(self._h, int(steps), int(batch), float(lr_max), float(lr_min), int(seed),
+                                    _cptr(losses)))
+        return losses
+
+    def predict(self, inputs) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        out = np.empty((x.shape[0], 2), dtype=np.float32)
+        _check(_lib().sbh_mlp_predict(self._h, _cptr(x), x.shape[0], _cptr(out), 0))
+        return out
+
+    def guess(self, centers, normals, k: float, direction) -> np.ndarray:
+        """GMRES initial guess x0 for one (k, direction) on a mesh."""
+        y = self.predict(panel_inputs(centers, normals, k, direction))
+        return target_to_density(y, centers, k, direction)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_with_guess(op: DenseMatrixOperator, b, x0: Optional[np.ndarray], config: GmresConfig):
+    """gmres_with_guess with the zero-guess fallback: a guess whose residual is not
+    below ||b|| is dropped.  Returns (solution, used_guess)."""
+    if x0 is not None:
+        r0 = np.linalg.norm(b - op.apply(x0))
+        if np.isfinite(r0) and r0 < np.linalg.norm(b):
+            return gmres_with_guess(op, b, x0, config), True
+    return gmres(op, b, config), False
