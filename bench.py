@@ -51,6 +51,8 @@ WORKLOADS = {
     "c2": {"subdiv": 5, "k": 10.0, "restart": 100,
            "name": "rigid sphere icosphere 20480 panels, k=10, beta=i/k, 1 plane wave +z, GMRES(100) rtol 1e-8, complex128"},
     "c3": {"name": "many right-hand sides (separate report)"},
+    "c4": {"name": "batch of 256 seeded rigid ellipsoids (separate report)"},
+    "c5": {"name": "frequency sweep with learned warm start (separate report)"},
     "c1": {"subdiv": 3, "k": 2.0, "restart": 50,
            "name": "rigid sphere icosphere 1280 panels, k=2, beta=i/k, 1 plane wave +z, GMRES(50) rtol 1e-8, complex128"},
 }
@@ -182,6 +184,215 @@ def run_c3(args):
     print(json.dumps(line))
 
 
+def _percentiles(values):
+    v = np.asarray(values, dtype=np.float64)
+    return {"p50_ms": float(np.percentile(v, 50)), "p99_ms": float(np.percentile(v, 99)),
+            "max_ms": float(v.max())}
+
+
+def run_c4(args):
+    """256 independent rigid ellipsoids (320-5120 panels, k U[1,8], 1-16 plane waves).
+
+    plain: the single-system FP64 path instance by instance (assemble, plane
+    waves, block GMRES(100)), per-instance wall time as its latency.
+    batched: every instance in one batch: persistent assembly, then lock-step
+    GMRES (FP64) or FP16 iterative refinement with each cycle replayed from a
+    captured HIP graph; per-instance latency = device clock from the start of
+    the batch assembly to the instance's last converged column.
+    """
+    from sonium_amd.mesh import random_ellipsoid_batch
+
+    ctx = sa.Context(0)
+    specs = random_ellipsoid_batch(256, seed=20261202)
+    tms = [sa.TbemMesh(ctx, s["mesh"]) for s in specs]
+    ncols = np.array([len(s["directions"]) for s in specs])
+    total_rhs = int(ncols.sum())
+    sizes = np.array([s["mesh"].num_elements for s in specs])
+    print(f"c4: {len(specs)} instances, {total_rhs} right-hand sides, panels {sizes.sum()}", file=sys.stderr, flush=True)
+    cfg = sa.GmresConfig(max_iterations=10, restart=100, tolerance=1e-8)
+
+    # plain FP64, one instance after another (routes built beforehand, as setup)
+    wss = [sa.TbemAssemblyWorkspace(ctx, tm) for tm in tms]
+    ctx.synchronize()
+    plain_lat, plain_x = [], []
+    plain_iters = []
+    t_all = time.perf_counter()
+    for s, tm, ws in zip(specs, tms, wss):
+        t0 = time.perf_counter()
+        ph = sa.PhysicsParams.from_wavenumber(s["k"])
+        beta = ph.burton_miller_beta()
+        system = ws.assemble(ph, beta)
+        B = sa.IncidentField(s["directions"]).compute_rhs_with_beta(tm, ph, beta).reshape(tm.num_elements, -1)
+        X, sols = sa.gmres_block(sa.DenseMatrixOperator(system), B, cfg)
+        ctx.synchronize()
+        plain_lat.append((time.perf_counter() - t0) * 1e3)
+        if not all(q.converged and q.residual <= 1e-8 for q in sols):
+            raise SystemExit("bench c4: a plain solve did not converge")
+        plain_x.append(X)
+        plain_iters.append(max(q.iterations for q in sols))
+        system.close()
+    t_plain = time.perf_counter() - t_all
+    del wss
+    print(f"c4: plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    batch = sa.Batch(ctx, tms)
+    batch.set_problem([s["k"] for s in specs], [s["directions"] for s in specs])
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    best, table = batch.autotune()
+    tune_s = time.perf_counter() - t0
+    print(f"c4: batch setup {setup_s:.2f} s, autotune {tune_s:.2f} s -> {best}", file=sys.stderr, flush=True)
+    forms = {}
+    for form, fmt in (("batched_fp64", "fp64"), ("batched_fp16_ir", "fp16_ir")):
+        for rep in range(2):  # first run captures the cycle graph
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            batch.assemble()
+            if fmt != "fp64":
+                batch.convert("fp16")
+            if fmt == "fp64":
+                sts = batch.solve("fp64", restart=100, max_cycles=10, tolerance=1e-8)
+            else:
+                sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4)
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+        if not all(q.converged for q in sts):
+            raise SystemExit(f"bench c4: {form} left columns unconverged")
+        lat, col, err = [], 0, 0.0
+        for i, s in enumerate(specs):
+            c = int(ncols[i])
+            lat.append(max(q.completion_ms for q in sts[col: col + c]))
+            col += c
+        for i in range(0, len(specs), 8):
+            X = batch.solution(i)
+            err = max(err, float(np.linalg.norm(X - plain_x[i]) / np.linalg.norm(plain_x[i])))
+        forms[form] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
+                       "max_iterations": max(q.iterations for q in sts), "max_cycles": max(q.cycles for q in sts),
+                       "max_residual": max(q.residual for q in sts), "max_error_vs_plain_sampled": err,
+                       "vs_plain": (total_rhs / el) / (total_rhs / t_plain)}
+        print(f"c4: {form} {el:.3f} s", file=sys.stderr, flush=True)
+    line = {"metric": "solved RHS/s at rel. residual 1e-8 (batch)",
+            "workload": "C4: 256 rigid ellipsoids, semi-axes U[0.5,1.5], 320/1280/5120 panels, k U[1,8], 1-16 plane waves, seed 20261202",
+            "instances": len(specs), "rhs": total_rhs, "panels": int(sizes.sum()),
+            "matrix_gb_fp64": float((sizes.astype(np.float64) ** 2).sum() * 16 / 1e9),
+            "plain_fp64_sequential": {"seconds": t_plain, "rhs_per_s": total_rhs / t_plain, **_percentiles(plain_lat),
+                                      "max_iterations": int(max(plain_iters))},
+            "batched": forms, "autotune": {"best": best, "candidates": table, "seconds": tune_s},
+            "batch_setup_s": setup_s}
+    print(json.dumps(line))
+
+
+def run_c5(args):
+    """Frequency sweep with the learned warm start.
+
+    Training data: the 10240-panel Fibonacci sphere at 512 wavenumbers in
+    [1, 12] (+z incidence) and 16 seeded ellipsoid variants of it (semi-axes
+    U[0.75, 1.25], 93 wavenumbers each, random incidence): 2000 solver
+    solutions.  Evaluation: 256 held-out wavenumbers (midpoints of the
+    training grid) on the sphere; per wavenumber a new matrix is assembled and
+    solved to 1e-8 from (a) zero, (b) the previous wavenumber's solution,
+    (c) the network's guess.
+    """
+    from sonium_amd import warmstart as wsm
+    from sonium_amd.mesh import Mesh, uniform_directions
+
+    ctx = sa.Context(0)
+    base = fibonacci_sphere(1.0, 65, 80)
+    n = base.num_elements
+    cfg = sa.GmresConfig(max_iterations=20, restart=100, tolerance=1e-8)
+    rng = np.random.Generator(np.random.PCG64(20261203))
+    meshes = [("sphere", base, np.linspace(1.0, 12.0, 512), None)]
+    for v in range(16):
+        axes = rng.uniform(0.75, 1.25, size=3)
+        ks = np.sort(rng.uniform(1.0, 12.0, size=93))
+        meshes.append((f"ellipsoid{v}", Mesh(base.nodes * axes[None, :], base.triangles), ks,
+                       uniform_directions(rng, 93)))
+    inputs, targets = [], []
+    t0 = time.perf_counter()
+    gen_iters = []
+    for name, m, ks, dirs in meshes:
+        tm = sa.TbemMesh(ctx, m)
+        c, nn, _ = tm.geometry()
+        ws = sa.TbemAssemblyWorkspace(ctx, tm)
+        system = sa.TbemSystem(ctx, n)
+        op = sa.DenseMatrixOperator(system)
+        for j, k in enumerate(ks):
+            d = np.array([0.0, 0.0, 1.0]) if dirs is None else dirs[j]
+            ph = sa.PhysicsParams.from_wavenumber(float(k))
+            beta = ph.burton_miller_beta()
+            ws.assemble(ph, beta, system)
+            b = sa.IncidentField([d]).compute_rhs_with_beta(tm, ph, beta)
+            sol = sa.gmres(op, b, cfg)
+            if not sol.converged:
+                raise SystemExit(f"bench c5: training solve {name} k={k} did not converge")
+            gen_iters.append(sol.iterations)
+            inputs.append(wsm.panel_inputs(c, nn, float(k), d))
+            targets.append(wsm.density_to_target(sol.x, c, float(k), d))
+        print(f"c5: generated {name} ({len(ks)} solves)", file=sys.stderr, flush=True)
+    gen_s = time.perf_counter() - t0
+    n_solutions = len(inputs)
+    net = wsm.WarmStartMLP(ctx, width=512, seed=20261204)
+    net.set_dataset(np.concatenate(inputs), np.concatenate(targets))
+    del inputs, targets
+    steps, batch_size = 4000, 16384
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    losses = net.train(steps, batch=batch_size, lr_max=2e-3, lr_min=2e-5, seed=7)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    print(f"c5: trained {steps} steps in {train_s:.1f} s, loss {losses[:3]} -> {losses[-3:]}", file=sys.stderr,
+          flush=True)
+
+    # evaluation on held-out wavenumbers
+    tm = sa.TbemMesh(ctx, base)
+    c, nn, _ = tm.geometry()
+    ws = sa.TbemAssemblyWorkspace(ctx, tm)
+    system = sa.TbemSystem(ctx, n)
+    op = sa.DenseMatrixOperator(system)
+    kgrid = np.linspace(1.0, 12.0, 512)
+    ks = 0.5 * (kgrid[0::2] + kgrid[1::2])
+    d = np.array([0.0, 0.0, 1.0])
+    res = {}
+    for mode in ("zero", "previous", "learned"):
+        iters, errs, used = [], [], 0
+        prev = None
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in ks:
+            ph = sa.PhysicsParams.from_wavenumber(float(k))
+            beta = ph.burton_miller_beta()
+            ws.assemble(ph, beta, system)
+            b = sa.IncidentField([d]).compute_rhs_with_beta(tm, ph, beta)
+            x0 = None
+            if mode == "previous":
+                x0 = prev
+            elif mode == "learned":
+                x0 = net.guess(c, nn, float(k), d)
+            sol, u = wsm.solve_with_guess(op, b, x0, cfg)
+            used += int(u)
+            if not (sol.converged and sol.residual <= 1e-8):
+                raise SystemExit(f"bench c5: {mode} solve at k={k} did not converge")
+            iters.append(sol.iterations)
+            prev = sol.x
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = {"rhs_per_s": len(ks) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
+                     "max_iterations": int(np.max(iters)), "guesses_used": used}
+        print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}", file=sys.stderr, flush=True)
+    for mode in ("previous", "learned"):
+        res[mode]["vs_zero"] = res[mode]["rhs_per_s"] / res["zero"]["rhs_per_s"]
+    line = {"metric": "solved RHS/s at rel. residual 1e-8 (frequency sweep)",
+            "workload": "C5: Fibonacci sphere 10240 panels, held-out k in [1,12] (256), +z; MLP 4x512 BF16 trained on "
+                        f"{n_solutions} solver solutions (sphere sweep + 16 ellipsoid variants)",
+            "panels": n, "training": {"solutions": n_solutions, "samples": n_solutions * n, "generate_s": gen_s,
+                                      "generation_mean_iterations": float(np.mean(gen_iters)),
+                                      "steps": steps, "batch": batch_size, "train_s": train_s,
+                                      "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-50:]))},
+            "results": res}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,8 +410,8 @@ def main():
         sa._check(sa._lib().sbh_debug_device_copy(ctx.handle, args.calibrate_copy, 3))
         print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 3}))
         return
-    if args.workload == "c3":
-        run_c3(args)
+    if args.workload in ("c3", "c4", "c5"):
+        {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

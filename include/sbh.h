@@ -355,6 +355,8 @@ typedef struct {
     int64_t max_cycles;       /* restart cycles (FP64) or refinement steps (IR) */
     double tolerance;         /* on the recomputed FP64 relative residual */
     double inner_tolerance;   /* IR: inner estimate relative to the current residual */
+    int32_t use_guess;        /* start from the solutions set by sbh_batch_set_guess (else zero) */
+    int32_t pad;
 } sbh_batch_solve_config;
 
 typedef struct {
@@ -370,6 +372,10 @@ typedef struct {
 /* status has sum(num_rhs) entries, instance-major. */
 sbh_status sbh_batch_solve(sbh_batch* batch, const sbh_batch_solve_config* config,
                            sbh_batch_column_status* status);
+/* Initial guess of one instance (n x num_rhs row-major, host memory), used
+ * by the next sbh_batch_solve with use_guess = 1 (gmres_with_guess,
+ * gmres.rs:564-580; the relative residual stays relative to ||b||). */
+sbh_status sbh_batch_set_guess(sbh_batch* batch, int64_t instance, const sbh_c64* x0);
 /* n x num_rhs row-major blocks of one instance (host memory). */
 sbh_status sbh_batch_download_solution(sbh_batch* batch, int64_t instance, sbh_c64* x);
 sbh_status sbh_batch_download_rhs(sbh_batch* batch, int64_t instance, sbh_c64* rhs);

@@ -779,14 +779,19 @@ class Batch:
         _check(_lib().sbh_batch_convert(self._h, {"fp16": 0, "bf16": 1}[fmt]))
 
     def solve(self, fmt: str = "fp64", restart: int = 100, max_cycles: int = 10, tolerance: float = 1e-8,
-              inner_tolerance: float = 1e-4, use_graph: bool = True):
+              inner_tolerance: float = 1e-4, use_graph: bool = True, use_guess: bool = False):
         cfg = _ffi.batch_solve_config_t(BATCH_FORMATS[fmt], 1 if use_graph else 0, int(restart), int(max_cycles),
-                                        float(tolerance), float(inner_tolerance))
+                                        float(tolerance), float(inner_tolerance), 1 if use_guess else 0, 0)
         ncol = int(self.num_rhs.sum())
         raws = (_ffi.batch_column_status_t * ncol)()
         _check(_lib().sbh_batch_solve(self._h, ctypes.byref(cfg), raws))
         return [BatchColumnStatus(int(r.iterations), int(r.cycles), int(r.reorthogonalizations), float(r.residual),
                                   bool(r.converged), float(r.completion_ms)) for r in raws]
+
+    def set_guess(self, i: int, X0) -> None:
+        """Initial guess of instance i (n x num_rhs), used by solve(use_guess=True)."""
+        x = np.ascontiguousarray(X0, dtype=np.complex128).reshape(self.meshes[i].num_elements, int(self.num_rhs[i]))
+        _check(_lib().sbh_batch_set_guess(self._h, int(i), _cptr(x)))
 
     def _download(self, fn, i, shape):
         out = np.empty(shape, dtype=np.complex128)

@@ -67,7 +67,7 @@ class gmres_ir_status_t(ctypes.Structure):
 class batch_solve_config_t(ctypes.Structure):
     _fields_ = [("format", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("restart", ctypes.c_int64),
                 ("max_cycles", ctypes.c_int64), ("tolerance", ctypes.c_double),
-                ("inner_tolerance", ctypes.c_double)]
+                ("inner_tolerance", ctypes.c_double), ("use_guess", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class batch_column_status_t(ctypes.Structure):
@@ -156,6 +156,7 @@ SIGNATURES = {
     "sbh_batch_assemble": (_int, [_vp]),
     "sbh_batch_convert": (_int, [_vp, _int]),
     "sbh_batch_solve": (_int, [_vp, ctypes.POINTER(batch_solve_config_t), _vp]),
+    "sbh_batch_set_guess": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_rhs": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_matrix": (_int, [_vp, _i64, _vp]),

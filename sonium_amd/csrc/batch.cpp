@@ -491,8 +491,9 @@ sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_
         ensure_solver(b, (int)cfg->restart);
         auto& s = *b->solver;
         const RaggedLayout lay = layout(b);
-        // start: x = 0, r = b, ||b||, first basis vector
-        SBH_HIP(hipMemsetAsync(b->d_x.ptr, 0, b->d_x.bytes(), ctx->stream));
+        // start: ||b|| from the residual of x = 0, then (with a guess) the guess's residual
+        const int refinement = cfg->format >= 0 ? 1 : 0;
+        if (!cfg->use_guess) SBH_HIP(hipMemsetAsync(b->d_x.ptr, 0, b->d_x.bytes(), ctx->stream));
         for (auto* a : {&s.iters, &s.cycles, &s.reorth_count})
             SBH_HIP(hipMemsetAsync(a->ptr, 0, a->bytes(), ctx->stream));
         SBH_HIP(hipMemsetAsync(s.t_done.ptr, 0, s.t_done.bytes(), ctx->stream));
@@ -501,8 +502,18 @@ sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_
         ragged_fill(ctx, s.R.ptr, b->L);
         ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
         ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 1, cfg->tolerance, cfg->inner_tolerance,
-                       cfg->format >= 0 ? 1 : 0, s.bnorm.ptr, s.rnorm.ptr, s.done.ptr, s.active.ptr, s.jlen.ptr,
-                       s.g.ptr, s.scale.ptr, s.tol_abs.ptr, s.cycles.ptr, s.t_done.ptr);
+                       refinement, s.bnorm.ptr, s.rnorm.ptr, s.done.ptr, s.active.ptr, s.jlen.ptr, s.g.ptr,
+                       s.scale.ptr, s.tol_abs.ptr, s.cycles.ptr, s.t_done.ptr);
+        if (cfg->use_guess) {
+            ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
+                         s.inst_pending.ptr);
+            batch_product(ctx, -1, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
+                          nullptr, b->d_x.ptr, s.R.ptr);
+            ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
+            ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 2, cfg->tolerance, cfg->inner_tolerance,
+                           refinement, s.bnorm.ptr, s.rnorm.ptr, s.done.ptr, s.active.ptr, s.jlen.ptr, s.g.ptr,
+                           s.scale.ptr, s.tol_abs.ptr, s.cycles.ptr, s.t_done.ptr);
+        }
         ragged_scale(ctx, lay, s.inst_pending.ptr, s.scale.ptr, s.R.ptr, s.V.ptr);
         ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
                      s.inst_pending.ptr);
@@ -597,6 +608,20 @@ sbh_status download_block(sbh_batch* b, int64_t instance, sbh_c64* out, int what
     });
 }
 }  // namespace
+
+sbh_status sbh_batch_set_guess(sbh_batch* b, int64_t instance, const sbh_c64* x0) {
+    return guard([&] {
+        check_problem(b);
+        SBH_REQUIRE(x0 && instance >= 0 && instance < b->B, "bad instance/x0");
+        use_device(b->ctx);
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
+        const BatchInstance& I = b->inst[instance];
+        const size_t count = (size_t)I.n * I.ncol;
+        for (size_t i = 0; i < count; ++i)
+            SBH_REQUIRE(std::isfinite(x0[i].re) && std::isfinite(x0[i].im), "initial guess is not finite");
+        SBH_HIP(hipMemcpy(b->d_x.ptr + I.vec_off, x0, count * sizeof(double2), hipMemcpyHostToDevice));
+    });
+}
 
 sbh_status sbh_batch_download_solution(sbh_batch* b, int64_t instance, sbh_c64* x) {
     return download_block(b, instance, x, 0);

@@ -48,7 +48,7 @@ void ragged_update(sbh_context* ctx, const RaggedLayout& lay, const int* flag, c
                    const double2* coef, double2* X);
 void ragged_residual(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* B, double2* R,
                      double2* part);
-void ragged_restart(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* pending_in, int init,
+void ragged_restart(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* pending_in, int mode,
                     double tol, double inner_tol, int refinement, double* bnorm, double* rnorm, int* done, int* active,
                     int* jlen, double2* g, double* scale, double* tol_abs, int64_t* cycles,
                     unsigned long long* t_done);

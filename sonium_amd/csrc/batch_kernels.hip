@@ -516,13 +516,15 @@ __global__ __launch_bounds__(BLOCK) void rresid_kernel(const VecChunk* __restric
     }
 }
 
-// After a residual: ||r|| per column, convergence test ||r|| / ||b|| < tol
-// (with init, ||b|| is this norm), completion time stamp, and the start of
-// the next cycle: g = (||r||, 0, ...), scale = 1/||r||, threshold of the
-// Arnoldi estimate = inner_tol * ||r|| (refinement) or tol * ||b|| (plain).
+// After a residual: ||r|| per column, convergence test ||r|| / ||b|| < tol,
+// completion time stamp, and the start of the next cycle: g = (||r||, 0, ...),
+// scale = 1/||r||, threshold of the Arnoldi estimate = inner_tol * ||r||
+// (refinement) or tol * ||b|| (plain).  mode 0: after a cycle (counted);
+// 1: the residual of x = 0, which also sets ||b||; 2: the residual of an
+// initial guess (not counted).
 __global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, const int* __restrict__ col_q,
                                 const int2* __restrict__ chunk_range, const double2* __restrict__ part, int MV,
-                                const int* __restrict__ pending_in, int init, double tol, double inner_tol,
+                                const int* __restrict__ pending_in, int mode, double tol, double inner_tol,
                                 int refinement, double* __restrict__ bnorm, double* __restrict__ rnorm,
                                 int* __restrict__ done, int* __restrict__ active, int* __restrict__ jlen,
                                 double2* __restrict__ g, double* __restrict__ scale, double* __restrict__ tol_abs,
@@ -540,7 +542,7 @@ __global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, 
     double s = 0.0;
     for (int ch = r.x; ch < r.y; ++ch) s += part[((int64_t)ch * MV) * MAXCOL + q].x;
     const double rn = sqrt(s);
-    if (init) bnorm[c] = rn;
+    if (mode == 1) bnorm[c] = rn;
     rnorm[c] = rn;
     const double bn = bnorm[c];
     const bool conv = bn == 0.0 || rn / bn < tol;
@@ -551,7 +553,7 @@ __global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, 
         t_done[c] = wall_clock64();
         return;
     }
-    if (!init) cycles[c] += 1;
+    if (mode == 0) cycles[c] += 1;
     double2* gc = g + (int64_t)c * (m + 1);
     gc[0] = make_double2(rn, 0.0);
     for (int i = 1; i <= m; ++i) gc[i] = make_double2(0.0, 0.0);
@@ -688,12 +690,12 @@ void ragged_residual(sbh_context* ctx, const RaggedLayout& lay, const int* flag,
     SBH_HIP(hipGetLastError());
 }
 
-void ragged_restart(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* pending_in, int init,
+void ragged_restart(sbh_context* ctx, const RaggedLayout& lay, const double2* part, const int* pending_in, int mode,
                     double tol, double inner_tol, int refinement, double* bnorm, double* rnorm, int* done, int* active,
                     int* jlen, double2* g, double* scale, double* tol_abs, int64_t* cycles,
                     unsigned long long* t_done) {
     hipLaunchKernelGGL(rrestart_kernel, dim3(nb(lay.C)), dim3(BLOCK), 0, ctx->stream, lay.C, lay.m, lay.col_inst,
-                       lay.col_q, lay.chunk_range, part, lay.MV, pending_in, init, tol, inner_tol, refinement, bnorm,
+                       lay.col_q, lay.chunk_range, part, lay.MV, pending_in, mode, tol, inner_tol, refinement, bnorm,
                        rnorm, done, active, jlen, g, scale, tol_abs, cycles, t_done);
     SBH_HIP(hipGetLastError());
 }
