@@ -164,7 +164,7 @@ SIGNATURES = {
                               ctypes.POINTER(_i64), _dp]),
     "sbh_batch_get_tuning": (_int, [_vp, ctypes.POINTER(batch_tuning_t)]),
     "sbh_batch_set_tuning": (_int, [_vp, ctypes.POINTER(batch_tuning_t)]),
-    "sbh_batch_autotune": (_int, [_vp, _dp, _int, ctypes.POINTER(_int), ctypes.POINTER(batch_tuning_t)]),
+    "sbh_batch_autotune": (_int, [_vp, _vp, _int, ctypes.POINTER(_int), ctypes.POINTER(batch_tuning_t)]),
     "sbh_mlp_create": (_int, [_vp, ctypes.c_int32, ctypes.c_uint64, _pp]),
     "sbh_mlp_destroy": (None, [_vp]),
     "sbh_mlp_num_params": (_i64, [_vp]),
@@ -172,7 +172,7 @@ SIGNATURES = {
     "sbh_mlp_set_params": (_int, [_vp, _vp]),
     "sbh_mlp_loss_and_gradients": (_int, [_vp, _vp, _vp, _i64, _dp, _vp]),
     "sbh_mlp_set_dataset": (_int, [_vp, _vp, _vp, _i64]),
-    "sbh_mlp_train": (_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp]),
+    "sbh_mlp_train": (_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _vp]),
     "sbh_mlp_predict": (_int, [_vp, _vp, _i64, _vp, _int]),
     "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),

@@ -176,13 +176,14 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
         }
         __syncthreads();
     }
-    // 16 x 16 tile: lane l, register v -> row 4*(l/16) + v, real column l % 16
+    // 16 x 16 tile of v_mfma_f64_16x16x4_f64: lane l, register v -> row 4*v + l/16,
+    // real column l % 16 (A operand lane l = A[l%16][l/16], B lane l = B[l/16][l%16])
     const int col = lane & 15;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t gr = row0 + 32 * w + 16 * a + 4 * (lane >> 4) + v;
+            const int64_t gr = row0 + 32 * w + 16 * a + 4 * v + (lane >> 4);
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 const double mine = acc[a][b][v];

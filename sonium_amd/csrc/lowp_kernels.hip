@@ -331,13 +331,13 @@ __global__ __launch_bounds__(BLOCK) void zgemm_f64_kernel(const double2* __restr
         }
         __syncthreads();
     }
-    // D layout of the 16x16 tile: lane l, register v -> row 4*(l/16) + v, column l % 16
+    // D layout of the 16x16 tile: lane l, register v -> row 4*v + l/16, column l % 16
     double* out = Ypart + (size_t)split * n * NR;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t gr = row0 + 32 * w + 16 * a + 4 * (lane >> 4) + v;
+            const int64_t gr = row0 + 32 * w + 16 * a + 4 * v + (lane >> 4);
             if (gr < n) {
 #pragma unroll
                 for (int b = 0; b < NT; ++b) out[gr * NR + 16 * b + (lane & 15)] = acc[a][b][v];
