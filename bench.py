@@ -252,9 +252,10 @@ def run_c4(args):
             if fmt != "fp64":
                 batch.convert("fp16")
             if fmt == "fp64":
-                sts = batch.solve("fp64", restart=100, max_cycles=10, tolerance=1e-8)
+                sts = batch.solve("fp64", restart=100, max_cycles=10, tolerance=1e-8, use_graph=not args.no_graph)
             else:
-                sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4)
+                sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
+                                  use_graph=not args.no_graph)
             ctx.synchronize()
             el = time.perf_counter() - t0
         if not all(q.converged for q in sts):
@@ -335,7 +336,7 @@ def run_c5(args):
     net = wsm.WarmStartMLP(ctx, width=512, seed=20261204)
     net.set_dataset(np.concatenate(inputs), np.concatenate(targets))
     del inputs, targets
-    steps, batch_size = 4000, 16384
+    steps, batch_size = 16000, 16384
     ctx.synchronize()
     t0 = time.perf_counter()
     losses = net.train(steps, batch=batch_size, lr_max=2e-3, lr_min=2e-5, seed=7)
@@ -354,9 +355,11 @@ def run_c5(args):
     ks = 0.5 * (kgrid[0::2] + kgrid[1::2])
     d = np.array([0.0, 0.0, 1.0])
     res = {}
+    x_zero = {}
     for mode in ("zero", "previous", "learned"):
-        iters, errs, used = [], [], 0
+        iters, used = [], 0
         prev = None
+        guess_err = []
         ctx.synchronize()
         t0 = time.perf_counter()
         for k in ks:
@@ -375,10 +378,18 @@ def run_c5(args):
                 raise SystemExit(f"bench c5: {mode} solve at k={k} did not converge")
             iters.append(sol.iterations)
             prev = sol.x
+            if mode == "zero":
+                x_zero[float(k)] = sol.x
+            elif x0 is not None:
+                ref = x_zero[float(k)]
+                guess_err.append(float(np.linalg.norm(x0 - ref) / np.linalg.norm(ref)))
         ctx.synchronize()
         el = time.perf_counter() - t0
         res[mode] = {"rhs_per_s": len(ks) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
                      "max_iterations": int(np.max(iters)), "guesses_used": used}
+        if guess_err:
+            res[mode]["guess_rel_error_median"] = float(np.median(guess_err))
+            res[mode]["guess_rel_error_max"] = float(np.max(guess_err))
         print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}", file=sys.stderr, flush=True)
     # chosen form: groups of frequencies as batch instances on the same mesh, assembled in
     # one persistent launch, started from the network's guesses, solved by lock-step
@@ -388,20 +399,30 @@ def run_c5(args):
     t0 = time.perf_counter()
     batch = sa.Batch(ctx, tms_batch)
     batch_setup_s = time.perf_counter() - t0
-    iters, residuals, used = [], [], 0
+    iters, residuals = [], []
+    parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
     ctx.synchronize()
     t0 = time.perf_counter()
     for g0 in range(0, len(ks), group):
         kg = ks[g0: g0 + group]
         if len(kg) < group:
             break
+        ta = time.perf_counter()
         batch.set_problem([float(k) for k in kg], [d.reshape(1, 3)] * group)
         batch.assemble()
         batch.convert("fp16")
-        for i, k in enumerate(kg):
-            batch.set_guess(i, net.guess(c, nn, float(k), d).reshape(n, 1))
+        ctx.synchronize()
+        tb = time.perf_counter()
+        X0 = net.guesses(c, nn, kg, d)
+        for i in range(group):
+            batch.set_guess(i, X0[i].reshape(n, 1))
+        tc = time.perf_counter()
         sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
                           use_guess=True)
+        td = time.perf_counter()
+        parts["assemble_convert"] += tb - ta
+        parts["guesses"] += tc - tb
+        parts["solve"] += td - tc
         if not all(q.converged for q in sts):
             raise SystemExit("bench c5: batched chosen form left a frequency unconverged")
         iters += [q.iterations for q in sts]
@@ -412,7 +433,8 @@ def run_c5(args):
     res["chosen_batched_fp16_ir_learned"] = {"rhs_per_s": nsolved / el, "seconds": el,
                                              "mean_iterations": float(np.mean(iters)),
                                              "max_residual": float(np.max(residuals)), "group": group,
-                                             "batch_setup_s": batch_setup_s}
+                                             "batch_setup_s": batch_setup_s,
+                                             "breakdown_s": parts}
     print(f"c5: chosen {el:.2f} s, mean Arnoldi steps {np.mean(iters):.1f}", file=sys.stderr, flush=True)
     for mode in ("previous", "learned", "chosen_batched_fp16_ir_learned"):
         res[mode]["vs_zero"] = res[mode]["rhs_per_s"] / res["zero"]["rhs_per_s"]
@@ -436,6 +458,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chosen-form", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched solves launch every kernel directly instead of replaying a captured cycle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 3 plain device copies of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <memory>
 #include <thread>
+#include <type_traits>
 
 #include "batch.hpp"
 
@@ -58,11 +59,13 @@ struct sbh_batch {
     sbh::DeviceArray<unsigned long long> d_t0;
     std::vector<int> col_inst, col_q;
     int nitems = 0, ntiles = 0;
+    std::vector<int2> tile_host;
     // vector-kernel chunks (depend on the tuning)
     sbh::DeviceArray<sbh::VecChunk> d_chunks;
     sbh::DeviceArray<int2> d_chunk_range;
     int nchunks = 0;
     int chunks_for = 0;  // chunk_elems the chunk table was built for
+    std::vector<sbh::VecChunk> chunk_host;
     // tuning
     sbh_batch_tuning tuning{16, 4, 4096, 0};
     int cu_count = 256;
@@ -116,10 +119,16 @@ void build_chunks(sbh_batch* b) {
         range[i].y = (int)chunks.size();
     }
     SBH_REQUIRE(chunks.size() < (size_t)INT32_MAX, "too many vector chunks");
+    b->chunks_for = b->tuning.chunk_elems;
+    const bool same = chunks.size() == b->chunk_host.size() &&
+                      std::equal(chunks.begin(), chunks.end(), b->chunk_host.begin(), [](const VecChunk& x, const VecChunk& y) {
+                          return x.inst == y.inst && x.rows == y.rows && x.r0 == y.r0;
+                      });
+    if (same) return;  // the solver workspace and its captured graph stay valid
     upload(b->d_chunks, chunks);
     upload(b->d_chunk_range, range);
-    b->nchunks = (int)chunks.size();
-    b->chunks_for = b->tuning.chunk_elems;
+    b->chunk_host = std::move(chunks);
+    b->nchunks = (int)b->chunk_host.size();
     b->solver.reset();  // partial-sum buffers are sized by the chunk count
 }
 
@@ -148,8 +157,14 @@ void build_tiles(sbh_batch* b) {
     std::vector<int2> tiles;
     for (int i : size_order(b))
         for (int64_t r0 = 0; r0 < b->inst[i].n; r0 += 128) tiles.push_back(make_int2(i, (int)r0));
+    const bool same = tiles.size() == b->tile_host.size() &&
+                      std::equal(tiles.begin(), tiles.end(), b->tile_host.begin(),
+                                 [](const int2& x, const int2& y) { return x.x == y.x && x.y == y.y; });
+    if (same) return;
     upload(b->d_tiles, tiles);
-    b->ntiles = (int)tiles.size();
+    b->tile_host = std::move(tiles);
+    b->ntiles = (int)b->tile_host.size();
+    b->solver.reset();  // a captured graph holds the tile table's address
 }
 
 RaggedLayout layout(const sbh_batch* b) {
@@ -432,15 +447,29 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
         b->L = vec_off;
         b->a_elems = a_off;
         b->lp_elems = lp_off;
-        upload(b->d_inst, b->inst);
-        b->d_dirs.allocate((size_t)dir_off * 3);
+        // same-sized tables are overwritten in place: a captured solve graph holds their addresses
+        auto put = [&](auto& d, const auto& h) {
+            using T = typename std::decay_t<decltype(h)>::value_type;
+            if (d.count != h.size()) {
+                upload(d, h);
+                b->solver.reset();
+            } else if (!h.empty()) {
+                SBH_HIP(hipMemcpy(d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+            }
+        };
+        put(b->d_inst, b->inst);
+        put(b->d_col_inst, b->col_inst);
+        put(b->d_col_q, b->col_q);
+        if (b->d_dirs.count != (size_t)dir_off * 3) b->d_dirs.allocate((size_t)dir_off * 3);
         SBH_HIP(hipMemcpy(b->d_dirs.ptr, directions, (size_t)dir_off * 3 * sizeof(double), hipMemcpyHostToDevice));
-        upload(b->d_col_inst, b->col_inst);
-        upload(b->d_col_q, b->col_q);
-        if (b->d_A.count != (size_t)a_off) b->d_A.allocate((size_t)a_off);
+        if (b->d_A.count != (size_t)a_off) {
+            b->d_A.allocate((size_t)a_off);
+            b->solver.reset();
+        }
         if (b->d_rhs.count != (size_t)vec_off) {
             b->d_rhs.allocate((size_t)vec_off);
             b->d_x.allocate((size_t)vec_off);
+            b->solver.reset();
         }
         SBH_HIP(hipMemset(b->d_x.ptr, 0, b->d_x.bytes()));
         b->lp_format = -2;
@@ -469,6 +498,7 @@ sbh_status sbh_batch_convert(sbh_batch* b, int format) {
         if (b->d_Alp.count != (size_t)b->lp_elems) {
             b->d_Alp.allocate((size_t)b->lp_elems);
             b->d_row_inv.allocate((size_t)b->total_panels);
+            b->solver.reset();
         }
         batch_lp_convert(b->ctx, format, b->total_panels, b->d_row_inst.ptr, b->d_inst.ptr, b->d_A.ptr, b->d_Alp.ptr,
                          b->d_row_inv.ptr);

@@ -51,13 +51,17 @@ constexpr int GLP_ROW = 128 + 8;  // LDS row (halves) of the transposed block op
 
 // Y_b = diag(row_inv) (A_lp,b X_b) on the instance's 128-row tile.  X holds
 // Krylov basis vectors (unit 2-norm per column), so its entries are converted
-// to the format without a scale; the row-scaled A keeps |A_ij| < 2^15.
+// to the format without a scale; the row-scaled A keeps |A_ij| < 2^15.  Each
+// stage of 128 real k moves the tile's rows of A (16 lanes per 256-byte row
+// segment) and the formed operand rows into LDS; the next stage is loaded
+// into registers while the current one runs on MFMA.
 template <int FMT>
 __global__ __launch_bounds__(BLOCK) void glp_gemm_kernel(const int2* __restrict__ tiles,
                                                          const BatchInstance* __restrict__ inst,
                                                          const int* __restrict__ flag, const u16* __restrict__ Alp,
                                                          const double* __restrict__ row_inv,
                                                          const double2* __restrict__ X, double2* __restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) u16 As[GP_BM * GLP_ROW];
     __shared__ __attribute__((aligned(16))) u16 Bs[2 * MAXCOL * GLP_ROW];
     const int2 tile = tiles[blockIdx.x];
     if (!flag[tile.x]) return;  // uniform over the block
@@ -66,40 +70,60 @@ __global__ __launch_bounds__(BLOCK) void glp_gemm_kernel(const int2* __restrict_
     const int wave = threadIdx.x >> 6;
     const int64_t n = I.n, ld = I.ld;
     const int kb = I.ncol;
-    const int64_t row_base = (int64_t)tile.y + wave * 32;
-    const int64_t arow = min(row_base + (lane & 31), n - 1);
-    const int kq = (lane >> 5) * 8;
-    const u32x4* ap = reinterpret_cast<const u32x4*>(Alp + I.lp_off + arow * 2 * ld + kq);
+    const int64_t row0 = tile.y;
+    const int ar = threadIdx.x >> 4;
+    const int aq = threadIdx.x & 15;
+    const u16* arow[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) arow[r] = Alp + I.lp_off + min(row0 + ar + 16 * r, n - 1) * 2 * ld + aq * 8;
     const double2* Xb = X + I.vec_off;
+    // operand stage: complex rows j0 .. j0+63 of X_b, 16 columns; thread takes 4 (row, column) pairs
+    u32x4 ra[8];
+    double2 rx[4];
+    auto load = [&](int64_t kc) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) ra[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(arow[r] + kc));
+        const int64_t j0 = kc / 2;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = threadIdx.x + i * BLOCK;
+            const int jl = e / MAXCOL, c = e % MAXCOL;
+            const int64_t j = j0 + jl;
+            rx[i] = (j < n && c < kb) ? Xb[j * kb + c] : make_double2(0.0, 0.0);
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    const int kq = (lane >> 5) * 8;
+    load(0);
     for (int64_t kc = 0; kc < 2 * ld; kc += 128) {
-        // complex rows j0 .. j0+63 of X_b -> real k rows kc .. kc+127 of the operand,
-        // stored per real output column: column 2c is Re, 2c+1 is Im of column c.
-        const int64_t j0 = kc / 2;
-        for (int e = threadIdx.x; e < 64 * MAXCOL; e += BLOCK) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) *reinterpret_cast<u32x4*>(&As[(ar + 16 * r) * GLP_ROW + aq * 8]) = ra[r];
+        // operand column 2c is Re, 2c+1 is Im of output column c:
+        // rows (2j, 2j+1) = (Re x, -Im x) and (Im x, Re x)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = threadIdx.x + i * BLOCK;
             const int jl = e / MAXCOL, c = e % MAXCOL;
-            const int64_t j = j0 + jl;
-            const double2 v = (j < n && c < kb) ? Xb[j * kb + c] : make_double2(0.0, 0.0);
-            const unsigned xr = Half<FMT>::from_float((float)v.x);
-            const unsigned xi = Half<FMT>::from_float((float)v.y);
-            const unsigned nxi = Half<FMT>::from_float((float)-v.y);
+            const unsigned xr = Half<FMT>::from_float((float)rx[i].x);
+            const unsigned xi = Half<FMT>::from_float((float)rx[i].y);
+            const unsigned nxi = Half<FMT>::from_float((float)-rx[i].y);
             *reinterpret_cast<unsigned*>(&Bs[(2 * c) * GLP_ROW + 2 * jl]) = xr | (nxi << 16);
             *reinterpret_cast<unsigned*>(&Bs[(2 * c + 1) * GLP_ROW + 2 * jl]) = xi | (xr << 16);
         }
-        u32x4 a[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) a[s] = __builtin_nontemporal_load(ap + (kc + 16 * s) / 8);
         __syncthreads();
+        if (kc + 128 < 2 * ld) load(kc + 128);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(&As[(wave * 32 + (lane & 31)) * GLP_ROW + s * 16 + kq]);
             const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[(lane & 31) * GLP_ROW + s * 16 + kq]);
-            acc = mfma32<FMT>(a[s], b, acc);
+            acc = mfma32<FMT>(a, b, acc);
         }
         __syncthreads();
     }
     // accumulator layout: column lane % 32, row 8*(v/4) + 4*(lane/32) + v%4
+    const int64_t row_base = row0 + wave * 32;
     const int col = lane & 31;
     const int c = col >> 1;
 #pragma unroll

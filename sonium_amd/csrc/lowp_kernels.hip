@@ -172,11 +172,12 @@ __global__ void x_to_lp_kernel(const double2* __restrict__ X, int64_t n, int k, 
 // Split-K MFMA product with FP32 accumulation.  Block = 4 waves x 32 rows
 // (128 rows) and one of LP_SPLIT K slabs; slab = blockIdx % LP_SPLIT, so on
 // the 8-XCD part every XCD streams one slab of the transposed block operand,
-// which stays in that XCD's L2.  The slab is staged through LDS 128 k at a
-// time (padded rows, conflict-free 16-byte reads) and shared by the 4 waves;
-// each wave streams its 32 rows of A straight into MFMA operands.  Partial
-// sums go to Ypart[slab][row][col] and lp_reduce_kernel adds the slabs in a
-// fixed order.
+// which stays in that XCD's L2.  Each stage of LP_BK k moves the block's 128
+// rows of A (16 lanes per 256-byte row segment, fully coalesced) and the
+// matching XT columns into LDS (rows padded to 136 halves); the next stage is
+// loaded into registers while the current one runs on MFMA.  Partial sums go
+// to Ypart[slab][row][col] and lp_reduce_kernel adds the slabs in a fixed
+// order.
 constexpr int LP_SPLIT = 8;
 constexpr int LP_BK = 128;
 constexpr int LP_LDS_ROW = LP_BK + 8;
@@ -185,45 +186,63 @@ template <int FMT, int NT>
 __global__ __launch_bounds__(BLOCK) void lp_gemm_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
                                                         const u16* __restrict__ XT, int bn,
                                                         float* __restrict__ Ypart) {
+    __shared__ __attribute__((aligned(16))) u16 As[128 * LP_LDS_ROW];
     __shared__ __attribute__((aligned(16))) u16 Bs[NT * 32 * LP_LDS_ROW];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int split = blockIdx.x % LP_SPLIT;
-    const int64_t row_base = (int64_t)(blockIdx.x / LP_SPLIT) * 128 + wave * 32;
+    const int64_t row_block = (int64_t)(blockIdx.x / LP_SPLIT) * 128;
     const int64_t slab = (2 * ld) / LP_SPLIT;  // a multiple of LP_BK (ld is a multiple of 512)
     const int64_t k_begin = split * slab;
-    const int64_t arow = min(row_base + (lane & 31), n - 1);
-    const int kq = (lane >> 5) * 8;
-    const u32x4* ap = reinterpret_cast<const u32x4*>(A + arow * 2 * ld + kq);
+    const int64_t k_end = k_begin + slab;
+    // global -> LDS mapping: thread t moves 16 bytes of row (t/16) + 16 r, column group t % 16
+    const int ar = threadIdx.x >> 4;
+    const int aq = threadIdx.x & 15;
+    const u16* arow[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) arow[r] = A + min(row_block + ar + 16 * r, n - 1) * 2 * ld + aq * 8;
+    constexpr int BR = (NT * 32 * (LP_BK / 8)) / BLOCK;  // 16-byte pieces of the XT stage per thread
+    u32x4 ra[8], rb[BR];
+    auto load = [&](int64_t kc) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) ra[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(arow[r] + kc));
+#pragma unroll
+        for (int i = 0; i < BR; ++i) {
+            const int e = threadIdx.x + i * BLOCK;
+            rb[i] = *reinterpret_cast<const u32x4*>(XT + (int64_t)(e >> 4) * 2 * ld + kc + (e & 15) * 8);
+        }
+    };
     f32x16 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
-    for (int64_t kc = k_begin; kc < k_begin + slab; kc += LP_BK) {
-        // stage XT[0 .. NT*32)[kc .. kc+LP_BK) into LDS (16 bytes per thread per round)
-        for (int e = threadIdx.x; e < NT * 32 * (LP_BK / 8); e += BLOCK) {
-            const int r = e / (LP_BK / 8);
-            const int q = e - r * (LP_BK / 8);
-            *reinterpret_cast<u32x4*>(&Bs[r * LP_LDS_ROW + q * 8]) =
-                *reinterpret_cast<const u32x4*>(XT + (int64_t)r * 2 * ld + kc + q * 8);
-        }
-        u32x4 a[LP_BK / 16];
+    const int kq = (lane >> 5) * 8;
+    load(k_begin);
+    for (int64_t kc = k_begin; kc < k_end; kc += LP_BK) {
 #pragma unroll
-        for (int s = 0; s < LP_BK / 16; ++s) a[s] = __builtin_nontemporal_load(ap + (kc + 16 * s) / 8);
+        for (int r = 0; r < 8; ++r) *reinterpret_cast<u32x4*>(&As[(ar + 16 * r) * LP_LDS_ROW + aq * 8]) = ra[r];
+#pragma unroll
+        for (int i = 0; i < BR; ++i) {
+            const int e = threadIdx.x + i * BLOCK;
+            *reinterpret_cast<u32x4*>(&Bs[(e >> 4) * LP_LDS_ROW + (e & 15) * 8]) = rb[i];
+        }
         __syncthreads();
+        if (kc + LP_BK < k_end) load(kc + LP_BK);
 #pragma unroll
         for (int s = 0; s < LP_BK / 16; ++s) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(&As[(wave * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[(t * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
-                acc[t] = mfma32<FMT>(a[s], b, acc[t]);
+                acc[t] = mfma32<FMT>(a, b, acc[t]);
             }
         }
         __syncthreads();
     }
     // C layout of v_mfma_f32_32x32x*: column = lane % 32,
     // row = 8*(v/4) + 4*(lane/32) + (v%4) for accumulator register v.
+    const int64_t row_base = row_block + wave * 32;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int col = t * 32 + (lane & 31);
@@ -481,12 +500,26 @@ __global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restr
     }
 }
 
-// out[o] = sum_q part[q * nout + o] for o < nout (fixed order over the chunks)
-__global__ void bsum_kernel(const double* __restrict__ part, int chunks, int64_t nout, double* __restrict__ out) {
+// Column sums of chunk partials in two coalesced passes with a fixed order:
+// group g of G adds part[q * nout + o] over q = g, g + G, ...; then
+// out[o] = sum over g of the group sums.
+constexpr int BSUM_GROUPS = 16;
+
+__global__ void bsum_groups_kernel(const double* __restrict__ part, int chunks, int64_t nout,
+                                   double* __restrict__ tmp) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout) return;
     double s = 0.0;
-    for (int q = 0; q < chunks; ++q) s += part[(int64_t)q * nout + o];
+    for (int q = blockIdx.y; q < chunks; q += BSUM_GROUPS) s += part[(int64_t)q * nout + o];
+    tmp[(int64_t)blockIdx.y * nout + o] = s;
+}
+
+__global__ void bsum_final_kernel(const double* __restrict__ tmp, int64_t nout, double* __restrict__ out) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout) return;
+    double s = 0.0;
+#pragma unroll
+    for (int g = 0; g < BSUM_GROUPS; ++g) s += tmp[(int64_t)g * nout + o];
     out[o] = s;
 }
 
@@ -635,6 +668,17 @@ static int batched_chunks(int64_t n, int k, int64_t* len) {
     return (int)chunks;
 }
 
+// out[o] = sum over the chunks of part[q * nout + o]; the group sums go right
+// after the partials in the same scratch buffer.
+static void column_sums(sbh_context* ctx, double* part, int chunks, int64_t nout, double* out) {
+    double* tmp = part + (int64_t)chunks * nout;
+    hipLaunchKernelGGL(bsum_groups_kernel, dim3(nblocks(nout), BSUM_GROUPS), dim3(BLOCK), 0, ctx->stream, part,
+                       chunks, nout, tmp);
+    SBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, tmp, nout, out);
+    SBH_HIP(hipGetLastError());
+}
+
 // Partials land in `scratch` (chunks x (nvec+1) x k complex); the reduced
 // (nvec+1) x k complex values land in `out`.
 void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
@@ -644,10 +688,8 @@ void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const doub
     hipLaunchKernelGGL(bdots_kernel, dim3((unsigned)chunks, (unsigned)(nvec + 1)), dim3(BLOCK), 0, ctx->stream, V, ldv,
                        nvec, W, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
-    const int64_t nout = (int64_t)(nvec + 1) * k * 2;
-    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream,
-                       reinterpret_cast<const double*>(scratch), chunks, nout, reinterpret_cast<double*>(out));
-    SBH_HIP(hipGetLastError());
+    column_sums(ctx, reinterpret_cast<double*>(scratch), chunks, (int64_t)(nvec + 1) * k * 2,
+                reinterpret_cast<double*>(out));
 }
 
 // W -= sum C_i V_i per column; the reduced k squared norms land in `out`.
@@ -658,8 +700,7 @@ void bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const d
     hipLaunchKernelGGL(bproject_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n, k,
                        len, scratch);
     SBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(k)), dim3(BLOCK), 0, ctx->stream, scratch, chunks, (int64_t)k, out);
-    SBH_HIP(hipGetLastError());
+    column_sums(ctx, scratch, chunks, (int64_t)k, out);
 }
 
 // R = B - R (B may be null: norms only); the reduced k squared norms land in `out`.
@@ -668,14 +709,13 @@ void bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k,
     const int chunks = batched_chunks(n, k, &len);
     hipLaunchKernelGGL(bresidual_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(bsum_kernel, dim3(nblocks(k)), dim3(BLOCK), 0, ctx->stream, scratch, chunks, (int64_t)k, out);
-    SBH_HIP(hipGetLastError());
+    column_sums(ctx, scratch, chunks, (int64_t)k, out);
 }
 
 int64_t batched_scratch_elems(int64_t n, int k, int max_vec) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    return (int64_t)chunks * (max_vec + 1) * k;  // complex elements
+    return (int64_t)(chunks + BSUM_GROUPS) * (max_vec + 1) * k;  // complex elements: partials + group sums
 }
 
 void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,

@@ -101,6 +101,14 @@ class WarmStartMLP:
         y = self.predict(panel_inputs(centers, normals, k, direction))
         return target_to_density(y, centers, k, direction)
 
+    def guesses(self, centers, normals, ks, direction) -> np.ndarray:
+        """Initial guesses for several wavenumbers in one forward pass: (len(ks), N)."""
+        n = centers.shape[0]
+        raw = np.concatenate([panel_inputs(centers, normals, float(k), direction) for k in ks])
+        y = self.predict(raw).reshape(len(ks), n, 2).astype(np.float64)
+        kd = np.asarray(ks, dtype=np.float64)[:, None] * (centers @ np.asarray(direction, dtype=np.float64).reshape(3))[None, :]
+        return (y[..., 0] + 1j * y[..., 1]) * np.exp(1j * kd)
+
     def close(self):
         if getattr(self, "_h", None):
             _lib().sbh_mlp_destroy(self._h)
