@@ -359,7 +359,7 @@ def run_c5(args):
     for mode in ("zero", "previous", "learned"):
         iters, used = [], 0
         prev = None
-        guess_err = []
+        guess_err, guess_res = [], []
         ctx.synchronize()
         t0 = time.perf_counter()
         for k in ks:
@@ -372,12 +372,14 @@ def run_c5(args):
                 x0 = prev
             elif mode == "learned":
                 x0 = net.guess(c, nn, float(k), d)
-            sol, u = wsm.solve_with_guess(op, b, x0, cfg)
+            sol, u, r0 = wsm.solve_with_guess(op, b, x0, cfg, with_residual=True)
             used += int(u)
             if not (sol.converged and sol.residual <= 1e-8):
                 raise SystemExit(f"bench c5: {mode} solve at k={k} did not converge")
             iters.append(sol.iterations)
             prev = sol.x
+            if x0 is not None:
+                guess_res.append(r0)
             if mode == "zero":
                 x_zero[float(k)] = sol.x
             elif x0 is not None:
@@ -390,6 +392,7 @@ def run_c5(args):
         if guess_err:
             res[mode]["guess_rel_error_median"] = float(np.median(guess_err))
             res[mode]["guess_rel_error_max"] = float(np.max(guess_err))
+            res[mode]["guess_rel_residual_median"] = float(np.median(guess_res))
         print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}", file=sys.stderr, flush=True)
     # chosen form: groups of frequencies as batch instances on the same mesh, assembled in
     # one persistent launch, started from the network's guesses, solved by lock-step

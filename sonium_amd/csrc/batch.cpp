@@ -59,6 +59,7 @@ struct sbh_batch {
     sbh::DeviceArray<unsigned long long> d_t0;
     std::vector<int> col_inst, col_q;
     int nitems = 0, ntiles = 0;
+    int max_cols = 16;  // largest right-hand-side count of any instance
     std::vector<int2> tile_host;
     // vector-kernel chunks (depend on the tuning)
     sbh::DeviceArray<sbh::VecChunk> d_chunks;
@@ -222,7 +223,7 @@ void cycle(sbh_batch* b, int format, double tol, double inner_tol) {
     const size_t L = (size_t)b->L;
     auto Vj = [&](int j) { return s.V.ptr + (size_t)j * L; };
     for (int j = 0; j < m; ++j) {
-        batch_product(ctx, format, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_active.ptr, b->d_A.ptr,
+        batch_product(ctx, format, b->max_cols, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_active.ptr, b->d_A.ptr,
                       b->d_Alp.ptr, b->d_row_inv.ptr, Vj(j), s.W.ptr);
         ragged_dots(ctx, lay, s.inst_active.ptr, s.V.ptr, j + 1, s.W.ptr, s.part.ptr);
         ragged_sum_dots(ctx, lay, s.part.ptr, j + 1, j, s.active.ptr, s.reorth.ptr, 0, s.H.ptr, s.coef.ptr,
@@ -245,7 +246,7 @@ void cycle(sbh_batch* b, int format, double tol, double inner_tol) {
     }
     ragged_solve(ctx, lay, s.H.ptr, s.g.ptr, s.jlen.ptr, s.coef.ptr);
     ragged_update(ctx, lay, s.inst_pending.ptr, s.V.ptr, s.jlen.ptr, s.coef.ptr, b->d_x.ptr);
-    batch_product(ctx, -1, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
+    batch_product(ctx, -1, b->max_cols, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
                   nullptr, b->d_x.ptr, s.R.ptr);
     ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
     ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 0, tol, inner_tol, refinement, s.bnorm.ptr,
@@ -445,6 +446,10 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
         }
         b->C = col;
         b->L = vec_off;
+        int maxc = 1;
+        for (const auto& I : b->inst) maxc = std::max(maxc, (int)I.ncol);
+        if (maxc != b->max_cols) b->solver.reset();  // the captured product kernels depend on it
+        b->max_cols = maxc;
         b->a_elems = a_off;
         b->lp_elems = lp_off;
         // same-sized tables are overwritten in place: a captured solve graph holds their addresses
@@ -537,7 +542,7 @@ sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_
         if (cfg->use_guess) {
             ragged_flags(ctx, lay, s.active.ptr, s.reorth.ptr, s.done.ptr, s.inst_active.ptr, s.inst_reorth.ptr,
                          s.inst_pending.ptr);
-            batch_product(ctx, -1, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
+            batch_product(ctx, -1, b->max_cols, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, s.inst_pending.ptr, b->d_A.ptr, nullptr,
                           nullptr, b->d_x.ptr, s.R.ptr);
             ragged_residual(ctx, lay, s.inst_pending.ptr, b->d_rhs.ptr, s.R.ptr, s.part.ptr);
             ragged_restart(ctx, lay, s.part.ptr, s.inst_pending.ptr, 2, cfg->tolerance, cfg->inner_tolerance,

@@ -26,9 +26,10 @@ struct RaggedLayout {
 void batch_lp_convert(sbh_context* ctx, int format, int64_t rows, const int* row_inst, const BatchInstance* inst,
                       const double2* A, uint16_t* Alp, double* row_inv);
 // format < 0: FP64 product; 0/1: FP16/BF16 copy with FP32 accumulation.
-void batch_product(sbh_context* ctx, int format, int ntiles, const int2* tiles, const BatchInstance* inst,
-                   const int* flag, const double2* A, const uint16_t* Alp, const double* row_inv, const double2* X,
-                   double2* Y);
+// max_cols: the largest right-hand-side count of any instance (1..16).
+void batch_product(sbh_context* ctx, int format, int max_cols, int ntiles, const int2* tiles,
+                   const BatchInstance* inst, const int* flag, const double2* A, const uint16_t* Alp,
+                   const double* row_inv, const double2* X, double2* Y);
 void ragged_dots(sbh_context* ctx, const RaggedLayout& lay, const int* flag, const double2* V, int nvec,
                  const double2* W, double2* part);
 void ragged_sum_dots(sbh_context* ctx, const RaggedLayout& lay, const double2* part, int nvec, int j,

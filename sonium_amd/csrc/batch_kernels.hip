@@ -138,18 +138,19 @@ __global__ __launch_bounds__(BLOCK) void glp_gemm_kernel(const int2* __restrict_
     }
 }
 
-// FP64: Y_b = A_b X_b on the instance's 128-row tile, 16 complex columns
-// (32 real) per tile; wave w owns rows 32w..32w+31 as 2 x 2 tiles of 16 x 16.
+// FP64: Y_b = A_b X_b on the instance's 128-row tile, 8*NT complex columns
+// (16*NT real) per tile; wave w owns rows 32w..32w+31 as 2 x NT tiles of 16 x 16.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int G64_BKC = 8;           // complex columns of A per stage
 constexpr int G64_AROW = GP_BM + 2;  // LDS row of the transposed A stage
 
+template <int NT>
 __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict__ tiles,
                                                          const BatchInstance* __restrict__ inst,
                                                          const int* __restrict__ flag, const double2* __restrict__ A,
                                                          const double2* __restrict__ X, double2* __restrict__ Y) {
     __shared__ double As[2 * G64_BKC][G64_AROW];
-    __shared__ __attribute__((aligned(16))) double Xs[2 * G64_BKC][2 * MAXCOL];
+    __shared__ __attribute__((aligned(16))) double Xs[2 * G64_BKC][16 * NT];
     const int2 tile = tiles[blockIdx.x];
     if (!flag[tile.x]) return;
     const BatchInstance& I = inst[tile.x];
@@ -161,11 +162,11 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
     const int64_t row0 = tile.y;
     const double2* Ab = A + I.a_off;
     const double2* Xb = X + I.vec_off;
-    f64x4 acc[2][2];
+    f64x4 acc[2][NT];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < NT; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int64_t k0 = 0; k0 < n; k0 += G64_BKC) {
 #pragma unroll
         for (int q = 0; q < (GP_BM * G64_BKC) / BLOCK; ++q) {
@@ -176,8 +177,8 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
             As[2 * cc][r] = v.x;
             As[2 * cc + 1][r] = v.y;
         }
-        if (t < G64_BKC * MAXCOL) {
-            const int j = t / MAXCOL, c = t % MAXCOL;
+        if (t < G64_BKC * 8 * NT) {
+            const int j = t / (8 * NT), c = t % (8 * NT);
             const int64_t gk = k0 + j;
             const double2 v = (gk < n && c < kb) ? Xb[gk * kb + c] : make_double2(0.0, 0.0);
             *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = v;
@@ -187,15 +188,15 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
 #pragma unroll
         for (int ks = 0; ks < (2 * G64_BKC) / 4; ++ks) {
             const int kr = 4 * ks + (lane >> 4);
-            double av[2], bv[2];
+            double av[2], bv[NT];
 #pragma unroll
             for (int a = 0; a < 2; ++a) av[a] = As[kr][32 * w + 16 * a + (lane & 15)];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = Xs[kr][16 * b + (lane & 15)];
+            for (int b = 0; b < NT; ++b) bv[b] = Xs[kr][16 * b + (lane & 15)];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < NT; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
         for (int v = 0; v < 4; ++v) {
             const int64_t gr = row0 + 32 * w + 16 * a + 4 * v + (lane >> 4);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
+            for (int b = 0; b < NT; ++b) {
                 const double mine = acc[a][b][v];
                 const double other = __shfl_xor(mine, 1, 64);
                 const int c = (16 * b + col) >> 1;
@@ -634,13 +635,16 @@ void batch_lp_convert(sbh_context* ctx, int format, int64_t rows, const int* row
     SBH_HIP(hipGetLastError());
 }
 
-void batch_product(sbh_context* ctx, int format, int ntiles, const int2* tiles, const BatchInstance* inst,
-                   const int* flag, const double2* A, const uint16_t* Alp, const double* row_inv, const double2* X,
-                   double2* Y) {
+void batch_product(sbh_context* ctx, int format, int max_cols, int ntiles, const int2* tiles,
+                   const BatchInstance* inst, const int* flag, const double2* A, const uint16_t* Alp,
+                   const double* row_inv, const double2* X, double2* Y) {
     if (ntiles == 0) return;
-    if (format < 0)
-        hipLaunchKernelGGL(g64_gemm_kernel, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag, A,
-                           X, Y);
+    if (format < 0 && max_cols <= 8)
+        hipLaunchKernelGGL(g64_gemm_kernel<1>, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag,
+                           A, X, Y);
+    else if (format < 0)
+        hipLaunchKernelGGL(g64_gemm_kernel<2>, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag,
+                           A, X, Y);
     else if (format == 0)
         hipLaunchKernelGGL(glp_gemm_kernel<0>, dim3((unsigned)ntiles), dim3(BLOCK), 0, ctx->stream, tiles, inst, flag,
                            Alp, row_inv, X, Y);

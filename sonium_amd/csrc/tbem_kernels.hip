@@ -542,7 +542,8 @@ __global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ 
     if (p >= npairs) return;
     const NearPair np = pairs[p];
     const PanelBc bc = general ? bcs[np.j] : rigid_bc();
-    const double2 v = near_entry(np, leaves, geom[np.j], sources[np.i], bc, k, wavruim, tau_r, beta, iwr, lane);
+    const PanelGeom G = geom[np.j];  // constant indexing only: lives in registers
+    const double2 v = near_entry(np, leaves, G, sources[np.i], bc, k, wavruim, tau_r, beta, iwr, lane);
     if (lane == 0) A[(int64_t)np.i * lda + np.j] = v;
 }
 
@@ -713,8 +714,8 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
             const int64_t pe = args.row_ptr[I.panel_off + r1];
             for (int64_t p = pb + wave; p < pe; p += 4) {
                 const NearPair np = args.pairs[p];
-                const double2 v =
-                    near_entry(np, args.leaves, geom[np.j], src[np.i], bc, k, k, I.tau, beta, iwr, lane);
+                const PanelGeom G = geom[np.j];
+                const double2 v = near_entry(np, args.leaves, G, src[np.i], bc, k, k, I.tau, beta, iwr, lane);
                 if (lane == 0) A[(int64_t)np.i * n + np.j] = v;
             }
             for (int64_t i = r0 + wave; i < r1; i += 4) {

@@ -121,11 +121,19 @@ class WarmStartMLP:
             pass
 
 
-def solve_with_guess(op: DenseMatrixOperator, b, x0: Optional[np.ndarray], config: GmresConfig):
+def solve_with_guess(op: DenseMatrixOperator, b, x0: Optional[np.ndarray], config: GmresConfig,
+                     with_residual: bool = False):
     """gmres_with_guess with the zero-guess fallback: a guess whose residual is not
-    below ||b|| is dropped.  Returns (solution, used_guess)."""
+    below ||b|| is dropped.  Returns (solution, used_guess), plus the guess's
+    relative residual ||b - A x0|| / ||b|| (1.0 without a guess) when asked."""
+    rel = 1.0
+    sol = None
     if x0 is not None:
         r0 = np.linalg.norm(b - op.apply(x0))
-        if np.isfinite(r0) and r0 < np.linalg.norm(b):
-            return gmres_with_guess(op, b, x0, config), True
-    return gmres(op, b, config), False
+        rel = float(r0 / np.linalg.norm(b))
+        if np.isfinite(r0) and rel < 1.0:
+            sol = gmres_with_guess(op, b, x0, config)
+    used = sol is not None
+    if sol is None:
+        sol = gmres(op, b, config)
+    return (sol, used, rel) if with_residual else (sol, used)

@@ -147,3 +147,20 @@ def test_batch_initial_guess(ctx, small_batch):
         b.set_guess(i, np.zeros_like(b.solution(i)))
     again = b.solve("fp64", restart=50, max_cycles=10, tolerance=1e-8, use_guess=True)
     assert [s.iterations for s in again] == [s.iterations for s in base]
+
+
+def test_batch_narrow_columns_fp64_path(ctx):
+    """Batches whose instances all have <= 8 right-hand sides take the 8-column
+    FP64 product; results match LAPACK and the single-system assembly."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    specs = [{"mesh": ellipsoid(rng.uniform(0.6, 1.4, size=3), lvl), "k": k, "directions": uniform_directions(rng, nd)}
+             for lvl, k, nd in ((1, 2.0, 1), (2, 4.0, 8), (1, 5.5, 3))]
+    tms = [sa.TbemMesh(ctx, s["mesh"]) for s in specs]
+    b = sa.Batch(ctx, tms)
+    b.set_problem([s["k"] for s in specs], [s["directions"] for s in specs])
+    b.assemble()
+    sts = b.solve("fp64", restart=60, max_cycles=10, tolerance=1e-8)
+    _check_solution(b, specs, sts, 1e-8)
+    b.convert("fp16")
+    sts = b.solve("fp16_ir", restart=40, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4)
+    _check_solution(b, specs, sts, 1e-8)
