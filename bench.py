@@ -120,9 +120,9 @@ def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
             "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products}, x[:, 0]
 
 
-def run_c3(args):
+def run_c3(args, ctx=None):
     """Many right-hand sides: 40960-panel Fibonacci sphere, k = 14, 64 plane waves."""
-    ctx = sa.Context(0)
+    ctx = ctx or sa.Context(0)
     m = fibonacci_sphere(1.0, 129, 160)
     n = m.num_elements
     k = 14.0
@@ -179,9 +179,14 @@ def run_c3(args):
                             "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
         lp.close()
     ctx.set_profiling(False)
-    line = {"metric": "solved RHS/s at rel. residual 1e-8 (multi-RHS)", "workload": "C3: Fibonacci sphere 40960 panels, k=14, 64 plane waves",
-            "panels": n, "assembly_s": t_asm, "results": res}
-    print(json.dumps(line))
+    res["fp16_ir"]["vs_plain_excl_assembly"] = res["fp16_ir"]["rhs_per_s_excl_assembly"] / res["plain_fp64"]["rhs_per_s_excl_assembly"]
+    res["fp16_ir"]["vs_plain_incl_assembly"] = res["fp16_ir"]["rhs_per_s_incl_assembly"] / res["plain_fp64"]["rhs_per_s_incl_assembly"]
+    res["bf16_ir"]["vs_plain_excl_assembly"] = res["bf16_ir"]["rhs_per_s_excl_assembly"] / res["plain_fp64"]["rhs_per_s_excl_assembly"]
+    lp = res["fp16_ir"]
+    return {"metric": "solved RHS/s at rel. residual 1e-8 (multi-RHS)", "workload": "C3: Fibonacci sphere 40960 panels, k=14, 64 plane waves",
+            "panels": n, "assembly_s": t_asm, "results": res,
+            "roofline_lp_gemm": {"kernel": "lp_gemm_kernel", "bound": "hbm", "achieved": lp["lowp_gemm_gbs"],
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": lp["lowp_gemm_gbs"] / HBM_PEAK_GBS}}
 
 
 def _percentiles(values):
@@ -190,7 +195,7 @@ def _percentiles(values):
             "max_ms": float(v.max())}
 
 
-def run_c4(args):
+def run_c4(args, ctx=None):
     """256 independent rigid ellipsoids (320-5120 panels, k U[1,8], 1-16 plane waves).
 
     plain: the single-system FP64 path instance by instance (assemble, plane
@@ -202,7 +207,7 @@ def run_c4(args):
     """
     from sonium_amd.mesh import random_ellipsoid_batch
 
-    ctx = sa.Context(0)
+    ctx = ctx or sa.Context(0)
     specs = random_ellipsoid_batch(256, seed=20261202)
     tms = [sa.TbemMesh(ctx, s["mesh"]) for s in specs]
     ncols = np.array([len(s["directions"]) for s in specs])
@@ -281,7 +286,7 @@ def run_c4(args):
                                       "max_iterations": int(max(plain_iters))},
             "batched": forms, "autotune": {"best": best, "candidates": table, "seconds": tune_s},
             "batch_setup_s": setup_s}
-    print(json.dumps(line))
+    return line
 
 
 def run_c5(args):
@@ -461,6 +466,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chosen-form", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the multi-RHS (C3) and batch (C4) comparisons that the default run appends")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched solves launch every kernel directly instead of replaying a captured cycle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -471,8 +478,11 @@ def main():
         sa._check(sa._lib().sbh_debug_device_copy(ctx.handle, args.calibrate_copy, 3))
         print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 3}))
         return
-    if args.workload in ("c3", "c4", "c5"):
-        {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    if args.workload in ("c3", "c4"):
+        print(json.dumps({"c3": run_c3, "c4": run_c4}[args.workload](args)))
+        return
+    if args.workload == "c5":
+        run_c5(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -556,13 +566,35 @@ def main():
         chosen["error_vs_fp64"] = float(np.linalg.norm(x_ir - x64) / np.linalg.norm(x64))
         chosen["vs_plain_fp64"] = chosen["value"] / value
 
+    others = None
+    if not args.no_other_configs and args.workload == "c2":
+        c3 = run_c3(args, ctx)
+        c4 = run_c4(args, ctx)
+        others = {
+            "c3": {"workload": c3["workload"], "plain_fp64_rhs_per_s": c3["results"]["plain_fp64"]["rhs_per_s_excl_assembly"],
+                   "chosen_fp16_ir_rhs_per_s": c3["results"]["fp16_ir"]["rhs_per_s_excl_assembly"],
+                   "vs_plain_excl_assembly": c3["results"]["fp16_ir"]["vs_plain_excl_assembly"],
+                   "vs_plain_incl_assembly": c3["results"]["fp16_ir"]["vs_plain_incl_assembly"],
+                   "max_error_vs_fp64": c3["results"]["fp16_ir"]["max_error_vs_fp64"],
+                   "roofline_lp_gemm": c3["roofline_lp_gemm"],
+                   "fp64_block_product_tflops": 8 * c3["panels"] ** 2 * 64 / (c3["results"]["plain_fp64"]["fp64_block_product_avg_ms"] * 1e-3) / 1e12},
+            "c4": {"workload": c4["workload"], "plain_fp64_rhs_per_s": c4["plain_fp64_sequential"]["rhs_per_s"],
+                   "plain_p50_ms": c4["plain_fp64_sequential"]["p50_ms"], "plain_p99_ms": c4["plain_fp64_sequential"]["p99_ms"],
+                   "chosen_batched_fp16_ir_rhs_per_s": c4["batched"]["batched_fp16_ir"]["rhs_per_s"],
+                   "chosen_p50_ms": c4["batched"]["batched_fp16_ir"]["p50_ms"],
+                   "chosen_p99_ms": c4["batched"]["batched_fp16_ir"]["p99_ms"],
+                   "vs_plain": c4["batched"]["batched_fp16_ir"]["vs_plain"],
+                   "batched_fp64_vs_plain": c4["batched"]["batched_fp64"]["vs_plain"],
+                   "autotuned": c4["autotune"]["best"]},
+        }
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(os.cpu_count() or 1, 16))
         cpu = cpu_baseline(m, k, beta, sol.spmv_count, args.cpu_rows, threads)
 
     line = {
-        "metric": "solved RHS/s at rel. residual 1e-8 (plain FP64 on GPU)",
+        "metric": "solved RHS/s at rel. residual 1e-8, chosen form vs plain FP64 on GPU; % roofline",
         "value": value,
         "unit": "RHS/s",
         "n_gpus": 1,
@@ -583,6 +615,7 @@ def main():
         "kernels": kernels,
         "roofline": roofline,
         "chosen_form": chosen,
+        "other_configs": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -543,7 +543,8 @@ __global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ 
     const NearPair np = pairs[p];
     const PanelBc bc = general ? bcs[np.j] : rigid_bc();
     const PanelGeom G = geom[np.j];  // constant indexing only: lives in registers
-    const double2 v = near_entry(np, leaves, G, sources[np.i], bc, k, wavruim, tau_r, beta, iwr, lane);
+    const Source S = sources[np.i];
+    const double2 v = near_entry(np, leaves, G, S, bc, k, wavruim, tau_r, beta, iwr, lane);
     if (lane == 0) A[(int64_t)np.i * lda + np.j] = v;
 }
 
@@ -715,7 +716,8 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
             for (int64_t p = pb + wave; p < pe; p += 4) {
                 const NearPair np = args.pairs[p];
                 const PanelGeom G = geom[np.j];
-                const double2 v = near_entry(np, args.leaves, G, src[np.i], bc, k, k, I.tau, beta, iwr, lane);
+                const Source S = src[np.i];
+                const double2 v = near_entry(np, args.leaves, G, S, bc, k, k, I.tau, beta, iwr, lane);
                 if (lane == 0) A[(int64_t)np.i * n + np.j] = v;
             }
             for (int64_t i = r0 + wave; i < r1; i += 4) {
