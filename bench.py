@@ -566,6 +566,22 @@ def main():
         chosen["error_vs_fp64"] = float(np.linalg.norm(x_ir - x64) / np.linalg.norm(x64))
         chosen["vs_plain_fp64"] = chosen["value"] / value
 
+    # matrix-free exact operator on the same workspace (apply_tbem_stabilized_exact)
+    exact_op = sa.TbemExactOperator(ws, ph, beta)
+    xr = np.random.default_rng(0).standard_normal(n) + 1j * np.random.default_rng(1).standard_normal(n)
+    y_exact = exact_op.apply(xr)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        exact_op.apply(xr)
+    ctx.synchronize()
+    ex_ms = (time.perf_counter() - t0) / 3 * 1e3
+    y_dense = op.apply(xr)
+    exact_info = {"apply_ms": ex_ms, "tflops_far_algorithmic": far_flops / (ex_ms * 1e-3) / 1e12,
+                  "rel_diff_vs_dense": float(np.linalg.norm(y_exact - y_dense) / np.linalg.norm(y_dense)),
+                  "device_bytes": int(16 * (2 * n + st["near_pairs"]))}
+    exact_op.close()
+
     others = None
     if not args.no_other_configs and args.workload == "c2":
         c3 = run_c3(args, ctx)
@@ -615,6 +631,7 @@ def main():
         "kernels": kernels,
         "roofline": roofline,
         "chosen_form": chosen,
+        "exact_operator": exact_info,
         "other_configs": others,
         "cpu_baseline": cpu,
     }

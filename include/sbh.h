@@ -192,6 +192,16 @@ sbh_status sbh_incident_plane_wave_rhs(sbh_context* ctx, const sbh_mesh* mesh,
  * methods of traits.rs:2949-3290.  The operator borrows sys; sys must outlive it. */
 typedef enum { SBH_ADJOINT_UNKNOWN = 0, SBH_ADJOINT_UNSUPPORTED = 1, SBH_ADJOINT_EXACT = 2 } sbh_adjoint_capability;
 sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out);
+/* Matrix-free exact operator — apply_tbem_stabilized_exact
+ * (crates/sonium-bem/src/core/assembly/hmatrix.rs:2363-2425): y = sign * A x
+ * with A the matrix sbh_workspace_assemble(ws, physics, beta) would store,
+ * evaluated on the fly in O(N) device memory (each product costs about one
+ * assembly).  Usable with sbh_operator_apply(_block) and sbh_gmres; the
+ * transpose/Hermitian products, the Jacobi preconditioner, block GMRES and
+ * GMRES-IR need a dense operator and return SBH_ERR_UNSUPPORTED.  The
+ * workspace (and its mesh) must outlive the operator. */
+sbh_status sbh_exact_operator_create(sbh_workspace* workspace, const sbh_physics* physics, sbh_c64 beta,
+                                     double sign, sbh_operator** out);
 void sbh_operator_destroy(sbh_operator* op);
 int64_t sbh_operator_num_rows(const sbh_operator* op);
 int64_t sbh_operator_num_cols(const sbh_operator* op);

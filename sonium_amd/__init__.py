@@ -294,6 +294,10 @@ class TbemAssemblyWorkspace:
         self._h = h
 
     @property
+    def handle(self):
+        return self._h
+
+    @property
     def stats(self):
         a = ctypes.c_int64()
         b = ctypes.c_int64()
@@ -454,6 +458,23 @@ class DenseMatrixOperator:
             self.close()
         except Exception:
             pass
+
+
+class TbemExactOperator(DenseMatrixOperator):
+    """Matrix-free exact operator: y = sign * A(beta) x with the entries of the
+    dense assembly, evaluated on the fly (apply_tbem_stabilized_exact,
+    hmatrix.rs:2363-2425).  O(N) device memory; each product costs about one
+    assembly.  Transpose/Hermitian products are not available."""
+
+    def __init__(self, workspace: "TbemAssemblyWorkspace", physics: PhysicsParams, beta: Optional[complex] = None,
+                 sign: float = 1.0):
+        self.workspace = workspace
+        self.ctx = workspace.ctx
+        bv = physics.burton_miller_beta() if beta is None else complex(beta)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_exact_operator_create(workspace.handle, ctypes.byref(physics._p),
+                                                _ffi.c64(bv.real, bv.imag), float(sign), ctypes.byref(h)))
+        self._h = h
 
 
 class _Precond:
@@ -865,5 +886,5 @@ __all__ = [
     "IdentityPreconditioner", "JacobiPreconditioner", "GmresConfig", "GmresSolution", "gmres",
     "gmres_with_guess", "gmres_preconditioned", "gmres_preconditioned_with_guess", "gmres_device",
     "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE", "gmres_block", "LowpMatrix", "GmresIrConfig",
-    "GmresIrStatus", "gmres_ir", "Batch", "BatchColumnStatus",
+    "GmresIrStatus", "gmres_ir", "Batch", "BatchColumnStatus", "TbemExactOperator",
 ]

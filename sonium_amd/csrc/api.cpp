@@ -27,6 +27,10 @@ thread_local std::string g_last_error;
 
 inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
 
+void require_dense(const sbh_operator* op, const char* what) {
+    if (op->exact()) throw Error(SBH_ERR_UNSUPPORTED, std::string(what) + " needs a dense (stored) operator");
+}
+
 // Device copy of a host or device input vector.
 struct DeviceInput {
     DeviceArray<double2> owned;
@@ -76,6 +80,14 @@ bool all_finite(const sbh_c64* p, size_t n) {
 }
 
 }  // namespace
+
+// y = A x on device vectors, for either kind of operator.
+void operator_apply_device(const sbh_operator* op, const double2* x, double2* y) {
+    if (op->exact())
+        launch_exact_apply(op, x, y);
+    else
+        zgemv_n(op->ctx, op->sys->d_A.ptr, op->sys->lda, op->n, op->n, x, y);
+}
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
@@ -615,8 +627,37 @@ sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out) {
         sbh_operator* op = new sbh_operator();
         op->ctx = sys->ctx;
         op->sys = sys;
+        op->n = sys->n;
         context_retain(op->ctx);
         *out = op;
+    });
+}
+
+sbh_status sbh_exact_operator_create(sbh_workspace* ws, const sbh_physics* physics, sbh_c64 beta, double sign,
+                                     sbh_operator** out) {
+    return guard([&] {
+        SBH_REQUIRE(ws && physics && out, "workspace/physics/out is null");
+        SBH_REQUIRE(std::isfinite(sign) && std::isfinite(beta.re) && std::isfinite(beta.im),
+                    "sign and beta must be finite");
+        const sbh_mesh* m = ws->mesh;
+        SBH_REQUIRE(std::isfinite(physics->wave_number), "wave number is not finite");
+        SBH_REQUIRE(std::fabs(physics->wave_number) * m->diameter < 5.0e5,
+                    "k * mesh diameter exceeds the supported phase range (5e5 rad)");
+        if (physics->harmonic_factor != 1.0)
+            throw Error(SBH_ERR_UNSUPPORTED, "only harmonic_factor = +1 (ACOUSTIC_CONVENTION) is supported");
+        use_device(ws->ctx);
+        std::unique_ptr<sbh_operator> op(new sbh_operator());
+        op->ctx = ws->ctx;
+        op->ws = ws;
+        op->n = m->n;
+        op->physics = *physics;
+        op->beta = make_double2(beta.re, beta.im);
+        op->sign = sign;
+        op->d_far.allocate((size_t)m->n);
+        op->d_self.allocate((size_t)m->n);
+        op->d_near.allocate((size_t)std::max<int64_t>(ws->near_pairs, 1));
+        context_retain(op->ctx);
+        *out = op.release();
     });
 }
 
@@ -626,26 +667,31 @@ void sbh_operator_destroy(sbh_operator* op) {
     delete op;
     context_release(ctx);
 }
-int64_t sbh_operator_num_rows(const sbh_operator* op) { return op ? op->sys->n : 0; }
-int64_t sbh_operator_num_cols(const sbh_operator* op) { return op ? op->sys->n : 0; }
-int sbh_operator_transpose_capability(const sbh_operator* op) { return op ? SBH_ADJOINT_EXACT : SBH_ADJOINT_UNKNOWN; }
-int sbh_operator_hermitian_capability(const sbh_operator* op) { return op ? SBH_ADJOINT_EXACT : SBH_ADJOINT_UNKNOWN; }
+int64_t sbh_operator_num_rows(const sbh_operator* op) { return op ? op->n : 0; }
+int64_t sbh_operator_num_cols(const sbh_operator* op) { return op ? op->n : 0; }
+int sbh_operator_transpose_capability(const sbh_operator* op) {
+    return !op ? SBH_ADJOINT_UNKNOWN : op->exact() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
+}
+int sbh_operator_hermitian_capability(const sbh_operator* op) {
+    return !op ? SBH_ADJOINT_UNKNOWN : op->exact() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
+}
+
 
 static sbh_status operator_apply(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory, int mode) {
     return guard([&] {
         SBH_REQUIRE(op && x && y, "operator/x/y is null");
         check_memory(memory);
-        const int64_t n = op->sys->n;
+        if (mode != 0) require_dense(op, "the transpose/Hermitian product");
+        const int64_t n = op->n;
         if (memory == SBH_MEM_HOST && !all_finite(x, (size_t)n))
             throw Error(SBH_ERR_NUMERICAL, "operator input contains non-finite values");
         use_device(op->ctx);
         DeviceInput in(op->ctx, x, (size_t)n, memory);
         DeviceOutput outv(op->ctx, y, (size_t)n, memory);
-        const sbh_system* s = op->sys;
         if (mode == 0)
-            zgemv_n(op->ctx, s->d_A.ptr, s->lda, n, n, in.ptr, outv.ptr);
+            operator_apply_device(op, in.ptr, outv.ptr);
         else
-            zgemv_t(op->ctx, s->d_A.ptr, s->lda, n, n, in.ptr, outv.ptr, mode == 2);
+            zgemv_t(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, n, in.ptr, outv.ptr, mode == 2);
         outv.finish();
     });
 }
@@ -665,16 +711,27 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
         SBH_REQUIRE(op && X && Y, "operator/X/Y is null");
         SBH_REQUIRE(k >= 1, "block width must be positive");
         check_memory(memory);
-        const int64_t n = op->sys->n;
+        const int64_t n = op->n;
         if (memory == SBH_MEM_HOST && !all_finite(X, (size_t)(n * k)))
             throw Error(SBH_ERR_NUMERICAL, "operator input contains non-finite values");
         use_device(op->ctx);
         DeviceInput in(op->ctx, X, (size_t)(n * k), memory);
         DeviceOutput outv(op->ctx, Y, (size_t)(n * k), memory);
-        if (k == 1)
-            zgemv_n(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, n, in.ptr, outv.ptr);
-        else
+        if (k == 1) {
+            operator_apply_device(op, in.ptr, outv.ptr);
+        } else if (op->exact()) {
+            // column by column (the trait's default apply_block): strided gather, product, scatter
+            DeviceArray<double2> xc((size_t)n), yc((size_t)n);
+            for (int64_t c = 0; c < k; ++c) {
+                SBH_HIP(hipMemcpy2DAsync(xc.ptr, sizeof(double2), in.ptr + c, sizeof(double2) * k, sizeof(double2),
+                                         n, hipMemcpyDeviceToDevice, op->ctx->stream));
+                operator_apply_device(op, xc.ptr, yc.ptr);
+                SBH_HIP(hipMemcpy2DAsync(outv.ptr + c, sizeof(double2) * k, yc.ptr, sizeof(double2), sizeof(double2),
+                                         n, hipMemcpyDeviceToDevice, op->ctx->stream));
+            }
+        } else {
             zgemm_f64(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, in.ptr, k, outv.ptr);
+        }
         outv.finish();
     });
 }
@@ -694,10 +751,11 @@ sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out) 
 sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out) {
     return guard([&] {
         SBH_REQUIRE(op && out, "operator/out is null");
+        require_dense(op, "the Jacobi preconditioner");
         use_device(op->ctx);
         std::unique_ptr<sbh_precond> p(new sbh_precond());
         p->ctx = op->ctx;
-        p->n = op->sys->n;
+        p->n = op->n;
         p->kind = 1;
         p->d_dinv.allocate(p->n);
         extract_inverse_diagonal(op->ctx, op->sys->d_A.ptr, op->sys->lda, p->n, p->d_dinv.ptr);
@@ -738,8 +796,8 @@ sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, c
     sbh_status g = guard([&] {
         SBH_REQUIRE(op && b && config && x && solution, "null argument");
         check_memory(memory);
-        if (precond) SBH_REQUIRE(precond->n == op->sys->n, "preconditioner size does not match the operator");
-        const int64_t n = op->sys->n;
+        if (precond) SBH_REQUIRE(precond->n == op->n, "preconditioner size does not match the operator");
+        const int64_t n = op->n;
         use_device(op->ctx);
         DeviceInput db(op->ctx, b, (size_t)n, memory);
         std::unique_ptr<DeviceInput> dx0;
@@ -758,7 +816,8 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
         SBH_REQUIRE(op && B && config && X && solutions, "null argument");
         SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
         check_memory(memory);
-        const int64_t n = op->sys->n;
+        require_dense(op, "block GMRES");
+        const int64_t n = op->n;
         use_device(op->ctx);
         DeviceInput db(op->ctx, B, (size_t)(n * k), memory);
         DeviceOutput dx(op->ctx, X, (size_t)(n * k), memory);
@@ -852,6 +911,7 @@ sbh_status sbh_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const sbh_c64* B,
                         const sbh_gmres_ir_config* config, sbh_c64* X, int memory, sbh_gmres_ir_status* status) {
     return guard([&] {
         SBH_REQUIRE(op && lp && B && config && X && status, "null argument");
+        require_dense(op, "GMRES-IR");
         SBH_REQUIRE(lp->sys == op->sys, "the low-precision copy belongs to another system");
         SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
         SBH_REQUIRE(config->max_outer_iterations >= 1 && config->max_inner_iterations >= 1 &&

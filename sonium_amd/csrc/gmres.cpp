@@ -55,8 +55,7 @@ struct Krylov {
     }
 
     void apply(const double2* x, double2* y) {
-        const sbh_system* s = op->sys;
-        zgemv_n(ctx, s->d_A.ptr, s->lda, s->n, s->n, x, y);
+        operator_apply_device(op, x, y);
         spmv += 1;
     }
     void precond(const double2* r, double2* y) {
@@ -120,7 +119,7 @@ struct Krylov {
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
                      const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol) {
     sbh_context* ctx = op->ctx;
-    const int64_t n = op->sys->n;
+    const int64_t n = op->n;
     *sol = sbh_gmres_solution{};
     auto finish = [&](int64_t it, int64_t rs, double res, double est, double bw, bool conv, int status) {
         sol->iterations = it;

@@ -271,9 +271,18 @@ struct sbh_lowp_matrix {
     sbh::DeviceArray<float> d_ypart;      // split-K partial sums, splits x n x 128
 };
 
+// A dense operator wraps a device-resident system; an exact operator
+// evaluates the same entries on the fly from a workspace (O(N) memory).
 struct sbh_operator {
     sbh_context* ctx = nullptr;
-    sbh_system* sys = nullptr;
+    int64_t n = 0;
+    sbh_system* sys = nullptr;          // dense
+    const sbh_workspace* ws = nullptr;  // exact
+    sbh_physics physics{};
+    double2 beta{};
+    double sign = 1.0;
+    sbh::DeviceArray<double2> d_far, d_near, d_self;  // exact-product partials
+    bool exact() const { return ws != nullptr; }
 };
 
 struct sbh_precond {
@@ -302,6 +311,10 @@ void gauss_legendre_rule(int order, double* x, double* w);
 // Assembly launchers (tbem_kernels.hip).
 void upload_quadrature_tables(sbh_context* ctx);
 void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_system* sys);
+// y = sign * A x without storing A (tbem_kernels.hip); x and y on the device.
+void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y);
+// y = A x for a dense or exact operator (api.cpp).
+void operator_apply_device(const sbh_operator* op, const double2* x, double2* y);
 void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,
                          sbh_c64 beta, double sign, const double* d_dirs, const double2* d_amps,
                          int64_t num_rhs, double2* d_rhs, bool accumulate);

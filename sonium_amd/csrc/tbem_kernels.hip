@@ -731,6 +731,128 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
 
 __global__ void reset_counters_kernel(unsigned* counter) { counter[threadIdx.x] = 0u; }
 
+// ---------------------------------------------------------------------------
+// Matrix-free exact product y = sign * A x (apply_tbem_stabilized_exact,
+// crates/sonium-bem/src/core/assembly/hmatrix.rs:2363-2425) with the entries of
+// the assembly kernels, evaluated on the fly in O(N) memory:
+//   exact_far_kernel   far[i] = sum over j != i of the 13-point entry times x_j
+//                      (lane per column, EXACT_ROWS rows per workgroup, row sums
+//                      reduced across the workgroup in a fixed order);
+//   exact_near_kernel  per near pair: (subdivided entry - 13-point entry) x_j;
+//   exact_self_kernel  per panel: diagonal entry times x_i;
+//   exact_combine_kernel  y_i = sign (far_i + near pairs of row i in CSR
+//                      order + self_i).
+constexpr int EXACT_ROWS = 4;
+
+template <int MODE>
+__device__ __forceinline__ double2 far_entry(const FarPanel& P, const Source& S, const PanelBc& bc, double k,
+                                             double k2, double tau_r, double2 beta, double2 iwr) {
+    double hr, hi, er, ei, gr, gi, tr, ti;
+    far_sums<MODE>(P, S, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
+    if (MODE == 0) return velocity_coefficient(hr, hi, er, ei, tau_r, beta);
+    Acc a;
+    a.g = make_double2(gr, gi);
+    a.h = make_double2(hr, hi);
+    a.ht = make_double2(tr, ti);
+    a.e = make_double2(er, ei);
+    return coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void exact_far_kernel(const FarPanel* __restrict__ panels,
+                                                        const Source* __restrict__ sources,
+                                                        const PanelBc* __restrict__ bcs, int64_t n, double k,
+                                                        double tau_r, double2 beta, double2 iwr,
+                                                        const double2* __restrict__ x, double2* __restrict__ out) {
+    __shared__ double2 red[4][EXACT_ROWS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * EXACT_ROWS;
+    const double k2 = k * k;
+    double2 acc[EXACT_ROWS];
+#pragma unroll
+    for (int r = 0; r < EXACT_ROWS; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int64_t jc = 0; jc < n; jc += 256) {
+        const int64_t j = jc + threadIdx.x;
+        const bool active = j < n;
+        const int64_t jj = active ? j : n - 1;
+        const FarPanel P = panels[jj];
+        PanelBc bc;
+        if (MODE == 1) bc = bcs[jj];
+        const double2 xj = x[jj];
+#pragma unroll
+        for (int r = 0; r < EXACT_ROWS; ++r) {
+            const int64_t i = min(i0 + r, n - 1);  // rows past n are computed and dropped
+            const double2 v = far_entry<MODE>(P, sources[i], bc, k, k2, tau_r, beta, iwr);
+            if (active && j != i) {
+                acc[r].x += v.x * xj.x - v.y * xj.y;
+                acc[r].y += v.x * xj.y + v.y * xj.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < EXACT_ROWS; ++r) {
+        acc[r].x = wave_sum(acc[r].x);
+        acc[r].y = wave_sum(acc[r].y);
+        if (lane == 0) red[wave][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < EXACT_ROWS && i0 + threadIdx.x < n) {
+        double2 t = red[0][threadIdx.x];
+        for (int w = 1; w < 4; ++w) t = cadd(t, red[w][threadIdx.x]);
+        out[i0 + threadIdx.x] = t;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void exact_near_kernel(const NearPair* __restrict__ pairs, int64_t npairs,
+                                                         const uint64_t* __restrict__ leaves,
+                                                         const PanelGeom* __restrict__ geom,
+                                                         const FarPanel* __restrict__ panels,
+                                                         const Source* __restrict__ sources,
+                                                         const PanelBc* __restrict__ bcs, double k, double wavruim,
+                                                         double tau_r, double2 beta, double2 iwr,
+                                                         const double2* __restrict__ x, double2* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= npairs) return;
+    const NearPair np = pairs[p];
+    const PanelBc bc = MODE == 1 ? bcs[np.j] : rigid_bc();
+    const PanelGeom G = geom[np.j];
+    const Source S = sources[np.i];
+    const double2 v = near_entry(np, leaves, G, S, bc, k, wavruim, tau_r, beta, iwr, lane);
+    if (lane == 0) {
+        const double2 f = far_entry<MODE>(panels[np.j], S, bc, k, k * k, tau_r, beta, iwr);
+        out[p] = cmul(csub(v, f), x[np.j]);
+    }
+}
+
+__global__ __launch_bounds__(256) void exact_self_kernel(const PanelGeom* __restrict__ geom,
+                                                         const Source* __restrict__ sources,
+                                                         const PanelBc* __restrict__ bcs, int general, int64_t n,
+                                                         double k, double wavruim, double tau_r, double2 beta,
+                                                         double2 iwr, const double2* __restrict__ x,
+                                                         double2* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const PanelBc bc = general ? bcs[i] : rigid_bc();
+    double2 diag, unused;
+    self_entry(geom[i], sources[i], bc, k, wavruim, tau_r, beta, iwr, lane, &diag, &unused);
+    if (lane == 0) out[i] = cmul(diag, x[i]);
+}
+
+__global__ void exact_combine_kernel(const double2* __restrict__ far, const double2* __restrict__ near,
+                                     const int64_t* __restrict__ row_ptr, const double2* __restrict__ self,
+                                     int64_t n, double sign, double2* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 s = far[i];
+    for (int64_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p) s = cadd(s, near[p]);
+    s = cadd(s, self[i]);
+    y[i] = make_double2(sign * s.x, sign * s.y);
+}
+
 __global__ void sincos_probe_kernel(const double* x, int64_t n, double* s, double* c) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) sincos_red(x[t], &s[t], &c[t]);
@@ -829,6 +951,48 @@ void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int 
     hipLaunchKernelGGL(batch_assemble_kernel<1>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
+}
+
+void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y) {
+    const sbh_workspace* ws = op->ws;
+    const sbh_mesh* m = ws->mesh;
+    sbh_context* ctx = op->ctx;
+    const int64_t n = m->n;
+    const sbh_physics& ph = op->physics;
+    const double k = ph.wave_number;
+    const double wavruim = ph.harmonic_factor * ph.wave_number;
+    const double2 beta = op->beta;
+    const double2 iwr = make_double2(0.0, ph.pressure_factor);
+    const int general = m->general ? 1 : 0;
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_MATVEC, &ev);
+    const dim3 far_grid((unsigned)((n + EXACT_ROWS - 1) / EXACT_ROWS));
+    if (general)
+        hipLaunchKernelGGL(exact_far_kernel<1>, far_grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, k, ph.tau, beta, iwr, x, op->d_far.ptr);
+    else
+        hipLaunchKernelGGL(exact_far_kernel<0>, far_grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, k, ph.tau, beta, iwr, x, op->d_far.ptr);
+    SBH_HIP(hipGetLastError());
+    if (ws->near_pairs > 0) {
+        const dim3 grid((unsigned)((ws->near_pairs + 3) / 4));
+        if (general)
+            hipLaunchKernelGGL(exact_near_kernel<1>, grid, dim3(256), 0, ctx->stream, ws->d_pairs.ptr, ws->near_pairs,
+                               ws->d_leaves.ptr, m->d_geom.ptr, m->d_far.ptr, m->d_src.ptr, m->d_bc.ptr, k, wavruim,
+                               ph.tau, beta, iwr, x, op->d_near.ptr);
+        else
+            hipLaunchKernelGGL(exact_near_kernel<0>, grid, dim3(256), 0, ctx->stream, ws->d_pairs.ptr, ws->near_pairs,
+                               ws->d_leaves.ptr, m->d_geom.ptr, m->d_far.ptr, m->d_src.ptr, m->d_bc.ptr, k, wavruim,
+                               ph.tau, beta, iwr, x, op->d_near.ptr);
+        SBH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(exact_self_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, m->d_geom.ptr,
+                       m->d_src.ptr, m->d_bc.ptr, general, n, k, wavruim, ph.tau, beta, iwr, x, op->d_self.ptr);
+    SBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(exact_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       op->d_far.ptr, op->d_near.ptr, ws->d_row_ptr.ptr, op->d_self.ptr, n, op->sign, y);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_MATVEC, ev);
 }
 
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {
