@@ -185,6 +185,30 @@ sbh_status sbh_incident_plane_wave_rhs(sbh_context* ctx, const sbh_mesh* mesh,
                                        const double* directions, const sbh_c64* amplitudes,
                                        int64_t num_rhs, sbh_c64* rhs, int memory, int accumulate);
 
+/* Pressure at points off the surface — compute_total_field(_checked) and
+ * compute_scattered_field(_checked)
+ * (crates/sonium-bem/src/core/postprocess/pressure.rs:250-280, 501-557, 868-894,
+ * 2105-2145; integration 1388-1480, 1566-1620, 1987-2044) for constant Tri3
+ * panels with the 7-point rule:
+ *   scattered[i] = sum_j sum_q [p_j dG/dn_y - v_j i pf G] J_j w_q  at x_i,
+ *   incident[i]  = sum_w amplitudes[w] exp(i hf k directions[w].x_i),
+ *   total[i]     = incident[i] + scattered[i]   (FieldPoint, pressure.rs:32-41).
+ * solution: the N solved unknowns.  surface_velocity: N values v_j (then
+ * p_j = solution[j]), or NULL to derive (p_j, v_j) from the panel BCs as
+ * constant_cauchy_data (pressure.rs:1481-1520) does, using the plane waves as
+ * the incident field on pressure panels.  directions: num_waves x 3
+ * (normalized as IncidentField::plane_wave), amplitudes: num_waves, host
+ * memory; num_waves = 0 is the reference's "no incident field".
+ * solution, surface_velocity, points (num_points x 3) and the outputs use
+ * `memory`; incident and total may be NULL.  Non-finite points give
+ * SBH_ERR_INVALID_ARGUMENT and a non-finite result SBH_ERR_NUMERICAL
+ * (validate_field_inputs / validate_finite_results, pressure.rs:612-750). */
+sbh_status sbh_field_pressure(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* physics,
+                              const sbh_c64* solution, const sbh_c64* surface_velocity,
+                              const double* directions, const sbh_c64* amplitudes, int64_t num_waves,
+                              const double* points, int64_t num_points, sbh_c64* scattered,
+                              sbh_c64* incident, sbh_c64* total, int memory);
+
 /* ---------------------------------------------------------------------------
  * LinearOperator<Complex64> over a device-resident dense matrix —
  * DenseMatrixOperator (bem_solver.rs:6548-6593) / DenseOperator

@@ -53,6 +53,8 @@ def lib():
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _dp]
         L.oracle_zgemv.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.oracle_field.argtypes = [_dp, ctypes.c_int64, _dp, _i64p, ctypes.c_int64, _dp, _dp, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int]
         L.oracle_gmres.argtypes = [_dp, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.oracle_gmres.restype = ctypes.c_int
@@ -178,6 +180,69 @@ def plane_wave_rhs(centers, normals, k, beta, direction=(0.0, 0.0, 1.0), amplitu
                                 beta.real, beta.imag, float(sign), _p(d), amp.real, amp.imag,
                                 _p(out.view(np.float64)))
     return out
+
+
+def field(points, nodes, tris, p_surf, k, *, v_surf=None, directions=None, amplitudes=None,
+          density=1.21, speed_of_sound=343.0, harmonic_factor=1.0, threads=None):
+    """Total (with directions) or scattered pressure at points (pressure.rs:1388-1480, 2105-2150)."""
+    pts = _f64(points).reshape(-1, 3)
+    nodes = _f64(nodes)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    n = tris.shape[0]
+    ps = np.ascontiguousarray(np.asarray(p_surf, dtype=np.complex128).reshape(n))
+    vs = None if v_surf is None else np.ascontiguousarray(np.asarray(v_surf, dtype=np.complex128).reshape(n))
+    pf = density * k * speed_of_sound * harmonic_factor
+    if directions is None:
+        dirs = np.zeros((0, 3))
+        amps = np.zeros(0, dtype=np.complex128)
+    else:
+        dirs = _f64(directions).reshape(-1, 3)
+        amps = np.ones(dirs.shape[0], dtype=np.complex128) if amplitudes is None else \
+            np.ascontiguousarray(np.asarray(amplitudes, dtype=np.complex128).reshape(dirs.shape[0]))
+    out = np.zeros(pts.shape[0], dtype=np.complex128)
+    lib().oracle_field(_p(pts), pts.shape[0], _p(nodes), tris.ctypes.data_as(_i64p), n, _p(ps.view(np.float64)),
+                       None if vs is None else _p(vs.view(np.float64)), float(k), float(harmonic_factor), pf,
+                       _p(dirs) if dirs.size else None, _p(amps.view(np.float64)) if amps.size else None,
+                       dirs.shape[0], _p(out.view(np.float64)), threads or default_threads())
+    return out
+
+
+def cauchy_data(centers, normals, solution, k, *, bc_type=None, bc_value=None, admittance=None,
+                directions=None, amplitudes=None, density=1.21, speed_of_sound=343.0, harmonic_factor=1.0):
+    """Per-panel (p, v) for the field integral when no velocity is given:
+    constant_cauchy_data (pressure.rs:1481-1520) with incident_cauchy_data and
+    normal_derivative_to_velocity (pressure.rs:1522-1548).  Velocity panels
+    give (sol, value), admittance panels (sol, value + adm sol), pressure
+    panels (value - p_inc(c), (sol - dp_inc/dn(c)) / (i pf)) with the plane
+    waves p_inc = A exp(i hf k d.c), dp_inc/dn = i hf k (d.n) p_inc
+    (incident.rs:216-330)."""
+    c = _f64(centers).reshape(-1, 3)
+    nrm = _f64(normals).reshape(-1, 3)
+    sol = np.asarray(solution, dtype=np.complex128).reshape(-1)
+    n = sol.shape[0]
+    bt = np.zeros(n, dtype=np.int32) if bc_type is None else np.asarray(bc_type).reshape(n)
+    bv = np.zeros(n, dtype=np.complex128) if bc_value is None else np.asarray(bc_value, dtype=np.complex128).reshape(n)
+    adm = np.full(n, np.nan + 0j) if admittance is None else np.asarray(admittance, dtype=np.complex128).reshape(n)
+    pf = density * k * speed_of_sound * harmonic_factor
+    p_inc = np.zeros(n, dtype=np.complex128)
+    q_inc = np.zeros(n, dtype=np.complex128)
+    if directions is not None:
+        d = _f64(directions).reshape(-1, 3)
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        amps = np.ones(d.shape[0], dtype=np.complex128) if amplitudes is None else \
+            np.asarray(amplitudes, dtype=np.complex128).reshape(d.shape[0])
+        for w in range(d.shape[0]):
+            pw = amps[w] * np.exp(1j * harmonic_factor * k * (c @ d[w]))
+            p_inc += pw
+            q_inc += 1j * harmonic_factor * k * (nrm @ d[w]) * pw
+    p = sol.copy()
+    v = bv.copy()
+    has_adm = ~np.isnan(adm.real)
+    v[has_adm] = bv[has_adm] + adm[has_adm] * sol[has_adm]
+    pr = bt == 1
+    p[pr] = bv[pr] - p_inc[pr]
+    v[pr] = (sol[pr] - q_inc[pr]) / (1j * pf) if abs(pf) > 1e-30 else 0.0
+    return p, v
 
 
 def zgemv(A, x, threads=None):

@@ -7,7 +7,9 @@ sphere, crates/sonium-wave/src/analytical/solutions_3d.rs:315-387
     p(r, theta) = sum_n (2n+1) i^n [j_n(kr) - a_n h_n(kr)] P_n(cos theta),
     a_n = j_n'(ka) / h_n'(ka),  h_n = j_n + i y_n,
 
-with the truncation rule suggested_sphere_terms (solutions_3d.rs:241-250).
+with the truncation rule suggested_sphere_terms (solutions_3d.rs:241-250),
+and the sound-soft sphere (a_n = j_n(ka)/h_n(ka)) for field checks on
+pressure panels.
 The spherical Bessel functions are evaluated with mpmath at 40 digits rather
 than by the reference's recurrences, so the coefficients are exact to double
 precision; they are pinned against the reference's own 45-digit fixture
@@ -107,11 +109,43 @@ def surface_pressure(k: float, radius: float, points: np.ndarray, direction=(0.0
     return np.tensordot(c, P, axes=(0, 0))
 
 
-def total_field(k: float, radius: float, r: float, theta: np.ndarray, num_terms: int | None = None) -> np.ndarray:
-    """Direct series sum_n (2n+1) i^n [j_n(kr) - a_n h_n(kr)] P_n(cos theta) (solutions_3d.rs:340-372)."""
+def soft_sphere_coefficients(ka: float, num_terms: int) -> np.ndarray:
+    """a_n = j_n(ka) / h_n(ka) for the sound-soft (p = 0) sphere.
+
+    Textbook Dirichlet counterpart of the rigid coefficients (the reference's
+    analytical module has no soft-sphere series; parity rests on the vanishing
+    total field at r = a and on scipy's independent Bessel routines)."""
+    res = []
+    with mpmath.workdps(_DPS):
+        for j, y, _, _ in _sph_and_derivative(num_terms, ka):
+            res.append(complex(mpmath.mpc(j, 0) / mpmath.mpc(j, y)))
+    return np.array(res, dtype=np.complex128)
+
+
+def robin_sphere_coefficients(k: float, radius: float, sigma: complex, num_terms: int) -> np.ndarray:
+    """a_n = (k j_n'(ka) - sigma j_n(ka)) / (k h_n'(ka) - sigma h_n(ka)): the
+    sphere whose total field satisfies dp/dr = sigma p on r = a (a locally
+    reacting surface; sigma = i rho omega Y for the admittance Y with the
+    outward normal).  sigma = 0 gives the rigid coefficients."""
+    res = []
+    with mpmath.workdps(_DPS):
+        kk = mpmath.mpf(k)
+        sg = mpmath.mpc(complex(sigma).real, complex(sigma).imag)
+        for j, y, jp, yp in _sph_and_derivative(num_terms, k * radius):
+            res.append(complex((kk * jp - sg * j) / (kk * mpmath.mpc(jp, yp) - sg * mpmath.mpc(j, y))))
+    return np.array(res, dtype=np.complex128)
+
+
+def total_field(k: float, radius: float, r: float, theta: np.ndarray, num_terms: int | None = None,
+                soft: bool = False, sigma: complex | None = None) -> np.ndarray:
+    """Direct series sum_n (2n+1) i^n [j_n(kr) - a_n h_n(kr)] P_n(cos theta) (solutions_3d.rs:340-372);
+    soft=True takes the sound-soft coefficients, sigma the Robin ones."""
     ka = k * radius
     nt = num_terms or suggested_sphere_terms(ka)
-    a = rigid_sphere_coefficients(ka, nt)
+    if sigma is not None:
+        a = robin_sphere_coefficients(k, radius, sigma, nt)
+    else:
+        a = soft_sphere_coefficients(ka, nt) if soft else rigid_sphere_coefficients(ka, nt)
     terms = []
     with mpmath.workdps(_DPS):
         for n in range(nt):

@@ -390,6 +390,76 @@ class IncidentField:
             _cptr(self.amplitudes), self.num_rhs, ptr, memory, int(accumulate)))
 
 
+def _field(mesh, solution, surface_velocity, physics, incident, points, with_incident):
+    n = mesh.num_elements
+    sol = _c128(solution, n)
+    vel = None if surface_velocity is None else _c128(surface_velocity, n)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    m = pts.shape[0]
+    scat = np.zeros(m, dtype=np.complex128)
+    inc = np.zeros(m, dtype=np.complex128) if with_incident else None
+    tot = np.zeros(m, dtype=np.complex128) if with_incident else None
+    nw = 0 if incident is None else incident.num_rhs
+    _check(_lib().sbh_field_pressure(
+        mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), _cptr(sol), None if vel is None else _cptr(vel),
+        None if incident is None else _cptr(incident.directions),
+        None if incident is None else _cptr(incident.amplitudes), nw, _cptr(pts), m, _cptr(scat),
+        None if inc is None else _cptr(inc), None if tot is None else _cptr(tot), MEM_HOST))
+    return pts, scat, inc, tot
+
+
+def compute_scattered_field(points, mesh: TbemMesh, solution, physics: PhysicsParams,
+                            surface_velocity=None) -> np.ndarray:
+    """Scattered pressure at points off the surface (compute_scattered_field(_checked),
+    pressure.rs:250-280, 868-894; integration pressure.rs:1388-1480, 1566-1620,
+    1987-2044): constant Tri3 panels, 7-point rule.  surface_velocity None takes
+    the panels' Cauchy data from their BCs (constant_cauchy_data,
+    pressure.rs:1481-1520) with no incident field."""
+    return _field(mesh, solution, surface_velocity, physics, None, points, False)[1]
+
+
+class FieldPoint:
+    """Position and incident/scattered/total pressure (pressure.rs:32-41)."""
+
+    __slots__ = ("position", "p_incident", "p_scattered", "p_total")
+
+    def __init__(self, position, p_incident, p_scattered, p_total):
+        self.position = position
+        self.p_incident = p_incident
+        self.p_scattered = p_scattered
+        self.p_total = p_total
+
+    def __repr__(self):
+        return f"FieldPoint({self.position}, total={self.p_total})"
+
+
+class FieldValues:
+    """compute_total_field's result as arrays: positions (M x 3), p_incident,
+    p_scattered and p_total (M complex each); indexing gives FieldPoint."""
+
+    def __init__(self, positions, p_incident, p_scattered, p_total):
+        self.positions = positions
+        self.p_incident = p_incident
+        self.p_scattered = p_scattered
+        self.p_total = p_total
+
+    def __len__(self):
+        return self.positions.shape[0]
+
+    def __getitem__(self, i):
+        return FieldPoint(self.positions[i].tolist(), complex(self.p_incident[i]), complex(self.p_scattered[i]),
+                          complex(self.p_total[i]))
+
+
+def compute_total_field(points, mesh: TbemMesh, solution, incident: "IncidentField", physics: PhysicsParams,
+                        surface_velocity=None) -> FieldValues:
+    """Incident plus scattered pressure at points off the surface
+    (compute_total_field(_checked), pressure.rs:501-557, 2105-2145): the plane
+    waves are the incident field, also for the Cauchy data of pressure panels."""
+    pts, scat, inc, tot = _field(mesh, solution, surface_velocity, physics, incident, points, True)
+    return FieldValues(pts, inc, scat, tot)
+
+
 AdjointCapability = {0: "Unknown", 1: "Unsupported", 2: "Exact"}
 
 

@@ -313,6 +313,11 @@ void upload_quadrature_tables(sbh_context* ctx);
 void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_system* sys);
 // y = sign * A x without storing A (tbem_kernels.hip); x and y on the device.
 void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y);
+// Pressure at points (tbem_kernels.hip): plane waves plus the scattered field
+// of the panels' Cauchy data; v == nullptr derives it from the BCs.
+void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, const double2* sol, const double2* v,
+                  const double* d_points, int64_t npts, const double* d_dirs, const double2* d_amps, int64_t nwaves,
+                  double2* scattered, double2* incident, double2* total);
 // y = A x for a dense or exact operator (api.cpp).
 void operator_apply_device(const sbh_operator* op, const double2* x, double2* y);
 void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,

@@ -53,6 +53,14 @@ __constant__ double kTrW[13] = {-0.149570044467682, 0.175615257433208, 0.1756152
                                 0.077113760890257, 0.077113760890257, 0.077113760890257,
                                 0.077113760890257};
 
+// TR7 (gauss.rs:516-524) for field evaluation; weights are used times 0.5.
+__constant__ double kTr7S[7] = {0.333333333333333, 0.797426985353087, 0.101286507323456, 0.101286507323456,
+                                0.470142064105115, 0.059715871789770, 0.470142064105115};
+__constant__ double kTr7T[7] = {0.333333333333333, 0.101286507323456, 0.797426985353087, 0.101286507323456,
+                                0.059715871789770, 0.470142064105115, 0.470142064105115};
+__constant__ double kTr7W[7] = {0.225, 0.125939180544827, 0.125939180544827, 0.125939180544827,
+                                0.132394152788506, 0.132394152788506, 0.132394152788506};
+
 // Local vertex and mid-side coordinates (singular.rs:119-124 CSI6/ETA6).
 __constant__ double kCsi6[6] = {0.0, 1.0, 0.0, 0.5, 0.5, 0.0};
 __constant__ double kEta6[6] = {0.0, 0.0, 1.0, 0.0, 0.5, 0.5};
@@ -853,6 +861,148 @@ __global__ void exact_combine_kernel(const double2* __restrict__ far, const doub
     y[i] = make_double2(sign * s.x, sign * s.y);
 }
 
+// ---------------------------------------------------------------------------
+// Pressure at evaluation points (compute_scattered_field / compute_total_field,
+// crates/sonium-bem/src/core/postprocess/pressure.rs:1388-1480, 1566-1620,
+// 1987-2044), constant basis on Tri3 panels:
+//   p(x) = sum_j sum_q [p_j G (-1/r + ik) (r.n_j)/r - v_j i rho omega G] J_j w_q
+// with r = y_q - x, G = e^{ikr}/(4 pi r) and the 7-point rule.  A workgroup
+// takes FIELD_ROWS points and one slice of the panels (lane per panel); the
+// slices' partial sums are added per point in slice order.
+constexpr int FIELD_ROWS = 4;
+
+template <int HAS_V>
+__global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__ panels, int64_t n,
+                                                    const double* __restrict__ points, int64_t m,
+                                                    const double2* __restrict__ p, const double2* __restrict__ v,
+                                                    double k, double pf, int64_t panels_per_slice,
+                                                    double2* __restrict__ part) {
+    __shared__ double2 red[4][FIELD_ROWS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * FIELD_ROWS;
+    const int64_t jbeg = (int64_t)blockIdx.y * panels_per_slice;
+    const int64_t jend = min(n, jbeg + panels_per_slice);
+    double2 acc[FIELD_ROWS];
+#pragma unroll
+    for (int r = 0; r < FIELD_ROWS; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int64_t jc = jbeg; jc < jend; jc += 256) {
+        const int64_t j = jc + threadIdx.x;
+        const bool active = j < jend && j < n;
+        const int64_t jj = active ? j : n - 1;
+        const FarPanel P = panels[jj];
+        const double2 pj = p[jj];
+        const double2 vj = HAS_V ? v[jj] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < FIELD_ROWS; ++r) {
+            const double* x = points + 3 * min(i0 + r, m - 1);  // rows past m are computed and dropped
+            const double rel0 = P.v0[0] - x[0], rel1 = P.v0[1] - x[1], rel2 = P.v0[2] - x[2];
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < 7; ++q) {
+                const double sq = kTr7S[q], tq = kTr7T[q];
+                const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
+                const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
+                const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (!(d2 > 1e-30)) continue;  // the point lies on the quadrature point
+                const double inv = rsqrt_nr(d2);
+                const double dis = d2 * inv;
+                double sn, cs;
+                sincos_red(k * dis, &sn, &cs);
+                const double wt = (kTr7W[q] * P.wj) * inv;  // w J / (4 pi r), times 0.5 for the rule
+                const double zgr = cs * wt, zgi = sn * wt;
+                const double drdn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;
+                // p_j * zg * (-1/r + ik) * drdn
+                const double ar = (-zgr * inv - zgi * k) * drdn, ai = (zgr * k - zgi * inv) * drdn;
+                sr += pj.x * ar - pj.y * ai;
+                si += pj.x * ai + pj.y * ar;
+                if (HAS_V) {
+                    // - v_j * (i pf) * zg
+                    const double br = -pf * zgi, bi = pf * zgr;
+                    sr -= vj.x * br - vj.y * bi;
+                    si -= vj.x * bi + vj.y * br;
+                }
+            }
+            if (active && P.wj > 0.0) {
+                acc[r].x += sr;
+                acc[r].y += si;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < FIELD_ROWS; ++r) {
+        acc[r].x = wave_sum(acc[r].x);
+        acc[r].y = wave_sum(acc[r].y);
+        if (lane == 0) red[wave][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < FIELD_ROWS && i0 + threadIdx.x < m) {
+        double2 t = red[0][threadIdx.x];
+        for (int w = 1; w < 4; ++w) t = cadd(t, red[w][threadIdx.x]);
+        part[(int64_t)blockIdx.y * m + i0 + threadIdx.x] = t;
+    }
+}
+
+// scattered[i] = sum over slices of the partials; incident[i] = plane waves at
+// x_i (incident.rs:216-260); total[i] = incident + scattered (FieldPoint::new,
+// pressure.rs:32-41).  incident and total may be null.
+__global__ void field_finish_kernel(const double2* __restrict__ part, int slices, const double* __restrict__ points,
+                                    int64_t m, const double* __restrict__ dirs, const double2* __restrict__ amps,
+                                    int64_t nwaves, double k, double hf, double2* __restrict__ scattered,
+                                    double2* __restrict__ incident, double2* __restrict__ total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int z = 0; z < slices; ++z) s = cadd(s, part[(int64_t)z * m + i]);
+    scattered[i] = s;
+    if (!incident && !total) return;
+    const double* x = points + 3 * i;
+    double2 inc = make_double2(0.0, 0.0);
+    for (int64_t w = 0; w < nwaves; ++w) {
+        const double* d = dirs + 3 * w;
+        double sn, cs;
+        sincos_red(hf * k * (d[0] * x[0] + d[1] * x[1] + d[2] * x[2]), &sn, &cs);
+        inc = cadd(inc, cmul(amps[w], make_double2(cs, sn)));
+    }
+    if (incident) incident[i] = inc;
+    if (total) total[i] = cadd(inc, s);
+}
+
+// Surface Cauchy data of each panel from its BC when no velocity is given
+// (constant_cauchy_data, pressure.rs:1481-1520): velocity panels (sol, value),
+// admittance panels (sol, value + adm sol), pressure panels
+// (value - p_inc(c), (sol - dp_inc/dn(c)) / (i pf)) with the plane waves at the
+// collocation point.
+__global__ void field_cauchy_kernel(const PanelBc* __restrict__ bcs, const Source* __restrict__ src,
+                                    const double2* __restrict__ sol, int64_t n, const double* __restrict__ dirs,
+                                    const double2* __restrict__ amps, int64_t nwaves, double k, double hf, double pf,
+                                    double2* __restrict__ pe, double2* __restrict__ ve) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const PanelBc bc = bcs[j];
+    const double2 s = sol[j];
+    const double2 val = make_double2(bc.value[0], bc.value[1]);
+    if (bc.type == 1) {
+        const Source c = src[j];
+        double2 pin = make_double2(0.0, 0.0), qin = make_double2(0.0, 0.0);
+        for (int64_t w = 0; w < nwaves; ++w) {
+            const double* d = dirs + 3 * w;
+            double sn, cs;
+            sincos_red(hf * k * (d[0] * c.c[0] + d[1] * c.c[1] + d[2] * c.c[2]), &sn, &cs);
+            const double2 p = cmul(amps[w], make_double2(cs, sn));
+            pin = cadd(pin, p);
+            qin = cadd(qin, cmul(make_double2(0.0, hf * k * (d[0] * c.n[0] + d[1] * c.n[1] + d[2] * c.n[2])), p));
+        }
+        pe[j] = csub(val, pin);
+        const double2 q = csub(s, qin);
+        // q / (i pf) as a complex division, zero when pf vanishes
+        ve[j] = fabs(pf) > 1e-30 ? make_double2(q.y * pf / (pf * pf), -q.x * pf / (pf * pf)) : make_double2(0.0, 0.0);
+    } else {
+        pe[j] = s;
+        ve[j] = bc.has_adm ? cadd(val, cmul(make_double2(bc.adm[0], bc.adm[1]), s)) : val;
+    }
+}
+
 __global__ void sincos_probe_kernel(const double* x, int64_t n, double* s, double* c) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) sincos_red(x[t], &s[t], &c[t]);
@@ -993,6 +1143,44 @@ void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y) {
                        op->d_far.ptr, op->d_near.ptr, ws->d_row_ptr.ptr, op->d_self.ptr, n, op->sign, y);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_MATVEC, ev);
+}
+
+void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, const double2* sol, const double2* v,
+                  const double* d_points, int64_t npts, const double* d_dirs, const double2* d_amps, int64_t nwaves,
+                  double2* scattered, double2* incident, double2* total) {
+    const int64_t n = m->n;
+    if (npts == 0) return;
+    const double k = ph.wave_number, hf = ph.harmonic_factor, pf = ph.pressure_factor;
+    const double2* p = sol;
+    DeviceArray<double2> pe, ve;
+    if (!v && m->general) {
+        pe.allocate((size_t)n);
+        ve.allocate((size_t)n);
+        hipLaunchKernelGGL(field_cauchy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           m->d_bc.ptr, m->d_src.ptr, sol, n, d_dirs, d_amps, nwaves, k, hf, pf, pe.ptr, ve.ptr);
+        SBH_HIP(hipGetLastError());
+        p = pe.ptr;
+        v = ve.ptr;
+    }
+    const int64_t row_blocks = (npts + FIELD_ROWS - 1) / FIELD_ROWS;
+    SBH_REQUIRE(row_blocks < ((int64_t)1 << 31), "too many evaluation points");
+    // Enough workgroups to fill the chip: split the panels when the points alone give few.
+    const int64_t chunks = (n + 255) / 256;
+    const int64_t slices = std::max<int64_t>(1, std::min<int64_t>({chunks, 2048 / row_blocks, 65535}));
+    const int64_t per_slice = (chunks + slices - 1) / slices * 256;
+    DeviceArray<double2> part((size_t)(slices * npts));
+    const dim3 grid((unsigned)row_blocks, (unsigned)slices);
+    if (v)
+        hipLaunchKernelGGL(field_kernel<1>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, n, d_points, npts, p, v,
+                           hf * k, pf, per_slice, part.ptr);
+    else
+        hipLaunchKernelGGL(field_kernel<0>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, n, d_points, npts, p, v,
+                           hf * k, pf, per_slice, part.ptr);
+    SBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(field_finish_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream, part.ptr,
+                       (int)slices, d_points, npts, d_dirs, d_amps, nwaves, k, hf, scattered, incident, total);
+    SBH_HIP(hipGetLastError());
+    SBH_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
 }
 
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {

@@ -92,3 +92,41 @@ def test_rigid_sphere_low_frequency_limit():
     pts = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])
     p = mie.surface_pressure(1e-3, 1.0, pts)
     assert np.max(np.abs(p - 1.0)) < 5e-3
+
+
+@pytest.mark.parametrize("ka", [0.5, 1.0, 5.0])
+def test_soft_sphere_series(ka):
+    """Sound-soft sphere: the total field vanishes on r = a, the coefficients
+    match scipy's spherical Bessel functions, and off the sphere the series
+    differs from the rigid one."""
+    n = mie.suggested_sphere_terms(ka)
+    a = mie.soft_sphere_coefficients(ka, n)
+    nn = np.arange(n)
+    j = special.spherical_jn(nn, ka)
+    y = special.spherical_yn(nn, ka)
+    ref = j / (j + 1j * y)
+    big = np.abs(ref) > 1e-12
+    assert np.max(np.abs(a[big] - ref[big]) / np.abs(ref[big])) < 1e-10
+    theta = np.linspace(0.0, math.pi, 9)
+    on = mie.total_field(ka, 1.0, 1.0, theta, soft=True)
+    assert np.max(np.abs(on)) < 1e-12
+    off_soft = mie.total_field(ka, 1.0, 2.0, theta, soft=True)
+    off_rigid = mie.total_field(ka, 1.0, 2.0, theta)
+    assert np.max(np.abs(off_soft - off_rigid)) > 1e-2
+
+
+def test_robin_sphere_series():
+    """dp/dr = sigma p on r = a (central difference of the series in r), and
+    sigma = 0 reproduces the rigid coefficients."""
+    k, a = 1.3, 1.0
+    n = mie.suggested_sphere_terms(k * a)
+    assert np.allclose(mie.robin_sphere_coefficients(k, a, 0.0, n), mie.rigid_sphere_coefficients(k * a, n),
+                       rtol=1e-13, atol=1e-16)
+    sigma = 0.25 + 0.5j
+    theta = np.linspace(0.0, math.pi, 7)
+    h = 1e-5
+    plus = mie.total_field(k, a, a + h, theta, sigma=sigma)
+    minus = mie.total_field(k, a, a - h, theta, sigma=sigma)
+    on = mie.total_field(k, a, a, theta, sigma=sigma)
+    dpdr = (plus - minus) / (2 * h)
+    assert np.max(np.abs(dpdr - sigma * on)) < 1e-8 * np.max(np.abs(on))

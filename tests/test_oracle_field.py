@@ -1,0 +1,135 @@
+"""Pins of the oracle's field evaluation (pressure.rs:1388-1480, 1987-2044) by
+identities that do not restate its formula:
+
+* k = 0, unit double-layer density on a closed polyhedron: the potential is
+  exactly -1 inside and 0 outside (Gauss); only the 7-point rule's error
+  remains;
+* k > 0, constant densities on the unit sphere (addition theorem, n = 0 term):
+  double layer -k j1(k) e^{ikR}/R, single layer -i rho omega sin(k) e^{ikR}/(kR);
+  the icosphere converges to both at O(h^2) (error / 4 per subdivision).
+"""
+
+import numpy as np
+from scipy.special import spherical_jn
+
+import oracle
+from oracle import mie
+from sonium_amd.mesh import icosphere
+
+
+def test_static_double_layer_is_minus_one_inside_zero_outside():
+    m = icosphere(1.0, 3)
+    one = np.ones(m.num_elements, dtype=np.complex128)
+    inside = oracle.field([[0, 0, 0], [0.1, 0.2, -0.3]], m.nodes, m.triangles, one, 0.0)
+    outside = oracle.field([[0, 0, 2.0], [1.5, 1.5, 0]], m.nodes, m.triangles, one, 0.0)
+    assert np.all(np.abs(inside + 1.0) < 1e-7)
+    assert np.all(np.abs(outside) < 1e-10)
+
+
+def test_sphere_layers_converge_to_addition_theorem():
+    k, R = 2.0, 5.0
+    x = np.array([[0, 0, R], [R, 0, 0], [0, -R, 0]])
+    pf = 1.21 * k * 343.0
+    exact_dl = -k * spherical_jn(1, k) * np.exp(1j * k * R) / R
+    exact_sl = -1j * pf * np.sin(k) * np.exp(1j * k * R) / (k * R)
+    errs = []
+    for lvl in (3, 4):
+        m = icosphere(1.0, lvl)
+        n = m.num_elements
+        one = np.ones(n, dtype=np.complex128)
+        dl = oracle.field(x, m.nodes, m.triangles, one, k)
+        sl = oracle.field(x, m.nodes, m.triangles, np.zeros(n, dtype=np.complex128), k, v_surf=one)
+        errs.append((np.max(np.abs(dl - exact_dl)) / abs(exact_dl), np.max(np.abs(sl - exact_sl)) / abs(exact_sl)))
+    (d3, s3), (d4, s4) = errs
+    assert d4 < 3e-3 and s4 < 5e-4
+    assert 3.0 < d3 / d4 < 5.0 and 3.0 < s3 / s4 < 5.0
+
+
+def test_incident_wave_is_added():
+    m = icosphere(1.0, 1)
+    zero = np.zeros(m.num_elements, dtype=np.complex128)
+    pts = np.array([[0.3, -2.0, 1.0]])
+    p = oracle.field(pts, m.nodes, m.triangles, zero, 3.0, directions=[[0, 0, 1.0]], amplitudes=[2.0])
+    assert abs(p[0] - 2.0 * np.exp(3j * 1.0)) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# Surface Cauchy data from the panel BCs (constant_cauchy_data,
+# pressure.rs:1481-1548) composed with the oracle's assembly, solve and field:
+# each BC kind against a closed-form sphere, converging as the mesh refines.
+
+_THETA = np.arange(9) * np.pi / 8
+_PTS = np.stack([2 * np.sin(_THETA), 0 * _THETA, 2 * np.cos(_THETA)], 1)  # r = 2a, the Mie CI grid
+
+
+def _sphere_field(level, k, beta, *, directions=((0.0, 0.0, 1.0),), **bc):
+    m = icosphere(1.0, level)
+    c, nn, _ = oracle.element_geometry(m.nodes, m.triangles)
+    A, rhs = oracle.build_tbem_system(m.nodes, m.triangles, k, beta, **bc)
+    b = rhs.copy()
+    for d in directions:
+        b = b + oracle.plane_wave_rhs(c, nn, k, beta, direction=d)
+    x = np.linalg.solve(A, b)
+    p, v = oracle.cauchy_data(c, nn, x, k, directions=directions or None, **bc)
+    return oracle.field(_PTS, m.nodes, m.triangles, p, k, v_surf=v, directions=directions or None)
+
+
+def _errors(got, ref):
+    return np.linalg.norm(got - ref) / np.linalg.norm(ref), np.max(np.abs(got - ref) / np.abs(ref))
+
+
+def test_pressure_panels_soft_sphere():
+    """All panels pressure BC with p = 0 (sound-soft sphere): the unknown is the
+    total normal derivative; p_j = -p_inc(c_j), v_j = (sol - dp_inc/dn)/(i rho omega).
+    Conventional BIE (beta = 0): converges to the soft-sphere series."""
+    k = 1.0
+    ref = mie.total_field(k, 1.0, 2.0, _THETA, soft=True)
+    errs = []
+    for level in (2, 3):
+        n = 20 * 4 ** level
+        got = _sphere_field(level, k, 0.0, bc_type=np.ones(n, np.int32), bc_value=np.zeros(n, np.complex128))
+        errs.append(_errors(got, ref))
+    assert errs[1][0] < 4e-3 and errs[1][1] < 6e-3, errs
+    assert errs[0][0] / errs[1][0] > 3.0, errs
+
+
+def test_admittance_panels_robin_sphere():
+    """Admittance Y on every panel: v_j = Y p_j, i.e. dp/dr = i rho omega Y p on
+    r = a; Burton-Miller beta = 0.2 i/k converges to the Robin-sphere series."""
+    k = 1.0
+    Y = 1.2e-3 * (1 - 0.5j)
+    sigma = 1j * 1.21 * 343.0 * k * Y
+    ref = mie.total_field(k, 1.0, 2.0, _THETA, sigma=sigma)
+    assert np.linalg.norm(ref - mie.total_field(k, 1.0, 2.0, _THETA)) > 0.1 * np.linalg.norm(ref)
+    errs = []
+    for level in (2, 3):
+        n = 20 * 4 ** level
+        got = _sphere_field(level, k, 0.2j / k, admittance=np.full(n, Y))
+        errs.append(_errors(got, ref))
+    assert errs[1][0] < 1.5e-3 and errs[1][1] < 2e-3, errs
+    assert errs[0][0] / errs[1][0] > 3.0, errs
+
+
+def test_velocity_panels_pulsating_sphere():
+    """Uniform normal velocity v0, no incident wave: the monopole
+    p(r) = i rho omega v0 h0(kr) / (k h0'(ka)) with h0(x) = -i e^{ix}/x."""
+    k = 1.5
+    v0 = 0.3 - 0.2j
+    pf = 1.21 * 343.0 * k
+    r = 2.0
+
+    def h0(x):
+        return -1j * np.exp(1j * x) / x
+
+    def h0p(x):
+        return np.exp(1j * x) * (1j / x ** 2 + 1.0 / x)
+
+    ref = np.full(_PTS.shape[0], 1j * pf * v0 * h0(k * r) / (k * h0p(k)))
+    errs = []
+    for level in (2, 3):
+        n = 20 * 4 ** level
+        got = _sphere_field(level, k, 0.2j / k, directions=(), bc_type=np.zeros(n, np.int32),
+                            bc_value=np.full(n, v0))
+        errs.append(_errors(got, ref))
+    assert errs[1][0] < 4e-3, errs
+    assert errs[0][0] / errs[1][0] > 3.0, errs
