@@ -39,6 +39,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_PER_EVAL = 70      # FP64 flops per kernel evaluation (FMA = 2, sqrt/div/sincos = 1)
 FLOPS_PER_COEFF = 14     # coefficient H*gamma*tau + E*beta per pair
+FIELD_FLOPS_PER_QP = 47  # field kernel, rigid panels: FP64 flops per (point, panel, quadrature point)
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (copy-calibrated; profiles/zgemv_n_kernel_and_far_kernel_c2_icosphere20480_k10.md).
@@ -582,6 +583,32 @@ def main():
                   "device_bytes": int(16 * (2 * n + st["near_pairs"]))}
     exact_op.close()
 
+    # exterior field on a 64 x 128 sphere grid at r = 2 (generate_sphere_eval_points, pressure.rs:2154)
+    th = np.pi * (np.arange(64) + 0.5) / 64
+    phi = 2 * np.pi * np.arange(128) / 128
+    T, P = np.meshgrid(th, phi, indexing="ij")
+    pts = 2.0 * np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    x64 = x.download()
+    sa.compute_total_field(pts, tm, x64, inc, ph)
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        fv = sa.compute_total_field(pts, tm, x64, inc, ph)
+    ctx.synchronize()
+    field_wall = (time.perf_counter() - t0) / reps
+    ctx.set_profiling(False)
+    f_launches, f_ms = ctx.kernel_stats("field")
+    f_avg_s = f_ms / max(f_launches, 1) / 1e3
+    f_flops = pts.shape[0] * n * 7 * FIELD_FLOPS_PER_QP
+    field_info = {"points": int(pts.shape[0]), "panels": n, "points_per_s": pts.shape[0] / field_wall,
+                  "call_ms": field_wall * 1e3, "field_kernel_avg_ms": f_avg_s * 1e3,
+                  "tflops_algorithmic": f_flops / f_avg_s / 1e12,
+                  "frac_fp64_peak": f_flops / f_avg_s / 1e12 / FP64_PEAK_TFLOPS,
+                  "flops_per_launch": f_flops, "max_abs_p_total": float(np.max(np.abs(fv.p_total)))}
+
     others = None
     if not args.no_other_configs and args.workload == "c2":
         c3 = run_c3(args, ctx)
@@ -632,6 +659,7 @@ def main():
         "roofline": roofline,
         "chosen_form": chosen,
         "exact_operator": exact_info,
+        "field_evaluation": field_info,
         "other_configs": others,
         "cpu_baseline": cpu,
     }

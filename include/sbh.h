@@ -98,7 +98,8 @@ typedef enum {
     SBH_KERNEL_KRYLOV_VECTOR = 5,
     SBH_KERNEL_BLOCK = 6,
     SBH_KERNEL_LOWP = 7,
-    SBH_KERNEL_CLASS_COUNT = 8
+    SBH_KERNEL_FIELD = 8,
+    SBH_KERNEL_CLASS_COUNT = 9
 } sbh_kernel_class;
 /* Synchronizes, then reports launches and summed event time of one class
  * since the last reset. */
@@ -478,6 +479,8 @@ sbh_status sbh_debug_subelements(const double* source_point, const double* eleme
 /* Test hook: the device sin/cos used by the assembly kernels, evaluated at
  * host-supplied points (for accuracy checks against the C library). */
 sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
+/* Test hook: the device 1/sqrt(x) used by the kernels, at host-supplied x > 0. */
+sbh_status sbh_debug_device_rsqrt(sbh_context* ctx, const double* x, int64_t n, double* y);
 /* Test hook: `reps` launches of a plain device copy of `bytes` bytes (reads
  * `bytes`, writes `bytes`), used to calibrate HBM byte counters. */
 sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps);

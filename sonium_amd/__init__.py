@@ -939,6 +939,14 @@ def device_sincos(ctx: Context, x) -> tuple:
     return s, c
 
 
+def device_rsqrt(ctx: Context, x) -> np.ndarray:
+    """The kernels' 1/sqrt evaluated on the device (test hook)."""
+    xv = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(xv)
+    _check(_lib().sbh_debug_device_rsqrt(ctx.handle, _cptr(xv), xv.size, _cptr(y)))
+    return y
+
+
 def debug_subelements(source_point, element_coords, element_area: float) -> np.ndarray:
     """The host route builder's subdivision of one pair (test hook, no GPU)."""
     src = np.ascontiguousarray(source_point, dtype=np.float64).reshape(3)

@@ -24,6 +24,7 @@ KERNEL_CLASSES = {
     "krylov_vector": 5,
     "block": 6,
     "lowp": 7,
+    "field": 8,
 }
 
 
@@ -179,6 +180,7 @@ SIGNATURES = {
     "sbh_mlp_predict": (_int, [_vp, _vp, _i64, _vp, _int]),
     "sbh_debug_subelements": (_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.POINTER(_int)]),
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
+    "sbh_debug_device_rsqrt": (_int, [_vp, _vp, _i64, _vp]),
     "sbh_debug_device_copy": (_int, [_vp, _i64, _int]),
 }
 

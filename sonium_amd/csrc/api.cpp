@@ -21,6 +21,7 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
                      const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol);
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
+void launch_rsqrt_probe(sbh_context* ctx, const double* x, int64_t n, double* y);
 
 namespace {
 thread_local std::string g_last_error;
@@ -1076,6 +1077,18 @@ sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n,
             SBH_HIP(hipMemcpy(s, ds.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
             SBH_HIP(hipMemcpy(c, dc.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
         }
+    });
+}
+
+sbh_status sbh_debug_device_rsqrt(sbh_context* ctx, const double* x, int64_t n, double* y) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && x && y && n >= 0, "null argument");
+        use_device(ctx);
+        DeviceArray<double> dx(n), dy(n);
+        if (n) SBH_HIP(hipMemcpy(dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        launch_rsqrt_probe(ctx, dx.ptr, n, dy.ptr);
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        if (n) SBH_HIP(hipMemcpy(y, dy.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
     });
 }
 

@@ -78,52 +78,54 @@ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
 // sin and cos of x for |x| < 2^20.  Cody & Waite (1980) two-constant
-// reduction by pi/2 with FMA, then Taylor polynomials of sin (to r^17) and
-// cos (to r^18) on |r| <= pi/4, whose truncation error is below 1e-19.
+// reduction by pi with FMA, x = q pi + r, |r| <= pi/2; then
+// sin r = r + r^3 P(r^2) and cos r = 1 - r^2/2 + r^4 Q(r^2) with P, Q of
+// degree 7 interpolating at Chebyshev nodes in r^2 (gen_sincos_coefficients.py;
+// error below 3.6e-17 for sin and 1.6e-17 for cos), and both signs flipped
+// for odd q.
 __device__ __forceinline__ void sincos_red(double x, double* sp, double* cp) {
-    constexpr double TWO_OVER_PI = 0.63661977236758134308;
-    constexpr double PIO2_HI = 1.5707963267948966;
-    constexpr double PIO2_LO = 6.123233995736766e-17;
-    double q = __builtin_rint(x * TWO_OVER_PI);
-    double r = __builtin_fma(-q, PIO2_HI, x);
-    r = __builtin_fma(-q, PIO2_LO, r);
-    double r2 = r * r;
-    constexpr double S3 = -1.0 / 6.0, S5 = 1.0 / 120.0, S7 = -1.0 / 5040.0, S9 = 1.0 / 362880.0,
-                     S11 = -1.0 / 39916800.0, S13 = 1.0 / 6227020800.0,
-                     S15 = -1.0 / 1307674368000.0, S17 = 1.0 / 355687428096000.0;
-    constexpr double C4 = 1.0 / 24.0, C6 = -1.0 / 720.0, C8 = 1.0 / 40320.0, C10 = -1.0 / 3628800.0,
-                     C12 = 1.0 / 479001600.0, C14 = -1.0 / 87178291200.0,
-                     C16 = 1.0 / 20922789888000.0, C18 = -1.0 / 6402373705728000.0;
-    double ps = __builtin_fma(r2, S17, S15);
-    ps = __builtin_fma(r2, ps, S13);
-    ps = __builtin_fma(r2, ps, S11);
-    ps = __builtin_fma(r2, ps, S9);
-    ps = __builtin_fma(r2, ps, S7);
-    ps = __builtin_fma(r2, ps, S5);
-    ps = __builtin_fma(r2, ps, S3);
-    double sr = __builtin_fma(r * r2, ps, r);
-    double pc = __builtin_fma(r2, C18, C16);
-    pc = __builtin_fma(r2, pc, C14);
-    pc = __builtin_fma(r2, pc, C12);
-    pc = __builtin_fma(r2, pc, C10);
-    pc = __builtin_fma(r2, pc, C8);
-    pc = __builtin_fma(r2, pc, C6);
-    pc = __builtin_fma(r2, pc, C4);
-    double cr = __builtin_fma(r2 * r2, pc, __builtin_fma(-0.5, r2, 1.0));
-    int iq = (int)q;
-    double s = (iq & 1) ? cr : sr;
-    double c = (iq & 1) ? sr : cr;
-    *sp = (iq & 2) ? -s : s;
-    *cp = ((iq + 1) & 2) ? -c : c;
+    constexpr double ONE_OVER_PI = 0.3183098861837907;
+    constexpr double PI_HI = 3.141592653589793;
+    constexpr double PI_LO = 1.2246467991473532e-16;
+    const double q = __builtin_rint(x * ONE_OVER_PI);
+    double r = __builtin_fma(-q, PI_HI, x);
+    r = __builtin_fma(-q, PI_LO, r);
+    const double r2 = r * r;
+    constexpr double P0 = -0.16666666666666666, P1 = 0.008333333333333316, P2 = -0.00019841269841254974,
+                     P3 = 2.7557319219163234e-06, P4 = -2.5052107616996182e-08, P5 = 1.6058977312464087e-10,
+                     P6 = -7.643970296798572e-13, P7 = 2.7314447669863995e-15;
+    constexpr double Q0 = 0.041666666666666664, Q1 = -0.001388888888888888, Q2 = 2.480158730157985e-05,
+                     Q3 = -2.7557319221569563e-07, Q4 = 2.0876756602875463e-09, Q5 = -1.1470712272749466e-11,
+                     Q6 = 4.7778779252458453e-14, Q7 = -1.5218638369243573e-16;
+    double ps = __builtin_fma(r2, P7, P6);
+    ps = __builtin_fma(r2, ps, P5);
+    ps = __builtin_fma(r2, ps, P4);
+    ps = __builtin_fma(r2, ps, P3);
+    ps = __builtin_fma(r2, ps, P2);
+    ps = __builtin_fma(r2, ps, P1);
+    ps = __builtin_fma(r2, ps, P0);
+    const double sr = __builtin_fma(r * r2, ps, r);
+    double pc = __builtin_fma(r2, Q7, Q6);
+    pc = __builtin_fma(r2, pc, Q5);
+    pc = __builtin_fma(r2, pc, Q4);
+    pc = __builtin_fma(r2, pc, Q3);
+    pc = __builtin_fma(r2, pc, Q2);
+    pc = __builtin_fma(r2, pc, Q1);
+    pc = __builtin_fma(r2, pc, Q0);
+    const double cr = __builtin_fma(r2 * r2, pc, __builtin_fma(-0.5, r2, 1.0));
+    // (-1)^q: bit 0 of q moved to the sign bit of both results
+    const unsigned long long sign = (unsigned long long)((unsigned)(int)q & 1u) << 63;
+    *sp = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, sr) ^ sign);
+    *cp = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, cr) ^ sign);
 }
 
-// 1/sqrt(d2) to ~1 ulp: hardware estimate plus two Newton steps.
+// 1/sqrt(d2) to about 1 ulp: the hardware estimate (24 correct bits) and one
+// third-order (Householder) step, y1 = y0 (1 + e/2 + 3e^2/8) with
+// e = 1 - d2 y0^2, whose truncation error (5/16) e^3 is below 1e-22.
 __device__ __forceinline__ double rsqrt_nr(double d2) {
-    double y = __builtin_amdgcn_rsq(d2);
-    double hd = 0.5 * d2;
-    y = y * __builtin_fma(-hd * y, y, 1.5);
-    y = y * __builtin_fma(-hd * y, y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(d2);
+    const double e = __builtin_fma(-d2 * y, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
 struct Acc {
@@ -867,9 +869,11 @@ __global__ void exact_combine_kernel(const double2* __restrict__ far, const doub
 // 1987-2044), constant basis on Tri3 panels:
 //   p(x) = sum_j sum_q [p_j G (-1/r + ik) (r.n_j)/r - v_j i rho omega G] J_j w_q
 // with r = y_q - x, G = e^{ikr}/(4 pi r) and the 7-point rule.  A workgroup
-// takes FIELD_ROWS points and one slice of the panels (lane per panel); the
+// takes FIELD_ROWS points and one slice of the panels (lane per panel); each
+// quadrature point's position and weight are formed once per panel and used
+// for all the workgroup's points, whose coordinates are wave-uniform.  The
 // slices' partial sums are added per point in slice order.
-constexpr int FIELD_ROWS = 4;
+constexpr int FIELD_ROWS = 8;
 
 template <int HAS_V>
 __global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__ panels, int64_t n,
@@ -883,49 +887,52 @@ __global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__
     const int64_t i0 = (int64_t)blockIdx.x * FIELD_ROWS;
     const int64_t jbeg = (int64_t)blockIdx.y * panels_per_slice;
     const int64_t jend = min(n, jbeg + panels_per_slice);
+    double xs[FIELD_ROWS][3];
+#pragma unroll
+    for (int r = 0; r < FIELD_ROWS; ++r) {
+        const double* x = points + 3 * min(i0 + r, m - 1);  // rows past m are computed and dropped
+        xs[r][0] = x[0];
+        xs[r][1] = x[1];
+        xs[r][2] = x[2];
+    }
     double2 acc[FIELD_ROWS];
 #pragma unroll
     for (int r = 0; r < FIELD_ROWS; ++r) acc[r] = make_double2(0.0, 0.0);
     for (int64_t jc = jbeg; jc < jend; jc += 256) {
         const int64_t j = jc + threadIdx.x;
-        const bool active = j < jend && j < n;
-        const int64_t jj = active ? j : n - 1;
-        const FarPanel P = panels[jj];
-        const double2 pj = p[jj];
-        const double2 vj = HAS_V ? v[jj] : make_double2(0.0, 0.0);
+        const bool active = j < jend;
+        const FarPanel P = panels[active ? j : n - 1];
+        const double2 pj = p[active ? j : n - 1];
+        const double2 vj = HAS_V ? v[active ? j : n - 1] : make_double2(0.0, 0.0);
+        const double scale = (active && P.wj > 0.0) ? P.wj : 0.0;  // w J / (4 pi), times 0.5 for the rule
 #pragma unroll
-        for (int r = 0; r < FIELD_ROWS; ++r) {
-            const double* x = points + 3 * min(i0 + r, m - 1);  // rows past m are computed and dropped
-            const double rel0 = P.v0[0] - x[0], rel1 = P.v0[1] - x[1], rel2 = P.v0[2] - x[2];
-            double sr = 0.0, si = 0.0;
-            for (int q = 0; q < 7; ++q) {
-                const double sq = kTr7S[q], tq = kTr7T[q];
-                const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
-                const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
-                const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+        for (int q = 0; q < 7; ++q) {
+            const double sq = kTr7S[q], tq = kTr7T[q];
+            const double y0 = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], P.v0[0]));
+            const double y1 = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], P.v0[1]));
+            const double y2 = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], P.v0[2]));
+            const double wq = kTr7W[q] * scale;
+#pragma unroll
+            for (int r = 0; r < FIELD_ROWS; ++r) {
+                const double dx = y0 - xs[r][0], dy = y1 - xs[r][1], dz = y2 - xs[r][2];
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                if (!(d2 > 1e-30)) continue;  // the point lies on the quadrature point
-                const double inv = rsqrt_nr(d2);
-                const double dis = d2 * inv;
+                // a point within 1e-15 of the quadrature point contributes nothing
+                const double inv = rsqrt_nr(fmax(d2, 1e-30));
                 double sn, cs;
-                sincos_red(k * dis, &sn, &cs);
-                const double wt = (kTr7W[q] * P.wj) * inv;  // w J / (4 pi r), times 0.5 for the rule
+                sincos_red(k * (d2 * inv), &sn, &cs);
+                const double wt = d2 > 1e-30 ? wq * inv : 0.0;
                 const double zgr = cs * wt, zgi = sn * wt;
                 const double drdn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;
                 // p_j * zg * (-1/r + ik) * drdn
                 const double ar = (-zgr * inv - zgi * k) * drdn, ai = (zgr * k - zgi * inv) * drdn;
-                sr += pj.x * ar - pj.y * ai;
-                si += pj.x * ai + pj.y * ar;
+                acc[r].x = __builtin_fma(pj.x, ar, __builtin_fma(-pj.y, ai, acc[r].x));
+                acc[r].y = __builtin_fma(pj.x, ai, __builtin_fma(pj.y, ar, acc[r].y));
                 if (HAS_V) {
-                    // - v_j * (i pf) * zg
-                    const double br = -pf * zgi, bi = pf * zgr;
-                    sr -= vj.x * br - vj.y * bi;
-                    si -= vj.x * bi + vj.y * br;
+                    // - v_j * (i pf) * zg = v_j * pf * (zgi - i zgr)
+                    const double br = pf * zgi, bi = -pf * zgr;
+                    acc[r].x = __builtin_fma(vj.x, br, __builtin_fma(-vj.y, bi, acc[r].x));
+                    acc[r].y = __builtin_fma(vj.x, bi, __builtin_fma(vj.y, br, acc[r].y));
                 }
-            }
-            if (active && P.wj > 0.0) {
-                acc[r].x += sr;
-                acc[r].y += si;
             }
         }
     }
@@ -1170,6 +1177,8 @@ void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, co
     const int64_t per_slice = (chunks + slices - 1) / slices * 256;
     DeviceArray<double2> part((size_t)(slices * npts));
     const dim3 grid((unsigned)row_blocks, (unsigned)slices);
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_FIELD, &ev);
     if (v)
         hipLaunchKernelGGL(field_kernel<1>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, n, d_points, npts, p, v,
                            hf * k, pf, per_slice, part.ptr);
@@ -1177,10 +1186,22 @@ void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, co
         hipLaunchKernelGGL(field_kernel<0>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, n, d_points, npts, p, v,
                            hf * k, pf, per_slice, part.ptr);
     SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_FIELD, ev);
     hipLaunchKernelGGL(field_finish_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream, part.ptr,
                        (int)slices, d_points, npts, d_dirs, d_amps, nwaves, k, hf, scattered, incident, total);
     SBH_HIP(hipGetLastError());
     SBH_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
+}
+
+__global__ void rsqrt_probe_kernel(const double* x, int64_t n, double* y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = rsqrt_nr(x[t]);
+}
+
+void launch_rsqrt_probe(sbh_context* ctx, const double* x, int64_t n, double* y) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(rsqrt_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, x, n, y);
+    SBH_HIP(hipGetLastError());
 }
 
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {

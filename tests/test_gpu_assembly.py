@@ -41,6 +41,16 @@ def test_device_sincos_accuracy(ctx):
     assert np.max(np.abs(c - np.cos(x))) < 4e-16
 
 
+def test_device_rsqrt_accuracy(ctx):
+    """Hardware estimate plus one third-order step: within about one ulp over
+    the distances squared the kernels see (1e-30 .. 1e8)."""
+    rng = np.random.default_rng(8)
+    x = np.concatenate([10.0 ** rng.uniform(-30.0, 8.0, 50000), np.array([1.0, 4.0, 0.25, 2.0, 3.0])])
+    y = sa.device_rsqrt(ctx, x)
+    ref = 1.0 / np.sqrt(x)  # correctly rounded sqrt and division: within 1 ulp of 1/sqrt
+    assert np.max(np.abs(y - ref) / ref) < 3.4e-16
+
+
 @pytest.mark.parametrize("subdiv,k", [(2, 1.0), (3, 2.0)])
 def test_rigid_sphere_matrix_matches_oracle(ctx, subdiv, k):
     """Config C1 (1280 panels, k=2, beta=i/k) and the reference's s=2, k=1 CUDA fixture."""

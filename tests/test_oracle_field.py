@@ -133,3 +133,26 @@ def test_velocity_panels_pulsating_sphere():
         errs.append(_errors(got, ref))
     assert errs[1][0] < 4e-3, errs
     assert errs[0][0] / errs[1][0] > 3.0, errs
+
+
+def test_device_sincos_coefficients_match_generator():
+    """The sin/cos polynomial constants compiled into the kernels are the
+    generator's Chebyshev interpolants, and their error on |r| <= pi/2 is well
+    below one ulp of the results."""
+    import importlib.util
+    import os
+    import re
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(here, "..", "sonium_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen", os.path.join(csrc, "gen_sincos_coefficients.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    P, Q = gen.coefficients()
+    src = open(os.path.join(csrc, "tbem_kernels.hip")).read()
+    for name, vals in (("P", P), ("Q", Q)):
+        for i, v in enumerate(vals):
+            m = re.search(rf"\b{name}{i} = ([-0-9.e+]+)", src)
+            assert m and float(m.group(1)) == v, (name, i)
+    es, ec = gen.max_errors(P, Q, samples=401)
+    assert es < 4e-17 and ec < 2e-17
