@@ -49,6 +49,14 @@ def lib():
             ctypes.POINTER(ctypes.c_uint8), _dp, ctypes.c_double, ctypes.c_double,
             ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.oracle_build_tbem_system.restype = ctypes.c_int
+        L.oracle_regular_integration_dk.argtypes = L.oracle_regular_integration.argtypes
+        L.oracle_singular_integration_dk.argtypes = L.oracle_singular_integration.argtypes
+        L.oracle_build_tbem_derivative.argtypes = [
+            _dp, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), _dp, _dp,
+            ctypes.POINTER(ctypes.c_uint8), _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.oracle_build_tbem_derivative.restype = ctypes.c_int
         L.oracle_plane_wave_rhs.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _dp]
@@ -112,21 +120,24 @@ def _integrals(v):
             "e": complex(v[6], v[7])}
 
 
-def regular_integration(src, srcn, coords, area, k, harmonic_factor=1.0):
+def regular_integration(src, srcn, coords, area, k, harmonic_factor=1.0, derivative=False):
+    """regular.rs:124-321; derivative=True is the wavenumber-derivative mode (regular.rs:92-121)."""
     out = np.zeros(8)
-    lib().oracle_regular_integration(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)),
-                                     float(area), float(k), float(harmonic_factor), _p(out))
+    fn = lib().oracle_regular_integration_dk if derivative else lib().oracle_regular_integration
+    fn(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)), float(area), float(k),
+       float(harmonic_factor), _p(out))
     return _integrals(out)
 
 
-def singular_integration(src, srcn, coords, k, params=None, harmonic_factor=1.0):
+def singular_integration(src, srcn, coords, k, params=None, harmonic_factor=1.0, derivative=False):
+    """singular.rs:297-578; derivative=True is the wavenumber-derivative mode (singular.rs:167-295)."""
     out = np.zeros(8)
     pp = None
     if params is not None:
         arr = (ctypes.c_int * 4)(*params)
         pp = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int))
-    lib().oracle_singular_integration(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)),
-                                      float(k), float(harmonic_factor), pp, _p(out))
+    fn = lib().oracle_singular_integration_dk if derivative else lib().oracle_singular_integration
+    fn(_p(_f64(src, 3)), _p(_f64(srcn, 3)), _p(_f64(coords, 9)), float(k), float(harmonic_factor), pp, _p(out))
     return _integrals(out)
 
 
@@ -164,6 +175,46 @@ def build_tbem_system(nodes, tris, k, beta, *, bc_type=None, bc_value=None, admi
         threads or default_threads())
     if rc != 0:
         raise ValueError("oracle_build_tbem_system rejected its inputs")
+    return mat, rhs
+
+
+def build_tbem_derivative(nodes, tris, k, beta, beta_derivative, *, sign=1.0, wavenumber_scale=1.0,
+                          pressure_factor_derivative=None, admittance_scale=0.0, bc_type=None, bc_value=None,
+                          admittance=None, rows=None, threads=None, density=1.21, speed_of_sound=343.0,
+                          tau=1.0, harmonic_factor=1.0):
+    """Rows of d/dp of the stabilized system and its BC right-hand side
+    (build_tbem_system_stabilized_parameter_derivative, tbem.rs:1858-2019).
+    The defaults are the wavenumber derivative (tbem.rs:1734-1752):
+    dk/dp = 1, d(pressure factor)/dk = density * c * harmonic_factor."""
+    nodes = _f64(nodes)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    n = tris.shape[0]
+    r0, r1 = (0, n) if rows is None else rows
+    omega = k * speed_of_sound
+    phys = np.array([k, harmonic_factor, density * omega * harmonic_factor, tau])
+    if pressure_factor_derivative is None:
+        pressure_factor_derivative = density * speed_of_sound * harmonic_factor
+    bt = None if bc_type is None else np.ascontiguousarray(bc_type, dtype=np.int32)
+    bv = None if bc_value is None else np.ascontiguousarray(np.asarray(bc_value, dtype=np.complex128).view(np.float64))
+    ad = has = None
+    if admittance is not None:
+        adc = np.asarray(admittance, dtype=np.complex128)
+        has = np.ascontiguousarray(~np.isnan(adc.real), dtype=np.uint8)
+        ad = np.ascontiguousarray(np.nan_to_num(adc).view(np.float64))
+    mat = np.zeros((r1 - r0, n), dtype=np.complex128)
+    rhs = np.zeros(r1 - r0, dtype=np.complex128)
+    beta = complex(beta)
+    dbeta = complex(beta_derivative)
+    rc = lib().oracle_build_tbem_derivative(
+        _p(nodes), tris.ctypes.data_as(_i64p), n,
+        None if bt is None else bt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        None if bv is None else _p(bv), None if ad is None else _p(ad),
+        None if has is None else has.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+        _p(phys), beta.real, beta.imag, dbeta.real, dbeta.imag, float(sign), float(wavenumber_scale),
+        float(pressure_factor_derivative), float(admittance_scale), r0, r1, _p(mat.view(np.float64)),
+        _p(rhs.view(np.float64)), threads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_build_tbem_derivative rejected its inputs")
     return mat, rhs
 
 
