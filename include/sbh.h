@@ -170,6 +170,27 @@ sbh_status sbh_system_set_rhs(sbh_system* sys, const sbh_c64* rhs, int memory);
  *   A_ii += -gamma/2 (BC 0) or +beta*tau/2 (BC 1), plus admittance terms. */
 sbh_status sbh_workspace_assemble(sbh_workspace* ws, const sbh_physics* physics, sbh_c64 beta,
                                   sbh_system* sys);
+/* d/dp of the stabilized system at fixed geometry —
+ * build_tbem_system_stabilized_parameter_derivative (tbem.rs:1858-2019) and
+ * CudaTbemAssemblyWorkspace::assemble_wavenumber_derivative (tbem.rs:4399-4591):
+ * for a parameter p with dk/dp = wavenumber_scale, d(rho omega hf)/dp =
+ * pressure_factor_derivative and dY/dp = admittance_scale * Y on admittance
+ * panels, writes into sys
+ *   dA = sign * [(dH gamma tau + dE beta) ws + E beta'] (BC 0, minus the
+ *        admittance terms), -sign * [(dG gamma tau + dHt beta) ws + Ht beta'] (BC 1),
+ *   plus the derivative free terms, and the derivative of the BC right-hand side.
+ * beta is the coupling of the value system and beta_derivative its total
+ * derivative (the reference's burton_miller_beta_wavenumber_derivative() *
+ * beta_scale * wavenumber_scale).  The routes and self-panel tiers are those of
+ * the value assembly at physics (held fixed, as in the reference).
+ * wavenumber: ws = 1, pf' = density*c*hf (tbem.rs:1734-1752); sound speed:
+ * ws = -k/c, pf' = 0 (:1754-1782); density: ws = 0, pf' = pf/density
+ * (:1784-1805); impedance Y = 1/(rho c): admittance_scale = -1/density or
+ * -1/c (:1807-1856). */
+sbh_status sbh_workspace_assemble_derivative(sbh_workspace* ws, const sbh_physics* physics, sbh_c64 beta,
+                                             sbh_c64 beta_derivative, double sign, double wavenumber_scale,
+                                             double pressure_factor_derivative, double admittance_scale,
+                                             sbh_system* sys);
 /* build_tbem_system (tbem.rs:877-883): beta = burton_miller_beta(). */
 sbh_status sbh_build_tbem_system(sbh_context* ctx, const sbh_mesh* mesh,
                                  const sbh_physics* physics, sbh_system* sys);

@@ -99,6 +99,16 @@ class PhysicsParams:
     def burton_miller_beta(self) -> complex:
         return self.burton_miller_beta_scaled(1.0)
 
+    def burton_miller_beta_wavenumber_derivative(self) -> complex:
+        """d(i hf / k)/dk = -i hf / k^2 for exterior problems (types.rs:322-333)."""
+        if self.tau > 0.0:
+            return complex(0.0, -self.harmonic_factor / (self.wave_number * self.wave_number))
+        return 0j
+
+    def pressure_factor_wavenumber_derivative(self) -> float:
+        """d(rho omega hf)/dk at fixed c and rho (types.rs:340-345)."""
+        return self.density * self.speed_of_sound * self.harmonic_factor
+
     def gamma(self) -> float:
         return 1.0
 
@@ -314,6 +324,66 @@ class TbemAssemblyWorkspace:
         b = _ffi.c64(complex(beta).real, complex(beta).imag)
         _check(_lib().sbh_workspace_assemble(self._h, ctypes.byref(physics._p), b, system.handle))
         return system
+
+    def assemble_parameter_derivative(self, physics: PhysicsParams, beta: complex, beta_derivative: complex, *,
+                                      sign: float = 1.0, wavenumber_scale: float = 1.0,
+                                      pressure_factor_derivative: Optional[float] = None,
+                                      admittance_scale: float = 0.0,
+                                      system: Optional[TbemSystem] = None) -> TbemSystem:
+        """d/dp of the stabilized system (tbem.rs:1858-2019) into a device
+        TbemSystem: dk/dp = wavenumber_scale, d(rho omega hf)/dp =
+        pressure_factor_derivative (default: the wavenumber derivative's
+        rho c hf), dY/dp = admittance_scale * Y; beta_derivative is the total
+        derivative of beta."""
+        if pressure_factor_derivative is None:
+            pressure_factor_derivative = physics.pressure_factor_wavenumber_derivative()
+        if system is None:
+            system = TbemSystem(self.ctx, self.mesh.num_elements)
+        b = _ffi.c64(complex(beta).real, complex(beta).imag)
+        bd = _ffi.c64(complex(beta_derivative).real, complex(beta_derivative).imag)
+        _check(_lib().sbh_workspace_assemble_derivative(
+            self._h, ctypes.byref(physics._p), b, bd, float(sign), float(wavenumber_scale),
+            float(pressure_factor_derivative), float(admittance_scale), system.handle))
+        return system
+
+    def _derivative(self, physics, sign, beta_scale, ws, pfd, adm, system):
+        beta = physics.burton_miller_beta_scaled(beta_scale)
+        beta_d = physics.burton_miller_beta_wavenumber_derivative() * beta_scale * ws
+        return self.assemble_parameter_derivative(physics, beta, beta_d, sign=sign, wavenumber_scale=ws,
+                                                  pressure_factor_derivative=pfd, admittance_scale=adm,
+                                                  system=system)
+
+    def assemble_wavenumber_derivative(self, physics: PhysicsParams, sign: float = 1.0, beta_scale: float = 1.0,
+                                       system: Optional[TbemSystem] = None) -> TbemSystem:
+        """build_tbem_system_stabilized_wavenumber_derivative (tbem.rs:1734-1752)."""
+        return self._derivative(physics, sign, beta_scale, 1.0, physics.pressure_factor_wavenumber_derivative(), 0.0,
+                                system)
+
+    def assemble_speed_derivative(self, physics: PhysicsParams, sign: float = 1.0, beta_scale: float = 1.0,
+                                  system: Optional[TbemSystem] = None) -> TbemSystem:
+        """build_tbem_system_stabilized_speed_derivative (tbem.rs:1754-1782): dk/dc = -k/c at fixed omega."""
+        return self._derivative(physics, sign, beta_scale, -physics.wave_number / physics.speed_of_sound, 0.0, 0.0,
+                                system)
+
+    def assemble_density_derivative(self, physics: PhysicsParams, sign: float = 1.0, beta_scale: float = 1.0,
+                                    system: Optional[TbemSystem] = None) -> TbemSystem:
+        """build_tbem_system_stabilized_density_derivative (tbem.rs:1784-1805)."""
+        return self._derivative(physics, sign, beta_scale, 0.0, physics.pressure_factor / physics.density, 0.0,
+                                system)
+
+    def assemble_impedance_density_derivative(self, physics: PhysicsParams, sign: float = 1.0,
+                                              beta_scale: float = 1.0,
+                                              system: Optional[TbemSystem] = None) -> TbemSystem:
+        """build_tbem_system_stabilized_impedance_density_derivative (tbem.rs:1807-1830): Y = 1/(rho c)."""
+        return self._derivative(physics, sign, beta_scale, 0.0, physics.pressure_factor / physics.density,
+                                -1.0 / physics.density, system)
+
+    def assemble_impedance_speed_derivative(self, physics: PhysicsParams, sign: float = 1.0,
+                                            beta_scale: float = 1.0,
+                                            system: Optional[TbemSystem] = None) -> TbemSystem:
+        """build_tbem_system_stabilized_impedance_speed_derivative (tbem.rs:1832-1856)."""
+        return self._derivative(physics, sign, beta_scale, -physics.wave_number / physics.speed_of_sound, 0.0,
+                                -1.0 / physics.speed_of_sound, system)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -126,6 +126,8 @@ SIGNATURES = {
     "sbh_build_tbem_system": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp]),
     "sbh_incident_plane_wave_rhs": (_int, [_vp, _vp, ctypes.POINTER(physics_t), c64, ctypes.c_double,
                                            _vp, _vp, _i64, _vp, _int, _int]),
+    "sbh_workspace_assemble_derivative": (_int, [_vp, ctypes.POINTER(physics_t), c64, c64, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
     "sbh_field_pressure": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
                                   _vp, _vp, _int]),
     "sbh_dense_operator_create": (_int, [_vp, _pp]),

@@ -562,6 +562,27 @@ sbh_status sbh_workspace_assemble(sbh_workspace* ws, const sbh_physics* ph, sbh_
     });
 }
 
+sbh_status sbh_workspace_assemble_derivative(sbh_workspace* ws, const sbh_physics* ph, sbh_c64 beta,
+                                             sbh_c64 beta_derivative, double sign, double wavenumber_scale,
+                                             double pressure_factor_derivative, double admittance_scale,
+                                             sbh_system* sys) {
+    return guard([&] {
+        SBH_REQUIRE(ws && ph && sys, "workspace/physics/system is null");
+        SBH_REQUIRE(sys->ctx == ws->ctx, "system belongs to another context");
+        SBH_REQUIRE(sys->n == ws->mesh->n, "system size does not match the mesh");
+        SBH_REQUIRE(std::isfinite(ph->wave_number) && ph->wave_number > 0.0, "wave number must be positive");
+        SBH_REQUIRE(std::isfinite(beta.re) && std::isfinite(beta.im) && std::isfinite(beta_derivative.re) &&
+                        std::isfinite(beta_derivative.im),
+                    "beta and its derivative must be finite");
+        SBH_REQUIRE(std::isfinite(sign) && std::isfinite(wavenumber_scale) &&
+                        std::isfinite(pressure_factor_derivative) && std::isfinite(admittance_scale),
+                    "derivative parameters must be finite");
+        use_device(ws->ctx);
+        launch_derivative_assembly(ws, *ph, beta, beta_derivative, sign, wavenumber_scale,
+                                   pressure_factor_derivative, admittance_scale, sys);
+    });
+}
+
 sbh_status sbh_build_tbem_system(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph,
                                  sbh_system* sys) {
     return guard([&] {

@@ -313,6 +313,12 @@ void upload_quadrature_tables(sbh_context* ctx);
 void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_system* sys);
 // y = sign * A x without storing A (tbem_kernels.hip); x and y on the device.
 void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y);
+// d/dp of the stabilized system into sys (tbem_kernels.hip): dk/dp =
+// wavenumber_scale, d(pressure factor)/dp = pf_derivative, dY/dp =
+// admittance_scale * Y; beta_d is the total derivative of beta.
+void launch_derivative_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_c64 beta_d,
+                                double sign, double wavenumber_scale, double pf_derivative, double admittance_scale,
+                                sbh_system* sys);
 // Pressure at points (tbem_kernels.hip): plane waves plus the scattered field
 // of the panels' Cauchy data; v == nullptr derives it from the BCs.
 void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, const double2* sol, const double2* v,

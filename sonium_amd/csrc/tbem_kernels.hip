@@ -179,12 +179,15 @@ __device__ __forceinline__ void wave_sum_acc(Acc& a) {
 
 // ---------------------------------------------------------------------------
 // Reference-order evaluations (no contraction): one quadrature point of the
-// regular kernel, regular.rs:214-281.
+// regular kernel, regular.rs:214-281.  DERIV also accumulates the analytic
+// wavenumber derivative of every channel into dr (regular.rs:229-278), from
+// the same point, as the reference's shared quadrature loop does.
 #pragma clang fp contract(off)
 
-__device__ inline void ref_point(const double pos[3], double wga, const double src[3],
-                                 const double srcn[3], const double nf[3], double wavruim, double k2,
-                                 Acc& r) {
+template <bool DERIV>
+__device__ inline void ref_point_t(const double pos[3], double wga, const double src[3],
+                                   const double srcn[3], const double nf[3], double wavruim, double k2,
+                                   double k, double hf, Acc& r, Acc& dr) {
     double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
     double dis = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
     if (!(dis > 1e-15)) return;  // normalize() returns 0, then `dis < 1e-15` skips
@@ -206,6 +209,20 @@ __device__ inline void ref_point(const double pos[3], double wga, const double s
     r.h = cadd(r.h, cscale(zb, re1_h));
     r.ht = cadd(r.ht, cscale(zb, re2_h));
     r.e = cadd(r.e, cmul(zg, zef));
+    if (DERIV) {
+        double2 dzg = cmul(zg, make_double2(0.0, hf * dis));
+        double2 dzb = cadd(cmul(dzg, z1), cmul(zg, make_double2(0.0, hf)));
+        double2 dzef = make_double2(-2.0 * k * rq, -hf / dis * (3.0 * rq + nxny));
+        dr.g = cadd(dr.g, dzg);
+        dr.h = cadd(dr.h, cscale(dzb, re1_h));
+        dr.ht = cadd(dr.ht, cscale(dzb, re2_h));
+        dr.e = cadd(dr.e, cadd(cmul(dzg, zef), cmul(zg, dzef)));
+    }
+}
+
+__device__ inline void ref_point(const double pos[3], double wga, const double src[3], const double srcn[3],
+                                 const double nf[3], double wavruim, double k2, Acc& r) {
+    ref_point_t<false>(pos, wga, src, srcn, nf, wavruim, k2, 0.0, 0.0, r, r);
 }
 
 // Shape-function position of local (s, t) (regular.rs:682-690).
@@ -239,8 +256,10 @@ __device__ inline void decode_leaf(uint64_t code, double xi[3], double et[3]) {
 
 // Point q of a subdivided leaf (regular.rs:164-196): affine map of the TR13
 // point into the leaf, weight times |det|, then compute_parameters.
-__device__ inline void near_point(const PanelGeom& G, uint64_t code, int q, const double src[3],
-                                  const double srcn[3], double wavruim, double k2, Acc& r) {
+template <bool DERIV>
+__device__ inline void near_point_t(const PanelGeom& G, uint64_t code, int q, const double src[3],
+                                    const double srcn[3], double wavruim, double k2, double k, double hf, Acc& r,
+                                    Acc& dr) {
     double xi[3], et[3];
     decode_leaf(code, xi, et);
     double cs = kTrS[q], ec = kTrT[q], wg = kTrW[q] * 0.5;
@@ -253,15 +272,26 @@ __device__ inline void near_point(const PanelGeom& G, uint64_t code, int q, cons
     double weih2 = wg * det;
     double pos[3];
     shape_pos(G.c, xio, eto, pos);
-    ref_point(pos, weih2 * G.jac, src, srcn, G.nf, wavruim, k2, r);
+    ref_point_t<DERIV>(pos, weih2 * G.jac, src, srcn, G.nf, wavruim, k2, k, hf, r, dr);
+}
+
+__device__ inline void near_point(const PanelGeom& G, uint64_t code, int q, const double src[3],
+                                  const double srcn[3], double wavruim, double k2, Acc& r) {
+    near_point_t<false>(G, code, q, src, srcn, wavruim, k2, 0.0, 0.0, r, r);
 }
 
 // Full TR13 on the unsubdivided panel in reference order (bc_rhs_kernel).
-__device__ inline void full_point(const PanelGeom& G, int q, const double src[3], const double srcn[3],
-                                  double wavruim, double k2, Acc& r) {
+template <bool DERIV>
+__device__ inline void full_point_t(const PanelGeom& G, int q, const double src[3], const double srcn[3],
+                                    double wavruim, double k2, double k, double hf, Acc& r, Acc& dr) {
     double pos[3];
     shape_pos(G.c, kTrS[q], kTrT[q], pos);
-    ref_point(pos, (kTrW[q] * 0.5) * G.jac, src, srcn, G.nf, wavruim, k2, r);
+    ref_point_t<DERIV>(pos, (kTrW[q] * 0.5) * G.jac, src, srcn, G.nf, wavruim, k2, k, hf, r, dr);
+}
+
+__device__ inline void full_point(const PanelGeom& G, int q, const double src[3], const double srcn[3],
+                                  double wavruim, double k2, Acc& r) {
+    full_point_t<false>(G, q, src, srcn, wavruim, k2, 0.0, 0.0, r, r);
 }
 
 struct SelfTier {
@@ -283,9 +313,12 @@ __device__ inline int self_samples(const SelfTier& t) {
 // Sample s of the self-panel rule (singular.rs:322-410 edge part, :412-578
 // bulk part).  Edge samples add to E only; bulk samples to G, H, Ht and
 // E += k^2 G (n_x . n_y).  The sub-triangle loop repeats the second
-// sub-triangle for every isec >= 1, as the reference does.
-__device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s, const double src[3],
-                                   const double srcn[3], double wavruim, double k2, Acc& r) {
+// sub-triangle for every isec >= 1, as the reference does.  DERIV also adds
+// the wavenumber derivatives (singular.rs:380-386, 509-545) to dr.
+template <bool DERIV>
+__device__ inline void self_sample_t(const PanelGeom& G, const SelfTier& t, int s, const double src[3],
+                                     const double srcn[3], double wavruim, double k2, double k, double hf, Acc& r,
+                                     Acc& dr) {
     const double* c = G.c;
     const int per_edge = t.nsec1 * t.ngpo1;
     const int nedge = 3 * per_edge;
@@ -330,6 +363,16 @@ __device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s,
         double2 w2 = csub(cscale(zd0, dpoo[1]), cscale(zd1, dpoo[0]));
         double2 dot = cadd(cadd(cscale(w0, srcn[0]), cscale(w1, srcn[1])), cscale(w2, srcn[2]));
         r.e = cadd(r.e, cscale(dot, wga));
+        if (DERIV) {
+            double2 dzg = cmul(zg, make_double2(0.0, hf * dis));
+            double2 zgd = cadd(cmul(dzg, z1), cmul(zg, make_double2(0.0, hf)));
+            double2 e0 = cscale(zgd, u[0]), e1 = cscale(zgd, u[1]), e2 = cscale(zgd, u[2]);
+            double2 v0 = csub(cscale(e1, dpoo[2]), cscale(e2, dpoo[1]));
+            double2 v1 = csub(cscale(e2, dpoo[0]), cscale(e0, dpoo[2]));
+            double2 v2 = csub(cscale(e0, dpoo[1]), cscale(e1, dpoo[0]));
+            double2 dd = cadd(cadd(cscale(v0, srcn[0]), cscale(v1, srcn[1])), cscale(v2, srcn[2]));
+            dr.e = cadd(dr.e, cscale(dd, wga));
+        }
         return;
     }
     s -= nedge;
@@ -377,6 +420,90 @@ __device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s,
     r.ht = cadd(r.ht, cscale(zb, re2_h));
     double nd = srcn[0] * G.nf[0] + srcn[1] * G.nf[1] + srcn[2] * G.nf[2];
     r.e = cadd(r.e, cscale(cscale(zg, k2), nd));
+    if (DERIV) {
+        double2 dzg = cmul(zg, make_double2(0.0, hf * dis));
+        double2 dzb = cadd(cmul(dzg, z1), cmul(zg, make_double2(0.0, hf)));
+        dr.g = cadd(dr.g, dzg);
+        dr.h = cadd(dr.h, cscale(dzb, re1_h));
+        dr.ht = cadd(dr.ht, cscale(dzb, re2_h));
+        dr.e = cadd(dr.e, cscale(cadd(cscale(dzg, k2), cscale(zg, 2.0 * k)), nd));
+    }
+}
+
+__device__ inline void self_sample(const PanelGeom& G, const SelfTier& t, int s, const double src[3],
+                                   const double srcn[3], double wavruim, double k2, Acc& r) {
+    self_sample_t<false>(G, t, s, src, srcn, wavruim, k2, 0.0, 0.0, r, r);
+}
+
+// Parameters of a derivative assembly d/dp (tbem.rs:1858-2019): dk/dp = ws,
+// beta and its total derivative, i rho omega and its derivative, and
+// dY/dp = adm_scale * Y.
+struct DerivParams {
+    double k, wavruim, hf, tau_r, sign, ws, adm_scale;
+    double2 beta, beta_d, iwr, iwr_d;
+};
+
+// d_cb and d_hb of one pair (tbem.rs:1972-1977); b = values, d = derivatives.
+__device__ inline void deriv_blocks(const Acc& b, const Acc& d, const DerivParams& P, double2* d_cb, double2* d_hb) {
+    const double2 gamma = make_double2(1.0, 0.0);
+    const double2 tau = make_double2(P.tau_r, 0.0);
+    *d_cb = cadd(cadd(cscale(cmul(cmul(d.h, gamma), tau), P.ws), cscale(cmul(d.e, P.beta), P.ws)),
+                 cmul(b.e, P.beta_d));
+    *d_hb = cadd(cadd(cscale(cmul(cmul(d.g, gamma), tau), P.ws), cscale(cmul(d.ht, P.beta), P.ws)),
+                 cmul(b.ht, P.beta_d));
+}
+
+__device__ inline double2 h_block(const Acc& b, const DerivParams& P) {
+    return cadd(cmul(cmul(b.g, make_double2(1.0, 0.0)), make_double2(P.tau_r, 0.0)), cmul(b.ht, P.beta));
+}
+
+// Matrix derivative of one pair for the field panel's BC (tbem.rs:1978-1995).
+__device__ inline double2 deriv_coefficient(const Acc& b, const Acc& d, const PanelBc& bc, const DerivParams& P) {
+    double2 d_cb, d_hb;
+    deriv_blocks(b, d, P, &d_cb, &d_hb);
+    if (bc.type == 0) {
+        double2 v = cscale(d_cb, P.sign);
+        if (bc.has_adm) {
+            const double2 y = make_double2(bc.adm[0], bc.adm[1]);
+            const double2 h_b = h_block(b, P);
+            v = csub(v, cmul(cscale(y, P.sign), cadd(cmul(d_hb, P.iwr), cmul(h_b, P.iwr_d))));
+            v = csub(v, cmul(cmul(cscale(cscale(y, P.adm_scale), P.sign), h_b), P.iwr));
+        }
+        return v;
+    }
+    if (bc.type == 1) return cscale(d_hb, -P.sign);
+    return make_double2(0.0, 0.0);
+}
+
+// Right-hand-side derivative of a nonzero-BC field panel (tbem.rs:1998-2013).
+__device__ inline double2 deriv_bc_contribution(const Acc& b, const Acc& d, const PanelBc& bc, const DerivParams& P) {
+    double2 d_cb, d_hb;
+    deriv_blocks(b, d, P, &d_cb, &d_hb);
+    const double2 sv = cscale(make_double2(bc.value[0], bc.value[1]), P.sign);
+    if (bc.type == 0) return cmul(sv, cadd(cmul(d_hb, P.iwr), cmul(h_block(b, P), P.iwr_d)));
+    if (bc.type == 1) return cmul(sv, d_cb);
+    return make_double2(0.0, 0.0);
+}
+
+// Source free terms of the derivative (tbem.rs:1896-1915).
+__device__ inline void deriv_free_terms(const PanelBc& bc, const DerivParams& P, double2* diag, double2* rhs) {
+    const double2 tau = make_double2(P.tau_r, 0.0);
+    const double2 btd = cadd(cmul(P.iwr_d, P.beta), cmul(P.iwr, P.beta_d));
+    double2 dg = make_double2(0.0, 0.0), rh = make_double2(0.0, 0.0);
+    if (bc.type == 0) {
+        if (bc.has_adm) {
+            const double2 y = make_double2(bc.adm[0], bc.adm[1]);
+            dg = csub(dg, cscale(cmul(cmul(cscale(y, P.sign), btd), tau), 0.5));
+            const double2 yd = cscale(y, P.adm_scale);
+            dg = csub(dg, cscale(cmul(cmul(cmul(cscale(yd, P.sign), P.iwr), P.beta), tau), 0.5));
+        }
+        const double2 avg = make_double2(bc.value[0], bc.value[1]);
+        rh = cadd(rh, cscale(cmul(cmul(cscale(avg, P.sign), btd), tau), 0.5));
+    } else if (bc.type == 1) {
+        dg = cadd(dg, cscale(cmul(cscale(P.beta_d, P.sign), tau), 0.5));
+    }
+    *diag = dg;
+    *rhs = rh;
 }
 
 #pragma clang fp contract(fast)
@@ -574,6 +701,209 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
     if (lane == 0) {
         A[i * lda + i] = diag;
         rhs[i] = local_rhs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parameter derivative of the stabilized system (tbem.rs:1858-2019 and the
+// CUDA workspace's assemble_wavenumber_derivative, tbem.rs:4399-4591): the same
+// routes and launch structure as the value assembly, each pair producing its
+// values b and wavenumber derivatives d from one pass over its points.
+
+// Far sums with derivatives (fast arithmetic, harmonic factor +1).  With
+// dG/dk = i r G, d/dk[G (ik - 1/r)] = -k r G, so dH = -k r zg (r.n_y) and
+// dHt = -k r zg (-r.n_x); dE = dG ef + G def with def = (-2k rq, -(3rq + n.n)/r).
+// MODE 0 (velocity panels without admittance) needs E, dH, dE; MODE 1 also G,
+// Ht, dG, dHt.
+template <int MODE>
+__device__ __forceinline__ void far_sums_d(const FarPanel& P, const Source& src, double k, double k2, Acc& b,
+                                           Acc& d) {
+    const double rel0 = P.v0[0] - src.c[0];
+    const double rel1 = P.v0[1] - src.c[1];
+    const double rel2 = P.v0[2] - src.c[2];
+    const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
+    b = acc_zero();
+    d = acc_zero();
+#pragma unroll 2
+    for (int q = 0; q < 13; ++q) {
+        const double sq = kTrS[q], tq = kTrT[q];
+        const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
+        const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
+        const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+        const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);  // diagonal: overwritten later
+        const double inv = rsqrt_nr(d2);
+        const double dis = d2 * inv;
+        const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;
+        const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;
+        double sn, cs;
+        sincos_red(k * dis, &sn, &cs);
+        const double re2 = (kTrW[q] * P.wj) * inv;
+        const double zgr = cs * re2, zgi = sn * re2;
+        const double inv2 = inv * inv;
+        const double rq = pn * qn;
+        const double t3 = 3.0 * rq + nxny;
+        const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
+        const double defi = -inv * t3;
+        const double efi = k * defi;
+        const double defr = -2.0 * k * rq;
+        const double dzgr = -zgi * dis, dzgi = zgr * dis;
+        b.e.x += zgr * efr - zgi * efi;
+        b.e.y += zgr * efi + zgi * efr;
+        const double kd = -k * dis;
+        const double hr = kd * zgr, hi = kd * zgi;  // d/dk of zg (ik - 1/r)
+        d.h.x = __builtin_fma(hr, pn, d.h.x);
+        d.h.y = __builtin_fma(hi, pn, d.h.y);
+        d.e.x += (dzgr * efr - dzgi * efi) + (zgr * defr - zgi * defi);
+        d.e.y += (dzgr * efi + dzgi * efr) + (zgr * defi + zgi * defr);
+        if (MODE == 1) {
+            const double zbr = -zgr * inv - zgi * k;
+            const double zbi = zgr * k - zgi * inv;
+            b.g.x += zgr;
+            b.g.y += zgi;
+            b.ht.x = __builtin_fma(zbr, qn, b.ht.x);
+            b.ht.y = __builtin_fma(zbi, qn, b.ht.y);
+            d.g.x += dzgr;
+            d.g.y += dzgi;
+            d.ht.x = __builtin_fma(hr, qn, d.ht.x);
+            d.ht.y = __builtin_fma(hi, qn, d.ht.y);
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void far_deriv_kernel(const FarPanel* __restrict__ panels,
+                                                        const Source* __restrict__ sources,
+                                                        const PanelBc* __restrict__ bcs, int64_t n, int64_t lda,
+                                                        int rows_per_block, DerivParams P,
+                                                        double2* __restrict__ A) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < n;
+    const int64_t jj = active ? j : n - 1;
+    const FarPanel F = panels[jj];
+    const PanelBc bc = MODE == 1 ? bcs[jj] : rigid_bc();
+    const double k2 = P.k * P.k;
+    const int64_t i0 = (int64_t)blockIdx.y * rows_per_block;
+    const int64_t i1 = min(n, i0 + rows_per_block);
+    for (int64_t i = i0; i < i1; ++i) {
+        const Source src = sources[i];
+        Acc b, d;
+        far_sums_d<MODE>(F, src, P.k, k2, b, d);
+        const double2 out = deriv_coefficient(b, d, bc, P);
+        if (active) A[i * lda + j] = out;
+    }
+}
+
+// One wave per near pair; lanes over (leaf, point); reference operation order.
+__global__ __launch_bounds__(256) void near_deriv_kernel(const NearPair* __restrict__ pairs, int64_t npairs,
+                                                         const uint64_t* __restrict__ leaves,
+                                                         const PanelGeom* __restrict__ geom,
+                                                         const Source* __restrict__ sources,
+                                                         const PanelBc* __restrict__ bcs, int general, int64_t lda,
+                                                         DerivParams P, double2* __restrict__ A) {
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= npairs) return;
+    const NearPair np = pairs[p];
+    const PanelBc bc = general ? bcs[np.j] : rigid_bc();
+    const PanelGeom G = geom[np.j];
+    const Source S = sources[np.i];
+    const double k2 = P.k * P.k;
+    Acc b = acc_zero(), d = acc_zero();
+    const int total = np.count * 13;
+    for (int t = lane; t < total; t += 64) {
+        const int leaf = t / 13;
+        near_point_t<true>(G, leaves[np.begin + leaf], t - leaf * 13, S.c, S.n, P.wavruim, k2, P.k, P.hf, b, d);
+    }
+    wave_sum_acc(b);
+    wave_sum_acc(d);
+    if (lane == 0) A[(int64_t)np.i * lda + np.j] = deriv_coefficient(b, d, bc, P);
+}
+
+// One wave per panel: self-integral derivatives, free terms, BC free-term RHS.
+__global__ __launch_bounds__(256) void self_deriv_kernel(const PanelGeom* __restrict__ geom,
+                                                         const Source* __restrict__ sources,
+                                                         const PanelBc* __restrict__ bcs, int general, int64_t n,
+                                                         int64_t lda, DerivParams P, double2* __restrict__ A,
+                                                         double2* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const PanelBc bc = general ? bcs[i] : rigid_bc();
+    const PanelGeom& G = geom[i];
+    const Source S = sources[i];
+    const double k2 = P.k * P.k;
+    const SelfTier t = tier_for(P.k * G.size);
+    const int ns = self_samples(t);
+    Acc b = acc_zero(), d = acc_zero();
+    for (int s = lane; s < ns; s += 64) self_sample_t<true>(G, t, s, S.c, S.n, P.wavruim, k2, P.k, P.hf, b, d);
+    wave_sum_acc(b);
+    wave_sum_acc(d);
+    if (lane == 0) {
+        double2 diag, local_rhs;
+        deriv_free_terms(bc, P, &diag, &local_rhs);
+        A[i * lda + i] = cadd(diag, deriv_coefficient(b, d, bc, P));
+        rhs[i] = local_rhs;
+    }
+}
+
+// Block per row: right-hand-side derivative over the nonzero-BC columns in a
+// fixed order, as bc_rhs_kernel.
+__global__ __launch_bounds__(256) void bc_rhs_deriv_kernel(const int32_t* __restrict__ cols, int64_t ncols,
+                                                           const NearPair* __restrict__ pairs,
+                                                           const int64_t* __restrict__ row_ptr,
+                                                           const uint64_t* __restrict__ leaves,
+                                                           const PanelGeom* __restrict__ geom,
+                                                           const Source* __restrict__ sources,
+                                                           const PanelBc* __restrict__ bcs, DerivParams P,
+                                                           double2* __restrict__ rhs) {
+    __shared__ double part[4][2];
+    const int64_t i = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const Source src = sources[i];
+    const double k2 = P.k * P.k;
+    double2 sum = make_double2(0.0, 0.0);
+    const int64_t pb = row_ptr[i], pe = row_ptr[i + 1];
+    for (int64_t c = wave; c < ncols; c += 4) {
+        const int32_t j = cols[c];
+        const PanelGeom G = geom[j];
+        Acc b = acc_zero(), d = acc_zero();
+        if (j == i) {
+            const SelfTier t = tier_for(P.k * G.size);
+            const int ns = self_samples(t);
+            for (int s = lane; s < ns; s += 64)
+                self_sample_t<true>(G, t, s, src.c, src.n, P.wavruim, k2, P.k, P.hf, b, d);
+        } else {
+            int64_t lo = pb, hi = pe;
+            while (lo < hi) {
+                int64_t mid = (lo + hi) / 2;
+                if (pairs[mid].j < j) lo = mid + 1; else hi = mid;
+            }
+            if (lo < pe && pairs[lo].j == j) {
+                const NearPair np = pairs[lo];
+                const int total = np.count * 13;
+                for (int t = lane; t < total; t += 64) {
+                    const int leaf = t / 13;
+                    near_point_t<true>(G, leaves[np.begin + leaf], t - leaf * 13, src.c, src.n, P.wavruim, k2, P.k,
+                                       P.hf, b, d);
+                }
+            } else if (lane < 13) {
+                full_point_t<true>(G, lane, src.c, src.n, P.wavruim, k2, P.k, P.hf, b, d);
+            }
+        }
+        wave_sum_acc(b);
+        wave_sum_acc(d);
+        sum = cadd(sum, deriv_bc_contribution(b, d, bcs[j], P));
+    }
+    if (lane == 0) {
+        part[wave][0] = sum.x;
+        part[wave][1] = sum.y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 t = rhs[i];
+        for (int w = 0; w < 4; ++w) t = cadd(t, make_double2(part[w][0], part[w][1]));
+        rhs[i] = t;
     }
 }
 
@@ -1075,6 +1405,67 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, s
                            m->d_nonzero_bc.ptr, ncols, ws->d_pairs.ptr, ws->d_row_ptr.ptr,
                            ws->d_leaves.ptr, m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, k, wavruim,
                            ph.tau, beta, iwr, sys->d_rhs.ptr);
+        SBH_HIP(hipGetLastError());
+        ctx->end_timed(SBH_KERNEL_RHS, ev);
+    }
+}
+
+void launch_derivative_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_c64 beta_d,
+                                double sign, double wavenumber_scale, double pf_derivative, double admittance_scale,
+                                sbh_system* sys) {
+    sbh_context* ctx = ws->ctx;
+    const sbh_mesh* m = ws->mesh;
+    const int64_t n = m->n;
+    if (n == 0) return;
+    SBH_REQUIRE(std::fabs(ph.wave_number) * m->diameter < 5.0e5,
+                "k * mesh diameter exceeds the supported phase range (5e5 rad)");
+    if (ph.harmonic_factor != 1.0)
+        throw Error(SBH_ERR_UNSUPPORTED, "only harmonic_factor = +1 (ACOUSTIC_CONVENTION) is supported");
+    DerivParams P;
+    P.k = ph.wave_number;
+    P.wavruim = ph.harmonic_factor * ph.wave_number;
+    P.hf = ph.harmonic_factor;
+    P.tau_r = ph.tau;
+    P.sign = sign;
+    P.ws = wavenumber_scale;
+    P.adm_scale = admittance_scale;
+    P.beta = make_double2(beta.re, beta.im);
+    P.beta_d = make_double2(beta_d.re, beta_d.im);
+    P.iwr = make_double2(0.0, ph.pressure_factor);
+    P.iwr_d = make_double2(0.0, pf_derivative);
+    const int general = m->general ? 1 : 0;
+    int rows_per_block = 16;
+    while ((n + rows_per_block - 1) / rows_per_block > 65535) rows_per_block *= 2;
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + rows_per_block - 1) / rows_per_block));
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_FAR, &ev);
+    if (general)
+        hipLaunchKernelGGL(far_deriv_kernel<1>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, P, sys->d_A.ptr);
+    else
+        hipLaunchKernelGGL(far_deriv_kernel<0>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, P, sys->d_A.ptr);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);
+    if (ws->near_pairs > 0) {
+        ctx->begin_timed(SBH_KERNEL_ASSEMBLE_NEAR, &ev);
+        hipLaunchKernelGGL(near_deriv_kernel, dim3((unsigned)((ws->near_pairs + 3) / 4)), dim3(256), 0, ctx->stream,
+                           ws->d_pairs.ptr, ws->near_pairs, ws->d_leaves.ptr, m->d_geom.ptr, m->d_src.ptr,
+                           m->d_bc.ptr, general, sys->lda, P, sys->d_A.ptr);
+        SBH_HIP(hipGetLastError());
+        ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
+    }
+    ctx->begin_timed(SBH_KERNEL_ASSEMBLE_SELF, &ev);
+    hipLaunchKernelGGL(self_deriv_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, m->d_geom.ptr,
+                       m->d_src.ptr, m->d_bc.ptr, general, n, sys->lda, P, sys->d_A.ptr, sys->d_rhs.ptr);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_ASSEMBLE_SELF, ev);
+    const int64_t ncols = (int64_t)m->nonzero_bc.size();
+    if (ncols > 0) {
+        ctx->begin_timed(SBH_KERNEL_RHS, &ev);
+        hipLaunchKernelGGL(bc_rhs_deriv_kernel, dim3((unsigned)n), dim3(256), 0, ctx->stream,
+                           m->d_nonzero_bc.ptr, ncols, ws->d_pairs.ptr, ws->d_row_ptr.ptr, ws->d_leaves.ptr,
+                           m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, P, sys->d_rhs.ptr);
         SBH_HIP(hipGetLastError());
         ctx->end_timed(SBH_KERNEL_RHS, ev);
     }
