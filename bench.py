@@ -583,6 +583,26 @@ def main():
                   "device_bytes": int(16 * (2 * n + st["near_pairs"]))}
     exact_op.close()
 
+    # wavenumber derivative of the same system (build_tbem_system_stabilized_wavenumber_derivative)
+    dsys = sa.TbemSystem(ctx, n)
+    ws.assemble_wavenumber_derivative(ph, system=dsys)
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ws.assemble_wavenumber_derivative(ph, system=dsys)
+    ctx.synchronize()
+    d_wall = (time.perf_counter() - t0) / 2
+    ctx.set_profiling(False)
+    d_stats = {name: ctx.kernel_stats(name) for name in ("assemble_far", "assemble_near", "assemble_self")}
+    deriv_info = {"ms": d_wall * 1e3,
+                  "far_kernel_avg_ms": d_stats["assemble_far"][1] / max(d_stats["assemble_far"][0], 1),
+                  "near_kernel_avg_ms": d_stats["assemble_near"][1] / max(d_stats["assemble_near"][0], 1),
+                  "self_kernel_avg_ms": d_stats["assemble_self"][1] / max(d_stats["assemble_self"][0], 1),
+                  "value_far_kernel_avg_ms": far_avg_s * 1e3}
+    del dsys
+
     # exterior field on a 64 x 128 sphere grid at r = 2 (generate_sphere_eval_points, pressure.rs:2154)
     th = np.pi * (np.arange(64) + 0.5) / 64
     phi = 2 * np.pi * np.arange(128) / 128
@@ -660,6 +680,7 @@ def main():
         "chosen_form": chosen,
         "exact_operator": exact_info,
         "field_evaluation": field_info,
+        "wavenumber_derivative_assembly": deriv_info,
         "other_configs": others,
         "cpu_baseline": cpu,
     }

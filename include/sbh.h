@@ -130,6 +130,17 @@ sbh_status sbh_mesh_create(sbh_context* ctx, const double* nodes, int64_t num_no
                            const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
                            const sbh_c64* bc_value, const sbh_c64* admittance,
                            const uint8_t* has_admittance, sbh_mesh** out);
+/* Normal rule of sbh_mesh_create_oriented: SBH_NORMALS_OUTWARD is the
+ * generators' compute_element_geometry (vertex-order normal flipped to point
+ * away from the origin, generators.rs:857-866), the rule sbh_mesh_create uses;
+ * SBH_NORMALS_VERTEX_ORDER keeps (v1-v0)x(v2-v0) as compute_element_normal
+ * (crates/sonium-bem/src/core/mesh/element.rs:368-392) does for meshes read
+ * from files (load_project, crates/sonium-bem/src/numcalc/project.rs:78-105). */
+typedef enum { SBH_NORMALS_OUTWARD = 0, SBH_NORMALS_VERTEX_ORDER = 1 } sbh_normal_rule;
+sbh_status sbh_mesh_create_oriented(sbh_context* ctx, const double* nodes, int64_t num_nodes,
+                                    const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
+                                    const sbh_c64* bc_value, const sbh_c64* admittance,
+                                    const uint8_t* has_admittance, int normal_rule, sbh_mesh** out);
 void sbh_mesh_destroy(sbh_mesh* mesh);
 int64_t sbh_mesh_num_elements(const sbh_mesh* mesh);
 /* Host copies of centres, outward normals (N x 3) and areas (N); any may be NULL. */
@@ -488,6 +499,68 @@ sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_m
                          double* losses);
 /* Predictions (M x 2 floats) for M raw inputs (M x 10 floats). */
 sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory);
+
+/* ---------------------------------------------------------------------------
+ * Mesh2HRTF / NumCalc projects (host only).
+ * sbh_nc_project_load — load_project (crates/sonium-bem/src/numcalc/project.rs:78-105):
+ *   parse_nc_input (crates/sonium-bem/src/core/io/nc_format.rs:205-636), read the
+ *   node and element files (load_nc_nodes / load_nc_elements, nc_format.rs:694-781;
+ *   the first num_object_files pairs are the object, the rest evaluation grids),
+ *   apply the BOUNDARY ranges (VELO, PRES, ADMI) over all elements in file order
+ *   and expand the frequency curve (positive, finite, sorted, unique).
+ * sbh_nc_parse — parse_nc_input_string (nc_format.rs:215): the NC.inp text only;
+ *   file names are joined to base_dir and no mesh file is read.
+ * Errors: SBH_ERR_INVALID_ARGUMENT with "parse error at line N: ...",
+ *   "missing required section: ...", "IO error: ...", or "boundary spec covers
+ *   element ..." (NcParseError / ProjectError, nc_format.rs:184-203,
+ *   project.rs:18-33); an unknown BC keyword gives SBH_ERR_UNSUPPORTED. */
+typedef struct sbh_nc_project sbh_nc_project;
+typedef struct {
+    int64_t num_nodes, num_elements, num_evaluation, num_frequencies, num_curve_points;
+    int64_t num_plane_waves, num_point_sources, num_boundary_specs, num_node_files, num_element_files;
+    int32_t mesh_loaded;
+    int32_t main_params_i[9];   /* element type, nodes, elements, object files, evaluation files,
+                                   BC type, method, FMM method, parallel */
+    double main_params_ii[7];   /* as read; preconditioner and solver are entries 0 and 1 */
+    double speed_of_sound, density, reference_pressure;
+    int32_t has_symmetry, symmetry_flags[3];
+    double symmetry_origin[3];
+} sbh_nc_info;
+typedef enum {
+    SBH_NC_VERSION = 0, SBH_NC_DESCRIPTION = 1, SBH_NC_NODE_FILE = 2, SBH_NC_ELEMENT_FILE = 3,
+    SBH_NC_BOUNDARY_TYPE = 4
+} sbh_nc_string;
+typedef enum {
+    SBH_NC_CONTROL_I = 0, SBH_NC_CONTROL_II = 1, SBH_NC_MAIN_III = 2, SBH_NC_MAIN_IV_RESERVED = 3,
+    SBH_NC_FREQUENCY_CURVE = 4,   /* (time, frequency, phase) triples */
+    SBH_NC_FREQUENCIES = 5,       /* the expanded solve frequencies */
+    SBH_NC_BOUNDARY_SPECS = 6,    /* (start, end, re, curve_re, im, curve_im) per record */
+    SBH_NC_PLANE_WAVES = 7,       /* (dx, dy, dz, re, curve_re, im, curve_im) per record */
+    SBH_NC_POINT_SOURCES = 8      /* (x, y, z, re, curve_re, im, curve_im) per record */
+} sbh_nc_values;
+sbh_status sbh_nc_project_load(const char* nc_inp_path, sbh_nc_project** out);
+sbh_status sbh_nc_parse(const char* content, const char* base_dir, sbh_nc_project** out);
+void sbh_nc_project_destroy(sbh_nc_project* project);
+sbh_status sbh_nc_project_info(const sbh_nc_project* project, sbh_nc_info* info);
+/* NUL-terminated copy (truncated to size - 1 bytes); index selects the file or record. */
+sbh_status sbh_nc_project_string(const sbh_nc_project* project, int which, int64_t index, char* buf,
+                                 int64_t size);
+/* *count = number of doubles; they are written when out != NULL and capacity suffices. */
+sbh_status sbh_nc_project_values(const sbh_nc_project* project, int which, double* out, int64_t capacity,
+                                 int64_t* count);
+/* All elements in file order (each array may be NULL): connectivity 4 per element
+ * (-1 padded, node indices into the combined node list), counts = nodes per element,
+ * is_evaluation, and the BC each element got (type 0/1, value, admittance). */
+sbh_status sbh_nc_project_mesh(const sbh_nc_project* project, double* nodes, int64_t* connectivity,
+                               int32_t* counts, uint8_t* is_evaluation, int32_t* bc_type, sbh_c64* bc_value,
+                               sbh_c64* admittance, uint8_t* has_admittance);
+/* Centroids of the evaluation elements (NumCalcProject::evaluation_centers,
+ * project.rs:65-71), num_evaluation x 3. */
+sbh_status sbh_nc_project_evaluation_centers(const sbh_nc_project* project, double* out);
+/* The object's Tri3 elements with their BCs as an sbh_mesh with vertex-order
+ * normals (run_project's surface_elements, runner.rs:63-68); Quad4 surface
+ * elements give SBH_ERR_UNSUPPORTED. */
+sbh_status sbh_nc_project_surface_mesh(const sbh_nc_project* project, sbh_context* ctx, sbh_mesh** out);
 
 /* ---------------------------------------------------------------------------
  * Test hooks (no GPU work): the host route builder's view of one pair, so

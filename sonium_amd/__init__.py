@@ -197,7 +197,12 @@ class TbemMesh:
     (zero = rigid); ``admittance`` complex per panel, NaN where absent.
     """
 
-    def __init__(self, ctx: Context, mesh: Mesh, bc_type=None, bc_value=None, admittance=None):
+    def __init__(self, ctx: Context, mesh: Mesh, bc_type=None, bc_value=None, admittance=None,
+                 normals: str = "outward"):
+        """normals: "outward" orients each vertex-order normal away from the
+        origin (compute_element_geometry, generators.rs:857-866); "vertex_order"
+        keeps (v1-v0)x(v2-v0), as meshes read from files are
+        (compute_element_normal, element.rs:368-392)."""
         self.ctx = ctx
         self.mesh = mesh
         n = mesh.num_elements
@@ -210,11 +215,12 @@ class TbemMesh:
             has = np.ascontiguousarray(~np.isnan(a.real), dtype=np.uint8)
             ad = np.nan_to_num(a)
         self._keep = (bt, bv, ad, has)
+        rule = {"outward": 0, "vertex_order": 1}[normals]
         h = ctypes.c_void_p()
-        _check(_lib().sbh_mesh_create(
+        _check(_lib().sbh_mesh_create_oriented(
             ctx.handle, _cptr(mesh.nodes), mesh.nodes.shape[0], _cptr(mesh.triangles), n,
             None if bt is None else _cptr(bt), None if bv is None else _cptr(bv),
-            None if ad is None else _cptr(ad), None if has is None else _cptr(has), ctypes.byref(h)))
+            None if ad is None else _cptr(ad), None if has is None else _cptr(has), rule, ctypes.byref(h)))
         self._h = h
 
     @property

@@ -38,6 +38,17 @@ class physics_t(ctypes.Structure):
         "harmonic_factor", "pressure_factor", "tau")]
 
 
+class nc_info_t(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "num_nodes", "num_elements", "num_evaluation", "num_frequencies", "num_curve_points",
+        "num_plane_waves", "num_point_sources", "num_boundary_specs", "num_node_files", "num_element_files")] + [
+        ("mesh_loaded", ctypes.c_int32), ("main_params_i", ctypes.c_int32 * 9),
+        ("main_params_ii", ctypes.c_double * 7), ("speed_of_sound", ctypes.c_double),
+        ("density", ctypes.c_double), ("reference_pressure", ctypes.c_double),
+        ("has_symmetry", ctypes.c_int32), ("symmetry_flags", ctypes.c_int32 * 3),
+        ("symmetry_origin", ctypes.c_double * 3)]
+
+
 class gmres_config_t(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64),
                 ("tolerance", ctypes.c_double)]
@@ -128,6 +139,16 @@ SIGNATURES = {
                                            _vp, _vp, _i64, _vp, _int, _int]),
     "sbh_workspace_assemble_derivative": (_int, [_vp, ctypes.POINTER(physics_t), c64, c64, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
+    "sbh_mesh_create_oriented": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _int, _pp]),
+    "sbh_nc_project_load": (_int, [ctypes.c_char_p, _pp]),
+    "sbh_nc_parse": (_int, [ctypes.c_char_p, ctypes.c_char_p, _pp]),
+    "sbh_nc_project_destroy": (None, [_vp]),
+    "sbh_nc_project_info": (_int, [_vp, ctypes.POINTER(nc_info_t)]),
+    "sbh_nc_project_string": (_int, [_vp, _int, _i64, ctypes.c_char_p, _i64]),
+    "sbh_nc_project_values": (_int, [_vp, _int, _vp, _i64, ctypes.POINTER(ctypes.c_int64)]),
+    "sbh_nc_project_mesh": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sbh_nc_project_evaluation_centers": (_int, [_vp, _vp]),
+    "sbh_nc_project_surface_mesh": (_int, [_vp, _vp, _pp]),
     "sbh_field_pressure": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
                                   _vp, _vp, _int]),
     "sbh_dense_operator_create": (_int, [_vp, _pp]),

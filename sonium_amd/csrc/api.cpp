@@ -337,8 +337,18 @@ sbh_status sbh_mesh_create(sbh_context* ctx, const double* nodes, int64_t num_no
                            const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
                            const sbh_c64* bc_value, const sbh_c64* admittance,
                            const uint8_t* has_admittance, sbh_mesh** out) {
+    return sbh_mesh_create_oriented(ctx, nodes, num_nodes, triangles, num_elements, bc_type, bc_value, admittance,
+                                    has_admittance, SBH_NORMALS_OUTWARD, out);
+}
+
+sbh_status sbh_mesh_create_oriented(sbh_context* ctx, const double* nodes, int64_t num_nodes,
+                                    const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
+                                    const sbh_c64* bc_value, const sbh_c64* admittance,
+                                    const uint8_t* has_admittance, int normal_rule, sbh_mesh** out) {
     return guard([&] {
         SBH_REQUIRE(ctx && out, "ctx/out is null");
+        SBH_REQUIRE(normal_rule == SBH_NORMALS_OUTWARD || normal_rule == SBH_NORMALS_VERTEX_ORDER,
+                    "unknown normal rule");
         *out = nullptr;
         SBH_REQUIRE(num_nodes >= 3 && nodes, "mesh needs at least 3 nodes");
         SBH_REQUIRE(num_elements >= 1 && triangles, "mesh needs at least one element");
@@ -354,6 +364,7 @@ sbh_status sbh_mesh_create(sbh_context* ctx, const double* nodes, int64_t num_no
         m->num_nodes = num_nodes;
         m->nodes.assign(nodes, nodes + 3 * num_nodes);
         m->tris.assign(triangles, triangles + 3 * num_elements);
+        m->vertex_order_normals = normal_rule == SBH_NORMALS_VERTEX_ORDER;
         build_mesh_geometry(m.get());
         m->bc.assign(num_elements, PanelBc{});
         bool general = false;

@@ -221,6 +221,7 @@ struct sbh_mesh {
     std::vector<sbh::PanelBc> bc;
     double diameter = 0.0;          // bounding-box diagonal of the nodes
     bool general = false;           // any pressure BC, admittance or nonzero BC value
+    bool vertex_order_normals = false;  // keep (v1-v0)x(v2-v0) instead of orienting away from the origin
     std::vector<int32_t> nonzero_bc;  // panels whose BC value is nonzero (tbem.rs:1240-1242)
     sbh::DeviceArray<sbh::FarPanel> d_far;
     sbh::DeviceArray<sbh::PanelGeom> d_geom;

@@ -272,9 +272,10 @@ void build_mesh_geometry(sbh_mesh* m) {
             nrm[1] = cr[1] / len;
             nrm[2] = cr[2] / len;
         }
-        // generators.rs:857-866 outward flip
+        // generators.rs:857-866 outward flip; project meshes keep the vertex
+        // order (compute_element_normal, element.rs:368-392, used by load_project)
         double ndc = nrm[0] * cen[0] + nrm[1] * cen[1] + nrm[2] * cen[2];
-        if (ndc < 0.0) {
+        if (!m->vertex_order_normals && ndc < 0.0) {
             nrm[0] = -nrm[0];
             nrm[1] = -nrm[1];
             nrm[2] = -nrm[2];
