@@ -242,6 +242,35 @@ sbh_status sbh_field_pressure(sbh_context* ctx, const sbh_mesh* mesh, const sbh_
                               const double* points, int64_t num_points, sbh_c64* scattered,
                               sbh_c64* incident, sbh_c64* total, int memory);
 
+/* One incident source: IncidentField::PlaneWave {direction, amplitude} or
+ * IncidentField::PointSource {position, strength}
+ * (crates/sonium-bem/src/core/incident.rs:17-39, 216-420):
+ *   plane wave   p = A exp(i hf k d.x),            dp/dn = i hf k (d.n) p
+ *   point source p = S exp(i hf k r) / (4 pi r),     dp/dn = (i hf k - 1/r) p (x-s).n / r
+ * with r = |x - s|; a point within 1e-10 of the source gets 0 (as the
+ * reference skips it).  Plane-wave directions are used as given here. */
+typedef enum { SBH_SOURCE_PLANE_WAVE = 0, SBH_SOURCE_POINT = 1 } sbh_source_kind;
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double vec[3];     /* direction (plane wave) or position (point source) */
+    sbh_c64 amplitude; /* amplitude or strength */
+} sbh_incident_source;
+/* IncidentField::compute_rhs_with_beta for the sum of the sources
+ * (MultiplePlaneWaves / MultiplePointSources, incident.rs:807-834):
+ * rhs += sign * -(gamma p_inc + beta tau dp_inc/dn) at the panel centres, one
+ * column of N values (device or host), accumulate = 0 overwrites. */
+sbh_status sbh_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* physics, sbh_c64 beta,
+                            double sign, const sbh_incident_source* sources, int64_t num_sources, sbh_c64* rhs,
+                            int memory, int accumulate);
+/* sbh_field_pressure with a general incident field: the sources are summed
+ * for the incident pressure and for the Cauchy data of pressure panels. */
+sbh_status sbh_field_pressure_sources(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* physics,
+                                      const sbh_c64* solution, const sbh_c64* surface_velocity,
+                                      const sbh_incident_source* sources, int64_t num_sources,
+                                      const double* points, int64_t num_points, sbh_c64* scattered,
+                                      sbh_c64* incident, sbh_c64* total, int memory);
+
 /* ---------------------------------------------------------------------------
  * LinearOperator<Complex64> over a device-resident dense matrix —
  * DenseMatrixOperator (bem_solver.rs:6548-6593) / DenseOperator

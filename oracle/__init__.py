@@ -233,9 +233,18 @@ def plane_wave_rhs(centers, normals, k, beta, direction=(0.0, 0.0, 1.0), amplitu
     return out
 
 
-def field(points, nodes, tris, p_surf, k, *, v_surf=None, directions=None, amplitudes=None,
-          density=1.21, speed_of_sound=343.0, harmonic_factor=1.0, threads=None):
-    """Total (with directions) or scattered pressure at points (pressure.rs:1388-1480, 2105-2150)."""
+def field(points, nodes, tris, p_surf, k, *, v_surf=None, directions=None, amplitudes=None, positions=None,
+          strengths=None, density=1.21, speed_of_sound=343.0, harmonic_factor=1.0, threads=None):
+    """Total (with incident sources) or scattered pressure at points
+    (pressure.rs:1388-1480, 2105-2150); point sources are added to the
+    plane waves through incident_values."""
+    if positions is not None:
+        scat = field(points, nodes, tris, p_surf, k, v_surf=v_surf, directions=directions, amplitudes=amplitudes,
+                     density=density, speed_of_sound=speed_of_sound, harmonic_factor=harmonic_factor,
+                     threads=threads)
+        p_ps, _ = incident_values(points, None, k, positions=positions, strengths=strengths,
+                                  harmonic_factor=harmonic_factor)
+        return scat + p_ps
     pts = _f64(points).reshape(-1, 3)
     nodes = _f64(nodes)
     tris = np.ascontiguousarray(tris, dtype=np.int64)
@@ -258,8 +267,53 @@ def field(points, nodes, tris, p_surf, k, *, v_surf=None, directions=None, ampli
     return out
 
 
+def incident_values(points, normals, k, *, directions=None, amplitudes=None, positions=None, strengths=None,
+                    harmonic_factor=1.0):
+    """p_inc and dp_inc/dn of summed plane waves (unit directions) and point
+    sources (incident.rs:216-420): p = A e^{i hf k d.x}, dp/dn = i hf k (d.n) p;
+    p = S e^{i hf k r}/(4 pi r), dp/dn = (i hf k - 1/r) p (x - s).n / r, zero
+    within 1e-10 of a source."""
+    x = _f64(points).reshape(-1, 3)
+    nrm = None if normals is None else _f64(normals).reshape(-1, 3)
+    p = np.zeros(x.shape[0], dtype=np.complex128)
+    q = np.zeros(x.shape[0], dtype=np.complex128)
+    hk = harmonic_factor * k
+    if directions is not None:
+        d = _f64(directions).reshape(-1, 3)
+        a = np.ones(d.shape[0], dtype=np.complex128) if amplitudes is None else \
+            np.asarray(amplitudes, dtype=np.complex128).reshape(d.shape[0])
+        for w in range(d.shape[0]):
+            pw = a[w] * np.exp(1j * hk * (x @ d[w]))
+            p += pw
+            if nrm is not None:
+                q += 1j * hk * (nrm @ d[w]) * pw
+    if positions is not None:
+        sp = _f64(positions).reshape(-1, 3)
+        st = np.ones(sp.shape[0], dtype=np.complex128) if strengths is None else \
+            np.asarray(strengths, dtype=np.complex128).reshape(sp.shape[0])
+        for w in range(sp.shape[0]):
+            dx = x - sp[w]
+            r = np.sqrt(np.sum(dx * dx, axis=1))
+            ok = r > 1e-10
+            rr = np.where(ok, r, 1.0)
+            g = np.where(ok, np.exp(1j * hk * rr) / (4.0 * np.pi * rr), 0.0)
+            p += st[w] * g
+            if nrm is not None:
+                q += st[w] * (1j * hk - 1.0 / rr) * g * np.sum(dx * nrm, axis=1) / rr
+    return p, q
+
+
+def source_rhs(centers, normals, k, beta, *, directions=None, amplitudes=None, positions=None, strengths=None,
+               sign=1.0, tau=1.0, harmonic_factor=1.0):
+    """compute_rhs_with_beta for summed sources (incident.rs:807-834), times sign."""
+    p, q = incident_values(centers, normals, k, directions=directions, amplitudes=amplitudes, positions=positions,
+                           strengths=strengths, harmonic_factor=harmonic_factor)
+    return sign * -(p + complex(beta) * tau * q)
+
+
 def cauchy_data(centers, normals, solution, k, *, bc_type=None, bc_value=None, admittance=None,
-                directions=None, amplitudes=None, density=1.21, speed_of_sound=343.0, harmonic_factor=1.0):
+                directions=None, amplitudes=None, positions=None, strengths=None, density=1.21,
+                speed_of_sound=343.0, harmonic_factor=1.0):
     """Per-panel (p, v) for the field integral when no velocity is given:
     constant_cauchy_data (pressure.rs:1481-1520) with incident_cauchy_data and
     normal_derivative_to_velocity (pressure.rs:1522-1548).  Velocity panels
@@ -275,17 +329,11 @@ def cauchy_data(centers, normals, solution, k, *, bc_type=None, bc_value=None, a
     bv = np.zeros(n, dtype=np.complex128) if bc_value is None else np.asarray(bc_value, dtype=np.complex128).reshape(n)
     adm = np.full(n, np.nan + 0j) if admittance is None else np.asarray(admittance, dtype=np.complex128).reshape(n)
     pf = density * k * speed_of_sound * harmonic_factor
-    p_inc = np.zeros(n, dtype=np.complex128)
-    q_inc = np.zeros(n, dtype=np.complex128)
     if directions is not None:
-        d = _f64(directions).reshape(-1, 3)
-        d = d / np.linalg.norm(d, axis=1, keepdims=True)
-        amps = np.ones(d.shape[0], dtype=np.complex128) if amplitudes is None else \
-            np.asarray(amplitudes, dtype=np.complex128).reshape(d.shape[0])
-        for w in range(d.shape[0]):
-            pw = amps[w] * np.exp(1j * harmonic_factor * k * (c @ d[w]))
-            p_inc += pw
-            q_inc += 1j * harmonic_factor * k * (nrm @ d[w]) * pw
+        directions = _f64(directions).reshape(-1, 3)
+        directions = directions / np.linalg.norm(directions, axis=1, keepdims=True)
+    p_inc, q_inc = incident_values(c, nrm, k, directions=directions, amplitudes=amplitudes, positions=positions,
+                                   strengths=strengths, harmonic_factor=harmonic_factor)
     p = sol.copy()
     v = bv.copy()
     has_adm = ~np.isnan(adm.real)

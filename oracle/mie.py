@@ -166,3 +166,38 @@ def weighted_relative_l2(p: np.ndarray, p_ref: np.ndarray, weights: np.ndarray) 
     num = np.sum(w * np.abs(p - p_ref) ** 2)
     den = np.sum(w * np.abs(p_ref) ** 2)
     return float(math.sqrt(num / den))
+
+
+def point_source_rigid_sphere(k: float, radius: float, source, points, strength: complex = 1.0,
+                              num_terms: int = 40) -> np.ndarray:
+    """Total field of a point source S e^{ikR}/(4 pi R) outside a rigid sphere.
+
+    With e^{ikR}/(4 pi R) = (ik/4pi) h0(kR) and the addition theorem
+    h0(k|x - s|) = sum_n (2n+1) j_n(k r<) h_n(k r>) P_n(cos g), the rigid
+    sphere adds -(ik S/4pi) sum_n (2n+1) a_n h_n(k r_s) h_n(k r) P_n(cos g),
+    a_n = j_n'(ka)/h_n'(ka) (the rigid coefficients above); g is the angle
+    between x and s.  Points must satisfy a <= |x|, |x| != |s|.
+    """
+    s_vec = np.asarray(source, dtype=np.float64)
+    rs = float(np.linalg.norm(s_vec))
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    a = rigid_sphere_coefficients(k * radius, num_terms)
+    out = np.zeros(pts.shape[0], dtype=np.complex128)
+    with mpmath.workdps(_DPS):
+        hs = [mpmath.mpc(*_sph(n, mpmath.mpf(k * rs))) for n in range(num_terms)]
+        for idx, x in enumerate(pts):
+            r = float(np.linalg.norm(x))
+            cg = float(np.clip(x @ s_vec / (r * rs), -1.0, 1.0))
+            P = _legendre_table(num_terms, np.array([cg]))[:, 0]
+            tot = mpmath.mpc(0)
+            for n in range(num_terms):
+                j, y = _sph(n, mpmath.mpf(k * r))
+                hr = mpmath.mpc(j, y)
+                if r < rs:
+                    inc = mpmath.mpf(j) * hs[n]
+                else:
+                    jn_s = _sph(n, mpmath.mpf(k * rs))[0]
+                    inc = mpmath.mpf(jn_s) * hr
+                tot += (2 * n + 1) * (inc - mpmath.mpc(a[n].real, a[n].imag) * hs[n] * hr) * P[n]
+            out[idx] = complex(mpmath.mpc(0, k) / (4 * mpmath.pi) * strength * tot)
+    return out

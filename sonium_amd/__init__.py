@@ -420,13 +420,21 @@ def build_tbem_system(ctx: Context, mesh: TbemMesh, physics: PhysicsParams) -> T
 
 
 class IncidentField:
-    """Plane-wave incident field (incident.rs:165-198)."""
+    """Incident field (incident.rs:17-39, 165-420): plane waves
+    IncidentField::plane_wave(direction, amplitude) (directions normalised, as
+    that constructor does) and point sources IncidentField::PointSource
+    {position, strength}.  Each source is one right-hand-side column; the
+    field evaluation sums them (MultiplePlaneWaves / MultiplePointSources)."""
+
+    PLANE_WAVE, POINT_SOURCE = 0, 1
 
     def __init__(self, directions, amplitudes=None):
         d = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 3)
         self.directions = d
         amps = np.ones(d.shape[0], dtype=np.complex128) if amplitudes is None else _c128(amplitudes, d.shape[0])
         self.amplitudes = amps
+        self.kinds = np.zeros(d.shape[0], dtype=np.int32)
+        self.normalise = True
 
     @classmethod
     def plane_wave_z(cls) -> "IncidentField":
@@ -436,15 +444,59 @@ class IncidentField:
     def plane_wave(cls, direction, amplitude: float = 1.0) -> "IncidentField":
         return cls([direction], [amplitude])
 
+    @classmethod
+    def point_sources(cls, positions, strengths=None) -> "IncidentField":
+        """IncidentField::PointSource {position, strength}: p = S e^{ikr}/(4 pi r)."""
+        f = cls(positions, strengths)
+        f.kinds[:] = cls.POINT_SOURCE
+        return f
+
+    @classmethod
+    def from_sources(cls, kinds, vectors, amplitudes) -> "IncidentField":
+        """Sources exactly as given (the enum variants built directly: plane-wave
+        directions are not normalised)."""
+        f = cls(vectors, amplitudes)
+        f.kinds = np.ascontiguousarray(kinds, dtype=np.int32).reshape(-1)
+        f.normalise = False
+        return f
+
     @property
     def num_rhs(self) -> int:
         return int(self.directions.shape[0])
+
+    @property
+    def _plane_only(self) -> bool:
+        return self.normalise and not np.any(self.kinds != self.PLANE_WAVE)
+
+    def source_array(self):
+        """The sources as sbh_incident_source records (plane-wave directions
+        normalised like IncidentField::plane_wave unless built from_sources)."""
+        arr = (_ffi.incident_source_t * self.num_rhs)()
+        for i in range(self.num_rhs):
+            v = self.directions[i].copy()
+            if self.kinds[i] == self.PLANE_WAVE and self.normalise:
+                ln = float(np.sqrt(v @ v))
+                v = v / ln if ln > 1e-10 else np.array([0.0, 0.0, -1.0])
+            arr[i].kind = int(self.kinds[i])
+            arr[i].vec[:] = [float(x) for x in v]
+            arr[i].re = float(self.amplitudes[i].real)
+            arr[i].im = float(self.amplitudes[i].imag)
+        return arr
 
     def compute_rhs_with_beta(self, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
                               sign: float = 1.0) -> np.ndarray:
         """sign * -(gamma p_inc + beta tau dp_inc/dn) at the panel centres, N x num_rhs."""
         out = np.zeros((mesh.num_elements, self.num_rhs), dtype=np.complex128)
-        self._launch(mesh, physics, beta, sign, _cptr(out), MEM_HOST, accumulate=False)
+        if self._plane_only:
+            self._launch(mesh, physics, beta, sign, _cptr(out), MEM_HOST, accumulate=False)
+        else:
+            src = self.source_array()
+            col = np.zeros(mesh.num_elements, dtype=np.complex128)
+            b = _ffi.c64(complex(beta).real, complex(beta).imag)
+            for i in range(self.num_rhs):
+                _check(_lib().sbh_incident_rhs(mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), b, float(sign),
+                                               ctypes.byref(src[i]), 1, _cptr(col), MEM_HOST, 0))
+                out[:, i] = col
         return out if self.num_rhs > 1 else out[:, 0]
 
     def add_to_system(self, system: TbemSystem, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
@@ -453,11 +505,22 @@ class IncidentField:
         if self.num_rhs != 1:
             raise ValueError("a TbemSystem holds one right-hand side")
         _, _, rhs_ptr = system.device_pointers()
-        self._launch(mesh, physics, beta, sign, ctypes.c_void_p(rhs_ptr), MEM_DEVICE, accumulate=True)
+        self.write_device_ptr(ctypes.c_void_p(rhs_ptr), mesh, physics, beta, sign, accumulate=True)
 
     def write_device(self, vec: DeviceVector, mesh: TbemMesh, physics: PhysicsParams, beta: complex,
                      sign: float = 1.0, accumulate: bool = False):
-        self._launch(mesh, physics, beta, sign, ctypes.c_void_p(vec.ptr), MEM_DEVICE, accumulate)
+        self.write_device_ptr(ctypes.c_void_p(vec.ptr), mesh, physics, beta, sign, accumulate)
+
+    def write_device_ptr(self, ptr, mesh, physics, beta, sign=1.0, accumulate=False):
+        if self._plane_only:
+            self._launch(mesh, physics, beta, sign, ptr, MEM_DEVICE, accumulate)
+            return
+        if self.num_rhs != 1:
+            raise ValueError("device right-hand sides with point sources take one source per column")
+        b = _ffi.c64(complex(beta).real, complex(beta).imag)
+        src = self.source_array()
+        _check(_lib().sbh_incident_rhs(mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), b, float(sign), src, 1,
+                                       ptr, MEM_DEVICE, int(accumulate)))
 
     def _launch(self, mesh, physics, beta, sign, ptr, memory, accumulate):
         b = _ffi.c64(complex(beta).real, complex(beta).imag)
@@ -476,11 +539,17 @@ def _field(mesh, solution, surface_velocity, physics, incident, points, with_inc
     inc = np.zeros(m, dtype=np.complex128) if with_incident else None
     tot = np.zeros(m, dtype=np.complex128) if with_incident else None
     nw = 0 if incident is None else incident.num_rhs
-    _check(_lib().sbh_field_pressure(
-        mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), _cptr(sol), None if vel is None else _cptr(vel),
-        None if incident is None else _cptr(incident.directions),
-        None if incident is None else _cptr(incident.amplitudes), nw, _cptr(pts), m, _cptr(scat),
-        None if inc is None else _cptr(inc), None if tot is None else _cptr(tot), MEM_HOST))
+    outs = (_cptr(pts), m, _cptr(scat), None if inc is None else _cptr(inc), None if tot is None else _cptr(tot),
+            MEM_HOST)
+    if incident is None or incident._plane_only:
+        _check(_lib().sbh_field_pressure(
+            mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), _cptr(sol), None if vel is None else _cptr(vel),
+            None if incident is None else _cptr(incident.directions),
+            None if incident is None else _cptr(incident.amplitudes), nw, *outs))
+    else:
+        _check(_lib().sbh_field_pressure_sources(
+            mesh.ctx.handle, mesh.handle, ctypes.byref(physics._p), _cptr(sol), None if vel is None else _cptr(vel),
+            incident.source_array(), nw, *outs))
     return pts, scat, inc, tot
 
 

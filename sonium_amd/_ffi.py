@@ -49,6 +49,11 @@ class nc_info_t(ctypes.Structure):
         ("symmetry_origin", ctypes.c_double * 3)]
 
 
+class incident_source_t(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("vec", ctypes.c_double * 3),
+                ("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
 class gmres_config_t(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64),
                 ("tolerance", ctypes.c_double)]
@@ -149,6 +154,10 @@ SIGNATURES = {
     "sbh_nc_project_mesh": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sbh_nc_project_evaluation_centers": (_int, [_vp, _vp]),
     "sbh_nc_project_surface_mesh": (_int, [_vp, _vp, _pp]),
+    "sbh_incident_rhs": (_int, [_vp, _vp, ctypes.POINTER(physics_t), c64, ctypes.c_double, _vp, _i64, _vp, _int,
+                                _int]),
+    "sbh_field_pressure_sources": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp, _vp, _vp, _i64, _vp, _i64, _vp,
+                                          _vp, _vp, _int]),
     "sbh_field_pressure": (_int, [_vp, _vp, ctypes.POINTER(physics_t), _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
                                   _vp, _vp, _int]),
     "sbh_dense_operator_create": (_int, [_vp, _pp]),

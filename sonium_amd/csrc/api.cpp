@@ -675,96 +675,159 @@ static std::vector<double> normalized_directions(const double* directions, int64
     return dirs;
 }
 
+// Validation and staging shared by the field entry points; sources are host memory.
+static void field_pressure_impl(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph, const sbh_c64* solution,
+                                const sbh_c64* surface_velocity, const std::vector<IncidentSource>& sources,
+                                const double* points, int64_t num_points, sbh_c64* scattered, sbh_c64* incident,
+                                sbh_c64* total, int memory) {
+    SBH_REQUIRE(ctx && mesh && ph && solution, "null argument");
+    SBH_REQUIRE(num_points >= 0, "counts must be non-negative");
+    SBH_REQUIRE(num_points == 0 || (points && scattered), "points/scattered is null");
+    SBH_REQUIRE(std::isfinite(ph->wave_number) && std::isfinite(ph->harmonic_factor) &&
+                    std::isfinite(ph->pressure_factor),
+                "physics parameters must be finite");
+    check_memory(memory);
+    use_device(ctx);
+    if (num_points == 0) return;
+    const int64_t n = mesh->n;
+    for (const IncidentSource& src : sources) {
+        SBH_REQUIRE(src.kind == SBH_SOURCE_PLANE_WAVE || src.kind == SBH_SOURCE_POINT, "unknown source kind");
+        SBH_REQUIRE(std::isfinite(src.v[0]) && std::isfinite(src.v[1]) && std::isfinite(src.v[2]) &&
+                        std::isfinite(src.amp[0]) && std::isfinite(src.amp[1]),
+                    "incident sources must be finite");
+    }
+    // validate_field_inputs (pressure.rs:627-631)
+    std::vector<double> pts_copy;
+    const double* host_pts = points;
+    if (memory == SBH_MEM_DEVICE) {
+        pts_copy.resize((size_t)(3 * num_points));
+        SBH_HIP(hipMemcpy(pts_copy.data(), points, pts_copy.size() * sizeof(double), hipMemcpyDeviceToHost));
+        host_pts = pts_copy.data();
+    }
+    for (int64_t i = 0; i < 3 * num_points; ++i)
+        if (!std::isfinite(host_pts[i]))
+            throw Error(SBH_ERR_INVALID_ARGUMENT,
+                        "evaluation point " + std::to_string(i / 3) + " has non-finite coordinates");
+    // NonFiniteSurfaceSample (validate_field_inputs, pressure.rs:690-733)
+    for (const sbh_c64* samples : {solution, surface_velocity}) {
+        if (!samples) continue;
+        std::vector<sbh_c64> copy;
+        const sbh_c64* h = samples;
+        if (memory == SBH_MEM_DEVICE) {
+            copy.resize((size_t)n);
+            SBH_HIP(hipMemcpy(copy.data(), samples, copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
+            h = copy.data();
+        }
+        for (int64_t j = 0; j < n; ++j)
+            if (!std::isfinite(h[j].re) || !std::isfinite(h[j].im))
+                throw Error(SBH_ERR_INVALID_ARGUMENT, "surface sample " + std::to_string(j) + " is not finite");
+    }
+    DeviceArray<IncidentSource> d_src(sources.size());
+    if (!sources.empty())
+        SBH_HIP(hipMemcpyAsync(d_src.ptr, sources.data(), d_src.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    const double2* d_sol = reinterpret_cast<const double2*>(solution);
+    const double2* d_vel = reinterpret_cast<const double2*>(surface_velocity);
+    const double* d_pts = points;
+    double2* outs[3] = {reinterpret_cast<double2*>(scattered), reinterpret_cast<double2*>(incident),
+                        reinterpret_cast<double2*>(total)};
+    DeviceArray<double2> h_sol, h_vel, h_out[3];
+    DeviceArray<double> h_pts;
+    if (memory == SBH_MEM_HOST) {
+        h_sol.allocate((size_t)n);
+        h_pts.allocate((size_t)(3 * num_points));
+        SBH_HIP(hipMemcpyAsync(h_sol.ptr, solution, h_sol.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        SBH_HIP(hipMemcpyAsync(h_pts.ptr, points, h_pts.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        if (surface_velocity) {
+            h_vel.allocate((size_t)n);
+            SBH_HIP(hipMemcpyAsync(h_vel.ptr, surface_velocity, h_vel.bytes(), hipMemcpyHostToDevice, ctx->stream));
+            d_vel = h_vel.ptr;
+        }
+        d_sol = h_sol.ptr;
+        d_pts = h_pts.ptr;
+        for (int o = 0; o < 3; ++o)
+            if (outs[o]) {
+                h_out[o].allocate((size_t)num_points);
+                outs[o] = h_out[o].ptr;
+            }
+    }
+    launch_field(ctx, mesh, *ph, d_sol, d_vel, d_pts, num_points, d_src.ptr, (int64_t)sources.size(), outs[0],
+                 outs[1], outs[2]);
+    // validate_finite_results (pressure.rs:735-742): every returned value must be finite
+    sbh_c64* user[3] = {scattered, incident, total};
+    std::vector<sbh_c64> host_copy((size_t)num_points);
+    for (int o = 0; o < 3; ++o) {
+        if (!outs[o]) continue;
+        SBH_HIP(hipMemcpy(host_copy.data(), outs[o], host_copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < num_points; ++i)
+            if (!std::isfinite(host_copy[i].re) || !std::isfinite(host_copy[i].im))
+                throw Error(SBH_ERR_NUMERICAL, "field value " + std::to_string(i) + " is not finite");
+        if (memory == SBH_MEM_HOST) std::memcpy(user[o], host_copy.data(), host_copy.size() * sizeof(sbh_c64));
+    }
+}
+
 sbh_status sbh_field_pressure(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph, const sbh_c64* solution,
                               const sbh_c64* surface_velocity, const double* directions, const sbh_c64* amplitudes,
                               int64_t num_waves, const double* points, int64_t num_points, sbh_c64* scattered,
                               sbh_c64* incident, sbh_c64* total, int memory) {
     return guard([&] {
-        SBH_REQUIRE(ctx && mesh && ph && solution, "null argument");
-        SBH_REQUIRE(num_points >= 0 && num_waves >= 0, "counts must be non-negative");
-        SBH_REQUIRE(num_points == 0 || (points && scattered), "points/scattered is null");
+        SBH_REQUIRE(num_waves >= 0, "counts must be non-negative");
         SBH_REQUIRE(num_waves == 0 || (directions && amplitudes), "directions/amplitudes is null");
-        SBH_REQUIRE(std::isfinite(ph->wave_number) && std::isfinite(ph->harmonic_factor) &&
-                        std::isfinite(ph->pressure_factor),
-                    "physics parameters must be finite");
+        const std::vector<double> dirs = normalized_directions(directions, num_waves);
+        std::vector<IncidentSource> src((size_t)num_waves);
+        for (int64_t w = 0; w < num_waves; ++w) {
+            src[w].kind = SBH_SOURCE_PLANE_WAVE;
+            src[w].pad = 0;
+            for (int d = 0; d < 3; ++d) src[w].v[d] = dirs[3 * w + d];
+            src[w].amp[0] = amplitudes[w].re;
+            src[w].amp[1] = amplitudes[w].im;
+        }
+        field_pressure_impl(ctx, mesh, ph, solution, surface_velocity, src, points, num_points, scattered, incident,
+                            total, memory);
+    });
+}
+
+sbh_status sbh_field_pressure_sources(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph,
+                                      const sbh_c64* solution, const sbh_c64* surface_velocity,
+                                      const sbh_incident_source* sources, int64_t num_sources, const double* points,
+                                      int64_t num_points, sbh_c64* scattered, sbh_c64* incident, sbh_c64* total,
+                                      int memory) {
+    return guard([&] {
+        SBH_REQUIRE(num_sources >= 0 && (num_sources == 0 || sources), "sources is null");
+        const IncidentSource* s = reinterpret_cast<const IncidentSource*>(sources);
+        std::vector<IncidentSource> src(s, s + num_sources);
+        field_pressure_impl(ctx, mesh, ph, solution, surface_velocity, src, points, num_points, scattered, incident,
+                            total, memory);
+    });
+}
+
+sbh_status sbh_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph, sbh_c64 beta, double sign,
+                            const sbh_incident_source* sources, int64_t num_sources, sbh_c64* rhs, int memory,
+                            int accumulate) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && mesh && ph && rhs, "null argument");
+        SBH_REQUIRE(num_sources >= 0 && (num_sources == 0 || sources), "sources is null");
         check_memory(memory);
         use_device(ctx);
-        if (num_points == 0) return;
-        const int64_t n = mesh->n;
-        const std::vector<double> dirs = normalized_directions(directions, num_waves);
-        for (int64_t w = 0; w < num_waves; ++w)
-            SBH_REQUIRE(std::isfinite(amplitudes[w].re) && std::isfinite(amplitudes[w].im),
-                        "amplitudes must be finite");
-        // validate_field_inputs (pressure.rs:627-631)
-        std::vector<double> pts_copy;
-        const double* host_pts = points;
+        const IncidentSource* s = reinterpret_cast<const IncidentSource*>(sources);
+        for (int64_t w = 0; w < num_sources; ++w) {
+            SBH_REQUIRE(s[w].kind == SBH_SOURCE_PLANE_WAVE || s[w].kind == SBH_SOURCE_POINT, "unknown source kind");
+            SBH_REQUIRE(std::isfinite(s[w].v[0]) && std::isfinite(s[w].v[1]) && std::isfinite(s[w].v[2]) &&
+                            std::isfinite(s[w].amp[0]) && std::isfinite(s[w].amp[1]),
+                        "incident sources must be finite");
+        }
+        DeviceArray<IncidentSource> d_src((size_t)num_sources);
+        if (num_sources)
+            SBH_HIP(hipMemcpyAsync(d_src.ptr, s, d_src.bytes(), hipMemcpyHostToDevice, ctx->stream));
         if (memory == SBH_MEM_DEVICE) {
-            pts_copy.resize((size_t)(3 * num_points));
-            SBH_HIP(hipMemcpy(pts_copy.data(), points, pts_copy.size() * sizeof(double), hipMemcpyDeviceToHost));
-            host_pts = pts_copy.data();
-        }
-        for (int64_t i = 0; i < 3 * num_points; ++i)
-            if (!std::isfinite(host_pts[i]))
-                throw Error(SBH_ERR_INVALID_ARGUMENT,
-                            "evaluation point " + std::to_string(i / 3) + " has non-finite coordinates");
-        // NonFiniteSurfaceSample (validate_field_inputs, pressure.rs:690-733)
-        for (const sbh_c64* samples : {solution, surface_velocity}) {
-            if (!samples) continue;
-            std::vector<sbh_c64> copy;
-            const sbh_c64* h = samples;
-            if (memory == SBH_MEM_DEVICE) {
-                copy.resize((size_t)n);
-                SBH_HIP(hipMemcpy(copy.data(), samples, copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
-                h = copy.data();
-            }
-            for (int64_t j = 0; j < n; ++j)
-                if (!std::isfinite(h[j].re) || !std::isfinite(h[j].im))
-                    throw Error(SBH_ERR_INVALID_ARGUMENT, "surface sample " + std::to_string(j) + " is not finite");
-        }
-        DeviceArray<double> d_dirs((size_t)(3 * num_waves));
-        DeviceArray<double2> d_amps((size_t)num_waves);
-        if (num_waves) {
-            SBH_HIP(hipMemcpyAsync(d_dirs.ptr, dirs.data(), d_dirs.bytes(), hipMemcpyHostToDevice, ctx->stream));
-            SBH_HIP(hipMemcpyAsync(d_amps.ptr, amplitudes, d_amps.bytes(), hipMemcpyHostToDevice, ctx->stream));
-        }
-        const double2* d_sol = reinterpret_cast<const double2*>(solution);
-        const double2* d_vel = reinterpret_cast<const double2*>(surface_velocity);
-        const double* d_pts = points;
-        double2* outs[3] = {reinterpret_cast<double2*>(scattered), reinterpret_cast<double2*>(incident),
-                            reinterpret_cast<double2*>(total)};
-        DeviceArray<double2> h_sol, h_vel, h_out[3];
-        DeviceArray<double> h_pts;
-        if (memory == SBH_MEM_HOST) {
-            h_sol.allocate((size_t)n);
-            h_pts.allocate((size_t)(3 * num_points));
-            SBH_HIP(hipMemcpyAsync(h_sol.ptr, solution, h_sol.bytes(), hipMemcpyHostToDevice, ctx->stream));
-            SBH_HIP(hipMemcpyAsync(h_pts.ptr, points, h_pts.bytes(), hipMemcpyHostToDevice, ctx->stream));
-            if (surface_velocity) {
-                h_vel.allocate((size_t)n);
-                SBH_HIP(hipMemcpyAsync(h_vel.ptr, surface_velocity, h_vel.bytes(), hipMemcpyHostToDevice,
-                                       ctx->stream));
-                d_vel = h_vel.ptr;
-            }
-            d_sol = h_sol.ptr;
-            d_pts = h_pts.ptr;
-            for (int o = 0; o < 3; ++o)
-                if (outs[o]) {
-                    h_out[o].allocate((size_t)num_points);
-                    outs[o] = h_out[o].ptr;
-                }
-        }
-        launch_field(ctx, mesh, *ph, d_sol, d_vel, d_pts, num_points, d_dirs.ptr, d_amps.ptr, num_waves, outs[0],
-                     outs[1], outs[2]);
-        // validate_finite_results (pressure.rs): every returned value must be finite
-        sbh_c64* user[3] = {scattered, incident, total};
-        std::vector<sbh_c64> host_copy((size_t)num_points);
-        for (int o = 0; o < 3; ++o) {
-            if (!outs[o]) continue;
-            SBH_HIP(hipMemcpy(host_copy.data(), outs[o], host_copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < num_points; ++i)
-                if (!std::isfinite(host_copy[i].re) || !std::isfinite(host_copy[i].im))
-                    throw Error(SBH_ERR_NUMERICAL, "field value " + std::to_string(i) + " is not finite");
-            if (memory == SBH_MEM_HOST) std::memcpy(user[o], host_copy.data(), host_copy.size() * sizeof(sbh_c64));
+            launch_source_rhs(ctx, mesh, *ph, beta, sign, d_src.ptr, num_sources, reinterpret_cast<double2*>(rhs),
+                              accumulate != 0);
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+            DeviceArray<double2> tmp((size_t)mesh->n);
+            if (accumulate) SBH_HIP(hipMemcpyAsync(tmp.ptr, rhs, tmp.bytes(), hipMemcpyHostToDevice, ctx->stream));
+            launch_source_rhs(ctx, mesh, *ph, beta, sign, d_src.ptr, num_sources, tmp.ptr, accumulate != 0);
+            SBH_HIP(hipMemcpyAsync(rhs, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
         }
     });
 }

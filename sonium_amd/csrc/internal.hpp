@@ -135,6 +135,15 @@ struct Source {
     double n[3];
 };
 
+// Incident source on the device (same layout as sbh_incident_source).
+struct IncidentSource {
+    int32_t kind;  // 0 plane wave, 1 point source
+    int32_t pad;
+    double v[3];
+    double amp[2];
+};
+static_assert(sizeof(IncidentSource) == sizeof(sbh_incident_source), "incident source layout");
+
 // Boundary condition of a field panel (GENERAL assembly mode only).
 struct PanelBc {
     int32_t type;       // 0 velocity, 1 pressure
@@ -323,8 +332,11 @@ void launch_derivative_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c6
 // Pressure at points (tbem_kernels.hip): plane waves plus the scattered field
 // of the panels' Cauchy data; v == nullptr derives it from the BCs.
 void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, const double2* sol, const double2* v,
-                  const double* d_points, int64_t npts, const double* d_dirs, const double2* d_amps, int64_t nwaves,
+                  const double* d_points, int64_t npts, const IncidentSource* d_sources, int64_t nsources,
                   double2* scattered, double2* incident, double2* total);
+// rhs (+)= sign * -(gamma p_inc + beta tau dp_inc/dn) of the summed sources.
+void launch_source_rhs(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, sbh_c64 beta, double sign,
+                       const IncidentSource* d_sources, int64_t nsources, double2* rhs, bool accumulate);
 // y = A x for a dense or exact operator (api.cpp).
 void operator_apply_device(const sbh_operator* op, const double2* x, double2* y);
 void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,

@@ -1280,26 +1280,77 @@ __global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__
     }
 }
 
+// Incident pressure and normal derivative of one source at x with normal nx
+// (incident.rs:216-420); n == nullptr skips the derivative.  Reference
+// operation order, as plane_wave_value.
+#pragma clang fp contract(off)
+__device__ inline void incident_value(const IncidentSource& s, const double* x, const double* nx, double k, double hf,
+                                      double2* p, double2* dpdn) {
+    const double2 amp = make_double2(s.amp[0], s.amp[1]);
+    if (s.kind == 0) {
+        double sn, cs;
+        sincos_red(hf * k * (s.v[0] * x[0] + s.v[1] * x[1] + s.v[2] * x[2]), &sn, &cs);
+        *p = cmul(amp, make_double2(cs, sn));
+        if (nx) *dpdn = cmul(make_double2(0.0, hf * k * (s.v[0] * nx[0] + s.v[1] * nx[1] + s.v[2] * nx[2])), *p);
+        return;
+    }
+    const double dx = x[0] - s.v[0], dy = x[1] - s.v[1], dz = x[2] - s.v[2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    *p = make_double2(0.0, 0.0);
+    if (nx) *dpdn = make_double2(0.0, 0.0);
+    if (!(r > 1e-10)) return;
+    double sn, cs;
+    sincos_red(hf * k * r, &sn, &cs);
+    const double w = 1.0 / (12.566370614359172 * r);
+    const double2 g = make_double2(cs * w, sn * w);
+    *p = cmul(amp, g);
+    if (nx) {
+        const double2 dgdr = cmul(make_double2(-1.0 / r, hf * k), g);
+        const double drdn = (dx * nx[0] + dy * nx[1] + dz * nx[2]) / r;
+        *dpdn = cmul(amp, cscale(dgdr, drdn));
+    }
+}
+
+// rhs_i (+)= sign * -(gamma p_inc + beta tau dp_inc/dn) over the sources in order.
+__global__ void source_rhs_kernel(const Source* __restrict__ centers, int64_t n, const IncidentSource* __restrict__ srcs,
+                                  int64_t nsrc, double k, double hf, double tau_r, double2 beta, double sign,
+                                  int accumulate, double2* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Source c = centers[i];
+    double2 p = make_double2(0.0, 0.0), q = make_double2(0.0, 0.0);
+    for (int64_t w = 0; w < nsrc; ++w) {
+        double2 pw, qw;
+        incident_value(srcs[w], c.c, c.n, k, hf, &pw, &qw);
+        p = cadd(p, pw);
+        q = cadd(q, qw);
+    }
+    // incident.rs:825-831 and bem_solver.rs:5005-5034, as plane_wave_value
+    double2 inc = cadd(cmul(make_double2(1.0, 0.0), p), cmul(cmul(beta, make_double2(tau_r, 0.0)), q));
+    inc = make_double2(-inc.x, -inc.y);
+    inc = cmul(inc, make_double2(sign, 0.0));
+    rhs[i] = accumulate ? cadd(rhs[i], inc) : inc;
+}
+#pragma clang fp contract(fast)
+
 // scattered[i] = sum over slices of the partials; incident[i] = plane waves at
 // x_i (incident.rs:216-260); total[i] = incident + scattered (FieldPoint::new,
 // pressure.rs:32-41).  incident and total may be null.
 __global__ void field_finish_kernel(const double2* __restrict__ part, int slices, const double* __restrict__ points,
-                                    int64_t m, const double* __restrict__ dirs, const double2* __restrict__ amps,
-                                    int64_t nwaves, double k, double hf, double2* __restrict__ scattered,
-                                    double2* __restrict__ incident, double2* __restrict__ total) {
+                                    int64_t m, const IncidentSource* __restrict__ srcs, int64_t nsrc, double k,
+                                    double hf, double2* __restrict__ scattered, double2* __restrict__ incident,
+                                    double2* __restrict__ total) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     double2 s = make_double2(0.0, 0.0);
     for (int z = 0; z < slices; ++z) s = cadd(s, part[(int64_t)z * m + i]);
     scattered[i] = s;
     if (!incident && !total) return;
-    const double* x = points + 3 * i;
     double2 inc = make_double2(0.0, 0.0);
-    for (int64_t w = 0; w < nwaves; ++w) {
-        const double* d = dirs + 3 * w;
-        double sn, cs;
-        sincos_red(hf * k * (d[0] * x[0] + d[1] * x[1] + d[2] * x[2]), &sn, &cs);
-        inc = cadd(inc, cmul(amps[w], make_double2(cs, sn)));
+    for (int64_t w = 0; w < nsrc; ++w) {
+        double2 p, unused;
+        incident_value(srcs[w], points + 3 * i, nullptr, k, hf, &p, &unused);
+        inc = cadd(inc, p);
     }
     if (incident) incident[i] = inc;
     if (total) total[i] = cadd(inc, s);
@@ -1311,9 +1362,9 @@ __global__ void field_finish_kernel(const double2* __restrict__ part, int slices
 // (value - p_inc(c), (sol - dp_inc/dn(c)) / (i pf)) with the plane waves at the
 // collocation point.
 __global__ void field_cauchy_kernel(const PanelBc* __restrict__ bcs, const Source* __restrict__ src,
-                                    const double2* __restrict__ sol, int64_t n, const double* __restrict__ dirs,
-                                    const double2* __restrict__ amps, int64_t nwaves, double k, double hf, double pf,
-                                    double2* __restrict__ pe, double2* __restrict__ ve) {
+                                    const double2* __restrict__ sol, int64_t n, const IncidentSource* __restrict__ srcs,
+                                    int64_t nsrc, double k, double hf, double pf, double2* __restrict__ pe,
+                                    double2* __restrict__ ve) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const PanelBc bc = bcs[j];
@@ -1322,13 +1373,11 @@ __global__ void field_cauchy_kernel(const PanelBc* __restrict__ bcs, const Sourc
     if (bc.type == 1) {
         const Source c = src[j];
         double2 pin = make_double2(0.0, 0.0), qin = make_double2(0.0, 0.0);
-        for (int64_t w = 0; w < nwaves; ++w) {
-            const double* d = dirs + 3 * w;
-            double sn, cs;
-            sincos_red(hf * k * (d[0] * c.c[0] + d[1] * c.c[1] + d[2] * c.c[2]), &sn, &cs);
-            const double2 p = cmul(amps[w], make_double2(cs, sn));
+        for (int64_t w = 0; w < nsrc; ++w) {
+            double2 p, q;
+            incident_value(srcs[w], c.c, c.n, k, hf, &p, &q);
             pin = cadd(pin, p);
-            qin = cadd(qin, cmul(make_double2(0.0, hf * k * (d[0] * c.n[0] + d[1] * c.n[1] + d[2] * c.n[2])), p));
+            qin = cadd(qin, q);
         }
         pe[j] = csub(val, pin);
         const double2 q = csub(s, qin);
@@ -1408,6 +1457,19 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, s
         SBH_HIP(hipGetLastError());
         ctx->end_timed(SBH_KERNEL_RHS, ev);
     }
+}
+
+void launch_source_rhs(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, sbh_c64 beta, double sign,
+                       const IncidentSource* d_sources, int64_t nsources, double2* rhs, bool accumulate) {
+    const int64_t n = m->n;
+    if (n == 0) return;
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_RHS, &ev);
+    hipLaunchKernelGGL(source_rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, m->d_src.ptr,
+                       n, d_sources, nsources, ph.wave_number, ph.harmonic_factor, ph.tau,
+                       make_double2(beta.re, beta.im), sign, accumulate ? 1 : 0, rhs);
+    SBH_HIP(hipGetLastError());
+    ctx->end_timed(SBH_KERNEL_RHS, ev);
 }
 
 void launch_derivative_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta, sbh_c64 beta_d,
@@ -1544,7 +1606,7 @@ void launch_exact_apply(const sbh_operator* op, const double2* x, double2* y) {
 }
 
 void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, const double2* sol, const double2* v,
-                  const double* d_points, int64_t npts, const double* d_dirs, const double2* d_amps, int64_t nwaves,
+                  const double* d_points, int64_t npts, const IncidentSource* d_sources, int64_t nsources,
                   double2* scattered, double2* incident, double2* total) {
     const int64_t n = m->n;
     if (npts == 0) return;
@@ -1555,7 +1617,7 @@ void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, co
         pe.allocate((size_t)n);
         ve.allocate((size_t)n);
         hipLaunchKernelGGL(field_cauchy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                           m->d_bc.ptr, m->d_src.ptr, sol, n, d_dirs, d_amps, nwaves, k, hf, pf, pe.ptr, ve.ptr);
+                           m->d_bc.ptr, m->d_src.ptr, sol, n, d_sources, nsources, k, hf, pf, pe.ptr, ve.ptr);
         SBH_HIP(hipGetLastError());
         p = pe.ptr;
         v = ve.ptr;
@@ -1579,7 +1641,7 @@ void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, co
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_FIELD, ev);
     hipLaunchKernelGGL(field_finish_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream, part.ptr,
-                       (int)slices, d_points, npts, d_dirs, d_amps, nwaves, k, hf, scattered, incident, total);
+                       (int)slices, d_points, npts, d_sources, nsources, k, hf, scattered, incident, total);
     SBH_HIP(hipGetLastError());
     SBH_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
 }

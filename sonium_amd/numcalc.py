@@ -358,16 +358,17 @@ class SweepOutput:
 
 
 def pick_incident(project: NumCalcProject) -> IncidentField:
-    """pick_incident (runner.rs:160-180): the first plane wave, else a zero-amplitude one."""
+    """pick_incident (runner.rs:160-180): the first plane wave (direction as
+    written), else the first point source, else a zero-amplitude plane wave."""
     if project.plane_waves:
         pw = project.plane_waves[0]
-        d = np.asarray(pw.direction, dtype=np.float64)
-        if abs(np.linalg.norm(d) - 1.0) > 1e-12:
-            raise SbhError(2, "plane-wave directions must be unit vectors (the reference uses them unnormalised)")
-        return IncidentField([d], [complex(pw.amplitude_re, pw.amplitude_im)])
+        return IncidentField.from_sources([IncidentField.PLANE_WAVE], [pw.direction],
+                                          [complex(pw.amplitude_re, pw.amplitude_im)])
     if project.point_sources:
-        raise SbhError(2, "point-source incident fields are not supported on this path")
-    return IncidentField([[0.0, 0.0, 1.0]], [0.0])
+        ps = project.point_sources[0]
+        return IncidentField.from_sources([IncidentField.POINT_SOURCE], [ps.position],
+                                          [complex(ps.amplitude_re, ps.amplitude_im)])
+    return IncidentField.from_sources([IncidentField.PLANE_WAVE], [[0.0, 0.0, 1.0]], [0.0])
 
 
 def run_project(project: NumCalcProject, ctx: Context, freq_start_idx: Optional[int] = None,

@@ -70,3 +70,29 @@ def test_run_project_frequency_ranges(ctx):
     for lo, hi in ((99, 200), (2, 0)):
         empty = nc.run_project(project, ctx, lo, hi, max_iterations=50, tolerance=1e-4)
         assert len(empty.frequencies) == 0 and empty.surface_pressure.size == 0
+
+
+def test_point_source_project_matches_oracle(ctx, tmp_path):
+    """A project whose incident field is a POINT SOURCES record (pick_incident,
+    runner.rs:160-180) on the mesh2hrtf_basic head, rigid."""
+    import shutil
+    src_dir = os.path.dirname(BASIC)
+    for name in os.listdir(src_dir):
+        shutil.copy(os.path.join(src_dir, name), tmp_path / name)
+    text = open(BASIC).read().replace("ELEM 0 TO 0 VELO 1.0 -1 0.0 -1\n", "")
+    text = text.replace("RETU\n##\nEND", "RETU\n##\nPOINT SOURCES\n1 0.0 0.3 0.2 2.0 -1 0.5 -1\n##\nEND")
+    (tmp_path / "NC.inp").write_text(text)
+    project = nc.load_project(str(tmp_path / "NC.inp"))
+    assert project.config.point_sources[0].position == (0.0, 0.3, 0.2)
+    out = nc.run_project(project, ctx, tolerance=1e-11)
+    mesh, bt, bv, adm = project.surface_arrays()
+    c, nn, _ = oracle.element_geometry(mesh.nodes, mesh.triangles)
+    pts = project.evaluation_centers()
+    src = dict(positions=[[0.0, 0.3, 0.2]], strengths=[2.0 + 0.5j])
+    for i, f in enumerate(out.frequencies):
+        k = project.physics_at(f).wave_number
+        A, rhs = oracle.build_tbem_system(mesh.nodes, mesh.triangles, k, 4j / k)
+        x = np.linalg.solve(A, rhs + oracle.source_rhs(c, nn, k, 4j / k, **src))
+        assert rel_l2(out.surface_pressure[i], x) < 1e-8
+        ref = oracle.field(pts, mesh.nodes, mesh.triangles, x, k, **src)
+        assert rel_l2(out.eval_pressure[i], ref) < 1e-8

@@ -130,3 +130,21 @@ def test_robin_sphere_series():
     on = mie.total_field(k, a, a, theta, sigma=sigma)
     dpdr = (plus - minus) / (2 * h)
     assert np.max(np.abs(dpdr - sigma * on)) < 1e-8 * np.max(np.abs(on))
+
+
+def test_point_source_series_reduces_to_free_field_and_is_rigid():
+    """Without the sphere term the series is the free-field monopole; with it,
+    dp/dr = 0 on r = a (central difference)."""
+    k, a = 1.1, 1.0
+    s = np.array([0.0, 0.0, 3.0])
+    x = np.array([[0.5, 0.2, 1.4], [-1.6, 0.3, 0.2]])
+    R = np.linalg.norm(x - s, axis=1)
+    free = np.exp(1j * k * R) / (4 * np.pi * R)
+    big = mie.point_source_rigid_sphere(k, 1e-6, s, x)  # a vanishing sphere scatters nothing
+    assert np.max(np.abs(big - free)) < 1e-10 * np.max(np.abs(free))
+    u = np.array([[0.6, 0.0, 0.8], [0.0, -1.0, 0.0]])
+    h = 1e-5
+    pp = mie.point_source_rigid_sphere(k, a, s, u * (a + h))
+    pm = mie.point_source_rigid_sphere(k, a, s, u * (a - h))
+    on = mie.point_source_rigid_sphere(k, a, s, u * a)
+    assert np.max(np.abs(pp - pm) / (2 * h)) < 1e-7 * np.max(np.abs(on))

@@ -156,3 +156,40 @@ def test_device_sincos_coefficients_match_generator():
             assert m and float(m.group(1)) == v, (name, i)
     es, ec = gen.max_errors(P, Q, samples=401)
     assert es < 4e-17 and ec < 2e-17
+
+
+def test_point_source_normal_derivative_is_the_directional_derivative():
+    """dp/dn of the point source equals the central difference of p along n."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((20, 3)) * 2.0
+    nrm = rng.standard_normal((20, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    src = dict(positions=[[0.3, -0.2, 0.1]], strengths=[1.5 - 0.5j])
+    k, h = 2.7, 1e-5
+    _, q = oracle.incident_values(x, nrm, k, **src)
+    pp, _ = oracle.incident_values(x + h * nrm, None, k, **src)
+    pm, _ = oracle.incident_values(x - h * nrm, None, k, **src)
+    assert np.max(np.abs(q - (pp - pm) / (2 * h))) < 1e-7 * np.max(np.abs(q))
+
+
+def test_point_source_outside_rigid_sphere_converges_to_series():
+    """A point source at distance 3 from a rigid unit sphere: the oracle's
+    point-source right-hand side, solve and field converge to the
+    addition-theorem series (oracle.mie.point_source_rigid_sphere) at r = 2."""
+    k = 1.3
+    s_pos = np.array([0.6, -1.2, 2.6])
+    s_pos *= 3.0 / np.linalg.norm(s_pos)
+    src = dict(positions=[s_pos], strengths=[1.0 - 0.5j])
+    ref = mie.point_source_rigid_sphere(k, 1.0, s_pos, _PTS, strength=1.0 - 0.5j)
+    inc_only, _ = oracle.incident_values(_PTS, None, k, **src)
+    assert np.linalg.norm(ref - inc_only) > 0.05 * np.linalg.norm(ref)  # the sphere matters
+    errs = []
+    for level in (2, 3):
+        m = icosphere(1.0, level)
+        c, nn, _ = oracle.element_geometry(m.nodes, m.triangles)
+        beta = 0.2j / k
+        A, rhs = oracle.build_tbem_system(m.nodes, m.triangles, k, beta)
+        x = np.linalg.solve(A, rhs + oracle.source_rhs(c, nn, k, beta, **src))
+        tot = oracle.field(_PTS, m.nodes, m.triangles, x, k, **src)
+        errs.append(np.linalg.norm(tot - ref) / np.linalg.norm(ref))
+    assert errs[1] < 2e-3 and errs[0] / errs[1] > 3.0, errs
