@@ -314,26 +314,48 @@ __global__ __launch_bounds__(BLOCK) void zgemm_f64_kernel(const double2* __restr
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < NT; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t k0 = kbeg; k0 < kend; k0 += ZG_BKC) {
-        // A stage: 128 rows x 8 complex, 4 per thread; 8 consecutive threads read one row's 128 B
+    // Stage s+1 is loaded into registers while stage s is multiplied.
+    constexpr int AQ = (ZG_BM * ZG_BKC) / BLOCK;                 // A elements per thread per stage
+    constexpr int XQ = (ZG_BKC * 8 * NT + BLOCK - 1) / BLOCK;    // X elements per thread per stage
+    double2 pa[AQ], px[XQ];
+    auto gload = [&](int64_t k0) {
+        // A stage: 128 rows x 8 complex; 8 consecutive threads read one row's 128 B
 #pragma unroll
-        for (int q = 0; q < (ZG_BM * ZG_BKC) / BLOCK; ++q) {
+        for (int q = 0; q < AQ; ++q) {
             const int e = t + q * BLOCK;
             const int r = e / ZG_BKC, cc = e % ZG_BKC;
             const int64_t gr = row0 + r, gk = k0 + cc;
-            double2 v = (gr < n && gk < kend) ? A[gr * lda + gk] : make_double2(0.0, 0.0);
-            As[2 * cc][r] = v.x;
-            As[2 * cc + 1][r] = v.y;
+            pa[q] = (gr < n && gk < kend) ? A[gr * lda + gk] : make_double2(0.0, 0.0);
         }
-        // Xt stage: 8 complex rows of X -> 16 real rows of Xt
-        for (int e = t; e < ZG_BKC * 8 * NT; e += BLOCK) {
+#pragma unroll
+        for (int q = 0; q < XQ; ++q) {
+            const int e = t + q * BLOCK;
             const int j = e / (8 * NT), c = e % (8 * NT);
             const int64_t gk = k0 + j;
-            double2 v = (gk < kend && c < kc) ? X[gk * ldx + c] : make_double2(0.0, 0.0);
-            *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = v;
-            *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-v.y, v.x);
+            px[q] = (e < ZG_BKC * 8 * NT && gk < kend && c < kc) ? X[gk * ldx + c] : make_double2(0.0, 0.0);
+        }
+    };
+    gload(kbeg);
+    for (int64_t k0 = kbeg; k0 < kend; k0 += ZG_BKC) {
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = t + q * BLOCK;
+            const int r = e / ZG_BKC, cc = e % ZG_BKC;
+            As[2 * cc][r] = pa[q].x;
+            As[2 * cc + 1][r] = pa[q].y;
+        }
+        // Xt stage: 8 complex rows of X -> 16 real rows of Xt
+#pragma unroll
+        for (int q = 0; q < XQ; ++q) {
+            const int e = t + q * BLOCK;
+            if (e < ZG_BKC * 8 * NT) {
+                const int j = e / (8 * NT), c = e % (8 * NT);
+                *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = px[q];
+                *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-px[q].y, px[q].x);
+            }
         }
         __syncthreads();
+        if (k0 + ZG_BKC < kend) gload(k0 + ZG_BKC);
 #pragma unroll
         for (int ks = 0; ks < (2 * ZG_BKC) / 4; ++ks) {
             const int kr = 4 * ks + (lane >> 4);
