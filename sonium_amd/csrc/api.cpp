@@ -10,7 +10,7 @@
 
 namespace sbh {
 sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, int k, const sbh_gmres_config& cfg,
-                           sbh_gmres_solution* sols);
+                           bool x_zero, sbh_gmres_solution* sols);
 sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_B, double2* d_X, int k,
                         const sbh_gmres_ir_config& cfg, sbh_gmres_ir_status* st);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
@@ -1039,7 +1039,7 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
         } else {
             SBH_HIP(hipMemsetAsync(dx.ptr, 0, sizeof(double2) * n * k, op->ctx->stream));
         }
-        sbh_status st = run_gmres_block(op, db.ptr, dx.ptr, (int)k, *config, solutions);
+        sbh_status st = run_gmres_block(op, db.ptr, dx.ptr, (int)k, *config, X0 == nullptr, solutions);
         dx.finish();
         if (st != SBH_OK) throw Error(st, "block GMRES failed");
     });

@@ -116,6 +116,10 @@ struct BatchedParams {
     double tol = 1e-8;
     bool exact_checks = true;  // confirm convergence with the true residual (gmres_core)
     bool finalize = true;      // finalize_gmres_solution
+    // X is zero on entry: the first residual is B itself and A*0 is not formed.
+    // The result is the same for any finite A (a non-finite A then shows at the
+    // first Krylov product instead of the first residual).
+    bool x_zero = false;
 };
 
 struct ColumnResult {
@@ -253,6 +257,23 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
     std::vector<std::vector<cplx>> coef, re;
     std::vector<double> wnb, wn, wn2;
     const double2* basis_for_update = nullptr;
+    // R_ = B - A X is kept with the version of X it belongs to, so a cycle
+    // start or the final check right after a true-residual check reuses it
+    // instead of forming A X again (every FP64 product is a full pass over A).
+    int64_t x_version = 0, r_version = -1;
+    std::vector<double> r_abs(k, 0.0), ax_abs(k, 0.0);
+    auto refresh_residual = [&]() {
+        if (r_version == x_version) return;
+        if (x_version == 0 && p.x_zero) {
+            zero_fill(ctx_, R_.ptr, n_ * k_);
+            std::fill(ax_abs.begin(), ax_abs.end(), 0.0);
+        } else {
+            A.apply(X, R_.ptr, k);
+            if (p.finalize) ax_abs = norms(R_.ptr);
+        }
+        r_abs = residual_norms(B, R_.ptr);
+        r_version = x_version;
+    };
 
     auto solve_y = [&](int c, int kk) {
         std::vector<cplx> y(kk, cplx(0.0, 0.0));
@@ -270,10 +291,11 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
             if (which[c]) ys[c] = solve_y(c, kk);
         upload_coeffs(ys, kk, which);
         bupdate(ctx_, flexible_ ? Z_.ptr : V_.ptr, n_ * k_, kk, C_.ptr, X, n_, k_);
+        ++x_version;
     };
     auto true_residuals = [&]() {
-        A.apply(X, R_.ptr, k);
-        std::vector<double> r = residual_norms(B, R_.ptr);
+        refresh_residual();
+        std::vector<double> r = r_abs;
         for (int c = 0; c < k; ++c) r[c] /= std::max(true_b[c], 2.2250738585072014e-308);
         return r;
     };
@@ -285,8 +307,8 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
             if (!done[c] && cycles[c] < p.max_iterations) any = true;
         if (!any) break;
         // start a cycle: r = b - A x
-        A.apply(X, R_.ptr, k);
-        std::vector<double> rn = residual_norms(B, R_.ptr);
+        refresh_residual();
+        const std::vector<double> rn = r_abs;
         bool anycyc = false;
         for (int c = 0; c < k; ++c) {
             cycling[c] = 0;
@@ -461,9 +483,9 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
     }
     if (!p.finalize) return;
     // finalize_gmres_solution (gmres.rs:159-250), per column
-    A.apply(X, R_.ptr, k);
-    std::vector<double> axn = norms(R_.ptr);
-    std::vector<double> rn = residual_norms(B, R_.ptr);
+    refresh_residual();
+    const std::vector<double> axn = ax_abs;
+    const std::vector<double> rn = r_abs;
     std::vector<double> xn2 = norms(X);
     for (int c = 0; c < k; ++c) {
         ColumnResult& r = out[c];
@@ -500,6 +522,7 @@ struct InnerGmresPreconditioner : BlockPreconditioner {
         params.tol = tol;
         params.exact_checks = false;
         params.finalize = false;
+        params.x_zero = true;
     }
     void apply(const double2* V, double2* Z, int k) override {
         zero_fill(ctx, Z, n * k);
@@ -510,7 +533,7 @@ struct InnerGmresPreconditioner : BlockPreconditioner {
 };
 
 sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, int k, const sbh_gmres_config& cfg,
-                           sbh_gmres_solution* sols) {
+                           bool x_zero, sbh_gmres_solution* sols) {
     DenseBlockOperator A(op->sys);
     const int m = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.restart, op->sys->n));
     BatchedGmres engine(op->ctx, op->sys->n, k, m, false);
@@ -518,6 +541,7 @@ sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, i
     p.max_iterations = cfg.max_iterations;
     p.restart = cfg.restart;
     p.tol = cfg.tolerance;
+    p.x_zero = x_zero;
     std::vector<ColumnResult> res;
     engine.run(A, nullptr, d_B, d_X, p, res);
     for (int c = 0; c < k; ++c) {
@@ -551,6 +575,7 @@ sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_
     p.restart = restart;
     p.max_iterations = (max_outer + restart - 1) / restart;
     p.tol = cfg.outer_tolerance;
+    p.x_zero = true;
     std::vector<ColumnResult> res;
     zero_fill(op->ctx, d_X, n * k);
     engine.run(A, &M, d_B, d_X, p, res);

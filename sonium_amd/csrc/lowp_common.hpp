@@ -17,10 +17,15 @@ typedef unsigned short u16;
 
 template <int FMT>
 struct Half;
+// from_float rounds an FP32 value (nearest even).  The empty asm pins f as an
+// FP32 register value: without it the compiler folds (half)(float)double into
+// one software double->half rounding, which is both slower than the hardware
+// conversions and a different rounding (no FP32 intermediate).
 template <>
 struct Half<0> {
     static constexpr int TOP = 14;
     __device__ static u16 from_float(float f) {
+        asm("" : "+v"(f));
         _Float16 h = (_Float16)f;
         return __builtin_bit_cast(u16, h);
     }
@@ -30,6 +35,7 @@ template <>
 struct Half<1> {
     static constexpr int TOP = 0;
     __device__ static u16 from_float(float f) {
+        asm("" : "+v"(f));
         __bf16 h = (__bf16)f;
         return __builtin_bit_cast(u16, h);
     }

@@ -38,13 +38,63 @@ __device__ __forceinline__ float wave_sumf(float v) {
     return v;
 }
 
-// One block per row.
+// Row conversion, the same arithmetic as convert_row.  A row is read twice
+// (its maximum, then the values), so the grid is persistent: CONV_GRID
+// workgroups walk the rows and only CONV_GRID rows are in flight at once
+// (256 rows of a 40960-panel system are 168 MB), which lets the second read
+// come from the last-level cache rather than HBM.  Each thread keeps four
+// 16-byte loads in flight and stores four complex entries in one 16-byte store.
+constexpr int CONV_THREADS = 1024;
+constexpr unsigned CONV_GRID = 256;
+
 template <int FMT>
-__global__ __launch_bounds__(BLOCK) void lp_convert_kernel(const double2* __restrict__ A, int64_t lda,
-                                                           int64_t n, int64_t ld, u16* __restrict__ out,
-                                                           double* __restrict__ row_inv) {
-    const int64_t i = blockIdx.x;
-    convert_row<FMT>(A + i * lda, n, ld, out + i * 2 * ld, row_inv + i);
+__global__ __launch_bounds__(CONV_THREADS) void lp_convert_kernel(const double2* __restrict__ A, int64_t lda,
+                                                                  int64_t n, int64_t ld, u16* __restrict__ out,
+                                                                  double* __restrict__ row_inv) {
+    __shared__ double sh[CONV_THREADS / 64];
+    const int t = threadIdx.x;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double2* __restrict__ row = A + i * lda;
+        double m = 0.0;
+        int64_t j = t;
+        for (; j + 3 * CONV_THREADS < n; j += 4 * CONV_THREADS) {
+            const double2 v0 = row[j], v1 = row[j + CONV_THREADS];
+            const double2 v2 = row[j + 2 * CONV_THREADS], v3 = row[j + 3 * CONV_THREADS];
+            m = fmax(m, fmax(fmax(fabs(v0.x), fabs(v0.y)), fmax(fabs(v1.x), fabs(v1.y))));
+            m = fmax(m, fmax(fmax(fabs(v2.x), fabs(v2.y)), fmax(fabs(v3.x), fabs(v3.y))));
+        }
+        for (; j < n; j += CONV_THREADS) {
+            const double2 v = row[j];
+            m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+        }
+        m = wave_max(m);
+        if ((t & 63) == 0) sh[t >> 6] = m;
+        __syncthreads();
+        m = sh[0];
+#pragma unroll
+        for (int w = 1; w < CONV_THREADS / 64; ++w) m = fmax(m, sh[w]);
+        int e = 0;
+        if (m > 0.0 && isfinite(m)) e = ilogb(m);
+        const double scale = ldexp(1.0, Half<FMT>::TOP - e);
+        if (t == 0) row_inv[i] = ldexp(1.0, e - Half<FMT>::TOP);
+        u32x4* __restrict__ o = reinterpret_cast<u32x4*>(out + i * 2 * ld);
+        for (int64_t q = t; q < ld / 4; q += CONV_THREADS) {
+            u32x4 w;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int64_t jj = 4 * q + s;
+                unsigned re = 0, im = 0;
+                if (jj < n) {
+                    const double2 v = row[jj];
+                    re = Half<FMT>::from_float((float)(v.x * scale));
+                    im = Half<FMT>::from_float((float)(v.y * scale));
+                }
+                w[s] = re | (im << 16);
+            }
+            o[q] = w;
+        }
+        __syncthreads();  // sh is rewritten for the next row
+    }
 }
 
 // y = diag(row_inv) * (A_lp x), x given in FP32 (ld entries, zero padded).
@@ -586,10 +636,11 @@ int64_t lowp_split_count() { return LP_SPLIT; }
 
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
                   uint16_t* out, double* row_inv) {
+    const dim3 grid((unsigned)std::min<int64_t>(n, CONV_GRID));
     if (format == 0)
-        hipLaunchKernelGGL(lp_convert_kernel<0>, dim3((unsigned)n), dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+        hipLaunchKernelGGL(lp_convert_kernel<0>, grid, dim3(CONV_THREADS), 0, ctx->stream, A, lda, n, ld, out, row_inv);
     else
-        hipLaunchKernelGGL(lp_convert_kernel<1>, dim3((unsigned)n), dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+        hipLaunchKernelGGL(lp_convert_kernel<1>, grid, dim3(CONV_THREADS), 0, ctx->stream, A, lda, n, ld, out, row_inv);
     SBH_HIP(hipGetLastError());
 }
 

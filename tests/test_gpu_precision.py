@@ -169,3 +169,52 @@ def test_lowp_update_after_reassembly(ctx, c1_block):
     A2 = sys.matrix
     assert rel_fro(D2, A2) < U["fp16"]
     assert rel_fro(D1, A2) > 1e-2
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_lowp_copy_is_bit_exact(ctx, c1_block, fmt):
+    """The device conversion of the assembled matrix equals the oracle's
+    row-scaled storage bit for bit (halves, padding and row scales)."""
+    from oracle import lowp as olp
+
+    lp = sa.LowpMatrix(c1_block["sys"], fmt)
+    halves, inv, ld = lp.download()
+    ref_h, ref_inv = olp.row_scaled_halves(c1_block["A"], fmt, ld)
+    assert np.array_equal(inv, ref_inv)
+    bad = np.argwhere(halves != ref_h)
+    for r, c in bad[:8]:
+        a = c1_block["A"][r, c // 2]
+        v = (a.real, a.imag)[c % 2] / inv[r]
+        print(f"row {r} col {c}: scaled {v!r} device {halves[r, c]:#06x} oracle {ref_h[r, c]:#06x}")
+    assert bad.shape[0] == 0, f"{bad.shape[0]} of {halves.size} halves differ"
+
+
+def test_device_block_solvers_equal_host_calls(ctx, c1_block):
+    """gmres_block_device / gmres_ir_device on HBM-resident blocks give the host
+    calls' solutions exactly; a zero start forms no FP64 product for the first
+    residual and the final check reuses the last true residual."""
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    B = c1_block["B"]
+    n, k = B.shape
+    Bd, Xd = sa.DeviceVector(ctx, n * k), sa.DeviceVector(ctx, n * k)
+    Bd.upload(B)
+    cfg = sa.GmresConfig(max_iterations=100, restart=50, tolerance=1e-8)
+    X, sols = sa.gmres_block(op, B, cfg)
+    dsols = sa.gmres_block_device(op, Bd, k, Xd, cfg)
+    assert np.array_equal(Xd.download().reshape(n, k), X)
+    assert [s.iterations for s in dsols] == [s.iterations for s in sols]
+    assert [s.residual for s in dsols] == [s.residual for s in sols]
+    lp = sa.LowpMatrix(c1_block["sys"], "fp16")
+    icfg = sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8)
+    Xi, st = sa.gmres_ir(op, lp, B, icfg)
+    dst = sa.gmres_ir_device(op, lp, Bd, k, Xd, icfg)
+    assert np.array_equal(Xd.download().reshape(n, k), Xi)
+    assert [s.final_outer_residual for s in dst] == [s.final_outer_residual for s in st]
+    outer = max(s.outer_iterations for s in st)
+    assert all(s.converged for s in st)
+    # one block product per outer step plus one true-residual check per step at
+    # which some column finished: none for the zero start, none for the final check
+    finishes = len({s.outer_iterations for s in st})
+    assert outer + 1 <= st[0].fp64_products <= outer + finishes
+    with pytest.raises(ValueError):
+        sa.gmres_ir_device(op, lp, Bd, k + 1, Xd, icfg)
