@@ -33,21 +33,28 @@
 namespace sbh {
 
 int64_t lowp_padded_ld(int64_t n);
-void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* x, float2* x32, double2* y);
-void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
-void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
-           double2* scratch, double2* out);
-void bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
-              int k, double* scratch, double* out);
-void bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* scratch, double* out);
 int64_t batched_scratch_elems(int64_t n, int k, int max_vec);
-void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,
-             int k);
-void bscale(sbh_context* ctx, const double2* X, const double* alpha, double2* Y, int64_t n, int k);
-void zero_fill(sbh_context* ctx, double2* X, int64_t count);
+// Block vectors of complex FP64 (double2) or FP32 (float2) entries; the
+// reductions return FP64 values for both.
+#define SBH_DECLARE_VECTOR_OPS(T)                                                                                 \
+    void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv, \
+                   const T* x, float2* x32, T* y);                                                                \
+    void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv, \
+                   const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y);       \
+    void bdots(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T* W, int64_t n, int k,                 \
+               double2* scratch, double2* out);                                                                   \
+    void bproject(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* C, T* W, int64_t n, int k,   \
+                  double* scratch, double* out);                                                                  \
+    void bresidual(sbh_context* ctx, const T* B, T* R, int64_t n, int k, double* scratch, double* out);           \
+    void bupdate(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* Yc, T* X, int64_t n, int k); \
+    void bscale(sbh_context* ctx, const T* X, const double* alpha, T* Y, int64_t n, int k);                       \
+    void zero_fill(sbh_context* ctx, T* X, int64_t count);
+SBH_DECLARE_VECTOR_OPS(double2)
+SBH_DECLARE_VECTOR_OPS(float2)
+#undef SBH_DECLARE_VECTOR_OPS
+void convert_block(sbh_context* ctx, const double2* X, float2* Y, int64_t count);
+void convert_block(sbh_context* ctx, const float2* X, double2* Y, int64_t count);
 
 namespace {
 
@@ -64,14 +71,17 @@ inline double cnorm(cplx a) { return std::hypot(a.real(), a.imag()); }
 
 }  // namespace
 
-// Block operator: Y (n x k) = A X (n x k), device memory, context stream.
-struct BlockOperator {
+// Block operator: Y (n x k) = A X (n x k), device memory, context stream;
+// T is the vector element type (double2 or float2).
+template <typename T>
+struct BlockOperatorT {
     sbh_context* ctx = nullptr;
     int64_t n = 0;
     int64_t applies = 0;
-    virtual void apply(const double2* X, double2* Y, int k) = 0;
-    virtual ~BlockOperator() = default;
+    virtual void apply(const T* X, T* Y, int k) = 0;
+    virtual ~BlockOperatorT() = default;
 };
+using BlockOperator = BlockOperatorT<double2>;
 
 struct DenseBlockOperator : BlockOperator {
     const sbh_system* sys;
@@ -88,27 +98,30 @@ struct DenseBlockOperator : BlockOperator {
     }
 };
 
-struct LowpBlockOperator : BlockOperator {
+template <typename T>
+struct LowpBlockOperator : BlockOperatorT<T> {
     sbh_lowp_matrix* lp;
     explicit LowpBlockOperator(sbh_lowp_matrix* m) : lp(m) {
-        ctx = m->ctx;
-        n = m->n;
+        this->ctx = m->ctx;
+        this->n = m->n;
     }
-    void apply(const double2* X, double2* Y, int k) override {
+    void apply(const T* X, T* Y, int k) override {
         if (k == 1)
-            lowp_gemv(ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, X, lp->d_x32.ptr, Y);
+            lowp_gemv(this->ctx, lp->d_A.ptr, lp->ld, this->n, lp->format, lp->d_row_inv.ptr, X, lp->d_x32.ptr, Y);
         else
-            lowp_gemm(ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, X, k, lp->d_XT.ptr,
+            lowp_gemm(this->ctx, lp->d_A.ptr, lp->ld, this->n, lp->format, lp->d_row_inv.ptr, X, k, lp->d_XT.ptr,
                       lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, Y);
-        applies += 1;
+        this->applies += 1;
     }
 };
 
 // Right preconditioner of the flexible variant: Z (n x k) = M(V).
-struct BlockPreconditioner {
-    virtual void apply(const double2* V, double2* Z, int k) = 0;
-    virtual ~BlockPreconditioner() = default;
+template <typename T>
+struct BlockPreconditionerT {
+    virtual void apply(const T* V, T* Z, int k) = 0;
+    virtual ~BlockPreconditionerT() = default;
 };
+using BlockPreconditioner = BlockPreconditionerT<double2>;
 
 struct BatchedParams {
     int64_t max_iterations = 100;  // restart cycles
@@ -129,7 +142,10 @@ struct ColumnResult {
     int status = SBH_SOLVER_MAX_ITERATIONS;
 };
 
-// Reusable workspace for one (n, k, restart, flexible) shape.
+// Reusable workspace for one (n, k, restart, flexible) shape; T is the
+// element type of the Krylov vectors (the Hessenberg matrix, the rotations
+// and every reduction stay FP64).
+template <typename T>
 class BatchedGmres {
    public:
     BatchedGmres(sbh_context* ctx, int64_t n, int k, int m, bool flexible)
@@ -151,7 +167,7 @@ class BatchedGmres {
         pinned_ = down_.ptr;
     }
 
-    void run(BlockOperator& A, BlockPreconditioner* M, const double2* B, double2* X, const BatchedParams& p,
+    void run(BlockOperatorT<T>& A, BlockPreconditionerT<T>* M, const T* B, T* X, const BatchedParams& p,
              std::vector<ColumnResult>& out);
 
    private:
@@ -159,13 +175,14 @@ class BatchedGmres {
     int64_t n_;
     int k_, m_;
     bool flexible_;
-    DeviceArray<double2> V_, Z_, W_, R_, part_, C_, scratch_;
+    DeviceArray<T> V_, Z_, W_, R_;
+    DeviceArray<double2> part_, C_, scratch_;
     DeviceArray<double> alpha_;
     PinnedArray<double> down_, up_coef_, up_alpha_;
     double* pinned_ = nullptr;  // download region
 
-    double2* vec(int i) { return V_.ptr + (size_t)i * n_ * k_; }
-    double2* zvec(int i) { return Z_.ptr + (size_t)i * n_ * k_; }
+    T* vec(int i) { return V_.ptr + (size_t)i * n_ * k_; }
+    T* zvec(int i) { return Z_.ptr + (size_t)i * n_ * k_; }
 
     void fetch(const void* dev, size_t doubles) {
         SBH_HIP(hipMemcpyAsync(pinned_, dev, sizeof(double) * doubles, hipMemcpyDeviceToHost, ctx_->stream));
@@ -176,15 +193,16 @@ class BatchedGmres {
         fetch(part_.ptr, (size_t)k_);
         out.assign(pinned_, pinned_ + k_);
     }
-    std::vector<double> norms(const double2* Xd) {
-        bresidual(ctx_, nullptr, const_cast<double2*>(Xd), n_, k_, reinterpret_cast<double*>(scratch_.ptr),
+    std::vector<double> norms(const T* Xd) {
+        bresidual(ctx_, static_cast<const T*>(nullptr), const_cast<T*>(Xd), n_, k_,
+                  reinterpret_cast<double*>(scratch_.ptr),
                   reinterpret_cast<double*>(part_.ptr));
         std::vector<double> s;
         column_sums(s);
         for (auto& v : s) v = std::sqrt(v);
         return s;
     }
-    std::vector<double> residual_norms(const double2* B, double2* R) {  // R = B - R
+    std::vector<double> residual_norms(const T* B, T* R) {  // R = B - R
         bresidual(ctx_, B, R, n_, k_, reinterpret_cast<double*>(scratch_.ptr), reinterpret_cast<double*>(part_.ptr));
         std::vector<double> s;
         column_sums(s);
@@ -192,7 +210,7 @@ class BatchedGmres {
         return s;
     }
     // coefficients <v_i, w> for i < nvec (per column) and ||w||
-    void dots(int nvec, const double2* Wd, std::vector<std::vector<cplx>>& coef, std::vector<double>& wn) {
+    void dots(int nvec, const T* Wd, std::vector<std::vector<cplx>>& coef, std::vector<double>& wn) {
         bdots(ctx_, V_.ptr, n_ * k_, nvec, Wd, n_, k_, scratch_.ptr, part_.ptr);
         fetch(part_.ptr, (size_t)(nvec + 1) * k_ * 2);
         coef.assign(k_, std::vector<cplx>(nvec));
@@ -221,8 +239,9 @@ class BatchedGmres {
     }
 };
 
-void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* B, double2* X,
-                       const BatchedParams& p, std::vector<ColumnResult>& out) {
+template <typename T>
+void BatchedGmres<T>::run(BlockOperatorT<T>& A, BlockPreconditionerT<T>* M, const T* B, T* X,
+                          const BatchedParams& p, std::vector<ColumnResult>& out) {
     const int k = k_;
     const int m = m_;
     out.assign(k, ColumnResult{});
@@ -256,7 +275,6 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
     std::vector<double> alpha(k, 0.0);
     std::vector<std::vector<cplx>> coef, re;
     std::vector<double> wnb, wn, wn2;
-    const double2* basis_for_update = nullptr;
     // R_ = B - A X is kept with the version of X it belongs to, so a cycle
     // start or the final check right after a true-residual check reuses it
     // instead of forming A X again (every FP64 product is a full pass over A).
@@ -299,7 +317,6 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
         for (int c = 0; c < k; ++c) r[c] /= std::max(true_b[c], 2.2250738585072014e-308);
         return r;
     };
-    (void)basis_for_update;
 
     for (;;) {
         bool any = false;
@@ -346,7 +363,7 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
             if (!live) break;
             for (int c = 0; c < k; ++c)
                 if (cycling[c]) out[c].iterations += 1;
-            const double2* zj = vec(j);
+            const T* zj = vec(j);
             if (flexible_) {
                 M->apply(vec(j), zvec(j), k);
                 zj = zvec(j);
@@ -507,16 +524,22 @@ void BatchedGmres::run(BlockOperator& A, BlockPreconditioner* M, const double2* 
     }
 }
 
-// Inner low-precision GMRES as the flexible preconditioner: z ≈ A_lp^{-1} v.
+// Inner low-precision GMRES as the flexible preconditioner: z ≈ A_lp^{-1} v,
+// with the inner Krylov vectors and products in FP32 as in the reference's
+// inner solve (gmres_ir.rs:33-48: v is rounded to f32, the f32 result is
+// widened back into the FP64 Z basis).
 struct InnerGmresPreconditioner : BlockPreconditioner {
-    BlockOperator& A;
-    BatchedGmres engine;
+    BlockOperatorT<float2>& A;
+    BatchedGmres<float2> engine;
     BatchedParams params;
     std::vector<int64_t> inner_iterations;
     sbh_context* ctx;
     int64_t n;
-    InnerGmresPreconditioner(BlockOperator& op, int64_t nn, int k, int max_inner, double tol)
+    DeviceArray<float2> V32, Z32;
+    InnerGmresPreconditioner(BlockOperatorT<float2>& op, int64_t nn, int k, int max_inner, double tol)
         : A(op), engine(op.ctx, nn, k, max_inner, false), inner_iterations(k, 0), ctx(op.ctx), n(nn) {
+        V32.allocate((size_t)nn * k);
+        Z32.allocate((size_t)nn * k);
         params.max_iterations = 1;
         params.restart = max_inner;
         params.tol = tol;
@@ -525,9 +548,11 @@ struct InnerGmresPreconditioner : BlockPreconditioner {
         params.x_zero = true;
     }
     void apply(const double2* V, double2* Z, int k) override {
-        zero_fill(ctx, Z, n * k);
+        convert_block(ctx, V, V32.ptr, n * k);
+        zero_fill(ctx, Z32.ptr, n * k);
         std::vector<ColumnResult> r;
-        engine.run(A, nullptr, V, Z, params, r);
+        engine.run(A, nullptr, V32.ptr, Z32.ptr, params, r);
+        convert_block(ctx, Z32.ptr, Z, n * k);
         for (int c = 0; c < k; ++c) inner_iterations[c] += r[c].iterations;
     }
 };
@@ -536,7 +561,7 @@ sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, i
                            bool x_zero, sbh_gmres_solution* sols) {
     DenseBlockOperator A(op->sys);
     const int m = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.restart, op->sys->n));
-    BatchedGmres engine(op->ctx, op->sys->n, k, m, false);
+    BatchedGmres<double2> engine(op->ctx, op->sys->n, k, m, false);
     BatchedParams p;
     p.max_iterations = cfg.max_iterations;
     p.restart = cfg.restart;
@@ -563,14 +588,14 @@ sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, i
 sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_B, double2* d_X, int k,
                         const sbh_gmres_ir_config& cfg, sbh_gmres_ir_status* st) {
     DenseBlockOperator A(op->sys);
-    LowpBlockOperator Alp(lp);
+    LowpBlockOperator<float2> Alp(lp);
     const int64_t n = op->sys->n;
     const int max_outer = (int)std::max<int64_t>(1, cfg.max_outer_iterations);
     const int restart = (int)std::max<int64_t>(1, cfg.restart_size > 0 ? std::min<int64_t>(cfg.restart_size, max_outer)
                                                                        : max_outer);
     const int max_inner = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.max_inner_iterations, n));
     InnerGmresPreconditioner M(Alp, n, k, max_inner, cfg.inner_tolerance);
-    BatchedGmres engine(op->ctx, n, k, std::min<int>(restart, (int)n), true);
+    BatchedGmres<double2> engine(op->ctx, n, k, std::min<int>(restart, (int)n), true);
     BatchedParams p;
     p.restart = restart;
     p.max_iterations = (max_outer + restart - 1) / restart;

@@ -38,6 +38,17 @@ __device__ __forceinline__ float wave_sumf(float v) {
     return v;
 }
 
+// Complex vectors are double2 (FP64) or float2 (FP32); ld2/st2 move them to
+// and from FP64 arithmetic (st2 rounds to nearest).
+__device__ __forceinline__ double2 ld2(double2 v) { return v; }
+__device__ __forceinline__ double2 ld2(float2 v) { return make_double2(v.x, v.y); }
+template <typename T>
+__device__ __forceinline__ T st2(double2 v);
+template <>
+__device__ __forceinline__ double2 st2<double2>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 st2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
 // Row conversion, the same arithmetic as convert_row.  A row is read twice
 // (its maximum, then the values), so the grid is persistent: CONV_GRID
 // workgroups walk the rows and only CONV_GRID rows are in flight at once
@@ -99,11 +110,11 @@ __global__ __launch_bounds__(CONV_THREADS) void lp_convert_kernel(const double2*
 
 // y = diag(row_inv) * (A_lp x), x given in FP32 (ld entries, zero padded).
 // Each wave owns 4 rows; a lane loads 4 complex entries (16 bytes) per row per step.
-template <int FMT>
+template <int FMT, typename T>
 __global__ __launch_bounds__(BLOCK) void lp_gemv_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
                                                         const float2* __restrict__ x,
                                                         const double* __restrict__ row_inv,
-                                                        double2* __restrict__ y) {
+                                                        T* __restrict__ y) {
     constexpr int ROWS = 4;
     const int lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * ROWS;
@@ -149,23 +160,25 @@ __global__ __launch_bounds__(BLOCK) void lp_gemv_kernel(const u16* __restrict__ 
         for (int r = 0; r < ROWS; ++r)
             if (r0 + r < n) {
                 double s = row_inv[r0 + r];
-                y[r0 + r] = make_double2((double)sr[r] * s, (double)si[r] * s);
+                y[r0 + r] = st2<T>(make_double2((double)sr[r] * s, (double)si[r] * s));
             }
     }
 }
 
-// FP64 vector -> FP32 vector (ld entries, zero padded).
-__global__ void to_f32_kernel(const double2* __restrict__ x, int64_t n, int64_t ld, float2* __restrict__ out) {
+// Vector -> FP32 vector (ld entries, zero padded).
+template <typename T>
+__global__ void to_f32_kernel(const T* __restrict__ x, int64_t n, int64_t ld, float2* __restrict__ out) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ld) return;
-    double2 v = j < n ? x[j] : make_double2(0.0, 0.0);
+    double2 v = j < n ? ld2(x[j]) : make_double2(0.0, 0.0);
     out[j] = make_float2((float)v.x, (float)v.y);
 }
 
 // Column max |re|,|im| of an n x k block (atomic max on the bit pattern of a
 // non-negative float, which orders like the value).
 // Each block reduces rows [b*R, (b+1)*R) in LDS first, then one atomic per column.
-__global__ __launch_bounds__(BLOCK) void colmax_kernel(const double2* __restrict__ X, int64_t n, int k,
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void colmax_kernel(const T* __restrict__ X, int64_t n, int k,
                                                        int64_t rows_per_block, unsigned* __restrict__ out) {
     __shared__ float sh[BLOCK];
     const int S = BLOCK / k;
@@ -176,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) void colmax_kernel(const double2* __restrict
     float m = 0.f;
     if (sub < S)
         for (int64_t r = b + sub; r < e; r += S) {
-            double2 v = X[r * k + c];
+            double2 v = ld2(X[r * k + c]);
             m = fmaxf(m, (float)fmax(fabs(v.x), fabs(v.y)));
         }
     sh[threadIdx.x] = m;
@@ -193,9 +206,9 @@ __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
     if (t < n) p[t] = v;
 }
 
-// X (n x k complex FP64) -> XT (BN x 2*ld halves), scaled by 2^(TOP - e_c) per column.
-template <int FMT>
-__global__ void x_to_lp_kernel(const double2* __restrict__ X, int64_t n, int k, int64_t ld,
+// X (n x k complex) -> XT (BN x 2*ld halves), scaled by 2^(TOP - e_c) per column.
+template <int FMT, typename T>
+__global__ void x_to_lp_kernel(const T* __restrict__ X, int64_t n, int k, int64_t ld,
                                const unsigned* __restrict__ colmax, u16* __restrict__ XT,
                                double* __restrict__ col_inv) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,7 +220,7 @@ __global__ void x_to_lp_kernel(const double2* __restrict__ X, int64_t n, int k, 
     if (m > 0.f && isfinite(m)) e = ilogb(m);
     double s = ldexp(1.0, Half<FMT>::TOP - e);
     if (j == 0) col_inv[c] = ldexp(1.0, e - Half<FMT>::TOP);
-    double2 v = X[t];
+    double2 v = ld2(X[t]);
     u16 xr = Half<FMT>::from_float((float)(v.x * s));
     u16 xi = Half<FMT>::from_float((float)(v.y * s));
     u16 nxi = Half<FMT>::from_float((float)(-v.y * s));
@@ -305,9 +318,10 @@ __global__ __launch_bounds__(BLOCK) void lp_gemm_kernel(const u16* __restrict__ 
 }
 
 // Y[row][c] = row_inv[row] * col_inv[c] * sum_slab (Ypart[..][row][c] + i Ypart[..][row][k+c])
+template <typename T>
 __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int k, int bn,
                                  const double* __restrict__ row_inv, const double* __restrict__ col_inv,
-                                 double2* __restrict__ Y) {
+                                 T* __restrict__ Y) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * k) return;
     const int64_t row = t / k;
@@ -319,7 +333,7 @@ __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int
         im += (double)p[k + c];
     }
     const double sc = row_inv[row] * col_inv[c];
-    Y[t] = make_double2(re * sc, im * sc);
+    Y[t] = st2<T>(make_double2(re * sc, im * sc));
 }
 
 // FP64 complex block product Y (n x k) = A (n x n, lda) X (n x k), k <= 64 per launch (column
@@ -456,7 +470,9 @@ __global__ void zgemm_reduce_kernel(const double* __restrict__ Ypart, int64_t n,
 // ---- column-batched Krylov vector kernels (n x k row-major blocks) ----
 // A block reduces over rows [chunk*len, ...) for every column: thread t owns
 // column t % k and rows sub, sub + S, ... (S = 256 / k); the per-column
-// partials are summed in a fixed order.
+// partials are summed in a fixed order.  The element type T is double2 (FP64
+// vectors) or float2 (FP32 vectors of the inner low-precision solve); either
+// way the arithmetic and the partial sums are FP64.
 
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
     return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
@@ -465,49 +481,86 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2
-__global__ __launch_bounds__(BLOCK) void bdots_kernel(const double2* __restrict__ V, int64_t ldv, int nvec,
-                                                      const double2* __restrict__ W, int64_t n, int k,
+// Rows per thread per pass of the reductions below: a thread keeps BT loads in
+// flight (one per row it owns) instead of one dependent load at a time.
+constexpr int BT = 8;
+
+// out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2.
+// One block walks every vector for its rows; its W values stay in registers
+// when the chunk is one pass (S * BT rows), which is the usual case.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, int64_t ldv, int nvec,
+                                                      const T* __restrict__ W, int64_t n, int k,
                                                       int64_t chunk_len, double2* __restrict__ out) {
     __shared__ double2 sh[BLOCK];
-    const int vec = blockIdx.y;
     const int S = BLOCK / k;
     const int c = threadIdx.x % k;
     const int sub = threadIdx.x / k;
     const int64_t b = (int64_t)blockIdx.x * chunk_len;
     const int64_t e = min(n, b + chunk_len);
-    double2 s = make_double2(0.0, 0.0);
-    if (sub < S) {
-        if (vec < nvec) {
-            const double2* v = V + (int64_t)vec * ldv;
-            for (int64_t r = b + sub; r < e; r += S) {
-                double2 p = cmulc(v[r * k + c], W[r * k + c]);
-                s.x += p.x;
-                s.y += p.y;
-            }
-        } else {
-            for (int64_t r = b + sub; r < e; r += S) {
-                double2 q = W[r * k + c];
-                s.x += q.x * q.x + q.y * q.y;
-            }
+    const bool one_pass = e - b <= (int64_t)S * BT;
+    double2 w[BT];
+    if (one_pass && sub < S) {
+#pragma unroll
+        for (int t = 0; t < BT; ++t) {
+            const int64_t r = b + sub + (int64_t)S * t;
+            w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
         }
     }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < k) {
-        double2 tsum = make_double2(0.0, 0.0);
-        for (int q = 0; q < S; ++q) {
-            double2 p = sh[q * k + threadIdx.x];
-            tsum.x += p.x;
-            tsum.y += p.y;
+    for (int vec = 0; vec <= nvec; ++vec) {
+        double2 s = make_double2(0.0, 0.0);
+        if (sub < S) {
+            const T* __restrict__ v = V + (int64_t)vec * ldv;
+            for (int64_t g = b + sub; g < e; g += (int64_t)S * BT) {
+                if (!one_pass) {
+#pragma unroll
+                    for (int t = 0; t < BT; ++t) {
+                        const int64_t r = g + (int64_t)S * t;
+                        w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
+                    }
+                }
+                if (vec < nvec) {
+                    double2 x[BT];
+#pragma unroll
+                    for (int t = 0; t < BT; ++t) {
+                        const int64_t r = g + (int64_t)S * t;
+                        x[t] = r < e ? ld2(v[r * k + c]) : make_double2(0.0, 0.0);
+                    }
+#pragma unroll
+                    for (int t = 0; t < BT; ++t) {
+                        if (g + (int64_t)S * t < e) {
+                            const double2 p = cmulc(x[t], w[t]);
+                            s.x += p.x;
+                            s.y += p.y;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < BT; ++t)
+                        if (g + (int64_t)S * t < e) s.x += w[t].x * w[t].x + w[t].y * w[t].y;
+                }
+            }
         }
-        out[((int64_t)blockIdx.x * gridDim.y + vec) * k + threadIdx.x] = tsum;
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x < k) {
+            double2 tsum = make_double2(0.0, 0.0);
+            for (int q = 0; q < S; ++q) {
+                const double2 p = sh[q * k + threadIdx.x];
+                tsum.x += p.x;
+                tsum.y += p.y;
+            }
+            out[((int64_t)blockIdx.x * (nvec + 1) + vec) * k + threadIdx.x] = tsum;
+        }
+        __syncthreads();  // sh is rewritten for the next vector
     }
 }
 
-// W[:,c] -= sum_i C[i*k+c] V_i[:,c]; norm partials per column into out[chunk*k + c]
-__global__ __launch_bounds__(BLOCK) void bproject_kernel(const double2* __restrict__ V, int64_t ldv, int nvec,
-                                                         const double2* __restrict__ C, double2* __restrict__ W,
+// W[:,c] -= sum_i C[i*k+c] V_i[:,c]; norm partials per column into out[chunk*k + c].
+// Per element the sum over i runs in order before the subtraction.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void bproject_kernel(const T* __restrict__ V, int64_t ldv, int nvec,
+                                                         const double2* __restrict__ C, T* __restrict__ W,
                                                          int64_t n, int k, int64_t chunk_len,
                                                          double* __restrict__ out) {
     __shared__ double sh[BLOCK];
@@ -518,18 +571,39 @@ __global__ __launch_bounds__(BLOCK) void bproject_kernel(const double2* __restri
     const int64_t e = min(n, b + chunk_len);
     double nr = 0.0;
     if (sub < S) {
-        for (int64_t r = b + sub; r < e; r += S) {
-            double sr = 0.0, si = 0.0;
+        for (int64_t g = b + sub; g < e; g += (int64_t)S * BT) {
+            double sr[BT], si[BT];
+#pragma unroll
+            for (int t = 0; t < BT; ++t) sr[t] = si[t] = 0.0;
             for (int i = 0; i < nvec; ++i) {
-                double2 p = cmul(C[i * k + c], V[(int64_t)i * ldv + r * k + c]);
-                sr += p.x;
-                si += p.y;
+                const double2 coef = C[i * k + c];
+                const T* __restrict__ v = V + (int64_t)i * ldv;
+                double2 x[BT];
+#pragma unroll
+                for (int t = 0; t < BT; ++t) {
+                    const int64_t r = g + (int64_t)S * t;
+                    x[t] = r < e ? ld2(v[r * k + c]) : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int t = 0; t < BT; ++t) {
+                    const double2 p = cmul(coef, x[t]);
+                    sr[t] += p.x;
+                    si[t] += p.y;
+                }
             }
-            double2 w = W[r * k + c];
-            w.x -= sr;
-            w.y -= si;
-            W[r * k + c] = w;
-            nr += w.x * w.x + w.y * w.y;
+#pragma unroll
+            for (int t = 0; t < BT; ++t) {
+                const int64_t r = g + (int64_t)S * t;
+                if (r < e) {
+                    double2 w = ld2(W[r * k + c]);
+                    w.x -= sr[t];
+                    w.y -= si[t];
+                    const T ws = st2<T>(w);
+                    W[r * k + c] = ws;
+                    w = ld2(ws);  // the norm of the stored vector
+                    nr += w.x * w.x + w.y * w.y;
+                }
+            }
         }
     }
     sh[threadIdx.x] = nr;
@@ -542,7 +616,8 @@ __global__ __launch_bounds__(BLOCK) void bproject_kernel(const double2* __restri
 }
 
 // R = B - R (or only the norm when B == nullptr); norm partials per column
-__global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restrict__ B, double2* __restrict__ R,
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void bresidual_kernel(const T* __restrict__ B, T* __restrict__ R,
                                                           int64_t n, int k, int64_t chunk_len,
                                                           double* __restrict__ out) {
     __shared__ double sh[BLOCK];
@@ -554,11 +629,12 @@ __global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restr
     double nr = 0.0;
     if (sub < S) {
         for (int64_t r = b + sub; r < e; r += S) {
-            double2 v = R[r * k + c];
+            double2 v = ld2(R[r * k + c]);
             if (B) {
-                double2 bb = B[r * k + c];
-                v = make_double2(bb.x - v.x, bb.y - v.y);
-                R[r * k + c] = v;
+                const double2 bb = ld2(B[r * k + c]);
+                const T vs = st2<T>(make_double2(bb.x - v.x, bb.y - v.y));
+                R[r * k + c] = vs;
+                v = ld2(vs);
             }
             nr += v.x * v.x + v.y * v.y;
         }
@@ -575,13 +651,14 @@ __global__ __launch_bounds__(BLOCK) void bresidual_kernel(const double2* __restr
 // Column sums of chunk partials in two coalesced passes with a fixed order:
 // group g of G adds part[q * nout + o] over q = g, g + G, ...; then
 // out[o] = sum over g of the group sums.
-constexpr int BSUM_GROUPS = 16;
+constexpr int BSUM_GROUPS = 64;
 
 __global__ void bsum_groups_kernel(const double* __restrict__ part, int chunks, int64_t nout,
                                    double* __restrict__ tmp) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout) return;
     double s = 0.0;
+#pragma unroll 8
     for (int q = blockIdx.y; q < chunks; q += BSUM_GROUPS) s += part[(int64_t)q * nout + o];
     tmp[(int64_t)blockIdx.y * nout + o] = s;
 }
@@ -596,34 +673,44 @@ __global__ void bsum_final_kernel(const double* __restrict__ tmp, int64_t nout, 
 }
 
 // X[:,c] += sum_i Yc[i*k+c] V_i[:,c]
-__global__ void bupdate_kernel(const double2* __restrict__ V, int64_t ldv, int nvec, const double2* __restrict__ Yc,
-                               double2* __restrict__ X, int64_t n, int k) {
+template <typename T>
+__global__ void bupdate_kernel(const T* __restrict__ V, int64_t ldv, int nvec, const double2* __restrict__ Yc,
+                               T* __restrict__ X, int64_t n, int k) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * k) return;
     const int c = (int)(t % k);
     double sr = 0.0, si = 0.0;
     for (int i = 0; i < nvec; ++i) {
-        double2 p = cmul(Yc[i * k + c], V[(int64_t)i * ldv + t]);
+        double2 p = cmul(Yc[i * k + c], ld2(V[(int64_t)i * ldv + t]));
         sr += p.x;
         si += p.y;
     }
-    double2 x = X[t];
-    X[t] = make_double2(x.x + sr, x.y + si);
+    const double2 x = ld2(X[t]);
+    X[t] = st2<T>(make_double2(x.x + sr, x.y + si));
 }
 
 // Y[:,c] = alpha[c] * X[:,c]
-__global__ void bscale_kernel(const double2* __restrict__ X, const double* __restrict__ alpha,
-                              double2* __restrict__ Y, int64_t n, int k) {
+template <typename T>
+__global__ void bscale_kernel(const T* __restrict__ X, const double* __restrict__ alpha,
+                              T* __restrict__ Y, int64_t n, int k) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * k) return;
     const double a = alpha[t % k];
-    double2 v = X[t];
-    Y[t] = make_double2(v.x * a, v.y * a);
+    const double2 v = ld2(X[t]);
+    Y[t] = st2<T>(make_double2(v.x * a, v.y * a));
 }
 
-__global__ void zero_kernel(double2* __restrict__ X, int64_t count) {
+template <typename T>
+__global__ void zero_kernel(T* __restrict__ X, int64_t count) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) X[t] = make_double2(0.0, 0.0);
+    if (t < count) X[t] = st2<T>(make_double2(0.0, 0.0));
+}
+
+// Element-type conversion of a block (FP64 <-> FP32, rounding to nearest).
+template <typename S, typename D>
+__global__ void convert_block_kernel(const S* __restrict__ X, D* __restrict__ Y, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) Y[t] = st2<D>(ld2(X[t]));
 }
 
 inline unsigned nblocks(int64_t n, int per = BLOCK) { return (unsigned)((n + per - 1) / per); }
@@ -644,38 +731,40 @@ void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, in
     SBH_HIP(hipGetLastError());
 }
 
-void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* x, float2* x32, double2* y) {
-    hipLaunchKernelGGL(to_f32_kernel, dim3(nblocks(ld)), dim3(BLOCK), 0, ctx->stream, x, n, ld, x32);
+template <typename T>
+static void lowp_gemv_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+                        const T* x, float2* x32, T* y) {
+    hipLaunchKernelGGL(to_f32_kernel<T>, dim3(nblocks(ld)), dim3(BLOCK), 0, ctx->stream, x, n, ld, x32);
     SBH_HIP(hipGetLastError());
     const int64_t waves = (n + 3) / 4;
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
     if (format == 0)
-        hipLaunchKernelGGL(lp_gemv_kernel<0>, dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld, n,
-                           x32, row_inv, y);
+        hipLaunchKernelGGL((lp_gemv_kernel<0, T>), dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld,
+                           n, x32, row_inv, y);
     else
-        hipLaunchKernelGGL(lp_gemv_kernel<1>, dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld, n,
-                           x32, row_inv, y);
+        hipLaunchKernelGGL((lp_gemv_kernel<1, T>), dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld,
+                           n, x32, row_inv, y);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_LOWP, ev);
 }
 
-void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y) {
+template <typename T>
+static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
+                        const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y) {
     // k <= 64: the real block operand has 2k columns, padded to NT*32
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, ctx->stream, colmax, k, 0u);
     {
         const int64_t rpb = std::max<int64_t>(1, 4096 / k);
-        hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)((n + rpb - 1) / rpb)), dim3(BLOCK), 0, ctx->stream, X, n, k,
-                           rpb, colmax);
+        hipLaunchKernelGGL(colmax_kernel<T>, dim3((unsigned)((n + rpb - 1) / rpb)), dim3(BLOCK), 0, ctx->stream, X, n,
+                           k, rpb, colmax);
     }
     if (format == 0)
-        hipLaunchKernelGGL(x_to_lp_kernel<0>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld, colmax, XT,
-                           col_inv);
+        hipLaunchKernelGGL((x_to_lp_kernel<0, T>), dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld,
+                           colmax, XT, col_inv);
     else
-        hipLaunchKernelGGL(x_to_lp_kernel<1>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld, colmax, XT,
-                           col_inv);
+        hipLaunchKernelGGL((x_to_lp_kernel<1, T>), dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld,
+                           colmax, XT, col_inv);
     SBH_HIP(hipGetLastError());
     const int nt = (2 * k + 31) / 32;
     const int bn = nt * 32;
@@ -698,11 +787,24 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
     }
 #undef SBH_LP_GEMM
     SBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(lp_reduce_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, Ypart, n, k, bn, row_inv,
-                       col_inv, Y);
+    hipLaunchKernelGGL(lp_reduce_kernel<T>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, Ypart, n, k, bn,
+                       row_inv, col_inv, Y);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_LOWP, ev);
 }
+
+#define SBH_LOWP_PRODUCTS(T)                                                                                         \
+    void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,    \
+                   const T* x, float2* x32, T* y) {                                                                  \
+        lowp_gemv_t(ctx, A, ld, n, format, row_inv, x, x32, y);                                                      \
+    }                                                                                                                \
+    void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,    \
+                   const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y) {         \
+        lowp_gemm_t(ctx, A, ld, n, format, row_inv, X, k, XT, colmax, col_inv, Ypart, Y);                            \
+    }
+SBH_LOWP_PRODUCTS(double2)
+SBH_LOWP_PRODUCTS(float2)
+#undef SBH_LOWP_PRODUCTS
 
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y) {
     hipEvent_t ev;
@@ -732,9 +834,10 @@ void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const
     ctx->end_timed(SBH_KERNEL_BLOCK, ev);
 }
 
-// Rows per block of the column-batched reductions: ~4096 elements per block.
+// Rows per block of the column-batched reductions: one pass of BT rows per
+// thread (S * BT rows, S = BLOCK / k threads per column), at most 4096 blocks.
 static int batched_chunks(int64_t n, int k, int64_t* len) {
-    const int64_t rpb = std::max<int64_t>(1, 4096 / k);
+    const int64_t rpb = (int64_t)(BLOCK / k) * BT;
     int64_t chunks = (n + rpb - 1) / rpb;
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 4096));
     *len = (n + chunks - 1) / chunks;
@@ -752,35 +855,42 @@ static void column_sums(sbh_context* ctx, double* part, int chunks, int64_t nout
     SBH_HIP(hipGetLastError());
 }
 
+// The wrappers below exist for double2 (FP64 vectors) and float2 (FP32
+// vectors); the partial sums and reduced values are FP64 either way.
+
 // Partials land in `scratch` (chunks x (nvec+1) x k complex); the reduced
 // (nvec+1) x k complex values land in `out`.
-void bdots(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* W, int64_t n, int k,
-           double2* scratch, double2* out) {
+template <typename T>
+static void bdots_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T* W, int64_t n, int k,
+                    double2* scratch, double2* out) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    hipLaunchKernelGGL(bdots_kernel, dim3((unsigned)chunks, (unsigned)(nvec + 1)), dim3(BLOCK), 0, ctx->stream, V, ldv,
-                       nvec, W, n, k, len, scratch);
+    hipLaunchKernelGGL(bdots_kernel<T>, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, W, n, k, len,
+                       scratch);
     SBH_HIP(hipGetLastError());
     column_sums(ctx, reinterpret_cast<double*>(scratch), chunks, (int64_t)(nvec + 1) * k * 2,
                 reinterpret_cast<double*>(out));
 }
 
 // W -= sum C_i V_i per column; the reduced k squared norms land in `out`.
-void bproject(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* C, double2* W, int64_t n,
-              int k, double* scratch, double* out) {
+template <typename T>
+static void bproject_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* C, T* W, int64_t n, int k,
+                       double* scratch, double* out) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    hipLaunchKernelGGL(bproject_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n, k,
-                       len, scratch);
+    hipLaunchKernelGGL(bproject_kernel<T>, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n,
+                       k, len, scratch);
     SBH_HIP(hipGetLastError());
     column_sums(ctx, scratch, chunks, (int64_t)k, out);
 }
 
 // R = B - R (B may be null: norms only); the reduced k squared norms land in `out`.
-void bresidual(sbh_context* ctx, const double2* B, double2* R, int64_t n, int k, double* scratch, double* out) {
+template <typename T>
+static void bresidual_t(sbh_context* ctx, const T* B, T* R, int64_t n, int k, double* scratch, double* out) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    hipLaunchKernelGGL(bresidual_kernel, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len, scratch);
+    hipLaunchKernelGGL(bresidual_kernel<T>, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, B, R, n, k, len,
+                       scratch);
     SBH_HIP(hipGetLastError());
     column_sums(ctx, scratch, chunks, (int64_t)k, out);
 }
@@ -791,20 +901,61 @@ int64_t batched_scratch_elems(int64_t n, int k, int max_vec) {
     return (int64_t)(chunks + BSUM_GROUPS) * (max_vec + 1) * k;  // complex elements: partials + group sums
 }
 
-void bupdate(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* Yc, double2* X, int64_t n,
-             int k) {
-    hipLaunchKernelGGL(bupdate_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, Yc, X, n, k);
+template <typename T>
+static void bupdate_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* Yc, T* X, int64_t n, int k) {
+    hipLaunchKernelGGL(bupdate_kernel<T>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, Yc, X, n,
+                       k);
     SBH_HIP(hipGetLastError());
 }
 
-void bscale(sbh_context* ctx, const double2* X, const double* alpha, double2* Y, int64_t n, int k) {
-    hipLaunchKernelGGL(bscale_kernel, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, alpha, Y, n, k);
+template <typename T>
+static void bscale_t(sbh_context* ctx, const T* X, const double* alpha, T* Y, int64_t n, int k) {
+    hipLaunchKernelGGL(bscale_kernel<T>, dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, alpha, Y, n, k);
     SBH_HIP(hipGetLastError());
 }
 
-void zero_fill(sbh_context* ctx, double2* X, int64_t count) {
+template <typename T>
+static void zero_fill_t(sbh_context* ctx, T* X, int64_t count) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(zero_kernel, dim3(nblocks(count)), dim3(BLOCK), 0, ctx->stream, X, count);
+    hipLaunchKernelGGL(zero_kernel<T>, dim3(nblocks(count)), dim3(BLOCK), 0, ctx->stream, X, count);
+    SBH_HIP(hipGetLastError());
+}
+
+#define SBH_VECTOR_OPS(T)                                                                                       \
+    void bdots(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T* W, int64_t n, int k,               \
+               double2* scratch, double2* out) {                                                                \
+        bdots_t(ctx, V, ldv, nvec, W, n, k, scratch, out);                                                      \
+    }                                                                                                           \
+    void bproject(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* C, T* W, int64_t n, int k, \
+                  double* scratch, double* out) {                                                               \
+        bproject_t(ctx, V, ldv, nvec, C, W, n, k, scratch, out);                                                \
+    }                                                                                                           \
+    void bresidual(sbh_context* ctx, const T* B, T* R, int64_t n, int k, double* scratch, double* out) {        \
+        bresidual_t(ctx, B, R, n, k, scratch, out);                                                             \
+    }                                                                                                           \
+    void bupdate(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* Yc, T* X, int64_t n,       \
+                 int k) {                                                                                       \
+        bupdate_t(ctx, V, ldv, nvec, Yc, X, n, k);                                                              \
+    }                                                                                                           \
+    void bscale(sbh_context* ctx, const T* X, const double* alpha, T* Y, int64_t n, int k) {                    \
+        bscale_t(ctx, X, alpha, Y, n, k);                                                                       \
+    }                                                                                                           \
+    void zero_fill(sbh_context* ctx, T* X, int64_t count) { zero_fill_t(ctx, X, count); }
+SBH_VECTOR_OPS(double2)
+SBH_VECTOR_OPS(float2)
+#undef SBH_VECTOR_OPS
+
+void convert_block(sbh_context* ctx, const double2* X, float2* Y, int64_t count) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((convert_block_kernel<double2, float2>), dim3(nblocks(count)), dim3(BLOCK), 0, ctx->stream, X,
+                       Y, count);
+    SBH_HIP(hipGetLastError());
+}
+
+void convert_block(sbh_context* ctx, const float2* X, double2* Y, int64_t count) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((convert_block_kernel<float2, double2>), dim3(nblocks(count)), dim3(BLOCK), 0, ctx->stream, X,
+                       Y, count);
     SBH_HIP(hipGetLastError());
 }
 
