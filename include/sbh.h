@@ -365,7 +365,8 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
  * Mixed precision.  A complex FP16 or BF16 copy of a system matrix, every row
  * scaled by a power of two (crates/sonium-compute/src/scaled_storage.rs:1-127
  * is the reference's explicit-scale storage precedent); products accumulate
- * in FP32 (MFMA for k > 1). */
+ * in FP32 on MFMA for every k.  The device copy is kept in MFMA fragment
+ * order (32-row tiles); sbh_lowp_matrix_download returns it row-major. */
 typedef struct sbh_lowp_matrix sbh_lowp_matrix;
 typedef enum { SBH_LOWP_FP16 = 0, SBH_LOWP_BF16 = 1 } sbh_lowp_format;
 /* Converts the system's current matrix; sys must outlive the copy. */
@@ -375,8 +376,8 @@ sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp);
 void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp);
 /* Y ~= A X in low precision, X and Y n x k row-major (1 <= k <= 64). */
 sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory);
-/* Test hook: the stored halves (n x 2*ld, {re, im} per entry), the row
- * inverse scales (n) and ld. */
+/* Test hook: the stored halves, row-major (n x 2*ld, {re, im} per entry),
+ * the row inverse scales (n) and ld. */
 sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, double* row_inv, int64_t* ld);
 
 /* FGMRES with iterative refinement — GmresIrConfig / GmresIrStatus and

@@ -57,3 +57,25 @@ def row_scaled_halves(A: np.ndarray, fmt: str, ld: int):
     out[:, 0:2 * cols:2] = half_bits(A.real * scale, fmt)
     out[:, 1:2 * cols:2] = half_bits(A.imag * scale, fmt)
     return out, np.ldexp(1.0, e - top)
+
+
+def quantized_columns(X: np.ndarray, fmt: str) -> np.ndarray:
+    """The block operand as the device quantizes it: per column a power-of-two
+    scale from the column's largest |re| or |im| (taken in FP32), entries
+    rounded through FP32 to the format, returned dequantized in FP64."""
+    X = np.asarray(X, dtype=np.complex128)
+    X2 = X.reshape(X.shape[0], -1)
+    m = np.maximum(np.abs(X2.real), np.abs(X2.imag)).astype(np.float32).max(axis=0).astype(np.float64)
+    ok = (m > 0) & np.isfinite(m)
+    _, ex = np.frexp(np.where(ok, m, 1.0))
+    e = np.where(ok, ex - 1, 0)
+    s = np.ldexp(1.0, TOP[fmt] - e)[None, :]
+
+    def deq(bits):
+        if fmt == "fp16":
+            return bits.view(np.float16).astype(np.float64)
+        return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+    q = (deq(half_bits(X2.real * s, fmt)) + 1j * deq(half_bits(X2.imag * s, fmt))) / s
+    return q.reshape(X.shape)
+

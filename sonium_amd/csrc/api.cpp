@@ -14,8 +14,7 @@ sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, i
 sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_B, double2* d_X, int k,
                         const sbh_gmres_ir_config& cfg, sbh_gmres_ir_status* st);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
-void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-               const double2* x, float2* x32, double2* y);
+int64_t lowp_padded_rows(int64_t n);
 void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
                const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y);
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
@@ -1056,12 +1055,11 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         lp->n = sys->n;
         lp->ld = lowp_padded_ld(sys->n);
         lp->format = format;
-        lp->d_A.allocate((size_t)lp->n * 2 * lp->ld);
+        lp->d_A.allocate((size_t)lowp_padded_rows(lp->n) * 2 * lp->ld);
         lp->d_row_inv.allocate((size_t)lp->n);
-        lp->d_XT.allocate((size_t)128 * 2 * lp->ld);
+        lp->d_XT.allocate((size_t)64 * 2 * lp->ld);
         lp->d_colmax.allocate(64);
         lp->d_colinv.allocate(64);
-        lp->d_x32.allocate((size_t)lp->ld);
         lp->d_ypart.allocate((size_t)lowp_split_count() * lp->n * 128);
         SBH_HIP(hipMemsetAsync(lp->d_XT.ptr, 0, lp->d_XT.bytes(), sys->ctx->stream));
         lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr);
@@ -1098,11 +1096,8 @@ sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64*
         const int64_t n = lp->n;
         DeviceInput in(lp->ctx, X, (size_t)(n * k), memory);
         DeviceOutput outv(lp->ctx, Y, (size_t)(n * k), memory);
-        if (k == 1)
-            lowp_gemv(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, lp->d_x32.ptr, outv.ptr);
-        else
-            lowp_gemm(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, (int)k, lp->d_XT.ptr,
-                      lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, outv.ptr);
+        lowp_gemm(lp->ctx, lp->d_A.ptr, lp->ld, n, lp->format, lp->d_row_inv.ptr, in.ptr, (int)k, lp->d_XT.ptr,
+                  lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, outv.ptr);
         outv.finish();
     });
 }
@@ -1112,7 +1107,21 @@ sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, doubl
         SBH_REQUIRE(lp, "matrix is null");
         use_device(lp->ctx);
         SBH_HIP(hipStreamSynchronize(lp->ctx->stream));
-        if (halves) SBH_HIP(hipMemcpy(halves, lp->d_A.ptr, lp->d_A.bytes(), hipMemcpyDeviceToHost));
+        if (halves) {
+            // the device copy is in MFMA fragment order (32-row tiles x 16 halves, 1 KB each);
+            // hand it back row-major, n x 2*ld halves
+            std::vector<uint16_t> packed(lp->d_A.count);
+            SBH_HIP(hipMemcpy(packed.data(), lp->d_A.ptr, lp->d_A.bytes(), hipMemcpyDeviceToHost));
+            const int64_t steps = lp->ld / 8;
+            for (int64_t row = 0; row < lp->n; ++row) {
+                const int64_t rt = row / 32, r = row % 32;
+                for (int64_t s = 0; s < steps; ++s)
+                    for (int h = 0; h < 2; ++h) {
+                        const uint16_t* src = packed.data() + ((rt * steps + s) * 64 + r + 32 * h) * 8;
+                        std::copy(src, src + 8, halves + row * 2 * lp->ld + s * 16 + h * 8);
+                    }
+            }
+        }
         if (row_inv) SBH_HIP(hipMemcpy(row_inv, lp->d_row_inv.ptr, lp->d_row_inv.bytes(), hipMemcpyDeviceToHost));
         if (ld) *ld = lp->ld;
     });

@@ -38,8 +38,6 @@ int64_t batched_scratch_elems(int64_t n, int k, int max_vec);
 // Block vectors of complex FP64 (double2) or FP32 (float2) entries; the
 // reductions return FP64 values for both.
 #define SBH_DECLARE_VECTOR_OPS(T)                                                                                 \
-    void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv, \
-                   const T* x, float2* x32, T* y);                                                                \
     void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv, \
                    const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y);       \
     void bdots(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T* W, int64_t n, int k,                 \
@@ -106,11 +104,8 @@ struct LowpBlockOperator : BlockOperatorT<T> {
         this->n = m->n;
     }
     void apply(const T* X, T* Y, int k) override {
-        if (k == 1)
-            lowp_gemv(this->ctx, lp->d_A.ptr, lp->ld, this->n, lp->format, lp->d_row_inv.ptr, X, lp->d_x32.ptr, Y);
-        else
-            lowp_gemm(this->ctx, lp->d_A.ptr, lp->ld, this->n, lp->format, lp->d_row_inv.ptr, X, k, lp->d_XT.ptr,
-                      lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, Y);
+        lowp_gemm(this->ctx, lp->d_A.ptr, lp->ld, this->n, lp->format, lp->d_row_inv.ptr, X, k, lp->d_XT.ptr,
+                  lp->d_colmax.ptr, lp->d_colinv.ptr, lp->d_ypart.ptr, Y);
         this->applies += 1;
     }
 };

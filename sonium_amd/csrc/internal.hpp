@@ -272,12 +272,11 @@ struct sbh_lowp_matrix {
     int64_t n = 0;
     int64_t ld = 0;  // complex entries per row, multiple of 64
     int format = 0;  // 0 FP16, 1 BF16
-    sbh::DeviceArray<uint16_t> d_A;       // n x 2*ld halves
+    sbh::DeviceArray<uint16_t> d_A;       // ceil(n/32)*32 x 2*ld halves, MFMA fragment order
     sbh::DeviceArray<double> d_row_inv;   // n
-    sbh::DeviceArray<uint16_t> d_XT;      // 128 x 2*ld halves (block operand)
+    sbh::DeviceArray<uint16_t> d_XT;      // 64 x 2*ld halves (real-part block operand)
     sbh::DeviceArray<unsigned> d_colmax;  // 64
     sbh::DeviceArray<double> d_colinv;    // 64
-    sbh::DeviceArray<float2> d_x32;       // ld (vector operand)
     sbh::DeviceArray<float> d_ypart;      // split-K partial sums, splits x n x 128
 };
 

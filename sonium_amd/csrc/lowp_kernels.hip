@@ -7,23 +7,28 @@
 // dense.rs:450-488) and the column-batched vector kernels that let k
 // right-hand sides share one block product per Krylov step.
 //
-// Layout.  The low-precision matrix is row-major with ld >= n complex
-// entries per row (ld a multiple of 512, zero padded); an entry is two 16-bit
-// halves {re, im}, so a row is also the real row [re0 im0 re1 im1 ...] of
-// length 2*ld.  Row i stores A_ij * 2^(T - e_i) where 2^e_i <= max_j |A_ij| <
-// 2^(e_i+1) and T = 14 (FP16: the largest scaled entry stays below 2^15) or
-// 0 (BF16 has the FP32 exponent range); row_inv[i] = 2^(e_i - T) undoes it
-// exactly.
+// Layout.  Row i of the low-precision matrix stores A_ij * 2^(T - e_i) where
+// 2^e_i <= max_j max(|Re A_ij|, |Im A_ij|) < 2^(e_i+1) and T = 14 (FP16: the
+// largest scaled entry stays below 2^15) or 0 (BF16 has the FP32 exponent
+// range); row_inv[i] = 2^(e_i - T) undoes it exactly.  An entry is two 16-bit
+// halves {re, im}, so a row is the real row [re0 im0 re1 im1 ...] of length
+// 2*ld (ld >= n complex entries, a multiple of 512, zero padded).  The rows
+// are stored in MFMA fragment order rather than row-major: the matrix is cut
+// into tiles of 32 rows x 16 halves, and tile (r, s) is 1 KB at fragment index
+// r * (ld / 8) + s, holding for lane l = 0..63 the 8 halves of row 32 r + l % 32
+// starting at half 16 s + 8 (l / 32) -- exactly the A operand of
+// v_mfma_f32_32x32x16_{f16,bf16}.  A wave therefore reads its operand for one
+// MFMA step as a single contiguous 1 KB load; rows beyond n are zero.
 //
 // Complex GEMM on MFMA.  With A_r the N x 2N real view of the complex
 // matrix, Y = A X is the real product A_r * X_r where X_r (2N x 2k) holds, for
 // complex row j, the rows [xr_j, xi_j] and [-xi_j, xr_j]; so output column
-// c < k is Re Y[:, c] and column k + c is Im Y[:, c].  X_r is stored
-// transposed (XT: BN x 2 ld, BN = 2k rounded up to 32) so each lane's MFMA B
-// operand (8 consecutive k for one column) is one 16-byte load, the same as
-// the A operand (8 consecutive k of one row).  Each column of X is scaled by
-// a power of two as well.  v_mfma_f32_32x32x16_{f16,bf16}: A 32x16, B 16x32,
-// C 32x32 FP32 in 16 registers per lane.
+// c < k is Re Y[:, c] and column k + c is Im Y[:, c].  Only the first half is
+// stored, transposed (XT: k x 2 ld halves, row c = [xr_0, -xi_0, xr_1, ...]);
+// the second half's operand [xi_j, xr_j] is the first's with the two halves of
+// every 32-bit word swapped and the new low half negated, made in registers.
+// Each lane's MFMA B operand (8 consecutive k for one column) is one 16-byte
+// LDS read.  Each column of X is scaled by a power of two as well.
 
 #include "internal.hpp"
 
@@ -49,129 +54,73 @@ __device__ __forceinline__ double2 st2<double2>(double2 v) { return v; }
 template <>
 __device__ __forceinline__ float2 st2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
-// Row conversion, the same arithmetic as convert_row.  A row is read twice
-// (its maximum, then the values), so the grid is persistent: CONV_GRID
-// workgroups walk the rows and only CONV_GRID rows are in flight at once
-// (256 rows of a 40960-panel system are 168 MB), which lets the second read
-// come from the last-level cache rather than HBM.  Each thread keeps four
-// 16-byte loads in flight and stores four complex entries in one 16-byte store.
-constexpr int CONV_THREADS = 1024;
-constexpr unsigned CONV_GRID = 256;
+// Row scales: one wave per row finds max |re|, |im| and writes row_inv.
+__global__ __launch_bounds__(BLOCK) void lp_rowscale_kernel(const double2* __restrict__ A, int64_t lda, int64_t n,
+                                                            int top, double* __restrict__ row_inv) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double2* __restrict__ row = A + i * lda;
+    double m = 0.0;
+    int64_t j = lane;
+    for (; j + 192 < n; j += 256) {
+        const double2 v0 = row[j], v1 = row[j + 64], v2 = row[j + 128], v3 = row[j + 192];
+        m = fmax(m, fmax(fmax(fabs(v0.x), fabs(v0.y)), fmax(fabs(v1.x), fabs(v1.y))));
+        m = fmax(m, fmax(fmax(fabs(v2.x), fabs(v2.y)), fmax(fabs(v3.x), fabs(v3.y))));
+    }
+    for (; j < n; j += 64) {
+        const double2 v = row[j];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    m = wave_max(m);
+    int e = 0;
+    if (m > 0.0 && isfinite(m)) e = ilogb(m);
+    if (lane == 0) row_inv[i] = ldexp(1.0, e - top);
+}
+
+// Packing: a block converts one 32-row tile x 64 complex entries (8 fragments).
+// Each wave-wide load reads 64 consecutive entries (1 KB) of one row; the
+// converted {re, im} words go through LDS (rows padded to 68 words so a
+// fragment read hits 16 distinct bank groups per 16 lanes) and leave as 8
+// contiguous 1 KB fragments.  Values are scaled by 1/row_inv (a power of two,
+// exact) and rounded through FP32 to the format.
+constexpr int LP_PACK_COLS = 64;               // complex entries per block (8 fragments)
+constexpr int LP_PACK_ROW = LP_PACK_COLS + 4;  // padded LDS row, in 32-bit words
 
 template <int FMT>
-__global__ __launch_bounds__(CONV_THREADS) void lp_convert_kernel(const double2* __restrict__ A, int64_t lda,
-                                                                  int64_t n, int64_t ld, u16* __restrict__ out,
-                                                                  double* __restrict__ row_inv) {
-    __shared__ double sh[CONV_THREADS / 64];
-    const int t = threadIdx.x;
-    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const double2* __restrict__ row = A + i * lda;
-        double m = 0.0;
-        int64_t j = t;
-        for (; j + 3 * CONV_THREADS < n; j += 4 * CONV_THREADS) {
-            const double2 v0 = row[j], v1 = row[j + CONV_THREADS];
-            const double2 v2 = row[j + 2 * CONV_THREADS], v3 = row[j + 3 * CONV_THREADS];
-            m = fmax(m, fmax(fmax(fabs(v0.x), fabs(v0.y)), fmax(fabs(v1.x), fabs(v1.y))));
-            m = fmax(m, fmax(fmax(fabs(v2.x), fabs(v2.y)), fmax(fabs(v3.x), fabs(v3.y))));
-        }
-        for (; j < n; j += CONV_THREADS) {
-            const double2 v = row[j];
-            m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
-        }
-        m = wave_max(m);
-        if ((t & 63) == 0) sh[t >> 6] = m;
-        __syncthreads();
-        m = sh[0];
+__global__ __launch_bounds__(BLOCK) void lp_pack_kernel(const double2* __restrict__ A, int64_t lda, int64_t n,
+                                                        int64_t ld, const double* __restrict__ row_inv,
+                                                        u16* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) unsigned sh[32 * LP_PACK_ROW];
+    const int64_t groups = ld / LP_PACK_COLS;
+    const int64_t rt = blockIdx.x / groups;
+    const int64_t g = blockIdx.x % groups;
+    const int64_t j_base = g * LP_PACK_COLS;
 #pragma unroll
-        for (int w = 1; w < CONV_THREADS / 64; ++w) m = fmax(m, sh[w]);
-        int e = 0;
-        if (m > 0.0 && isfinite(m)) e = ilogb(m);
-        const double scale = ldexp(1.0, Half<FMT>::TOP - e);
-        if (t == 0) row_inv[i] = ldexp(1.0, e - Half<FMT>::TOP);
-        u32x4* __restrict__ o = reinterpret_cast<u32x4*>(out + i * 2 * ld);
-        for (int64_t q = t; q < ld / 4; q += CONV_THREADS) {
-            u32x4 w;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int64_t jj = 4 * q + s;
-                unsigned re = 0, im = 0;
-                if (jj < n) {
-                    const double2 v = row[jj];
-                    re = Half<FMT>::from_float((float)(v.x * scale));
-                    im = Half<FMT>::from_float((float)(v.y * scale));
-                }
-                w[s] = re | (im << 16);
-            }
-            o[q] = w;
+    for (int i = 0; i < 32 * LP_PACK_COLS / BLOCK; ++i) {
+        const int e = i * BLOCK + threadIdx.x;
+        const int r = e / LP_PACK_COLS, col = e % LP_PACK_COLS;
+        const int64_t row = rt * 32 + r, j = j_base + col;
+        unsigned w = 0;
+        if (row < n && j < n) {
+            const double2 v = A[row * lda + j];
+            const double scale = 1.0 / row_inv[row];
+            w = (unsigned)Half<FMT>::from_float((float)(v.x * scale)) |
+                ((unsigned)Half<FMT>::from_float((float)(v.y * scale)) << 16);
         }
-        __syncthreads();  // sh is rewritten for the next row
+        sh[r * LP_PACK_ROW + col] = w;
     }
-}
-
-// y = diag(row_inv) * (A_lp x), x given in FP32 (ld entries, zero padded).
-// Each wave owns 4 rows; a lane loads 4 complex entries (16 bytes) per row per step.
-template <int FMT, typename T>
-__global__ __launch_bounds__(BLOCK) void lp_gemv_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
-                                                        const float2* __restrict__ x,
-                                                        const double* __restrict__ row_inv,
-                                                        T* __restrict__ y) {
-    constexpr int ROWS = 4;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * ROWS;
-    if (r0 >= n) return;
-    const u32x4* rows[ROWS];
-    float sr[ROWS], si[ROWS];
+    const int wave = threadIdx.x >> 6;
+    const int64_t steps = ld / 8;
+    u32x4* __restrict__ dst = reinterpret_cast<u32x4*>(out) + (rt * steps + g * 8) * 64 + lane;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        int64_t rr = min(r0 + r, n - 1);
-        rows[r] = reinterpret_cast<const u32x4*>(A + rr * 2 * ld);
-        sr[r] = 0.f;
-        si[r] = 0.f;
+    for (int f = 0; f < 8 / (BLOCK / 64); ++f) {
+        const int q = wave * (8 / (BLOCK / 64)) + f;  // fragment within the block's 8
+        const u32x4 v = *reinterpret_cast<const u32x4*>(&sh[(lane & 31) * LP_PACK_ROW + q * 8 + 4 * (lane >> 5)]);
+        dst[(int64_t)q * 64] = v;
     }
-    const int64_t quads = ld / 4;  // 4 complex entries per 16-byte load
-    const float4* xq = reinterpret_cast<const float4*>(x);
-    for (int64_t q = lane; q < quads; q += 64) {
-        float4 x01 = xq[2 * q];
-        float4 x23 = xq[2 * q + 1];
-        float xr[4] = {x01.x, x01.z, x23.x, x23.z};
-        float xi[4] = {x01.y, x01.w, x23.y, x23.w};
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            u32x4 a = __builtin_nontemporal_load(rows[r] + q);
-            unsigned w[4] = {a[0], a[1], a[2], a[3]};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float ar = Half<FMT>::to_float((u16)(w[e] & 0xffffu));
-                float ai = Half<FMT>::to_float((u16)(w[e] >> 16));
-                sr[r] = __builtin_fmaf(ar, xr[e], sr[r]);
-                sr[r] = __builtin_fmaf(-ai, xi[e], sr[r]);
-                si[r] = __builtin_fmaf(ar, xi[e], si[r]);
-                si[r] = __builtin_fmaf(ai, xr[e], si[r]);
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        sr[r] = wave_sumf(sr[r]);
-        si[r] = wave_sumf(si[r]);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r)
-            if (r0 + r < n) {
-                double s = row_inv[r0 + r];
-                y[r0 + r] = st2<T>(make_double2((double)sr[r] * s, (double)si[r] * s));
-            }
-    }
-}
-
-// Vector -> FP32 vector (ld entries, zero padded).
-template <typename T>
-__global__ void to_f32_kernel(const T* __restrict__ x, int64_t n, int64_t ld, float2* __restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ld) return;
-    double2 v = j < n ? ld2(x[j]) : make_double2(0.0, 0.0);
-    out[j] = make_float2((float)v.x, (float)v.y);
 }
 
 // Column max |re|,|im| of an n x k block (atomic max on the bit pattern of a
@@ -206,7 +155,7 @@ __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
     if (t < n) p[t] = v;
 }
 
-// X (n x k complex) -> XT (BN x 2*ld halves), scaled by 2^(TOP - e_c) per column.
+// X (n x k complex) -> XT (k x 2*ld halves: [xr, -xi] pairs), scaled by 2^(TOP - e_c) per column.
 template <int FMT, typename T>
 __global__ void x_to_lp_kernel(const T* __restrict__ X, int64_t n, int k, int64_t ld,
                                const unsigned* __restrict__ colmax, u16* __restrict__ XT,
@@ -221,103 +170,115 @@ __global__ void x_to_lp_kernel(const T* __restrict__ X, int64_t n, int k, int64_
     double s = ldexp(1.0, Half<FMT>::TOP - e);
     if (j == 0) col_inv[c] = ldexp(1.0, e - Half<FMT>::TOP);
     double2 v = ld2(X[t]);
-    u16 xr = Half<FMT>::from_float((float)(v.x * s));
-    u16 xi = Half<FMT>::from_float((float)(v.y * s));
-    u16 nxi = Half<FMT>::from_float((float)(-v.y * s));
+    const u16 xr = Half<FMT>::from_float((float)(v.x * s));
+    const u16 nxi = Half<FMT>::from_float((float)(-v.y * s));
     u16* re_row = XT + (int64_t)c * 2 * ld;
-    u16* im_row = XT + (int64_t)(k + c) * 2 * ld;
     re_row[2 * j] = xr;
     re_row[2 * j + 1] = nxi;
-    im_row[2 * j] = xi;
-    im_row[2 * j + 1] = xr;
 }
 
-// Split-K MFMA product with FP32 accumulation.  Block = 4 waves x 32 rows
-// (128 rows) and one of LP_SPLIT K slabs; slab = blockIdx % LP_SPLIT, so on
-// the 8-XCD part every XCD streams one slab of the transposed block operand,
-// which stays in that XCD's L2.  Each stage of LP_BK k moves the block's 128
-// rows of A (16 lanes per 256-byte row segment, fully coalesced) and the
-// matching XT columns into LDS (rows padded to 136 halves); the next stage is
-// loaded into registers while the current one runs on MFMA.  Partial sums go
-// to Ypart[slab][row][col] and lp_reduce_kernel adds the slabs in a fixed
-// order.
+// Split-K MFMA product with FP32 accumulation.  Block = LP_WAVES waves, each
+// owning one 32-row tile, and one of LP_SPLIT K slabs; slab = blockIdx %
+// LP_SPLIT, so on the 8-XCD part every XCD streams one slab of the block
+// operand, which stays in that XCD's L2.  A wave loads its A fragments for a
+// stage of LP_BK halves (8 x 1 KB, contiguous) straight into registers, one
+// stage ahead; the block's XT columns for the stage go through LDS (rows padded
+// to 136 halves), shared by all waves.  Per 16-half step a wave does NTR MFMAs
+// for the real-part columns and NTR for the imaginary-part columns (operand
+// derived in registers).  Partial sums go to Ypart[slab][row][col] (real part
+// at col, imaginary part at NTR * 32 + col) and lp_reduce_kernel adds the
+// slabs in a fixed order.
 constexpr int LP_SPLIT = 8;
 constexpr int LP_BK = 128;
 constexpr int LP_LDS_ROW = LP_BK + 8;
+constexpr int LP_WAVES = 8;
+constexpr int LP_THREADS = LP_WAVES * 64;
 
-template <int FMT, int NT>
-__global__ __launch_bounds__(BLOCK) void lp_gemm_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
-                                                        const u16* __restrict__ XT, int bn,
-                                                        float* __restrict__ Ypart) {
-    __shared__ __attribute__((aligned(16))) u16 As[128 * LP_LDS_ROW];
-    __shared__ __attribute__((aligned(16))) u16 Bs[NT * 32 * LP_LDS_ROW];
+__device__ __forceinline__ u32x4 imag_operand(u32x4 b) {  // [xr, -xi] -> [xi, xr] per 32-bit word
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = ((b[i] << 16) | (b[i] >> 16)) ^ 0x8000u;
+    return r;
+}
+
+template <int FMT, int NTR>
+__global__ __launch_bounds__(LP_THREADS) void lp_gemm_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
+                                                             const u16* __restrict__ XT, float* __restrict__ Ypart) {
+    constexpr int BN = 2 * NTR * 32;
+    __shared__ __attribute__((aligned(16))) u16 Bs[NTR * 32 * LP_LDS_ROW];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int split = blockIdx.x % LP_SPLIT;
-    const int64_t row_block = (int64_t)(blockIdx.x / LP_SPLIT) * 128;
+    const int64_t n_tiles = (n + 31) / 32;
+    const int64_t rt = (int64_t)(blockIdx.x / LP_SPLIT) * LP_WAVES + wave;
+    const bool live = rt < n_tiles;
     const int64_t slab = (2 * ld) / LP_SPLIT;  // a multiple of LP_BK (ld is a multiple of 512)
     const int64_t k_begin = split * slab;
     const int64_t k_end = k_begin + slab;
-    // global -> LDS mapping: thread t moves 16 bytes of row (t/16) + 16 r, column group t % 16
-    const int ar = threadIdx.x >> 4;
-    const int aq = threadIdx.x & 15;
-    const u16* arow[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) arow[r] = A + min(row_block + ar + 16 * r, n - 1) * 2 * ld + aq * 8;
-    constexpr int BR = (NT * 32 * (LP_BK / 8)) / BLOCK;  // 16-byte pieces of the XT stage per thread
-    u32x4 ra[8], rb[BR];
+    const u32x4* __restrict__ afrag = reinterpret_cast<const u32x4*>(A) + (live ? rt : 0) * (ld / 8) * 64 + lane;
+    constexpr int AS = LP_BK / 16;                                // fragments per stage
+    constexpr int BR = (NTR * 32 * (LP_BK / 8)) / LP_THREADS;     // 16-byte XT pieces per thread per stage
+    u32x4 ra[AS], rb[BR];
     auto load = [&](int64_t kc) {
+        if (live) {
+            const int64_t s0 = kc / 16;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) ra[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(arow[r] + kc));
+            for (int i = 0; i < AS; ++i) ra[i] = __builtin_nontemporal_load(afrag + (s0 + i) * 64);
+        }
 #pragma unroll
         for (int i = 0; i < BR; ++i) {
-            const int e = threadIdx.x + i * BLOCK;
+            const int e = threadIdx.x + i * LP_THREADS;
             rb[i] = *reinterpret_cast<const u32x4*>(XT + (int64_t)(e >> 4) * 2 * ld + kc + (e & 15) * 8);
         }
     };
-    f32x16 acc[NT];
+    f32x16 acc[2 * NTR];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int t = 0; t < 2 * NTR; ++t)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
     const int kq = (lane >> 5) * 8;
     load(k_begin);
     for (int64_t kc = k_begin; kc < k_end; kc += LP_BK) {
+        u32x4 ca[AS];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) *reinterpret_cast<u32x4*>(&As[(ar + 16 * r) * LP_LDS_ROW + aq * 8]) = ra[r];
+        for (int i = 0; i < AS; ++i) ca[i] = ra[i];
 #pragma unroll
         for (int i = 0; i < BR; ++i) {
-            const int e = threadIdx.x + i * BLOCK;
+            const int e = threadIdx.x + i * LP_THREADS;
             *reinterpret_cast<u32x4*>(&Bs[(e >> 4) * LP_LDS_ROW + (e & 15) * 8]) = rb[i];
         }
         __syncthreads();
         if (kc + LP_BK < k_end) load(kc + LP_BK);
+        if (live) {
 #pragma unroll
-        for (int s = 0; s < LP_BK / 16; ++s) {
-            const u32x4 a = *reinterpret_cast<const u32x4*>(&As[(wave * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
+            for (int s = 0; s < AS; ++s) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[(t * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
-                acc[t] = mfma32<FMT>(a, b, acc[t]);
+                for (int t = 0; t < NTR; ++t) {
+                    const u32x4 b =
+                        *reinterpret_cast<const u32x4*>(&Bs[(t * 32 + (lane & 31)) * LP_LDS_ROW + s * 16 + kq]);
+                    acc[t] = mfma32<FMT>(ca[s], b, acc[t]);
+                    acc[NTR + t] = mfma32<FMT>(ca[s], imag_operand(b), acc[NTR + t]);
+                }
             }
         }
         __syncthreads();
     }
+    if (!live) return;
     // C layout of v_mfma_f32_32x32x*: column = lane % 32,
     // row = 8*(v/4) + 4*(lane/32) + (v%4) for accumulator register v.
-    const int64_t row_base = row_block + wave * 32;
+    const int64_t row_base = rt * 32;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < 2 * NTR; ++t) {
         const int col = t * 32 + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int64_t row = row_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            if (row < n) Ypart[((int64_t)split * n + row) * bn + col] = acc[t][v];
+            if (row < n) Ypart[((int64_t)split * n + row) * BN + col] = acc[t][v];
         }
     }
 }
 
-// Y[row][c] = row_inv[row] * col_inv[c] * sum_slab (Ypart[..][row][c] + i Ypart[..][row][k+c])
+// Y[row][c] = row_inv[row] * col_inv[c] * sum_slab (Ypart[..][row][c] + i Ypart[..][row][bn/2 + c])
 template <typename T>
 __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int k, int bn,
                                  const double* __restrict__ row_inv, const double* __restrict__ col_inv,
@@ -330,7 +291,7 @@ __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int
     for (int s = 0; s < LP_SPLIT; ++s) {
         const float* p = Ypart + ((int64_t)s * n + row) * bn;
         re += (double)p[c];
-        im += (double)p[k + c];
+        im += (double)p[bn / 2 + c];
     }
     const double sc = row_inv[row] * col_inv[c];
     Y[t] = st2<T>(make_double2(re * sc, im * sc));
@@ -721,38 +682,32 @@ int64_t lowp_padded_ld(int64_t n) { return (n + 511) / 512 * 512; }
 
 int64_t lowp_split_count() { return LP_SPLIT; }
 
+// Rows of the fragment-ordered copy: n rounded up to whole 32-row tiles.
+int64_t lowp_padded_rows(int64_t n) { return (n + 31) / 32 * 32; }
+
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
                   uint16_t* out, double* row_inv) {
-    const dim3 grid((unsigned)std::min<int64_t>(n, CONV_GRID));
+    const int top = format == 0 ? Half<0>::TOP : Half<1>::TOP;
+    hipLaunchKernelGGL(lp_rowscale_kernel, dim3(nblocks(n, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, lda, n, top,
+                       row_inv);
+    SBH_HIP(hipGetLastError());
+    const int64_t pack_blocks = (n + 31) / 32 * (ld / LP_PACK_COLS);
+    SBH_REQUIRE(pack_blocks < ((int64_t)1 << 31), "low-precision copy too large");
+    const dim3 grid((unsigned)pack_blocks);
     if (format == 0)
-        hipLaunchKernelGGL(lp_convert_kernel<0>, grid, dim3(CONV_THREADS), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+        hipLaunchKernelGGL(lp_pack_kernel<0>, grid, dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, row_inv,
+                           reinterpret_cast<u16*>(out));
     else
-        hipLaunchKernelGGL(lp_convert_kernel<1>, grid, dim3(CONV_THREADS), 0, ctx->stream, A, lda, n, ld, out, row_inv);
+        hipLaunchKernelGGL(lp_pack_kernel<1>, grid, dim3(BLOCK), 0, ctx->stream, A, lda, n, ld, row_inv,
+                           reinterpret_cast<u16*>(out));
     SBH_HIP(hipGetLastError());
 }
 
-template <typename T>
-static void lowp_gemv_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
-                        const T* x, float2* x32, T* y) {
-    hipLaunchKernelGGL(to_f32_kernel<T>, dim3(nblocks(ld)), dim3(BLOCK), 0, ctx->stream, x, n, ld, x32);
-    SBH_HIP(hipGetLastError());
-    const int64_t waves = (n + 3) / 4;
-    hipEvent_t ev;
-    ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
-    if (format == 0)
-        hipLaunchKernelGGL((lp_gemv_kernel<0, T>), dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld,
-                           n, x32, row_inv, y);
-    else
-        hipLaunchKernelGGL((lp_gemv_kernel<1, T>), dim3(nblocks(waves, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, ld,
-                           n, x32, row_inv, y);
-    SBH_HIP(hipGetLastError());
-    ctx->end_timed(SBH_KERNEL_LOWP, ev);
-}
-
+// Y = A X for k in 1..64 columns.  Rows k..32*ntr-1 of XT (left from a wider
+// product or zero) feed only output columns that lp_reduce_kernel discards.
 template <typename T>
 static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
                         const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y) {
-    // k <= 64: the real block operand has 2k columns, padded to NT*32
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, ctx->stream, colmax, k, 0u);
     {
         const int64_t rpb = std::max<int64_t>(1, 4096 / k);
@@ -766,24 +721,23 @@ static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t
         hipLaunchKernelGGL((x_to_lp_kernel<1, T>), dim3(nblocks(n * k)), dim3(BLOCK), 0, ctx->stream, X, n, k, ld,
                            colmax, XT, col_inv);
     SBH_HIP(hipGetLastError());
-    const int nt = (2 * k + 31) / 32;
-    const int bn = nt * 32;
-    const unsigned grid = (unsigned)(((n + 127) / 128) * LP_SPLIT);
+    const int ntr = (k + 31) / 32;  // 32-column tiles of the real part (the imaginary part has as many)
+    const int bn = 2 * ntr * 32;
+    const int64_t n_tiles = (n + 31) / 32;
+    const int64_t row_blocks = (n_tiles + LP_WAVES - 1) / LP_WAVES;
+    SBH_REQUIRE(row_blocks * LP_SPLIT < ((int64_t)1 << 31), "low-precision product too large");
+    const unsigned grid = (unsigned)(row_blocks * LP_SPLIT);
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
-#define SBH_LP_GEMM(F, NTV)                                                                                        \
-    hipLaunchKernelGGL((lp_gemm_kernel<F, NTV>), dim3(grid), dim3(BLOCK), 0, ctx->stream, reinterpret_cast<const u16*>(A), \
-                       ld, n, reinterpret_cast<const u16*>(XT), bn, Ypart)
+#define SBH_LP_GEMM(F, NTV)                                                                                          \
+    hipLaunchKernelGGL((lp_gemm_kernel<F, NTV>), dim3(grid), dim3(LP_THREADS), 0, ctx->stream,                       \
+                       reinterpret_cast<const u16*>(A), ld, n, reinterpret_cast<const u16*>(XT), Ypart)
     if (format == 0) {
-        if (nt == 1) SBH_LP_GEMM(0, 1);
-        else if (nt == 2) SBH_LP_GEMM(0, 2);
-        else if (nt == 3) SBH_LP_GEMM(0, 3);
-        else SBH_LP_GEMM(0, 4);
+        if (ntr == 1) SBH_LP_GEMM(0, 1);
+        else SBH_LP_GEMM(0, 2);
     } else {
-        if (nt == 1) SBH_LP_GEMM(1, 1);
-        else if (nt == 2) SBH_LP_GEMM(1, 2);
-        else if (nt == 3) SBH_LP_GEMM(1, 3);
-        else SBH_LP_GEMM(1, 4);
+        if (ntr == 1) SBH_LP_GEMM(1, 1);
+        else SBH_LP_GEMM(1, 2);
     }
 #undef SBH_LP_GEMM
     SBH_HIP(hipGetLastError());
@@ -794,10 +748,6 @@ static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t
 }
 
 #define SBH_LOWP_PRODUCTS(T)                                                                                         \
-    void lowp_gemv(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,    \
-                   const T* x, float2* x32, T* y) {                                                                  \
-        lowp_gemv_t(ctx, A, ld, n, format, row_inv, x, x32, y);                                                      \
-    }                                                                                                                \
     void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,    \
                    const T* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, T* Y) {         \
         lowp_gemm_t(ctx, A, ld, n, format, row_inv, X, k, XT, colmax, col_inv, Ypart, Y);                            \

@@ -73,20 +73,19 @@ def test_lowp_products(c1_block, fmt, k):
     D = lp.dequantized()
     rng = np.random.default_rng(k)
     X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
-    Y = lp.apply(X if k > 1 else X[:, 0])
-    if k == 1:
-        x32 = X[:, 0].astype(np.complex64).astype(np.complex128)
-        ref = D @ x32
-        assert rel_l2(Y, ref) < 2e-6  # FP32 accumulation of the dequantized operands
-        assert rel_l2(Y, A @ X[:, 0]) < 4 * U[fmt]
-    else:
-        # the block operand is quantized per column with a power-of-two scale
-        assert rel_fro(Y, D @ X) < 4 * U[fmt]
-        assert rel_fro(Y, A @ X) < 6 * U[fmt]
-        # each column equals the single-column product of its own quantized vector
-        col = 3 % k
-        ycol = lp.apply(X[:, col].copy())
-        assert rel_l2(Y[:, col], ycol) < 4 * U[fmt]
+    Y = lp.apply(X if k > 1 else X[:, 0]).reshape(n, k)
+    # the operand is quantized per column with a power-of-two scale; the product
+    # is then the dequantized one up to FP32 accumulation
+    from oracle import lowp as olp
+
+    Xq = olp.quantized_columns(X, fmt)
+    assert rel_fro(Y, D @ Xq) < 5e-6
+    assert rel_fro(Y, D @ X) < 4 * U[fmt]
+    assert rel_fro(Y, A @ X) < 6 * U[fmt]
+    # each column equals the single-column product of its own quantized vector
+    col = 3 % k
+    ycol = lp.apply(X[:, col].copy())
+    assert rel_l2(Y[:, col], ycol) < 5e-6
 
 
 def test_fp64_block_apply(c1_block):

@@ -67,3 +67,15 @@ def test_dequantized_error_is_one_rounding(fmt, u):
     rowmax = np.maximum(np.abs(A.real), np.abs(A.imag)).max(axis=1)
     s = rowmax / inv / 2.0 ** top
     assert np.all((s >= 1.0) & (s < 2.0))
+
+
+def test_quantized_columns_scale_and_rounding():
+    # column 0: max 3 -> scale 2^13 (FP16)
+    X = np.array([[3.0 + 0j, 0.5j], [1 + 2.0 ** -12 + 0j, 0.25 + 0j]])
+    q = lowp.quantized_columns(X, "fp16")
+    assert q[0, 0] == 3.0 and q[1, 1] == 0.25 and q[0, 1] == 0.5j
+    # (1 + 2^-12) * 2^13 = 8194, and the FP16 spacing at 2^13 is 8 -> 8192
+    assert q[1, 0] == 1.0
+    # a zero column stays zero; a 1-D vector keeps its shape
+    assert np.all(lowp.quantized_columns(np.zeros(4, complex), "bf16") == 0)
+    assert lowp.quantized_columns(np.ones(3, complex), "bf16").shape == (3,)
