@@ -217,3 +217,33 @@ def test_device_block_solvers_equal_host_calls(ctx, c1_block):
     assert outer + 1 <= st[0].fp64_products <= outer + finishes
     with pytest.raises(ValueError):
         sa.gmres_ir_device(op, lp, Bd, k + 1, Xd, icfg)
+
+
+@pytest.mark.parametrize("mesh_args", [("ico", 1), ("fib", (11, 7)), ("fib", (13, 13))])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_lowp_ragged_sizes(ctx, mesh_args, fmt):
+    """Panel counts that are not a multiple of the 32-row tile (80, 140, 312):
+    the packed copy downloads bit for bit as the oracle's row-major storage
+    (padding rows never leak), and products for k on both sides of the
+    32-column tile boundary equal the exactly quantized product."""
+    from oracle import lowp as olp
+    from sonium_amd.mesh import fibonacci_sphere
+
+    kind, arg = mesh_args
+    m = icosphere(1.0, arg) if kind == "ico" else fibonacci_sphere(1.0, *arg)
+    n = m.num_elements
+    assert n % 32
+    tm = sa.TbemMesh(ctx, m)
+    ph = sa.PhysicsParams.from_wavenumber(3.0)
+    system = sa.TbemAssemblyWorkspace(ctx, tm).assemble(ph, ph.burton_miller_beta())
+    A = system.matrix
+    lp = sa.LowpMatrix(system, fmt)
+    halves, inv, ld = lp.download()
+    ref_h, ref_inv = olp.row_scaled_halves(A, fmt, ld)
+    assert np.array_equal(inv, ref_inv) and np.array_equal(halves, ref_h)
+    D = lp.dequantized()
+    rng = np.random.default_rng(n)
+    for k in (1, 3, 32, 33, 64):
+        X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+        Y = lp.apply(X if k > 1 else X[:, 0]).reshape(n, k)
+        assert rel_fro(Y, D @ olp.quantized_columns(X, fmt)) < 5e-6, k
