@@ -128,6 +128,12 @@ struct BatchedParams {
     // The result is the same for any finite A (a non-finite A then shows at the
     // first Krylov product instead of the first residual).
     bool x_zero = false;
+    // Unconditional two-pass classical Gram-Schmidt whose coefficients go from
+    // the dot products straight into the projection on the device, as the
+    // reference's inner solve does (gmres_inner, gmres_ir.rs:360-394: h = d1 + d2,
+    // then the norm after the second pass); one host read per Arnoldi step.
+    // Otherwise: one pass plus the DGKS pass when ||w|| < 0.707 ||w0|| (gmres_core).
+    bool cgs2 = false;
 };
 
 struct ColumnResult {
@@ -156,7 +162,8 @@ class BatchedGmres {
         alpha_.allocate((size_t)k);
         // Separate pinned regions: each is rewritten by the host only after a
         // stream synchronisation has retired the copy that last used it.
-        down_.allocate(((size_t)(m + 2) * k + 16) * 2);
+        down_.allocate(((size_t)(m + 2) * k * 2 + 16) * 2 + k);
+        gs_.allocate((size_t)2 * (m + 2) * k + (k + 1) / 2 + 1);
         up_coef_.allocate((size_t)(m + 1) * k * 2 + 16);
         up_alpha_.allocate((size_t)k + 16);
         pinned_ = down_.ptr;
@@ -171,7 +178,7 @@ class BatchedGmres {
     int k_, m_;
     bool flexible_;
     DeviceArray<T> V_, Z_, W_, R_;
-    DeviceArray<double2> part_, C_, scratch_;
+    DeviceArray<double2> part_, C_, scratch_, gs_;
     DeviceArray<double> alpha_;
     PinnedArray<double> down_, up_coef_, up_alpha_;
     double* pinned_ = nullptr;  // download region
@@ -364,6 +371,32 @@ void BatchedGmres<T>::run(BlockOperatorT<T>& A, BlockPreconditionerT<T>* M, cons
                 zj = zvec(j);
             }
             A.apply(zj, W_.ptr, k);
+            if (p.cgs2) {
+                // pass 1 and pass 2 of CGS2 with device-side coefficients; a column that has
+                // stopped has zero vectors from here on, so projecting it changes nothing
+                const size_t region = (size_t)(m + 2) * k;
+                double2* d1 = gs_.ptr;
+                double2* d2 = gs_.ptr + region;
+                double* nrm = reinterpret_cast<double*>(gs_.ptr + 2 * region);
+                double* scratch = reinterpret_cast<double*>(scratch_.ptr);
+                bdots(ctx_, V_.ptr, n_ * k_, j + 1, W_.ptr, n_, k, scratch_.ptr, d1);
+                bproject(ctx_, V_.ptr, n_ * k_, j + 1, d1, W_.ptr, n_, k, scratch, reinterpret_cast<double*>(part_.ptr));
+                bdots(ctx_, V_.ptr, n_ * k_, j + 1, W_.ptr, n_, k, scratch_.ptr, d2);
+                bproject(ctx_, V_.ptr, n_ * k_, j + 1, d2, W_.ptr, n_, k, scratch, nrm);
+                fetch(gs_.ptr, 2 * region * 2 + k);
+                const double* h1 = pinned_;
+                const double* h2 = pinned_ + 2 * region;
+                const double* hn = pinned_ + 4 * region;
+                wn.assign(k, 0.0);
+                for (int c = 0; c < k; ++c) {
+                    if (!cycling[c]) continue;
+                    for (int i = 0; i <= j; ++i) {
+                        const size_t idx = ((size_t)i * k + c) * 2;
+                        h(c, i, j) = cplx(h1[idx], h1[idx + 1]) + cplx(h2[idx], h2[idx + 1]);
+                    }
+                    wn[c] = std::sqrt(hn[c]);
+                }
+            } else {
             dots(j + 1, W_.ptr, coef, wnb);
             for (int c = 0; c < k; ++c)
                 if (cycling[c])
@@ -394,6 +427,7 @@ void BatchedGmres<T>::run(BlockOperatorT<T>& A, BlockPreconditionerT<T>* M, cons
                 column_sums(wn2);
                 for (int c = 0; c < k; ++c)
                     if (reo[c]) wn[c] = std::sqrt(wn2[c]);
+            }
             }
             for (int c = 0; c < k; ++c) {
                 alpha[c] = 0.0;
@@ -541,6 +575,7 @@ struct InnerGmresPreconditioner : BlockPreconditioner {
         params.exact_checks = false;
         params.finalize = false;
         params.x_zero = true;
+        params.cgs2 = true;
     }
     void apply(const double2* V, double2* Z, int k) override {
         convert_block(ctx, V, V32.ptr, n * k);
