@@ -317,7 +317,7 @@ __global__ void lp_reduce_kernel(const float* __restrict__ Ypart, int64_t n, int
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int ZG_SPLIT = 8;
 constexpr int ZG_BM = 128;
-constexpr int ZG_BKC = 8;            // complex columns of A per stage
+constexpr int ZG_BKC = 16;           // complex columns of A per stage
 constexpr int ZG_AROW = ZG_BM + 2;   // LDS row of the transposed A stage (doubles)
 
 template <int NT>
