@@ -167,24 +167,37 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < NT; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t k0 = 0; k0 < n; k0 += G64_BKC) {
+    // Stage s+1 is loaded into registers while stage s is multiplied.
+    constexpr int AQ = (GP_BM * G64_BKC) / BLOCK;
+    double2 pa[AQ], px;
+    auto gload = [&](int64_t k0) {
 #pragma unroll
-        for (int q = 0; q < (GP_BM * G64_BKC) / BLOCK; ++q) {
+        for (int q = 0; q < AQ; ++q) {
             const int e = t + q * BLOCK;
             const int r = e / G64_BKC, cc = e % G64_BKC;
             const int64_t gr = row0 + r, gk = k0 + cc;
-            const double2 v = (gr < n && gk < n) ? Ab[gr * n + gk] : make_double2(0.0, 0.0);
-            As[2 * cc][r] = v.x;
-            As[2 * cc + 1][r] = v.y;
+            pa[q] = (gr < n && gk < n) ? Ab[gr * n + gk] : make_double2(0.0, 0.0);
+        }
+        const int j = t / (8 * NT), c = t % (8 * NT);
+        const int64_t gk = k0 + j;
+        px = (t < G64_BKC * 8 * NT && gk < n && c < kb) ? Xb[gk * kb + c] : make_double2(0.0, 0.0);
+    };
+    gload(0);
+    for (int64_t k0 = 0; k0 < n; k0 += G64_BKC) {
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = t + q * BLOCK;
+            const int r = e / G64_BKC, cc = e % G64_BKC;
+            As[2 * cc][r] = pa[q].x;
+            As[2 * cc + 1][r] = pa[q].y;
         }
         if (t < G64_BKC * 8 * NT) {
             const int j = t / (8 * NT), c = t % (8 * NT);
-            const int64_t gk = k0 + j;
-            const double2 v = (gk < n && c < kb) ? Xb[gk * kb + c] : make_double2(0.0, 0.0);
-            *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = v;
-            *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-v.y, v.x);
+            *reinterpret_cast<double2*>(&Xs[2 * j][2 * c]) = px;
+            *reinterpret_cast<double2*>(&Xs[2 * j + 1][2 * c]) = make_double2(-px.y, px.x);
         }
         __syncthreads();
+        if (k0 + G64_BKC < n) gload(k0 + G64_BKC);
 #pragma unroll
         for (int ks = 0; ks < (2 * G64_BKC) / 4; ++ks) {
             const int kr = 4 * ks + (lane >> 4);
@@ -234,6 +247,9 @@ __global__ __launch_bounds__(BLOCK) void glp_convert_kernel(const int* __restric
 // ---- ragged vector kernels ----
 
 // slot s < nvec: sum conj(V_s) W per column; slot nvec: ||W||^2.
+// Rows per thread per pass of the vector reductions (loads kept in flight together).
+constexpr int RB = 8;
+
 __global__ __launch_bounds__(BLOCK) void rdots_kernel(const VecChunk* __restrict__ chunks,
                                                       const BatchInstance* __restrict__ inst,
                                                       const int* __restrict__ flag, const double2* __restrict__ V,
@@ -251,17 +267,33 @@ __global__ __launch_bounds__(BLOCK) void rdots_kernel(const VecChunk* __restrict
     const int64_t base = I.vec_off + ch.r0 * kb + c;
     double2 s = make_double2(0.0, 0.0);
     if (sub < S) {
-        if (slot < nvec) {
-            const double2* v = V + (int64_t)slot * L;
-            for (int r = sub; r < ch.rows; r += S) {
-                const double2 p = cmulc(v[base + (int64_t)r * kb], W[base + (int64_t)r * kb]);
-                s.x += p.x;
-                s.y += p.y;
+        // RB rows per pass, their loads issued together; sums in row order
+        for (int r0 = sub; r0 < ch.rows; r0 += S * RB) {
+            double2 y[RB];
+#pragma unroll
+            for (int t = 0; t < RB; ++t) {
+                const int r = r0 + S * t;
+                y[t] = r < ch.rows ? W[base + (int64_t)r * kb] : make_double2(0.0, 0.0);
             }
-        } else {
-            for (int r = sub; r < ch.rows; r += S) {
-                const double2 q = W[base + (int64_t)r * kb];
-                s.x += q.x * q.x + q.y * q.y;
+            if (slot < nvec) {
+                const double2* v = V + (int64_t)slot * L;
+                double2 x[RB];
+#pragma unroll
+                for (int t = 0; t < RB; ++t) {
+                    const int r = r0 + S * t;
+                    x[t] = r < ch.rows ? v[base + (int64_t)r * kb] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int t = 0; t < RB; ++t)
+                    if (r0 + S * t < ch.rows) {
+                        const double2 p = cmulc(x[t], y[t]);
+                        s.x += p.x;
+                        s.y += p.y;
+                    }
+            } else {
+#pragma unroll
+                for (int t = 0; t < RB; ++t)
+                    if (r0 + S * t < ch.rows) s.x += y[t].x * y[t].x + y[t].y * y[t].y;
             }
         }
     }
@@ -331,19 +363,40 @@ __global__ __launch_bounds__(BLOCK) void rproject_kernel(const VecChunk* __restr
     const double2* cf = coef + (int64_t)(I.col_off + c) * MV;
     double nr = 0.0;
     if (sub < S) {
-        for (int r = sub; r < ch.rows; r += S) {
-            const int64_t idx = base + (int64_t)r * kb;
-            double sr = 0.0, si = 0.0;
+        // RB rows per pass: per vector their RB loads are issued together; per
+        // element the sum over vectors runs in order before the subtraction
+        for (int r0 = sub; r0 < ch.rows; r0 += S * RB) {
+            double sr[RB], si[RB];
+#pragma unroll
+            for (int t = 0; t < RB; ++t) sr[t] = si[t] = 0.0;
             for (int s = 0; s < nvec; ++s) {
-                const double2 p = cmul(cf[s], V[(int64_t)s * L + idx]);
-                sr += p.x;
-                si += p.y;
+                const double2 cs = cf[s];
+                const double2* v = V + (int64_t)s * L;
+                double2 x[RB];
+#pragma unroll
+                for (int t = 0; t < RB; ++t) {
+                    const int r = r0 + S * t;
+                    x[t] = r < ch.rows ? v[base + (int64_t)r * kb] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int t = 0; t < RB; ++t) {
+                    const double2 p = cmul(cs, x[t]);
+                    sr[t] += p.x;
+                    si[t] += p.y;
+                }
             }
-            double2 w = W[idx];
-            w.x -= sr;
-            w.y -= si;
-            W[idx] = w;
-            nr += w.x * w.x + w.y * w.y;
+#pragma unroll
+            for (int t = 0; t < RB; ++t) {
+                const int r = r0 + S * t;
+                if (r < ch.rows) {
+                    const int64_t idx = base + (int64_t)r * kb;
+                    double2 w = W[idx];
+                    w.x -= sr[t];
+                    w.y -= si[t];
+                    W[idx] = w;
+                    nr += w.x * w.x + w.y * w.y;
+                }
+            }
         }
     }
     sh[threadIdx.x] = nr;

@@ -447,8 +447,10 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 constexpr int BT = 8;
 
 // out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2.
-// One block walks every vector for its rows; its W values stay in registers
-// when the chunk is one pass (S * BT rows), which is the usual case.
+// Block (chunk, group) walks vectors group*BV .. group*BV + BV - 1 for its rows;
+// its W values stay in registers across them when the chunk is one pass
+// (S * BT rows), which is the usual case.
+constexpr int BV = 8;
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, int64_t ldv, int nvec,
                                                       const T* __restrict__ W, int64_t n, int k,
@@ -468,7 +470,8 @@ __global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, i
             w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
         }
     }
-    for (int vec = 0; vec <= nvec; ++vec) {
+    const int vec_end = min(nvec + 1, ((int)blockIdx.y + 1) * BV);
+    for (int vec = (int)blockIdx.y * BV; vec < vec_end; ++vec) {
         double2 s = make_double2(0.0, 0.0);
         if (sub < S) {
             const T* __restrict__ v = V + (int64_t)vec * ldv;
@@ -815,8 +818,8 @@ static void bdots_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T
                     double2* scratch, double2* out) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    hipLaunchKernelGGL(bdots_kernel<T>, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, W, n, k, len,
-                       scratch);
+    hipLaunchKernelGGL(bdots_kernel<T>, dim3((unsigned)chunks, (unsigned)((nvec + BV) / BV)), dim3(BLOCK), 0,
+                       ctx->stream, V, ldv, nvec, W, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
     column_sums(ctx, reinterpret_cast<double*>(scratch), chunks, (int64_t)(nvec + 1) * k * 2,
                 reinterpret_cast<double*>(out));
