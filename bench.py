@@ -272,7 +272,7 @@ def run_c4(args, ctx=None):
             if fmt == "fp64":
                 sts = batch.solve("fp64", restart=100, max_cycles=10, tolerance=1e-8, use_graph=not args.no_graph)
             else:
-                sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
+                sts = batch.solve("fp16_ir", restart=30, max_cycles=30, tolerance=1e-8, inner_tolerance=1e-3,
                                   use_graph=not args.no_graph)
             ctx.synchronize()
             el = time.perf_counter() - t0
