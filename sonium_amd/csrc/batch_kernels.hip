@@ -298,16 +298,8 @@ __global__ __launch_bounds__(BLOCK) void rdots_kernel(const VecChunk* __restrict
         }
     }
     sh[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < kb) {
-        double2 tsum = make_double2(0.0, 0.0);
-        for (int q = 0; q < S; ++q) {
-            const double2 p = sh[q * kb + threadIdx.x];
-            tsum.x += p.x;
-            tsum.y += p.y;
-        }
-        part[((int64_t)blockIdx.x * MV + slot) * MAXCOL + threadIdx.x] = tsum;
-    }
+    column_tree(sh, S, kb, sub, c);
+    if (threadIdx.x < kb) part[((int64_t)blockIdx.x * MV + slot) * MAXCOL + threadIdx.x] = sh[threadIdx.x];
 }
 
 // Column sums of the dot partials (thread per (column, slot)).  pass 0:
@@ -400,12 +392,8 @@ __global__ __launch_bounds__(BLOCK) void rproject_kernel(const VecChunk* __restr
         }
     }
     sh[threadIdx.x] = nr;
-    __syncthreads();
-    if (threadIdx.x < kb) {
-        double tsum = 0.0;
-        for (int q = 0; q < S; ++q) tsum += sh[q * kb + threadIdx.x];
-        part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(tsum, 0.0);
-    }
+    column_tree(sh, S, kb, sub, c);
+    if (threadIdx.x < kb) part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(sh[threadIdx.x], 0.0);
 }
 
 // ||W|| per column from slot 0 of the partials; pass 0 also flags the
@@ -587,12 +575,8 @@ __global__ __launch_bounds__(BLOCK) void rresid_kernel(const VecChunk* __restric
         }
     }
     sh[threadIdx.x] = nr;
-    __syncthreads();
-    if (threadIdx.x < kb) {
-        double tsum = 0.0;
-        for (int q = 0; q < S; ++q) tsum += sh[q * kb + threadIdx.x];
-        part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(tsum, 0.0);
-    }
+    column_tree(sh, S, kb, sub, c);
+    if (threadIdx.x < kb) part[((int64_t)blockIdx.x * MV) * MAXCOL + threadIdx.x] = make_double2(sh[threadIdx.x], 0.0);
 }
 
 // After a residual: ||r|| per column, convergence test ||r|| / ||b|| < tol,

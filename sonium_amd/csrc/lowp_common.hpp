@@ -89,3 +89,21 @@ __device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
     else
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
+
+// Column-batched reductions over n x k row-major blocks: the sum of the S
+// partials sh[q * k + c] (q < S) of every column c lands in sh[c].  It is a
+// pairwise tree of fixed shape (width S, then ceil(width / 2), ...), so the
+// order is fixed and no thread adds more than one value per level.  The caller
+// has written sh; every thread of the block calls this.
+__device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double add2(double a, double b) { return a + b; }
+template <typename V>
+__device__ __forceinline__ void column_tree(V* sh, int S, int k, int sub, int c) {
+    for (int width = S; width > 1;) {
+        const int half = (width + 1) / 2;
+        __syncthreads();
+        if (sub < width - half) sh[sub * k + c] = add2(sh[sub * k + c], sh[(sub + half) * k + c]);
+        width = half;
+    }
+    __syncthreads();
+}

@@ -446,6 +446,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 // flight (one per row it owns) instead of one dependent load at a time.
 constexpr int BT = 8;
 
+
 // out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2.
 // Block (chunk, group) walks vectors group*BV .. group*BV + BV - 1 for its rows;
 // its W values stay in registers across them when the chunk is one pass
@@ -506,16 +507,8 @@ __global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, i
             }
         }
         sh[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x < k) {
-            double2 tsum = make_double2(0.0, 0.0);
-            for (int q = 0; q < S; ++q) {
-                const double2 p = sh[q * k + threadIdx.x];
-                tsum.x += p.x;
-                tsum.y += p.y;
-            }
-            out[((int64_t)blockIdx.x * (nvec + 1) + vec) * k + threadIdx.x] = tsum;
-        }
+        column_tree(sh, S, k, sub, c);
+        if (threadIdx.x < k) out[((int64_t)blockIdx.x * (nvec + 1) + vec) * k + threadIdx.x] = sh[threadIdx.x];
         __syncthreads();  // sh is rewritten for the next vector
     }
 }
@@ -571,12 +564,8 @@ __global__ __launch_bounds__(BLOCK) void bproject_kernel(const T* __restrict__ V
         }
     }
     sh[threadIdx.x] = nr;
-    __syncthreads();
-    if (threadIdx.x < k) {
-        double tsum = 0.0;
-        for (int q = 0; q < S; ++q) tsum += sh[q * k + threadIdx.x];
-        out[(int64_t)blockIdx.x * k + threadIdx.x] = tsum;
-    }
+    column_tree(sh, S, k, sub, c);
+    if (threadIdx.x < k) out[(int64_t)blockIdx.x * k + threadIdx.x] = sh[threadIdx.x];
 }
 
 // R = B - R (or only the norm when B == nullptr); norm partials per column
@@ -604,12 +593,8 @@ __global__ __launch_bounds__(BLOCK) void bresidual_kernel(const T* __restrict__ 
         }
     }
     sh[threadIdx.x] = nr;
-    __syncthreads();
-    if (threadIdx.x < k) {
-        double tsum = 0.0;
-        for (int q = 0; q < S; ++q) tsum += sh[q * k + threadIdx.x];
-        out[(int64_t)blockIdx.x * k + threadIdx.x] = tsum;
-    }
+    column_tree(sh, S, k, sub, c);
+    if (threadIdx.x < k) out[(int64_t)blockIdx.x * k + threadIdx.x] = sh[threadIdx.x];
 }
 
 // Column sums of chunk partials in two coalesced passes with a fixed order:
@@ -627,12 +612,13 @@ __global__ void bsum_groups_kernel(const double* __restrict__ part, int chunks, 
     tmp[(int64_t)blockIdx.y * nout + o] = s;
 }
 
-__global__ void bsum_final_kernel(const double* __restrict__ tmp, int64_t nout, double* __restrict__ out) {
+// out[o] = sum over g < groups of tmp[g * nout + o], in g order
+__global__ void bsum_final_kernel(const double* __restrict__ tmp, int64_t nout, int groups, double* __restrict__ out) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout) return;
     double s = 0.0;
-#pragma unroll
-    for (int g = 0; g < BSUM_GROUPS; ++g) s += tmp[(int64_t)g * nout + o];
+#pragma unroll 8
+    for (int g = 0; g < groups; ++g) s += tmp[(int64_t)g * nout + o];
     out[o] = s;
 }
 
@@ -800,11 +786,19 @@ static int batched_chunks(int64_t n, int k, int64_t* len) {
 // out[o] = sum over the chunks of part[q * nout + o]; the group sums go right
 // after the partials in the same scratch buffer.
 static void column_sums(sbh_context* ctx, double* part, int chunks, int64_t nout, double* out) {
+    if (chunks <= BSUM_GROUPS) {
+        // one pass in chunk order: the same sums as the two passes, whose groups then hold one chunk each
+        hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, part, nout, chunks,
+                           out);
+        SBH_HIP(hipGetLastError());
+        return;
+    }
     double* tmp = part + (int64_t)chunks * nout;
     hipLaunchKernelGGL(bsum_groups_kernel, dim3(nblocks(nout), BSUM_GROUPS), dim3(BLOCK), 0, ctx->stream, part,
                        chunks, nout, tmp);
     SBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, tmp, nout, out);
+    hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, tmp, nout, BSUM_GROUPS,
+                       out);
     SBH_HIP(hipGetLastError());
 }
 
