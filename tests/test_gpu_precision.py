@@ -247,3 +247,23 @@ def test_lowp_ragged_sizes(ctx, mesh_args, fmt):
         X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
         Y = lp.apply(X if k > 1 else X[:, 0]).reshape(n, k)
         assert rel_fro(Y, D @ olp.quantized_columns(X, fmt)) < 5e-6, k
+
+
+def test_block_gmres_initial_guess_matches_single(c1_block):
+    """With X0 given, the first residual is formed from A X0 (no zero-start
+    shortcut); every column then follows gmres_with_guess."""
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    B = c1_block["B"][:, :4]
+    A = c1_block["A"]
+    X0 = 0.9 * np.linalg.solve(A, B) + 0.01
+    cfg = sa.GmresConfig(max_iterations=100, restart=50, tolerance=1e-8)
+    X, sols = sa.gmres_block(op, B, cfg, X0=X0)
+    for c in range(B.shape[1]):
+        single = sa.gmres_with_guess(op, B[:, c], X0[:, c], cfg)
+        assert sols[c].converged and sols[c].residual <= 1e-8
+        assert sols[c].iterations == single.iterations
+        assert rel_l2(X[:, c], single.x) < 1e-10
+    # an exact guess converges without an Arnoldi step
+    Xe = np.linalg.solve(A, B)
+    _, sols = sa.gmres_block(op, B, cfg, X0=Xe)
+    assert all(s.converged and s.iterations == 0 for s in sols)
