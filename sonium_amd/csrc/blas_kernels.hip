@@ -352,6 +352,26 @@ int dots_and_norm_partials(sbh_context* ctx, const double2* V, int64_t ldv, int 
     return chunks;
 }
 
+// s[v] = sum over c < chunks of part[v * chunks + c], added in c order (the
+// order in which the host adds the same partials).
+__global__ void chunk_sums_kernel(const double2* __restrict__ part, int nv, int chunks, double2* __restrict__ s) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        const double2 p = part[(int64_t)v * chunks + c];
+        sr += p.x;
+        si += p.y;
+    }
+    s[v] = make_double2(sr, si);
+}
+
+void sum_chunk_partials(sbh_context* ctx, const double2* part, int nv, int chunks, double2* s) {
+    hipLaunchKernelGGL(chunk_sums_kernel, dim3((unsigned)((nv + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                       part, nv, chunks, s);
+    SBH_HIP(hipGetLastError());
+}
+
 int project_and_norm(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* c,
                      double2* w, int64_t n, double* norm_out) {
     unsigned nb = blocks_for(n);

@@ -81,25 +81,6 @@ struct Krylov {
         int nb = residual_and_norm(ctx, b, r, n, reinterpret_cast<double*>(part.ptr));
         return std::sqrt(fetch_sum(nb));
     }
-    // coefficients <v_i, w> (i <= j) and ||w||
-    double dots(const double2* V, int nvec, const double2* w, std::vector<cplx>& out) {
-        int chunks = dots_and_norm_partials(ctx, V, n, nvec, w, n, part.ptr);
-        size_t cnt = (size_t)(nvec + 1) * chunks;
-        SBH_HIP(hipMemcpyAsync(host, part.ptr, sizeof(double2) * cnt, hipMemcpyDeviceToHost, ctx->stream));
-        SBH_HIP(hipStreamSynchronize(ctx->stream));
-        out.assign(nvec, cplx(0.0, 0.0));
-        for (int v = 0; v < nvec; ++v) {
-            double sr = 0.0, si = 0.0;
-            for (int c = 0; c < chunks; ++c) {
-                sr += host[2 * ((size_t)v * chunks + c)];
-                si += host[2 * ((size_t)v * chunks + c) + 1];
-            }
-            out[v] = cplx(sr, si);
-        }
-        double nn = 0.0;
-        for (int c = 0; c < chunks; ++c) nn += host[2 * ((size_t)nvec * chunks + c)];
-        return std::sqrt(nn);
-    }
     void upload_coeffs(const std::vector<cplx>& c, int count) {
         for (int i = 0; i < count; ++i) {
             host[2 * i] = c[i].real();
@@ -107,10 +88,24 @@ struct Krylov {
         }
         SBH_HIP(hipMemcpyAsync(coeff.ptr, host, sizeof(double2) * count, hipMemcpyHostToDevice, ctx->stream));
     }
-    // w -= sum c_i v_i ; returns ||w||
-    double project(const double2* V, int nvec, double2* w) {
-        int nb = project_and_norm(ctx, V, n, nvec, coeff.ptr, w, n, reinterpret_cast<double*>(part.ptr));
-        return std::sqrt(fetch_sum(nb));
+    // One classical Gram-Schmidt pass with one host read: the coefficients
+    // <v_i, w> are summed on the device in chunk order (the order the host
+    // used), w -= sum c_i v_i runs with them, and coefficients, ||w|| before
+    // and the norm partials after come back together.  Returns ||w|| after.
+    double dots_project(const double2* V, int nvec, double2* w, std::vector<cplx>& out, double* before) {
+        const int chunks = dots_and_norm_partials(ctx, V, n, nvec, w, n, part.ptr);
+        sum_chunk_partials(ctx, part.ptr, nvec + 1, chunks, coeff.ptr);
+        const int nb = project_and_norm(ctx, V, n, nvec, coeff.ptr, w, n, reinterpret_cast<double*>(part.ptr));
+        SBH_HIP(hipMemcpyAsync(host, coeff.ptr, sizeof(double2) * (nvec + 1), hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipMemcpyAsync(host + 2 * (nvec + 1), part.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        out.assign(nvec, cplx(0.0, 0.0));
+        for (int v = 0; v < nvec; ++v) out[v] = cplx(host[2 * v], host[2 * v + 1]);
+        *before = std::sqrt(host[2 * nvec]);
+        double s = 0.0;
+        for (int i = 0; i < nb; ++i) s += host[2 * (nvec + 1) + i];
+        return std::sqrt(s);
     }
 };
 
@@ -228,16 +223,14 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
                 K.apply(vec(j), opb.ptr);
                 K.precond(opb.ptr, w.ptr);
             }
-            double wnb = K.dots(V.ptr, j + 1, w.ptr, coeffs);
+            double wnb = 0.0;
+            double wn = K.dots_project(V.ptr, j + 1, w.ptr, coeffs, &wnb);
             for (int i = 0; i <= j; ++i) h(i, j) = coeffs[i];
-            K.upload_coeffs(coeffs, j + 1);
-            double wn = K.project(V.ptr, j + 1, w.ptr);
             if (wn < 0.707 * wnb) {
                 reorth += 1;
-                K.dots(V.ptr, j + 1, w.ptr, re);
+                double unused = 0.0;
+                wn = K.dots_project(V.ptr, j + 1, w.ptr, re, &unused);
                 for (int i = 0; i <= j; ++i) h(i, j) += re[i];
-                K.upload_coeffs(re, j + 1);
-                wn = K.project(V.ptr, j + 1, w.ptr);
             }
             h(j + 1, j) = cplx(wn, 0.0);
             if (wn < 1e-20) {

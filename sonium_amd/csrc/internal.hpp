@@ -374,6 +374,8 @@ void zgemm_block(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, con
 // out holds (nvec + 1) x chunks complex partials (the last row is the norm).
 int dots_and_norm_partials(sbh_context* ctx, const double2* V, int64_t ldv, int nvec,
                            const double2* w, int64_t n, double2* out);
+// s[v] = sum of the chunk partials of slot v (v < nv), in chunk order, on the device.
+void sum_chunk_partials(sbh_context* ctx, const double2* part, int nv, int chunks, double2* s);
 // w -= sum_i c_i v_i; writes ||w||^2 partials into norm_out (chunks).
 int project_and_norm(sbh_context* ctx, const double2* V, int64_t ldv, int nvec, const double2* c,
                      double2* w, int64_t n, double* norm_out);
