@@ -512,30 +512,57 @@ __device__ inline void deriv_free_terms(const PanelBc& bc, const DerivParams& P,
 // Far kernel.  MODE 0: every field panel has a velocity BC without
 // admittance (rigid or vibrating body), so only H and E are needed and
 // A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
+// Per-panel constants of the TR13 points y_q = v0 + s_q e1 + t_q e2: the Gram
+// entries of the two edges.  Callers compute them once per field panel.
+struct FarPoints {
+    double e11, e12x2, e22;  // e1.e1, 2 e1.e2, e2.e2
+};
+
+__device__ __forceinline__ FarPoints far_points(const FarPanel& P) {
+    FarPoints f;
+    f.e11 = P.e1[0] * P.e1[0] + P.e1[1] * P.e1[1] + P.e1[2] * P.e1[2];
+    f.e12x2 = 2.0 * (P.e1[0] * P.e2[0] + P.e1[1] * P.e2[1] + P.e1[2] * P.e2[2]);
+    f.e22 = P.e2[0] * P.e2[0] + P.e2[1] * P.e2[1] + P.e2[2] * P.e2[2];
+    return f;
+}
+
 // 13-point sums of one (collocation point, field panel) pair.  MODE 0 needs only
-// H and E; MODE 1 also G and Ht.
+// H and E; MODE 1 also G and Ht.  With rel = v0 - x:
+//   |y_q - x|^2 = |rel|^2 + s_q (2 rel.e1 + s_q e1.e1 + 2 t_q e1.e2) + t_q (2 rel.e2 + t_q e2.e2)
+//   (y_q - x).n_y = rel.n_y                        (e1 and e2 lie in the panel plane)
+//   (y_q - x).n_x = rel.n_x + s_q e1.n_x + t_q e2.n_x
+// so the per-point geometry is five FMAs for the distance and one product for
+// each normal projection, after six dot products per pair.  Off the diagonal
+// (which the self-panel rule overwrites) the point-to-point distance is at
+// least a fraction of the panel size, so the expansion loses no accuracy.
 template <int MODE>
-__device__ __forceinline__ void far_sums(const FarPanel& P, const Source& src, double k, double k2, double& hr,
-                                         double& hi, double& er, double& ei, double& gr, double& gi, double& tr,
-                                         double& ti) {
+__device__ __forceinline__ void far_sums(const FarPanel& P, const FarPoints& F, const Source& src, double k,
+                                         double k2, double& hr, double& hi, double& er, double& ei, double& gr,
+                                         double& gi, double& tr, double& ti) {
     const double rel0 = P.v0[0] - src.c[0];
     const double rel1 = P.v0[1] - src.c[1];
     const double rel2 = P.v0[2] - src.c[2];
     const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
+    const double a = rel0 * rel0 + rel1 * rel1 + rel2 * rel2;
+    const double b1 = 2.0 * (rel0 * P.e1[0] + rel1 * P.e1[1] + rel2 * P.e1[2]);
+    const double b2 = 2.0 * (rel0 * P.e2[0] + rel1 * P.e2[1] + rel2 * P.e2[2]);
+    const double pr = rel0 * P.nf[0] + rel1 * P.nf[1] + rel2 * P.nf[2];
+    const double g0 = -(rel0 * src.n[0] + rel1 * src.n[1] + rel2 * src.n[2]);
+    const double g1 = -(P.e1[0] * src.n[0] + P.e1[1] * src.n[1] + P.e1[2] * src.n[2]);
+    const double g2 = -(P.e2[0] * src.n[0] + P.e2[1] * src.n[1] + P.e2[2] * src.n[2]);
     hr = hi = er = ei = gr = gi = tr = ti = 0.0;
     // TR13 constants are uniform: scalar loads from constant memory.
 #pragma unroll 2
     for (int q = 0; q < 13; ++q) {
         const double sq = kTrS[q], tq = kTrT[q];
-        const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
-        const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
-        const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
+        const double u1 = __builtin_fma(tq, F.e12x2, __builtin_fma(sq, F.e11, b1));
+        const double u2 = __builtin_fma(tq, F.e22, b2);
         // The diagonal is overwritten by the self-panel rule; keep it finite.
-        const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);
+        const double d2 = fmax(__builtin_fma(sq, u1, __builtin_fma(tq, u2, a)), 1e-300);
         const double inv = rsqrt_nr(d2);
         const double dis = d2 * inv;
-        const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;     // r^.n_y
-        const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;  // -r^.n_x
+        const double pn = pr * inv;                                                  // r^.n_y
+        const double qn = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0)) * inv;    // -r^.n_x
         double sn, cs;
         sincos_red(k * dis, &sn, &cs);
         const double re2 = (kTrW[q] * P.wj) * inv;
@@ -581,6 +608,7 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
     const bool active = j < n;
     const int64_t jj = active ? j : n - 1;
     const FarPanel P = panels[jj];
+    const FarPoints F = far_points(P);
     PanelBc bc;
     if (MODE == 1) bc = bcs[jj];
     const double k2 = k * k;
@@ -589,7 +617,7 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
     for (int64_t i = i0; i < i1; ++i) {
         const Source src = sources[i];
         double hr, hi, er, ei, gr, gi, tr, ti;
-        far_sums<MODE>(P, src, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
+        far_sums<MODE>(P, F, src, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
         double2 out;
         if (MODE == 0) {
             out = velocity_coefficient(hr, hi, er, ei, tau_r, beta);
@@ -716,25 +744,33 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
 // MODE 0 (velocity panels without admittance) needs E, dH, dE; MODE 1 also G,
 // Ht, dG, dHt.
 template <int MODE>
-__device__ __forceinline__ void far_sums_d(const FarPanel& P, const Source& src, double k, double k2, Acc& b,
-                                           Acc& d) {
+// Same geometry as far_sums (distances from the edge Gram entries, normal
+// projections from per-pair dot products).
+__device__ __forceinline__ void far_sums_d(const FarPanel& P, const FarPoints& F, const Source& src, double k,
+                                           double k2, Acc& b, Acc& d) {
     const double rel0 = P.v0[0] - src.c[0];
     const double rel1 = P.v0[1] - src.c[1];
     const double rel2 = P.v0[2] - src.c[2];
     const double nxny = src.n[0] * P.nf[0] + src.n[1] * P.nf[1] + src.n[2] * P.nf[2];
+    const double a = rel0 * rel0 + rel1 * rel1 + rel2 * rel2;
+    const double b1 = 2.0 * (rel0 * P.e1[0] + rel1 * P.e1[1] + rel2 * P.e1[2]);
+    const double b2 = 2.0 * (rel0 * P.e2[0] + rel1 * P.e2[1] + rel2 * P.e2[2]);
+    const double pr = rel0 * P.nf[0] + rel1 * P.nf[1] + rel2 * P.nf[2];
+    const double g0 = -(rel0 * src.n[0] + rel1 * src.n[1] + rel2 * src.n[2]);
+    const double g1 = -(P.e1[0] * src.n[0] + P.e1[1] * src.n[1] + P.e1[2] * src.n[2]);
+    const double g2 = -(P.e2[0] * src.n[0] + P.e2[1] * src.n[1] + P.e2[2] * src.n[2]);
     b = acc_zero();
     d = acc_zero();
 #pragma unroll 2
     for (int q = 0; q < 13; ++q) {
         const double sq = kTrS[q], tq = kTrT[q];
-        const double dx = __builtin_fma(sq, P.e1[0], __builtin_fma(tq, P.e2[0], rel0));
-        const double dy = __builtin_fma(sq, P.e1[1], __builtin_fma(tq, P.e2[1], rel1));
-        const double dz = __builtin_fma(sq, P.e1[2], __builtin_fma(tq, P.e2[2], rel2));
-        const double d2 = fmax(dx * dx + dy * dy + dz * dz, 1e-300);  // diagonal: overwritten later
+        const double u1 = __builtin_fma(tq, F.e12x2, __builtin_fma(sq, F.e11, b1));
+        const double u2 = __builtin_fma(tq, F.e22, b2);
+        const double d2 = fmax(__builtin_fma(sq, u1, __builtin_fma(tq, u2, a)), 1e-300);  // diagonal: overwritten later
         const double inv = rsqrt_nr(d2);
         const double dis = d2 * inv;
-        const double pn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;
-        const double qn = -(dx * src.n[0] + dy * src.n[1] + dz * src.n[2]) * inv;
+        const double pn = pr * inv;
+        const double qn = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0)) * inv;
         double sn, cs;
         sincos_red(k * dis, &sn, &cs);
         const double re2 = (kTrW[q] * P.wj) * inv;
@@ -780,6 +816,7 @@ __global__ __launch_bounds__(256) void far_deriv_kernel(const FarPanel* __restri
     const bool active = j < n;
     const int64_t jj = active ? j : n - 1;
     const FarPanel F = panels[jj];
+    const FarPoints FP = far_points(F);
     const PanelBc bc = MODE == 1 ? bcs[jj] : rigid_bc();
     const double k2 = P.k * P.k;
     const int64_t i0 = (int64_t)blockIdx.y * rows_per_block;
@@ -787,7 +824,7 @@ __global__ __launch_bounds__(256) void far_deriv_kernel(const FarPanel* __restri
     for (int64_t i = i0; i < i1; ++i) {
         const Source src = sources[i];
         Acc b, d;
-        far_sums_d<MODE>(F, src, P.k, k2, b, d);
+        far_sums_d<MODE>(F, FP, src, P.k, k2, b, d);
         const double2 out = deriv_coefficient(b, d, bc, P);
         if (active) A[i * lda + j] = out;
     }
@@ -1034,9 +1071,10 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
                 const int64_t j = jc + threadIdx.x;
                 const bool active = j < n;
                 const FarPanel P = far[active ? j : n - 1];
+                const FarPoints F = far_points(P);
                 for (int64_t i = r0; i < r1; ++i) {
                     double hr, hi, er, ei, gr, gi, tr, ti;
-                    far_sums<0>(P, src[i], k, k2, hr, hi, er, ei, gr, gi, tr, ti);
+                    far_sums<0>(P, F, src[i], k, k2, hr, hi, er, ei, gr, gi, tr, ti);
                     if (active) A[i * n + j] = velocity_coefficient(hr, hi, er, ei, I.tau, beta);
                 }
             }
@@ -1085,10 +1123,11 @@ __global__ void reset_counters_kernel(unsigned* counter) { counter[threadIdx.x] 
 constexpr int EXACT_ROWS = 4;
 
 template <int MODE>
-__device__ __forceinline__ double2 far_entry(const FarPanel& P, const Source& S, const PanelBc& bc, double k,
-                                             double k2, double tau_r, double2 beta, double2 iwr) {
+__device__ __forceinline__ double2 far_entry(const FarPanel& P, const FarPoints& F, const Source& S,
+                                             const PanelBc& bc, double k, double k2, double tau_r, double2 beta,
+                                             double2 iwr) {
     double hr, hi, er, ei, gr, gi, tr, ti;
-    far_sums<MODE>(P, S, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
+    far_sums<MODE>(P, F, S, k, k2, hr, hi, er, ei, gr, gi, tr, ti);
     if (MODE == 0) return velocity_coefficient(hr, hi, er, ei, tau_r, beta);
     Acc a;
     a.g = make_double2(gr, gi);
@@ -1117,13 +1156,14 @@ __global__ __launch_bounds__(256) void exact_far_kernel(const FarPanel* __restri
         const bool active = j < n;
         const int64_t jj = active ? j : n - 1;
         const FarPanel P = panels[jj];
+        const FarPoints F = far_points(P);
         PanelBc bc;
         if (MODE == 1) bc = bcs[jj];
         const double2 xj = x[jj];
 #pragma unroll
         for (int r = 0; r < EXACT_ROWS; ++r) {
             const int64_t i = min(i0 + r, n - 1);  // rows past n are computed and dropped
-            const double2 v = far_entry<MODE>(P, sources[i], bc, k, k2, tau_r, beta, iwr);
+            const double2 v = far_entry<MODE>(P, F, sources[i], bc, k, k2, tau_r, beta, iwr);
             if (active && j != i) {
                 acc[r].x += v.x * xj.x - v.y * xj.y;
                 acc[r].y += v.x * xj.y + v.y * xj.x;
@@ -1162,7 +1202,8 @@ __global__ __launch_bounds__(256) void exact_near_kernel(const NearPair* __restr
     const Source S = sources[np.i];
     const double2 v = near_entry(np, leaves, G, S, bc, k, wavruim, tau_r, beta, iwr, lane);
     if (lane == 0) {
-        const double2 f = far_entry<MODE>(panels[np.j], S, bc, k, k * k, tau_r, beta, iwr);
+        const FarPanel P = panels[np.j];
+        const double2 f = far_entry<MODE>(P, far_points(P), S, bc, k, k * k, tau_r, beta, iwr);
         out[p] = cmul(csub(v, f), x[np.j]);
     }
 }
@@ -1235,6 +1276,11 @@ __global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__
         const double2 pj = p[active ? j : n - 1];
         const double2 vj = HAS_V ? v[active ? j : n - 1] : make_double2(0.0, 0.0);
         const double scale = (active && P.wj > 0.0) ? P.wj : 0.0;  // w J / (4 pi), times 0.5 for the rule
+        // (y_q - x).n_y = (v0 - x).n_y for every point y_q of the flat panel
+        double pr[FIELD_ROWS];
+#pragma unroll
+        for (int r = 0; r < FIELD_ROWS; ++r)
+            pr[r] = (P.v0[0] - xs[r][0]) * P.nf[0] + (P.v0[1] - xs[r][1]) * P.nf[1] + (P.v0[2] - xs[r][2]) * P.nf[2];
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
             const double sq = kTr7S[q], tq = kTr7T[q];
@@ -1252,7 +1298,7 @@ __global__ __launch_bounds__(256) void field_kernel(const FarPanel* __restrict__
                 sincos_red(k * (d2 * inv), &sn, &cs);
                 const double wt = d2 > 1e-30 ? wq * inv : 0.0;
                 const double zgr = cs * wt, zgi = sn * wt;
-                const double drdn = (dx * P.nf[0] + dy * P.nf[1] + dz * P.nf[2]) * inv;
+                const double drdn = pr[r] * inv;
                 // p_j * zg * (-1/r + ik) * drdn
                 const double ar = (-zgr * inv - zgi * k) * drdn, ai = (zgr * k - zgi * inv) * drdn;
                 acc[r].x = __builtin_fma(pj.x, ar, __builtin_fma(-pj.y, ai, acc[r].x));
