@@ -178,41 +178,48 @@ __device__ __forceinline__ void wave_sum_acc(Acc& a) {
 }
 
 // ---------------------------------------------------------------------------
-// Reference-order evaluations (no contraction): one quadrature point of the
-// regular kernel, regular.rs:214-281.  DERIV also accumulates the analytic
-// wavenumber derivative of every channel into dr (regular.rs:229-278), from
-// the same point, as the reference's shared quadrature loop does.
+// One quadrature point of the regular kernel, regular.rs:214-281.  DERIV also
+// accumulates the analytic wavenumber derivative of every channel into dr
+// (regular.rs:229-278), from the same point, as the reference's shared
+// quadrature loop does.  The point evaluation uses 1/r from rsqrt_nr and FMA
+// contraction (the reference divides by r nine times); the quadrature rules,
+// subdivisions and channel formulas are the reference's.  The surrounding
+// code keeps the reference's operation order (no contraction).
 #pragma clang fp contract(off)
 
 template <bool DERIV>
 __device__ inline void ref_point_t(const double pos[3], double wga, const double src[3],
                                    const double srcn[3], const double nf[3], double wavruim, double k2,
                                    double k, double hf, Acc& r, Acc& dr) {
-    double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
-    double dis = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    if (!(dis > 1e-15)) return;  // normalize() returns 0, then `dis < 1e-15` skips
-    double u0 = d0 / dis, u1 = d1 / dis, u2 = d2 / dis;
-    double re1 = wavruim * dis;
-    double re2 = wga / (FOUR_PI * dis);
+#pragma clang fp contract(fast)
+    const double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
+    const double dsq = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(dsq > 1e-30)) return;  // |d| <= 1e-15: normalize() returns 0, then `dis < 1e-15` skips
+    const double inv = rsqrt_nr(dsq);
+    const double dis = dsq * inv;
+    const double u0 = d0 * inv, u1 = d1 * inv, u2 = d2 * inv;
+    const double re1 = wavruim * dis;
+    const double re2 = (wga * (1.0 / FOUR_PI)) * inv;
     double s, c;
     sincos_red(re1, &s, &c);
-    double2 zg = make_double2(c * re2, s * re2);
-    double2 z1 = make_double2(-1.0 / dis, wavruim);
-    double2 zb = cmul(zg, z1);
-    double re1_h = u0 * nf[0] + u1 * nf[1] + u2 * nf[2];
-    double re2_h = -(u0 * srcn[0] + u1 * srcn[1] + u2 * srcn[2]);
-    double rq = re1_h * re2_h;
-    double nxny = srcn[0] * nf[0] + srcn[1] * nf[1] + srcn[2] * nf[2];
-    double dq = dis * dis;
-    double2 zef = make_double2((3.0 / dq - k2) * rq + nxny / dq, -wavruim / dis * (3.0 * rq + nxny));
+    const double2 zg = make_double2(c * re2, s * re2);
+    const double2 z1 = make_double2(-inv, wavruim);
+    const double2 zb = cmul(zg, z1);
+    const double re1_h = u0 * nf[0] + u1 * nf[1] + u2 * nf[2];
+    const double re2_h = -(u0 * srcn[0] + u1 * srcn[1] + u2 * srcn[2]);
+    const double rq = re1_h * re2_h;
+    const double nxny = srcn[0] * nf[0] + srcn[1] * nf[1] + srcn[2] * nf[2];
+    const double inv2 = inv * inv;
+    const double t3 = 3.0 * rq + nxny;
+    const double2 zef = make_double2((3.0 * inv2 - k2) * rq + nxny * inv2, -wavruim * inv * t3);
     r.g = cadd(r.g, zg);
     r.h = cadd(r.h, cscale(zb, re1_h));
     r.ht = cadd(r.ht, cscale(zb, re2_h));
     r.e = cadd(r.e, cmul(zg, zef));
     if (DERIV) {
-        double2 dzg = cmul(zg, make_double2(0.0, hf * dis));
-        double2 dzb = cadd(cmul(dzg, z1), cmul(zg, make_double2(0.0, hf)));
-        double2 dzef = make_double2(-2.0 * k * rq, -hf / dis * (3.0 * rq + nxny));
+        const double2 dzg = cmul(zg, make_double2(0.0, hf * dis));
+        const double2 dzb = cadd(cmul(dzg, z1), cmul(zg, make_double2(0.0, hf)));
+        const double2 dzef = make_double2(-2.0 * k * rq, -hf * inv * t3);
         dr.g = cadd(dr.g, dzg);
         dr.h = cadd(dr.h, cscale(dzb, re1_h));
         dr.ht = cadd(dr.ht, cscale(dzb, re2_h));
