@@ -20,6 +20,7 @@ sample of rows on this host's cores).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -663,6 +664,15 @@ def main():
                    "autotuned": c4["autotune"]["best"]},
         }
 
+    # measured ceilings beside the published peaks: a streaming read of a
+    # matrix-sized buffer and independent FP64 FMAs on every CU
+    gbs, tfl = ctypes.c_double(), ctypes.c_double()
+    sa._check(sa._lib().sbh_debug_hbm_read_rate(ctx.handle, 16 * n * n, 5, ctypes.byref(gbs)))
+    sa._check(sa._lib().sbh_debug_fp64_fma_rate(ctx.handle, ctypes.byref(tfl)))
+    ceilings = {"hbm_stream_read_gbs": gbs.value, "fp64_fma_tflops": tfl.value,
+                "matvec_frac_of_stream_read": kernels["matvec"]["gbs"] / gbs.value,
+                "far_kernel_frac_of_fma_rate": kernels["assemble_far"]["tflops"] / tfl.value}
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(os.cpu_count() or 1, 16))
@@ -689,6 +699,7 @@ def main():
         "routes": st,
         "kernels": kernels,
         "roofline": roofline,
+        "measured_ceilings": ceilings,
         "chosen_form": chosen,
         "exact_operator": exact_info,
         "field_evaluation": field_info,

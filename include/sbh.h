@@ -608,6 +608,11 @@ sbh_status sbh_debug_device_rsqrt(sbh_context* ctx, const double* x, int64_t n, 
 /* Test hook: `reps` launches of a plain device copy of `bytes` bytes (reads
  * `bytes`, writes `bytes`), used to calibrate HBM byte counters. */
 sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps);
+/* Roofline ceiling probes (no counterpart in the reference): the measured
+ * rate of a streaming read of `bytes` bytes (GB/s, average of `reps`
+ * launches) and of independent FP64 FMAs on every CU (TFLOP/s, FMA = 2). */
+sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, double* gbs);
+sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops);
 
 #ifdef __cplusplus
 }

@@ -1157,6 +1157,26 @@ sbh_status sbh_debug_subelements(const double* source_point, const double* eleme
     });
 }
 
+sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, double* gbs) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && gbs && bytes >= 16 && reps >= 1, "invalid probe arguments");
+        use_device(ctx);
+        DeviceArray<double2> a(bytes / 16);
+        DeviceArray<unsigned> sink(1);
+        SBH_HIP(hipMemsetAsync(a.ptr, 0, a.bytes(), ctx->stream));
+        *gbs = probe_stream_read(ctx, a.ptr, (int64_t)a.bytes(), reps, sink.ptr);
+    });
+}
+
+sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops) {
+    return guard([&] {
+        SBH_REQUIRE(ctx && tflops, "invalid probe arguments");
+        use_device(ctx);
+        DeviceArray<double> sink(1);
+        *tflops = probe_fma_rate(ctx, sink.ptr);
+    });
+}
+
 sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps) {
     return guard([&] {
         SBH_REQUIRE(ctx && bytes >= 16 && reps >= 1, "invalid copy arguments");

@@ -16,6 +16,8 @@
 //  Every reduction writes per-block partials that the host sums in index
 //  order, so results are deterministic.
 
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace sbh {
@@ -417,6 +419,107 @@ int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out) {
 void diag_scale(sbh_context* ctx, const double2* d, const double2* x, double2* y, int64_t n) {
     hipLaunchKernelGGL(diag_scale_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, ctx->stream, d, x, y, n);
     SBH_HIP(hipGetLastError());
+}
+
+// Ceiling probes for the rooflines.  stream_read_kernel reads `count` 16-byte
+// words: workgroup b owns the contiguous range of tiles [b*full/grid,
+// (b+1)*full/grid), a tile being STREAM_LOADS x 256 words (32 KB) read with
+// STREAM_LOADS non-temporal loads in flight per thread; the words past the last
+// full tile are read grid-stride.  The XOR of the data is stored only if it
+// equals `magic`, so the loads stay.
+typedef unsigned probe_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int STREAM_LOADS = 8;
+
+__global__ __launch_bounds__(256) void stream_read_kernel(const probe_u32x4* __restrict__ x, int64_t count,
+                                                         unsigned magic, unsigned* __restrict__ sink) {
+    constexpr int64_t TILE = (int64_t)STREAM_LOADS * 256;
+    const int64_t full = count / TILE;
+    const int64_t t0 = full * blockIdx.x / gridDim.x, t1 = full * (blockIdx.x + 1) / gridDim.x;
+    probe_u32x4 acc = {0u, 0u, 0u, 0u};
+    for (int64_t t = t0; t < t1; ++t) {
+        const probe_u32x4* p = x + t * TILE + threadIdx.x;
+        probe_u32x4 v[STREAM_LOADS];
+#pragma unroll
+        for (int j = 0; j < STREAM_LOADS; ++j) v[j] = __builtin_nontemporal_load(p + j * 256);
+#pragma unroll
+        for (int j = 0; j < STREAM_LOADS; ++j) acc ^= v[j];
+    }
+    for (int64_t i = full * TILE + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc ^= x[i];
+    const unsigned t = acc.x ^ acc.y ^ acc.z ^ acc.w;
+    if (t == magic) sink[0] = t;
+}
+
+// FMA_CHAINS independent FP64 FMA chains of `iters` steps per thread; the sum
+// is stored only if it equals `magic`.
+constexpr int FMA_CHAINS = 16;
+__global__ __launch_bounds__(256) void fma_rate_kernel(int64_t iters, double b, double c, double magic,
+                                                      double* __restrict__ sink) {
+    double a[FMA_CHAINS];
+#pragma unroll
+    for (int u = 0; u < FMA_CHAINS; ++u) a[u] = (double)(threadIdx.x + u);
+    for (int64_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < FMA_CHAINS; ++u) a[u] = __builtin_fma(a[u], b, c);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < FMA_CHAINS; ++u) t += a[u];
+    if (t == magic) sink[0] = t;
+}
+
+// The best rate over launch shapes of 1, 2, 4 and 8 workgroups (4 .. 32 waves)
+// per CU: few long streams per CU can read faster than a full occupancy.
+double probe_stream_read(sbh_context* ctx, const void* x, int64_t bytes, int reps, unsigned* sink) {
+    const int64_t count = bytes / 16;
+    hipEvent_t e0, e1;
+    SBH_HIP(hipEventCreate(&e0));
+    SBH_HIP(hipEventCreate(&e1));
+    int cus = 0;
+    SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const probe_u32x4* px = static_cast<const probe_u32x4*>(x);
+    double best = 0.0;
+    for (int per_cu = 1; per_cu <= 8; per_cu *= 2) {
+        const dim3 grid((unsigned)std::max(cus, 1) * per_cu), block(256);
+        hipLaunchKernelGGL(stream_read_kernel, grid, block, 0, ctx->stream, px, count, 0x9e3779b9u, sink);
+        SBH_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(stream_read_kernel, grid, block, 0, ctx->stream, px, count, 0x9e3779b9u, sink);
+        SBH_HIP(hipGetLastError());
+        SBH_HIP(hipEventRecord(e1, ctx->stream));
+        SBH_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SBH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::max(best, (double)count * 16.0 * reps / (ms * 1e-3) / 1e9);
+    }
+    SBH_HIP(hipEventDestroy(e0));
+    SBH_HIP(hipEventDestroy(e1));
+    return best;
+}
+
+double probe_fma_rate(sbh_context* ctx, double* sink) {
+    int cus = 0;
+    SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const dim3 grid((unsigned)std::max(cus, 1) * 16), block(256);
+    const int64_t iters = 1 << 14;
+    hipEvent_t e0, e1;
+    SBH_HIP(hipEventCreate(&e0));
+    SBH_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(fma_rate_kernel, grid, block, 0, ctx->stream, iters, 0.999999, 1e-7, -1.0, sink);
+    SBH_HIP(hipEventRecord(e0, ctx->stream));
+    const int reps = 3;
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(fma_rate_kernel, grid, block, 0, ctx->stream, iters, 0.999999, 1e-7, -1.0, sink);
+    SBH_HIP(hipGetLastError());
+    SBH_HIP(hipEventRecord(e1, ctx->stream));
+    SBH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SBH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SBH_HIP(hipEventDestroy(e0));
+    SBH_HIP(hipEventDestroy(e1));
+    const double flops = 2.0 * FMA_CHAINS * (double)iters * grid.x * block.x * reps;
+    return flops / (ms * 1e-3) / 1e12;
 }
 
 void device_copy(sbh_context* ctx, const double2* src, double2* dst, int64_t n) {
