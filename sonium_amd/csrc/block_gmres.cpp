@@ -380,7 +380,7 @@ void BatchedGmres<T>::run(BlockOperatorT<T>& A, BlockPreconditionerT<T>* M, cons
                 double* nrm = reinterpret_cast<double*>(gs_.ptr + 2 * region);
                 double* scratch = reinterpret_cast<double*>(scratch_.ptr);
                 bdots(ctx_, V_.ptr, n_ * k_, j + 1, W_.ptr, n_, k, scratch_.ptr, d1);
-                bproject(ctx_, V_.ptr, n_ * k_, j + 1, d1, W_.ptr, n_, k, scratch, reinterpret_cast<double*>(part_.ptr));
+                bproject(ctx_, V_.ptr, n_ * k_, j + 1, d1, W_.ptr, n_, k, scratch, nullptr);
                 bdots(ctx_, V_.ptr, n_ * k_, j + 1, W_.ptr, n_, k, scratch_.ptr, d2);
                 bproject(ctx_, V_.ptr, n_ * k_, j + 1, d2, W_.ptr, n_, k, scratch, nrm);
                 fetch(gs_.ptr, 2 * region * 2 + k);

@@ -11,6 +11,8 @@ constexpr int BLOCK = 256;
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
@@ -90,6 +92,15 @@ __device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
+// v_dot2_f32_{f16,bf16}: c + a.lo * b.lo + a.hi * b.hi over two packed halves.
+template <int FMT>
+__device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
+    if constexpr (FMT == 0)
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b), c, false);
+    else
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf2, a), __builtin_bit_cast(bf2, b), c, false);
+}
+
 // Column-batched reductions over n x k row-major blocks: the sum of the S
 // partials sh[q * k + c] (q < S) of every column c lands in sh[c].  It is a
 // pairwise tree of fixed shape (width S, then ceil(width / 2), ...), so the
@@ -103,6 +114,24 @@ __device__ __forceinline__ void column_tree(V* sh, int S, int k, int sub, int c)
         const int half = (width + 1) / 2;
         __syncthreads();
         if (sub < width - half) sh[sub * k + c] = add2(sh[sub * k + c], sh[(sub + half) * k + c]);
+        width = half;
+    }
+    __syncthreads();
+}
+
+// column_tree over nv slices of BLOCK values each (slice v at sh + v * BLOCK),
+// one barrier per level for all of them; per slice the additions are those of
+// column_tree.
+template <typename V>
+__device__ __forceinline__ void column_tree_multi(V* sh, int S, int k, int sub, int c, int nv) {
+    for (int width = S; width > 1;) {
+        const int half = (width + 1) / 2;
+        __syncthreads();
+        if (sub < width - half)
+            for (int v = 0; v < nv; ++v) {
+                V* p = sh + v * BLOCK;
+                p[sub * k + c] = add2(p[sub * k + c], p[(sub + half) * k + c]);
+            }
         width = half;
     }
     __syncthreads();

@@ -136,18 +136,23 @@ __global__ __launch_bounds__(BLOCK) void colmax_kernel(const T* __restrict__ X, 
     const int64_t b = (int64_t)blockIdx.x * rows_per_block;
     const int64_t e = min(n, b + rows_per_block);
     float m = 0.f;
-    if (sub < S)
+    if (sub < S) {
+#pragma unroll 8
         for (int64_t r = b + sub; r < e; r += S) {
             double2 v = ld2(X[r * k + c]);
             m = fmaxf(m, (float)fmax(fabs(v.x), fabs(v.y)));
         }
-    sh[threadIdx.x] = m;
-    __syncthreads();
-    if (threadIdx.x < k) {
-        float t = 0.f;
-        for (int q = 0; q < S; ++q) t = fmaxf(t, sh[q * k + threadIdx.x]);
-        atomicMax(out + threadIdx.x, __float_as_uint(t));
     }
+    sh[threadIdx.x] = m;
+    // the S partials of a column by a tree (a maximum is exact in any order)
+    for (int width = S; width > 1;) {
+        const int half = (width + 1) / 2;
+        __syncthreads();
+        if (sub < width - half) sh[sub * k + c] = fmaxf(sh[sub * k + c], sh[(sub + half) * k + c]);
+        width = half;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) atomicMax(out + threadIdx.x, __float_as_uint(sh[threadIdx.x]));
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
@@ -275,6 +280,61 @@ __global__ __launch_bounds__(LP_THREADS) void lp_gemm_kernel(const u16* __restri
             const int64_t row = row_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
             if (row < n) Ypart[((int64_t)split * n + row) * BN + col] = acc[t][v];
         }
+    }
+}
+
+// One-column product (k = 1) on v_dot2_f32_{f16,bf16} instead of 32-column
+// MFMA tiles, 31 of whose columns would be discarded.  A wave owns one 32-row
+// tile and one of LP_SPLIT K slabs (slab = blockIdx % LP_SPLIT, as in
+// lp_gemm_kernel).  Lane l streams the 16-byte pieces of row 32 r + l % 32 that
+// the fragment layout gives it and reads the 16 bytes of XT row 0 at the same
+// half offsets, so per complex entry Re += dot2(a, [xr, -xi]) and
+// Im += dot2(a, [xi, xr]), in FP32 with four partial sums each.  Lanes l and
+// l + 32 hold the two halves of a row's slab sum; Ypart[slab][row] = (re, im),
+// which lp_reduce_kernel reads with bn = 2.
+constexpr int LV_WAVES = 4;
+constexpr int LV_UNROLL = 8;
+
+__device__ __forceinline__ unsigned imag_word(unsigned b) {  // [xr, -xi] -> [xi, xr]
+    return ((b << 16) | (b >> 16)) ^ 0x8000u;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(LV_WAVES * 64) void lp_gemv_kernel(const u16* __restrict__ A, int64_t ld, int64_t n,
+                                                                const u16* __restrict__ XT,
+                                                                float* __restrict__ Ypart) {
+    const int lane = threadIdx.x & 63;
+    const int split = blockIdx.x % LP_SPLIT;
+    const int64_t rt = (int64_t)(blockIdx.x / LP_SPLIT) * LV_WAVES + (threadIdx.x >> 6);
+    if (rt >= (n + 31) / 32) return;  // no barrier below
+    const int64_t fslab = ld / (8 * LP_SPLIT);  // fragments per slab (ld / 8 per tile), a multiple of LV_UNROLL
+    const int64_t f0 = split * fslab;
+    const u32x4* __restrict__ a = reinterpret_cast<const u32x4*>(A) + (rt * (ld / 8) + f0) * 64 + lane;
+    const u32x4* __restrict__ x = reinterpret_cast<const u32x4*>(XT) + 2 * f0 + (lane >> 5);
+    float re[4] = {0.f, 0.f, 0.f, 0.f}, im[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t s = 0; s < fslab; s += LV_UNROLL) {
+        u32x4 av[LV_UNROLL], xv[LV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < LV_UNROLL; ++u) {
+            av[u] = __builtin_nontemporal_load(a + (s + u) * 64);
+            xv[u] = x[2 * (s + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < LV_UNROLL; ++u)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                re[w] = dot2<FMT>(av[u][w], xv[u][w], re[w]);
+                im[w] = dot2<FMT>(av[u][w], imag_word(xv[u][w]), im[w]);
+            }
+    }
+    float sr = (re[0] + re[1]) + (re[2] + re[3]);
+    float si = (im[0] + im[1]) + (im[2] + im[3]);
+    sr += __shfl_xor(sr, 32, 64);
+    si += __shfl_xor(si, 32, 64);
+    const int64_t row = rt * 32 + lane;
+    if (lane < 32 && row < n) {
+        Ypart[((int64_t)split * n + row) * 2] = sr;
+        Ypart[((int64_t)split * n + row) * 2 + 1] = si;
     }
 }
 
@@ -448,21 +508,25 @@ constexpr int BT = 8;
 
 
 // out[(chunk * (nvec + 1) + vec) * k + c] = sum conj(V_vec[:,c]) W[:,c]; vec == nvec -> ||W[:,c]||^2.
-// Block (chunk, group) walks vectors group*BV .. group*BV + BV - 1 for its rows;
-// its W values stay in registers across them when the chunk is one pass
-// (S * BT rows), which is the usual case.
+// Block (chunk, group) takes vectors group*BV .. group*BV + BV - 1 for its rows
+// together: per row pass it issues the loads of all of them, keeps one
+// partial per vector in registers, and reduces the BV partials in one tree.
+// Its W values stay in registers when the chunk is one pass (S * BT rows),
+// which is the usual case.  Per vector a thread adds its rows in row order.
 constexpr int BV = 8;
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, int64_t ldv, int nvec,
                                                       const T* __restrict__ W, int64_t n, int k,
                                                       int64_t chunk_len, double2* __restrict__ out) {
-    __shared__ double2 sh[BLOCK];
+    __shared__ double2 sh[BV * BLOCK];
     const int S = BLOCK / k;
     const int c = threadIdx.x % k;
     const int sub = threadIdx.x / k;
     const int64_t b = (int64_t)blockIdx.x * chunk_len;
     const int64_t e = min(n, b + chunk_len);
     const bool one_pass = e - b <= (int64_t)S * BT;
+    const int vec0 = (int)blockIdx.y * BV;
+    const int nv = min(nvec + 1, vec0 + BV) - vec0;
     double2 w[BT];
     if (one_pass && sub < S) {
 #pragma unroll
@@ -471,51 +535,61 @@ __global__ __launch_bounds__(BLOCK) void bdots_kernel(const T* __restrict__ V, i
             w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
         }
     }
-    const int vec_end = min(nvec + 1, ((int)blockIdx.y + 1) * BV);
-    for (int vec = (int)blockIdx.y * BV; vec < vec_end; ++vec) {
-        double2 s = make_double2(0.0, 0.0);
-        if (sub < S) {
-            const T* __restrict__ v = V + (int64_t)vec * ldv;
-            for (int64_t g = b + sub; g < e; g += (int64_t)S * BT) {
-                if (!one_pass) {
+    double2 s[BV];
 #pragma unroll
-                    for (int t = 0; t < BT; ++t) {
-                        const int64_t r = g + (int64_t)S * t;
-                        w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
-                    }
+    for (int v = 0; v < BV; ++v) s[v] = make_double2(0.0, 0.0);
+    if (sub < S) {
+        for (int64_t g = b + sub; g < e; g += (int64_t)S * BT) {
+            if (!one_pass) {
+#pragma unroll
+                for (int t = 0; t < BT; ++t) {
+                    const int64_t r = g + (int64_t)S * t;
+                    w[t] = r < e ? ld2(W[r * k + c]) : make_double2(0.0, 0.0);
                 }
+            }
+#pragma unroll
+            for (int v = 0; v < BV; ++v) {
+                if (v >= nv) break;
+                const int vec = vec0 + v;
                 if (vec < nvec) {
+                    const T* __restrict__ vp = V + (int64_t)vec * ldv;
                     double2 x[BT];
 #pragma unroll
                     for (int t = 0; t < BT; ++t) {
                         const int64_t r = g + (int64_t)S * t;
-                        x[t] = r < e ? ld2(v[r * k + c]) : make_double2(0.0, 0.0);
+                        x[t] = r < e ? ld2(vp[r * k + c]) : make_double2(0.0, 0.0);
                     }
 #pragma unroll
                     for (int t = 0; t < BT; ++t) {
                         if (g + (int64_t)S * t < e) {
                             const double2 p = cmulc(x[t], w[t]);
-                            s.x += p.x;
-                            s.y += p.y;
+                            s[v].x += p.x;
+                            s[v].y += p.y;
                         }
                     }
                 } else {
 #pragma unroll
                     for (int t = 0; t < BT; ++t)
-                        if (g + (int64_t)S * t < e) s.x += w[t].x * w[t].x + w[t].y * w[t].y;
+                        if (g + (int64_t)S * t < e) s[v].x += w[t].x * w[t].x + w[t].y * w[t].y;
                 }
             }
         }
-        sh[threadIdx.x] = s;
-        column_tree(sh, S, k, sub, c);
-        if (threadIdx.x < k) out[((int64_t)blockIdx.x * (nvec + 1) + vec) * k + threadIdx.x] = sh[threadIdx.x];
-        __syncthreads();  // sh is rewritten for the next vector
     }
+#pragma unroll
+    for (int v = 0; v < BV; ++v)
+        if (v < nv) sh[v * BLOCK + threadIdx.x] = s[v];
+    column_tree_multi(sh, S, k, sub, c, nv);
+    if (threadIdx.x < k)
+        for (int v = 0; v < nv; ++v)
+            out[((int64_t)blockIdx.x * (nvec + 1) + vec0 + v) * k + threadIdx.x] = sh[v * BLOCK + threadIdx.x];
 }
 
 // W[:,c] -= sum_i C[i*k+c] V_i[:,c]; norm partials per column into out[chunk*k + c].
-// Per element the sum over i runs in order before the subtraction.
-template <typename T>
+// Per element the sum over i runs in order before the subtraction.  A thread
+// owns R rows per pass and unrolls the vector loop by U: R = 8, U = 1 for wide
+// blocks; R = 2, U = 4 for short chunks (at most 2 rows per thread), where the
+// loads of four vectors are in flight together.  Both give the same sums.
+template <typename T, int R, int U>
 __global__ __launch_bounds__(BLOCK) void bproject_kernel(const T* __restrict__ V, int64_t ldv, int nvec,
                                                          const double2* __restrict__ C, T* __restrict__ W,
                                                          int64_t n, int k, int64_t chunk_len,
@@ -528,28 +602,29 @@ __global__ __launch_bounds__(BLOCK) void bproject_kernel(const T* __restrict__ V
     const int64_t e = min(n, b + chunk_len);
     double nr = 0.0;
     if (sub < S) {
-        for (int64_t g = b + sub; g < e; g += (int64_t)S * BT) {
-            double sr[BT], si[BT];
+        for (int64_t g = b + sub; g < e; g += (int64_t)S * R) {
+            double sr[R], si[R];
 #pragma unroll
-            for (int t = 0; t < BT; ++t) sr[t] = si[t] = 0.0;
+            for (int t = 0; t < R; ++t) sr[t] = si[t] = 0.0;
+#pragma unroll U
             for (int i = 0; i < nvec; ++i) {
                 const double2 coef = C[i * k + c];
                 const T* __restrict__ v = V + (int64_t)i * ldv;
-                double2 x[BT];
+                double2 x[R];
 #pragma unroll
-                for (int t = 0; t < BT; ++t) {
+                for (int t = 0; t < R; ++t) {
                     const int64_t r = g + (int64_t)S * t;
                     x[t] = r < e ? ld2(v[r * k + c]) : make_double2(0.0, 0.0);
                 }
 #pragma unroll
-                for (int t = 0; t < BT; ++t) {
+                for (int t = 0; t < R; ++t) {
                     const double2 p = cmul(coef, x[t]);
                     sr[t] += p.x;
                     si[t] += p.y;
                 }
             }
 #pragma unroll
-            for (int t = 0; t < BT; ++t) {
+            for (int t = 0; t < R; ++t) {
                 const int64_t r = g + (int64_t)S * t;
                 if (r < e) {
                     double2 w = ld2(W[r * k + c]);
@@ -612,13 +687,19 @@ __global__ void bsum_groups_kernel(const double* __restrict__ part, int chunks, 
     tmp[(int64_t)blockIdx.y * nout + o] = s;
 }
 
-// out[o] = sum over g < groups of tmp[g * nout + o], in g order
-__global__ void bsum_final_kernel(const double* __restrict__ tmp, int64_t nout, int groups, double* __restrict__ out) {
+// out[o] = sum over g < groups (at most BSUM_GROUPS) of tmp[g * nout + o], in
+// g order; all loads are issued before the additions.
+__global__ __launch_bounds__(BLOCK) void bsum_small_kernel(const double* __restrict__ tmp, int64_t nout, int groups,
+                                                          double* __restrict__ out) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout) return;
+    double v[BSUM_GROUPS];
+#pragma unroll
+    for (int g = 0; g < BSUM_GROUPS; ++g) v[g] = g < groups ? tmp[(int64_t)g * nout + o] : 0.0;
     double s = 0.0;
-#pragma unroll 8
-    for (int g = 0; g < groups; ++g) s += tmp[(int64_t)g * nout + o];
+#pragma unroll
+    for (int g = 0; g < BSUM_GROUPS; ++g)
+        if (g < groups) s += v[g];
     out[o] = s;
 }
 
@@ -692,7 +773,7 @@ void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, in
     SBH_HIP(hipGetLastError());
 }
 
-// Y = A X for k in 1..64 columns.  Rows k..32*ntr-1 of XT (left from a wider
+// Y = A X for k in 1..64 columns (k = 1 on lp_gemv_kernel).  Rows k..32*ntr-1 of XT (left from a wider
 // product or zero) feed only output columns that lp_reduce_kernel discards.
 template <typename T>
 static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int format, const double* row_inv,
@@ -711,8 +792,26 @@ static void lowp_gemm_t(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t
                            colmax, XT, col_inv);
     SBH_HIP(hipGetLastError());
     const int ntr = (k + 31) / 32;  // 32-column tiles of the real part (the imaginary part has as many)
-    const int bn = 2 * ntr * 32;
     const int64_t n_tiles = (n + 31) / 32;
+    if (k == 1) {
+        const int64_t blocks = (n_tiles + LV_WAVES - 1) / LV_WAVES * LP_SPLIT;
+        SBH_REQUIRE(blocks < ((int64_t)1 << 31), "low-precision product too large");
+        hipEvent_t ev;
+        ctx->begin_timed(SBH_KERNEL_LOWP, &ev);
+        if (format == 0)
+            hipLaunchKernelGGL(lp_gemv_kernel<0>, dim3((unsigned)blocks), dim3(LV_WAVES * 64), 0, ctx->stream,
+                               reinterpret_cast<const u16*>(A), ld, n, reinterpret_cast<const u16*>(XT), Ypart);
+        else
+            hipLaunchKernelGGL(lp_gemv_kernel<1>, dim3((unsigned)blocks), dim3(LV_WAVES * 64), 0, ctx->stream,
+                               reinterpret_cast<const u16*>(A), ld, n, reinterpret_cast<const u16*>(XT), Ypart);
+        SBH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(lp_reduce_kernel<T>, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, Ypart, n, 1, 2,
+                           row_inv, col_inv, Y);
+        SBH_HIP(hipGetLastError());
+        ctx->end_timed(SBH_KERNEL_LOWP, ev);
+        return;
+    }
+    const int bn = 2 * ntr * 32;
     const int64_t row_blocks = (n_tiles + LP_WAVES - 1) / LP_WAVES;
     SBH_REQUIRE(row_blocks * LP_SPLIT < ((int64_t)1 << 31), "low-precision product too large");
     const unsigned grid = (unsigned)(row_blocks * LP_SPLIT);
@@ -778,6 +877,9 @@ void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const
 static int batched_chunks(int64_t n, int k, int64_t* len) {
     const int64_t rpb = (int64_t)(BLOCK / k) * BT;
     int64_t chunks = (n + rpb - 1) / rpb;
+    // few columns: shorter chunks (down to one row per thread) for more
+    // workgroups, up to the count whose sums still take one pass
+    if (chunks < BSUM_GROUPS) chunks = std::min<int64_t>(BSUM_GROUPS, (n + BLOCK / k - 1) / (BLOCK / k));
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 4096));
     *len = (n + chunks - 1) / chunks;
     return (int)chunks;
@@ -788,7 +890,7 @@ static int batched_chunks(int64_t n, int k, int64_t* len) {
 static void column_sums(sbh_context* ctx, double* part, int chunks, int64_t nout, double* out) {
     if (chunks <= BSUM_GROUPS) {
         // one pass in chunk order: the same sums as the two passes, whose groups then hold one chunk each
-        hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, part, nout, chunks,
+        hipLaunchKernelGGL(bsum_small_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, part, nout, chunks,
                            out);
         SBH_HIP(hipGetLastError());
         return;
@@ -797,7 +899,7 @@ static void column_sums(sbh_context* ctx, double* part, int chunks, int64_t nout
     hipLaunchKernelGGL(bsum_groups_kernel, dim3(nblocks(nout), BSUM_GROUPS), dim3(BLOCK), 0, ctx->stream, part,
                        chunks, nout, tmp);
     SBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(bsum_final_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, tmp, nout, BSUM_GROUPS,
+    hipLaunchKernelGGL(bsum_small_kernel, dim3(nblocks(nout)), dim3(BLOCK), 0, ctx->stream, tmp, nout, BSUM_GROUPS,
                        out);
     SBH_HIP(hipGetLastError());
 }
@@ -819,16 +921,21 @@ static void bdots_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const T
                 reinterpret_cast<double*>(out));
 }
 
-// W -= sum C_i V_i per column; the reduced k squared norms land in `out`.
+// W -= sum C_i V_i per column; the reduced k squared norms land in `out`
+// (out == nullptr: no norms wanted, and their sums are not launched).
 template <typename T>
 static void bproject_t(sbh_context* ctx, const T* V, int64_t ldv, int nvec, const double2* C, T* W, int64_t n, int k,
                        double* scratch, double* out) {
     int64_t len;
     const int chunks = batched_chunks(n, k, &len);
-    hipLaunchKernelGGL(bproject_kernel<T>, dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv, nvec, C, W, n,
-                       k, len, scratch);
+    if (len <= 2 * (int64_t)(BLOCK / k))
+        hipLaunchKernelGGL((bproject_kernel<T, 2, 4>), dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv,
+                           nvec, C, W, n, k, len, scratch);
+    else
+        hipLaunchKernelGGL((bproject_kernel<T, BT, 1>), dim3((unsigned)chunks), dim3(BLOCK), 0, ctx->stream, V, ldv,
+                           nvec, C, W, n, k, len, scratch);
     SBH_HIP(hipGetLastError());
-    column_sums(ctx, scratch, chunks, (int64_t)k, out);
+    if (out) column_sums(ctx, scratch, chunks, (int64_t)k, out);
 }
 
 // R = B - R (B may be null: norms only); the reduced k squared norms land in `out`.
