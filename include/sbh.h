@@ -609,7 +609,8 @@ sbh_status sbh_debug_device_rsqrt(sbh_context* ctx, const double* x, int64_t n, 
  * `bytes`, writes `bytes`), used to calibrate HBM byte counters. */
 sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps);
 /* Roofline ceiling probes (no counterpart in the reference): the measured
- * rate of a streaming read of `bytes` bytes (GB/s, average of `reps`
+ * rate of a streaming read of `bytes` bytes (GB/s: the best over launch
+ * shapes of 1, 2, 4 and 8 workgroups per CU, each the average of `reps`
  * launches) and of independent FP64 FMAs on every CU (TFLOP/s, FMA = 2). */
 sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, double* gbs);
 sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops);

@@ -112,6 +112,26 @@ def test_block_gmres_columns_equal_single_gmres(c1_block):
         assert rel_l2(X[:, c], xr) < 1e-8
 
 
+@pytest.mark.parametrize("k", [1, 64])
+def test_block_gmres_one_and_64_columns(c1_block, k):
+    """One column runs the short-chunk Krylov kernels (at most 2 rows per thread,
+    the projection unrolled over 4 basis vectors); 64 columns run the wide ones
+    (8 rows per thread).  Each column follows the single-RHS GMRES either way."""
+    op = sa.DenseMatrixOperator(c1_block["sys"])
+    ph = sa.PhysicsParams.from_wavenumber(2.0)
+    B = sa.IncidentField(fibonacci_directions(64)).compute_rhs_with_beta(c1_block["tm"], ph, c1_block["beta"])
+    B = B[:, :k].copy()
+    cfg = sa.GmresConfig(max_iterations=100, restart=50, tolerance=1e-8)
+    X, sols = sa.gmres_block(op, B, cfg)
+    for c in sorted({0, k // 2, k - 1}):
+        single = sa.gmres(op, B[:, c], cfg)
+        assert sols[c].converged and sols[c].residual <= 1e-8
+        assert abs(sols[c].iterations - single.iterations) <= 1
+        assert rel_l2(X[:, c], single.x) < 1e-9
+    R = B - c1_block["A"] @ X
+    assert np.all(np.linalg.norm(R, axis=0) <= 1e-8 * np.linalg.norm(B, axis=0) * 1.01)
+
+
 def test_block_gmres_short_restart_matches_single(c1_block):
     op = sa.DenseMatrixOperator(c1_block["sys"])
     B = c1_block["B"][:, :5]
