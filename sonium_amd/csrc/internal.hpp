@@ -370,11 +370,11 @@ void zgemv_t(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int6
              const double2* x, double2* y, bool conjugate);
 void zgemm_block(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X,
                  double2* Y, int64_t k);
-// Partial sums for dot products <v_i, w> (i < nvec) and ||w||^2 in one launch;
-// out holds (nvec + 1) x chunks complex partials (the last row is the norm).
 // Roofline ceiling probes (GB/s of a streaming read; FP64 FMA TFLOP/s).
 double probe_stream_read(sbh_context* ctx, const void* x, int64_t bytes, int reps, unsigned* sink);
 double probe_fma_rate(sbh_context* ctx, double* sink);
+// Partial sums for dot products <v_i, w> (i < nvec) and ||w||^2 in one launch;
+// out holds (nvec + 1) x chunks complex partials (the last row is the norm).
 int dots_and_norm_partials(sbh_context* ctx, const double2* V, int64_t ldv, int nvec,
                            const double2* w, int64_t n, double2* out);
 // s[v] = sum of the chunk partials of slot v (v < nv), in chunk order, on the device.
