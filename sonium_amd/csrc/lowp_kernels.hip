@@ -54,22 +54,29 @@ __device__ __forceinline__ double2 st2<double2>(double2 v) { return v; }
 template <>
 __device__ __forceinline__ float2 st2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
-// Row scales: one wave per row finds max |re|, |im| and writes row_inv.
+// Row scales: one wave per row finds max |re|, |im| and writes row_inv.  The
+// matrix is read once here, so a lane keeps RS_LOADS non-temporal 16-byte
+// loads in flight.
+typedef double rs_v2d __attribute__((ext_vector_type(2)));
+constexpr int RS_LOADS = 16;
+
 __global__ __launch_bounds__(BLOCK) void lp_rowscale_kernel(const double2* __restrict__ A, int64_t lda, int64_t n,
                                                             int top, double* __restrict__ row_inv) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
     if (i >= n) return;
-    const double2* __restrict__ row = A + i * lda;
+    const rs_v2d* __restrict__ row = reinterpret_cast<const rs_v2d*>(A + i * lda);
     double m = 0.0;
     int64_t j = lane;
-    for (; j + 192 < n; j += 256) {
-        const double2 v0 = row[j], v1 = row[j + 64], v2 = row[j + 128], v3 = row[j + 192];
-        m = fmax(m, fmax(fmax(fabs(v0.x), fabs(v0.y)), fmax(fabs(v1.x), fabs(v1.y))));
-        m = fmax(m, fmax(fmax(fabs(v2.x), fabs(v2.y)), fmax(fabs(v3.x), fabs(v3.y))));
+    for (; j + 64 * (RS_LOADS - 1) < n; j += 64 * RS_LOADS) {
+        rs_v2d v[RS_LOADS];
+#pragma unroll
+        for (int u = 0; u < RS_LOADS; ++u) v[u] = __builtin_nontemporal_load(row + j + 64 * u);
+#pragma unroll
+        for (int u = 0; u < RS_LOADS; ++u) m = fmax(m, fmax(fabs(v[u].x), fabs(v[u].y)));
     }
     for (; j < n; j += 64) {
-        const double2 v = row[j];
+        const rs_v2d v = row[j];
         m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
     }
     m = wave_max(m);
@@ -92,10 +99,17 @@ __global__ __launch_bounds__(BLOCK) void lp_pack_kernel(const double2* __restric
                                                         int64_t ld, const double* __restrict__ row_inv,
                                                         u16* __restrict__ out) {
     __shared__ __attribute__((aligned(16))) unsigned sh[32 * LP_PACK_ROW];
+    __shared__ double scale_sh[32];
     const int64_t groups = ld / LP_PACK_COLS;
     const int64_t rt = blockIdx.x / groups;
     const int64_t g = blockIdx.x % groups;
     const int64_t j_base = g * LP_PACK_COLS;
+    if (threadIdx.x < 32) {
+        const int64_t row = rt * 32 + threadIdx.x;
+        scale_sh[threadIdx.x] = row < n ? 1.0 / row_inv[row] : 0.0;  // a power of two, exact
+    }
+    __syncthreads();
+    const rs_v2d* __restrict__ a = reinterpret_cast<const rs_v2d*>(A);
 #pragma unroll
     for (int i = 0; i < 32 * LP_PACK_COLS / BLOCK; ++i) {
         const int e = i * BLOCK + threadIdx.x;
@@ -103,8 +117,8 @@ __global__ __launch_bounds__(BLOCK) void lp_pack_kernel(const double2* __restric
         const int64_t row = rt * 32 + r, j = j_base + col;
         unsigned w = 0;
         if (row < n && j < n) {
-            const double2 v = A[row * lda + j];
-            const double scale = 1.0 / row_inv[row];
+            const rs_v2d v = __builtin_nontemporal_load(a + row * lda + j);  // read once here
+            const double scale = scale_sh[r];
             w = (unsigned)Half<FMT>::from_float((float)(v.x * scale)) |
                 ((unsigned)Half<FMT>::from_float((float)(v.y * scale)) << 16);
         }
