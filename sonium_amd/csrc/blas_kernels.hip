@@ -469,13 +469,32 @@ __global__ __launch_bounds__(256) void fma_rate_kernel(int64_t iters, double b, 
     if (t == magic) sink[0] = t;
 }
 
+// Two timing events, destroyed on every exit path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair() {
+        SBH_HIP(hipEventCreate(&e0));
+        const hipError_t r = hipEventCreate(&e1);
+        if (r != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            e0 = nullptr;
+            SBH_HIP(r);
+        }
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+};
+
 // The best rate over launch shapes of 1, 2, 4 and 8 workgroups (4 .. 32 waves)
 // per CU: few long streams per CU can read faster than a full occupancy.
 double probe_stream_read(sbh_context* ctx, const void* x, int64_t bytes, int reps, unsigned* sink) {
     const int64_t count = bytes / 16;
-    hipEvent_t e0, e1;
-    SBH_HIP(hipEventCreate(&e0));
-    SBH_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     int cus = 0;
     SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const probe_u32x4* px = static_cast<const probe_u32x4*>(x);
@@ -493,8 +512,6 @@ double probe_stream_read(sbh_context* ctx, const void* x, int64_t bytes, int rep
         SBH_HIP(hipEventElapsedTime(&ms, e0, e1));
         best = std::max(best, (double)count * 16.0 * reps / (ms * 1e-3) / 1e9);
     }
-    SBH_HIP(hipEventDestroy(e0));
-    SBH_HIP(hipEventDestroy(e1));
     return best;
 }
 
@@ -503,9 +520,8 @@ double probe_fma_rate(sbh_context* ctx, double* sink) {
     SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const dim3 grid((unsigned)std::max(cus, 1) * 16), block(256);
     const int64_t iters = 1 << 14;
-    hipEvent_t e0, e1;
-    SBH_HIP(hipEventCreate(&e0));
-    SBH_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     hipLaunchKernelGGL(fma_rate_kernel, grid, block, 0, ctx->stream, iters, 0.999999, 1e-7, -1.0, sink);
     SBH_HIP(hipEventRecord(e0, ctx->stream));
     const int reps = 3;
@@ -516,8 +532,6 @@ double probe_fma_rate(sbh_context* ctx, double* sink) {
     SBH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     SBH_HIP(hipEventElapsedTime(&ms, e0, e1));
-    SBH_HIP(hipEventDestroy(e0));
-    SBH_HIP(hipEventDestroy(e1));
     const double flops = 2.0 * FMA_CHAINS * (double)iters * grid.x * block.x * reps;
     return flops / (ms * 1e-3) / 1e12;
 }
