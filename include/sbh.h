@@ -326,6 +326,21 @@ typedef struct {
     int64_t max_iterations; /* restart cycles */
     int64_t restart;
     double tolerance;
+    /* Convergence is tested after inner step j when j % max(1, interval) == 0,
+     * at the last step of a cycle and at breakdown (gmres.rs:871-876); 0 and 1
+     * test every step (the default). */
+    int64_t convergence_check_interval;
+    /* exact_residual_check_interval (gmres.rs:37-42): every converged return is
+     * confirmed with the recomputed residual whatever its value
+     * (exact_residual_check_due, gmres_common.rs:8-14), so it changes no
+     * result; negative values are rejected. */
+    int64_t exact_residual_check_interval;
+    /* Adaptive restart (min_restart/max_restart, gmres.rs:43-46, 1013-1029):
+     * with both > 0, a full cycle whose projected residual fell by less than
+     * 10 % doubles the restart (up to max_restart) and one that fell below a
+     * tenth halves it (down to min_restart); 0 = None (fixed restart). */
+    int64_t min_restart;
+    int64_t max_restart;
 } sbh_gmres_config;
 
 typedef enum {
@@ -345,6 +360,7 @@ typedef struct {
     int32_t status;            /* sbh_solver_status */
     int64_t spmv_count;
     int64_t reorthogonalizations;
+    int64_t exact_residual_checks; /* KrylovOpCounts::exact_residual_checks */
 } sbh_gmres_solution;
 
 /* b, x0 (may be NULL = zero guess) and x use `memory`; precond may be NULL

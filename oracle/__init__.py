@@ -49,6 +49,11 @@ def lib():
             ctypes.POINTER(ctypes.c_uint8), _dp, ctypes.c_double, ctypes.c_double,
             ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.oracle_build_tbem_system.restype = ctypes.c_int
+        L.oracle_build_tbem_row_list.argtypes = [
+            _dp, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), _dp, _dp,
+            ctypes.POINTER(ctypes.c_uint8), _dp, ctypes.c_double, ctypes.c_double,
+            _i64p, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.oracle_build_tbem_row_list.restype = ctypes.c_int
         L.oracle_regular_integration_dk.argtypes = L.oracle_regular_integration.argtypes
         L.oracle_singular_integration_dk.argtypes = L.oracle_singular_integration.argtypes
         L.oracle_build_tbem_derivative.argtypes = [
@@ -66,6 +71,10 @@ def lib():
         L.oracle_gmres.argtypes = [_dp, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.oracle_gmres.restype = ctypes.c_int
+        L.oracle_gmres_ex.argtypes = [_dp, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _dp, _dp]
+        L.oracle_gmres_ex.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -148,12 +157,19 @@ def build_tbem_system(nodes, tris, k, beta, *, bc_type=None, bc_value=None, admi
 
     Mirrors build_tbem_system_with_beta_parallel (tbem.rs:1284-1466) for a
     Tri3 mesh with identity DOF numbering.  ``admittance`` is a complex array
-    with NaN marking panels without admittance.
+    with NaN marking panels without admittance.  ``rows`` is None (all rows),
+    a (begin, end) pair, or an integer array of row indices (output row r is
+    matrix row rows[r]).
     """
     nodes = _f64(nodes)
     tris = np.ascontiguousarray(tris, dtype=np.int64)
     n = tris.shape[0]
-    r0, r1 = (0, n) if rows is None else rows
+    row_list = None
+    if rows is not None and not (isinstance(rows, tuple) and len(rows) == 2):
+        row_list = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
+        r0, r1 = 0, row_list.shape[0]
+    else:
+        r0, r1 = (0, n) if rows is None else rows
     omega = k * speed_of_sound
     phys = np.array([k, harmonic_factor, density * omega * harmonic_factor, tau])
     bt = None if bc_type is None else np.ascontiguousarray(bc_type, dtype=np.int32)
@@ -166,13 +182,18 @@ def build_tbem_system(nodes, tris, k, beta, *, bc_type=None, bc_value=None, admi
     mat = np.zeros((r1 - r0, n), dtype=np.complex128)
     rhs = np.zeros(r1 - r0, dtype=np.complex128)
     beta = complex(beta)
-    rc = lib().oracle_build_tbem_system(
-        _p(nodes), tris.ctypes.data_as(_i64p), n,
-        None if bt is None else bt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-        None if bv is None else _p(bv), None if ad is None else _p(ad),
-        None if has is None else has.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-        _p(phys), beta.real, beta.imag, r0, r1, _p(mat.view(np.float64)), _p(rhs.view(np.float64)),
-        threads or default_threads())
+    common = (_p(nodes), tris.ctypes.data_as(_i64p), n,
+              None if bt is None else bt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+              None if bv is None else _p(bv), None if ad is None else _p(ad),
+              None if has is None else has.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+              _p(phys), beta.real, beta.imag)
+    if row_list is None:
+        rc = lib().oracle_build_tbem_system(*common, r0, r1, _p(mat.view(np.float64)), _p(rhs.view(np.float64)),
+                                            threads or default_threads())
+    else:
+        rc = lib().oracle_build_tbem_row_list(*common, row_list.ctypes.data_as(_i64p), row_list.shape[0],
+                                              _p(mat.view(np.float64)), _p(rhs.view(np.float64)),
+                                              threads or default_threads())
     if rc != 0:
         raise ValueError("oracle_build_tbem_system rejected its inputs")
     return mat, rhs
@@ -356,7 +377,8 @@ def zgemv(A, x, threads=None):
 STATUS = {0: "converged", 1: "max_iterations", 2: "breakdown", 3: "numerical_error"}
 
 
-def gmres(A, b, x0=None, *, restart=50, max_iterations=100, tol=1e-8, jacobi=False, threads=None):
+def gmres(A, b, x0=None, *, restart=50, max_iterations=100, tol=1e-8, jacobi=False, threads=None,
+          convergence_check_interval=1, min_restart=None, max_restart=None):
     """Restated gmres()/gmres_preconditioned_with_guess (gmres.rs:286, 608)."""
     A = np.ascontiguousarray(A, dtype=np.complex128)
     b = np.ascontiguousarray(b, dtype=np.complex128)
@@ -367,9 +389,10 @@ def gmres(A, b, x0=None, *, restart=50, max_iterations=100, tol=1e-8, jacobi=Fal
     if x0 is not None:
         x0 = np.ascontiguousarray(x0, dtype=np.complex128)
         x0p = _p(x0.view(np.float64))
-    lib().oracle_gmres(_p(A.view(np.float64)), n, _p(b.view(np.float64)), x0p, restart,
-                       max_iterations, tol, 1 if jacobi else 0, threads or default_threads(),
-                       _p(x.view(np.float64)), _p(st))
+    lib().oracle_gmres_ex(_p(A.view(np.float64)), n, _p(b.view(np.float64)), x0p, restart,
+                          max_iterations, tol, 1 if jacobi else 0, threads or default_threads(),
+                          int(convergence_check_interval), int(min_restart or 0), int(max_restart or 0),
+                          _p(x.view(np.float64)), _p(st))
     stats = {"iterations": int(st[0]), "restarts": int(st[1]), "residual": st[2],
              "estimated_residual": st[3], "backward_error": st[4], "converged": bool(st[5]),
              "status": STATUS[int(st[6])], "spmv_count": int(st[7])}

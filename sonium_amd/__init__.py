@@ -731,11 +731,18 @@ class JacobiPreconditioner(_Precond):
 
 @dataclass
 class GmresConfig:
-    """gmres.rs:26-71; defaults of GmresConfig<f64>::default()."""
+    """gmres.rs:26-71; defaults of GmresConfig<f64>::default().
+
+    ``min_restart``/``max_restart`` of None keep the restart fixed; with both
+    set, the restart adapts between them (gmres.rs:1013-1029)."""
 
     max_iterations: int = 100
     restart: int = 30
     tolerance: float = 1e-6
+    convergence_check_interval: int = 1
+    exact_residual_check_interval: int = 0
+    min_restart: Optional[int] = None
+    max_restart: Optional[int] = None
 
     @classmethod
     def for_small_problems(cls) -> "GmresConfig":
@@ -746,7 +753,9 @@ class GmresConfig:
         return cls(restart=restart)
 
     def _c(self):
-        return _ffi.gmres_config_t(int(self.max_iterations), int(self.restart), float(self.tolerance))
+        return _ffi.gmres_config_t(int(self.max_iterations), int(self.restart), float(self.tolerance),
+                                   int(self.convergence_check_interval), int(self.exact_residual_check_interval),
+                                   int(self.min_restart or 0), int(self.max_restart or 0))
 
 
 @dataclass
@@ -763,6 +772,7 @@ class GmresSolution:
     status: str
     spmv_count: int = 0
     reorthogonalizations: int = 0
+    exact_residual_checks: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -771,7 +781,8 @@ def _solution(raw: _ffi.gmres_solution_t, x) -> GmresSolution:
                          residual=float(raw.residual), estimated_residual=float(raw.estimated_residual),
                          backward_error=float(raw.backward_error), converged=bool(raw.converged),
                          status=SOLVER_STATUS[int(raw.status)], spmv_count=int(raw.spmv_count),
-                         reorthogonalizations=int(raw.reorthogonalizations))
+                         reorthogonalizations=int(raw.reorthogonalizations),
+                         exact_residual_checks=int(raw.exact_residual_checks))
 
 
 def gmres_preconditioned_with_guess(op: DenseMatrixOperator, precond, b, x0, config: GmresConfig) -> GmresSolution:

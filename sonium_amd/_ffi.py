@@ -56,7 +56,9 @@ class incident_source_t(ctypes.Structure):
 
 class gmres_config_t(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64),
-                ("tolerance", ctypes.c_double)]
+                ("tolerance", ctypes.c_double), ("convergence_check_interval", ctypes.c_int64),
+                ("exact_residual_check_interval", ctypes.c_int64), ("min_restart", ctypes.c_int64),
+                ("max_restart", ctypes.c_int64)]
 
 
 class gmres_solution_t(ctypes.Structure):
@@ -64,7 +66,7 @@ class gmres_solution_t(ctypes.Structure):
                 ("residual", ctypes.c_double), ("estimated_residual", ctypes.c_double),
                 ("backward_error", ctypes.c_double), ("converged", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("spmv_count", ctypes.c_int64),
-                ("reorthogonalizations", ctypes.c_int64)]
+                ("reorthogonalizations", ctypes.c_int64), ("exact_residual_checks", ctypes.c_int64)]
 
 
 class gmres_ir_config_t(ctypes.Structure):

@@ -229,7 +229,7 @@ void sbh_context_destroy(sbh_context* ctx) {
 }
 
 sbh_status sbh_context_synchronize(sbh_context* ctx) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx, "ctx is null");
         use_device(ctx);
         SBH_HIP(hipStreamSynchronize(ctx->stream));
@@ -237,14 +237,14 @@ sbh_status sbh_context_synchronize(sbh_context* ctx) {
 }
 
 sbh_status sbh_context_set_profiling(sbh_context* ctx, int enabled) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx, "ctx is null");
         ctx->profiling = enabled != 0;
     });
 }
 
 sbh_status sbh_context_kernel_stats(sbh_context* ctx, int cls, int64_t* launches, double* total_ms) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx, "ctx is null");
         SBH_REQUIRE(cls >= 0 && cls < SBH_KERNEL_CLASS_COUNT, "kernel class out of range");
         use_device(ctx);
@@ -265,7 +265,7 @@ sbh_status sbh_context_kernel_stats(sbh_context* ctx, int cls, int64_t* launches
 }
 
 sbh_status sbh_context_reset_stats(sbh_context* ctx) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx, "ctx is null");
         use_device(ctx);
         SBH_HIP(hipStreamSynchronize(ctx->stream));
@@ -283,7 +283,7 @@ sbh_status sbh_context_reset_stats(sbh_context* ctx) {
 }
 
 sbh_status sbh_vector_create(sbh_context* ctx, int64_t length, sbh_vector** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out && length >= 1, "invalid vector arguments");
         use_device(ctx);
         std::unique_ptr<sbh_vector> v(new sbh_vector());
@@ -311,7 +311,7 @@ int64_t sbh_vector_length(const sbh_vector* v) { return v ? v->length : 0; }
 sbh_c64* sbh_vector_data(sbh_vector* v) { return v ? reinterpret_cast<sbh_c64*>(v->d.ptr) : nullptr; }
 
 sbh_status sbh_vector_upload(sbh_vector* v, const sbh_c64* host, int64_t offset, int64_t count) {
-    return guard([&] {
+    return guard_on(ctx_of(v), [&] {
         SBH_REQUIRE(v && host, "vector/host is null");
         SBH_REQUIRE(offset >= 0 && count >= 0 && offset + count <= v->length, "range out of bounds");
         use_device(v->ctx);
@@ -322,7 +322,7 @@ sbh_status sbh_vector_upload(sbh_vector* v, const sbh_c64* host, int64_t offset,
 }
 
 sbh_status sbh_vector_download(const sbh_vector* v, sbh_c64* host, int64_t offset, int64_t count) {
-    return guard([&] {
+    return guard_on(ctx_of(v), [&] {
         SBH_REQUIRE(v && host, "vector/host is null");
         SBH_REQUIRE(offset >= 0 && count >= 0 && offset + count <= v->length, "range out of bounds");
         use_device(v->ctx);
@@ -344,7 +344,7 @@ sbh_status sbh_mesh_create_oriented(sbh_context* ctx, const double* nodes, int64
                                     const int64_t* triangles, int64_t num_elements, const int32_t* bc_type,
                                     const sbh_c64* bc_value, const sbh_c64* admittance,
                                     const uint8_t* has_admittance, int normal_rule, sbh_mesh** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out, "ctx/out is null");
         SBH_REQUIRE(normal_rule == SBH_NORMALS_OUTWARD || normal_rule == SBH_NORMALS_VERTEX_ORDER,
                     "unknown normal rule");
@@ -407,14 +407,14 @@ sbh_status sbh_mesh_create_oriented(sbh_context* ctx, const double* nodes, int64
         m->d_geom.allocate(num_elements);
         m->d_src.allocate(num_elements);
         m->d_bc.allocate(num_elements);
-        SBH_HIP(hipMemcpy(m->d_far.ptr, far.data(), m->d_far.bytes(), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemcpy(m->d_geom.ptr, m->geom.data(), m->d_geom.bytes(), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemcpy(m->d_src.ptr, src.data(), m->d_src.bytes(), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemcpy(m->d_bc.ptr, m->bc.data(), m->d_bc.bytes(), hipMemcpyHostToDevice));
+        copy_on_stream(ctx, m->d_far.ptr, far.data(), m->d_far.bytes(), hipMemcpyHostToDevice);
+        copy_on_stream(ctx, m->d_geom.ptr, m->geom.data(), m->d_geom.bytes(), hipMemcpyHostToDevice);
+        copy_on_stream(ctx, m->d_src.ptr, src.data(), m->d_src.bytes(), hipMemcpyHostToDevice);
+        copy_on_stream(ctx, m->d_bc.ptr, m->bc.data(), m->d_bc.bytes(), hipMemcpyHostToDevice);
         if (!m->nonzero_bc.empty()) {
             m->d_nonzero_bc.allocate(m->nonzero_bc.size());
-            SBH_HIP(hipMemcpy(m->d_nonzero_bc.ptr, m->nonzero_bc.data(), m->d_nonzero_bc.bytes(),
-                              hipMemcpyHostToDevice));
+            copy_on_stream(ctx, m->d_nonzero_bc.ptr, m->nonzero_bc.data(), m->d_nonzero_bc.bytes(),
+                           hipMemcpyHostToDevice);
         }
         context_retain(ctx);
         *out = m.release();
@@ -433,7 +433,7 @@ void sbh_mesh_destroy(sbh_mesh* mesh) {
 int64_t sbh_mesh_num_elements(const sbh_mesh* mesh) { return mesh ? mesh->n : 0; }
 
 sbh_status sbh_mesh_geometry(const sbh_mesh* m, double* centers, double* normals, double* areas) {
-    return guard([&] {
+    return guard_on(ctx_of(m), [&] {
         SBH_REQUIRE(m, "mesh is null");
         if (centers) std::memcpy(centers, m->centers.data(), sizeof(double) * 3 * m->n);
         if (normals) std::memcpy(normals, m->normals.data(), sizeof(double) * 3 * m->n);
@@ -443,7 +443,7 @@ sbh_status sbh_mesh_geometry(const sbh_mesh* m, double* centers, double* normals
 
 sbh_status sbh_workspace_create(sbh_context* ctx, const sbh_mesh* mesh, int route_threads,
                                 sbh_workspace** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && mesh && out, "ctx/mesh/out is null");
         SBH_REQUIRE(mesh->ctx == ctx, "mesh belongs to another context");
         *out = nullptr;
@@ -457,14 +457,14 @@ sbh_status sbh_workspace_create(sbh_context* ctx, const sbh_mesh* mesh, int rout
         ws->leaves = (int64_t)rt.leaves.size();
         if (ws->near_pairs) {
             ws->d_pairs.allocate(rt.pairs.size());
-            SBH_HIP(hipMemcpy(ws->d_pairs.ptr, rt.pairs.data(), ws->d_pairs.bytes(), hipMemcpyHostToDevice));
+            copy_on_stream(ctx, ws->d_pairs.ptr, rt.pairs.data(), ws->d_pairs.bytes(), hipMemcpyHostToDevice);
         }
         if (ws->leaves) {
             ws->d_leaves.allocate(rt.leaves.size());
-            SBH_HIP(hipMemcpy(ws->d_leaves.ptr, rt.leaves.data(), ws->d_leaves.bytes(), hipMemcpyHostToDevice));
+            copy_on_stream(ctx, ws->d_leaves.ptr, rt.leaves.data(), ws->d_leaves.bytes(), hipMemcpyHostToDevice);
         }
         ws->d_row_ptr.allocate(rt.row_ptr.size());
-        SBH_HIP(hipMemcpy(ws->d_row_ptr.ptr, rt.row_ptr.data(), ws->d_row_ptr.bytes(), hipMemcpyHostToDevice));
+        copy_on_stream(ctx, ws->d_row_ptr.ptr, rt.row_ptr.data(), ws->d_row_ptr.bytes(), hipMemcpyHostToDevice);
         ws->build_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         context_retain(ctx);
@@ -483,7 +483,7 @@ void sbh_workspace_destroy(sbh_workspace* ws) {
 
 sbh_status sbh_workspace_stats(const sbh_workspace* ws, int64_t* near_pairs, int64_t* leaves,
                                double* build_ms) {
-    return guard([&] {
+    return guard_on(ctx_of(ws), [&] {
         SBH_REQUIRE(ws, "workspace is null");
         if (near_pairs) *near_pairs = ws->near_pairs;
         if (leaves) *leaves = ws->leaves;
@@ -492,7 +492,7 @@ sbh_status sbh_workspace_stats(const sbh_workspace* ws, int64_t* near_pairs, int
 }
 
 sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out, "ctx/out is null");
         SBH_REQUIRE(num_dofs >= 1, "num_dofs must be positive");
         *out = nullptr;
@@ -522,7 +522,7 @@ void sbh_system_destroy(sbh_system* sys) {
 int64_t sbh_system_num_dofs(const sbh_system* sys) { return sys ? sys->n : 0; }
 
 sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t* lda, sbh_c64** rhs) {
-    return guard([&] {
+    return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys, "system is null");
         if (matrix) *matrix = reinterpret_cast<sbh_c64*>(sys->d_A.ptr);
         if (lda) *lda = sys->lda;
@@ -532,7 +532,7 @@ sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t
 
 sbh_status sbh_system_download(sbh_system* sys, int64_t row_begin, int64_t row_end, sbh_c64* rows,
                                sbh_c64* rhs) {
-    return guard([&] {
+    return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys, "system is null");
         SBH_REQUIRE(row_begin >= 0 && row_begin <= row_end && row_end <= sys->n, "row range out of bounds");
         use_device(sys->ctx);
@@ -549,7 +549,7 @@ sbh_status sbh_system_download(sbh_system* sys, int64_t row_begin, int64_t row_e
 }
 
 sbh_status sbh_system_set_rhs(sbh_system* sys, const sbh_c64* rhs, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys && rhs, "system/rhs is null");
         check_memory(memory);
         use_device(sys->ctx);
@@ -561,7 +561,7 @@ sbh_status sbh_system_set_rhs(sbh_system* sys, const sbh_c64* rhs, int memory) {
 }
 
 sbh_status sbh_workspace_assemble(sbh_workspace* ws, const sbh_physics* ph, sbh_c64 beta, sbh_system* sys) {
-    return guard([&] {
+    return guard_on(ctx_of(ws), [&] {
         SBH_REQUIRE(ws && ph && sys, "workspace/physics/system is null");
         SBH_REQUIRE(sys->ctx == ws->ctx, "system belongs to another context");
         SBH_REQUIRE(sys->n == ws->mesh->n, "system size does not match the mesh");
@@ -576,7 +576,7 @@ sbh_status sbh_workspace_assemble_derivative(sbh_workspace* ws, const sbh_physic
                                              sbh_c64 beta_derivative, double sign, double wavenumber_scale,
                                              double pressure_factor_derivative, double admittance_scale,
                                              sbh_system* sys) {
-    return guard([&] {
+    return guard_on(ctx_of(ws), [&] {
         SBH_REQUIRE(ws && ph && sys, "workspace/physics/system is null");
         SBH_REQUIRE(sys->ctx == ws->ctx, "system belongs to another context");
         SBH_REQUIRE(sys->n == ws->mesh->n, "system size does not match the mesh");
@@ -595,7 +595,7 @@ sbh_status sbh_workspace_assemble_derivative(sbh_workspace* ws, const sbh_physic
 
 sbh_status sbh_build_tbem_system(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph,
                                  sbh_system* sys) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && mesh && ph && sys, "null argument");
         sbh_workspace* ws = nullptr;
         sbh_status st = sbh_workspace_create(ctx, mesh, 0, &ws);
@@ -611,7 +611,7 @@ sbh_status sbh_incident_plane_wave_rhs(sbh_context* ctx, const sbh_mesh* mesh, c
                                        sbh_c64 beta, double sign, const double* directions,
                                        const sbh_c64* amplitudes, int64_t num_rhs, sbh_c64* rhs,
                                        int memory, int accumulate) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && mesh && ph && directions && amplitudes && rhs, "null argument");
         SBH_REQUIRE(num_rhs >= 1, "num_rhs must be positive");
         check_memory(memory);
@@ -700,7 +700,7 @@ static void field_pressure_impl(sbh_context* ctx, const sbh_mesh* mesh, const sb
     const double* host_pts = points;
     if (memory == SBH_MEM_DEVICE) {
         pts_copy.resize((size_t)(3 * num_points));
-        SBH_HIP(hipMemcpy(pts_copy.data(), points, pts_copy.size() * sizeof(double), hipMemcpyDeviceToHost));
+        copy_on_stream(ctx, pts_copy.data(), points, pts_copy.size() * sizeof(double), hipMemcpyDeviceToHost);
         host_pts = pts_copy.data();
     }
     for (int64_t i = 0; i < 3 * num_points; ++i)
@@ -714,7 +714,7 @@ static void field_pressure_impl(sbh_context* ctx, const sbh_mesh* mesh, const sb
         const sbh_c64* h = samples;
         if (memory == SBH_MEM_DEVICE) {
             copy.resize((size_t)n);
-            SBH_HIP(hipMemcpy(copy.data(), samples, copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
+            copy_on_stream(ctx, copy.data(), samples, copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost);
             h = copy.data();
         }
         for (int64_t j = 0; j < n; ++j)
@@ -756,7 +756,7 @@ static void field_pressure_impl(sbh_context* ctx, const sbh_mesh* mesh, const sb
     std::vector<sbh_c64> host_copy((size_t)num_points);
     for (int o = 0; o < 3; ++o) {
         if (!outs[o]) continue;
-        SBH_HIP(hipMemcpy(host_copy.data(), outs[o], host_copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost));
+        copy_on_stream(ctx, host_copy.data(), outs[o], host_copy.size() * sizeof(sbh_c64), hipMemcpyDeviceToHost);
         for (int64_t i = 0; i < num_points; ++i)
             if (!std::isfinite(host_copy[i].re) || !std::isfinite(host_copy[i].im))
                 throw Error(SBH_ERR_NUMERICAL, "field value " + std::to_string(i) + " is not finite");
@@ -768,7 +768,7 @@ sbh_status sbh_field_pressure(sbh_context* ctx, const sbh_mesh* mesh, const sbh_
                               const sbh_c64* surface_velocity, const double* directions, const sbh_c64* amplitudes,
                               int64_t num_waves, const double* points, int64_t num_points, sbh_c64* scattered,
                               sbh_c64* incident, sbh_c64* total, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(num_waves >= 0, "counts must be non-negative");
         SBH_REQUIRE(num_waves == 0 || (directions && amplitudes), "directions/amplitudes is null");
         const std::vector<double> dirs = normalized_directions(directions, num_waves);
@@ -790,7 +790,7 @@ sbh_status sbh_field_pressure_sources(sbh_context* ctx, const sbh_mesh* mesh, co
                                       const sbh_incident_source* sources, int64_t num_sources, const double* points,
                                       int64_t num_points, sbh_c64* scattered, sbh_c64* incident, sbh_c64* total,
                                       int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(num_sources >= 0 && (num_sources == 0 || sources), "sources is null");
         const IncidentSource* s = reinterpret_cast<const IncidentSource*>(sources);
         std::vector<IncidentSource> src(s, s + num_sources);
@@ -802,7 +802,7 @@ sbh_status sbh_field_pressure_sources(sbh_context* ctx, const sbh_mesh* mesh, co
 sbh_status sbh_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics* ph, sbh_c64 beta, double sign,
                             const sbh_incident_source* sources, int64_t num_sources, sbh_c64* rhs, int memory,
                             int accumulate) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && mesh && ph && rhs, "null argument");
         SBH_REQUIRE(num_sources >= 0 && (num_sources == 0 || sources), "sources is null");
         check_memory(memory);
@@ -832,7 +832,7 @@ sbh_status sbh_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_ph
 }
 
 sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out) {
-    return guard([&] {
+    return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys && out, "system/out is null");
         sbh_operator* op = new sbh_operator();
         op->ctx = sys->ctx;
@@ -845,7 +845,7 @@ sbh_status sbh_dense_operator_create(sbh_system* sys, sbh_operator** out) {
 
 sbh_status sbh_exact_operator_create(sbh_workspace* ws, const sbh_physics* physics, sbh_c64 beta, double sign,
                                      sbh_operator** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ws), [&] {
         SBH_REQUIRE(ws && physics && out, "workspace/physics/out is null");
         SBH_REQUIRE(std::isfinite(sign) && std::isfinite(beta.re) && std::isfinite(beta.im),
                     "sign and beta must be finite");
@@ -888,7 +888,7 @@ int sbh_operator_hermitian_capability(const sbh_operator* op) {
 
 
 static sbh_status operator_apply(sbh_operator* op, const sbh_c64* x, sbh_c64* y, int memory, int mode) {
-    return guard([&] {
+    return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && x && y, "operator/x/y is null");
         check_memory(memory);
         if (mode != 0) require_dense(op, "the transpose/Hermitian product");
@@ -917,7 +917,7 @@ sbh_status sbh_operator_apply_hermitian(sbh_operator* op, const sbh_c64* x, sbh_
 }
 
 sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && X && Y, "operator/X/Y is null");
         SBH_REQUIRE(k >= 1, "block width must be positive");
         check_memory(memory);
@@ -947,7 +947,7 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
 }
 
 sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out && n >= 1, "invalid identity preconditioner arguments");
         sbh_precond* p = new sbh_precond();
         p->ctx = ctx;
@@ -959,7 +959,7 @@ sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out) 
 }
 
 sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out) {
-    return guard([&] {
+    return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && out, "operator/out is null");
         require_dense(op, "the Jacobi preconditioner");
         use_device(op->ctx);
@@ -985,7 +985,7 @@ void sbh_precond_destroy(sbh_precond* p) {
 }
 
 sbh_status sbh_precond_apply(sbh_precond* p, const sbh_c64* r, sbh_c64* y, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(p), [&] {
         SBH_REQUIRE(p && r && y, "preconditioner/r/y is null");
         check_memory(memory);
         use_device(p->ctx);
@@ -1003,7 +1003,7 @@ sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, c
                      const sbh_gmres_config* config, sbh_c64* x, int memory,
                      sbh_gmres_solution* solution) {
     sbh_status st = SBH_OK;
-    sbh_status g = guard([&] {
+    sbh_status g = guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && b && config && x && solution, "null argument");
         check_memory(memory);
         if (precond) SBH_REQUIRE(precond->n == op->n, "preconditioner size does not match the operator");
@@ -1022,11 +1022,13 @@ sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, c
 sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0, int64_t k,
                            const sbh_gmres_config* config, sbh_c64* X, int memory,
                            sbh_gmres_solution* solutions) {
-    return guard([&] {
+    return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && B && config && X && solutions, "null argument");
         SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
         check_memory(memory);
         require_dense(op, "block GMRES");
+        if (config->convergence_check_interval > 1 || config->min_restart != 0 || config->max_restart != 0)
+            throw Error(SBH_ERR_UNSUPPORTED, "block GMRES checks convergence every step with a fixed restart");
         const int64_t n = op->n;
         use_device(op->ctx);
         DeviceInput db(op->ctx, B, (size_t)(n * k), memory);
@@ -1045,7 +1047,7 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
 }
 
 sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix** out) {
-    return guard([&] {
+    return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys && out, "system/out is null");
         SBH_REQUIRE(format == SBH_LOWP_FP16 || format == SBH_LOWP_BF16, "format must be FP16 or BF16");
         use_device(sys->ctx);
@@ -1070,7 +1072,7 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
 }
 
 sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp) {
-    return guard([&] {
+    return guard_on(ctx_of(lp), [&] {
         SBH_REQUIRE(lp, "matrix is null");
         use_device(lp->ctx);
         lowp_convert(lp->ctx, lp->sys->d_A.ptr, lp->sys->lda, lp->n, lp->ld, lp->format, lp->d_A.ptr,
@@ -1088,7 +1090,7 @@ void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp) {
 }
 
 sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64* Y, int64_t k, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(lp), [&] {
         SBH_REQUIRE(lp && X && Y, "null argument");
         SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
         check_memory(memory);
@@ -1103,7 +1105,7 @@ sbh_status sbh_lowp_matrix_apply(sbh_lowp_matrix* lp, const sbh_c64* X, sbh_c64*
 }
 
 sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, double* row_inv, int64_t* ld) {
-    return guard([&] {
+    return guard_on(ctx_of(lp), [&] {
         SBH_REQUIRE(lp, "matrix is null");
         use_device(lp->ctx);
         SBH_HIP(hipStreamSynchronize(lp->ctx->stream));
@@ -1111,7 +1113,7 @@ sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, doubl
             // the device copy is in MFMA fragment order (32-row tiles x 16 halves, 1 KB each);
             // hand it back row-major, n x 2*ld halves
             std::vector<uint16_t> packed(lp->d_A.count);
-            SBH_HIP(hipMemcpy(packed.data(), lp->d_A.ptr, lp->d_A.bytes(), hipMemcpyDeviceToHost));
+            copy_on_stream(lp->ctx, packed.data(), lp->d_A.ptr, lp->d_A.bytes(), hipMemcpyDeviceToHost);
             const int64_t steps = lp->ld / 8;
             for (int64_t row = 0; row < lp->n; ++row) {
                 const int64_t rt = row / 32, r = row % 32;
@@ -1122,14 +1124,14 @@ sbh_status sbh_lowp_matrix_download(sbh_lowp_matrix* lp, uint16_t* halves, doubl
                     }
             }
         }
-        if (row_inv) SBH_HIP(hipMemcpy(row_inv, lp->d_row_inv.ptr, lp->d_row_inv.bytes(), hipMemcpyDeviceToHost));
+        if (row_inv) copy_on_stream(lp->ctx, row_inv, lp->d_row_inv.ptr, lp->d_row_inv.bytes(), hipMemcpyDeviceToHost);
         if (ld) *ld = lp->ld;
     });
 }
 
 sbh_status sbh_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const sbh_c64* B, int64_t k,
                         const sbh_gmres_ir_config* config, sbh_c64* X, int memory, sbh_gmres_ir_status* status) {
-    return guard([&] {
+    return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && lp && B && config && X && status, "null argument");
         require_dense(op, "GMRES-IR");
         SBH_REQUIRE(lp->sys == op->sys, "the low-precision copy belongs to another system");
@@ -1158,7 +1160,7 @@ sbh_status sbh_debug_subelements(const double* source_point, const double* eleme
 }
 
 sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, double* gbs) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && gbs && bytes >= 16 && reps >= 1, "invalid probe arguments");
         use_device(ctx);
         DeviceArray<double2> a(bytes / 16);
@@ -1169,7 +1171,7 @@ sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, do
 }
 
 sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && tflops, "invalid probe arguments");
         use_device(ctx);
         DeviceArray<double> sink(1);
@@ -1178,7 +1180,7 @@ sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops) {
 }
 
 sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && bytes >= 16 && reps >= 1, "invalid copy arguments");
         use_device(ctx);
         const int64_t n = bytes / 16;
@@ -1190,29 +1192,29 @@ sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps) {
 }
 
 sbh_status sbh_debug_device_sincos(sbh_context* ctx, const double* x, int64_t n, double* s, double* c) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && x && s && c && n >= 0, "null argument");
         use_device(ctx);
         DeviceArray<double> dx(n), ds(n), dc(n);
-        if (n) SBH_HIP(hipMemcpy(dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (n) copy_on_stream(ctx, dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice);
         launch_sincos_probe(ctx, dx.ptr, n, ds.ptr, dc.ptr);
         SBH_HIP(hipStreamSynchronize(ctx->stream));
         if (n) {
-            SBH_HIP(hipMemcpy(s, ds.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(c, dc.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+            copy_on_stream(ctx, s, ds.ptr, sizeof(double) * n, hipMemcpyDeviceToHost);
+            copy_on_stream(ctx, c, dc.ptr, sizeof(double) * n, hipMemcpyDeviceToHost);
         }
     });
 }
 
 sbh_status sbh_debug_device_rsqrt(sbh_context* ctx, const double* x, int64_t n, double* y) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && x && y && n >= 0, "null argument");
         use_device(ctx);
         DeviceArray<double> dx(n), dy(n);
-        if (n) SBH_HIP(hipMemcpy(dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (n) copy_on_stream(ctx, dx.ptr, x, sizeof(double) * n, hipMemcpyHostToDevice);
         launch_rsqrt_probe(ctx, dx.ptr, n, dy.ptr);
         SBH_HIP(hipStreamSynchronize(ctx->stream));
-        if (n) SBH_HIP(hipMemcpy(y, dy.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (n) copy_on_stream(ctx, y, dy.ptr, sizeof(double) * n, hipMemcpyDeviceToHost);
     });
 }
 

@@ -89,6 +89,10 @@ struct sbh_batch {
 };
 
 namespace sbh {
+inline sbh_context* ctx_of(const sbh_batch* b) { return b ? b->ctx : nullptr; }
+}  // namespace sbh
+
+namespace sbh {
 
 namespace {
 
@@ -97,9 +101,9 @@ inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
 constexpr int kMaxCol = 16;
 
 template <typename T>
-void upload(DeviceArray<T>& d, const std::vector<T>& h) {
+void upload(sbh_context* ctx, DeviceArray<T>& d, const std::vector<T>& h) {
     d.allocate(h.size());
-    if (!h.empty()) SBH_HIP(hipMemcpy(d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!h.empty()) copy_on_stream(ctx, d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
 void build_chunks(sbh_batch* b) {
@@ -126,8 +130,8 @@ void build_chunks(sbh_batch* b) {
                           return x.inst == y.inst && x.rows == y.rows && x.r0 == y.r0;
                       });
     if (same) return;  // the solver workspace and its captured graph stay valid
-    upload(b->d_chunks, chunks);
-    upload(b->d_chunk_range, range);
+    upload(b->ctx, b->d_chunks, chunks);
+    upload(b->ctx, b->d_chunk_range, range);
     b->chunk_host = std::move(chunks);
     b->nchunks = (int)b->chunk_host.size();
     b->solver.reset();  // partial-sum buffers are sized by the chunk count
@@ -148,7 +152,7 @@ void build_items(sbh_batch* b) {
     for (int i : size_order(b))
         for (int64_t r0 = 0; r0 < b->inst[i].n; r0 += R) items.push_back(make_int2(i, (int)r0));
     SBH_REQUIRE(items.size() < (size_t)INT32_MAX, "too many assembly items");
-    upload(b->d_items, items);
+    upload(b->ctx, b->d_items, items);
     b->nitems = (int)items.size();
 }
 
@@ -162,7 +166,7 @@ void build_tiles(sbh_batch* b) {
                       std::equal(tiles.begin(), tiles.end(), b->tile_host.begin(),
                                  [](const int2& x, const int2& y) { return x.x == y.x && x.y == y.y; });
     if (same) return;
-    upload(b->d_tiles, tiles);
+    upload(b->ctx, b->d_tiles, tiles);
     b->tile_host = std::move(tiles);
     b->ntiles = (int)b->tile_host.size();
     b->solver.reset();  // a captured graph holds the tile table's address
@@ -303,7 +307,7 @@ extern "C" {
 
 sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_mesh* const* meshes,
                             int route_threads, sbh_batch** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out, "ctx/out is null");
         SBH_REQUIRE(num_instances > 0 && num_instances < (1 << 24), "num_instances out of range");
         SBH_REQUIRE(meshes, "meshes is null");
@@ -357,9 +361,9 @@ sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_m
         int64_t off = 0;
         for (int i = 0; i < b->B; ++i) {
             const sbh_mesh* m = b->meshes[i];
-            SBH_HIP(hipMemcpy(b->d_far.ptr + off, m->d_far.ptr, m->n * sizeof(FarPanel), hipMemcpyDeviceToDevice));
-            SBH_HIP(hipMemcpy(b->d_src.ptr + off, m->d_src.ptr, m->n * sizeof(Source), hipMemcpyDeviceToDevice));
-            SBH_HIP(hipMemcpy(b->d_geom.ptr + off, m->d_geom.ptr, m->n * sizeof(PanelGeom), hipMemcpyDeviceToDevice));
+            copy_on_stream(b->ctx, b->d_far.ptr + off, m->d_far.ptr, m->n * sizeof(FarPanel), hipMemcpyDeviceToDevice);
+            copy_on_stream(b->ctx, b->d_src.ptr + off, m->d_src.ptr, m->n * sizeof(Source), hipMemcpyDeviceToDevice);
+            copy_on_stream(b->ctx, b->d_geom.ptr + off, m->d_geom.ptr, m->n * sizeof(PanelGeom), hipMemcpyDeviceToDevice);
             const int64_t pair_off = (int64_t)pairs.size();
             const int64_t leaf_off = (int64_t)leaves.size();
             for (NearPair p : rts[i].pairs) {
@@ -379,13 +383,13 @@ sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_m
             rts[i] = RouteTable();
         }
         row_ptr[panels] = (int64_t)pairs.size();
-        upload(b->d_pairs, pairs);
-        upload(b->d_leaves, leaves);
-        upload(b->d_row_ptr, row_ptr);
-        upload(b->d_row_inst, row_inst);
+        upload(b->ctx, b->d_pairs, pairs);
+        upload(b->ctx, b->d_leaves, leaves);
+        upload(b->ctx, b->d_row_ptr, row_ptr);
+        upload(b->ctx, b->d_row_inst, row_inst);
         b->d_counter.allocate(2);
         b->d_t0.allocate(1);
-        SBH_HIP(hipMemset(b->d_t0.ptr, 0, sizeof(unsigned long long)));
+        zero_on_stream(b->ctx, b->d_t0.ptr, sizeof(unsigned long long));
         int cus = 0;
         SBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         b->cu_count = std::max(cus, 1);
@@ -405,7 +409,7 @@ void sbh_batch_destroy(sbh_batch* b) {
 
 sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const sbh_c64* beta,
                                  const int32_t* num_rhs, const double* directions, double sign) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         SBH_REQUIRE(b && physics && beta && num_rhs && directions, "null argument");
         use_device(b->ctx);
         SBH_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -456,17 +460,17 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
         auto put = [&](auto& d, const auto& h) {
             using T = typename std::decay_t<decltype(h)>::value_type;
             if (d.count != h.size()) {
-                upload(d, h);
+                upload(b->ctx, d, h);
                 b->solver.reset();
             } else if (!h.empty()) {
-                SBH_HIP(hipMemcpy(d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+                copy_on_stream(b->ctx, d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
             }
         };
         put(b->d_inst, b->inst);
         put(b->d_col_inst, b->col_inst);
         put(b->d_col_q, b->col_q);
         if (b->d_dirs.count != (size_t)dir_off * 3) b->d_dirs.allocate((size_t)dir_off * 3);
-        SBH_HIP(hipMemcpy(b->d_dirs.ptr, directions, (size_t)dir_off * 3 * sizeof(double), hipMemcpyHostToDevice));
+        copy_on_stream(b->ctx, b->d_dirs.ptr, directions, (size_t)dir_off * 3 * sizeof(double), hipMemcpyHostToDevice);
         if (b->d_A.count != (size_t)a_off) {
             b->d_A.allocate((size_t)a_off);
             b->solver.reset();
@@ -476,7 +480,7 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
             b->d_x.allocate((size_t)vec_off);
             b->solver.reset();
         }
-        SBH_HIP(hipMemset(b->d_x.ptr, 0, b->d_x.bytes()));
+        zero_on_stream(b->ctx, b->d_x.ptr, b->d_x.bytes());
         b->lp_format = -2;
         build_items(b);
         build_chunks(b);  // also drops the solver workspace and its graph
@@ -486,7 +490,7 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
 }
 
 sbh_status sbh_batch_assemble(sbh_batch* b) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         use_device(b->ctx);
         ragged_stamp(b->ctx, b->d_t0.ptr);
@@ -496,7 +500,7 @@ sbh_status sbh_batch_assemble(sbh_batch* b) {
 }
 
 sbh_status sbh_batch_convert(sbh_batch* b, int format) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         SBH_REQUIRE(format == 0 || format == 1, "format must be FP16 (0) or BF16 (1)");
         use_device(b->ctx);
@@ -512,7 +516,7 @@ sbh_status sbh_batch_convert(sbh_batch* b, int format) {
 }
 
 sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_batch_column_status* status) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         SBH_REQUIRE(cfg, "config is null");
         SBH_REQUIRE(cfg->format >= -1 && cfg->format <= 1, "format must be -1, 0 or 1");
@@ -593,14 +597,14 @@ sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_
             std::vector<double> rn(C), bn(C);
             std::vector<int> dn(C);
             std::vector<unsigned long long> td(C), t0(1);
-            SBH_HIP(hipMemcpy(it.data(), s.iters.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(cy.data(), s.cycles.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(ro.data(), s.reorth_count.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(rn.data(), s.rnorm.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(bn.data(), s.bnorm.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(dn.data(), s.done.ptr, C * 4, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(td.data(), s.t_done.ptr, C * 8, hipMemcpyDeviceToHost));
-            SBH_HIP(hipMemcpy(t0.data(), b->d_t0.ptr, 8, hipMemcpyDeviceToHost));
+            copy_on_stream(b->ctx, it.data(), s.iters.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, cy.data(), s.cycles.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, ro.data(), s.reorth_count.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, rn.data(), s.rnorm.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, bn.data(), s.bnorm.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, dn.data(), s.done.ptr, C * 4, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, td.data(), s.t_done.ptr, C * 8, hipMemcpyDeviceToHost);
+            copy_on_stream(b->ctx, t0.data(), b->d_t0.ptr, 8, hipMemcpyDeviceToHost);
             int rate_khz = 0;
             SBH_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
             for (size_t c = 0; c < C; ++c) {
@@ -621,7 +625,7 @@ sbh_status sbh_batch_solve(sbh_batch* b, const sbh_batch_solve_config* cfg, sbh_
 
 namespace {
 sbh_status download_block(sbh_batch* b, int64_t instance, sbh_c64* out, int what) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         SBH_REQUIRE(out && instance >= 0 && instance < b->B, "bad instance/out");
         use_device(b->ctx);
@@ -639,13 +643,13 @@ sbh_status download_block(sbh_batch* b, int64_t instance, sbh_c64* out, int what
             src = b->d_A.ptr + I.a_off;
             count = (size_t)I.n * I.n;
         }
-        SBH_HIP(hipMemcpy(out, src, count * sizeof(double2), hipMemcpyDeviceToHost));
+        copy_on_stream(b->ctx, out, src, count * sizeof(double2), hipMemcpyDeviceToHost);
     });
 }
 }  // namespace
 
 sbh_status sbh_batch_set_guess(sbh_batch* b, int64_t instance, const sbh_c64* x0) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         SBH_REQUIRE(x0 && instance >= 0 && instance < b->B, "bad instance/x0");
         use_device(b->ctx);
@@ -654,7 +658,7 @@ sbh_status sbh_batch_set_guess(sbh_batch* b, int64_t instance, const sbh_c64* x0
         const size_t count = (size_t)I.n * I.ncol;
         for (size_t i = 0; i < count; ++i)
             SBH_REQUIRE(std::isfinite(x0[i].re) && std::isfinite(x0[i].im), "initial guess is not finite");
-        SBH_HIP(hipMemcpy(b->d_x.ptr + I.vec_off, x0, count * sizeof(double2), hipMemcpyHostToDevice));
+        copy_on_stream(b->ctx, b->d_x.ptr + I.vec_off, x0, count * sizeof(double2), hipMemcpyHostToDevice);
     });
 }
 
@@ -670,7 +674,7 @@ sbh_status sbh_batch_download_matrix(sbh_batch* b, int64_t instance, sbh_c64* a)
 
 sbh_status sbh_batch_info(const sbh_batch* b, int64_t* num_instances, int64_t* num_columns, int64_t* total_panels,
                           int64_t* matrix_entries, double* route_ms) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         SBH_REQUIRE(b, "batch is null");
         if (num_instances) *num_instances = b->B;
         if (num_columns) *num_columns = b->C;
@@ -681,14 +685,14 @@ sbh_status sbh_batch_info(const sbh_batch* b, int64_t* num_instances, int64_t* n
 }
 
 sbh_status sbh_batch_get_tuning(const sbh_batch* b, sbh_batch_tuning* t) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         SBH_REQUIRE(b && t, "null argument");
         *t = b->tuning;
     });
 }
 
 sbh_status sbh_batch_set_tuning(sbh_batch* b, const sbh_batch_tuning* t) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         SBH_REQUIRE(b && t, "null argument");
         SBH_REQUIRE(t->rows_per_item >= 1 && t->rows_per_item <= 64, "rows_per_item must be in 1..64");
         SBH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 16, "blocks_per_cu must be in 1..16");
@@ -707,7 +711,7 @@ sbh_status sbh_batch_set_tuning(sbh_batch* b, const sbh_batch_tuning* t) {
 
 sbh_status sbh_batch_autotune(sbh_batch* b, double* table, int max_entries, int* num_entries,
                               sbh_batch_tuning* best) {
-    return guard([&] {
+    return guard_on(ctx_of(b), [&] {
         check_problem(b);
         sbh_context* ctx = b->ctx;
         use_device(ctx);

@@ -1,8 +1,8 @@
 // Restarted GMRES over device-resident vectors.
 //
 // Follows gmres_core (crates/sonium-solvers/crates/sonium-solvers-iterative/
-// src/gmres.rs:631-1057) with the convergence check at every inner step and
-// no periodic exact-residual checks (GmresConfig::default), then
+// src/gmres.rs:631-1057) — convergence checked every convergence_check_interval
+// inner steps, adaptive restart when min_restart and max_restart are set — then
 // finalize_gmres_solution (gmres.rs:159-250).  Left preconditioning,
 // classical Gram–Schmidt with one DGKS reorthogonalisation pass when the
 // projected norm drops below 0.707 of the unprojected one (gmres.rs:799-829),
@@ -125,10 +125,16 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
         sol->converged = conv ? 1 : 0;
         sol->status = status;
     };
-    const int m_cfg = (int)std::min<int64_t>(cfg.restart, std::max<int64_t>(n, 1));
-    const int m = std::max(1, m_cfg);
-    Krylov K(ctx, op, M, n, m);
-    DeviceArray<double2> V((size_t)(m + 1) * n), w(n), opb(n);
+    const int64_t nmax = std::max<int64_t>(n, 1);
+    const bool adaptive = cfg.min_restart > 0 && cfg.max_restart > 0;
+    int m = std::max(1, (int)std::min<int64_t>(cfg.restart, nmax));
+    // the basis is allocated for the largest restart the cycle can reach
+    const int alloc_m = adaptive ? std::max(m, (int)std::min<int64_t>(cfg.max_restart, nmax)) : m;
+    const int min_m = adaptive ? (int)std::min<int64_t>(cfg.min_restart, nmax) : m;
+    const int max_m = alloc_m;
+    const int64_t check_every = std::max<int64_t>(1, cfg.convergence_check_interval);
+    Krylov K(ctx, op, M, n, alloc_m);
+    DeviceArray<double2> V((size_t)(alloc_m + 1) * n), w(n), opb(n);
     // Initial guess.
     if (d_x0)
         SBH_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
@@ -139,7 +145,9 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
     double bnorm_check = K.norm(d_b);
     double xnorm_check = d_x0 ? K.norm(d_x0) : 0.0;
     if (cfg.max_iterations <= 0 || cfg.restart <= 0 || !std::isfinite(cfg.tolerance) ||
-        cfg.tolerance <= 0.0 || !std::isfinite(bnorm_check) || !std::isfinite(xnorm_check)) {
+        cfg.tolerance <= 0.0 || !std::isfinite(bnorm_check) || !std::isfinite(xnorm_check) ||
+        cfg.convergence_check_interval < 0 || cfg.exact_residual_check_interval < 0 ||
+        cfg.min_restart < 0 || cfg.max_restart < 0) {
         finish(0, 0, INFINITY, INFINITY, INFINITY, false, SBH_SOLVER_NUMERICAL_ERROR);
         sol->spmv_count = K.spmv;
         return SBH_OK;
@@ -151,10 +159,11 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
     const double true_b_norm = rhs_zero ? 1.0 : raw_b;
     K.precond(d_b, w.ptr);
     const double raw_pb = (!M || M->kind == 0) ? raw_b : K.norm(w.ptr);
-    int64_t total = 0, restarts = 0, reorth = 0;
+    int64_t total = 0, restarts = 0, reorth = 0, exact_checks = 0;
     auto done = [&]() {
         sol->spmv_count = K.spmv;
         sol->reorthogonalizations = reorth;
+        sol->exact_residual_checks = exact_checks;
         return SBH_OK;
     };
     if (!rhs_zero && raw_pb < tol_threshold) {
@@ -163,11 +172,13 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
     }
     const double pb_norm = rhs_zero ? 1.0 : raw_pb;
 
-    std::vector<cplx> H((size_t)(m + 1) * m), g(m + 1), cs, sn, coeffs, re;
-    auto h = [&](int i, int j) -> cplx& { return H[(size_t)i * m + j]; };
+    std::vector<cplx> H((size_t)(alloc_m + 1) * alloc_m), g(alloc_m + 1), cs, sn, coeffs, re;
+    auto h = [&](int i, int j) -> cplx& { return H[(size_t)i * alloc_m + j]; };
     auto vec = [&](int i) { return V.ptr + (size_t)i * n; };
 
+    // exact_relative_residual (gmres_common.rs:16-40)
     auto exact_rel = [&]() {
+        exact_checks += 1;
         K.apply(d_x, opb.ptr);
         double rn = K.residual(d_b, opb.ptr);
         return rn / std::max(true_b_norm, 2.2250738585072014e-308);
@@ -202,6 +213,7 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
             returned = true;
             break;
         }
+        const double rel_before_cycle = rel;
         if (!std::isfinite(beta) || beta == 0.0) {
             finish(total, restarts, true_rel, rel, true_rel, false, SBH_SOLVER_BREAKDOWN);
             returned = true;
@@ -267,7 +279,8 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
             double abs_res = cnorm(g[j + 1]);
             double rrel = abs_res / pb_norm;
             bool converged = rrel < tol || (!inner_converged && abs_res < 1e-20);
-            if (converged) {
+            const bool should_check = j % check_every == 0 || j == m - 1 || inner_converged;
+            if (should_check && converged) {
                 solve_update(j + 1);
                 double t = exact_rel();
                 if (t >= tol && t >= 1e-20) {
@@ -294,6 +307,14 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
             finish(total, restarts, t, t, t, true, SBH_SOLVER_CONVERGED);
             returned = true;
             break;
+        }
+        if (adaptive) {
+            // gmres.rs:1013-1029: stalling cycles double the restart, fast ones halve it
+            const double ratio = (cnorm(g[m]) / pb_norm) / rel_before_cycle;
+            if (ratio > 0.9 && m < max_m)
+                m = std::min(m * 2, max_m);
+            else if (ratio < 0.1 && m > min_m)
+                m = std::max(m / 2, min_m);
         }
         restarts += 1;
     }

@@ -203,7 +203,11 @@ struct sbh_context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool profiling = false;
-    std::mutex mu;
+    std::mutex mu;  // kernel-timing records
+    // Held by every entry point that enqueues work on `stream` or touches the
+    // shared scratch below (and by calls on objects made on this context), so
+    // concurrent calls from several host threads run one after another.
+    std::recursive_mutex op_mu;
     sbh::KernelTiming timing[SBH_KERNEL_CLASS_COUNT];
     // device scratch for partial sums of transposed products, grown on demand
     double2* scratch = nullptr;
@@ -302,6 +306,42 @@ struct sbh_precond {
 };
 
 namespace sbh {
+
+// Copies and fills ordered on the context's stream and waited for before
+// returning.  The stream is created non-blocking, so plain hipMemcpy and
+// hipMemset on the null stream would not be ordered with its kernels.
+inline void copy_on_stream(sbh_context* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (!bytes) return;
+    SBH_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    SBH_HIP(hipStreamSynchronize(ctx->stream));
+}
+inline void zero_on_stream(sbh_context* ctx, void* dst, size_t bytes) {
+    if (!bytes) return;
+    SBH_HIP(hipMemsetAsync(dst, 0, bytes, ctx->stream));
+    SBH_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+inline sbh_context* ctx_of(sbh_context* c) { return c; }
+inline sbh_context* ctx_of(const sbh_vector* v) { return v ? v->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_mesh* m) { return m ? m->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_workspace* w) { return w ? w->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_system* s) { return s ? s->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_operator* o) { return o ? o->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_precond* p) { return p ? p->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_lowp_matrix* l) { return l ? l->ctx : nullptr; }
+
+// guard() with the context's operation lock held and its device current on
+// the calling thread.  A null context runs f unlocked, so f reports its own
+// null-argument error.
+template <typename F>
+sbh_status guard_on(sbh_context* ctx, F&& f) {
+    if (!ctx) return guard(std::forward<F>(f));
+    std::lock_guard<std::recursive_mutex> lock(ctx->op_mu);
+    return guard([&] {
+        SBH_HIP(hipSetDevice(ctx->device));
+        f();
+    });
+}
 
 void context_retain(sbh_context* ctx);
 void context_release(sbh_context* ctx);

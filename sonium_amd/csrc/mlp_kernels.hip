@@ -278,6 +278,10 @@ struct sbh_mlp {
 };
 
 namespace sbh {
+inline sbh_context* ctx_of(const sbh_mlp* n) { return n ? n->ctx : nullptr; }
+}  // namespace sbh
+
+namespace sbh {
 namespace {
 
 constexpr int kSplitW = 16;  // split K (samples) of the weight-gradient products
@@ -410,7 +414,7 @@ using namespace sbh;
 extern "C" {
 
 sbh_status sbh_mlp_create(sbh_context* ctx, int32_t width, uint64_t seed, sbh_mlp** out) {
-    return guard([&] {
+    return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx && out, "ctx/out is null");
         SBH_REQUIRE(width >= 64 && width <= 4096 && width % 64 == 0, "width must be a multiple of 64 in 64..4096");
         use_device(ctx);
@@ -445,10 +449,10 @@ sbh_status sbh_mlp_create(sbh_context* ctx, int32_t width, uint64_t seed, sbh_ml
         net->grads.allocate((size_t)off);
         net->m1.allocate((size_t)off);
         net->m2.allocate((size_t)off);
-        SBH_HIP(hipMemcpy(net->params.ptr, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemset(net->grads.ptr, 0, net->grads.bytes()));
-        SBH_HIP(hipMemset(net->m1.ptr, 0, net->m1.bytes()));
-        SBH_HIP(hipMemset(net->m2.ptr, 0, net->m2.bytes()));
+        copy_on_stream(net->ctx, net->params.ptr, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice);
+        zero_on_stream(net->ctx, net->grads.ptr, net->grads.bytes());
+        zero_on_stream(net->ctx, net->m1.ptr, net->m1.bytes());
+        zero_on_stream(net->ctx, net->m2.ptr, net->m2.bytes());
         net->Wb.allocate((size_t)woff);
         net->WTb.allocate((size_t)woff);
         refresh_weights(net.get());
@@ -470,20 +474,20 @@ void sbh_mlp_destroy(sbh_mlp* net) {
 int64_t sbh_mlp_num_params(const sbh_mlp* net) { return net ? net->nparams : 0; }
 
 sbh_status sbh_mlp_get_params(sbh_mlp* net, float* out) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && out, "null argument");
         use_device(net->ctx);
         SBH_HIP(hipStreamSynchronize(net->ctx->stream));
-        SBH_HIP(hipMemcpy(out, net->params.ptr, net->params.bytes(), hipMemcpyDeviceToHost));
+        copy_on_stream(net->ctx, out, net->params.ptr, net->params.bytes(), hipMemcpyDeviceToHost);
     });
 }
 
 sbh_status sbh_mlp_set_params(sbh_mlp* net, const float* in) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && in, "null argument");
         use_device(net->ctx);
         SBH_HIP(hipStreamSynchronize(net->ctx->stream));
-        SBH_HIP(hipMemcpy(net->params.ptr, in, net->params.bytes(), hipMemcpyHostToDevice));
+        copy_on_stream(net->ctx, net->params.ptr, in, net->params.bytes(), hipMemcpyHostToDevice);
         refresh_weights(net);
         SBH_HIP(hipStreamSynchronize(net->ctx->stream));
     });
@@ -491,38 +495,38 @@ sbh_status sbh_mlp_set_params(sbh_mlp* net, const float* in) {
 
 sbh_status sbh_mlp_loss_and_gradients(sbh_mlp* net, const float* inputs, const float* targets, int64_t M,
                                       double* loss, float* grads) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && inputs && targets, "null argument");
         SBH_REQUIRE(M > 0 && M % 256 == 0 && M <= (1 << 24), "M must be a positive multiple of 256");
         use_device(net->ctx);
         ensure_batch(net, (int)M);
         DeviceArray<float> din((size_t)M * RAW), dout((size_t)M * 2);
-        SBH_HIP(hipMemcpy(din.ptr, inputs, din.bytes(), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemcpy(dout.ptr, targets, dout.bytes(), hipMemcpyHostToDevice));
+        copy_on_stream(net->ctx, din.ptr, inputs, din.bytes(), hipMemcpyHostToDevice);
+        copy_on_stream(net->ctx, dout.ptr, targets, dout.bytes(), hipMemcpyHostToDevice);
         SBH_HIP(hipMemsetAsync(net->grads.ptr, 0, net->grads.bytes(), net->ctx->stream));
         const int blocks = forward_backward(net, din.ptr, nullptr, (int)M, dout.ptr, true);
         const double l = read_loss(net, blocks) / (double)M;
         if (loss) *loss = l;
-        if (grads) SBH_HIP(hipMemcpy(grads, net->grads.ptr, net->grads.bytes(), hipMemcpyDeviceToHost));
+        if (grads) copy_on_stream(net->ctx, grads, net->grads.ptr, net->grads.bytes(), hipMemcpyDeviceToHost);
     });
 }
 
 sbh_status sbh_mlp_set_dataset(sbh_mlp* net, const float* inputs, const float* targets, int64_t count) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && inputs && targets && count > 0, "null argument or empty dataset");
         use_device(net->ctx);
         SBH_HIP(hipStreamSynchronize(net->ctx->stream));
         net->data_in.allocate((size_t)count * RAW);
         net->data_out.allocate((size_t)count * 2);
-        SBH_HIP(hipMemcpy(net->data_in.ptr, inputs, net->data_in.bytes(), hipMemcpyHostToDevice));
-        SBH_HIP(hipMemcpy(net->data_out.ptr, targets, net->data_out.bytes(), hipMemcpyHostToDevice));
+        copy_on_stream(net->ctx, net->data_in.ptr, inputs, net->data_in.bytes(), hipMemcpyHostToDevice);
+        copy_on_stream(net->ctx, net->data_out.ptr, targets, net->data_out.bytes(), hipMemcpyHostToDevice);
         net->data_count = count;
     });
 }
 
 sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_max, double lr_min, uint64_t seed,
                          double* losses) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net, "net is null");
         SBH_REQUIRE(net->data_count > 0, "sbh_mlp_set_dataset has not been called");
         SBH_REQUIRE(batch > 0 && batch % 256 == 0 && batch <= (1 << 20), "batch must be a multiple of 256");
@@ -553,7 +557,7 @@ sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_m
 }
 
 sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory) {
-    return guard([&] {
+    return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && inputs && out, "null argument");
         SBH_REQUIRE(M > 0 && M <= (1 << 24), "M out of range");
         use_device(net->ctx);

@@ -144,3 +144,43 @@ def test_gmres_restart_cycles_match_oracle(c1):
     assert sol.restarts == st["restarts"]
     assert abs(sol.iterations - st["iterations"]) <= 1
     assert rel_l2(sol.x, x_ref) <= 1e-7
+
+
+@pytest.mark.parametrize("interval", [0, 4])
+def test_gmres_convergence_check_interval_matches_oracle(c1, interval):
+    """GmresConfig::convergence_check_interval (gmres.rs:871-876)."""
+    op = sa.DenseMatrixOperator(c1["sys"])
+    cfg = sa.GmresConfig(max_iterations=10, restart=50, tolerance=1e-8, convergence_check_interval=interval,
+                         exact_residual_check_interval=3)
+    sol = sa.gmres(op, c1["b"], cfg)
+    x_ref, st = oracle.gmres(c1["A"], c1["b"], restart=50, max_iterations=10, tol=1e-8,
+                             convergence_check_interval=interval)
+    assert sol.converged and st["converged"]
+    assert sol.iterations == st["iterations"] and sol.restarts == st["restarts"]
+    assert rel_l2(sol.x, x_ref) <= 1e-8
+    # every converged return is confirmed by one recomputed residual whatever the interval
+    assert sol.exact_residual_checks == 1
+
+
+def test_gmres_adaptive_restart_matches_oracle(c1):
+    """min_restart / max_restart (gmres.rs:1013-1029): a short restart on C1
+    stalls and grows; the cycle lengths, and so the step count, match."""
+    op = sa.DenseMatrixOperator(c1["sys"])
+    cfg = sa.GmresConfig(max_iterations=30, restart=4, tolerance=1e-9, min_restart=2, max_restart=40)
+    sol = sa.gmres(op, c1["b"], cfg)
+    x_ref, st = oracle.gmres(c1["A"], c1["b"], restart=4, max_iterations=30, tol=1e-9, min_restart=2,
+                             max_restart=40)
+    _, fixed = oracle.gmres(c1["A"], c1["b"], restart=4, max_iterations=30, tol=1e-9)
+    print("adaptive", sol.iterations, sol.restarts, "oracle", st["iterations"], st["restarts"], "fixed",
+          fixed["iterations"], fixed["restarts"], fixed["converged"])
+    assert sol.converged == st["converged"]
+    assert sol.restarts == st["restarts"] and abs(sol.iterations - st["iterations"]) <= 1
+    assert st["iterations"] != fixed["iterations"]
+    if st["converged"]:
+        assert rel_l2(sol.x, x_ref) <= 1e-7
+
+
+def test_gmres_rejects_negative_options(c1):
+    op = sa.DenseMatrixOperator(c1["sys"])
+    sol = sa.gmres(op, c1["b"], sa.GmresConfig(10, 10, 1e-8, convergence_check_interval=-1))
+    assert sol.status == "numerical_error" and sol.iterations == 0
