@@ -75,6 +75,10 @@ def lib():
                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, _dp, _dp]
         L.oracle_gmres_ex.restype = ctypes.c_int
+        L.oracle_block_gmres.argtypes = [_dp, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _dp, _dp]
+        L.oracle_block_gmres.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -397,3 +401,28 @@ def gmres(A, b, x0=None, *, restart=50, max_iterations=100, tol=1e-8, jacobi=Fal
              "estimated_residual": st[3], "backward_error": st[4], "converged": bool(st[5]),
              "status": STATUS[int(st[6])], "spmv_count": int(st[7])}
     return x, stats
+
+
+def block_gmres(A, B, X0=None, *, restart=30, max_iterations=100, tol=1e-6, true_residual_check=True,
+                exact_residual_check_interval=0, jacobi=False, threads=None):
+    """Restated block_gmres_preconditioned_with_guess (block_gmres.rs:163-215,
+    666-1120): one shared block Krylov basis with rank-drop deflation.  B is
+    n x s; returns (X, stats) with per-column projected and true residuals."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1)
+    n, s = B.shape
+    X = np.zeros((n, s), dtype=np.complex128)
+    st = np.zeros(2 + 3 * s)
+    x0p = None
+    if X0 is not None:
+        X0 = np.ascontiguousarray(X0, dtype=np.complex128).reshape(n, s)
+        x0p = _p(X0.view(np.float64))
+    lib().oracle_block_gmres(_p(A.view(np.float64)), n, _p(B.view(np.float64)), x0p, s, int(restart),
+                             int(max_iterations), float(tol), 1 if true_residual_check else 0,
+                             int(exact_residual_check_interval), 1 if jacobi else 0, threads or default_threads(),
+                             _p(X.view(np.float64)), _p(st))
+    stats = {"iterations": int(st[0]), "converged": bool(st[1]), "residuals": st[2::3].copy(),
+             "true_residuals": st[3::3].copy(), "backward_errors": st[4::3].copy()}
+    return X, stats
