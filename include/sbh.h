@@ -307,6 +307,19 @@ sbh_status sbh_operator_apply_hermitian(sbh_operator* op, const sbh_c64* x, sbh_
 sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k,
                                     int memory);
 
+/* Allocation-free variant of sbh_operator_apply_block.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Dense operator only; X and Y are device pointers (n x k row-major).  The
+ * split-K partial sums of the block product live in a fixed 64 MiB device
+ * array per GPU instead of the context's growable scratch, so the call makes
+ * no hipMalloc/hipFree: it needs 8 * n * 16 * t doubles with t = 1, 2, 4, 8
+ * for min(k, 64) up to 8, 16, 32, 64 columns (n <= 8192 at 64 columns), and
+ * larger shapes are rejected with SBH_ERR_INVALID_ARGUMENT.  k = 1 uses the
+ * matrix-vector kernel (no scratch).  Calls that overlap on one GPU (several
+ * contexts or threads on one device) race on the shared array. */
+sbh_status sbh_operator_apply_block_noalloc(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k);
+
 /* ---------------------------------------------------------------------------
  * Preconditioner<Complex64> (traits.rs:3645-3780): identity
  * (IdentityPreconditioner) and the inverse diagonal of a dense operator. */

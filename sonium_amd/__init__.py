@@ -663,6 +663,22 @@ class DenseMatrixOperator:
     def apply_device(self, x: DeviceVector, y: DeviceVector) -> None:
         _check(_lib().sbh_operator_apply(self._h, ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr), MEM_DEVICE))
 
+    def apply_block_device(self, X: DeviceVector, Y: DeviceVector, k: int) -> None:
+        """Y = A X for device-resident n x k row-major blocks."""
+        if X.length < self.num_cols() * k or Y.length < self.num_rows() * k:
+            raise ValueError("device blocks are shorter than n x k")
+        _check(_lib().sbh_operator_apply_block(self._h, ctypes.c_void_p(X.ptr), ctypes.c_void_p(Y.ptr), int(k),
+                                               MEM_DEVICE))
+
+    def apply_block_noalloc(self, X: DeviceVector, Y: DeviceVector, k: int) -> None:
+        """Y = A X like apply_block_device, with no device allocation (the
+        split-K partials use the library's fixed per-GPU scratch)."""
+        if X.length < self.num_cols() * k or Y.length < self.num_rows() * k:
+            raise ValueError("device blocks are shorter than n x k")
+        _check(_lib().sbh_operator_apply_block_noalloc(self._h, ctypes.c_void_p(X.ptr), ctypes.c_void_p(Y.ptr),
+                                                       int(k)))
+
+
     def close(self):
         if getattr(self, "_h", None):
             _lib().sbh_operator_destroy(self._h)

@@ -173,6 +173,7 @@ SIGNATURES = {
     "sbh_operator_apply_transpose": (_int, [_vp, _vp, _vp, _int]),
     "sbh_operator_apply_hermitian": (_int, [_vp, _vp, _vp, _int]),
     "sbh_operator_apply_block": (_int, [_vp, _vp, _vp, _i64, _int]),
+    "sbh_operator_apply_block_noalloc": (_int, [_vp, _vp, _vp, _i64]),
     "sbh_precond_identity": (_int, [_vp, _i64, _pp]),
     "sbh_precond_jacobi": (_int, [_vp, _pp]),
     "sbh_precond_destroy": (None, [_vp]),

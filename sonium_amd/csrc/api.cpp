@@ -220,6 +220,7 @@ sbh_status sbh_context_create(int device, sbh_context** out) {
         SBH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->pinned_rw(1 << 16);
         upload_quadrature_tables(ctx.get());
+        noalloc_scratch_init(device);
         *out = ctx.release();
     });
 }
@@ -943,6 +944,31 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
             zgemm_f64(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, in.ptr, k, outv.ptr);
         }
         outv.finish();
+    });
+}
+
+sbh_status sbh_operator_apply_block_noalloc(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k) {
+    return guard_on(ctx_of(op), [&] {
+        SBH_REQUIRE(op && X && Y, "operator/X/Y is null");
+        SBH_REQUIRE(k >= 1, "block width must be positive");
+        if (op->exact())
+            throw Error(SBH_ERR_UNSUPPORTED, "the allocation-free block product needs a dense (stored) operator");
+        const int64_t n = op->n;
+        if (k >= 2) {
+            const size_t need = noalloc_block_scratch_need(n, k);
+            if (need > NOALLOC_SCRATCH_DOUBLES)
+                throw Error(SBH_ERR_INVALID_ARGUMENT,
+                            "n x k too large for the allocation-free block product: its partial sums need " +
+                                std::to_string(need * sizeof(double) >> 20) + " MiB, the fixed scratch holds " +
+                                std::to_string(NOALLOC_SCRATCH_DOUBLES * sizeof(double) >> 20) + " MiB");
+        }
+        const double2* x = reinterpret_cast<const double2*>(X);
+        double2* y = reinterpret_cast<double2*>(Y);
+        if (k == 1)
+            zgemv_n_untimed(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, n, x, y);
+        else
+            zgemm_f64_noalloc(op->ctx, op->sys->d_A.ptr, op->sys->lda, n, x, k, y);
+        SBH_HIP(hipStreamSynchronize(op->ctx->stream));
     });
 }
 

@@ -316,6 +316,12 @@ static void zgemv_n_strided(sbh_context* ctx, const double2* A, int64_t lda, int
     SBH_HIP(hipGetLastError());
 }
 
+void zgemv_n_untimed(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols,
+                     const double2* x, double2* y) {
+    if (rows == 0) return;
+    zgemv_n_strided(ctx, A, lda, rows, cols, x, 1, y, 1);
+}
+
 void zgemv_t(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols,
              const double2* x, double2* y, bool conjugate) {
     if (cols == 0) return;

@@ -437,6 +437,17 @@ void extract_inverse_diagonal(sbh_context* ctx, const double2* A, int64_t lda, i
 void device_copy(sbh_context* ctx, const double2* src, double2* dst, int64_t n);
 // partials of ||x||^2
 int norm_partials(sbh_context* ctx, const double2* x, int64_t n, double* out);
+// Allocation-free block product (lowp_kernels.hip): the split-K partials sit
+// in a fixed device array of NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) per GPU,
+// whose address noalloc_scratch_init takes once per device.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)8 << 20;
+void noalloc_scratch_init(int device);
+size_t noalloc_block_scratch_need(int64_t n, int64_t k);
+void zgemm_f64_noalloc(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k,
+                       double2* Y);
+// y = A x without kernel-timing events (blas_kernels.hip)
+void zgemv_n_untimed(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols, const double2* x,
+                     double2* y);
 // Low-precision conversion (lowp_kernels.hip)
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
                   uint16_t* out, double* row_inv);

@@ -858,9 +858,18 @@ SBH_LOWP_PRODUCTS(double2)
 SBH_LOWP_PRODUCTS(float2)
 #undef SBH_LOWP_PRODUCTS
 
-void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y) {
-    hipEvent_t ev;
-    ctx->begin_timed(SBH_KERNEL_BLOCK, &ev);
+// Split-K partials of one 64-column chunk: ZG_SPLIT x n x nr doubles.
+static size_t zgemm_partial_doubles(int64_t n, int64_t k) {
+    const int kc = (int)std::min<int64_t>(64, k);
+    const int nt = kc <= 8 ? 1 : kc <= 16 ? 2 : kc <= 32 ? 4 : 8;
+    return (size_t)ZG_SPLIT * (size_t)n * (size_t)(16 * nt);
+}
+
+// Y = A X in 64-column chunks; part (null: the context's scratch, grown on
+// demand) holds the split-K partials of one chunk and is fully rewritten by
+// every chunk.
+static void zgemm_f64_into(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k,
+                           double2* Y, double* part) {
     const int64_t kslice = ((n + ZG_SPLIT - 1) / ZG_SPLIT + ZG_BKC - 1) / ZG_BKC * ZG_BKC;
     const int64_t row_tiles = (n + ZG_BM - 1) / ZG_BM;
     SBH_REQUIRE(row_tiles * ZG_SPLIT < ((int64_t)1 << 31), "block product too large");
@@ -868,22 +877,66 @@ void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const
         const int kc = (int)std::min<int64_t>(64, k - c0);
         const int nt = kc <= 8 ? 1 : kc <= 16 ? 2 : kc <= 32 ? 4 : 8;
         const int nr = 16 * nt;
-        double* part = reinterpret_cast<double*>(ctx->device_scratch((size_t)ZG_SPLIT * n * nr / 2));
+        double* chunk_part = part ? part : reinterpret_cast<double*>(ctx->device_scratch((size_t)ZG_SPLIT * n * nr / 2));
         const dim3 grid((unsigned)(row_tiles * ZG_SPLIT));
 #define SBH_ZG(NT_)                                                                                           \
     hipLaunchKernelGGL(zgemm_f64_kernel<NT_>, grid, dim3(BLOCK), 0, ctx->stream, A, lda, n, X + c0, k, kc, kslice, \
-                       part)
+                       chunk_part)
         if (nt == 1) SBH_ZG(1);
         else if (nt == 2) SBH_ZG(2);
         else if (nt == 4) SBH_ZG(4);
         else SBH_ZG(8);
 #undef SBH_ZG
         SBH_HIP(hipGetLastError());
-        hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(nblocks(n * kc)), dim3(BLOCK), 0, ctx->stream, part, n, nr, kc,
-                           Y + c0, k);
+        hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(nblocks(n * kc)), dim3(BLOCK), 0, ctx->stream, chunk_part, n, nr,
+                           kc, Y + c0, k);
         SBH_HIP(hipGetLastError());
     }
+}
+
+void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y) {
+    hipEvent_t ev;
+    ctx->begin_timed(SBH_KERNEL_BLOCK, &ev);
+    zgemm_f64_into(ctx, A, lda, n, X, k, Y, nullptr);
     ctx->end_timed(SBH_KERNEL_BLOCK, ev);
+}
+
+// ---- allocation-free block product (sbh_operator_apply_block_noalloc) ----
+// The split-K partials live in a fixed device array instead of the context's
+// scratch.  Host code never passes the array's name to a kernel: its device
+// address on each GPU comes from hipGetSymbolAddress, taken once per device
+// at context set-up with that device current.  Every call on one GPU shares
+// the array; calls that overlap on one GPU race on it.  Each chunk's product
+// rewrites every partial it reads, so nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    SBH_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, "device index beyond the allocation-free scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    SBH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SBH_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned no address for the block-product scratch");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+size_t noalloc_block_scratch_need(int64_t n, int64_t k) { return zgemm_partial_doubles(n, k); }
+
+void zgemm_f64_noalloc(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k,
+                       double2* Y) {
+    double* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ptr = (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    if (!ptr) throw Error(SBH_ERR_BACKEND, "allocation-free scratch was not set up on this device");
+    zgemm_f64_into(ctx, A, lda, n, X, k, Y, ptr);
 }
 
 // Rows per block of the column-batched reductions: one pass of BT rows per
