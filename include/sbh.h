@@ -382,13 +382,54 @@ sbh_status sbh_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, c
                      const sbh_gmres_config* config, sbh_c64* x, int memory,
                      sbh_gmres_solution* solution);
 
-/* Many right-hand sides — BemSolver::solve_many (bem_solver.rs:2032,
- * 2175-2181) over apply_block: B, X0 (NULL = zero) and X are n x k row-major
- * (1 <= k <= 64); solutions[k].  Each column follows gmres_core exactly;
- * the columns share one block product per Arnoldi step. */
+/* Many right-hand sides, column by column: B, X0 (NULL = zero) and X are
+ * n x k row-major (1 <= k <= 64); solutions[k].  Each column follows
+ * gmres_core exactly (the single-RHS GMRES of gmres.rs:631-1057); the columns
+ * share one block product per Arnoldi step (apply_block, traits.rs:3000-3017).
+ * The reference's own multi-RHS solver is the shared-basis block method,
+ * sbh_block_gmres below. */
 sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0, int64_t k,
                            const sbh_gmres_config* config, sbh_c64* X, int memory,
                            sbh_gmres_solution* solutions);
+
+/* ---------------------------------------------------------------------------
+ * Block GMRES with one shared block Krylov basis — BlockGmresConfig /
+ * BlockGmresSolution and try_block_gmres_preconditioned(_with_guess)
+ * (crates/sonium-solvers/crates/sonium-solvers-iterative/src/block_gmres.rs:
+ * 58-148, 163-330, 666-1120), the solver BemSolver::solve_many routes dense
+ * multi-RHS solves to (bem_solver.rs:2175-2181).  One block Arnoldi
+ * recurrence on K_m(A, R0), block CGS with the DGKS trigger on the block
+ * Frobenius norm, per-column CGS2 QR of each new block with rank-drop
+ * deflation at sqrt(eps), a block-Hessenberg Givens least-squares solve
+ * after every block step, and per-RHS true residuals at exit. */
+typedef struct {
+    int64_t max_iterations;               /* restart cycles (default 100) */
+    int64_t restart;                      /* block Arnoldi steps per cycle (default 30) */
+    double tolerance;                     /* per right-hand side (default 1e-6) */
+    int32_t true_residual_check;          /* gate `converged` on the true residuals (default 1) */
+    int32_t reserved;
+    int64_t exact_residual_check_interval; /* true-residual checkpoint every N block steps (0 = off) */
+} sbh_block_gmres_config;
+
+typedef struct {
+    int64_t iterations;            /* total block Arnoldi steps */
+    int32_t converged;
+    int32_t status;                /* sbh_solver_status */
+    int64_t spmv_count;            /* KrylovOpCounts::spmv_count (columns multiplied) */
+    int64_t exact_residual_checks; /* KrylovOpCounts::exact_residual_checks */
+    int64_t restart_cycles;
+    int64_t max_basis;             /* largest basis reached (vectors) */
+    int64_t deflated;              /* directions dropped by rank-drop deflation */
+} sbh_block_gmres_info;
+
+/* B, X0 (NULL = zero) and X are n x k row-major (1 <= k <= 64) in `memory`;
+ * precond NULL = none (identity).  residuals, true_residuals and
+ * backward_errors (k each, may be NULL) receive BlockGmresSolution's
+ * per-RHS vectors.  Invalid configurations, non-finite B or X0 and
+ * non-finite results are errors, as in the checked reference boundary. */
+sbh_status sbh_block_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* B, const sbh_c64* X0, int64_t k,
+                           const sbh_block_gmres_config* config, sbh_c64* X, int memory, sbh_block_gmres_info* info,
+                           double* residuals, double* true_residuals, double* backward_errors);
 
 /* ---------------------------------------------------------------------------
  * Mixed precision.  A complex FP16 or BF16 copy of a system matrix, every row

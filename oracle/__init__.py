@@ -17,7 +17,28 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_LIB_V3_PATH = os.path.join(_HERE, "_build", "liboracle_x86_64_v3.so")
 _lib = None
+# instructions that -march=x86-64-v3 may use
+_V3_FLAGS = {"avx", "avx2", "bmi1", "bmi2", "f16c", "fma", "abm", "movbe", "xsave"}
+
+
+def host_supports_x86_64_v3() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return _V3_FLAGS <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def library_path() -> str:
+    """The AVX2 build where the host has it and it was built, else the baseline build."""
+    if os.path.exists(_LIB_V3_PATH) and host_supports_x86_64_v3():
+        return _LIB_V3_PATH
+    return _LIB_PATH
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -33,7 +54,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(library_path())
         L.oracle_gauss_legendre.argtypes = [ctypes.c_int, _dp, _dp]
         L.oracle_triangle_quadrature.argtypes = [ctypes.c_int, _dp]
         L.oracle_triangle_quadrature.restype = ctypes.c_int

@@ -678,7 +678,6 @@ class DenseMatrixOperator:
         _check(_lib().sbh_operator_apply_block_noalloc(self._h, ctypes.c_void_p(X.ptr), ctypes.c_void_p(Y.ptr),
                                                        int(k)))
 
-
     def close(self):
         if getattr(self, "_h", None):
             _lib().sbh_operator_destroy(self._h)
@@ -859,6 +858,90 @@ def gmres_block(op: DenseMatrixOperator, B, config: GmresConfig, X0=None):
     _check(_lib().sbh_gmres_block(op.handle, _cptr(Bv), None if X0v is None else _cptr(X0v), k,
                                   ctypes.byref(cfg), _cptr(X), MEM_HOST, raws))
     return X, [_solution(raws[c], X[:, c]) for c in range(k)]
+
+
+@dataclass
+class BlockGmresConfig:
+    """BlockGmresConfig (block_gmres.rs:58-89); defaults of BlockGmresConfig<f64>::default()."""
+
+    max_iterations: int = 100
+    restart: int = 30
+    tolerance: float = 1e-6
+    true_residual_check: bool = True
+    exact_residual_check_interval: int = 0
+
+    def _c(self):
+        return _ffi.block_gmres_config_t(int(self.max_iterations), int(self.restart), float(self.tolerance),
+                                         1 if self.true_residual_check else 0, 0,
+                                         int(self.exact_residual_check_interval))
+
+
+@dataclass
+class BlockGmresSolution:
+    """BlockGmresSolution (block_gmres.rs:117-148): x is n x k (None for
+    device-resident solves), iterations counts block Arnoldi steps."""
+
+    x: Optional[np.ndarray]
+    iterations: int
+    residuals: np.ndarray
+    true_residuals: np.ndarray
+    backward_errors: np.ndarray
+    converged: bool
+    status: str
+    spmv_count: int
+    exact_residual_checks: int
+    restart_cycles: int
+    max_basis: int
+    deflated: int
+
+
+def _block_solution(info, x, res, tres, berr) -> BlockGmresSolution:
+    return BlockGmresSolution(x=x, iterations=int(info.iterations), residuals=res, true_residuals=tres,
+                              backward_errors=berr, converged=bool(info.converged),
+                              status=SOLVER_STATUS[int(info.status)], spmv_count=int(info.spmv_count),
+                              exact_residual_checks=int(info.exact_residual_checks),
+                              restart_cycles=int(info.restart_cycles), max_basis=int(info.max_basis),
+                              deflated=int(info.deflated))
+
+
+def block_gmres_preconditioned_with_guess(op: DenseMatrixOperator, B, precond=None, X0=None,
+                                          config: Optional[BlockGmresConfig] = None) -> BlockGmresSolution:
+    """Shared-basis block GMRES (block_gmres.rs:163-215): B is n x k (k <= 64),
+    host arrays; precond None = no preconditioner."""
+    config = config or BlockGmresConfig()
+    n = op.num_rows()
+    Bv = np.ascontiguousarray(B, dtype=np.complex128)
+    if Bv.ndim == 1:
+        Bv = Bv.reshape(n, 1)
+    k = Bv.shape[1]
+    X = np.zeros_like(Bv)
+    X0v = None if X0 is None else np.ascontiguousarray(X0, dtype=np.complex128).reshape(n, k)
+    info = _ffi.block_gmres_info_t()
+    res, tres, berr = (np.zeros(k) for _ in range(3))
+    _check(_lib().sbh_block_gmres(op.handle, None if precond is None else precond._h, _cptr(Bv),
+                                  None if X0v is None else _cptr(X0v), k, ctypes.byref(config._c()), _cptr(X),
+                                  MEM_HOST, ctypes.byref(info), _cptr(res), _cptr(tres), _cptr(berr)))
+    return _block_solution(info, X, res, tres, berr)
+
+
+def block_gmres(op: DenseMatrixOperator, B, config: Optional[BlockGmresConfig] = None) -> BlockGmresSolution:
+    """block_gmres (block_gmres.rs:149-161)."""
+    return block_gmres_preconditioned_with_guess(op, B, None, None, config)
+
+
+def block_gmres_device(op: DenseMatrixOperator, B: DeviceVector, k: int, X: DeviceVector,
+                       config: Optional[BlockGmresConfig] = None, X0: Optional[DeviceVector] = None,
+                       precond=None) -> BlockGmresSolution:
+    """Shared-basis block GMRES on device-resident row-major n x k blocks."""
+    config = config or BlockGmresConfig()
+    _block_vectors(op, B, X, k, X0)
+    info = _ffi.block_gmres_info_t()
+    res, tres, berr = (np.zeros(k) for _ in range(3))
+    _check(_lib().sbh_block_gmres(op.handle, None if precond is None else precond._h, ctypes.c_void_p(B.ptr),
+                                  None if X0 is None else ctypes.c_void_p(X0.ptr), k, ctypes.byref(config._c()),
+                                  ctypes.c_void_p(X.ptr), MEM_DEVICE, ctypes.byref(info), _cptr(res), _cptr(tres),
+                                  _cptr(berr)))
+    return _block_solution(info, None, res, tres, berr)
 
 
 def _block_vectors(op, B: DeviceVector, X: DeviceVector, k: int, X0: Optional[DeviceVector] = None):

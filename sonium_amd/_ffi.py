@@ -69,6 +69,18 @@ class gmres_solution_t(ctypes.Structure):
                 ("reorthogonalizations", ctypes.c_int64), ("exact_residual_checks", ctypes.c_int64)]
 
 
+class block_gmres_config_t(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64), ("tolerance", ctypes.c_double),
+                ("true_residual_check", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("exact_residual_check_interval", ctypes.c_int64)]
+
+
+class block_gmres_info_t(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("converged", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("spmv_count", ctypes.c_int64), ("exact_residual_checks", ctypes.c_int64),
+                ("restart_cycles", ctypes.c_int64), ("max_basis", ctypes.c_int64), ("deflated", ctypes.c_int64)]
+
+
 class gmres_ir_config_t(ctypes.Structure):
     _fields_ = [("max_outer_iterations", ctypes.c_int64), ("outer_tolerance", ctypes.c_double),
                 ("max_inner_iterations", ctypes.c_int64), ("inner_tolerance", ctypes.c_double),
@@ -187,6 +199,8 @@ SIGNATURES = {
     "sbh_lowp_matrix_destroy": (None, [_vp]),
     "sbh_lowp_matrix_apply": (_int, [_vp, _vp, _vp, _i64, _int]),
     "sbh_lowp_matrix_download": (_int, [_vp, _vp, _vp, ctypes.POINTER(_i64)]),
+    "sbh_block_gmres": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.POINTER(block_gmres_config_t), _vp, _int,
+                               ctypes.POINTER(block_gmres_info_t), _vp, _vp, _vp]),
     "sbh_gmres_ir": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(gmres_ir_config_t), _vp, _int,
                             ctypes.POINTER(gmres_ir_status_t)]),
     "sbh_batch_create": (_int, [_vp, _i64, _vp, _int, _pp]),

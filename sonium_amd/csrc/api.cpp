@@ -11,6 +11,10 @@
 namespace sbh {
 sbh_status run_gmres_block(sbh_operator* op, const double2* d_B, double2* d_X, int k, const sbh_gmres_config& cfg,
                            bool x_zero, sbh_gmres_solution* sols);
+int bk_frob_partials(sbh_context* ctx, const double2* X, int64_t count, double* part);
+sbh_status run_block_gmres(sbh_operator* op, const sbh_precond* M, const double2* d_B, double2* d_X, int k,
+                           bool x_zero, const sbh_block_gmres_config& cfg, sbh_block_gmres_info* info,
+                           double* residuals, double* true_residuals, double* backward_errors);
 sbh_status run_gmres_ir(sbh_operator* op, sbh_lowp_matrix* lp, const double2* d_B, double2* d_X, int k,
                         const sbh_gmres_ir_config& cfg, sbh_gmres_ir_status* st);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
@@ -102,6 +106,7 @@ void context_release(sbh_context* ctx) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     (void)hipStreamDestroy(ctx->stream);
@@ -135,17 +140,30 @@ double2* sbh_context::device_scratch(size_t count) {
     return scratch;
 }
 
+hipEvent_t sbh_context::take_event() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    SBH_HIP(hipEventCreate(&e));
+    return e;
+}
+
 void sbh_context::begin_timed(int cls, hipEvent_t* start) {
     *start = nullptr;
     if (!profiling) return;
-    SBH_HIP(hipEventCreate(start));
+    *start = take_event();
     SBH_HIP(hipEventRecord(*start, stream));
 }
 
 void sbh_context::end_timed(int cls, hipEvent_t start) {
     if (!profiling || !start) return;
-    hipEvent_t stop;
-    SBH_HIP(hipEventCreate(&stop));
+    hipEvent_t stop = take_event();
     SBH_HIP(hipEventRecord(stop, stream));
     std::lock_guard<std::mutex> lk(mu);
     timing[cls].events.emplace_back(start, stop);
@@ -241,6 +259,16 @@ sbh_status sbh_context_set_profiling(sbh_context* ctx, int enabled) {
     return guard_on(ctx_of(ctx), [&] {
         SBH_REQUIRE(ctx, "ctx is null");
         ctx->profiling = enabled != 0;
+        if (ctx->profiling) {
+            // warm the event pool so that timed regions create no events
+            use_device(ctx);
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            while (ctx->event_pool.size() < 4096) {
+                hipEvent_t e = nullptr;
+                SBH_HIP(hipEventCreate(&e));
+                ctx->event_pool.push_back(e);
+            }
+        }
     });
 }
 
@@ -256,8 +284,8 @@ sbh_status sbh_context_kernel_stats(sbh_context* ctx, int cls, int64_t* launches
             float ms = 0.0f;
             SBH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
             t.total_ms += ms;
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+            ctx->event_pool.push_back(e.first);
+            ctx->event_pool.push_back(e.second);
         }
         t.events.clear();
         if (launches) *launches = t.launches;
@@ -273,8 +301,8 @@ sbh_status sbh_context_reset_stats(sbh_context* ctx) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         for (auto& t : ctx->timing) {
             for (auto& e : t.events) {
-                (void)hipEventDestroy(e.first);
-                (void)hipEventDestroy(e.second);
+                ctx->event_pool.push_back(e.first);
+                ctx->event_pool.push_back(e.second);
             }
             t.events.clear();
             t.launches = 0;
@@ -1067,6 +1095,63 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
             SBH_HIP(hipMemsetAsync(dx.ptr, 0, sizeof(double2) * n * k, op->ctx->stream));
         }
         sbh_status st = run_gmres_block(op, db.ptr, dx.ptr, (int)k, *config, X0 == nullptr, solutions);
+        dx.finish();
+        if (st != SBH_OK) throw Error(st, "block GMRES failed");
+    });
+}
+
+sbh_status sbh_block_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64* B, const sbh_c64* X0, int64_t k,
+                           const sbh_block_gmres_config* config, sbh_c64* X, int memory, sbh_block_gmres_info* info,
+                           double* residuals, double* true_residuals, double* backward_errors) {
+    return guard_on(ctx_of(op), [&] {
+        SBH_REQUIRE(op && B && config && X, "null argument");
+        SBH_REQUIRE(k >= 1 && k <= 64, "block width must be in 1..64");
+        check_memory(memory);
+        // BlockGmresConfig::validate (block_gmres.rs:91-115)
+        SBH_REQUIRE(config->max_iterations > 0, "max_iterations must be greater than zero");
+        SBH_REQUIRE(config->restart > 0, "restart must be greater than zero");
+        SBH_REQUIRE(std::isfinite(config->tolerance) && config->tolerance > 0.0,
+                    "tolerance must be finite and positive");
+        SBH_REQUIRE(config->exact_residual_check_interval >= 0, "exact_residual_check_interval must be >= 0");
+        if (precond) {
+            SBH_REQUIRE(precond->ctx == op->ctx && precond->n == op->n, "preconditioner does not match the operator");
+        }
+        const int64_t n = op->n;
+        if (memory == SBH_MEM_HOST) {
+            if (!all_finite(B, (size_t)(n * k)))
+                throw Error(SBH_ERR_NUMERICAL, "right-hand side contains NaN or infinity");
+            if (X0 && !all_finite(X0, (size_t)(n * k)))
+                throw Error(SBH_ERR_NUMERICAL, "initial guess contains NaN or infinity");
+        }
+        use_device(op->ctx);
+        DeviceInput db(op->ctx, B, (size_t)(n * k), memory);
+        DeviceOutput dx(op->ctx, X, (size_t)(n * k), memory);
+        if (X0) {
+            DeviceInput dx0(op->ctx, X0, (size_t)(n * k), memory);
+            SBH_HIP(hipMemcpyAsync(dx.ptr, dx0.ptr, sizeof(double2) * n * k, hipMemcpyDeviceToDevice, op->ctx->stream));
+            SBH_HIP(hipStreamSynchronize(op->ctx->stream));
+        } else {
+            SBH_HIP(hipMemsetAsync(dx.ptr, 0, sizeof(double2) * n * k, op->ctx->stream));
+        }
+        if (memory == SBH_MEM_DEVICE) {
+            // device inputs: the same finiteness checks through one norm each
+            // (bk_frob_partials writes at most 256 partials)
+            DeviceArray<double> part(256 + 16);
+            auto finite_norm = [&](const double2* v) {
+                const int chunks = bk_frob_partials(op->ctx, v, n * k, part.ptr);
+                std::vector<double> h((size_t)chunks);
+                SBH_HIP(hipMemcpyAsync(h.data(), part.ptr, sizeof(double) * chunks, hipMemcpyDeviceToHost,
+                                       op->ctx->stream));
+                SBH_HIP(hipStreamSynchronize(op->ctx->stream));
+                double acc = 0.0;
+                for (double x : h) acc += x;
+                return std::isfinite(acc);
+            };
+            if (!finite_norm(db.ptr)) throw Error(SBH_ERR_NUMERICAL, "right-hand side contains NaN or infinity");
+            if (X0 && !finite_norm(dx.ptr)) throw Error(SBH_ERR_NUMERICAL, "initial guess contains NaN or infinity");
+        }
+        sbh_status st = run_block_gmres(op, precond, db.ptr, dx.ptr, (int)k, X0 == nullptr, *config, info, residuals,
+                                        true_residuals, backward_errors);
         dx.finish();
         if (st != SBH_OK) throw Error(st, "block GMRES failed");
     });

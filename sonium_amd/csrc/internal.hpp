@@ -209,6 +209,11 @@ struct sbh_context {
     // concurrent calls from several host threads run one after another.
     std::recursive_mutex op_mu;
     sbh::KernelTiming timing[SBH_KERNEL_CLASS_COUNT];
+    // Kernel-timing events are reused: taken from this pool by begin/end_timed
+    // and returned when their times are read, so a timed region records
+    // events but creates none once the pool is warm.
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t take_event();
     // device scratch for partial sums of transposed products, grown on demand
     double2* scratch = nullptr;
     size_t scratch_count = 0;
