@@ -61,7 +61,7 @@ void bk_true_metrics(sbh_context* ctx, const double2* B, const double2* AX, int 
 size_t bk_qr_state_bytes();
 size_t bk_qr_partial_elems(int64_t n);
 void bk_block_qr(sbh_context* ctx, double2* W, double2* Q, int64_t n, int sj, int budget, void* state,
-                 double2* partA, double2* partB, double2* partC);
+                 double2* part);
 void zgemm_f64(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, const double2* X, int64_t k, double2* Y);
 
 namespace {
@@ -216,10 +216,12 @@ class BlockLeastSquares {
 struct HostQrState {
     int snext[BK_MAXC + 1];
     int accepted[BK_MAXC];
-    int pad[3];
+    unsigned counter;
+    int pad[2];
     double orig[BK_MAXC];
     double wnorm[BK_MAXC];
     double r[BK_MAXC][BK_MAXC][2];
+    double coef[BK_MAXC][2];
 };
 
 }  // namespace
@@ -234,6 +236,7 @@ struct BlockGmresParams {
 
 struct BlockGmresResult {
     int64_t iterations = 0, spmv = 0, exact_checks = 0, cycles = 0, max_basis = 0, deflated = 0;
+    int64_t cholqr_blocks = 0, chain_blocks = 0;
     bool converged = false;
     std::vector<double> projected, true_res, backward;
 };
@@ -245,6 +248,8 @@ class BlockGmresEngine {
         max_steps_ = (int)std::max<int64_t>(1, restart);
         max_basis_ = (max_steps_ + 1) * s;
         const size_t blk = (size_t)n_ * s;
+        admit_device(ctx_, (int64_t)(sizeof(double2) * ((size_t)(max_basis_ + s) * n_ + 6 * blk)),
+                     "the shared block Krylov basis", false);
         V_.allocate((size_t)(max_basis_ + s) * n_);
         W_.allocate(blk);
         Rcm_.allocate(blk);
@@ -264,15 +269,17 @@ class BlockGmresEngine {
         dl_bytes_ = qr_off_ + bk_qr_state_bytes();
         dl_dev_.allocate((dl_bytes_ + 15) / 16);
         dl_host_.allocate((dl_bytes_ + 7) / 8);
-        qpartA_.allocate(bk_qr_partial_elems(n_));
-        qpartB_.allocate(bk_qr_partial_elems(n_));
-        qpartC_.allocate(bk_qr_partial_elems(n_));
+        qpart_.allocate(bk_qr_partial_elems(n_));
         nchunks_ = bk_norm_chunks((int64_t)n_ * s);
         npart_.allocate((size_t)std::max<int64_t>(3 * s, 4) * bk_norm_chunks(n_) + 2 * nchunks_ + 64);
         small_.allocate(4 * (size_t)s + 64);
         flag_.allocate(4);
         small_host_.allocate(4 * (size_t)s + 64);
+        qg_.allocate((size_t)s * s + 16);
+        qg_host_.allocate((size_t)2 * s * s + 32);
         SBH_REQUIRE(sizeof(HostQrState) == bk_qr_state_bytes(), "block QR state layout mismatch");
+        // the QR chain's workgroup counter starts at zero and every launch leaves it there
+        zero_on_stream(ctx_, dl_dev_.ptr, dl_dev_.bytes());
     }
 
     void run(const double2* B, double2* X, bool x_zero, const BlockGmresParams& prm, BlockGmresResult& out);
@@ -284,10 +291,14 @@ class BlockGmresEngine {
     int s_;
     const double2* dinv_;
     int max_steps_ = 0, max_basis_ = 0;
-    DeviceArray<double2> V_, W_, Rcm_, Xc_, AX_, Vrm_, Yrm_, gram_, qpartA_, qpartB_, qpartC_, dl_dev_;
+    DeviceArray<double2> V_, W_, Rcm_, Xc_, AX_, Vrm_, Yrm_, gram_, qpart_, dl_dev_;
     DeviceArray<double> gpart_, npart_, small_;
     DeviceArray<int> flag_;
     PinnedArray<double> dl_host_, small_host_;
+    DeviceArray<double2> qg_;     // sj x sj Gram matrix of the block being factored
+    PinnedArray<double> qg_host_;  // its host copy
+    HostQrState qrh_;              // outcome of the block QR, whichever way it ran
+    int64_t cholqr_blocks_ = 0, chain_blocks_ = 0;
     size_t hacc_off_ = 0, qr_off_ = 0, dl_bytes_ = 0;
     int nchunks_ = 1;
     int64_t spmv_ = 0, exact_checks_ = 0;
@@ -295,16 +306,14 @@ class BlockGmresEngine {
     double2* vec(int i) { return V_.ptr + (size_t)i * n_; }
     double2* hacc_dev() { return reinterpret_cast<double2*>(reinterpret_cast<char*>(dl_dev_.ptr) + hacc_off_); }
     void* qr_dev() { return reinterpret_cast<char*>(dl_dev_.ptr) + qr_off_; }
-    const HostQrState* qr_host() const {
-        return reinterpret_cast<const HostQrState*>(reinterpret_cast<const char*>(dl_host_.ptr) + qr_off_);
-    }
+    const HostQrState* qr_host() const { return &qrh_; }
     const double2* hacc_host() const {
         return reinterpret_cast<const double2*>(reinterpret_cast<const char*>(dl_host_.ptr) + hacc_off_);
     }
 
     // Y (n x k row-major) = A X (n x k row-major)
     void apply_rm(const double2* X, double2* Y, int k) {
-        if (!op_->exact()) {
+        if (op_->dense()) {
             if (k == 1)
                 zgemv_n(ctx_, op_->sys->d_A.ptr, op_->sys->lda, n_, n_, X, Y);
             else
@@ -320,7 +329,7 @@ class BlockGmresEngine {
 
     // W (column-major, sj vectors) = M^-1 A V[first .. first + sj)
     void apply_cm(int first, int sj) {
-        if (!op_->exact()) {
+        if (op_->dense()) {
             bk_to_rowmajor(ctx_, vec(first), n_, sj, n_, Vrm_.ptr);
             if (sj == 1)
                 zgemv_n(ctx_, op_->sys->d_A.ptr, op_->sys->lda, n_, n_, Vrm_.ptr, Yrm_.ptr);
@@ -355,16 +364,119 @@ class BlockGmresEngine {
         bk_dgks_flag(ctx_, before, after, chunks, flag_.ptr, frob);
         bk_gram(ctx_, V_.ptr, n_, p, W_.ptr, n_, sj, n_, gpart_.ptr, gram_.ptr, hacc_dev(), 2, flag_.ptr);
         bk_combine(ctx_, V_.ptr, n_, p, gram_.ptr, sj, sj, n_, W_.ptr, n_, 1, -1.0, flag_.ptr);
-        bk_block_qr(ctx_, W_.ptr, vec(p), n_, sj, max_basis_ - p, qr_dev(), qpartA_.ptr, qpartB_.ptr, qpartC_.ptr);
+        bk_gram(ctx_, W_.ptr, n_, sj, W_.ptr, n_, sj, n_, gpart_.ptr, qg_.ptr, nullptr, 0, nullptr);
     }
 
-    // Reads the step's Hessenberg block (p x sj) and QR outcome.
+    // Reads the step's Hessenberg block (p x sj) and forms the QR of W into
+    // the basis slots from p on.
     void read_step(int p, int sj) {
         SBH_HIP(hipMemcpyAsync(dl_host_.ptr, dl_dev_.ptr, sizeof(double2) * (size_t)p * sj, hipMemcpyDeviceToHost,
                                ctx_->stream));
-        SBH_HIP(hipMemcpyAsync(reinterpret_cast<char*>(dl_host_.ptr) + qr_off_, qr_dev(), bk_qr_state_bytes(),
-                               hipMemcpyDeviceToHost, ctx_->stream));
+        block_qr(p, sj, max_basis_ - p);
+    }
+
+    double2* qg_host() { return reinterpret_cast<double2*>(qg_host_.ptr); }
+
+    // sj x sj Gram matrix qg_ -> host (the stream is synchronized)
+    std::vector<cplx> fetch_gram(int sj) {
+        SBH_HIP(hipMemcpyAsync(qg_host(), qg_.ptr, sizeof(double2) * (size_t)sj * sj, hipMemcpyDeviceToHost,
+                               ctx_->stream));
         SBH_HIP(hipStreamSynchronize(ctx_->stream));
+        std::vector<cplx> g((size_t)sj * sj);
+        for (size_t i = 0; i < g.size(); ++i) g[i] = cplx(qg_host()[i].x, qg_host()[i].y);
+        return g;
+    }
+
+    // Upper R with R^H R = G (G Hermitian, sj x sj row-major); false when a
+    // pivot is not clearly positive: below `floor` times sqrt(G_cc).
+    static bool cholesky(const std::vector<cplx>& G, int m, double floor, std::vector<cplx>& R,
+                         std::vector<double>* diag_norm) {
+        R.assign((size_t)m * m, cplx(0.0, 0.0));
+        for (int j = 0; j < m; ++j) {
+            double d = G[(size_t)j * m + j].real();
+            const double gjj = d;
+            for (int k = 0; k < j; ++k) d -= cnorm_sqr(R[(size_t)k * m + j]);
+            if (!(gjj > 0.0) || !(d > floor * floor * gjj) || !std::isfinite(d)) return false;
+            const double rjj = std::sqrt(d);
+            R[(size_t)j * m + j] = cplx(rjj, 0.0);
+            if (diag_norm) (*diag_norm)[j] = std::sqrt(gjj);
+            for (int i = j + 1; i < m; ++i) {
+                cplx v = G[(size_t)j * m + i];
+                for (int k = 0; k < j; ++k) v -= std::conj(R[(size_t)k * m + j]) * R[(size_t)k * m + i];
+                R[(size_t)j * m + i] = v / rjj;
+            }
+        }
+        return true;
+    }
+
+    // T = R^-1 (upper triangular, positive real diagonal)
+    static std::vector<cplx> triangular_inverse(const std::vector<cplx>& R, int m) {
+        std::vector<cplx> T((size_t)m * m, cplx(0.0, 0.0));
+        for (int c = 0; c < m; ++c) {
+            T[(size_t)c * m + c] = cplx(1.0 / R[(size_t)c * m + c].real(), 0.0);
+            for (int i = c - 1; i >= 0; --i) {
+                cplx acc(0.0, 0.0);
+                for (int k = i + 1; k <= c; ++k) acc += R[(size_t)i * m + k] * T[(size_t)k * m + c];
+                T[(size_t)i * m + c] = -acc / R[(size_t)i * m + i].real();
+            }
+        }
+        return T;
+    }
+
+    // out (column-major, sj vectors) = in * T, T sj x sj upper (coefficients through gram_)
+    void apply_triangular(const double2* in, const std::vector<cplx>& T, int sj, double2* out) {
+        std::vector<double2> th(T.size());
+        for (size_t i = 0; i < T.size(); ++i) th[i] = make_double2(T[i].real(), T[i].imag());
+        SBH_HIP(hipMemcpyAsync(gram_.ptr, th.data(), sizeof(double2) * th.size(), hipMemcpyHostToDevice,
+                               ctx_->stream));
+        SBH_HIP(hipMemsetAsync(out, 0, sizeof(double2) * (size_t)sj * n_, ctx_->stream));
+        bk_combine(ctx_, in, n_, sj, gram_.ptr, sj, sj, n_, out, n_, 1, 1.0, nullptr);
+        SBH_HIP(hipStreamSynchronize(ctx_->stream));  // th must outlive the copy
+    }
+
+    // QR of the sj column-major vectors in W_ (its Gram matrix already in
+    // qg_) into the basis slots dst..: Cholesky QR twice when every pivot is
+    // clearly positive (the block is far from the rank-drop threshold) and the
+    // block fits the budget, which gives the per-column CGS2 factors to
+    // rounding; otherwise the per-column CGS2 chain with rank-drop deflation.
+    // W_ is left unchanged by the first way, so the second can always follow.
+    void block_qr(int dst, int sj, int budget) {
+        constexpr double kPivotFloor = 1e-4;  // four orders above sqrt(eps)
+        std::vector<cplx> G1 = fetch_gram(sj), R1, R2;
+        std::vector<double> orig(sj, 0.0);
+        bool ok = sj <= budget && cholesky(G1, sj, kPivotFloor, R1, &orig);
+        if (ok) {
+            apply_triangular(W_.ptr, triangular_inverse(R1, sj), sj, Yrm_.ptr);
+            bk_gram(ctx_, Yrm_.ptr, n_, sj, Yrm_.ptr, n_, sj, n_, gpart_.ptr, qg_.ptr, nullptr, 0, nullptr);
+            ok = cholesky(fetch_gram(sj), sj, 0.5, R2, nullptr);
+        }
+        if (ok) {
+            apply_triangular(Yrm_.ptr, triangular_inverse(R2, sj), sj, vec(dst));
+            qrh_ = HostQrState{};
+            for (int c = 0; c < sj; ++c) {
+                // R = R2 R1: column c of the product
+                for (int a = 0; a <= c; ++a) {
+                    cplx v(0.0, 0.0);
+                    for (int k = a; k <= c; ++k) v += R2[(size_t)a * sj + k] * R1[(size_t)k * sj + c];
+                    if (a < c) {
+                        qrh_.r[c][a][0] = v.real();
+                        qrh_.r[c][a][1] = v.imag();
+                    } else {
+                        qrh_.wnorm[c] = v.real();
+                    }
+                }
+                qrh_.snext[c] = c;
+                qrh_.accepted[c] = c;
+                qrh_.orig[c] = orig[c];
+            }
+            qrh_.snext[sj] = sj;
+            cholqr_blocks_ += 1;
+            return;
+        }
+        bk_block_qr(ctx_, W_.ptr, vec(dst), n_, sj, budget, qr_dev(), qpart_.ptr);
+        SBH_HIP(hipMemcpyAsync(&qrh_, qr_dev(), sizeof(HostQrState), hipMemcpyDeviceToHost, ctx_->stream));
+        SBH_HIP(hipStreamSynchronize(ctx_->stream));
+        chain_blocks_ += 1;
     }
 
     std::vector<double> read_small(int count) {
@@ -450,10 +562,8 @@ void BlockGmresEngine::run(const double2* B, double2* X, bool x_zero, const Bloc
         if (all_converged) break;
         // initial block QR with rank-drop deflation (CGS2 per column) of a copy of R0
         device_copy(ctx_, Rcm_.ptr, W_.ptr, n_ * s);
-        bk_block_qr(ctx_, W_.ptr, vec(0), n_, s, 1 << 30, qr_dev(), qpartA_.ptr, qpartB_.ptr, qpartC_.ptr);
-        SBH_HIP(hipMemcpyAsync(reinterpret_cast<char*>(dl_host_.ptr) + qr_off_, qr_dev(), bk_qr_state_bytes(),
-                               hipMemcpyDeviceToHost, ctx_->stream));
-        SBH_HIP(hipStreamSynchronize(ctx_->stream));
+        bk_gram(ctx_, W_.ptr, n_, s, W_.ptr, n_, s, n_, gpart_.ptr, qg_.ptr, nullptr, 0, nullptr);
+        block_qr(0, s, 1 << 30);
         const int s1 = qr_host()->snext[s];
         out.deflated += s - s1;
         if (s1 == 0) break;
@@ -566,6 +676,8 @@ void BlockGmresEngine::run(const double2* B, double2* X, bool x_zero, const Bloc
     out.converged = conv;
     out.spmv = spmv_;
     out.exact_checks = exact_checks_;
+    out.cholqr_blocks = cholqr_blocks_;
+    out.chain_blocks = chain_blocks_;
 }
 
 sbh_status run_block_gmres(sbh_operator* op, const sbh_precond* M, const double2* d_B, double2* d_X, int k,
@@ -595,6 +707,8 @@ sbh_status run_block_gmres(sbh_operator* op, const sbh_precond* M, const double2
         info->restart_cycles = r.cycles;
         info->max_basis = r.max_basis;
         info->deflated = r.deflated;
+        info->cholesky_qr_blocks = r.cholqr_blocks;
+        info->chain_qr_blocks = r.chain_blocks;
     }
     for (int j = 0; j < k; ++j) {
         if (residuals) residuals[j] = r.projected[j];

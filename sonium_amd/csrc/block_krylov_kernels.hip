@@ -41,7 +41,7 @@ constexpr int BK_XCD = 8;          // XCDs of the MI355X; K slices are grouped p
 constexpr int BK_BM = 64;          // basis vectors per Gram tile
 constexpr int BK_STAGE = 16;       // complex K entries per LDS stage
 constexpr int BK_CK = 64;          // complex output entries per combine tile
-constexpr int QR_CHUNK = 512;      // rows per workgroup in the in-block QR chain
+constexpr int QR_CHUNK = 256;      // rows per workgroup in the in-block QR chain (one per thread)
 constexpr int QR_MAXC = 64;        // block width limit
 constexpr int QR_SLOTS = QR_MAXC + 1;  // dots + squared norm per chunk
 
@@ -420,19 +420,47 @@ __global__ __launch_bounds__(BK_THREADS) void bk_true_partials_kernel(const doub
 struct QrState {
     int snext[QR_MAXC + 1];   // accepted directions before column c (snext[0] = 0)
     int accepted[QR_MAXC];    // slot of column c, or -1 if deflated
-    int pad[3];
+    unsigned counter;         // finished workgroups of the running launch
+    int pad[2];
     double orig[QR_MAXC];     // ||w_c|| before the passes
     double wnorm[QR_MAXC];    // ||w_c|| after the passes
     double2 r[QR_MAXC][QR_MAXC];  // h[p + a][p_start + c] += d1[a] + d2[a]  (a < snext[c])
+    double2 coef[QR_MAXC];    // coefficients of the pass in progress
 };
 
-// Sum of the chunk partials of `slots` values (fixed order: four strided
-// quarters, then the quarters in order) into LDS out[slot].
-__device__ void qr_sum_partials(const double2* __restrict__ part, int chunks, int slots, double2* out,
-                                double2 (*quarter)[4]) {
+struct QrArgs {
+    double2* W;       // sj column-major vectors of length n
+    double2* Q;       // the block's accepted directions: slot a at Q + a*n
+    int64_t n;
+    int sj;
+    int budget;       // accepted directions allowed in this block
+    double defl_tol;  // sqrt(eps)
+    QrState* st;
+    double2* part;    // chunks x QR_SLOTS partial sums (dots in slots 0..63, ||w||^2 in slot 64)
+};
+
+// True in the workgroup that finishes last: each workgroup publishes its
+// partials, then takes a ticket; the last one sums every chunk's partials
+// (in chunk order) and clears the counter for the next launch.
+__device__ bool qr_last_block(unsigned* counter) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+// out[v] = sum over chunks of part[chunk][v] for v < cnt and, with the norm,
+// v = QR_MAXC: four strided quarter sums, added in order.
+__device__ void qr_sum_chunks(const double2* __restrict__ part, int chunks, int cnt, bool with_norm, double2* out,
+                              double2 (*quarter)[4]) {
     const int t = threadIdx.x;
-    if (t < 4 * slots) {
-        const int v = t >> 2, q = t & 3;
+    const int items = 4 * (cnt + (with_norm ? 1 : 0));
+    for (int w = t; w < items; w += BK_THREADS) {
+        const int idx = w >> 2, q = w & 3;
+        const int v = idx < cnt ? idx : QR_MAXC;
         double re = 0.0, im = 0.0;
         for (int ch = q; ch < chunks; ch += 4) {
             const double2 x = part[(int64_t)ch * QR_SLOTS + v];
@@ -442,7 +470,7 @@ __device__ void qr_sum_partials(const double2* __restrict__ part, int chunks, in
         quarter[v][q] = make_double2(re, im);
     }
     __syncthreads();
-    if (t < slots) {
+    if (t < cnt || (with_norm && t == QR_MAXC)) {
         double re = quarter[t][0].x, im = quarter[t][0].y;
         for (int q = 1; q < 4; ++q) {
             re += quarter[t][q].x;
@@ -453,12 +481,12 @@ __device__ void qr_sum_partials(const double2* __restrict__ part, int chunks, in
     __syncthreads();
 }
 
-// Chunk partials of conj(q_a) . w for a < cnt (slots 0..cnt-1) and ||w||^2
-// (slot QR_MAXC), w in LDS.  Thread (a = t % 64, segment = t / 64) sums a
-// quarter of the chunk in row order; the quarters are added in order.
-// qlds (may be null) replaces slot cnt-1's vector by the LDS copy just written.
+// This chunk's partials of conj(q_a) . w (a < cnt; the LDS copy ql replaces
+// slot cnt - 1 when given) and ||w||^2, w in LDS.  Thread (a = t % 64,
+// segment = t / 64) sums a quarter of the chunk in row order; the quarters
+// are added in order.
 __device__ void qr_chunk_dots(const double2* __restrict__ Q, int64_t n, int64_t r0, int len, int cnt,
-                              const double2* wl, const double2* qlds, double2* __restrict__ part_out,
+                              const double2* wl, const double2* ql, double2* __restrict__ part_out,
                               double2 (*seg)[4], double* red) {
     const int t = threadIdx.x;
     const int a = t & 63, sg = t >> 6;
@@ -466,17 +494,17 @@ __device__ void qr_chunk_dots(const double2* __restrict__ Q, int64_t n, int64_t 
     const int b = sg * slen, e = min(len, b + slen);
     if (a < cnt) {
         double re = 0.0, im = 0.0;
-        const bool from_lds = qlds && a == cnt - 1;
+        const bool from_lds = ql && a == cnt - 1;
         const double2* q = Q + (int64_t)a * n + r0;
+#pragma unroll 8
         for (int k = b; k < e; ++k) {
-            const double2 qa = from_lds ? qlds[k] : q[k];
+            const double2 qa = from_lds ? ql[k] : q[k];
             const double2 d = cmulc(qa, wl[k]);
             re += d.x;
             im += d.y;
         }
         seg[a][sg] = make_double2(re, im);
     }
-    // ||w||^2 over the chunk: each thread its rows t, t + 256, then a tree
     double s2 = 0.0;
     for (int k = t; k < len; k += BK_THREADS) s2 += wl[k].x * wl[k].x + wl[k].y * wl[k].y;
     red[t] = s2;
@@ -494,167 +522,105 @@ __device__ void qr_chunk_dots(const double2* __restrict__ Q, int64_t n, int64_t 
         part_out[t] = make_double2(re, im);
     }
     if (t == 0) part_out[QR_MAXC] = make_double2(red[0], 0.0);
-    __syncthreads();
 }
 
-struct QrArgs {
-    double2* W;       // sj column-major vectors of length n
-    double2* Q;       // the block's accepted directions: slot a at Q + a*n
-    int64_t n;
-    int sj;
-    int budget;       // accepted directions allowed in this block
-    double defl_tol;  // sqrt(eps)
-    QrState* st;
-    double2* partA;   // chunks x QR_SLOTS: pass-1 dots and ||w||^2 before (start, finish)
-    double2* partB;   // pass-2 dots (pass)
-    double2* partC;   // ||w||^2 after both passes (pass2)
-};
-
-// Phase A of column c: partial dots against the accepted directions and ||w_c||^2.
+// Start of column c (c == sj: finish only): the previous column's direction
+// q = w * (1/||w||) if it was accepted, then pass-1 partials of column c;
+// the last workgroup turns them into d1 (the first h entries) and ||w_c||.
 __global__ __launch_bounds__(BK_THREADS) void bk_qr_start_kernel(QrArgs a, int c) {
-    __shared__ double2 wl[QR_CHUNK];
-    __shared__ double2 seg[QR_MAXC][4];
-    __shared__ double red[BK_THREADS];
-    const int64_t r0 = (int64_t)blockIdx.x * QR_CHUNK;
-    const int len = (int)min((int64_t)QR_CHUNK, a.n - r0);
-    const double2* wc = a.W + (int64_t)c * a.n + r0;
-    for (int k = threadIdx.x; k < len; k += BK_THREADS) wl[k] = wc[k];
-    if (c == 0 && blockIdx.x == 0 && threadIdx.x == 0) a.st->snext[0] = 0;
-    __syncthreads();
-    const int cnt = c == 0 ? 0 : a.st->snext[c];
-    qr_chunk_dots(a.Q, a.n, r0, len, cnt, wl, nullptr, a.partA + (int64_t)blockIdx.x * QR_SLOTS, seg, red);
-}
-
-// Phase B: d1 from the partials; w_c -= sum_a d1[a] q_a (the reference's axpy
-// order); partial dots of the second pass.
-__global__ __launch_bounds__(BK_THREADS) void bk_qr_pass_kernel(QrArgs a, int c, int chunks) {
-    __shared__ double2 wl[QR_CHUNK];
-    __shared__ double2 d[QR_SLOTS];
-    __shared__ double2 quarter[QR_SLOTS][4];
-    __shared__ double2 seg[QR_MAXC][4];
-    __shared__ double red[BK_THREADS];
-    const int cnt = a.st->snext[c];
-    const int64_t r0 = (int64_t)blockIdx.x * QR_CHUNK;
-    const int len = (int)min((int64_t)QR_CHUNK, a.n - r0);
-    // slots 0..cnt-1 and the norm slot
-    {
-        const int t = threadIdx.x;
-        // sum dots (cnt slots) and the norm slot separately through the same helper
-        qr_sum_partials(a.partA, chunks, cnt, d, quarter);
-        if (t < 4) {
-            double s = 0.0;
-            for (int ch = t; ch < chunks; ch += 4) s += a.partA[(int64_t)ch * QR_SLOTS + QR_MAXC].x;
-            quarter[QR_MAXC][t] = make_double2(s, 0.0);
-        }
-        __syncthreads();
-        if (t == 0) {
-            double s = quarter[QR_MAXC][0].x;
-            for (int q = 1; q < 4; ++q) s += quarter[QR_MAXC][q].x;
-            d[QR_MAXC] = make_double2(s, 0.0);
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x == 0) {
-        if (threadIdx.x < cnt) a.st->r[c][threadIdx.x] = d[threadIdx.x];
-        if (threadIdx.x == 0) a.st->orig[c] = sqrt(d[QR_MAXC].x);
-    }
-    double2* wc = a.W + (int64_t)c * a.n + r0;
-    for (int k = threadIdx.x; k < len; k += BK_THREADS) {
-        double2 v = wc[k];
-        for (int q = 0; q < cnt; ++q) {
-            const double2 nd = make_double2(-d[q].x, -d[q].y);
-            const double2 t2 = cmul(nd, a.Q[(int64_t)q * a.n + r0 + k]);
-            v = make_double2(v.x + t2.x, v.y + t2.y);
-        }
-        wc[k] = v;
-        wl[k] = v;
-    }
-    __syncthreads();
-    qr_chunk_dots(a.Q, a.n, r0, len, cnt, wl, nullptr, a.partB + (int64_t)blockIdx.x * QR_SLOTS, seg, red);
-}
-
-// Phase C: d2; h entries += d2; w_c -= sum d2 q; partial ||w_c||^2 (into partC).
-__global__ __launch_bounds__(BK_THREADS) void bk_qr_pass2_kernel(QrArgs a, int c, int chunks) {
-    __shared__ double2 wl[QR_CHUNK];
-    __shared__ double2 d[QR_SLOTS];
-    __shared__ double2 quarter[QR_SLOTS][4];
-    __shared__ double2 seg[QR_MAXC][4];
-    __shared__ double red[BK_THREADS];
-    const int cnt = a.st->snext[c];
-    const int64_t r0 = (int64_t)blockIdx.x * QR_CHUNK;
-    const int len = (int)min((int64_t)QR_CHUNK, a.n - r0);
-    qr_sum_partials(a.partB, chunks, cnt, d, quarter);
-    if (blockIdx.x == 0 && (int)threadIdx.x < cnt) {
-        const double2 h = a.st->r[c][threadIdx.x];
-        a.st->r[c][threadIdx.x] = make_double2(h.x + d[threadIdx.x].x, h.y + d[threadIdx.x].y);
-    }
-    double2* wc = a.W + (int64_t)c * a.n + r0;
-    for (int k = threadIdx.x; k < len; k += BK_THREADS) {
-        double2 v = wc[k];
-        for (int q = 0; q < cnt; ++q) {
-            const double2 nd = make_double2(-d[q].x, -d[q].y);
-            const double2 t2 = cmul(nd, a.Q[(int64_t)q * a.n + r0 + k]);
-            v = make_double2(v.x + t2.x, v.y + t2.y);
-        }
-        wc[k] = v;
-        wl[k] = v;
-    }
-    __syncthreads();
-    // only the norm is needed: cnt = 0 dots
-    qr_chunk_dots(a.Q, a.n, r0, len, 0, wl, nullptr, a.partC + (int64_t)blockIdx.x * QR_SLOTS, seg, red);
-}
-
-// Phase D: wn from the partials (partC); accept when wn > defl_tol * max(orig, tiny)
-// and the block budget allows (block_gmres.rs:1014-1024); q = w * (1/wn);
-// then phase A of column c + 1 against the updated directions.
-__global__ __launch_bounds__(BK_THREADS) void bk_qr_finish_kernel(QrArgs a, int c, int chunks) {
     __shared__ double2 wl[QR_CHUNK];
     __shared__ double2 ql[QR_CHUNK];
     __shared__ double2 seg[QR_MAXC][4];
+    __shared__ double2 quarter[QR_SLOTS][4];
+    __shared__ double2 sums[QR_SLOTS];
     __shared__ double red[BK_THREADS];
-    __shared__ double wn_s;
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * QR_CHUNK;
+    const int len = (int)min((int64_t)QR_CHUNK, a.n - r0);
+    bool fresh = false;
+    if (c > 0) {
+        const int slot = a.st->accepted[c - 1];
+        if (slot >= 0) {
+            const double inv = 1.0 / a.st->wnorm[c - 1];
+            const double2* wp = a.W + (int64_t)(c - 1) * a.n + r0;
+            double2* q = a.Q + (int64_t)slot * a.n + r0;
+            for (int k = t; k < len; k += BK_THREADS) {
+                const double2 v = wp[k];
+                const double2 qv = make_double2(v.x * inv, v.y * inv);
+                q[k] = qv;
+                ql[k] = qv;
+            }
+            fresh = true;
+        }
+    }
+    if (c >= a.sj) return;
+    const int cnt = c == 0 ? 0 : a.st->snext[c];
+    const double2* wc = a.W + (int64_t)c * a.n + r0;
+    for (int k = t; k < len; k += BK_THREADS) wl[k] = wc[k];
+    __syncthreads();
+    qr_chunk_dots(a.Q, a.n, r0, len, cnt, wl, fresh ? ql : nullptr, a.part + (int64_t)blockIdx.x * QR_SLOTS, seg,
+                  red);
+    if (!qr_last_block(&a.st->counter)) return;
+    qr_sum_chunks(a.part, gridDim.x, cnt, true, sums, quarter);
+    if (t < cnt) {
+        a.st->r[c][t] = sums[t];
+        a.st->coef[t] = sums[t];
+    }
+    if (t == 0) {
+        if (c == 0) a.st->snext[0] = 0;
+        a.st->orig[c] = sqrt(sums[QR_MAXC].x);
+        a.st->counter = 0;
+    }
+}
+
+// w_c -= sum_a coef[a] q_a (the reference's axpy order), then (second) the
+// pass-2 partials, whose sums become coef and are added into the h entries,
+// or (third) the partial ||w_c||^2, whose square root decides acceptance:
+// wn > defl_tol * max(orig, tiny) and room left in the block
+// (block_gmres.rs:1014-1024).
+__global__ __launch_bounds__(BK_THREADS) void bk_qr_project_kernel(QrArgs a, int c, int third) {
+    __shared__ double2 wl[QR_CHUNK];
+    __shared__ double2 d[QR_MAXC];
+    __shared__ double2 seg[QR_MAXC][4];
+    __shared__ double2 quarter[QR_SLOTS][4];
+    __shared__ double2 sums[QR_SLOTS];
+    __shared__ double red[BK_THREADS];
     const int t = threadIdx.x;
     const int cnt = a.st->snext[c];
     const int64_t r0 = (int64_t)blockIdx.x * QR_CHUNK;
     const int len = (int)min((int64_t)QR_CHUNK, a.n - r0);
-    if (t < 4) {
-        double s = 0.0;
-        for (int ch = t; ch < chunks; ch += 4) s += a.partC[(int64_t)ch * QR_SLOTS + QR_MAXC].x;
-        red[t] = s;
-    }
+    if (t < cnt) d[t] = a.st->coef[t];
     __syncthreads();
-    if (t == 0) {
-        double s = red[0];
-        for (int q = 1; q < 4; ++q) s += red[q];
-        wn_s = sqrt(s);
-    }
-    __syncthreads();
-    const double wn = wn_s;
-    const double orig = a.st->orig[c];
-    const bool accept = wn > a.defl_tol * fmax(orig, 1e-30) && cnt < a.budget;
-    const double2* wc = a.W + (int64_t)c * a.n + r0;
-    if (accept) {
-        const double inv = 1.0 / wn;
-        double2* q = a.Q + (int64_t)cnt * a.n + r0;
-        for (int k = t; k < len; k += BK_THREADS) {
-            const double2 v = wc[k];
-            const double2 qv = make_double2(v.x * inv, v.y * inv);
-            q[k] = qv;
-            ql[k] = qv;
+    double2* wc = a.W + (int64_t)c * a.n + r0;
+    for (int k = t; k < len; k += BK_THREADS) {
+        double2 v = wc[k];
+        for (int q = 0; q < cnt; ++q) {
+            const double2 nd = make_double2(-d[q].x, -d[q].y);
+            const double2 t2 = cmul(nd, a.Q[(int64_t)q * a.n + r0 + k]);
+            v = make_double2(v.x + t2.x, v.y + t2.y);
         }
+        wc[k] = v;
+        wl[k] = v;
     }
-    if (blockIdx.x == 0 && t == 0) {
+    __syncthreads();
+    qr_chunk_dots(a.Q, a.n, r0, len, third ? 0 : cnt, wl, nullptr, a.part + (int64_t)blockIdx.x * QR_SLOTS, seg,
+                  red);
+    if (!qr_last_block(&a.st->counter)) return;
+    qr_sum_chunks(a.part, gridDim.x, third ? 0 : cnt, third != 0, sums, quarter);
+    if (!third) {
+        if (t < cnt) {
+            const double2 h = a.st->r[c][t];
+            a.st->r[c][t] = make_double2(h.x + sums[t].x, h.y + sums[t].y);
+            a.st->coef[t] = sums[t];
+        }
+    } else if (t == 0) {
+        const double wn = sqrt(sums[QR_MAXC].x);
+        const bool accept = wn > a.defl_tol * fmax(a.st->orig[c], 1e-30) && cnt < a.budget;
         a.st->wnorm[c] = wn;
         a.st->accepted[c] = accept ? cnt : -1;
         a.st->snext[c + 1] = cnt + (accept ? 1 : 0);
     }
-    if (c + 1 >= a.sj) return;
-    const double2* wn1 = a.W + (int64_t)(c + 1) * a.n + r0;
-    for (int k = t; k < len; k += BK_THREADS) wl[k] = wn1[k];
-    __syncthreads();
-    const int cnt1 = cnt + (accept ? 1 : 0);
-    qr_chunk_dots(a.Q, a.n, r0, len, cnt1, wl, accept ? ql : nullptr, a.partA + (int64_t)blockIdx.x * QR_SLOTS, seg,
-                  red);
+    if (t == 0) a.st->counter = 0;
 }
 
 inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
@@ -784,9 +750,11 @@ int bk_qr_chunks(int64_t n) { return (int)((n + QR_CHUNK - 1) / QR_CHUNK); }
 size_t bk_qr_partial_elems(int64_t n) { return (size_t)bk_qr_chunks(n) * QR_SLOTS; }
 
 // Block QR with rank-drop deflation of the sj column-major vectors W into the
-// directions Q (slots 0..); state and partials on the device.
+// directions Q (slots 0..); state (its counter zero between launches) and
+// partials on the device.  Three launches per column plus one to write the
+// last direction.
 void bk_block_qr(sbh_context* ctx, double2* W, double2* Q, int64_t n, int sj, int budget, void* state,
-                 double2* partA, double2* partB, double2* partC) {
+                 double2* part) {
     SBH_REQUIRE(sj >= 1 && sj <= QR_MAXC, "block QR width");
     QrArgs a;
     a.W = W;
@@ -796,19 +764,16 @@ void bk_block_qr(sbh_context* ctx, double2* W, double2* Q, int64_t n, int sj, in
     a.budget = budget;
     a.defl_tol = std::sqrt(2.220446049250313e-16);
     a.st = static_cast<QrState*>(state);
-    a.partA = partA;
-    a.partB = partB;
-    a.partC = partC;
+    a.part = part;
     const int chunks = bk_qr_chunks(n);
     const dim3 grid((unsigned)chunks);
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_KRYLOV_VECTOR, &ev);
-    hipLaunchKernelGGL(bk_qr_start_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, 0);
-    SBH_HIP(hipGetLastError());
-    for (int c = 0; c < sj; ++c) {
-        hipLaunchKernelGGL(bk_qr_pass_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c, chunks);
-        hipLaunchKernelGGL(bk_qr_pass2_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c, chunks);
-        hipLaunchKernelGGL(bk_qr_finish_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c, chunks);
+    for (int c = 0; c <= sj; ++c) {
+        hipLaunchKernelGGL(bk_qr_start_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c);
+        if (c == sj) break;
+        hipLaunchKernelGGL(bk_qr_project_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c, 0);
+        hipLaunchKernelGGL(bk_qr_project_kernel, grid, dim3(BK_THREADS), 0, ctx->stream, a, c, 1);
     }
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_KRYLOV_VECTOR, ev);
