@@ -321,10 +321,92 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
 sbh_status sbh_operator_apply_block_noalloc(sbh_operator* op, const sbh_c64* X, sbh_c64* Y, int64_t k);
 
 /* ---------------------------------------------------------------------------
+ * Failure injection — CudaFailureInjectionPoint (crates/sonium-compute/src/
+ * backend/mod.rs:120-150) and live_allocation_bytes, for acceptance tests of
+ * the failure paths (the reference's tbem.rs:6740-6794).  Inert until armed:
+ * an armed point fails, once, the next boundary of its kind reached by an
+ * entry point running on that context, before the boundary takes effect
+ * (allocation: before hipMalloc, SBH_ERR_OUT_OF_MEMORY; launch: before the
+ * assembly or operator kernels, transfers: before a host copy moves data,
+ * synchronization: before the stream wait; SBH_ERR_BACKEND).  The call
+ * returns its error, releases what it had allocated, and publishes nothing. */
+typedef enum {
+    SBH_INJECT_NONE = 0,
+    SBH_INJECT_ALLOCATION = 1,
+    SBH_INJECT_LAUNCH = 2,
+    SBH_INJECT_TRANSFER_H2D = 3,
+    SBH_INJECT_TRANSFER_D2H = 4,
+    SBH_INJECT_SYNCHRONIZATION = 5
+} sbh_injection_point;
+sbh_status sbh_context_inject_failure(sbh_context* ctx, int point);
+/* Bytes held by the library's device allocations in this process. */
+int64_t sbh_live_device_bytes(void);
+
+/* ---------------------------------------------------------------------------
+ * Resource admission — BemResourceBudget / BemResourceRequest::estimate /
+ * admit (crates/sonium-bem/src/core/admission.rs:17-330) for the dense
+ * (Tbem) method: a decision before anything is allocated.  host_bytes and its
+ * parts follow the reference's conservative estimate (dense operator n*n*16,
+ * right-hand sides n*rhs*16 per frequency, scratch n*(4*16 + 8)); device_bytes
+ * is this library's allocation for the same work (system, right-hand sides
+ * and solutions, GMRES(restart) basis; the shared block basis for rhs > 1).
+ * Errors: a zero request -> SBH_ERR_INVALID_ARGUMENT (InvalidRequest);
+ * overflow -> SBH_ERR_INVALID_ARGUMENT (ArithmeticOverflow); over a limit
+ * -> SBH_ERR_OUT_OF_MEMORY (LimitExceeded, the message names the resource);
+ * a zero host or work budget -> SBH_ERR_INVALID_ARGUMENT (InvalidBudget).
+ * Host-only: no device is touched. */
+typedef struct {
+    int64_t max_host_bytes;
+    int64_t max_device_bytes;
+    int64_t max_active_work;
+} sbh_resource_budget;
+
+typedef struct {
+    int64_t host_bytes;
+    int64_t device_bytes;
+    int64_t active_work;
+    int64_t operator_bytes;
+    int64_t rhs_bytes;
+    int64_t scratch_bytes;
+} sbh_resource_estimate;
+
+sbh_status sbh_admission_estimate(int64_t active_dofs, int64_t rhs_count, int64_t frequency_count, int64_t restart,
+                                  sbh_resource_estimate* out);
+sbh_status sbh_admission_admit(const sbh_resource_budget* budget, int64_t active_dofs, int64_t rhs_count,
+                               int64_t frequency_count, int64_t restart, sbh_resource_estimate* out);
+
+/* Device-side admission on a context: with a budget > 0 every large device
+ * allocation (systems, low-precision copies, batch matrices, Krylov
+ * workspaces) is refused with SBH_ERR_OUT_OF_MEMORY before it is made when the
+ * bytes held by the context's objects plus the request exceed it; requests
+ * beyond the device's free memory are refused the same way with or without a
+ * budget.  0 removes the budget. */
+sbh_status sbh_context_set_device_budget(sbh_context* ctx, int64_t max_device_bytes);
+sbh_status sbh_context_device_budget(sbh_context* ctx, int64_t* max_device_bytes, int64_t* held_bytes);
+
+/* A caller-supplied LinearOperator<Complex64> (traits.rs:2949-3290: `gmres`
+ * is generic over any operator, gmres.rs:286).  apply computes y = A x for
+ * vectors of length n in `memory` (SBH_MEM_HOST: the library stages them
+ * through host buffers; SBH_MEM_DEVICE: device pointers, and the product
+ * must be complete in device memory when apply returns).  The library's
+ * stream is synchronized before each call.  A nonzero return fails the
+ * calling entry point with SBH_ERR_BACKEND.  The operator works with
+ * sbh_operator_apply(_block), sbh_gmres and sbh_block_gmres (one apply per
+ * column); its transpose and Hermitian products are Unsupported. */
+typedef int (*sbh_apply_fn)(void* user, const sbh_c64* x, sbh_c64* y, int64_t n, int memory);
+sbh_status sbh_operator_create_callback(sbh_context* ctx, int64_t n, sbh_apply_fn apply, void* user, int memory,
+                                        sbh_operator** out);
+
+/* ---------------------------------------------------------------------------
  * Preconditioner<Complex64> (traits.rs:3645-3780): identity
  * (IdentityPreconditioner) and the inverse diagonal of a dense operator. */
 sbh_status sbh_precond_identity(sbh_context* ctx, int64_t n, sbh_precond** out);
 sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out);
+/* A caller-supplied Preconditioner<Complex64> (traits.rs:3645-3780): y = M^-1 r,
+ * with the calling convention of sbh_operator_create_callback.  Accepted by
+ * sbh_gmres and sbh_precond_apply (sbh_block_gmres takes identity or Jacobi). */
+sbh_status sbh_precond_create_callback(sbh_context* ctx, int64_t n, sbh_apply_fn apply, void* user, int memory,
+                                       sbh_precond** out);
 void sbh_precond_destroy(sbh_precond* p);
 sbh_status sbh_precond_apply(sbh_precond* p, const sbh_c64* r, sbh_c64* y, int memory);
 
@@ -420,6 +502,8 @@ typedef struct {
     int64_t restart_cycles;
     int64_t max_basis;             /* largest basis reached (vectors) */
     int64_t deflated;              /* directions dropped by rank-drop deflation */
+    int64_t cholesky_qr_blocks;    /* block QRs done as Cholesky QR twice (clearly full rank) */
+    int64_t chain_qr_blocks;       /* block QRs done column by column (CGS2 with rank-drop deflation) */
 } sbh_block_gmres_info;
 
 /* B, X0 (NULL = zero) and X are n x k row-major (1 <= k <= 64) in `memory`;
@@ -539,6 +623,11 @@ typedef struct {
 /* status has sum(num_rhs) entries, instance-major. */
 sbh_status sbh_batch_solve(sbh_batch* batch, const sbh_batch_solve_config* config,
                            sbh_batch_column_status* status);
+/* Y = A X for every instance at once with the batch's FP64 matrices (format
+ * SBH_BATCH_FP64) or its FP16/BF16 copies (after sbh_batch_convert): X and Y
+ * are the packed host vectors (instance b's n_b x num_rhs_b row-major block
+ * after those of instances 0..b-1). */
+sbh_status sbh_batch_apply(sbh_batch* batch, int format, const sbh_c64* X, sbh_c64* Y);
 /* Initial guess of one instance (n x num_rhs row-major, host memory), used
  * by the next sbh_batch_solve with use_guess = 1 (gmres_with_guess,
  * gmres.rs:564-580; the relative residual stays relative to ||b||). */

@@ -46,6 +46,14 @@ class SbhError(RuntimeError):
         self.status = status
 
 
+INJECTION_POINTS = {"none": 0, "allocation": 1, "launch": 2, "h2d": 3, "d2h": 4, "synchronization": 5}
+
+
+def live_device_bytes() -> int:
+    """Bytes held by the library's device allocations in this process."""
+    return int(_lib().sbh_live_device_bytes())
+
+
 def _lib():
     return _ffi.load()
 
@@ -128,6 +136,20 @@ class Context:
 
     def synchronize(self):
         _check(_lib().sbh_context_synchronize(self._h))
+
+    def inject_failure(self, point: str) -> None:
+        """Arm a one-shot failure: "allocation", "launch", "h2d", "d2h" or "synchronization"."""
+        _check(_lib().sbh_context_inject_failure(self._h, INJECTION_POINTS[point]))
+
+    def set_device_budget(self, max_device_bytes: int) -> None:
+        """Device-memory admission budget of this context (0 = none)."""
+        _check(_lib().sbh_context_set_device_budget(self._h, int(max_device_bytes)))
+
+    def device_budget(self):
+        """(budget, bytes held by this context's systems, copies and batches)."""
+        b, h = ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib().sbh_context_device_budget(self._h, ctypes.byref(b), ctypes.byref(h)))
+        return int(b.value), int(h.value)
 
     def set_profiling(self, enabled: bool):
         _check(_lib().sbh_context_set_profiling(self._h, int(enabled)))
@@ -707,6 +729,42 @@ class TbemExactOperator(DenseMatrixOperator):
         self._h = h
 
 
+_APPLY_FN = _ffi.APPLY_FN
+
+
+def _host_callback(fn):
+    """A C apply callback (host memory) calling fn(x) -> y on complex128 arrays;
+    any exception becomes a nonzero status."""
+
+    def tramp(user, xp, yp, n, memory):
+        try:
+            x = np.ctypeslib.as_array((ctypes.c_double * (2 * n)).from_address(xp)).view(np.complex128)
+            y = np.ctypeslib.as_array((ctypes.c_double * (2 * n)).from_address(yp)).view(np.complex128)
+            out = np.asarray(fn(x.copy()), dtype=np.complex128).reshape(n)
+            if not np.all(np.isfinite(out)):
+                return 2
+            y[:] = out
+            return 0
+        except Exception:
+            return 1
+
+    return _APPLY_FN(tramp)
+
+
+class CallbackOperator(DenseMatrixOperator):
+    """A caller-supplied LinearOperator<Complex64> (traits.rs:2949-3290):
+    ``fn(x) -> A x`` on host arrays of length n, usable with gmres, block GMRES
+    and apply/apply_block; adjoint products are Unsupported."""
+
+    def __init__(self, ctx: "Context", n: int, fn):
+        self.system = None
+        self.ctx = ctx
+        self._cb = _host_callback(fn)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_operator_create_callback(ctx.handle, int(n), self._cb, None, MEM_HOST, ctypes.byref(h)))
+        self._h = h
+
+
 class _Precond:
     def apply(self, r) -> np.ndarray:
         rv = _c128(r, self.n)
@@ -724,6 +782,17 @@ class _Precond:
             self.close()
         except Exception:
             pass
+
+
+class CallbackPreconditioner(_Precond):
+    """A caller-supplied Preconditioner<Complex64> (traits.rs:3645-3780): ``fn(r) -> M^-1 r``."""
+
+    def __init__(self, ctx: "Context", n: int, fn):
+        self.n = int(n)
+        self._cb = _host_callback(fn)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_precond_create_callback(ctx.handle, self.n, self._cb, None, MEM_HOST, ctypes.byref(h)))
+        self._h = h
 
 
 class IdentityPreconditioner(_Precond):
@@ -860,6 +929,29 @@ def gmres_block(op: DenseMatrixOperator, B, config: GmresConfig, X0=None):
     return X, [_solution(raws[c], X[:, c]) for c in range(k)]
 
 
+def _estimate_dict(e) -> dict:
+    return {name: int(getattr(e, name)) for name, _ in _ffi.resource_estimate_t._fields_}
+
+
+def admission_estimate(active_dofs: int, rhs_count: int = 1, frequency_count: int = 1, restart: int = 30) -> dict:
+    """BemResourceRequest::estimate for the dense method (admission.rs:112-200),
+    plus this library's device bytes for the same work."""
+    e = _ffi.resource_estimate_t()
+    _check(_lib().sbh_admission_estimate(int(active_dofs), int(rhs_count), int(frequency_count), int(restart),
+                                         ctypes.byref(e)))
+    return _estimate_dict(e)
+
+
+def admission_admit(max_host_bytes: int, max_device_bytes: int, max_active_work: int, active_dofs: int,
+                    rhs_count: int = 1, frequency_count: int = 1, restart: int = 30) -> dict:
+    """BemResourceBudget::admit (admission.rs:284-312)."""
+    b = _ffi.resource_budget_t(int(max_host_bytes), int(max_device_bytes), int(max_active_work))
+    e = _ffi.resource_estimate_t()
+    _check(_lib().sbh_admission_admit(ctypes.byref(b), int(active_dofs), int(rhs_count), int(frequency_count),
+                                      int(restart), ctypes.byref(e)))
+    return _estimate_dict(e)
+
+
 @dataclass
 class BlockGmresConfig:
     """BlockGmresConfig (block_gmres.rs:58-89); defaults of BlockGmresConfig<f64>::default()."""
@@ -893,6 +985,8 @@ class BlockGmresSolution:
     restart_cycles: int
     max_basis: int
     deflated: int
+    cholesky_qr_blocks: int = 0
+    chain_qr_blocks: int = 0
 
 
 def _block_solution(info, x, res, tres, berr) -> BlockGmresSolution:
@@ -901,7 +995,8 @@ def _block_solution(info, x, res, tres, berr) -> BlockGmresSolution:
                               status=SOLVER_STATUS[int(info.status)], spmv_count=int(info.spmv_count),
                               exact_residual_checks=int(info.exact_residual_checks),
                               restart_cycles=int(info.restart_cycles), max_basis=int(info.max_basis),
-                              deflated=int(info.deflated))
+                              deflated=int(info.deflated), cholesky_qr_blocks=int(info.cholesky_qr_blocks),
+                              chain_qr_blocks=int(info.chain_qr_blocks))
 
 
 def block_gmres_preconditioned_with_guess(op: DenseMatrixOperator, B, precond=None, X0=None,
@@ -1171,6 +1266,25 @@ class Batch:
         """Initial guess of instance i (n x num_rhs), used by solve(use_guess=True)."""
         x = np.ascontiguousarray(X0, dtype=np.complex128).reshape(self.meshes[i].num_elements, int(self.num_rhs[i]))
         _check(_lib().sbh_batch_set_guess(self._h, int(i), _cptr(x)))
+
+    def apply(self, X, fmt: str = "fp64"):
+        """Products A_i X_i of every instance with its own matrix in one batched
+        launch (the stored format: "fp64", or "fp16"/"bf16" after convert).
+        X is a list of n_i x num_rhs[i] arrays; returns the list of products."""
+        if self.num_rhs is None:
+            raise ValueError("set_problem first: the column layout follows the right-hand sides")
+        if len(X) != len(self.meshes):
+            raise ValueError("one block per instance")
+        blocks = [np.ascontiguousarray(x, dtype=np.complex128).reshape(m.num_elements, int(c))
+                  for x, m, c in zip(X, self.meshes, self.num_rhs)]
+        flat = np.ascontiguousarray(np.concatenate([b.reshape(-1) for b in blocks]))
+        out = np.empty_like(flat)
+        _check(_lib().sbh_batch_apply(self._h, {"fp64": -1, "fp16": 0, "bf16": 1}[fmt], _cptr(flat), _cptr(out)))
+        res, o = [], 0
+        for b in blocks:
+            res.append(out[o:o + b.size].reshape(b.shape))
+            o += b.size
+        return res
 
     def _download(self, fn, i, shape):
         out = np.empty(shape, dtype=np.complex128)

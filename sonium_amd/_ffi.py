@@ -78,7 +78,18 @@ class block_gmres_config_t(ctypes.Structure):
 class block_gmres_info_t(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("converged", ctypes.c_int32), ("status", ctypes.c_int32),
                 ("spmv_count", ctypes.c_int64), ("exact_residual_checks", ctypes.c_int64),
-                ("restart_cycles", ctypes.c_int64), ("max_basis", ctypes.c_int64), ("deflated", ctypes.c_int64)]
+                ("restart_cycles", ctypes.c_int64), ("max_basis", ctypes.c_int64), ("deflated", ctypes.c_int64),
+                ("cholesky_qr_blocks", ctypes.c_int64), ("chain_qr_blocks", ctypes.c_int64)]
+
+
+class resource_budget_t(ctypes.Structure):
+    _fields_ = [("max_host_bytes", ctypes.c_int64), ("max_device_bytes", ctypes.c_int64),
+                ("max_active_work", ctypes.c_int64)]
+
+
+class resource_estimate_t(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "host_bytes", "device_bytes", "active_work", "operator_bytes", "rhs_bytes", "scratch_bytes")]
 
 
 class gmres_ir_config_t(ctypes.Structure):
@@ -118,6 +129,10 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
 
+# sbh_apply_fn: int (*)(void* user, const sbh_c64* x, sbh_c64* y, int64_t n, int memory)
+APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                            ctypes.c_int)
+
 # name: (restype, argtypes)
 SIGNATURES = {
     "sbh_last_error": (ctypes.c_char_p, []),
@@ -131,6 +146,13 @@ SIGNATURES = {
     "sbh_context_destroy": (None, [_vp]),
     "sbh_context_synchronize": (_int, [_vp]),
     "sbh_context_set_profiling": (_int, [_vp, _int]),
+    "sbh_context_set_device_budget": (_int, [_vp, _i64]),
+    "sbh_context_device_budget": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "sbh_admission_estimate": (_int, [_i64, _i64, _i64, _i64, ctypes.POINTER(resource_estimate_t)]),
+    "sbh_admission_admit": (_int, [ctypes.POINTER(resource_budget_t), _i64, _i64, _i64, _i64,
+                                   ctypes.POINTER(resource_estimate_t)]),
+    "sbh_context_inject_failure": (_int, [_vp, _int]),
+    "sbh_live_device_bytes": (_i64, []),
     "sbh_context_kernel_stats": (_int, [_vp, _int, ctypes.POINTER(_i64), _dp]),
     "sbh_context_reset_stats": (_int, [_vp]),
     "sbh_vector_create": (_int, [_vp, _i64, _pp]),
@@ -176,6 +198,7 @@ SIGNATURES = {
                                   _vp, _vp, _int]),
     "sbh_dense_operator_create": (_int, [_vp, _pp]),
     "sbh_exact_operator_create": (_int, [_vp, ctypes.POINTER(physics_t), c64, ctypes.c_double, _pp]),
+    "sbh_operator_create_callback": (_int, [_vp, _i64, APPLY_FN, _vp, _int, _pp]),
     "sbh_operator_destroy": (None, [_vp]),
     "sbh_operator_num_rows": (_i64, [_vp]),
     "sbh_operator_num_cols": (_i64, [_vp]),
@@ -188,6 +211,7 @@ SIGNATURES = {
     "sbh_operator_apply_block_noalloc": (_int, [_vp, _vp, _vp, _i64]),
     "sbh_precond_identity": (_int, [_vp, _i64, _pp]),
     "sbh_precond_jacobi": (_int, [_vp, _pp]),
+    "sbh_precond_create_callback": (_int, [_vp, _i64, APPLY_FN, _vp, _int, _pp]),
     "sbh_precond_destroy": (None, [_vp]),
     "sbh_precond_apply": (_int, [_vp, _vp, _vp, _int]),
     "sbh_gmres": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(gmres_config_t), _vp, _int,
@@ -210,6 +234,7 @@ SIGNATURES = {
     "sbh_batch_convert": (_int, [_vp, _int]),
     "sbh_batch_solve": (_int, [_vp, ctypes.POINTER(batch_solve_config_t), _vp]),
     "sbh_batch_set_guess": (_int, [_vp, _i64, _vp]),
+    "sbh_batch_apply": (_int, [_vp, _int, _vp, _vp]),
     "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_rhs": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_matrix": (_int, [_vp, _i64, _vp]),

@@ -32,7 +32,7 @@ thread_local std::string g_last_error;
 inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
 
 void require_dense(const sbh_operator* op, const char* what) {
-    if (op->exact()) throw Error(SBH_ERR_UNSUPPORTED, std::string(what) + " needs a dense (stored) operator");
+    if (!op->dense()) throw Error(SBH_ERR_UNSUPPORTED, std::string(what) + " needs a dense (stored) operator");
 }
 
 // Device copy of a host or device input vector.
@@ -44,6 +44,7 @@ struct DeviceInput {
             ptr = reinterpret_cast<const double2*>(p);
         } else {
             owned.allocate(n);
+            injection_check(INJECT_TRANSFER_H2D);
             if (n) SBH_HIP(hipMemcpyAsync(owned.ptr, p, n * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
             ptr = owned.ptr;
         }
@@ -67,8 +68,10 @@ struct DeviceOutput {
     }
     void finish() {
         if (host && n) {
+            injection_check(INJECT_TRANSFER_D2H);
             SBH_HIP(hipMemcpyAsync(host, ptr, n * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
         }
+        injection_check(INJECT_SYNCHRONIZATION);
         SBH_HIP(hipStreamSynchronize(ctx->stream));
     }
 };
@@ -86,14 +89,79 @@ bool all_finite(const sbh_c64* p, size_t n) {
 }  // namespace
 
 // y = A x on device vectors, for either kind of operator.
+void callback_apply_device(sbh_context* ctx, sbh_apply_fn fn, void* user, int memory, const double2* x, double2* y,
+                           int64_t n, const char* what) {
+    SBH_HIP(hipStreamSynchronize(ctx->stream));
+    int rc = 0;
+    if (memory == SBH_MEM_DEVICE) {
+        rc = fn(user, reinterpret_cast<const sbh_c64*>(x), reinterpret_cast<sbh_c64*>(y), n, SBH_MEM_DEVICE);
+    } else {
+        std::vector<sbh_c64> hx((size_t)n), hy((size_t)n);
+        SBH_HIP(hipMemcpyAsync(hx.data(), x, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        rc = fn(user, hx.data(), hy.data(), n, SBH_MEM_HOST);
+        if (rc == 0) {
+            SBH_HIP(hipMemcpyAsync(y, hy.data(), sizeof(double2) * n, hipMemcpyHostToDevice, ctx->stream));
+            SBH_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    if (rc != 0) throw Error(SBH_ERR_BACKEND, std::string(what) + " callback failed with status " + std::to_string(rc));
+}
+
 void operator_apply_device(const sbh_operator* op, const double2* x, double2* y) {
-    if (op->exact())
+    injection_check(INJECT_LAUNCH);
+    if (op->callback)
+        callback_apply_device(op->ctx, op->callback, op->user, op->callback_memory, x, y, op->n, "operator");
+    else if (op->exact())
         launch_exact_apply(op, x, y);
     else
         zgemv_n(op->ctx, op->sys->d_A.ptr, op->sys->lda, op->n, op->n, x, y);
 }
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+thread_local sbh_context* g_current_ctx = nullptr;
+std::atomic<int64_t> g_live_device_bytes{0};
+
+void injection_check(int point) {
+    sbh_context* ctx = g_current_ctx;
+    if (!ctx) return;
+    int expected = point;
+    if (!ctx->armed_failure.compare_exchange_strong(expected, INJECT_NONE)) return;
+    switch (point) {
+        case INJECT_ALLOCATION:
+            throw Error(SBH_ERR_OUT_OF_MEMORY, "injected allocation failure");
+        case INJECT_LAUNCH:
+            throw Error(SBH_ERR_BACKEND, "injected launch failure");
+        case INJECT_TRANSFER_H2D:
+            throw Error(SBH_ERR_BACKEND, "injected host-to-device transfer failure");
+        case INJECT_TRANSFER_D2H:
+            throw Error(SBH_ERR_BACKEND, "injected device-to-host transfer failure");
+        default:
+            throw Error(SBH_ERR_BACKEND, "injected synchronization failure");
+    }
+}
+
+void admit_device(sbh_context* ctx, int64_t bytes, const char* what, bool reserve) {
+    if (bytes <= 0) return;
+    const int64_t budget = ctx->device_budget.load();
+    const int64_t held = ctx->device_reserved.load();
+    if (budget > 0 && held + bytes > budget)
+        throw Error(SBH_ERR_OUT_OF_MEMORY, std::string("admission refused: ") + what + " needs " +
+                                               std::to_string(bytes) + " device bytes; the context's budget is " +
+                                               std::to_string(budget) + " with " + std::to_string(held) + " held");
+    size_t free_b = 0, total_b = 0;
+    SBH_HIP(hipMemGetInfo(&free_b, &total_b));
+    if ((uint64_t)bytes > (uint64_t)free_b)
+        throw Error(SBH_ERR_OUT_OF_MEMORY, std::string("admission refused: ") + what + " needs " +
+                                               std::to_string(bytes) + " device bytes; the device has " +
+                                               std::to_string(free_b) + " free of " + std::to_string(total_b));
+    if (reserve) ctx->device_reserved += bytes;
+}
+
+void release_device(sbh_context* ctx, int64_t bytes) {
+    if (bytes > 0) ctx->device_reserved -= bytes;
+}
 
 void context_retain(sbh_context* ctx) { ctx->refs.fetch_add(1); }
 
@@ -254,6 +322,98 @@ sbh_status sbh_context_synchronize(sbh_context* ctx) {
         SBH_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
+
+namespace {
+bool mul_ok(int64_t a, int64_t b, int64_t* r) { return !__builtin_mul_overflow(a, b, r); }
+bool add_ok(int64_t a, int64_t b, int64_t* r) { return !__builtin_add_overflow(a, b, r); }
+}  // namespace
+
+sbh_status sbh_admission_estimate(int64_t n, int64_t rhs, int64_t freq, int64_t restart,
+                                  sbh_resource_estimate* out) {
+    return guard([&] {
+        SBH_REQUIRE(out, "out is null");
+        SBH_REQUIRE(n > 0, "invalid request: active_dofs must be greater than zero");
+        SBH_REQUIRE(rhs > 0, "invalid request: rhs_count must be greater than zero");
+        SBH_REQUIRE(freq > 0, "invalid request: frequency_count must be greater than zero");
+        SBH_REQUIRE(restart > 0, "invalid request: restart must be greater than zero");
+        const int64_t C = 16;  // bytes of a complex double
+        auto prod = [](std::initializer_list<int64_t> v, const char* what) {
+            int64_t r = 1;
+            for (int64_t x : v)
+                if (!mul_ok(r, x, &r)) throw Error(SBH_ERR_INVALID_ARGUMENT, std::string("arithmetic overflow: ") + what);
+            return r;
+        };
+        auto sum = [](std::initializer_list<int64_t> v, const char* what) {
+            int64_t r = 0;
+            for (int64_t x : v)
+                if (!add_ok(r, x, &r)) throw Error(SBH_ERR_INVALID_ARGUMENT, std::string("arithmetic overflow: ") + what);
+            return r;
+        };
+        sbh_resource_estimate e{};
+        e.operator_bytes = prod({n, n, C}, "dense operator size");
+        const int64_t rhs_bytes = prod({n, rhs, C}, "right-hand-side size");
+        e.active_work = prod({freq, rhs}, "frequency/RHS work count");
+        e.scratch_bytes = prod({n, 4 * C + 8}, "assembly scratch size");
+        e.rhs_bytes = prod({rhs_bytes, freq}, "frequency right-hand-side size");
+        e.host_bytes = sum({e.operator_bytes, e.rhs_bytes, e.scratch_bytes}, "peak host allocation");
+        // this library: system (matrix + one right-hand side), right-hand
+        // sides and solutions, Krylov basis (shared block basis for rhs > 1)
+        const int64_t basis_vectors = rhs == 1 ? sum({restart, 3}, "basis size")
+                                               : prod({sum({restart, 2}, "basis size"), rhs}, "basis size");
+        e.device_bytes = sum({e.operator_bytes, prod({n, C}, "rhs"), prod({2, rhs_bytes}, "blocks"),
+                              prod({basis_vectors, n, C}, "Krylov basis"),
+                              prod({6, rhs_bytes}, "block scratch")},
+                             "peak device allocation");
+        *out = e;
+    });
+}
+
+sbh_status sbh_admission_admit(const sbh_resource_budget* budget, int64_t n, int64_t rhs, int64_t freq,
+                               int64_t restart, sbh_resource_estimate* out) {
+    return guard([&] {
+        SBH_REQUIRE(budget && out, "null argument");
+        SBH_REQUIRE(budget->max_host_bytes > 0, "invalid budget: max_host_bytes must be greater than zero");
+        SBH_REQUIRE(budget->max_active_work > 0, "invalid budget: max_active_work must be greater than zero");
+        sbh_resource_estimate e{};
+        const sbh_status st = sbh_admission_estimate(n, rhs, freq, restart, &e);
+        if (st != SBH_OK) throw Error(st, sbh_last_error());
+        auto exceeded = [](const char* res, int64_t req, int64_t lim) {
+            throw Error(SBH_ERR_OUT_OF_MEMORY, std::string("limit exceeded: ") + res + " requested " +
+                                                   std::to_string(req) + ", limit " + std::to_string(lim));
+        };
+        if (e.host_bytes > budget->max_host_bytes) exceeded("host memory", e.host_bytes, budget->max_host_bytes);
+        if (e.device_bytes > budget->max_device_bytes)
+            exceeded("device memory", e.device_bytes, budget->max_device_bytes);
+        if (e.active_work > budget->max_active_work) exceeded("active work", e.active_work, budget->max_active_work);
+        *out = e;
+    });
+}
+
+sbh_status sbh_context_set_device_budget(sbh_context* ctx, int64_t bytes) {
+    return guard_on(ctx_of(ctx), [&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        SBH_REQUIRE(bytes >= 0, "the device budget must be >= 0");
+        ctx->device_budget = bytes;
+    });
+}
+
+sbh_status sbh_context_device_budget(sbh_context* ctx, int64_t* budget, int64_t* held) {
+    return guard_on(ctx_of(ctx), [&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        if (budget) *budget = ctx->device_budget.load();
+        if (held) *held = ctx->device_reserved.load();
+    });
+}
+
+sbh_status sbh_context_inject_failure(sbh_context* ctx, int point) {
+    return guard([&] {
+        SBH_REQUIRE(ctx, "ctx is null");
+        SBH_REQUIRE(point >= SBH_INJECT_NONE && point <= SBH_INJECT_SYNCHRONIZATION, "unknown injection point");
+        ctx->armed_failure = point;
+    });
+}
+
+int64_t sbh_live_device_bytes(void) { return g_live_device_bytes.load(); }
 
 sbh_status sbh_context_set_profiling(sbh_context* ctx, int enabled) {
     return guard_on(ctx_of(ctx), [&] {
@@ -526,6 +686,9 @@ sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** ou
         SBH_REQUIRE(num_dofs >= 1, "num_dofs must be positive");
         *out = nullptr;
         use_device(ctx);
+        const int64_t bytes = (int64_t)sizeof(double2) * (num_dofs * num_dofs + num_dofs);
+        SBH_REQUIRE(num_dofs <= ((int64_t)1 << 31), "num_dofs too large");
+        admit_device(ctx, bytes, "a dense system", false);
         std::unique_ptr<sbh_system> s(new sbh_system());
         s->ctx = ctx;
         s->n = num_dofs;
@@ -534,6 +697,8 @@ sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** ou
         s->d_rhs.allocate((size_t)num_dofs);
         SBH_HIP(hipMemsetAsync(s->d_rhs.ptr, 0, s->d_rhs.bytes(), ctx->stream));
         SBH_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->device_reserved += bytes;
+        s->admitted = bytes;
         context_retain(ctx);
         *out = s.release();
     });
@@ -544,6 +709,7 @@ void sbh_system_destroy(sbh_system* sys) {
     sbh_context* ctx = sys->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    release_device(ctx, sys->admitted);
     delete sys;
     context_release(ctx);
 }
@@ -909,10 +1075,10 @@ void sbh_operator_destroy(sbh_operator* op) {
 int64_t sbh_operator_num_rows(const sbh_operator* op) { return op ? op->n : 0; }
 int64_t sbh_operator_num_cols(const sbh_operator* op) { return op ? op->n : 0; }
 int sbh_operator_transpose_capability(const sbh_operator* op) {
-    return !op ? SBH_ADJOINT_UNKNOWN : op->exact() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
+    return !op ? SBH_ADJOINT_UNKNOWN : !op->dense() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
 }
 int sbh_operator_hermitian_capability(const sbh_operator* op) {
-    return !op ? SBH_ADJOINT_UNKNOWN : op->exact() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
+    return !op ? SBH_ADJOINT_UNKNOWN : !op->dense() ? SBH_ADJOINT_UNSUPPORTED : SBH_ADJOINT_EXACT;
 }
 
 
@@ -958,7 +1124,7 @@ sbh_status sbh_operator_apply_block(sbh_operator* op, const sbh_c64* X, sbh_c64*
         DeviceOutput outv(op->ctx, Y, (size_t)(n * k), memory);
         if (k == 1) {
             operator_apply_device(op, in.ptr, outv.ptr);
-        } else if (op->exact()) {
+        } else if (!op->dense()) {
             // column by column (the trait's default apply_block): strided gather, product, scatter
             DeviceArray<double2> xc((size_t)n), yc((size_t)n);
             for (int64_t c = 0; c < k; ++c) {
@@ -979,7 +1145,7 @@ sbh_status sbh_operator_apply_block_noalloc(sbh_operator* op, const sbh_c64* X, 
     return guard_on(ctx_of(op), [&] {
         SBH_REQUIRE(op && X && Y, "operator/X/Y is null");
         SBH_REQUIRE(k >= 1, "block width must be positive");
-        if (op->exact())
+        if (!op->dense())
             throw Error(SBH_ERR_UNSUPPORTED, "the allocation-free block product needs a dense (stored) operator");
         const int64_t n = op->n;
         if (k >= 2) {
@@ -1029,6 +1195,39 @@ sbh_status sbh_precond_jacobi(sbh_operator* op, sbh_precond** out) {
     });
 }
 
+sbh_status sbh_precond_create_callback(sbh_context* ctx, int64_t n, sbh_apply_fn apply, void* user, int memory,
+                                       sbh_precond** out) {
+    return guard_on(ctx_of(ctx), [&] {
+        SBH_REQUIRE(ctx && apply && out && n >= 1, "invalid callback preconditioner arguments");
+        check_memory(memory);
+        std::unique_ptr<sbh_precond> p(new sbh_precond());
+        p->ctx = ctx;
+        p->n = n;
+        p->kind = 2;
+        p->callback = apply;
+        p->user = user;
+        p->callback_memory = memory;
+        context_retain(ctx);
+        *out = p.release();
+    });
+}
+
+sbh_status sbh_operator_create_callback(sbh_context* ctx, int64_t n, sbh_apply_fn apply, void* user, int memory,
+                                        sbh_operator** out) {
+    return guard_on(ctx_of(ctx), [&] {
+        SBH_REQUIRE(ctx && apply && out && n >= 1, "invalid callback operator arguments");
+        check_memory(memory);
+        std::unique_ptr<sbh_operator> op(new sbh_operator());
+        op->ctx = ctx;
+        op->n = n;
+        op->callback = apply;
+        op->user = user;
+        op->callback_memory = memory;
+        context_retain(ctx);
+        *out = op.release();
+    });
+}
+
 void sbh_precond_destroy(sbh_precond* p) {
     if (!p) return;
     sbh_context* ctx = p->ctx;
@@ -1047,8 +1246,11 @@ sbh_status sbh_precond_apply(sbh_precond* p, const sbh_c64* r, sbh_c64* y, int m
         DeviceOutput outv(p->ctx, y, (size_t)p->n, memory);
         if (p->kind == 0)
             SBH_HIP(hipMemcpyAsync(outv.ptr, in.ptr, sizeof(double2) * p->n, hipMemcpyDeviceToDevice, p->ctx->stream));
-        else
+        else if (p->kind == 1)
             diag_scale(p->ctx, p->d_dinv.ptr, in.ptr, outv.ptr, p->n);
+        else
+            callback_apply_device(p->ctx, p->callback, p->user, p->callback_memory, in.ptr, outv.ptr, p->n,
+                                  "preconditioner");
         outv.finish();
     });
 }
@@ -1115,6 +1317,8 @@ sbh_status sbh_block_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64
         SBH_REQUIRE(config->exact_residual_check_interval >= 0, "exact_residual_check_interval must be >= 0");
         if (precond) {
             SBH_REQUIRE(precond->ctx == op->ctx && precond->n == op->n, "preconditioner does not match the operator");
+            if (precond->kind == 2)
+                throw Error(SBH_ERR_UNSUPPORTED, "block GMRES takes the identity or Jacobi preconditioner");
         }
         const int64_t n = op->n;
         if (memory == SBH_MEM_HOST) {
@@ -1162,11 +1366,15 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         SBH_REQUIRE(sys && out, "system/out is null");
         SBH_REQUIRE(format == SBH_LOWP_FP16 || format == SBH_LOWP_BF16, "format must be FP16 or BF16");
         use_device(sys->ctx);
+        const int64_t ld = lowp_padded_ld(sys->n);
+        const int64_t bytes = 2 * lowp_padded_rows(sys->n) * 2 * ld + 8 * sys->n + 2 * 64 * 2 * ld +
+                              4 * lowp_split_count() * sys->n * 128 + 64 * 12;
+        admit_device(sys->ctx, bytes, "a low-precision matrix copy", false);
         std::unique_ptr<sbh_lowp_matrix> lp(new sbh_lowp_matrix());
         lp->ctx = sys->ctx;
         lp->sys = sys;
         lp->n = sys->n;
-        lp->ld = lowp_padded_ld(sys->n);
+        lp->ld = ld;
         lp->format = format;
         lp->d_A.allocate((size_t)lowp_padded_rows(lp->n) * 2 * lp->ld);
         lp->d_row_inv.allocate((size_t)lp->n);
@@ -1177,6 +1385,8 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         SBH_HIP(hipMemsetAsync(lp->d_XT.ptr, 0, lp->d_XT.bytes(), sys->ctx->stream));
         lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr);
         SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
+        sys->ctx->device_reserved += bytes;
+        lp->admitted = bytes;
         context_retain(sys->ctx);
         *out = lp.release();
     });
@@ -1196,6 +1406,7 @@ void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp) {
     sbh_context* ctx = lp->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    release_device(ctx, lp->admitted);
     delete lp;
     context_release(ctx);
 }

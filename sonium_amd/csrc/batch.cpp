@@ -403,6 +403,7 @@ void sbh_batch_destroy(sbh_batch* b) {
     sbh_context* ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    release_device(ctx, (int64_t)b->d_A.bytes());
     delete b;
     context_release(ctx);
 }
@@ -472,8 +473,13 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
         if (b->d_dirs.count != (size_t)dir_off * 3) b->d_dirs.allocate((size_t)dir_off * 3);
         copy_on_stream(b->ctx, b->d_dirs.ptr, directions, (size_t)dir_off * 3 * sizeof(double), hipMemcpyHostToDevice);
         if (b->d_A.count != (size_t)a_off) {
-            b->d_A.allocate((size_t)a_off);
+            const int64_t bytes = (int64_t)sizeof(double2) * a_off;
+            const int64_t old_bytes = (int64_t)b->d_A.bytes();
+            b->d_A.release();
             b->solver.reset();
+            release_device(b->ctx, old_bytes);
+            admit_device(b->ctx, bytes, "the batch's FP64 matrices", true);
+            b->d_A.allocate((size_t)a_off);
         }
         if (b->d_rhs.count != (size_t)vec_off) {
             b->d_rhs.allocate((size_t)vec_off);
@@ -659,6 +665,27 @@ sbh_status sbh_batch_set_guess(sbh_batch* b, int64_t instance, const sbh_c64* x0
         for (size_t i = 0; i < count; ++i)
             SBH_REQUIRE(std::isfinite(x0[i].re) && std::isfinite(x0[i].im), "initial guess is not finite");
         copy_on_stream(b->ctx, b->d_x.ptr + I.vec_off, x0, count * sizeof(double2), hipMemcpyHostToDevice);
+    });
+}
+
+sbh_status sbh_batch_apply(sbh_batch* b, int format, const sbh_c64* X, sbh_c64* Y) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(X && Y, "X/Y is null");
+        SBH_REQUIRE(format == SBH_BATCH_FP64 || format == 0 || format == 1, "format must be FP64 (-1), FP16 or BF16");
+        SBH_REQUIRE(format < 0 || (b->d_Alp.count == (size_t)b->lp_elems && b->lp_format == format),
+                    "the batch has no copy in that format (sbh_batch_convert)");
+        use_device(b->ctx);
+        const size_t L = (size_t)b->L;
+        for (size_t i = 0; i < L; ++i)
+            SBH_REQUIRE(std::isfinite(X[i].re) && std::isfinite(X[i].im), "X is not finite");
+        DeviceArray<double2> dx(L), dy(L);
+        DeviceArray<int> active((size_t)b->B);
+        fill_int(b->ctx, active, 1);
+        copy_on_stream(b->ctx, dx.ptr, X, L * sizeof(double2), hipMemcpyHostToDevice);
+        batch_product(b->ctx, format, b->max_cols, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, active.ptr, b->d_A.ptr,
+                      format < 0 ? nullptr : b->d_Alp.ptr, format < 0 ? nullptr : b->d_row_inv.ptr, dx.ptr, dy.ptr);
+        copy_on_stream(b->ctx, Y, dy.ptr, L * sizeof(double2), hipMemcpyDeviceToHost);
     });
 }
 

@@ -152,6 +152,8 @@ class BatchedGmres {
     BatchedGmres(sbh_context* ctx, int64_t n, int k, int m, bool flexible)
         : ctx_(ctx), n_(n), k_(k), m_(m), flexible_(flexible) {
         const size_t blk = (size_t)n * k;
+        admit_device(ctx, (int64_t)(sizeof(T) * blk * ((size_t)(m + 1) + (flexible ? (size_t)m : 0) + 3)),
+                     "the block GMRES Krylov basis", false);
         V_.allocate((size_t)(m + 1) * blk);
         if (flexible) Z_.allocate((size_t)m * blk);
         W_.allocate(blk);

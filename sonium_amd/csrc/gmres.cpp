@@ -61,8 +61,10 @@ struct Krylov {
     void precond(const double2* r, double2* y) {
         if (!M || M->kind == 0) {
             if (r != y) SBH_HIP(hipMemcpyAsync(y, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
-        } else {
+        } else if (M->kind == 1) {
             diag_scale(ctx, M->d_dinv.ptr, r, y, n);
+        } else {
+            callback_apply_device(ctx, M->callback, M->user, M->callback_memory, r, y, n, "preconditioner");
         }
     }
     double fetch_sum(int count) {
@@ -134,6 +136,7 @@ sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const
     const int max_m = alloc_m;
     const int64_t check_every = std::max<int64_t>(1, cfg.convergence_check_interval);
     Krylov K(ctx, op, M, n, alloc_m);
+    admit_device(ctx, (int64_t)sizeof(double2) * ((int64_t)(alloc_m + 1) * n + 2 * n), "the GMRES Krylov basis", false);
     DeviceArray<double2> V((size_t)(alloc_m + 1) * n), w(n), opb(n);
     // Initial guess.
     if (d_x0)

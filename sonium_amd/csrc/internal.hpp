@@ -49,6 +49,22 @@ sbh_status guard(F&& f) {
         }                                                                                     \
     } while (0)
 
+// One-shot failure injection (the reference's CudaFailureInjectionPoint,
+// crates/sonium-compute/src/backend/mod.rs:120-150): a point armed on a
+// context fails the next boundary of that kind reached by an entry point
+// running on that context, before the boundary's effect.
+enum InjectionPoint {
+    INJECT_NONE = 0,
+    INJECT_ALLOCATION = 1,
+    INJECT_LAUNCH = 2,
+    INJECT_TRANSFER_H2D = 3,
+    INJECT_TRANSFER_D2H = 4,
+    INJECT_SYNCHRONIZATION = 5
+};
+void injection_check(int point);
+// Bytes held by the library's device allocations in this process.
+extern std::atomic<int64_t> g_live_device_bytes;
+
 #define SBH_REQUIRE(cond, msg)                                              \
     do {                                                                    \
         if (!(cond)) throw ::sbh::Error(SBH_ERR_INVALID_ARGUMENT, (msg));   \
@@ -72,11 +88,16 @@ struct DeviceArray {
     void allocate(size_t n) {
         release();
         if (n == 0) return;
+        injection_check(INJECT_ALLOCATION);
         SBH_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)));
         count = n;
+        g_live_device_bytes += (int64_t)(n * sizeof(T));
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            (void)hipFree(ptr);
+            g_live_device_bytes -= (int64_t)(count * sizeof(T));
+        }
         ptr = nullptr;
         count = 0;
     }
@@ -214,6 +235,12 @@ struct sbh_context {
     // events but creates none once the pool is warm.
     std::vector<hipEvent_t> event_pool;
     hipEvent_t take_event();
+    // Device-memory admission (the reference's BemResourceBudget, device
+    // side): an optional budget in bytes (0 = none) and the bytes held by this
+    // context's long-lived objects (systems, low-precision copies, batches).
+    std::atomic<int64_t> device_budget{0};
+    std::atomic<int64_t> device_reserved{0};
+    std::atomic<int> armed_failure{0};  // sbh::InjectionPoint, one shot
     // device scratch for partial sums of transposed products, grown on demand
     double2* scratch = nullptr;
     size_t scratch_count = 0;
@@ -263,6 +290,7 @@ struct sbh_system {
     sbh_context* ctx = nullptr;
     int64_t n = 0;
     int64_t lda = 0;
+    int64_t admitted = 0;  // device bytes counted against the context's budget
     sbh::DeviceArray<double2> d_A;
     sbh::DeviceArray<double2> d_rhs;
 };
@@ -281,6 +309,7 @@ struct sbh_lowp_matrix {
     int64_t n = 0;
     int64_t ld = 0;  // complex entries per row, multiple of 64
     int format = 0;  // 0 FP16, 1 BF16
+    int64_t admitted = 0;  // device bytes counted against the context's budget
     sbh::DeviceArray<uint16_t> d_A;       // ceil(n/32)*32 x 2*ld halves, MFMA fragment order
     sbh::DeviceArray<double> d_row_inv;   // n
     sbh::DeviceArray<uint16_t> d_XT;      // 64 x 2*ld halves (real-part block operand)
@@ -300,14 +329,21 @@ struct sbh_operator {
     double2 beta{};
     double sign = 1.0;
     sbh::DeviceArray<double2> d_far, d_near, d_self;  // exact-product partials
+    sbh_apply_fn callback = nullptr;                  // caller-supplied
+    void* user = nullptr;
+    int callback_memory = 0;
     bool exact() const { return ws != nullptr; }
+    bool dense() const { return sys != nullptr; }
 };
 
 struct sbh_precond {
     sbh_context* ctx = nullptr;
     int64_t n = 0;
-    int kind = 0;  // 0 identity, 1 inverse diagonal
+    int kind = 0;  // 0 identity, 1 inverse diagonal, 2 caller-supplied
     sbh::DeviceArray<double2> d_dinv;
+    sbh_apply_fn callback = nullptr;
+    void* user = nullptr;
+    int callback_memory = 0;
 };
 
 namespace sbh {
@@ -338,15 +374,33 @@ inline sbh_context* ctx_of(const sbh_lowp_matrix* l) { return l ? l->ctx : nullp
 // guard() with the context's operation lock held and its device current on
 // the calling thread.  A null context runs f unlocked, so f reports its own
 // null-argument error.
+// The context whose entry point the calling thread is running (failure
+// injection is attributed to it).
+extern thread_local sbh_context* g_current_ctx;
+
 template <typename F>
 sbh_status guard_on(sbh_context* ctx, F&& f) {
     if (!ctx) return guard(std::forward<F>(f));
     std::lock_guard<std::recursive_mutex> lock(ctx->op_mu);
+    struct Current {
+        sbh_context* prev;
+        explicit Current(sbh_context* c) : prev(g_current_ctx) { g_current_ctx = c; }
+        ~Current() { g_current_ctx = prev; }
+    } current(ctx);
     return guard([&] {
         SBH_HIP(hipSetDevice(ctx->device));
         f();
     });
 }
+
+// Admission before a device allocation of `bytes` (BemResourceRequest
+// admission, crates/sonium-bem/src/core/admission.rs): refused with
+// SBH_ERR_OUT_OF_MEMORY when it would exceed the context's budget (counting
+// the bytes its long-lived objects hold) or the device's free memory, before
+// anything is allocated.  With `reserve` the bytes are added to the
+// context's held bytes (release them with release_device).
+void admit_device(sbh_context* ctx, int64_t bytes, const char* what, bool reserve);
+void release_device(sbh_context* ctx, int64_t bytes);
 
 void context_retain(sbh_context* ctx);
 void context_release(sbh_context* ctx);
@@ -381,8 +435,11 @@ void launch_field(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, co
 // rhs (+)= sign * -(gamma p_inc + beta tau dp_inc/dn) of the summed sources.
 void launch_source_rhs(sbh_context* ctx, const sbh_mesh* m, const sbh_physics& ph, sbh_c64 beta, double sign,
                        const IncidentSource* d_sources, int64_t nsources, double2* rhs, bool accumulate);
-// y = A x for a dense or exact operator (api.cpp).
+// y = A x for a dense, exact or caller-supplied operator (api.cpp).
 void operator_apply_device(const sbh_operator* op, const double2* x, double2* y);
+// y = M^-1 r through a caller-supplied callback (api.cpp), device vectors.
+void callback_apply_device(sbh_context* ctx, sbh_apply_fn fn, void* user, int memory, const double2* x, double2* y,
+                           int64_t n, const char* what);
 void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physics& ph,
                          sbh_c64 beta, double sign, const double* d_dirs, const double2* d_amps,
                          int64_t num_rhs, double2* d_rhs, bool accumulate);

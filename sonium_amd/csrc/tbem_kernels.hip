@@ -1469,6 +1469,7 @@ void upload_quadrature_tables(sbh_context* ctx) {
 }
 
 void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, sbh_system* sys) {
+    injection_check(INJECT_LAUNCH);
     sbh_context* ctx = ws->ctx;
     const sbh_mesh* m = ws->mesh;
     const int64_t n = m->n;

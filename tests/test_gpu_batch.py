@@ -164,3 +164,29 @@ def test_batch_narrow_columns_fp64_path(ctx):
     b.convert("fp16")
     sts = b.solve("fp16_ir", restart=40, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4)
     _check_solution(b, specs, sts, 1e-8)
+
+
+def test_batch_apply_products(ctx, small_batch):
+    """sbh_batch_apply: every instance's product with its own matrix in one
+    launch.  FP64 agrees with the downloaded matrix times X to rounding; the
+    FP16 and BF16 copies (per-row scaled) to their unit roundoff."""
+    b = small_batch["batch"]
+    rng = np.random.default_rng(11)
+    X = [rng.standard_normal((s["mesh"].num_elements, int(c))) + 1j * rng.standard_normal((s["mesh"].num_elements, int(c)))
+         for s, c in zip(small_batch["specs"], b.num_rhs)]
+    ref = [b.matrix(i) @ x for i, x in enumerate(X)]
+    Y = b.apply(X, "fp64")
+    for y, r in zip(Y, ref):
+        assert rel_fro(y, r) < 1e-14
+    for fmt, tol in (("fp16", 2e-3), ("bf16", 1.5e-2)):
+        b.convert(fmt)
+        Y = b.apply(X, fmt)
+        errs = [rel_fro(y, r) for y, r in zip(Y, ref)]
+        print(fmt, "max relative error of the batched product", max(errs))
+        assert max(errs) < tol
+    with pytest.raises(sa.SbhError):
+        b.apply(X, "fp16")  # the batch now holds the BF16 copy only
+    Xn = [x.copy() for x in X]
+    Xn[2][0, 0] = np.nan
+    with pytest.raises(sa.SbhError):
+        b.apply(Xn, "fp64")
