@@ -12,9 +12,13 @@ Default workload: rigid sphere, icosphere with 20480 panels, k = 10,
 beta = i/k, one plane wave along +z, GMRES(100), rtol 1e-8, FP64.
 
 Printed: one JSON line with the metric, the per-kernel roofline of the
-dominant kernel (timed with HIP events on the library's stream), and the CPU
-baseline (the oracle's C++ restatement of the reference algorithm timed on a
-sample of rows on this host's cores).
+dominant kernel (timed with HIP events on the library's stream inside the
+timed region; the events come from a per-context pool, and the plain and
+mixed-precision legs are timed the same way), the chosen form against the
+plain FP64 path for every configuration (the 40960-panel 64-RHS solve, the
+256-ellipsoid batch and the frequency sweep are appended), and the CPU
+baseline (the oracle's C++ restatement of the reference algorithm on this
+host's cores).
 """
 
 from __future__ import annotations
@@ -60,10 +64,46 @@ WORKLOADS = {
 }
 
 
+def host_info():
+    """CPU model, logical CPUs, the CPUs this process may run on, and the
+    thread count used: the affinity set, capped by OMP_NUM_THREADS when the
+    host sets it (the GPU hosts allot each GPU a share of their cores that
+    way)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    threads = affinity
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"cpu_model": model, "nproc": nproc, "affinity_cpus": affinity, "threads_used": threads,
+            "omp_num_threads": omp or None}
+
+
 def cpu_baseline(m, k, beta, spmv_count, rows, threads):
-    """Oracle (C++ restatement of the reference CPU path) on `rows` rows; extrapolated to N."""
+    """The oracle (C++ restatement of the reference CPU path) on this host.
+
+    value: the C2 step (20480 panels) from the assembly of rows [0, rows) and
+    three matvecs of that row block, extrapolated to N (the full matrix is
+    6.7 GB and its assembly minutes of CPU time).  Beside it: the 1280-panel
+    solve timed in full, and the reference's own crossover measurement
+    (bem_assembly_crossover.rs:128-152: build_tbem_system_stabilized + 10
+    dense matvecs, icosphere, ka = 1; published 6.20 s at N = 1280 and
+    83.50 s at N = 5120, bem_solver.rs:222-231) restated at both sizes."""
     import oracle
 
+    info = host_info()
     n = m.num_elements
     rows = min(rows, n)
     t0 = time.perf_counter()
@@ -78,6 +118,32 @@ def cpu_baseline(m, k, beta, spmv_count, rows, threads):
     scale = n / rows
     t_solve = spmv_count * t_mv * scale
     t_total = t_asm * scale + t_solve
+    del A_rows
+    # 1280 panels, k = 2, GMRES(50): timed in full
+    m1 = icosphere(1.0, 3)
+    t0 = time.perf_counter()
+    A1, _ = oracle.build_tbem_system(m1.nodes, m1.triangles, 2.0, 0.5j, threads=threads)
+    c1, n1, _ = m1.element_geometry()
+    b1 = oracle.plane_wave_rhs(c1, n1, 2.0, 0.5j)
+    t1a = time.perf_counter() - t0
+    x1, st1 = oracle.gmres(A1, b1, restart=50, max_iterations=100, tol=1e-8, threads=threads)
+    t1 = time.perf_counter() - t0
+    # the reference's crossover sizes (ka = 1 on the unit sphere)
+    cross = {}
+    for subdiv, th, published in ((3, 1, 6.20), (3, threads, 6.20), (4, threads, 83.50)):
+        mm = icosphere(1.0, subdiv)
+        t0 = time.perf_counter()
+        Am, _ = oracle.build_tbem_system(mm.nodes, mm.triangles, 1.0, 1.0j, threads=th)
+        ta = time.perf_counter() - t0
+        xm = np.ones(mm.num_elements, dtype=np.complex128)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            oracle.zgemv(Am, xm, threads=th)
+        tmv = time.perf_counter() - t0
+        cross[f"n{mm.num_elements}_threads{th}"] = {"assembly_s": ta, "matvec10_s": tmv, "total_s": ta + tmv,
+                                                     "reference_published_s": published,
+                                                     "reference_over_restatement": published / (ta + tmv)}
+        del Am
     return {
         "value": 1.0 / t_total,
         "unit": "RHS/s",
@@ -86,13 +152,18 @@ def cpu_baseline(m, k, beta, spmv_count, rows, threads):
         "sample": (f"oracle assembly of rows [0,{rows}) of {n} and {reps} matvecs of that row block, "
                    f"extrapolated x{scale:.1f}; solve = {spmv_count} matvecs (the GPU run's count); "
                    f"assembly {t_asm * scale:.1f} s + solve {t_solve:.1f} s per RHS"),
+        "host": info,
+        "oracle_library": os.path.basename(oracle.library_path()),
+        "c1_full": {"panels": m1.num_elements, "assembly_s": t1a, "total_s": t1, "iterations": st1["iterations"],
+                    "rhs_per_s": 1.0 / t1},
+        "reference_crossover": cross,
     }
 
 
 def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
     """Mixed-precision form of the same step: assemble FP64, convert to the
     row-scaled complex FP16/BF16 copy, FGMRES-IR to an FP64-recomputed
-    residual of 1e-8."""
+    residual of 1e-8.  Timed like the plain leg: kernel events on (pooled)."""
     system = sa.TbemSystem(ctx, n)
     op = sa.DenseMatrixOperator(system)
     inc = sa.IncidentField.plane_wave_z()
@@ -109,11 +180,15 @@ def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
         return sa.gmres_ir_device(op, lp, bd, 1, xd, cfg)
 
     step()
+    ctx.set_profiling(True)
+    ctx.reset_stats()
     ctx.synchronize()
     t0 = time.perf_counter()
     outs = [step() for _ in range(steps)]
     ctx.synchronize()
     el = time.perf_counter() - t0
+    stats = {name: ctx.kernel_stats(name) for name in ("lowp", "matvec", "krylov_vector", "assemble_far")}
+    ctx.set_profiling(False)
     st = outs[-1]
     x = xd.download()
     if not all(o[0].converged and o[0].final_outer_residual <= 1e-8 for o in outs):
@@ -121,11 +196,19 @@ def run_ir_leg(ctx, ws, tm, ph, beta, n, steps, fmt, inner_tol):
     return {"form": f"{fmt} storage + FP32 accumulate, FGMRES-IR with FP64 residual/Arnoldi", "value": steps / el,
             "ms_per_step": el / steps * 1e3, "outer_iterations": st[0].outer_iterations,
             "inner_iterations": st[0].total_inner_iterations, "residual": st[0].final_outer_residual,
-            "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products}, x
+            "fp64_products": st[0].fp64_products, "lowp_products": st[0].lowp_products,
+            "kernel_ms_per_step": {k: v[1] / steps for k, v in stats.items()}}, x
 
 
 def run_c3(args, ctx=None):
-    """Many right-hand sides: 40960-panel Fibonacci sphere, k = 14, 64 plane waves."""
+    """Many right-hand sides: 40960-panel Fibonacci sphere, k = 14, 64 plane waves.
+
+    plain: the reference's multi-RHS solver (solve_many -> block GMRES with one
+    shared block Krylov basis, block_gmres.rs:666-1120) in FP64, restart 100.
+    Also timed: the column-by-column GMRES (each column the single-RHS
+    method, one block product per step).  Chosen: FP16 and BF16 storage with
+    FGMRES iterative refinement (FP64 residual, Arnoldi and Givens).
+    """
     ctx = ctx or sa.Context(0)
     m = fibonacci_sphere(1.0, 129, 160)
     n = m.num_elements
@@ -147,19 +230,38 @@ def run_c3(args, ctx=None):
     Xd = sa.DeviceVector(ctx, n * 64)
     inc.write_device(B, tm, ph, beta)
     res = {}
+    ctx.set_profiling(True)
     ctx.synchronize()
     ctx.reset_stats()
-    ctx.set_profiling(True)
     t0 = time.perf_counter()
-    sols = sa.gmres_block_device(op, B, 64, Xd, sa.GmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
-    t_plain = time.perf_counter() - t0
+    bsol = sa.block_gmres_device(op, B, 64, Xd, sa.BlockGmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
+    ctx.synchronize()
+    t_block = time.perf_counter() - t0
     X64 = Xd.download().reshape(n, 64)
     blk = ctx.kernel_stats("block")
-    res["plain_fp64"] = {"solve_s": t_plain, "rhs_per_s_excl_assembly": 64 / t_plain,
-                         "rhs_per_s_incl_assembly": 64 / (t_plain + t_asm),
-                         "max_iterations": max(s.iterations for s in sols),
-                         "max_residual": max(s.residual for s in sols),
-                         "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
+    kry = ctx.kernel_stats("krylov_vector")
+    if not (bsol.converged and bsol.true_residuals.max() <= 1e-8):
+        raise SystemExit(f"bench c3: block GMRES did not converge: {bsol.true_residuals.max()}")
+    res["plain_fp64"] = {"solver": "block GMRES, shared basis (solve_many)", "solve_s": t_block,
+                         "rhs_per_s_excl_assembly": 64 / t_block, "rhs_per_s_incl_assembly": 64 / (t_block + t_asm),
+                         "block_steps": bsol.iterations, "basis_vectors": bsol.max_basis,
+                         "column_products": bsol.spmv_count, "max_residual": float(bsol.true_residuals.max()),
+                         "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1), "fp64_block_products": blk[0],
+                         "block_krylov_ms": kry[1]}
+    ctx.reset_stats()
+    X2 = sa.DeviceVector(ctx, n * 64)
+    t0 = time.perf_counter()
+    sols = sa.gmres_block_device(op, B, 64, X2, sa.GmresConfig(max_iterations=100, restart=100, tolerance=1e-8))
+    ctx.synchronize()
+    t_cols = time.perf_counter() - t0
+    Xc = X2.download().reshape(n, 64)
+    X2.close()
+    res["per_column_fp64"] = {"solver": "GMRES per column, one block product per step", "solve_s": t_cols,
+                              "rhs_per_s_excl_assembly": 64 / t_cols,
+                              "max_iterations": max(s.iterations for s in sols),
+                              "max_residual": max(s.residual for s in sols),
+                              "max_diff_vs_block": float(np.max(np.linalg.norm(Xc - X64, axis=0)
+                                                                / np.linalg.norm(X64, axis=0)))}
     for fmt, itol in (("fp16", 1e-3), ("bf16", 1e-2)):
         # the low-precision copy is allocated once, like the FP64 system; the
         # timed conversion refills it from the current FP64 matrix
@@ -173,11 +275,12 @@ def run_c3(args, ctx=None):
         t0 = time.perf_counter()
         st = sa.gmres_ir_device(op, lp, B, 64, Xd, sa.GmresIrConfig(max_outer_iterations=30, outer_tolerance=1e-8,
                                                                      max_inner_iterations=50, inner_tolerance=itol))
+        ctx.synchronize()
         t_ir = time.perf_counter() - t0
         X = Xd.download().reshape(n, 64)
         lpk = ctx.kernel_stats("lowp")
         blk = ctx.kernel_stats("block")
-        err = [float(np.linalg.norm(X[:, c] - X64[:, c]) / np.linalg.norm(X64[:, c])) for c in range(64)]
+        err = np.linalg.norm(X - X64, axis=0) / np.linalg.norm(X64, axis=0)
         lp_avg = lpk[1] / max(lpk[0], 1) / 1e3
         lp_bytes = 4 * n * n
         lp_flops = 8 * n * n * 64
@@ -187,20 +290,29 @@ def run_c3(args, ctx=None):
                             "max_outer": max(s.outer_iterations for s in st),
                             "max_inner": max(s.total_inner_iterations for s in st),
                             "max_residual": max(s.final_outer_residual for s in st),
-                            "max_error_vs_fp64": max(err), "converged": all(s.converged for s in st),
+                            "error_vs_fp64_per_rhs_max": float(err.max()),
+                            "error_vs_fp64_per_rhs_median": float(np.median(err)),
+                            "converged": all(s.converged for s in st),
                             "lowp_gemm_avg_ms": lp_avg * 1e3, "lowp_gemm_gbs": lp_bytes / lp_avg / 1e9,
                             "lowp_gemm_tflops": lp_flops / lp_avg / 1e12,
                             "fp64_block_product_avg_ms": blk[1] / max(blk[0], 1)}
+        res[fmt + "_ir"]["vs_plain_excl_assembly"] = (res[fmt + "_ir"]["rhs_per_s_excl_assembly"]
+                                                      / res["plain_fp64"]["rhs_per_s_excl_assembly"])
+        res[fmt + "_ir"]["vs_plain_incl_assembly"] = (res[fmt + "_ir"]["rhs_per_s_incl_assembly"]
+                                                      / res["plain_fp64"]["rhs_per_s_incl_assembly"])
         lp.close()
     ctx.set_profiling(False)
-    res["fp16_ir"]["vs_plain_excl_assembly"] = res["fp16_ir"]["rhs_per_s_excl_assembly"] / res["plain_fp64"]["rhs_per_s_excl_assembly"]
-    res["fp16_ir"]["vs_plain_incl_assembly"] = res["fp16_ir"]["rhs_per_s_incl_assembly"] / res["plain_fp64"]["rhs_per_s_incl_assembly"]
-    res["bf16_ir"]["vs_plain_excl_assembly"] = res["bf16_ir"]["rhs_per_s_excl_assembly"] / res["plain_fp64"]["rhs_per_s_excl_assembly"]
     lp = res["fp16_ir"]
-    return {"metric": "solved RHS/s at rel. residual 1e-8 (multi-RHS)", "workload": "C3: Fibonacci sphere 40960 panels, k=14, 64 plane waves",
+    zg_ms = res["plain_fp64"]["fp64_block_product_avg_ms"]
+    return {"metric": "solved RHS/s at rel. residual 1e-8 (multi-RHS)",
+            "workload": "C3: Fibonacci sphere 40960 panels, k=14, 64 plane waves",
             "panels": n, "assembly_s": t_asm, "results": res,
             "roofline_lp_gemm": {"kernel": "lp_gemm_kernel", "bound": "hbm", "achieved": lp["lowp_gemm_gbs"],
-                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": lp["lowp_gemm_gbs"] / HBM_PEAK_GBS}}
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": lp["lowp_gemm_gbs"] / HBM_PEAK_GBS},
+            "roofline_fp64_block_product": {"kernel": "zgemm_f64_kernel", "bound": "fp64-mfma",
+                                            "achieved": 8 * n * n * 64 / (zg_ms * 1e-3) / 1e12,
+                                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                            "frac": 8 * n * n * 64 / (zg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
 
 
 def _percentiles(values):
@@ -212,8 +324,13 @@ def _percentiles(values):
 def run_c4(args, ctx=None):
     """256 independent rigid ellipsoids (320-5120 panels, k U[1,8], 1-16 plane waves).
 
-    plain: the single-system FP64 path instance by instance (assemble, plane
-    waves, block GMRES(100)), per-instance wall time as its latency.
+    plain: the single-system FP64 path instance by instance, as a caller of
+    the reference-shaped API runs it: assemble into a device system, write the
+    plane-wave right-hand sides on the device, solve all of an instance's
+    right-hand sides with the reference's multi-RHS solver (block GMRES,
+    shared basis, restart 100).  Systems and right-hand-side/solution blocks
+    are allocated once per size before the timed loop.  The same loop with the
+    column-by-column GMRES is timed too.  Per-instance latency = wall time.
     batched: every instance in one batch: persistent assembly, then lock-step
     GMRES (FP64) or FP16 iterative refinement with each cycle replayed from a
     captured HIP graph; per-instance latency = device clock from the start of
@@ -228,31 +345,55 @@ def run_c4(args, ctx=None):
     total_rhs = int(ncols.sum())
     sizes = np.array([s["mesh"].num_elements for s in specs])
     print(f"c4: {len(specs)} instances, {total_rhs} right-hand sides, panels {sizes.sum()}", file=sys.stderr, flush=True)
-    cfg = sa.GmresConfig(max_iterations=10, restart=100, tolerance=1e-8)
 
     # plain FP64, one instance after another (routes built beforehand, as setup)
     wss = [sa.TbemAssemblyWorkspace(ctx, tm) for tm in tms]
-    ctx.synchronize()
-    plain_lat, plain_x = [], []
-    plain_iters = []
-    t_all = time.perf_counter()
-    for s, tm, ws in zip(specs, tms, wss):
-        t0 = time.perf_counter()
-        ph = sa.PhysicsParams.from_wavenumber(s["k"])
-        beta = ph.burton_miller_beta()
-        system = ws.assemble(ph, beta)
-        B = sa.IncidentField(s["directions"]).compute_rhs_with_beta(tm, ph, beta).reshape(tm.num_elements, -1)
-        X, sols = sa.gmres_block(sa.DenseMatrixOperator(system), B, cfg)
+    systems = {int(nn): sa.TbemSystem(ctx, int(nn)) for nn in np.unique(sizes)}
+    ops = {nn: sa.DenseMatrixOperator(sy) for nn, sy in systems.items()}
+    blocks = {}
+    for nn, c in set(zip(sizes.tolist(), ncols.tolist())):
+        blocks[(nn, c)] = (sa.DeviceVector(ctx, nn * c), sa.DeviceVector(ctx, nn * c))
+    bcfg = sa.BlockGmresConfig(max_iterations=10, restart=100, tolerance=1e-8)
+    ccfg = sa.GmresConfig(max_iterations=10, restart=100, tolerance=1e-8)
+    plain = {}
+    plain_x = {}
+    for form in ("block", "per_column"):
+        lat, iters = [], []
         ctx.synchronize()
-        plain_lat.append((time.perf_counter() - t0) * 1e3)
-        if not all(q.converged and q.residual <= 1e-8 for q in sols):
-            raise SystemExit("bench c4: a plain solve did not converge")
-        plain_x.append(X)
-        plain_iters.append(max(q.iterations for q in sols))
-        system.close()
-    t_plain = time.perf_counter() - t_all
+        t_all = time.perf_counter()
+        for i, (s, tm, ws) in enumerate(zip(specs, tms, wss)):
+            t0 = time.perf_counter()
+            nn, c = int(sizes[i]), int(ncols[i])
+            ph = sa.PhysicsParams.from_wavenumber(s["k"])
+            beta = ph.burton_miller_beta()
+            ws.assemble(ph, beta, systems[nn])
+            Bd, Xd = blocks[(nn, c)]
+            sa.IncidentField(s["directions"]).write_device(Bd, tm, ph, beta)
+            if form == "block":
+                sol = sa.block_gmres_device(ops[nn], Bd, c, Xd, bcfg)
+                ok = sol.converged and sol.true_residuals.max() <= 1e-8
+                iters.append(sol.iterations)
+            else:
+                sols = sa.gmres_block_device(ops[nn], Bd, c, Xd, ccfg)
+                ok = all(q.converged and q.residual <= 1e-8 for q in sols)
+                iters.append(max(q.iterations for q in sols))
+            ctx.synchronize()
+            lat.append((time.perf_counter() - t0) * 1e3)
+            if not ok:
+                raise SystemExit(f"bench c4: plain {form} solve of instance {i} did not converge")
+            if form == "block" and i % 8 == 0:
+                plain_x[i] = Xd.download().reshape(nn, c)
+        el = time.perf_counter() - t_all
+        plain[form] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
+                       "max_iterations": int(max(iters)), "mean_iterations": float(np.mean(iters))}
+        print(f"c4: plain {form} {el:.2f} s", file=sys.stderr, flush=True)
+    for Bd, Xd in blocks.values():
+        Bd.close()
+        Xd.close()
+    for o in list(ops.values()) + list(systems.values()):
+        o.close()
     del wss
-    print(f"c4: plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+    t_plain = plain["block"]["seconds"]
 
     t0 = time.perf_counter()
     batch = sa.Batch(ctx, tms)
@@ -279,31 +420,71 @@ def run_c4(args, ctx=None):
             el = time.perf_counter() - t0
         if not all(q.converged for q in sts):
             raise SystemExit(f"bench c4: {form} left columns unconverged")
-        lat, col, err = [], 0, 0.0
+        lat, col, errs = [], 0, []
         for i, s in enumerate(specs):
             c = int(ncols[i])
             lat.append(max(q.completion_ms for q in sts[col: col + c]))
             col += c
-        for i in range(0, len(specs), 8):
+        for i in sorted(plain_x):
             X = batch.solution(i)
-            err = max(err, float(np.linalg.norm(X - plain_x[i]) / np.linalg.norm(plain_x[i])))
+            errs.append(np.linalg.norm(X - plain_x[i], axis=0) / np.linalg.norm(plain_x[i], axis=0))
+        errs = np.concatenate(errs)
         forms[form] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
                        "max_iterations": max(q.iterations for q in sts), "max_cycles": max(q.cycles for q in sts),
-                       "max_residual": max(q.residual for q in sts), "max_error_vs_plain_sampled": err,
+                       "max_residual": max(q.residual for q in sts),
+                       "error_vs_plain_per_rhs_max": float(errs.max()),
+                       "error_vs_plain_per_rhs_median": float(np.median(errs)),
                        "vs_plain": (total_rhs / el) / (total_rhs / t_plain)}
         print(f"c4: {form} {el:.3f} s", file=sys.stderr, flush=True)
+    batch.close()
+    # Latency knob: the same solve over sub-batches served one after another
+    # (each sub-batch's instances assembled and solved together); an
+    # instance's service latency is counted from its sub-batch's start.
+    sub = {}
+    for size in (64, 16):
+        groups = [list(range(g, min(g + size, len(specs)))) for g in range(0, len(specs), size)]
+        batches = []
+        for idx in groups:
+            bb = sa.Batch(ctx, [tms[i] for i in idx])
+            bb.set_problem([specs[i]["k"] for i in idx], [specs[i]["directions"] for i in idx])
+            bb.set_tuning(**best)
+            batches.append(bb)
+        for rep in range(2):  # the first pass captures each sub-batch's cycle graph
+            lat = []
+            ctx.synchronize()
+            t_all = time.perf_counter()
+            for idx, bb in zip(groups, batches):
+                t0 = time.perf_counter()
+                bb.assemble()
+                bb.convert("fp16")
+                sts = bb.solve("fp16_ir", restart=30, max_cycles=30, tolerance=1e-8, inner_tolerance=1e-3,
+                               use_graph=not args.no_graph)
+                if not all(q.converged for q in sts):
+                    raise SystemExit("bench c4: a sub-batch left columns unconverged")
+                col = 0
+                for i in idx:
+                    c = int(ncols[i])
+                    lat.append(max(q.completion_ms for q in sts[col: col + c]))
+                    col += c
+            ctx.synchronize()
+            el = time.perf_counter() - t_all
+        sub[f"sub_batches_of_{size}"] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
+                                         "vs_plain": (total_rhs / el) / (total_rhs / t_plain)}
+        for bb in batches:
+            bb.close()
+        print(f"c4: sub-batches of {size}: {el:.3f} s", file=sys.stderr, flush=True)
+    forms["batched_fp16_ir_sub_batches"] = sub
     line = {"metric": "solved RHS/s at rel. residual 1e-8 (batch)",
             "workload": "C4: 256 rigid ellipsoids, semi-axes U[0.5,1.5], 320/1280/5120 panels, k U[1,8], 1-16 plane waves, seed 20261202",
             "instances": len(specs), "rhs": total_rhs, "panels": int(sizes.sum()),
             "matrix_gb_fp64": float((sizes.astype(np.float64) ** 2).sum() * 16 / 1e9),
-            "plain_fp64_sequential": {"seconds": t_plain, "rhs_per_s": total_rhs / t_plain, **_percentiles(plain_lat),
-                                      "max_iterations": int(max(plain_iters))},
+            "plain_fp64_sequential": plain["block"], "plain_fp64_sequential_per_column": plain["per_column"],
             "batched": forms, "autotune": {"best": best, "candidates": table, "seconds": tune_s},
             "batch_setup_s": setup_s}
     return line
 
 
-def run_c5(args):
+def run_c5(args, ctx=None):
     """Frequency sweep with the learned warm start.
 
     Training data: the 10240-panel Fibonacci sphere at 512 wavenumbers in
@@ -317,7 +498,7 @@ def run_c5(args):
     from sonium_amd import warmstart as wsm
     from sonium_amd.mesh import Mesh, uniform_directions
 
-    ctx = sa.Context(0)
+    ctx = ctx or sa.Context(0)
     base = fibonacci_sphere(1.0, 65, 80)
     n = base.num_elements
     cfg = sa.GmresConfig(max_iterations=20, restart=100, tolerance=1e-8)
@@ -413,53 +594,61 @@ def run_c5(args):
             res[mode]["guess_rel_error_max"] = float(np.max(guess_err))
             res[mode]["guess_rel_residual_median"] = float(np.median(guess_res))
         print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}", file=sys.stderr, flush=True)
-    # chosen form: groups of frequencies as batch instances on the same mesh, assembled in
-    # one persistent launch, started from the network's guesses, solved by lock-step
-    # FP16 iterative refinement to the FP64 residual bar
+    # batched forms: groups of frequencies as batch instances on the same mesh, assembled in
+    # one persistent launch and solved by lock-step FP16 iterative refinement to the FP64
+    # residual bar, from zero guesses or from the network's guesses (the chosen form)
     group = 64
     tms_batch = [tm] * group
     t0 = time.perf_counter()
     batch = sa.Batch(ctx, tms_batch)
     batch_setup_s = time.perf_counter() - t0
-    iters, residuals = [], []
-    parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for g0 in range(0, len(ks), group):
-        kg = ks[g0: g0 + group]
-        if len(kg) < group:
-            break
-        ta = time.perf_counter()
-        batch.set_problem([float(k) for k in kg], [d.reshape(1, 3)] * group)
-        batch.assemble()
-        batch.convert("fp16")
+    x_ref = np.stack([x_zero[float(k)] for k in ks])
+    for mode in ("batched_fp16_ir_zero", "chosen_batched_fp16_ir_learned"):
+        learned = mode.endswith("learned")
+        iters, residuals, errs = [], [], []
+        parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
         ctx.synchronize()
-        tb = time.perf_counter()
-        X0 = net.guesses(c, nn, kg, d)
-        for i in range(group):
-            batch.set_guess(i, X0[i].reshape(n, 1))
-        tc = time.perf_counter()
-        sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
-                          use_guess=True)
-        td = time.perf_counter()
-        parts["assemble_convert"] += tb - ta
-        parts["guesses"] += tc - tb
-        parts["solve"] += td - tc
-        if not all(q.converged for q in sts):
-            raise SystemExit("bench c5: batched chosen form left a frequency unconverged")
-        iters += [q.iterations for q in sts]
-        residuals += [q.residual for q in sts]
-    ctx.synchronize()
-    el = time.perf_counter() - t0
-    nsolved = len(iters)
-    res["chosen_batched_fp16_ir_learned"] = {"rhs_per_s": nsolved / el, "seconds": el,
-                                             "mean_iterations": float(np.mean(iters)),
-                                             "max_residual": float(np.max(residuals)), "group": group,
-                                             "batch_setup_s": batch_setup_s,
-                                             "breakdown_s": parts}
-    print(f"c5: chosen {el:.2f} s, mean Arnoldi steps {np.mean(iters):.1f}", file=sys.stderr, flush=True)
-    for mode in ("previous", "learned", "chosen_batched_fp16_ir_learned"):
+        t0 = time.perf_counter()
+        for g0 in range(0, len(ks), group):
+            kg = ks[g0: g0 + group]
+            if len(kg) < group:
+                break
+            ta = time.perf_counter()
+            batch.set_problem([float(k) for k in kg], [d.reshape(1, 3)] * group)
+            batch.assemble()
+            batch.convert("fp16")
+            ctx.synchronize()
+            tb = time.perf_counter()
+            if learned:
+                X0 = net.guesses(c, nn, kg, d)
+                for i in range(group):
+                    batch.set_guess(i, X0[i].reshape(n, 1))
+            tc = time.perf_counter()
+            sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
+                              use_guess=learned)
+            td = time.perf_counter()
+            parts["assemble_convert"] += tb - ta
+            parts["guesses"] += tc - tb
+            parts["solve"] += td - tc
+            if not all(q.converged for q in sts):
+                raise SystemExit(f"bench c5: batched {mode} left a frequency unconverged")
+            iters += [q.iterations for q in sts]
+            residuals += [q.residual for q in sts]
+            for i in range(0, group, 16):
+                xs = batch.solution(i)[:, 0]
+                errs.append(float(np.linalg.norm(xs - x_ref[g0 + i]) / np.linalg.norm(x_ref[g0 + i])))
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = {"rhs_per_s": len(iters) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
+                     "max_residual": float(np.max(residuals)), "group": group, "batch_setup_s": batch_setup_s,
+                     "breakdown_s": parts, "error_vs_fp64_max_sampled": float(np.max(errs))}
+        print(f"c5: {mode} {el:.2f} s, mean Arnoldi steps {np.mean(iters):.1f}", file=sys.stderr, flush=True)
+    for mode in ("previous", "learned", "batched_fp16_ir_zero", "chosen_batched_fp16_ir_learned"):
         res[mode]["vs_zero"] = res[mode]["rhs_per_s"] / res["zero"]["rhs_per_s"]
+    res["chosen_batched_fp16_ir_learned"]["vs_batched_zero"] = (res["chosen_batched_fp16_ir_learned"]["rhs_per_s"]
+                                                              / res["batched_fp16_ir_zero"]["rhs_per_s"])
+    batch.close()
+    net.close()
     line = {"metric": "solved RHS/s at rel. residual 1e-8 (frequency sweep)",
             "workload": "C5: Fibonacci sphere 10240 panels, held-out k in [1,12] (256), +z; MLP 4x512 BF16 trained on "
                         f"{n_solutions} solver solutions (sphere sweep + 16 ellipsoid variants)",
@@ -468,7 +657,7 @@ def run_c5(args):
                                       "steps": steps, "batch": batch_size, "train_s": train_s,
                                       "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-50:]))},
             "results": res}
-    print(json.dumps(line))
+    return line
 
 
 def main():
@@ -492,11 +681,8 @@ def main():
         sa._check(sa._lib().sbh_debug_device_copy(ctx.handle, args.calibrate_copy, 3))
         print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 3}))
         return
-    if args.workload in ("c3", "c4"):
-        print(json.dumps({"c3": run_c3, "c4": run_c4}[args.workload](args)))
-        return
-    if args.workload == "c5":
-        run_c5(args)
+    if args.workload in ("c3", "c4", "c5"):
+        print(json.dumps({"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -529,21 +715,29 @@ def main():
 
     for _ in range(args.warmup):
         sol = step()
-    ctx.synchronize()
-    ctx.reset_stats()
+    # Timed region: kernel events on, taken from the context's pool (the
+    # mixed-precision leg below is timed the same way).
     ctx.set_profiling(True)
+    ctx.reset_stats()
     ctx.synchronize()
     t0 = time.perf_counter()
     sols = [step() for _ in range(args.steps)]
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
+    stats = {name: ctx.kernel_stats(name) for name in sa._ffi.KERNEL_CLASSES}
     ctx.set_profiling(False)
+    # the same steps with no events at all, to show what the events cost
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    elapsed_no_events = time.perf_counter() - t0
     sol = sols[-1]
     if not all(s.converged and s.residual <= 1e-8 for s in sols):
         raise SystemExit(f"bench: a timed solve did not converge: {[(s.status, s.residual) for s in sols]}")
     ms_per_step = elapsed / args.steps * 1e3
     value = args.steps / elapsed
-    stats = {name: ctx.kernel_stats(name) for name in sa._ffi.KERNEL_CLASSES}
 
     far_launches, far_ms = stats["assemble_far"]
     mv_launches, mv_ms = stats["matvec"]
@@ -643,26 +837,42 @@ def main():
                   "flops_per_launch": f_flops, "max_abs_p_total": float(np.max(np.abs(fv.p_total)))}
 
     others = None
+    summary = {"c2": {"plain_fp64_rhs_per_s": value, "plain_fp64_rhs_per_s_no_events": args.steps / elapsed_no_events}}
+    if chosen:
+        summary["c2"].update({"chosen_form": "FP16 FGMRES-IR", "chosen_rhs_per_s": chosen["value"],
+                              "chosen_vs_plain": chosen["vs_plain_fp64"],
+                              "chosen_error_vs_fp64": chosen["error_vs_fp64"]})
     if not args.no_other_configs and args.workload == "c2":
         c3 = run_c3(args, ctx)
+        print("bench: c3 done", file=sys.stderr, flush=True)
         c4 = run_c4(args, ctx)
-        others = {
-            "c3": {"workload": c3["workload"], "plain_fp64_rhs_per_s": c3["results"]["plain_fp64"]["rhs_per_s_excl_assembly"],
-                   "chosen_fp16_ir_rhs_per_s": c3["results"]["fp16_ir"]["rhs_per_s_excl_assembly"],
-                   "vs_plain_excl_assembly": c3["results"]["fp16_ir"]["vs_plain_excl_assembly"],
-                   "vs_plain_incl_assembly": c3["results"]["fp16_ir"]["vs_plain_incl_assembly"],
-                   "max_error_vs_fp64": c3["results"]["fp16_ir"]["max_error_vs_fp64"],
-                   "roofline_lp_gemm": c3["roofline_lp_gemm"],
-                   "fp64_block_product_tflops": 8 * c3["panels"] ** 2 * 64 / (c3["results"]["plain_fp64"]["fp64_block_product_avg_ms"] * 1e-3) / 1e12},
-            "c4": {"workload": c4["workload"], "plain_fp64_rhs_per_s": c4["plain_fp64_sequential"]["rhs_per_s"],
-                   "plain_p50_ms": c4["plain_fp64_sequential"]["p50_ms"], "plain_p99_ms": c4["plain_fp64_sequential"]["p99_ms"],
-                   "chosen_batched_fp16_ir_rhs_per_s": c4["batched"]["batched_fp16_ir"]["rhs_per_s"],
-                   "chosen_p50_ms": c4["batched"]["batched_fp16_ir"]["p50_ms"],
-                   "chosen_p99_ms": c4["batched"]["batched_fp16_ir"]["p99_ms"],
-                   "vs_plain": c4["batched"]["batched_fp16_ir"]["vs_plain"],
-                   "batched_fp64_vs_plain": c4["batched"]["batched_fp64"]["vs_plain"],
-                   "autotuned": c4["autotune"]["best"]},
-        }
+        print("bench: c4 done", file=sys.stderr, flush=True)
+        c5 = run_c5(args, ctx)
+        print("bench: c5 done", file=sys.stderr, flush=True)
+        r3, b4, r5 = c3["results"], c4["batched"], c5["results"]
+        others = {"c3": c3, "c4": c4, "c5": c5}
+        summary["c3"] = {"plain": "block GMRES FP64 (shared basis)",
+                         "plain_rhs_per_s_excl_assembly": r3["plain_fp64"]["rhs_per_s_excl_assembly"],
+                         "plain_rhs_per_s_incl_assembly": r3["plain_fp64"]["rhs_per_s_incl_assembly"],
+                         "chosen": "FP16 FGMRES-IR", "chosen_rhs_per_s_excl_assembly": r3["fp16_ir"]["rhs_per_s_excl_assembly"],
+                         "chosen_vs_plain_excl_assembly": r3["fp16_ir"]["vs_plain_excl_assembly"],
+                         "chosen_vs_plain_incl_assembly": r3["fp16_ir"]["vs_plain_incl_assembly"],
+                         "chosen_error_vs_fp64_max": r3["fp16_ir"]["error_vs_fp64_per_rhs_max"],
+                         "bf16_vs_plain_excl_assembly": r3["bf16_ir"]["vs_plain_excl_assembly"],
+                         "bf16_error_vs_fp64_max": r3["bf16_ir"]["error_vs_fp64_per_rhs_max"]}
+        summary["c4"] = {"plain": "per-instance FP64, block GMRES", "plain_rhs_per_s": c4["plain_fp64_sequential"]["rhs_per_s"],
+                         "plain_p50_ms": c4["plain_fp64_sequential"]["p50_ms"],
+                         "chosen": "batched FP16 IR", "chosen_rhs_per_s": b4["batched_fp16_ir"]["rhs_per_s"],
+                         "chosen_p50_ms": b4["batched_fp16_ir"]["p50_ms"], "chosen_vs_plain": b4["batched_fp16_ir"]["vs_plain"],
+                         "chosen_error_vs_plain_max": b4["batched_fp16_ir"]["error_vs_plain_per_rhs_max"]}
+        summary["c5"] = {"plain": "zero-guess FP64 GMRES", "plain_rhs_per_s": r5["zero"]["rhs_per_s"],
+                         "previous_frequency_vs_plain": r5["previous"]["vs_zero"],
+                         "learned_vs_plain": r5["learned"]["vs_zero"],
+                         "chosen": "batched FP16 IR from learned guesses",
+                         "chosen_rhs_per_s": r5["chosen_batched_fp16_ir_learned"]["rhs_per_s"],
+                         "chosen_vs_plain": r5["chosen_batched_fp16_ir_learned"]["vs_zero"],
+                         "batched_zero_guess_vs_plain": r5["batched_fp16_ir_zero"]["vs_zero"],
+                         "chosen_error_vs_fp64_max": r5["chosen_batched_fp16_ir_learned"]["error_vs_fp64_max_sampled"]}
 
     # measured ceilings beside the published peaks: a streaming read of a
     # matrix-sized buffer and independent FP64 FMAs on every CU
@@ -675,7 +885,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = host_info()["threads_used"]
         cpu = cpu_baseline(m, k, beta, sol.spmv_count, args.cpu_rows, threads)
 
     line = {
@@ -700,6 +910,8 @@ def main():
         "kernels": kernels,
         "roofline": roofline,
         "measured_ceilings": ceilings,
+        "summary_chosen_vs_plain": summary,
+        "plain_events_off_ms_per_step": elapsed_no_events / args.steps * 1e3,
         "chosen_form": chosen,
         "exact_operator": exact_info,
         "field_evaluation": field_info,
