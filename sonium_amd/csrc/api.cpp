@@ -23,6 +23,9 @@ void lowp_gemm(sbh_context* ctx, const uint16_t* A, int64_t ld, int64_t n, int f
                const double2* X, int k, uint16_t* XT, unsigned* colmax, double* col_inv, float* Ypart, double2* Y);
 sbh_status run_gmres(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
                      const sbh_gmres_config& cfg, double2* d_x, sbh_gmres_solution* sol);
+sbh_status run_gcro_dr(sbh_operator* op, sbh_precond* M, const double2* d_b, const double2* d_x0,
+                       sbh_gcro_subspace* sub, const sbh_gcro_dr_config& cfg, double2* d_x,
+                       sbh_gmres_solution* sol);
 void launch_sincos_probe(sbh_context* ctx, const double* x, int64_t n, double* s, double* c);
 void launch_rsqrt_probe(sbh_context* ctx, const double* x, int64_t n, double* y);
 
@@ -1358,6 +1361,93 @@ sbh_status sbh_block_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64
                                         true_residuals, backward_errors);
         dx.finish();
         if (st != SBH_OK) throw Error(st, "block GMRES failed");
+    });
+}
+
+sbh_status sbh_gcro_subspace_create(sbh_context* ctx, int64_t n, int64_t k, sbh_gcro_subspace** out) {
+    return guard_on(ctx_of(ctx), [&] {
+        SBH_REQUIRE(ctx && out, "context/out is null");
+        SBH_REQUIRE(n >= 1, "n must be positive");
+        SBH_REQUIRE(k >= 0 && k <= 256, "the recycled subspace size must be in 0..256");
+        use_device(ctx);
+        admit_device(ctx, (int64_t)sizeof(double2) * std::max<int64_t>(k, 1) * n, "a GCRO-DR recycled subspace", false);
+        std::unique_ptr<sbh_gcro_subspace> g(new sbh_gcro_subspace());
+        g->ctx = ctx;
+        g->n = n;
+        g->k = (int)k;
+        g->U.allocate((size_t)std::max<int64_t>(k, 1) * n);
+        context_retain(ctx);
+        *out = g.release();
+    });
+}
+
+void sbh_gcro_subspace_destroy(sbh_gcro_subspace* g) {
+    if (!g) return;
+    sbh_context* ctx = g->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete g;
+    context_release(ctx);
+}
+
+int64_t sbh_gcro_subspace_size(const sbh_gcro_subspace* g) { return g ? g->count : -1; }
+
+sbh_status sbh_gcro_subspace_reset(sbh_gcro_subspace* g) {
+    return guard_on(ctx_of(g), [&] {
+        SBH_REQUIRE(g, "subspace is null");
+        g->count = 0;
+    });
+}
+
+sbh_status sbh_gcro_subspace_download(const sbh_gcro_subspace* g, sbh_c64* u) {
+    return guard_on(ctx_of(g), [&] {
+        SBH_REQUIRE(g && u, "null argument");
+        use_device(g->ctx);
+        injection_check(INJECT_TRANSFER_D2H);
+        copy_on_stream(g->ctx, u, g->U.ptr, sizeof(double2) * (size_t)g->count * (size_t)g->n,
+                       hipMemcpyDeviceToHost);
+    });
+}
+
+sbh_status sbh_gcro_dr(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, const sbh_c64* x0,
+                       sbh_gcro_subspace* subspace, const sbh_gcro_dr_config* config, sbh_c64* x, int memory,
+                       sbh_gmres_solution* solution) {
+    return guard_on(ctx_of(op), [&] {
+        SBH_REQUIRE(op && b && subspace && config && x && solution, "null argument");
+        check_memory(memory);
+        // validate_gcro_request (gcro_dr.rs:240-290)
+        SBH_REQUIRE(config->max_iterations > 0 && config->restart > 0 && std::isfinite(config->tolerance) &&
+                        config->tolerance > 0.0,
+                    "invalid GCRO-DR configuration");
+        SBH_REQUIRE(subspace->ctx == op->ctx && subspace->n == op->n, "the recycled subspace does not match the operator");
+        if (precond) SBH_REQUIRE(precond->ctx == op->ctx && precond->n == op->n, "preconditioner does not match the operator");
+        const int64_t n = op->n;
+        if (memory == SBH_MEM_HOST) {
+            if (!all_finite(b, (size_t)n)) throw Error(SBH_ERR_NUMERICAL, "right-hand side contains NaN or infinity");
+            if (x0 && !all_finite(x0, (size_t)n)) throw Error(SBH_ERR_NUMERICAL, "initial guess contains NaN or infinity");
+        }
+        use_device(op->ctx);
+        DeviceInput db(op->ctx, b, (size_t)n, memory);
+        std::unique_ptr<DeviceInput> dx0;
+        if (x0) dx0.reset(new DeviceInput(op->ctx, x0, (size_t)n, memory));
+        DeviceOutput dx(op->ctx, x, (size_t)n, memory);
+        if (memory == SBH_MEM_DEVICE) {
+            DeviceArray<double> part(256 + 16);
+            auto finite_norm = [&](const double2* v) {
+                const int chunks = bk_frob_partials(op->ctx, v, n, part.ptr);
+                std::vector<double> h((size_t)chunks);
+                SBH_HIP(hipMemcpyAsync(h.data(), part.ptr, sizeof(double) * chunks, hipMemcpyDeviceToHost,
+                                       op->ctx->stream));
+                SBH_HIP(hipStreamSynchronize(op->ctx->stream));
+                double acc = 0.0;
+                for (double v2 : h) acc += v2;
+                return std::isfinite(acc);
+            };
+            if (!finite_norm(db.ptr)) throw Error(SBH_ERR_NUMERICAL, "right-hand side contains NaN or infinity");
+            if (dx0 && !finite_norm(dx0->ptr)) throw Error(SBH_ERR_NUMERICAL, "initial guess contains NaN or infinity");
+        }
+        run_gcro_dr(op, precond, db.ptr, dx0 ? dx0->ptr : nullptr, subspace, *config, dx.ptr, solution);
+        dx.finish();
     });
 }
 
