@@ -475,6 +475,42 @@ sbh_status sbh_gmres_block(sbh_operator* op, const sbh_c64* B, const sbh_c64* X0
                            sbh_gmres_solution* solutions);
 
 /* ---------------------------------------------------------------------------
+ * GCRO-DR Krylov recycling across a sequence of systems — GcroDrSubspace,
+ * GcroDrConfig and try_gcro_dr_preconditioned
+ * (crates/sonium-solvers/crates/sonium-solvers-iterative/src/gcro_dr.rs:
+ * 63-157, 199-290, 292-760), the sweep accelerator of
+ * frequency_sweep.rs:495-508,636-650.  The subspace holds up to k recycled
+ * vectors U on the device; each solve rebuilds C = orth(M^-1 A U) for the
+ * current operator (k products), keeps every Arnoldi vector orthogonal to C,
+ * and after a solve accepted by the true residual replaces U with harmonic
+ * Ritz vectors of the augmented (k + m) problem.  A failed solve leaves U
+ * unchanged. */
+typedef struct sbh_gcro_subspace sbh_gcro_subspace;
+
+typedef struct {
+    int64_t max_iterations; /* restart cycles (default 100) */
+    int64_t restart;        /* Krylov dimension m per cycle (default 50) */
+    double tolerance;       /* relative to ||M^-1 b|| (default 1e-6) */
+} sbh_gcro_dr_config;
+
+/* GcroDrSubspace::new(k) for systems of order n; k = 0 never recycles. */
+sbh_status sbh_gcro_subspace_create(sbh_context* ctx, int64_t n, int64_t k, sbh_gcro_subspace** out);
+void sbh_gcro_subspace_destroy(sbh_gcro_subspace* subspace);
+/* num_vectors (gcro_dr.rs:82-85); -1 for a null handle. */
+int64_t sbh_gcro_subspace_size(const sbh_gcro_subspace* subspace);
+/* reset (gcro_dr.rs:92-96). */
+sbh_status sbh_gcro_subspace_reset(sbh_gcro_subspace* subspace);
+/* The recycled vectors, size() x n row-major, host memory. */
+sbh_status sbh_gcro_subspace_download(const sbh_gcro_subspace* subspace, sbh_c64* u);
+/* b, x0 (NULL = zero) and x use `memory`; precond NULL = identity.  Invalid
+ * configurations, non-finite b or x0 and non-finite results are errors (the
+ * checked reference boundary); the solution reports iterations (Arnoldi
+ * steps, without the k products that rebuild C) and restarts as GMRES does. */
+sbh_status sbh_gcro_dr(sbh_operator* op, sbh_precond* precond, const sbh_c64* b, const sbh_c64* x0,
+                       sbh_gcro_subspace* subspace, const sbh_gcro_dr_config* config, sbh_c64* x, int memory,
+                       sbh_gmres_solution* solution);
+
+/* ---------------------------------------------------------------------------
  * Block GMRES with one shared block Krylov basis — BlockGmresConfig /
  * BlockGmresSolution and try_block_gmres_preconditioned(_with_guess)
  * (crates/sonium-solvers/crates/sonium-solvers-iterative/src/block_gmres.rs:
@@ -605,7 +641,8 @@ typedef struct {
     int64_t restart;          /* Arnoldi steps per cycle (1..200) */
     int64_t max_cycles;       /* restart cycles (FP64) or refinement steps (IR) */
     double tolerance;         /* on the recomputed FP64 relative residual */
-    double inner_tolerance;   /* IR: inner estimate relative to the current residual */
+    double inner_tolerance;   /* IR: inner estimate relative to the current residual (never below
+                                 tolerance * ||b|| / 4: a cycle stops once it passes the outer target) */
     int32_t use_guess;        /* start from the solutions set by sbh_batch_set_guess (else zero) */
     int32_t pad;
 } sbh_batch_solve_config;

@@ -909,6 +909,101 @@ def gmres_device(op: DenseMatrixOperator, b_ptr: int, x: DeviceVector, config: G
     return _solution(raw, None)
 
 
+@dataclass
+class GcroDrConfig:
+    """GcroDrConfig (gcro_dr.rs:112-157); from_gmres keeps max_iterations,
+    restart and tolerance."""
+
+    max_iterations: int = 100
+    restart: int = 50
+    tolerance: float = 1e-6
+
+    @staticmethod
+    def from_gmres(g: "GmresConfig") -> "GcroDrConfig":
+        return GcroDrConfig(g.max_iterations, g.restart, g.tolerance)
+
+    def _c(self):
+        return _ffi.gcro_dr_config_t(int(self.max_iterations), int(self.restart), float(self.tolerance))
+
+
+class GcroDrSubspace:
+    """GcroDrSubspace (gcro_dr.rs:63-110): up to k recycled vectors of order
+    n, kept on the device between solves."""
+
+    def __init__(self, ctx: Context, n: int, k: int):
+        self.ctx = ctx
+        self.n = int(n)
+        self.k = int(k)
+        h = ctypes.c_void_p()
+        _check(_lib().sbh_gcro_subspace_create(ctx.handle, self.n, self.k, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_vectors(self) -> int:
+        return int(_lib().sbh_gcro_subspace_size(self._h))
+
+    def is_populated(self) -> bool:
+        return self.num_vectors() > 0
+
+    def target_size(self) -> int:
+        return self.k
+
+    def reset(self) -> None:
+        _check(_lib().sbh_gcro_subspace_reset(self._h))
+
+    def vectors(self) -> np.ndarray:
+        """The recycled vectors, num_vectors() x n."""
+        out = np.empty((self.num_vectors(), self.n), dtype=np.complex128)
+        if out.size:
+            _check(_lib().sbh_gcro_subspace_download(self._h, _cptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sbh_gcro_subspace_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gcro_dr_preconditioned(op: DenseMatrixOperator, precond, b, x0, subspace: GcroDrSubspace,
+                           config: GcroDrConfig) -> GmresSolution:
+    """try_gcro_dr_preconditioned (gcro_dr.rs:199-290), host vectors; x0 may be None."""
+    n = op.num_rows()
+    bv = _c128(b, n)
+    x0v = None if x0 is None else _c128(x0, n)
+    x = np.zeros(n, dtype=np.complex128)
+    raw = _ffi.gmres_solution_t()
+    cfg = config._c()
+    _check(_lib().sbh_gcro_dr(op.handle, None if precond is None else precond._h, _cptr(bv),
+                              None if x0v is None else _cptr(x0v), subspace.handle, ctypes.byref(cfg), _cptr(x),
+                              MEM_HOST, ctypes.byref(raw)))
+    return _solution(raw, x)
+
+
+def gcro_dr(op: DenseMatrixOperator, b, x0, subspace: GcroDrSubspace, config: GcroDrConfig) -> GmresSolution:
+    """try_gcro_dr (gcro_dr.rs:183-197): the identity preconditioner."""
+    return gcro_dr_preconditioned(op, None, b, x0, subspace, config)
+
+
+def gcro_dr_device(op: DenseMatrixOperator, b_ptr: int, x: DeviceVector, subspace: GcroDrSubspace,
+                   config: GcroDrConfig, precond=None, x0: Optional[DeviceVector] = None) -> GmresSolution:
+    """GCRO-DR on device-resident b (device pointer) and x."""
+    raw = _ffi.gmres_solution_t()
+    cfg = config._c()
+    _check(_lib().sbh_gcro_dr(op.handle, None if precond is None else precond._h, ctypes.c_void_p(b_ptr),
+                              None if x0 is None else ctypes.c_void_p(x0.ptr), subspace.handle, ctypes.byref(cfg),
+                              ctypes.c_void_p(x.ptr), MEM_DEVICE, ctypes.byref(raw)))
+    return _solution(raw, None)
+
+
 def gmres_block(op: DenseMatrixOperator, B, config: GmresConfig, X0=None):
     """Many right-hand sides (BemSolver::solve_many, bem_solver.rs:2032): B is n x k (k <= 64).
 

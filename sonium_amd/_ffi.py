@@ -129,6 +129,10 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
 
+class gcro_dr_config_t(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int64), ("restart", ctypes.c_int64), ("tolerance", ctypes.c_double)]
+
+
 # sbh_apply_fn: int (*)(void* user, const sbh_c64* x, sbh_c64* y, int64_t n, int memory)
 APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                             ctypes.c_int)
@@ -225,6 +229,13 @@ SIGNATURES = {
     "sbh_lowp_matrix_download": (_int, [_vp, _vp, _vp, ctypes.POINTER(_i64)]),
     "sbh_block_gmres": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.POINTER(block_gmres_config_t), _vp, _int,
                                ctypes.POINTER(block_gmres_info_t), _vp, _vp, _vp]),
+    "sbh_gcro_subspace_create": (_int, [_vp, _i64, _i64, _pp]),
+    "sbh_gcro_subspace_destroy": (None, [_vp]),
+    "sbh_gcro_subspace_size": (_i64, [_vp]),
+    "sbh_gcro_subspace_reset": (_int, [_vp]),
+    "sbh_gcro_subspace_download": (_int, [_vp, _vp]),
+    "sbh_gcro_dr": (_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(gcro_dr_config_t), _vp, _int,
+                           ctypes.POINTER(gmres_solution_t)]),
     "sbh_gmres_ir": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(gmres_ir_config_t), _vp, _int,
                             ctypes.POINTER(gmres_ir_status_t)]),
     "sbh_batch_create": (_int, [_vp, _i64, _vp, _int, _pp]),

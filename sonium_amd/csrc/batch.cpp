@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <thread>
 #include <type_traits>
@@ -49,6 +50,9 @@ struct sbh_batch {
     sbh::DeviceArray<sbh::BatchInstance> d_inst;
     sbh::DeviceArray<double> d_dirs;
     sbh::DeviceArray<int2> d_items;
+    sbh::DeviceArray<sbh::FreqFamily> d_families;
+    std::vector<sbh::FreqFamily> families;
+    int64_t family_max_n = 0;
     sbh::DeviceArray<unsigned> d_counter;
     sbh::DeviceArray<int2> d_tiles;
     sbh::DeviceArray<double2> d_A;
@@ -143,6 +147,64 @@ std::vector<int> size_order(const sbh_batch* b) {
     for (int i = 0; i < b->B; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->inst[x].n > b->inst[y].n; });
     return order;
+}
+
+// Frequency families: instances on the same mesh whose wavenumbers, in
+// increasing order, step by a constant dk (each within 1e-12 relative of
+// k0 + f dk), cut into groups of at most FAMILY_MAX.  Their far entries come
+// from one pass over the pair geometry (batch_far_family_kernel); an instance
+// without such neighbours keeps the per-instance far loop.
+void build_families(sbh_batch* b) {
+    b->families.clear();
+    for (auto& I : b->inst) I.in_family = 0;
+    std::vector<const sbh_mesh*> mesh_order;
+    std::map<const sbh_mesh*, std::vector<int>> by_mesh;
+    for (int i = 0; i < b->B; ++i) {
+        auto& v = by_mesh[b->meshes[i]];
+        if (v.empty()) mesh_order.push_back(b->meshes[i]);
+        v.push_back(i);
+    }
+    int64_t max_n = 0;
+    for (const sbh_mesh* m : mesh_order) {
+        std::vector<int> ids = by_mesh[m];
+        std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->inst[x].k < b->inst[y].k; });
+        size_t s = 0;
+        while (s < ids.size()) {
+            size_t e = s + 1;
+            if (e < ids.size()) {
+                const double k0 = b->inst[ids[s]].k;
+                const double dk = b->inst[ids[s + 1]].k - k0;
+                if (dk > 0.0) {
+                    e = s + 2;
+                    while (e < ids.size() && (int)(e - s) < FAMILY_MAX) {
+                        const double kf = b->inst[ids[e]].k;
+                        if (std::fabs(kf - (k0 + (double)(e - s) * dk)) > 1e-12 * std::max(1.0, std::fabs(kf))) break;
+                        ++e;
+                    }
+                }
+            }
+            if (e - s >= 2) {
+                FreqFamily fam;
+                std::memset(&fam, 0, sizeof(fam));
+                const BatchInstance& first = b->inst[ids[s]];
+                fam.panel_off = first.panel_off;
+                fam.n = first.n;
+                fam.k0 = first.k;
+                fam.dk = b->inst[ids[s + 1]].k - first.k;
+                fam.count = (int32_t)(e - s);
+                for (size_t f = 0; f < e - s; ++f) {
+                    fam.member[f] = ids[s + f];
+                    fam.kf[f] = b->inst[ids[s + f]].k;
+                    b->inst[ids[s + f]].in_family = 1;
+                }
+                b->families.push_back(fam);
+                max_n = std::max<int64_t>(max_n, fam.n);
+            }
+            s = e;
+        }
+    }
+    b->family_max_n = max_n;
+    if (!b->families.empty()) upload(b->ctx, b->d_families, b->families);
 }
 
 // Assembly items (instance, first row); they depend on the tuning.
@@ -280,6 +342,9 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 
 void run_assembly(sbh_batch* b) {
     BatchAssemblyArgs args;
+    args.families = b->d_families.ptr;
+    args.nfamilies = (int)b->families.size();
+    args.family_max_n = b->family_max_n;
     args.inst = b->d_inst.ptr;
     args.items = b->d_items.ptr;
     args.nitems = b->nitems;
@@ -467,6 +532,7 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
                 copy_on_stream(b->ctx, d.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
             }
         };
+        build_families(b);  // sets in_family on the instances before they are uploaded
         put(b->d_inst, b->inst);
         put(b->d_col_inst, b->col_inst);
         put(b->d_col_q, b->col_q);

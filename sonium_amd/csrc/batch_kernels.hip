@@ -581,8 +581,11 @@ __global__ __launch_bounds__(BLOCK) void rresid_kernel(const VecChunk* __restric
 
 // After a residual: ||r|| per column, convergence test ||r|| / ||b|| < tol,
 // completion time stamp, and the start of the next cycle: g = (||r||, 0, ...),
-// scale = 1/||r||, threshold of the Arnoldi estimate = inner_tol * ||r||
-// (refinement) or tol * ||b|| (plain).  mode 0: after a cycle (counted);
+// scale = 1/||r||, threshold of the Arnoldi estimate = inner_tol * ||r||,
+// but not below a quarter of the outer target tol * ||b|| (refinement: a
+// cycle that starts near the target stops as soon as its low-precision
+// estimate passes it instead of gaining another inner_tol), or tol * ||b||
+// (plain).  mode 0: after a cycle (counted);
 // 1: the residual of x = 0, which also sets ||b||; 2: the residual of an
 // initial guess (not counted).
 __global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, const int* __restrict__ col_q,
@@ -621,7 +624,7 @@ __global__ void rrestart_kernel(int C, int m, const int* __restrict__ col_inst, 
     gc[0] = make_double2(rn, 0.0);
     for (int i = 1; i <= m; ++i) gc[i] = make_double2(0.0, 0.0);
     scale[c] = 1.0 / rn;
-    tol_abs[c] = refinement ? inner_tol * rn : tol * bn;
+    tol_abs[c] = refinement ? fmax(inner_tol * rn, 0.25 * tol * bn) : tol * bn;
     active[c] = 1;
 }
 

@@ -200,7 +200,22 @@ struct BatchInstance {
     double tau;
     double sign;
     double beta[2];
-    double pad2;
+    int32_t in_family;  // far entries written by the frequency-family kernel
+    int32_t pad2;
+};
+
+// Instances on one mesh whose wavenumbers are k0 + f dk (f < count): the far
+// entries of all of them come from one pass over the pair geometry, with the
+// phase exp(i k_f r) advanced from f to f + 1 by one complex product.
+constexpr int FAMILY_MAX = 16;
+struct FreqFamily {
+    int64_t panel_off;  // the shared geometry (that of the first member)
+    int64_t n;
+    double k0, dk;
+    int32_t count;
+    int32_t pad;
+    int32_t member[FAMILY_MAX];  // instance indices, in order of f
+    double kf[FAMILY_MAX];       // the members' own wavenumbers
 };
 
 // A subdivision leaf is stored as its path from the root triangle: bits
@@ -346,6 +361,15 @@ struct sbh_precond {
     int callback_memory = 0;
 };
 
+// Recycled GCRO-DR subspace: up to k orthonormal vectors of order n, rows of U.
+struct sbh_gcro_subspace {
+    sbh_context* ctx = nullptr;
+    int64_t n = 0;
+    int k = 0;
+    int count = 0;
+    sbh::DeviceArray<double2> U;
+};
+
 namespace sbh {
 
 // Copies and fills ordered on the context's stream and waited for before
@@ -370,6 +394,7 @@ inline sbh_context* ctx_of(const sbh_system* s) { return s ? s->ctx : nullptr; }
 inline sbh_context* ctx_of(const sbh_operator* o) { return o ? o->ctx : nullptr; }
 inline sbh_context* ctx_of(const sbh_precond* p) { return p ? p->ctx : nullptr; }
 inline sbh_context* ctx_of(const sbh_lowp_matrix* l) { return l ? l->ctx : nullptr; }
+inline sbh_context* ctx_of(const sbh_gcro_subspace* g) { return g ? g->ctx : nullptr; }
 
 // guard() with the context's operation lock held and its device current on
 // the calling thread.  A null context runs f unlocked, so f reports its own
@@ -448,6 +473,9 @@ void launch_incident_rhs(sbh_context* ctx, const sbh_mesh* mesh, const sbh_physi
 // first row) pairs; each writes the far entries, near pairs, diagonal and
 // plane-wave right-hand sides of rows_per_item rows.  counter holds 2 words.
 struct BatchAssemblyArgs {
+    const FreqFamily* families;
+    int nfamilies;
+    int64_t family_max_n;
     const BatchInstance* inst;
     const int2* items;
     int nitems;

@@ -1054,6 +1054,104 @@ __global__ void plane_wave_kernel(const Source* __restrict__ sources, int64_t n,
 
 #pragma clang fp contract(fast)
 
+// Far entries of a frequency family (instances on one mesh with wavenumbers
+// k_f = k0 + f dk): one lane per column, FAMILY_ROWS rows per workgroup,
+// blockIdx.z the family.  Per TR13 point the k-independent geometry
+// (distance, 1/r, normal projections, weight) is formed once, the phase
+// exp(i k0 r) and the step exp(i dk r) come from two sincos evaluations, and
+// each member then needs only the phase advance (one complex product) and
+// its channel sums.  The members' own k_f enter the channel formulas; the
+// phase recurrence differs from a direct sincos(k_f r) by at most about
+// F ulp of the phase (F <= 16).  MODE 0 (velocity panels without
+// admittance): A_ij = H tau + E beta, the far_sums/velocity_coefficient
+// formulas of the single-frequency far kernel.
+constexpr int FAMILY_ROWS = 16;
+
+template <int F>
+__global__ __launch_bounds__(256) void batch_far_family_kernel(const FreqFamily* __restrict__ families,
+                                                               const BatchInstance* __restrict__ inst,
+                                                               const FarPanel* __restrict__ far,
+                                                               const Source* __restrict__ sources,
+                                                               double2* __restrict__ A) {
+    const FreqFamily& fam = families[blockIdx.z];
+    const int64_t n = fam.n;
+    const int64_t i0 = (int64_t)blockIdx.y * FAMILY_ROWS;
+    if ((int64_t)blockIdx.x * 256 >= n || i0 >= n) return;
+    const int count = fam.count;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < n;
+    const FarPanel P = far[fam.panel_off + (active ? j : n - 1)];
+    const FarPoints FP = far_points(P);
+    const Source* src = sources + fam.panel_off;
+    const double k0 = fam.k0, dk = fam.dk;
+    const int64_t i1 = min(n, i0 + (int64_t)FAMILY_ROWS);
+    for (int64_t i = i0; i < i1; ++i) {
+        const Source S = src[i];
+        const double rel0 = P.v0[0] - S.c[0];
+        const double rel1 = P.v0[1] - S.c[1];
+        const double rel2 = P.v0[2] - S.c[2];
+        const double nxny = S.n[0] * P.nf[0] + S.n[1] * P.nf[1] + S.n[2] * P.nf[2];
+        const double a = rel0 * rel0 + rel1 * rel1 + rel2 * rel2;
+        const double b1 = 2.0 * (rel0 * P.e1[0] + rel1 * P.e1[1] + rel2 * P.e1[2]);
+        const double b2 = 2.0 * (rel0 * P.e2[0] + rel1 * P.e2[1] + rel2 * P.e2[2]);
+        const double pr = rel0 * P.nf[0] + rel1 * P.nf[1] + rel2 * P.nf[2];
+        const double g0 = -(rel0 * S.n[0] + rel1 * S.n[1] + rel2 * S.n[2]);
+        const double g1 = -(P.e1[0] * S.n[0] + P.e1[1] * S.n[1] + P.e1[2] * S.n[2]);
+        const double g2 = -(P.e2[0] * S.n[0] + P.e2[1] * S.n[1] + P.e2[2] * S.n[2]);
+        double hr[F], hi[F], er[F], ei[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) hr[f] = hi[f] = er[f] = ei[f] = 0.0;
+#pragma clang loop unroll(disable)
+        for (int q = 0; q < 13; ++q) {
+            const double sq = kTrS[q], tq = kTrT[q];
+            const double u1 = __builtin_fma(tq, FP.e12x2, __builtin_fma(sq, FP.e11, b1));
+            const double u2 = __builtin_fma(tq, FP.e22, b2);
+            const double d2 = fmax(__builtin_fma(sq, u1, __builtin_fma(tq, u2, a)), 1e-300);
+            const double inv = rsqrt_nr(d2);
+            const double dis = d2 * inv;
+            const double pn = pr * inv;                                                // r^.n_y
+            const double qn = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0)) * inv;  // -r^.n_x
+            const double re2 = (kTrW[q] * P.wj) * inv;
+            const double inv2 = inv * inv;
+            const double rq = pn * qn;
+            // ef = (3/r^2 - k^2) rq + n.n/r^2  -  i k (3 rq + n.n)/r
+            const double ef0 = (3.0 * inv2) * rq + nxny * inv2;
+            const double ef1 = -inv * (3.0 * rq + nxny);
+            double phr, phi, str, sti;
+            sincos_red(k0 * dis, &phi, &phr);
+            sincos_red(dk * dis, &sti, &str);
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                if (f < count) {
+                    const double kf = fam.kf[f];
+                    const double zgr = phr * re2, zgi = phi * re2;
+                    const double zbr = -zgr * inv - zgi * kf;
+                    const double zbi = zgr * kf - zgi * inv;
+                    hr[f] = __builtin_fma(zbr, pn, hr[f]);
+                    hi[f] = __builtin_fma(zbi, pn, hi[f]);
+                    const double efr = __builtin_fma(-kf * kf, rq, ef0);
+                    const double efi = kf * ef1;
+                    er[f] += zgr * efr - zgi * efi;
+                    ei[f] += zgr * efi + zgi * efr;
+                    const double nr = phr * str - phi * sti;
+                    phi = phr * sti + phi * str;
+                    phr = nr;
+                }
+            }
+        }
+        if (active) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                if (f < count) {
+                    const BatchInstance& I = inst[fam.member[f]];
+                    A[I.a_off + i * n + j] =
+                        velocity_coefficient(hr[f], hi[f], er[f], ei[f], I.tau, make_double2(I.beta[0], I.beta[1]));
+                }
+            }
+        }
+    }
+}
+
 // Batch of rigid instances, two persistent launches over the same items
 // (instance, first row); the host orders items by decreasing instance size
 // and each workgroup takes items from a counter until none are left.
@@ -1085,7 +1183,8 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
         if (PHASE == 0) {
             const FarPanel* far = args.far + I.panel_off;
             const double k2 = k * k;
-            for (int64_t jc = 0; jc < n; jc += 256) {
+            // members of a frequency family got their far entries from batch_far_family_kernel
+            for (int64_t jc = 0; jc < (I.in_family ? 0 : n); jc += 256) {
                 const int64_t j = jc + threadIdx.x;
                 const bool active = j < n;
                 const FarPanel P = far[active ? j : n - 1];
@@ -1619,6 +1718,14 @@ void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int 
     ctx->begin_timed(SBH_KERNEL_ASSEMBLE_FAR, &ev);
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(2), 0, ctx->stream, args.counter);
     SBH_HIP(hipGetLastError());
+    if (args.nfamilies > 0) {
+        const int64_t nb = (args.family_max_n + 255) / 256;
+        const int64_t rb = (args.family_max_n + FAMILY_ROWS - 1) / FAMILY_ROWS;
+        SBH_REQUIRE(nb < 65536 && rb < 65536 && args.nfamilies < 65536, "frequency-family grid out of range");
+        hipLaunchKernelGGL(batch_far_family_kernel<FAMILY_MAX>, dim3((unsigned)nb, (unsigned)rb, (unsigned)args.nfamilies),
+                           dim3(256), 0, ctx->stream, args.families, args.inst, args.far, args.src, args.A);
+        SBH_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(batch_assemble_kernel<0>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);

@@ -137,3 +137,83 @@ def solve_with_guess(op: DenseMatrixOperator, b, x0: Optional[np.ndarray], confi
     if sol is None:
         sol = gmres(op, b, config)
     return (sol, used, rel) if with_residual else (sol, used)
+
+
+def minimal_residual_combination(b, candidates, products):
+    """x0 = sum_j c_j x_j with c minimising ||b - sum_j c_j (A x_j)||.
+
+    candidates and products are lists of length-n vectors (A x_j computed by
+    the caller, on the device); an all-zero candidate gets coefficient zero.
+    Returns (x0, ||b - A x0|| / ||b||).  A minimal-residual combination of
+    earlier solutions generalises the previous-frequency guess
+    (frequency_sweep.rs:718-730): with one candidate it is that guess scaled
+    by its optimal factor."""
+    P = np.stack([np.asarray(p, dtype=np.complex128) for p in products], axis=1)
+    X = np.stack([np.asarray(x, dtype=np.complex128) for x in candidates], axis=1)
+    b = np.asarray(b, dtype=np.complex128)
+    coef, *_ = np.linalg.lstsq(P, b, rcond=None)
+    r = b - P @ coef
+    return X @ coef, float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def nearest_solved(solved: dict, j: int):
+    """Indices of the nearest solved sweep points below and above j (None when absent)."""
+    below = [i for i in solved if i < j]
+    above = [i for i in solved if i > j]
+    return (max(below) if below else None), (min(above) if above else None)
+
+
+def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmStartMLP"], centers, normals, ks,
+                  direction, products: str = "fp64"):
+    """Initial guesses for a batch whose instance i is sweep point indices[i]
+    (one right-hand side each), set on the batch for solve(use_guess=True).
+
+    kind:
+      "learned"             the network's guess;
+      "previous"            the solution at the nearest solved point below (zero if none);
+      "neighbours"          minimal-residual combination of the solutions at the nearest solved
+                            points below and above;
+      "neighbours+learned"  the same with the network's guess as a third candidate.
+    The products A_i x for the combinations come from one batched launch per
+    candidate slot (sbh_batch_apply).  A combination whose residual is not
+    below ||b|| falls back to zero.  Returns the guesses (len(indices), n) and
+    the combinations' relative residuals."""
+    n = centers.shape[0]
+    nb = len(indices)
+    zeros = np.zeros(n, dtype=np.complex128)
+    learned = net.guesses(centers, normals, ks, direction) if "learned" in kind else None
+    rels = []
+    if kind == "learned":
+        X0 = learned
+    elif kind == "previous":
+        X0 = np.stack([solved[lo] if lo is not None else zeros
+                       for lo, _ in (nearest_solved(solved, j) for j in indices)])
+    elif kind in ("neighbours", "neighbours+learned"):
+        slots = []
+        near = [nearest_solved(solved, j) for j in indices]
+        slots.append([solved[lo] if lo is not None else zeros for lo, _ in near])
+        slots.append([solved[hi] if hi is not None else zeros for _, hi in near])
+        if learned is not None:
+            slots.append([learned[i] for i in range(nb)])
+        prods = [batch.apply([x.reshape(n, 1) for x in slot], products) for slot in slots]
+        # all instances at once: the m x m normal equations of every instance
+        # (m <= 3 candidates) from batched products, solved by pseudo-inverse
+        Q = np.stack([np.stack([p[i][:, 0] for p in prods]) for i in range(nb)])  # nb x m x n: A_i x_c
+        Xc = np.stack([np.stack([sl[i] for sl in slots]) for i in range(nb)])    # nb x m x n: x_c
+        Bv = np.stack([batch.rhs(i)[:, 0] for i in range(nb)])                   # nb x n
+        G = Q.conj() @ np.swapaxes(Q, 1, 2)
+        h = (Q.conj() @ Bv[:, :, None])[:, :, 0]
+        coef = (np.linalg.pinv(G, rcond=1e-13, hermitian=True) @ h[:, :, None])[:, :, 0]
+        X0 = (np.swapaxes(Xc, 1, 2) @ coef[:, :, None])[:, :, 0]
+        R = Bv - (np.swapaxes(Q, 1, 2) @ coef[:, :, None])[:, :, 0]
+        rel_all = np.linalg.norm(R, axis=1) / np.linalg.norm(Bv, axis=1)
+        for i in range(nb):
+            rel = float(rel_all[i])
+            if not (np.isfinite(rel) and rel < 1.0):
+                X0[i], rel = zeros, 1.0
+            rels.append(rel)
+    else:
+        raise ValueError(f"unknown guess kind {kind!r}")
+    for i in range(nb):
+        batch.set_guess(i, X0[i].reshape(n, 1))
+    return X0, rels

@@ -190,3 +190,106 @@ def test_batch_apply_products(ctx, small_batch):
     Xn[2][0, 0] = np.nan
     with pytest.raises(sa.SbhError):
         b.apply(Xn, "fp64")
+
+
+def test_batch_sweep_guesses_from_solved_neighbours(ctx):
+    """Interleaved sweep groups: the second group starts from minimal-residual
+    combinations of the first group's solutions (warmstart.batch_guesses),
+    whose residuals, formed from batched products on the device, match the
+    host recomputation, and which converge in fewer Arnoldi steps than zero
+    starts; "previous" takes the nearest solved point below."""
+    from sonium_amd import warmstart as wsm
+    from sonium_amd.mesh import icosphere
+
+    m = icosphere(1.0, 2)
+    tm = sa.TbemMesh(ctx, m)
+    c, nn, _ = tm.geometry()
+    ks = np.linspace(2.0, 2.7, 8)
+    d = np.array([0.0, 0.0, 1.0])
+    b = sa.Batch(ctx, [tm] * 4)
+    solved = {}
+    idx = [0, 2, 4, 6]
+    b.set_problem([float(ks[i]) for i in idx], [d.reshape(1, 3)] * 4)
+    b.assemble()
+    st = b.solve("fp64", restart=50, max_cycles=10, tolerance=1e-10, use_graph=False)
+    assert all(s.converged for s in st)
+    for i, j in enumerate(idx):
+        solved[j] = b.solution(i)[:, 0].copy()
+    idx = [1, 3, 5, 7]
+    b.set_problem([float(ks[i]) for i in idx], [d.reshape(1, 3)] * 4)
+    b.assemble()
+    zero = b.solve("fp64", restart=50, max_cycles=10, tolerance=1e-10, use_graph=False)
+    X0, rels = wsm.batch_guesses(b, "neighbours", idx, solved, None, c, nn, ks[idx], d)
+    for i in range(4):
+        A, rhs = b.matrix(i), b.rhs(i)[:, 0]
+        r = np.linalg.norm(rhs - A @ X0[i]) / np.linalg.norm(rhs)
+        assert abs(r - rels[i]) <= 1e-12 + 1e-8 * r
+        assert r < (0.05 if i < 3 else 0.2)  # the last point has a solved neighbour below only
+    guessed = b.solve("fp64", restart=50, max_cycles=10, tolerance=1e-10, use_graph=False, use_guess=True)
+    print("zero-start steps", [s.iterations for s in zero], "neighbour-combination steps",
+          [s.iterations for s in guessed], "guess residuals", ["%.1e" % r for r in rels])
+    assert all(s.converged for s in guessed)
+    assert sum(s.iterations for s in guessed) < sum(s.iterations for s in zero)
+    for i in range(4):
+        A, rhs = b.matrix(i), b.rhs(i)[:, 0]
+        assert np.linalg.norm(rhs - A @ b.solution(i)[:, 0]) / np.linalg.norm(rhs) < 1e-10
+    Xp, _ = wsm.batch_guesses(b, "previous", idx, solved, None, c, nn, ks[idx], d)
+    for i, j in enumerate(idx):
+        assert np.array_equal(Xp[i], solved[j - 1])
+    b.close()
+    tm.close()
+
+
+def test_frequency_family_matches_single_system(ctx):
+    """Instances on one mesh with equally spaced wavenumbers share one pass over
+    the pair geometry (batch_far_family_kernel, phases advanced by complex
+    products).  Their matrices agree with the single-system assembly to
+    rounding (1e-13 relative Frobenius, the recurrence and a different
+    summation of the E channel) and with the oracle to the 1e-12 bar; the
+    right-hand sides are unchanged; an instance without an equally spaced
+    partner keeps the per-instance loop, bit for bit."""
+    from sonium_amd.mesh import icosphere
+
+    m = icosphere(1.0, 3)
+    tm = sa.TbemMesh(ctx, m)
+    ws = sa.TbemAssemblyWorkspace(ctx, tm)
+    d = np.array([[0.0, 0.0, 1.0]])
+    ks = 2.0 + 0.05 * np.arange(20)
+    perm = np.random.default_rng(4).permutation(20)  # the batch order need not be sorted
+    b = sa.Batch(ctx, [tm] * 20)
+    b.set_problem([float(ks[p]) for p in perm], [d] * 20)
+    b.assemble()
+    worst = 0.0
+    for i in range(20):
+        k = float(ks[perm[i]])
+        ph = sa.PhysicsParams.from_wavenumber(k)
+        beta = ph.burton_miller_beta()
+        A1 = ws.assemble(ph, beta).matrix
+        err = rel_fro(b.matrix(i), A1)
+        worst = max(worst, err)
+        assert err < 1e-13, f"instance {i} (k={k}): {err:.2e}"
+        if i in (0, 19):
+            B1 = sa.IncidentField(d).compute_rhs_with_beta(tm, ph, beta).reshape(-1, 1)
+            assert rel_fro(b.rhs(i), B1) < 1e-15
+            A_ref = oracle.build_tbem_system(m.nodes, m.triangles, k, beta)[0]
+            assert rel_fro(b.matrix(i), A_ref) < 1e-12
+    print(f"frequency families (16 + 4 members): worst relative Frobenius difference {worst:.2e}")
+    sts = b.solve("fp64", restart=60, max_cycles=5, tolerance=1e-8)
+    assert all(s.converged and s.residual < 1e-8 for s in sts)
+    # irregular spacing: 2.0 and 2.13 form a family of two, 2.5 is left alone and keeps
+    # the per-instance loop, bit for bit
+    b2 = sa.Batch(ctx, [tm] * 3)
+    kk = [2.0, 2.13, 2.5]
+    b2.set_problem(kk, [d] * 3)
+    b2.assemble()
+    for i, k in enumerate(kk):
+        ph = sa.PhysicsParams.from_wavenumber(k)
+        A1 = ws.assemble(ph, ph.burton_miller_beta()).matrix
+        if i == 2:
+            assert np.array_equal(b2.matrix(i), A1)
+        else:
+            assert rel_fro(b2.matrix(i), A1) < 1e-13
+    b.close()
+    b2.close()
+    ws.close()
+    tm.close()

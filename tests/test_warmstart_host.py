@@ -1,0 +1,42 @@
+"""Host logic of the sweep warm starts (no GPU): the minimal-residual
+combination of candidate solutions and the neighbour lookup."""
+
+import numpy as np
+
+from sonium_amd.warmstart import minimal_residual_combination, nearest_solved
+
+
+def test_minimal_residual_combination_recovers_a_solution_in_the_span():
+    rng = np.random.default_rng(5)
+    n = 40
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 8 * np.eye(n)
+    x1 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x2 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x = (0.3 - 0.7j) * x1 + (1.1 + 0.2j) * x2
+    b = A @ x
+    x0, rel = minimal_residual_combination(b, [x1, x2], [A @ x1, A @ x2])
+    assert rel < 1e-13
+    np.testing.assert_allclose(x0, x, rtol=0, atol=1e-12 * np.abs(x).max())
+    # outside the span: the residual is the least-squares one, below the best single candidate's
+    x3 = rng.standard_normal(n) + 0j
+    b3 = A @ x3
+    x0, rel = minimal_residual_combination(b3, [x1, x2], [A @ x1, A @ x2])
+    P = np.stack([A @ x1, A @ x2], 1)
+    c = np.linalg.lstsq(P, b3, rcond=None)[0]
+    assert abs(rel - np.linalg.norm(b3 - P @ c) / np.linalg.norm(b3)) < 1e-12
+    for xc in (x1, x2):
+        Ax = A @ xc
+        best = np.linalg.norm(b3 - Ax * (np.vdot(Ax, b3) / np.vdot(Ax, Ax))) / np.linalg.norm(b3)
+        assert rel <= best + 1e-12
+    # an all-zero candidate takes no weight
+    z = np.zeros(n, complex)
+    x0, rel = minimal_residual_combination(b, [x1, z, x2], [A @ x1, z, A @ x2])
+    assert rel < 1e-13 and np.all(np.isfinite(x0))
+
+
+def test_nearest_solved():
+    solved = {0: None, 4: None, 8: None}
+    assert nearest_solved(solved, 2) == (0, 4)
+    assert nearest_solved(solved, 9) == (8, None)
+    assert nearest_solved({}, 3) == (None, None)
+    assert nearest_solved({5: None}, 1) == (None, 5)
