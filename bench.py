@@ -439,10 +439,17 @@ def run_c4(args, ctx=None):
     batch.close()
     # Latency knob: the same solve over sub-batches served one after another
     # (each sub-batch's instances assembled and solved together); an
-    # instance's service latency is counted from its sub-batch's start.
+    # instance's service latency is counted from its sub-batch's start, and
+    # its job latency from the start of the whole job.  "size_classes" serves
+    # one sub-batch per panel count, smallest first, so the many cheap
+    # instances finish before the large ones start.
     sub = {}
-    for size in (64, 16):
-        groups = [list(range(g, min(g + size, len(specs)))) for g in range(0, len(specs), size)]
+    order_by_size = sorted(range(len(specs)), key=lambda i: (sizes[i], i))
+    for size in (64, 16, "size_classes"):
+        if size == "size_classes":
+            groups = [[i for i in order_by_size if sizes[i] == nn] for nn in sorted(set(sizes.tolist()))]
+        else:
+            groups = [list(range(g, min(g + size, len(specs)))) for g in range(0, len(specs), size)]
         batches = []
         for idx in groups:
             bb = sa.Batch(ctx, [tms[i] for i in idx])
@@ -450,11 +457,12 @@ def run_c4(args, ctx=None):
             bb.set_tuning(**best)
             batches.append(bb)
         for rep in range(2):  # the first pass captures each sub-batch's cycle graph
-            lat = []
+            lat, job_lat = [], []
             ctx.synchronize()
             t_all = time.perf_counter()
             for idx, bb in zip(groups, batches):
                 t0 = time.perf_counter()
+                offset_ms = (t0 - t_all) * 1e3
                 bb.assemble()
                 bb.convert("fp16")
                 sts = bb.solve("fp16_ir", restart=30, max_cycles=30, tolerance=1e-8, inner_tolerance=1e-3,
@@ -465,14 +473,18 @@ def run_c4(args, ctx=None):
                 for i in idx:
                     c = int(ncols[i])
                     lat.append(max(q.completion_ms for q in sts[col: col + c]))
+                    job_lat.append(offset_ms + lat[-1])
                     col += c
             ctx.synchronize()
             el = time.perf_counter() - t_all
-        sub[f"sub_batches_of_{size}"] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
-                                         "vs_plain": (total_rhs / el) / (total_rhs / t_plain)}
+        jp = _percentiles(job_lat)
+        key = "size_classes_smallest_first" if size == "size_classes" else f"sub_batches_of_{size}"
+        sub[key] = {"seconds": el, "rhs_per_s": total_rhs / el, **_percentiles(lat),
+                    "job_p50_ms": jp["p50_ms"], "job_p99_ms": jp["p99_ms"],
+                    "vs_plain": (total_rhs / el) / (total_rhs / t_plain)}
         for bb in batches:
             bb.close()
-        print(f"c4: sub-batches of {size}: {el:.3f} s", file=sys.stderr, flush=True)
+        print(f"c4: {key}: {el:.3f} s, job p50 {jp['p50_ms']:.1f} ms", file=sys.stderr, flush=True)
     forms["batched_fp16_ir_sub_batches"] = sub
     line = {"metric": "solved RHS/s at rel. residual 1e-8 (batch)",
             "workload": "C4: 256 rigid ellipsoids, semi-axes U[0.5,1.5], 320/1280/5120 panels, k U[1,8], 1-16 plane waves, seed 20261202",
@@ -493,7 +505,12 @@ def run_c5(args, ctx=None):
     solutions.  Evaluation: 256 held-out wavenumbers (midpoints of the
     training grid) on the sphere; per wavenumber a new matrix is assembled and
     solved to 1e-8 from (a) zero, (b) the previous wavenumber's solution,
-    (c) the network's guess.
+    (c) the network's guess, (d) GCRO-DR recycling, alone and with (b).
+    Batched forms: 4 interleaved groups of 64 frequencies (indices g, g+4,
+    ...; order 0, 2, 1, 3), each group one batch on the shared mesh (its
+    equally spaced wavenumbers take the frequency-family assembly), solved by
+    lock-step FP16 refinement to the FP64 residual bar from zero, previous,
+    learned, or minimal-residual neighbour-combination guesses.
     """
     from sonium_amd import warmstart as wsm
     from sonium_amd.mesh import Mesh, uniform_directions
@@ -594,38 +611,78 @@ def run_c5(args, ctx=None):
             res[mode]["guess_rel_error_max"] = float(np.max(guess_err))
             res[mode]["guess_rel_residual_median"] = float(np.median(guess_res))
         print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}", file=sys.stderr, flush=True)
-    # batched forms: groups of frequencies as batch instances on the same mesh, assembled in
-    # one persistent launch and solved by lock-step FP16 iterative refinement to the FP64
-    # residual bar, from zero guesses or from the network's guesses (the chosen form)
+    # the reference's other sweep accelerator: GCRO-DR recycling 10 harmonic Ritz vectors from
+    # one frequency to the next (frequency_sweep.rs:495-508,636-650), alone and with the
+    # previous solution as the initial guess; each solve also spends 10 products rebuilding C
+    gcfg = sa.GcroDrConfig.from_gmres(cfg)
+    for mode in ("gcro_dr", "gcro_dr_previous"):
+        sub = sa.GcroDrSubspace(ctx, n, 10)
+        iters, spmv = [], []
+        prev = None
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in ks:
+            ph = sa.PhysicsParams.from_wavenumber(float(k))
+            beta = ph.burton_miller_beta()
+            ws.assemble(ph, beta, system)
+            b = sa.IncidentField([d]).compute_rhs_with_beta(tm, ph, beta)
+            sol = sa.gcro_dr(op, b, prev if mode == "gcro_dr_previous" else None, sub, gcfg)
+            if not (sol.converged and sol.residual <= 1e-8):
+                raise SystemExit(f"bench c5: {mode} solve at k={k} did not converge")
+            iters.append(sol.iterations)
+            spmv.append(sol.spmv_count)
+            prev = sol.x
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = {"rhs_per_s": len(ks) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
+                     "max_iterations": int(np.max(iters)), "mean_products": float(np.mean(spmv)),
+                     "recycled_vectors": 10}
+        sub.close()
+        print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}, mean products {np.mean(spmv):.1f}",
+              file=sys.stderr, flush=True)
+    # batched forms: groups of 64 frequencies as batch instances on the same mesh, assembled in
+    # one persistent launch and solved by lock-step FP16 refinement to the FP64 residual bar.
+    # The 256 frequencies are taken in 4 interleaved groups (indices g, g+4, ...), in the order
+    # 0, 2, 1, 3, so every group after the first has solved neighbours on both sides.
     group = 64
+    stride = len(ks) // group
+    order = [0, 2, 1, 3] if stride == 4 else list(range(stride))
     tms_batch = [tm] * group
     t0 = time.perf_counter()
     batch = sa.Batch(ctx, tms_batch)
     batch_setup_s = time.perf_counter() - t0
     x_ref = np.stack([x_zero[float(k)] for k in ks])
-    for mode in ("batched_fp16_ir_zero", "chosen_batched_fp16_ir_learned"):
-        learned = mode.endswith("learned")
-        iters, residuals, errs = [], [], []
+    batched_modes = {
+        # name: (guess in the first group, guess in later groups)
+        "batched_fp16_ir_zero": (None, None),
+        "batched_fp16_ir_previous": (None, "previous"),
+        "batched_fp16_ir_learned": ("learned", "learned"),
+        "batched_fp16_ir_neighbours": (None, "neighbours"),
+        "chosen_batched_fp16_ir_learned_then_neighbours": ("learned", "neighbours"),
+        "batched_fp16_ir_learned_then_neighbours_and_learned": ("learned", "neighbours+learned"),
+    }
+    for mode, (first, later) in batched_modes.items():
+        iters, residuals, errs, guess_res = [], [], [], []
         parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
+        solved = {}
         ctx.synchronize()
         t0 = time.perf_counter()
-        for g0 in range(0, len(ks), group):
-            kg = ks[g0: g0 + group]
-            if len(kg) < group:
-                break
+        for gi, g in enumerate(order):
+            idx = list(range(g, len(ks), stride))
+            kg = ks[idx]
             ta = time.perf_counter()
             batch.set_problem([float(k) for k in kg], [d.reshape(1, 3)] * group)
             batch.assemble()
             batch.convert("fp16")
             ctx.synchronize()
             tb = time.perf_counter()
-            if learned:
-                X0 = net.guesses(c, nn, kg, d)
-                for i in range(group):
-                    batch.set_guess(i, X0[i].reshape(n, 1))
+            kind = first if gi == 0 else later
+            if kind is not None:
+                X0, rel = wsm.batch_guesses(batch, kind, idx, solved, net, c, nn, kg, d)
+                guess_res += rel
             tc = time.perf_counter()
             sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
-                              use_guess=learned)
+                              use_guess=kind is not None)
             td = time.perf_counter()
             parts["assemble_convert"] += tb - ta
             parts["guesses"] += tc - tb
@@ -634,19 +691,24 @@ def run_c5(args, ctx=None):
                 raise SystemExit(f"bench c5: batched {mode} left a frequency unconverged")
             iters += [q.iterations for q in sts]
             residuals += [q.residual for q in sts]
+            if later in ("previous", "neighbours", "neighbours+learned") and gi + 1 < len(order):
+                for j, x in zip(idx, batch.solutions()):
+                    solved[j] = x[:, 0]
             for i in range(0, group, 16):
                 xs = batch.solution(i)[:, 0]
-                errs.append(float(np.linalg.norm(xs - x_ref[g0 + i]) / np.linalg.norm(x_ref[g0 + i])))
+                errs.append(float(np.linalg.norm(xs - x_ref[idx[i]]) / np.linalg.norm(x_ref[idx[i]])))
         ctx.synchronize()
         el = time.perf_counter() - t0
         res[mode] = {"rhs_per_s": len(iters) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
                      "max_residual": float(np.max(residuals)), "group": group, "batch_setup_s": batch_setup_s,
                      "breakdown_s": parts, "error_vs_fp64_max_sampled": float(np.max(errs))}
+        if guess_res:
+            res[mode]["guess_rel_residual_median"] = float(np.median(guess_res))
         print(f"c5: {mode} {el:.2f} s, mean Arnoldi steps {np.mean(iters):.1f}", file=sys.stderr, flush=True)
-    for mode in ("previous", "learned", "batched_fp16_ir_zero", "chosen_batched_fp16_ir_learned"):
+    for mode in ["previous", "learned", "gcro_dr", "gcro_dr_previous"] + list(batched_modes):
         res[mode]["vs_zero"] = res[mode]["rhs_per_s"] / res["zero"]["rhs_per_s"]
-    res["chosen_batched_fp16_ir_learned"]["vs_batched_zero"] = (res["chosen_batched_fp16_ir_learned"]["rhs_per_s"]
-                                                              / res["batched_fp16_ir_zero"]["rhs_per_s"])
+    for mode in batched_modes:
+        res[mode]["vs_batched_zero"] = res[mode]["rhs_per_s"] / res["batched_fp16_ir_zero"]["rhs_per_s"]
     batch.close()
     net.close()
     line = {"metric": "solved RHS/s at rel. residual 1e-8 (frequency sweep)",
@@ -686,7 +748,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU row sharding is described in DESIGN.md but not built)", file=sys.stderr)
+              "(row-block sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[args.workload]
     ctx = sa.Context(0)
@@ -865,14 +927,18 @@ def main():
                          "chosen": "batched FP16 IR", "chosen_rhs_per_s": b4["batched_fp16_ir"]["rhs_per_s"],
                          "chosen_p50_ms": b4["batched_fp16_ir"]["p50_ms"], "chosen_vs_plain": b4["batched_fp16_ir"]["vs_plain"],
                          "chosen_error_vs_plain_max": b4["batched_fp16_ir"]["error_vs_plain_per_rhs_max"]}
+        ch = "chosen_batched_fp16_ir_learned_then_neighbours"
         summary["c5"] = {"plain": "zero-guess FP64 GMRES", "plain_rhs_per_s": r5["zero"]["rhs_per_s"],
                          "previous_frequency_vs_plain": r5["previous"]["vs_zero"],
                          "learned_vs_plain": r5["learned"]["vs_zero"],
-                         "chosen": "batched FP16 IR from learned guesses",
-                         "chosen_rhs_per_s": r5["chosen_batched_fp16_ir_learned"]["rhs_per_s"],
-                         "chosen_vs_plain": r5["chosen_batched_fp16_ir_learned"]["vs_zero"],
-                         "batched_zero_guess_vs_plain": r5["batched_fp16_ir_zero"]["vs_zero"],
-                         "chosen_error_vs_fp64_max": r5["chosen_batched_fp16_ir_learned"]["error_vs_fp64_max_sampled"]}
+                         "gcro_dr_vs_plain": r5["gcro_dr"]["vs_zero"],
+                         "gcro_dr_previous_vs_plain": r5["gcro_dr_previous"]["vs_zero"],
+                         "chosen": "batched FP16 IR over interleaved frequency groups (frequency-family "
+                                   "assembly); first group from learned guesses, later groups from "
+                                   "minimal-residual combinations of the solved neighbours",
+                         "chosen_rhs_per_s": r5[ch]["rhs_per_s"], "chosen_vs_plain": r5[ch]["vs_zero"],
+                         "chosen_error_vs_fp64_max": r5[ch]["error_vs_fp64_max_sampled"],
+                         "batched_vs_plain": {m: r5[m]["vs_zero"] for m in r5 if m.startswith(("batched", "chosen"))}}
 
     # measured ceilings beside the published peaks: a streaming read of a
     # matrix-sized buffer and independent FP64 FMAs on every CU

@@ -669,6 +669,10 @@ sbh_status sbh_batch_apply(sbh_batch* batch, int format, const sbh_c64* X, sbh_c
  * by the next sbh_batch_solve with use_guess = 1 (gmres_with_guess,
  * gmres.rs:564-580; the relative residual stays relative to ||b||). */
 sbh_status sbh_batch_set_guess(sbh_batch* batch, int64_t instance, const sbh_c64* x0);
+/* Every instance's guess at once (the packed layout of sbh_batch_apply). */
+sbh_status sbh_batch_set_guesses(sbh_batch* batch, const sbh_c64* X0);
+/* Every instance's solutions (what = 0) or right-hand sides (what = 1), packed. */
+sbh_status sbh_batch_download_all(sbh_batch* batch, int what, sbh_c64* out);
 /* n x num_rhs row-major blocks of one instance (host memory). */
 sbh_status sbh_batch_download_solution(sbh_batch* batch, int64_t instance, sbh_c64* x);
 sbh_status sbh_batch_download_rhs(sbh_batch* batch, int64_t instance, sbh_c64* rhs);

@@ -1362,6 +1362,36 @@ class Batch:
         x = np.ascontiguousarray(X0, dtype=np.complex128).reshape(self.meshes[i].num_elements, int(self.num_rhs[i]))
         _check(_lib().sbh_batch_set_guess(self._h, int(i), _cptr(x)))
 
+    def _packed_shapes(self):
+        return [(m.num_elements, int(c)) for m, c in zip(self.meshes, self.num_rhs)]
+
+    def _unpack(self, flat):
+        res, o = [], 0
+        for shp in self._packed_shapes():
+            cnt = shp[0] * shp[1]
+            res.append(flat[o:o + cnt].reshape(shp))
+            o += cnt
+        return res
+
+    def set_guesses(self, X0) -> None:
+        """Every instance's initial guess (list of n_i x num_rhs[i]) in one upload."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.complex128).reshape(-1) for x in X0]))
+        if flat.size != sum(a * b for a, b in self._packed_shapes()):
+            raise ValueError("one n_i x num_rhs[i] block per instance")
+        _check(_lib().sbh_batch_set_guesses(self._h, _cptr(flat)))
+
+    def solutions(self):
+        """Every instance's solution block, in one download."""
+        out = np.empty(sum(a * b for a, b in self._packed_shapes()), dtype=np.complex128)
+        _check(_lib().sbh_batch_download_all(self._h, 0, _cptr(out)))
+        return self._unpack(out)
+
+    def rhs_all(self):
+        """Every instance's right-hand-side block, in one download."""
+        out = np.empty(sum(a * b for a, b in self._packed_shapes()), dtype=np.complex128)
+        _check(_lib().sbh_batch_download_all(self._h, 1, _cptr(out)))
+        return self._unpack(out)
+
     def apply(self, X, fmt: str = "fp64"):
         """Products A_i X_i of every instance with its own matrix in one batched
         launch (the stored format: "fp64", or "fp16"/"bf16" after convert).

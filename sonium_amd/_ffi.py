@@ -246,6 +246,8 @@ SIGNATURES = {
     "sbh_batch_solve": (_int, [_vp, ctypes.POINTER(batch_solve_config_t), _vp]),
     "sbh_batch_set_guess": (_int, [_vp, _i64, _vp]),
     "sbh_batch_apply": (_int, [_vp, _int, _vp, _vp]),
+    "sbh_batch_set_guesses": (_int, [_vp, _vp]),
+    "sbh_batch_download_all": (_int, [_vp, _int, _vp]),
     "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_rhs": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_matrix": (_int, [_vp, _i64, _vp]),

@@ -755,6 +755,29 @@ sbh_status sbh_batch_apply(sbh_batch* b, int format, const sbh_c64* X, sbh_c64* 
     });
 }
 
+sbh_status sbh_batch_set_guesses(sbh_batch* b, const sbh_c64* X0) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(X0, "X0 is null");
+        use_device(b->ctx);
+        const size_t L = (size_t)b->L;
+        for (size_t i = 0; i < L; ++i)
+            SBH_REQUIRE(std::isfinite(X0[i].re) && std::isfinite(X0[i].im), "initial guess is not finite");
+        copy_on_stream(b->ctx, b->d_x.ptr, X0, L * sizeof(double2), hipMemcpyHostToDevice);
+    });
+}
+
+sbh_status sbh_batch_download_all(sbh_batch* b, int what, sbh_c64* out) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(out, "out is null");
+        SBH_REQUIRE(what == 0 || what == 1, "what must be 0 (solutions) or 1 (right-hand sides)");
+        use_device(b->ctx);
+        copy_on_stream(b->ctx, out, what == 0 ? b->d_x.ptr : b->d_rhs.ptr, (size_t)b->L * sizeof(double2),
+                       hipMemcpyDeviceToHost);
+    });
+}
+
 sbh_status sbh_batch_download_solution(sbh_batch* b, int64_t instance, sbh_c64* x) {
     return download_block(b, instance, x, 0);
 }

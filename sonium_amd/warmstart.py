@@ -104,7 +104,12 @@ class WarmStartMLP:
     def guesses(self, centers, normals, ks, direction) -> np.ndarray:
         """Initial guesses for several wavenumbers in one forward pass: (len(ks), N)."""
         n = centers.shape[0]
-        raw = np.concatenate([panel_inputs(centers, normals, float(k), direction) for k in ks])
+        raw = np.empty((len(ks), n, RAW), dtype=np.float32)
+        raw[:, :, 0:3] = centers[None]
+        raw[:, :, 3:6] = normals[None]
+        raw[:, :, 6:9] = np.asarray(direction, dtype=np.float64).reshape(1, 1, 3)
+        raw[:, :, 9] = np.asarray(ks, dtype=np.float64)[:, None]
+        raw = raw.reshape(-1, RAW)
         y = self.predict(raw).reshape(len(ks), n, 2).astype(np.float64)
         kd = np.asarray(ks, dtype=np.float64)[:, None] * (centers @ np.asarray(direction, dtype=np.float64).reshape(3))[None, :]
         return (y[..., 0] + 1j * y[..., 1]) * np.exp(1j * kd)
@@ -200,7 +205,7 @@ def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmSt
         # (m <= 3 candidates) from batched products, solved by pseudo-inverse
         Q = np.stack([np.stack([p[i][:, 0] for p in prods]) for i in range(nb)])  # nb x m x n: A_i x_c
         Xc = np.stack([np.stack([sl[i] for sl in slots]) for i in range(nb)])    # nb x m x n: x_c
-        Bv = np.stack([batch.rhs(i)[:, 0] for i in range(nb)])                   # nb x n
+        Bv = np.stack([r[:, 0] for r in batch.rhs_all()])                        # nb x n
         G = Q.conj() @ np.swapaxes(Q, 1, 2)
         h = (Q.conj() @ Bv[:, :, None])[:, :, 0]
         coef = (np.linalg.pinv(G, rcond=1e-13, hermitian=True) @ h[:, :, None])[:, :, 0]
@@ -214,6 +219,5 @@ def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmSt
             rels.append(rel)
     else:
         raise ValueError(f"unknown guess kind {kind!r}")
-    for i in range(nb):
-        batch.set_guess(i, X0[i].reshape(n, 1))
+    batch.set_guesses([x.reshape(n, 1) for x in X0])
     return X0, rels

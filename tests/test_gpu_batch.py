@@ -190,6 +190,14 @@ def test_batch_apply_products(ctx, small_batch):
     Xn[2][0, 0] = np.nan
     with pytest.raises(sa.SbhError):
         b.apply(Xn, "fp64")
+    # packed transfers equal the per-instance ones
+    for i, r in enumerate(b.rhs_all()):
+        assert np.array_equal(r, b.rhs(i))
+    b.set_guesses(X)
+    for i, x in enumerate(b.solutions()):
+        assert np.array_equal(x, X[i]) and np.array_equal(b.solution(i), X[i])
+    with pytest.raises(sa.SbhError):
+        b.set_guesses(Xn)
 
 
 def test_batch_sweep_guesses_from_solved_neighbours(ctx):
