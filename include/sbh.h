@@ -164,7 +164,8 @@ sbh_status sbh_workspace_stats(const sbh_workspace* ws, int64_t* near_pairs, int
 sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** out);
 void sbh_system_destroy(sbh_system* sys);
 int64_t sbh_system_num_dofs(const sbh_system* sys);
-/* Device pointers of the matrix (leading dimension in *lda, in elements) and rhs. */
+/* Device pointers of the matrix (leading dimension in *lda, in elements) and rhs
+ * (each output may be NULL). */
 sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t* lda,
                                       sbh_c64** rhs);
 /* Copies rows [row_begin, row_end) of the matrix (dense, N per row) and the

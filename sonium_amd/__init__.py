@@ -290,6 +290,12 @@ class TbemSystem:
         _check(_lib().sbh_system_device_pointers(self._h, ctypes.byref(a), ctypes.byref(lda), ctypes.byref(r)))
         return a.value, int(lda.value), r.value
 
+    def rhs_pointer(self) -> int:
+        """Device address of the right-hand side."""
+        r = ctypes.c_void_p()
+        _check(_lib().sbh_system_device_pointers(self._h, None, None, ctypes.byref(r)))
+        return r.value
+
     def matrix_rows(self, row_begin: int, row_end: int) -> np.ndarray:
         out = np.empty((row_end - row_begin, self.num_dofs), dtype=np.complex128)
         _check(_lib().sbh_system_download(self._h, row_begin, row_end, _cptr(out), None))
@@ -526,7 +532,7 @@ class IncidentField:
         """add_incident_field_rhs_stabilized (bem_solver.rs:5005-5034) on the device rhs."""
         if self.num_rhs != 1:
             raise ValueError("a TbemSystem holds one right-hand side")
-        _, _, rhs_ptr = system.device_pointers()
+        rhs_ptr = system.rhs_pointer()
         self.write_device_ptr(ctypes.c_void_p(rhs_ptr), mesh, physics, beta, sign, accumulate=True)
 
     def write_device(self, vec: DeviceVector, mesh: TbemMesh, physics: PhysicsParams, beta: complex,

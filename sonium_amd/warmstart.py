@@ -161,15 +161,19 @@ def minimal_residual_combination(b, candidates, products):
     return X @ coef, float(np.linalg.norm(r) / np.linalg.norm(b))
 
 
-def nearest_solved(solved: dict, j: int):
-    """Indices of the nearest solved sweep points below and above j (None when absent)."""
-    below = [i for i in solved if i < j]
-    above = [i for i in solved if i > j]
-    return (max(below) if below else None), (min(above) if above else None)
+def nearest_solved(solved: dict, j: int, per_side: int = 1):
+    """Indices of the nearest solved sweep points below and above j (None when
+    absent); with per_side > 1, lists of up to that many on each side, nearest
+    first."""
+    below = sorted((i for i in solved if i < j), reverse=True)
+    above = sorted(i for i in solved if i > j)
+    if per_side == 1:
+        return (below[0] if below else None), (above[0] if above else None)
+    return below[:per_side], above[:per_side]
 
 
 def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmStartMLP"], centers, normals, ks,
-                  direction, products: str = "fp64"):
+                  direction, products: str = "fp64", per_side: int = 1):
     """Initial guesses for a batch whose instance i is sweep point indices[i]
     (one right-hand side each), set on the batch for solve(use_guess=True).
 
@@ -177,7 +181,7 @@ def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmSt
       "learned"             the network's guess;
       "previous"            the solution at the nearest solved point below (zero if none);
       "neighbours"          minimal-residual combination of the solutions at the nearest solved
-                            points below and above;
+                            points below and above (``per_side`` of them on each side);
       "neighbours+learned"  the same with the network's guess as a third candidate.
     The products A_i x for the combinations come from one batched launch per
     candidate slot (sbh_batch_apply).  A combination whose residual is not
@@ -195,14 +199,18 @@ def batch_guesses(batch, kind: str, indices, solved: dict, net: Optional["WarmSt
                        for lo, _ in (nearest_solved(solved, j) for j in indices)])
     elif kind in ("neighbours", "neighbours+learned"):
         slots = []
-        near = [nearest_solved(solved, j) for j in indices]
-        slots.append([solved[lo] if lo is not None else zeros for lo, _ in near])
-        slots.append([solved[hi] if hi is not None else zeros for _, hi in near])
+        near = [nearest_solved(solved, j, per_side) for j in indices]
+        if per_side == 1:
+            near = [([lo] if lo is not None else [], [hi] if hi is not None else []) for lo, hi in near]
+        for r in range(per_side):
+            slots.append([solved[lo[r]] if r < len(lo) else zeros for lo, _ in near])
+            slots.append([solved[hi[r]] if r < len(hi) else zeros for _, hi in near])
         if learned is not None:
             slots.append([learned[i] for i in range(nb)])
         prods = [batch.apply([x.reshape(n, 1) for x in slot], products) for slot in slots]
         # all instances at once: the m x m normal equations of every instance
-        # (m <= 3 candidates) from batched products, solved by pseudo-inverse
+        # (m = 2 per_side, + 1 with the learned guess) from batched products,
+        # solved by pseudo-inverse
         Q = np.stack([np.stack([p[i][:, 0] for p in prods]) for i in range(nb)])  # nb x m x n: A_i x_c
         Xc = np.stack([np.stack([sl[i] for sl in slots]) for i in range(nb)])    # nb x m x n: x_c
         Bv = np.stack([r[:, 0] for r in batch.rhs_all()])                        # nb x n

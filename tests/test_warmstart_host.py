@@ -40,3 +40,28 @@ def test_nearest_solved():
     assert nearest_solved(solved, 9) == (8, None)
     assert nearest_solved({}, 3) == (None, None)
     assert nearest_solved({5: None}, 1) == (None, 5)
+
+
+def test_sweep_group_order_and_neighbours():
+    from sonium_amd.sweep import group_neighbours, group_order
+
+    assert group_order(1) == [0]
+    assert group_order(4) == [0, 2, 1, 3]
+    assert group_order(8) == [0, 4, 2, 6, 1, 3, 5, 7]
+    for stride in (2, 4, 8, 16):
+        order = group_order(stride)
+        assert sorted(order) == list(range(stride))
+        nb = group_neighbours(stride)
+        assert nb[0] is None
+        for pos, g in enumerate(order[1:], start=1):
+            lo, hi = nb[g]
+            # both bracketing groups come earlier in the order and are the nearest ones
+            assert lo in order[:pos] and (hi == stride or hi in order[:pos])
+            assert lo < g < hi
+            assert not any(lo < e < hi for e in order[:pos])
+    try:
+        group_order(6)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("a stride that is not a power of two must be refused")
