@@ -48,6 +48,9 @@ FIELD_FLOPS_PER_QP = 47  # field kernel, rigid panels: FP64 flops per (point, pa
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (copy-calibrated; profiles/zgemv_n_kernel_and_far_kernel_c2_icosphere20480_k10.md).
+# C5 batched sweep: solved points per side in the minimal-residual guesses
+C5_NEIGHBOURS_PER_SIDE = 1
+
 MEASURED_TRAFFIC = {
     ("zgemv_n_kernel", 20480): 6.7134e9 + 3.2971e5,
     ("far_kernel", 20480): 1.8935e7 + 6.7109e9,
@@ -640,20 +643,16 @@ def run_c5(args, ctx=None):
         sub.close()
         print(f"c5: {mode} {el:.2f} s, mean iterations {np.mean(iters):.1f}, mean products {np.mean(spmv):.1f}",
               file=sys.stderr, flush=True)
-    # batched forms: groups of 64 frequencies as batch instances on the same mesh, assembled in
-    # one persistent launch and solved by lock-step FP16 refinement to the FP64 residual bar.
-    # The 256 frequencies are taken in 4 interleaved groups (indices g, g+4, ...), in the order
-    # 0, 2, 1, 3, so every group after the first has solved neighbours on both sides.
-    group = 64
-    stride = len(ks) // group
-    order = [0, 2, 1, 3] if stride == 4 else list(range(stride))
-    tms_batch = [tm] * group
+    # batched forms (sonium_amd.sweep): 4 interleaved groups of 64 frequencies, coarse to fine,
+    # each one frequency-family batch solved by lock-step FP16 refinement to the FP64 residual bar
+    from sonium_amd import sweep
+    stride = 4
     t0 = time.perf_counter()
-    batch = sa.Batch(ctx, tms_batch)
+    sb = sweep.SweepBatch(ctx, base, len(ks) // stride)
     batch_setup_s = time.perf_counter() - t0
     x_ref = np.stack([x_zero[float(k)] for k in ks])
     batched_modes = {
-        # name: (guess in the first group, guess in later groups)
+        # name: (guess of the first group, guess of later groups)
         "batched_fp16_ir_zero": (None, None),
         "batched_fp16_ir_previous": (None, "previous"),
         "batched_fp16_ir_learned": ("learned", "learned"),
@@ -662,54 +661,24 @@ def run_c5(args, ctx=None):
         "batched_fp16_ir_learned_then_neighbours_and_learned": ("learned", "neighbours+learned"),
     }
     for mode, (first, later) in batched_modes.items():
-        iters, residuals, errs, guess_res = [], [], [], []
-        parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
-        solved = {}
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for gi, g in enumerate(order):
-            idx = list(range(g, len(ks), stride))
-            kg = ks[idx]
-            ta = time.perf_counter()
-            batch.set_problem([float(k) for k in kg], [d.reshape(1, 3)] * group)
-            batch.assemble()
-            batch.convert("fp16")
-            ctx.synchronize()
-            tb = time.perf_counter()
-            kind = first if gi == 0 else later
-            if kind is not None:
-                X0, rel = wsm.batch_guesses(batch, kind, idx, solved, net, c, nn, kg, d)
-                guess_res += rel
-            tc = time.perf_counter()
-            sts = batch.solve("fp16_ir", restart=60, max_cycles=20, tolerance=1e-8, inner_tolerance=1e-4,
-                              use_guess=kind is not None)
-            td = time.perf_counter()
-            parts["assemble_convert"] += tb - ta
-            parts["guesses"] += tc - tb
-            parts["solve"] += td - tc
-            if not all(q.converged for q in sts):
-                raise SystemExit(f"bench c5: batched {mode} left a frequency unconverged")
-            iters += [q.iterations for q in sts]
-            residuals += [q.residual for q in sts]
-            if later in ("previous", "neighbours", "neighbours+learned") and gi + 1 < len(order):
-                for j, x in zip(idx, batch.solutions()):
-                    solved[j] = x[:, 0]
-            for i in range(0, group, 16):
-                xs = batch.solution(i)[:, 0]
-                errs.append(float(np.linalg.norm(xs - x_ref[idx[i]]) / np.linalg.norm(x_ref[idx[i]])))
-        ctx.synchronize()
-        el = time.perf_counter() - t0
-        res[mode] = {"rhs_per_s": len(iters) / el, "seconds": el, "mean_iterations": float(np.mean(iters)),
-                     "max_residual": float(np.max(residuals)), "group": group, "batch_setup_s": batch_setup_s,
-                     "breakdown_s": parts, "error_vs_fp64_max_sampled": float(np.max(errs))}
-        if guess_res:
-            res[mode]["guess_rel_residual_median"] = float(np.median(guess_res))
-        print(f"c5: {mode} {el:.2f} s, mean Arnoldi steps {np.mean(iters):.1f}", file=sys.stderr, flush=True)
+        r = sweep.batched_sweep(ctx, base, ks, d, stride=stride, net=net, first_guess=first, later_guess=later,
+                                per_side=C5_NEIGHBOURS_PER_SIDE, sweep_batch=sb, use_graph=not args.no_graph)
+        errs = [float(np.linalg.norm(r.solutions[j] - x_ref[j]) / np.linalg.norm(x_ref[j]))
+                for j in range(0, len(ks), 16)]
+        res[mode] = {"rhs_per_s": len(ks) / r.seconds, "seconds": r.seconds,
+                     "mean_iterations": float(r.iterations.mean()), "max_iterations": int(r.iterations.max()),
+                     "max_residual": float(r.residuals.max()), "group": len(ks) // stride,
+                     "batch_setup_s": batch_setup_s, "breakdown_s": r.breakdown_s,
+                     "error_vs_fp64_max_sampled": float(np.max(errs))}
+        if r.guess_residuals:
+            res[mode]["guess_rel_residual_median"] = float(np.median(list(r.guess_residuals.values())))
+        print(f"c5: {mode} {r.seconds:.2f} s, mean Arnoldi steps {r.iterations.mean():.1f}", file=sys.stderr,
+              flush=True)
     for mode in ["previous", "learned", "gcro_dr", "gcro_dr_previous"] + list(batched_modes):
         res[mode]["vs_zero"] = res[mode]["rhs_per_s"] / res["zero"]["rhs_per_s"]
     for mode in batched_modes:
         res[mode]["vs_batched_zero"] = res[mode]["rhs_per_s"] / res["batched_fp16_ir_zero"]["rhs_per_s"]
-    batch.close()
+    sb.close()
     net.close()
     line = {"metric": "solved RHS/s at rel. residual 1e-8 (frequency sweep)",
             "workload": "C5: Fibonacci sphere 10240 panels, held-out k in [1,12] (256), +z; MLP 4x512 BF16 trained on "
@@ -768,7 +737,7 @@ def main():
     x = sa.DeviceVector(ctx, n)
     inc = sa.IncidentField.plane_wave_z()
     cfg = sa.GmresConfig(max_iterations=100, restart=wl["restart"], tolerance=1e-8)
-    _, _, rhs_ptr = system.device_pointers()
+    rhs_ptr = system.rhs_pointer()
 
     def step():
         ws.assemble(ph, beta, system)

@@ -91,14 +91,16 @@ def batched_sweep(ctx: Context, mesh, ks, direction, *, stride: int = 4, net: Op
                   first_guess: Optional[str] = "learned", later_guess: Optional[str] = "neighbours",
                   fmt: str = "fp16", restart: int = 30, max_cycles: int = 30, tolerance: float = 1e-8,
                   inner_tolerance: float = 1e-3, products: str = "fp16", per_side: int = 1,
-                  sweep_batch: Optional[SweepBatch] = None) -> SweepResult:
+                  sweep_batch: Optional[SweepBatch] = None, use_graph: bool = True) -> SweepResult:
     """Solve the rigid-mesh sweep ``ks`` (equally spaced, len divisible by
     ``stride``) for one plane wave along ``direction``.  first_guess: None or
     "learned"; later_guess: None, "previous", "learned", "neighbours" or
     "neighbours+learned" (see ``warmstart.batch_guesses``; ``products`` is the
     format of the batched products the combinations use, ``per_side`` the
     solved points per side).  ``sweep_batch`` lets a caller keep the batch
-    between sweeps (its setup, the routes, is then outside the timing)."""
+    between sweeps (its setup, the routes, is then outside the timing);
+    ``use_graph`` False launches every solver kernel directly instead of
+    replaying the captured cycle."""
     ks = np.asarray(ks, dtype=np.float64)
     if len(ks) % stride:
         raise ValueError("the number of wavenumbers must be a multiple of stride")
@@ -143,7 +145,8 @@ def batched_sweep(ctx: Context, mesh, ks, direction, *, stride: int = 4, net: Op
                     guess_res[j] = r
             tc = time.perf_counter()
             sts = batch.solve(fmt if fmt == "fp64" else fmt + "_ir", restart=restart, max_cycles=max_cycles,
-                              tolerance=tolerance, inner_tolerance=inner_tolerance, use_guess=kind is not None)
+                              tolerance=tolerance, inner_tolerance=inner_tolerance, use_graph=use_graph,
+                              use_guess=kind is not None)
             if not all(q.converged for q in sts):
                 raise RuntimeError(f"sweep group {g} left a frequency unconverged")
             for i, (j, x) in enumerate(zip(idx, batch.solutions())):

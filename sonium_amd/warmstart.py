@@ -38,6 +38,24 @@ def incident_phase(centers, k: float, direction) -> np.ndarray:
     return np.exp(1j * k * (centers @ np.asarray(direction, dtype=np.float64).reshape(3)))
 
 
+def incident_phases(centers, ks, direction) -> np.ndarray:
+    """exp(i k d.c) for every k of ks (len(ks) x N).  For equally spaced ks the
+    rows follow from the first by repeated products with exp(i dk d.c)
+    (relative error about one ulp per row), else one complex exp each."""
+    ks = np.asarray(ks, dtype=np.float64)
+    dc = centers @ np.asarray(direction, dtype=np.float64).reshape(3)
+    out = np.empty((len(ks), dc.shape[0]), dtype=np.complex128)
+    if len(ks) > 2 and np.allclose(np.diff(ks), ks[1] - ks[0], rtol=1e-12, atol=0.0):
+        ph = np.exp(1j * ks[0] * dc)
+        step = np.exp(1j * (ks[1] - ks[0]) * dc)
+        for f in range(len(ks)):
+            out[f] = ph
+            ph = ph * step
+    else:
+        out[:] = np.exp(1j * ks[:, None] * dc[None, :])
+    return out
+
+
 def density_to_target(x, centers, k: float, direction) -> np.ndarray:
     """u = x exp(-i k d.c) as (N, 2) float32."""
     u = np.asarray(x) * np.conj(incident_phase(centers, k, direction))
@@ -110,9 +128,8 @@ class WarmStartMLP:
         raw[:, :, 6:9] = np.asarray(direction, dtype=np.float64).reshape(1, 1, 3)
         raw[:, :, 9] = np.asarray(ks, dtype=np.float64)[:, None]
         raw = raw.reshape(-1, RAW)
-        y = self.predict(raw).reshape(len(ks), n, 2).astype(np.float64)
-        kd = np.asarray(ks, dtype=np.float64)[:, None] * (centers @ np.asarray(direction, dtype=np.float64).reshape(3))[None, :]
-        return (y[..., 0] + 1j * y[..., 1]) * np.exp(1j * kd)
+        y = self.predict(raw).reshape(len(ks), n, 2)
+        return (y[..., 0] + 1j * y[..., 1]) * incident_phases(centers, ks, direction)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -65,3 +65,14 @@ def test_sweep_group_order_and_neighbours():
         pass
     else:
         raise AssertionError("a stride that is not a power of two must be refused")
+
+
+def test_incident_phases_recurrence_and_direct():
+    from sonium_amd.warmstart import incident_phases
+
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal((300, 3))
+    d = np.array([0.0, 0.6, 0.8])
+    for ks in (np.linspace(1.0, 12.0, 64), np.array([1.0, 2.5, 2.7]), np.array([3.0])):
+        ref = np.exp(1j * ks[:, None] * (c @ d)[None, :])
+        assert np.abs(incident_phases(c, ks, d) - ref).max() < 1e-13
