@@ -670,6 +670,12 @@ sbh_status sbh_batch_apply(sbh_batch* batch, int format, const sbh_c64* X, sbh_c
  * by the next sbh_batch_solve with use_guess = 1 (gmres_with_guess,
  * gmres.rs:564-580; the relative residual stays relative to ||b||). */
 sbh_status sbh_batch_set_guess(sbh_batch* batch, int64_t instance, const sbh_c64* x0);
+/* The FP16/BF16 copy of one instance after sbh_batch_convert: n x 2*ld halves
+ * row-major (entry j of row r as halves 2j, 2j+1 = re, im of A_rj / row_inv[r],
+ * zero padding to ld entries) and the n power-of-two row scales; *ld may be
+ * read alone by passing NULL arrays. */
+sbh_status sbh_batch_download_lowp(sbh_batch* batch, int64_t instance, uint16_t* halves, double* row_inv,
+                                   int64_t* ld);
 /* Every instance's guess at once (the packed layout of sbh_batch_apply). */
 sbh_status sbh_batch_set_guesses(sbh_batch* batch, const sbh_c64* X0);
 /* Every instance's solutions (what = 0) or right-hand sides (what = 1), packed. */

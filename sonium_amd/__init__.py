@@ -1368,6 +1368,16 @@ class Batch:
         x = np.ascontiguousarray(X0, dtype=np.complex128).reshape(self.meshes[i].num_elements, int(self.num_rhs[i]))
         _check(_lib().sbh_batch_set_guess(self._h, int(i), _cptr(x)))
 
+    def lowp_copy(self, i: int):
+        """(halves n x 2*ld as uint16, row scales, ld) of instance i's FP16/BF16 copy."""
+        n = self.meshes[i].num_elements
+        ld = ctypes.c_int64()
+        _check(_lib().sbh_batch_download_lowp(self._h, int(i), None, None, ctypes.byref(ld)))
+        halves = np.empty((n, 2 * ld.value), dtype=np.uint16)
+        inv = np.empty(n)
+        _check(_lib().sbh_batch_download_lowp(self._h, int(i), _cptr(halves), _cptr(inv), ctypes.byref(ld)))
+        return halves, inv, int(ld.value)
+
     def _packed_shapes(self):
         return [(m.num_elements, int(c)) for m, c in zip(self.meshes, self.num_rhs)]
 

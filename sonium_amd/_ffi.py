@@ -247,6 +247,7 @@ SIGNATURES = {
     "sbh_batch_set_guess": (_int, [_vp, _i64, _vp]),
     "sbh_batch_apply": (_int, [_vp, _int, _vp, _vp]),
     "sbh_batch_set_guesses": (_int, [_vp, _vp]),
+    "sbh_batch_download_lowp": (_int, [_vp, _i64, _vp, _vp, ctypes.POINTER(_i64)]),
     "sbh_batch_download_all": (_int, [_vp, _int, _vp]),
     "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),
     "sbh_batch_download_rhs": (_int, [_vp, _i64, _vp]),

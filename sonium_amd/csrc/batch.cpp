@@ -755,6 +755,23 @@ sbh_status sbh_batch_apply(sbh_batch* b, int format, const sbh_c64* X, sbh_c64* 
     });
 }
 
+sbh_status sbh_batch_download_lowp(sbh_batch* b, int64_t instance, uint16_t* halves, double* row_inv, int64_t* ld) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(instance >= 0 && instance < b->B, "bad instance");
+        SBH_REQUIRE(b->lp_format >= 0 && b->d_Alp.count == (size_t)b->lp_elems, "the batch has no low-precision copy");
+        use_device(b->ctx);
+        const BatchInstance& I = b->inst[instance];
+        if (ld) *ld = I.ld;
+        if (halves)
+            copy_on_stream(b->ctx, halves, b->d_Alp.ptr + I.lp_off, (size_t)I.n * 2 * I.ld * sizeof(uint16_t),
+                           hipMemcpyDeviceToHost);
+        if (row_inv)
+            copy_on_stream(b->ctx, row_inv, b->d_row_inv.ptr + I.panel_off, (size_t)I.n * sizeof(double),
+                           hipMemcpyDeviceToHost);
+    });
+}
+
 sbh_status sbh_batch_set_guesses(sbh_batch* b, const sbh_c64* X0) {
     return guard_on(ctx_of(b), [&] {
         check_problem(b);

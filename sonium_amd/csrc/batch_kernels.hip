@@ -233,15 +233,17 @@ __global__ __launch_bounds__(BLOCK) void g64_gemm_kernel(const int2* __restrict_
 }
 
 // Row-scaled FP16/BF16 copy of every instance matrix: one block per global row.
+constexpr int CONV_BLOCK = 1024;
+
 template <int FMT>
-__global__ __launch_bounds__(BLOCK) void glp_convert_kernel(const int* __restrict__ row_inst,
+__global__ __launch_bounds__(CONV_BLOCK) void glp_convert_kernel(const int* __restrict__ row_inst,
                                                             const BatchInstance* __restrict__ inst,
                                                             const double2* __restrict__ A, u16* __restrict__ Alp,
                                                             double* __restrict__ row_inv) {
     const int64_t g = blockIdx.x;
     const BatchInstance& I = inst[row_inst[g]];
     const int64_t r = g - I.panel_off;
-    convert_row<FMT>(A + I.a_off + r * I.n, I.n, I.ld, Alp + I.lp_off + r * 2 * I.ld, row_inv + g);
+    convert_row<FMT, CONV_BLOCK>(A + I.a_off + r * I.n, I.n, I.ld, Alp + I.lp_off + r * 2 * I.ld, row_inv + g);
 }
 
 // ---- ragged vector kernels ----
@@ -667,10 +669,10 @@ void batch_lp_convert(sbh_context* ctx, int format, int64_t rows, const int* row
                       const double2* A, uint16_t* Alp, double* row_inv) {
     if (rows == 0) return;
     if (format == 0)
-        hipLaunchKernelGGL(glp_convert_kernel<0>, dim3((unsigned)rows), dim3(BLOCK), 0, ctx->stream, row_inst, inst, A,
+        hipLaunchKernelGGL(glp_convert_kernel<0>, dim3((unsigned)rows), dim3(CONV_BLOCK), 0, ctx->stream, row_inst, inst, A,
                            Alp, row_inv);
     else
-        hipLaunchKernelGGL(glp_convert_kernel<1>, dim3((unsigned)rows), dim3(BLOCK), 0, ctx->stream, row_inst, inst, A,
+        hipLaunchKernelGGL(glp_convert_kernel<1>, dim3((unsigned)rows), dim3(CONV_BLOCK), 0, ctx->stream, row_inst, inst, A,
                            Alp, row_inv);
     SBH_HIP(hipGetLastError());
 }

@@ -50,36 +50,89 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// One block of BLOCK threads converts one row: row maximum, power-of-two
-// scale, rounding of the scaled FP64 values through FP32 to the format, zero
-// padding up to ld complex entries.
-template <int FMT>
-__device__ __forceinline__ void convert_row(const double2* __restrict__ row, int64_t n, int64_t ld,
-                                            u16* __restrict__ o, double* __restrict__ inv_out) {
-    __shared__ double sh[BLOCK / 64];
-    double m = 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += BLOCK) {
-        double2 v = row[j];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
-    }
+// One block of NT threads converts one row: row maximum, power-of-two scale,
+// then every entry as {re, im} halves packed in one 32-bit word (row padded
+// with zeros to ld entries).  A row of at most NT * CONV_HELD entries is read
+// once and held in registers between the maximum and the conversion (thread t
+// holds entries t, t + NT, ...); a longer row is read twice, CONV_LOADS loads
+// in flight per thread per pass.
+constexpr int CONV_HELD = 12;
+constexpr int CONV_LOADS = 8;
+
+template <int NT>
+__device__ __forceinline__ double block_max(double m) {
+    __shared__ double sh[NT / 64];
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    m = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) m = fmax(m, sh[w]);
+    return m;
+}
+
+template <int FMT>
+__device__ __forceinline__ unsigned pack_scaled(double2 v, double scale) {
+    const unsigned re = Half<FMT>::from_float((float)(v.x * scale));
+    const unsigned im = Half<FMT>::from_float((float)(v.y * scale));
+    return re | (im << 16);
+}
+
+template <int FMT>
+__device__ __forceinline__ double row_scale(double m, double* __restrict__ inv_out) {
     int e = 0;
     if (m > 0.0 && isfinite(m)) e = ilogb(m);
-    const double scale = ldexp(1.0, Half<FMT>::TOP - e);
     if (threadIdx.x == 0) *inv_out = ldexp(1.0, e - Half<FMT>::TOP);
-    for (int64_t j = threadIdx.x; j < ld; j += BLOCK) {
-        u16 re = 0, im = 0;
-        if (j < n) {
-            double2 v = row[j];
-            re = Half<FMT>::from_float((float)(v.x * scale));
-            im = Half<FMT>::from_float((float)(v.y * scale));
+    return ldexp(1.0, Half<FMT>::TOP - e);
+}
+
+template <int FMT, int NT>
+__device__ __forceinline__ void convert_row(const double2* __restrict__ row, int64_t n, int64_t ld,
+                                            u16* __restrict__ o, double* __restrict__ inv_out) {
+    unsigned* __restrict__ o32 = reinterpret_cast<unsigned*>(o);
+    const int t = threadIdx.x;
+    if (n <= (int64_t)NT * CONV_HELD) {
+        double2 v[CONV_HELD];
+        double m = 0.0;
+#pragma unroll
+        for (int u = 0; u < CONV_HELD; ++u) {
+            const int64_t j = t + (int64_t)NT * u;
+            v[u] = j < n ? row[j] : make_double2(0.0, 0.0);
         }
-        o[2 * j] = re;
-        o[2 * j + 1] = im;
+#pragma unroll
+        for (int u = 0; u < CONV_HELD; ++u) m = fmax(m, fmax(fabs(v[u].x), fabs(v[u].y)));
+        const double scale = row_scale<FMT>(block_max<NT>(m), inv_out);
+#pragma unroll
+        for (int u = 0; u < CONV_HELD; ++u) {
+            const int64_t j = t + (int64_t)NT * u;
+            if (j < n) o32[j] = pack_scaled<FMT>(v[u], scale);
+        }
+        for (int64_t j = n + t; j < ld; j += NT) o32[j] = 0u;
+        return;
     }
+    double m = 0.0;
+    int64_t j0 = t;
+    for (; j0 + (int64_t)NT * (CONV_LOADS - 1) < n; j0 += (int64_t)NT * CONV_LOADS) {
+        double2 v[CONV_LOADS];
+#pragma unroll
+        for (int u = 0; u < CONV_LOADS; ++u) v[u] = row[j0 + (int64_t)NT * u];
+#pragma unroll
+        for (int u = 0; u < CONV_LOADS; ++u) m = fmax(m, fmax(fabs(v[u].x), fabs(v[u].y)));
+    }
+    for (; j0 < n; j0 += NT) {
+        const double2 v = row[j0];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    const double scale = row_scale<FMT>(block_max<NT>(m), inv_out);
+    j0 = t;
+    for (; j0 + (int64_t)NT * (CONV_LOADS - 1) < n; j0 += (int64_t)NT * CONV_LOADS) {
+        double2 v[CONV_LOADS];
+#pragma unroll
+        for (int u = 0; u < CONV_LOADS; ++u) v[u] = row[j0 + (int64_t)NT * u];
+#pragma unroll
+        for (int u = 0; u < CONV_LOADS; ++u) o32[j0 + (int64_t)NT * u] = pack_scaled<FMT>(v[u], scale);
+    }
+    for (; j0 < ld; j0 += NT) o32[j0] = j0 < n ? pack_scaled<FMT>(row[j0], scale) : 0u;
 }
 
 // v_mfma_f32_32x32x16_{f16,bf16}: A 32x16 (lane l: row l%32, k 8*(l/32)..+7),

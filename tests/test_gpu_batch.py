@@ -95,6 +95,45 @@ def test_batch_fp64_gmres(ctx, small_batch):
 
 
 @pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_batch_lowp_copy_is_bit_exact(ctx, small_batch, fmt):
+    """The batch conversion (one row per workgroup) equals the oracle's
+    row-scaled storage bit for bit, for every instance size."""
+    from oracle import lowp as olp
+
+    b = small_batch["batch"]
+    b.convert(fmt)
+    for i in range(len(small_batch["specs"])):
+        halves, inv, ld = b.lowp_copy(i)
+        ref_h, ref_inv = olp.row_scaled_halves(b.matrix(i), fmt, ld)
+        assert np.array_equal(inv, ref_inv), f"instance {i}: row scales differ"
+        assert np.array_equal(halves, ref_h), f"instance {i}: {np.count_nonzero(halves != ref_h)} halves differ"
+
+
+def test_batch_lowp_copy_long_rows(ctx):
+    """Rows longer than a conversion block holds in registers (12800 panels)
+    take the two-pass conversion; sampled rows equal the oracle bit for bit."""
+    from oracle import lowp as olp
+    from sonium_amd.mesh import fibonacci_sphere
+
+    tm = sa.TbemMesh(ctx, fibonacci_sphere(1.0, 65, 100))
+    b = sa.Batch(ctx, [tm])
+    try:
+        b.set_problem([3.0], [np.array([[0.0, 0.0, 1.0]])])
+        b.assemble()
+        b.convert("fp16")
+        halves, inv, ld = b.lowp_copy(0)
+        A = b.matrix(0)
+        rows = np.unique(np.concatenate([[0, len(A) - 1], np.random.Generator(np.random.PCG64(3)).integers(
+            0, len(A), 256)]))
+        ref_h, ref_inv = olp.row_scaled_halves(A[rows], "fp16", ld)
+        assert np.array_equal(inv[rows], ref_inv)
+        assert np.array_equal(halves[rows], ref_h)
+    finally:
+        b.close()
+        tm.close()
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
 def test_batch_refinement(ctx, small_batch, fmt):
     b = small_batch["batch"]
     b.convert(fmt)
