@@ -40,6 +40,7 @@ struct sbh_batch {
     sbh::DeviceArray<uint64_t> d_leaves;
     sbh::DeviceArray<int64_t> d_row_ptr;
     sbh::DeviceArray<int> d_row_inst;
+    std::vector<int64_t> pair_begin;  // per instance, and the total at the end
     // problem
     bool has_problem = false;
     int C = 0;
@@ -53,6 +54,7 @@ struct sbh_batch {
     sbh::DeviceArray<sbh::FreqFamily> d_families;
     std::vector<sbh::FreqFamily> families;
     int64_t family_max_n = 0;
+    int64_t family_max_pairs = 0;
     sbh::DeviceArray<unsigned> d_counter;
     sbh::DeviceArray<int2> d_tiles;
     sbh::DeviceArray<double2> d_A;
@@ -164,7 +166,7 @@ void build_families(sbh_batch* b) {
         if (v.empty()) mesh_order.push_back(b->meshes[i]);
         v.push_back(i);
     }
-    int64_t max_n = 0;
+    int64_t max_n = 0, max_pairs = 0;
     for (const sbh_mesh* m : mesh_order) {
         std::vector<int> ids = by_mesh[m];
         std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->inst[x].k < b->inst[y].k; });
@@ -189,6 +191,8 @@ void build_families(sbh_batch* b) {
                 const BatchInstance& first = b->inst[ids[s]];
                 fam.panel_off = first.panel_off;
                 fam.n = first.n;
+                fam.pair_begin = b->pair_begin[ids[s]];
+                fam.npairs = b->pair_begin[ids[s] + 1] - fam.pair_begin;
                 fam.k0 = first.k;
                 fam.dk = b->inst[ids[s + 1]].k - first.k;
                 fam.count = (int32_t)(e - s);
@@ -199,11 +203,13 @@ void build_families(sbh_batch* b) {
                 }
                 b->families.push_back(fam);
                 max_n = std::max<int64_t>(max_n, fam.n);
+                max_pairs = std::max<int64_t>(max_pairs, fam.npairs);
             }
             s = e;
         }
     }
     b->family_max_n = max_n;
+    b->family_max_pairs = max_pairs;
     if (!b->families.empty()) upload(b->ctx, b->d_families, b->families);
 }
 
@@ -345,6 +351,7 @@ void run_assembly(sbh_batch* b) {
     args.families = b->d_families.ptr;
     args.nfamilies = (int)b->families.size();
     args.family_max_n = b->family_max_n;
+    args.family_max_pairs = b->family_max_pairs;
     args.inst = b->d_inst.ptr;
     args.items = b->d_items.ptr;
     args.nitems = b->nitems;
@@ -423,6 +430,7 @@ sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_m
         std::vector<int64_t> row_ptr((size_t)panels + 1, 0);
         std::vector<int> row_inst((size_t)panels);
         b->inst.resize(b->B);
+        b->pair_begin.assign((size_t)b->B + 1, 0);
         int64_t off = 0;
         for (int i = 0; i < b->B; ++i) {
             const sbh_mesh* m = b->meshes[i];
@@ -430,6 +438,7 @@ sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_m
             copy_on_stream(b->ctx, b->d_src.ptr + off, m->d_src.ptr, m->n * sizeof(Source), hipMemcpyDeviceToDevice);
             copy_on_stream(b->ctx, b->d_geom.ptr + off, m->d_geom.ptr, m->n * sizeof(PanelGeom), hipMemcpyDeviceToDevice);
             const int64_t pair_off = (int64_t)pairs.size();
+            b->pair_begin[i] = pair_off;
             const int64_t leaf_off = (int64_t)leaves.size();
             for (NearPair p : rts[i].pairs) {
                 p.begin += leaf_off;
@@ -448,6 +457,7 @@ sbh_status sbh_batch_create(sbh_context* ctx, int64_t num_instances, const sbh_m
             rts[i] = RouteTable();
         }
         row_ptr[panels] = (int64_t)pairs.size();
+        b->pair_begin[b->B] = (int64_t)pairs.size();
         upload(b->ctx, b->d_pairs, pairs);
         upload(b->ctx, b->d_leaves, leaves);
         upload(b->ctx, b->d_row_ptr, row_ptr);

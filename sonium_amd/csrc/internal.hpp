@@ -211,6 +211,8 @@ constexpr int FAMILY_MAX = 16;
 struct FreqFamily {
     int64_t panel_off;  // the shared geometry (that of the first member)
     int64_t n;
+    int64_t pair_begin;  // the first member's near pairs (global pair index)
+    int64_t npairs;
     double k0, dk;
     int32_t count;
     int32_t pad;
@@ -476,6 +478,7 @@ struct BatchAssemblyArgs {
     const FreqFamily* families;
     int nfamilies;
     int64_t family_max_n;
+    int64_t family_max_pairs;
     const BatchInstance* inst;
     const int2* items;
     int nitems;

@@ -1152,6 +1152,122 @@ __global__ __launch_bounds__(256) void batch_far_family_kernel(const FreqFamily*
     }
 }
 
+// Near pairs of a frequency family: the subdivided-leaf quadrature of each
+// pair (regular.rs:164-281, the rule of near_entry) with the point geometry
+// formed once for all members.  Sixteen lanes take one pair (a wave four),
+// lanes over (leaf, point); per point the phase exp(i k0 r) and the step
+// exp(i dk r) come from two sincos evaluations and each member advances the
+// phase by one complex product.  Rigid panels need the H and E channels
+// only; the 16-lane sums use row DPP, and lane f writes member f's entry.
+constexpr int NEAR_FAMILY_LANES = 16;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_read(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+// Sum over the 16 lanes of a DPP row; every lane of the row gets a total.
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_read<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_read<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_read<0x141>(v);  // row_half_mirror: lane i <- 7 - i within each half row
+    v += dpp_read<0x140>(v);  // row_mirror: lane i <- 15 - i
+    return v;
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void batch_near_family_kernel(const FreqFamily* __restrict__ families,
+                                                                const BatchInstance* __restrict__ inst,
+                                                                const NearPair* __restrict__ pairs,
+                                                                const uint64_t* __restrict__ leaves,
+                                                                const PanelGeom* __restrict__ geom,
+                                                                const Source* __restrict__ sources,
+                                                                double2* __restrict__ A) {
+#pragma clang fp contract(fast)
+    const FreqFamily& fam = families[blockIdx.y];
+    const int sub = threadIdx.x & (NEAR_FAMILY_LANES - 1);
+    const int count = fam.count;
+    const double k0 = fam.k0, dk = fam.dk;
+    const int64_t n = fam.n;
+    const int64_t step = (int64_t)gridDim.x * (256 / NEAR_FAMILY_LANES);
+    for (int64_t p = (int64_t)blockIdx.x * (256 / NEAR_FAMILY_LANES) + (threadIdx.x / NEAR_FAMILY_LANES);
+         p < fam.npairs; p += step) {
+        const NearPair np = pairs[fam.pair_begin + p];
+        const PanelGeom G = geom[fam.panel_off + np.j];
+        const Source S = sources[fam.panel_off + np.i];
+        const double nxny = S.n[0] * G.nf[0] + S.n[1] * G.nf[1] + S.n[2] * G.nf[2];
+        double hr[F], hi[F], er[F], ei[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) hr[f] = hi[f] = er[f] = ei[f] = 0.0;
+        const int total = np.count * 13;
+        for (int t = sub; t < total; t += NEAR_FAMILY_LANES) {
+            const int leaf = t / 13;
+            const int q = t - leaf * 13;
+            double xi[3], et[3];
+            decode_leaf(leaves[np.begin + leaf], xi, et);
+            const double cs = kTrS[q], ec = kTrT[q];
+            const double l0 = 1.0 - cs - ec;
+            const double xio = xi[0] * l0 + xi[1] * cs + xi[2] * ec;
+            const double eto = et[0] * l0 + et[1] * cs + et[2] * ec;
+            const double det = fabs((xi[1] - xi[0]) * (et[2] - et[0]) - (xi[2] - xi[0]) * (et[1] - et[0]));
+            const double wga = ((kTrW[q] * 0.5) * det) * G.jac;
+            double pos[3];
+            shape_pos(G.c, xio, eto, pos);
+            const double d0 = pos[0] - S.c[0], d1 = pos[1] - S.c[1], d2 = pos[2] - S.c[2];
+            const double dsq = d0 * d0 + d1 * d1 + d2 * d2;
+            if (!(dsq > 1e-30)) continue;  // the point evaluation skips |d| <= 1e-15
+            const double inv = rsqrt_nr(dsq);
+            const double dis = dsq * inv;
+            const double u0 = d0 * inv, u1 = d1 * inv, u2 = d2 * inv;
+            const double re2 = (wga * (1.0 / FOUR_PI)) * inv;
+            const double pn = u0 * G.nf[0] + u1 * G.nf[1] + u2 * G.nf[2];   // r^.n_y
+            const double qn = -(u0 * S.n[0] + u1 * S.n[1] + u2 * S.n[2]);  // -r^.n_x
+            const double rq = pn * qn;
+            const double inv2 = inv * inv;
+            const double ef0 = (3.0 * inv2) * rq + nxny * inv2;
+            const double ef1 = -inv * (3.0 * rq + nxny);
+            double phr, phi, str, sti;
+            sincos_red(k0 * dis, &phi, &phr);
+            sincos_red(dk * dis, &sti, &str);
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                if (f < count) {
+                    const double kf = fam.kf[f];
+                    const double zgr = phr * re2, zgi = phi * re2;
+                    const double zbr = -zgr * inv - zgi * kf;
+                    const double zbi = zgr * kf - zgi * inv;
+                    hr[f] += zbr * pn;
+                    hi[f] += zbi * pn;
+                    const double efr = ef0 - (kf * kf) * rq;
+                    const double efi = kf * ef1;
+                    er[f] += zgr * efr - zgi * efi;
+                    ei[f] += zgr * efi + zgi * efr;
+                    const double nr = phr * str - phi * sti;
+                    phi = phr * sti + phi * str;
+                    phr = nr;
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            if (f < count) {
+                const double h_r = row16_sum(hr[f]), h_i = row16_sum(hi[f]);
+                const double e_r = row16_sum(er[f]), e_i = row16_sum(ei[f]);
+                if (sub == f) {
+                    const BatchInstance& I = inst[fam.member[f]];
+                    const double tau = I.tau;
+                    A[I.a_off + (int64_t)np.i * n + np.j] =
+                        make_double2(h_r * tau + (e_r * I.beta[0] - e_i * I.beta[1]),
+                                     h_i * tau + (e_r * I.beta[1] + e_i * I.beta[0]));
+                }
+            }
+        }
+    }
+}
+
 // Batch of rigid instances, two persistent launches over the same items
 // (instance, first row); the host orders items by decreasing instance size
 // and each workgroup takes items from a counter until none are left.
@@ -1206,8 +1322,9 @@ __global__ __launch_bounds__(256) void batch_assemble_kernel(BatchAssemblyArgs a
             const PanelGeom* geom = args.geom + I.panel_off;
             const PanelBc bc = rigid_bc();
             const double2 iwr = make_double2(0.0, 0.0);  // no admittance on rigid panels
+            // members of a frequency family got their near entries from batch_near_family_kernel
             const int64_t pb = args.row_ptr[I.panel_off + r0];
-            const int64_t pe = args.row_ptr[I.panel_off + r1];
+            const int64_t pe = I.in_family ? pb : args.row_ptr[I.panel_off + r1];
             for (int64_t p = pb + wave; p < pe; p += 4) {
                 const NearPair np = args.pairs[p];
                 const PanelGeom G = geom[np.j];
@@ -1730,6 +1847,14 @@ void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int 
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);
     ctx->begin_timed(SBH_KERNEL_ASSEMBLE_NEAR, &ev);
+    if (args.nfamilies > 0 && args.family_max_pairs > 0) {
+        const int64_t per_block = 256 / NEAR_FAMILY_LANES;
+        const int64_t nb = std::min<int64_t>((args.family_max_pairs + per_block - 1) / per_block, 16384);
+        hipLaunchKernelGGL(batch_near_family_kernel<FAMILY_MAX>, dim3((unsigned)nb, (unsigned)args.nfamilies),
+                           dim3(256), 0, ctx->stream, args.families, args.inst, args.pairs, args.leaves, args.geom,
+                           args.src, args.A);
+        SBH_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(batch_assemble_kernel<1>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
