@@ -55,6 +55,7 @@ struct sbh_batch {
     std::vector<sbh::FreqFamily> families;
     int64_t family_max_n = 0;
     int64_t family_max_pairs = 0;
+    int family_class_end[sbh::FAMILY_CLASSES] = {0, 0, 0, 0};
     sbh::DeviceArray<unsigned> d_counter;
     sbh::DeviceArray<int2> d_tiles;
     sbh::DeviceArray<double2> d_A;
@@ -208,6 +209,12 @@ void build_families(sbh_batch* b) {
             s = e;
         }
     }
+    std::stable_sort(b->families.begin(), b->families.end(), [](const FreqFamily& x, const FreqFamily& y) {
+        return family_class(x.count) < family_class(y.count);
+    });
+    for (int c = 0; c < FAMILY_CLASSES; ++c) b->family_class_end[c] = 0;
+    for (const FreqFamily& f : b->families) ++b->family_class_end[family_class(f.count)];
+    for (int c = 1; c < FAMILY_CLASSES; ++c) b->family_class_end[c] += b->family_class_end[c - 1];
     b->family_max_n = max_n;
     b->family_max_pairs = max_pairs;
     if (!b->families.empty()) upload(b->ctx, b->d_families, b->families);
@@ -352,6 +359,7 @@ void run_assembly(sbh_batch* b) {
     args.nfamilies = (int)b->families.size();
     args.family_max_n = b->family_max_n;
     args.family_max_pairs = b->family_max_pairs;
+    for (int c = 0; c < FAMILY_CLASSES; ++c) args.family_class_end[c] = b->family_class_end[c];
     args.inst = b->d_inst.ptr;
     args.items = b->d_items.ptr;
     args.nitems = b->nitems;

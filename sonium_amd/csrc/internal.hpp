@@ -208,6 +208,10 @@ struct BatchInstance {
 // entries of all of them come from one pass over the pair geometry, with the
 // phase exp(i k_f r) advanced from f to f + 1 by one complex product.
 constexpr int FAMILY_MAX = 16;
+// Family kernels are compiled for 16, 8, 4 and 2 members; a family takes the
+// smallest that holds it (class 0..3) and families are stored by class.
+constexpr int FAMILY_CLASSES = 4;
+inline int family_class(int count) { return count > 8 ? 0 : count > 4 ? 1 : count > 2 ? 2 : 3; }
 struct FreqFamily {
     int64_t panel_off;  // the shared geometry (that of the first member)
     int64_t n;
@@ -479,6 +483,7 @@ struct BatchAssemblyArgs {
     int nfamilies;
     int64_t family_max_n;
     int64_t family_max_pairs;
+    int family_class_end[FAMILY_CLASSES];  // families [end[c-1], end[c]) are of class c
     const BatchInstance* inst;
     const int2* items;
     int nitems;

@@ -1117,26 +1117,27 @@ __global__ __launch_bounds__(256) void batch_far_family_kernel(const FreqFamily*
             // ef = (3/r^2 - k^2) rq + n.n/r^2  -  i k (3 rq + n.n)/r
             const double ef0 = (3.0 * inv2) * rq + nxny * inv2;
             const double ef1 = -inv * (3.0 * rq + nxny);
+            // member f, phase p = exp(i k_f r): H += p re2 pn (-1/r + i k_f),
+            // E += p re2 (ef0 - k_f^2 rq + i k_f ef1); the k_f-free factors once here
+            const double wb = re2 * pn, wa = -wb * inv;
+            const double E0 = re2 * ef0, R = re2 * rq, E1 = re2 * ef1;
             double phr, phi, str, sti;
             sincos_red(k0 * dis, &phi, &phr);
             sincos_red(dk * dis, &sti, &str);
+            // all F members, also past count (kf = 0 there; those sums are dropped)
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                if (f < count) {
-                    const double kf = fam.kf[f];
-                    const double zgr = phr * re2, zgi = phi * re2;
-                    const double zbr = -zgr * inv - zgi * kf;
-                    const double zbi = zgr * kf - zgi * inv;
-                    hr[f] = __builtin_fma(zbr, pn, hr[f]);
-                    hi[f] = __builtin_fma(zbi, pn, hi[f]);
-                    const double efr = __builtin_fma(-kf * kf, rq, ef0);
-                    const double efi = kf * ef1;
-                    er[f] += zgr * efr - zgi * efi;
-                    ei[f] += zgr * efi + zgi * efr;
-                    const double nr = phr * str - phi * sti;
-                    phi = phr * sti + phi * str;
-                    phr = nr;
-                }
+                const double kf = fam.kf[f];
+                const double u = __builtin_fma(-kf * kf, R, E0);
+                const double v = kf * E1;
+                const double t = kf * wb;
+                hr[f] = __builtin_fma(-phi, t, __builtin_fma(phr, wa, hr[f]));
+                hi[f] = __builtin_fma(phr, t, __builtin_fma(phi, wa, hi[f]));
+                er[f] = __builtin_fma(-phi, v, __builtin_fma(phr, u, er[f]));
+                ei[f] = __builtin_fma(phi, u, __builtin_fma(phr, v, ei[f]));
+                const double nr = __builtin_fma(phr, str, -phi * sti);
+                phi = __builtin_fma(phr, sti, phi * str);
+                phr = nr;
             }
         }
         if (active) {
@@ -1229,26 +1230,26 @@ __global__ __launch_bounds__(256) void batch_near_family_kernel(const FreqFamily
             const double inv2 = inv * inv;
             const double ef0 = (3.0 * inv2) * rq + nxny * inv2;
             const double ef1 = -inv * (3.0 * rq + nxny);
+            // the channel factors of batch_far_family_kernel
+            const double wb = re2 * pn, wa = -wb * inv;
+            const double E0 = re2 * ef0, R = re2 * rq, E1 = re2 * ef1;
             double phr, phi, str, sti;
             sincos_red(k0 * dis, &phi, &phr);
             sincos_red(dk * dis, &sti, &str);
+            // all F members, also past count (kf = 0 there; those sums are dropped)
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                if (f < count) {
-                    const double kf = fam.kf[f];
-                    const double zgr = phr * re2, zgi = phi * re2;
-                    const double zbr = -zgr * inv - zgi * kf;
-                    const double zbi = zgr * kf - zgi * inv;
-                    hr[f] += zbr * pn;
-                    hi[f] += zbi * pn;
-                    const double efr = ef0 - (kf * kf) * rq;
-                    const double efi = kf * ef1;
-                    er[f] += zgr * efr - zgi * efi;
-                    ei[f] += zgr * efi + zgi * efr;
-                    const double nr = phr * str - phi * sti;
-                    phi = phr * sti + phi * str;
-                    phr = nr;
-                }
+                const double kf = fam.kf[f];
+                const double u = E0 - (kf * kf) * R;
+                const double v = kf * E1;
+                const double t = kf * wb;
+                hr[f] += phr * wa - phi * t;
+                hi[f] += phi * wa + phr * t;
+                er[f] += phr * u - phi * v;
+                ei[f] += phr * v + phi * u;
+                const double nr = phr * str - phi * sti;
+                phi = phr * sti + phi * str;
+                phr = nr;
             }
         }
 #pragma unroll
@@ -1839,9 +1840,25 @@ void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int 
         const int64_t nb = (args.family_max_n + 255) / 256;
         const int64_t rb = (args.family_max_n + FAMILY_ROWS - 1) / FAMILY_ROWS;
         SBH_REQUIRE(nb < 65536 && rb < 65536 && args.nfamilies < 65536, "frequency-family grid out of range");
-        hipLaunchKernelGGL(batch_far_family_kernel<FAMILY_MAX>, dim3((unsigned)nb, (unsigned)rb, (unsigned)args.nfamilies),
-                           dim3(256), 0, ctx->stream, args.families, args.inst, args.far, args.src, args.A);
-        SBH_HIP(hipGetLastError());
+        for (int c = 0; c < FAMILY_CLASSES; ++c) {
+            const int fb = c ? args.family_class_end[c - 1] : 0, fe = args.family_class_end[c];
+            if (fe == fb) continue;
+            const dim3 grid((unsigned)nb, (unsigned)rb, (unsigned)(fe - fb));
+            const FreqFamily* fams = args.families + fb;
+            if (c == 0)
+                hipLaunchKernelGGL(batch_far_family_kernel<16>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.far, args.src, args.A);
+            else if (c == 1)
+                hipLaunchKernelGGL(batch_far_family_kernel<8>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.far, args.src, args.A);
+            else if (c == 2)
+                hipLaunchKernelGGL(batch_far_family_kernel<4>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.far, args.src, args.A);
+            else
+                hipLaunchKernelGGL(batch_far_family_kernel<2>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.far, args.src, args.A);
+            SBH_HIP(hipGetLastError());
+        }
     }
     hipLaunchKernelGGL(batch_assemble_kernel<0>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
     SBH_HIP(hipGetLastError());
@@ -1850,10 +1867,25 @@ void launch_batch_assembly(sbh_context* ctx, const BatchAssemblyArgs& args, int 
     if (args.nfamilies > 0 && args.family_max_pairs > 0) {
         const int64_t per_block = 256 / NEAR_FAMILY_LANES;
         const int64_t nb = std::min<int64_t>((args.family_max_pairs + per_block - 1) / per_block, 16384);
-        hipLaunchKernelGGL(batch_near_family_kernel<FAMILY_MAX>, dim3((unsigned)nb, (unsigned)args.nfamilies),
-                           dim3(256), 0, ctx->stream, args.families, args.inst, args.pairs, args.leaves, args.geom,
-                           args.src, args.A);
-        SBH_HIP(hipGetLastError());
+        for (int c = 0; c < FAMILY_CLASSES; ++c) {
+            const int fb = c ? args.family_class_end[c - 1] : 0, fe = args.family_class_end[c];
+            if (fe == fb) continue;
+            const dim3 grid((unsigned)nb, (unsigned)(fe - fb));
+            const FreqFamily* fams = args.families + fb;
+            if (c == 0)
+                hipLaunchKernelGGL(batch_near_family_kernel<16>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.pairs, args.leaves, args.geom, args.src, args.A);
+            else if (c == 1)
+                hipLaunchKernelGGL(batch_near_family_kernel<8>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.pairs, args.leaves, args.geom, args.src, args.A);
+            else if (c == 2)
+                hipLaunchKernelGGL(batch_near_family_kernel<4>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.pairs, args.leaves, args.geom, args.src, args.A);
+            else
+                hipLaunchKernelGGL(batch_near_family_kernel<2>, grid, dim3(256), 0, ctx->stream, fams, args.inst,
+                                   args.pairs, args.leaves, args.geom, args.src, args.A);
+            SBH_HIP(hipGetLastError());
+        }
     }
     hipLaunchKernelGGL(batch_assemble_kernel<1>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, args);
     SBH_HIP(hipGetLastError());
