@@ -649,7 +649,10 @@ def run_c5(args, ctx=None):
     stride = 4
     t0 = time.perf_counter()
     sb = sweep.SweepBatch(ctx, base, len(ks) // stride)
+    sb.ensure_store(len(ks))
     batch_setup_s = time.perf_counter() - t0
+    # the network's inference workspace for one whole group, allocated before the timed sweeps
+    net.predict(np.zeros((len(ks) // stride * n, wsm.RAW), dtype=np.float32))
     x_ref = np.stack([x_zero[float(k)] for k in ks])
     batched_modes = {
         # name: (guess of the first group, guess of later groups)

@@ -680,6 +680,28 @@ sbh_status sbh_batch_download_lowp(sbh_batch* batch, int64_t instance, uint16_t*
 sbh_status sbh_batch_set_guesses(sbh_batch* batch, const sbh_c64* X0);
 /* Every instance's solutions (what = 0) or right-hand sides (what = 1), packed. */
 sbh_status sbh_batch_download_all(sbh_batch* batch, int what, sbh_c64* out);
+/* Device-resident sweep data.  Pointer tables hold one device pointer per
+ * instance (of this batch's device), each addressing an n_b x num_rhs_b
+ * row-major block; a null entry stands for a zero block.
+ *   - sbh_batch_set_guesses_device: the initial guesses of the next
+ *     sbh_batch_solve with use_guess = 1, copied from x0[b];
+ *   - sbh_batch_store_solutions: each instance's current solution copied to
+ *     dst[b] (null entries skipped);
+ *   - sbh_batch_guess_gram: the minimal-residual data of m (1..6) candidate
+ *     guesses per column: candidates[a * B + b] is instance b's candidate a;
+ *     with P_a = A X_a (format as in sbh_batch_apply) it returns per column c
+ *     gram[(c m + a) m + a'] = P_a^H P_a', projections[c m + a] = P_a^H b_c and
+ *     rhs_norm2[c] = ||b_c||^2;
+ *   - sbh_batch_guess_combine: x0 = sum_a coefficients[c m + a] X_a per column
+ *     from the candidates of the last sbh_batch_guess_gram (same m), as the
+ *     next initial guesses.
+ * The previous-frequency guess of the reference's sweep
+ * (frequency_sweep.rs:718-730) is the case m = 1 with coefficient 1. */
+sbh_status sbh_batch_set_guesses_device(sbh_batch* batch, const sbh_c64* const* x0);
+sbh_status sbh_batch_store_solutions(sbh_batch* batch, sbh_c64* const* dst);
+sbh_status sbh_batch_guess_gram(sbh_batch* batch, int format, int m, const sbh_c64* const* candidates,
+                                sbh_c64* gram, sbh_c64* projections, double* rhs_norm2);
+sbh_status sbh_batch_guess_combine(sbh_batch* batch, int m, const sbh_c64* coefficients);
 /* n x num_rhs row-major blocks of one instance (host memory). */
 sbh_status sbh_batch_download_solution(sbh_batch* batch, int64_t instance, sbh_c64* x);
 sbh_status sbh_batch_download_rhs(sbh_batch* batch, int64_t instance, sbh_c64* rhs);
@@ -736,6 +758,12 @@ sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_m
                          double* losses);
 /* Predictions (M x 2 floats) for M raw inputs (M x 10 floats). */
 sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory);
+/* The network's initial guess x0 = u exp(i k d.c) for every column of every
+ * instance of a batch (panel centroids and outward normals, the instance's
+ * wavenumber, the column's direction), computed on the device and written
+ * to the device vector out (packed layout of sbh_batch_apply) or, with out
+ * NULL, as the batch's initial guesses; net and batch share one context. */
+sbh_status sbh_batch_learned_guesses(sbh_batch* batch, sbh_mlp* net, sbh_c64* out);
 
 /* ---------------------------------------------------------------------------
  * Mesh2HRTF / NumCalc projects (host only).

@@ -1396,6 +1396,53 @@ class Batch:
             raise ValueError("one n_i x num_rhs[i] block per instance")
         _check(_lib().sbh_batch_set_guesses(self._h, _cptr(flat)))
 
+    @staticmethod
+    def _pointer_table(ptrs):
+        return (ctypes.c_void_p * len(ptrs))(*[None if p is None else int(p) for p in ptrs])
+
+    def set_guesses_device(self, pointers) -> None:
+        """Initial guesses copied on the device: pointers[i] is a device
+        address of instance i's n_i x num_rhs[i] block (None: zero)."""
+        if len(pointers) != len(self.meshes):
+            raise ValueError("one pointer per instance")
+        _check(_lib().sbh_batch_set_guesses_device(self._h, self._pointer_table(pointers)))
+
+    def store_solutions(self, pointers) -> None:
+        """Each instance's solution block copied to the device address pointers[i] (None: skipped)."""
+        if len(pointers) != len(self.meshes):
+            raise ValueError("one pointer per instance")
+        _check(_lib().sbh_batch_store_solutions(self._h, self._pointer_table(pointers)))
+
+    def guess_gram(self, candidates, fmt: str = "fp64"):
+        """Minimal-residual data of device-resident candidate guesses.
+        candidates[a][i] is the device address of instance i's candidate a
+        (None: zero).  Returns (G, h, bb) per column: G[c] = P^H P (m x m),
+        h[c] = P^H b, bb[c] = ||b||^2 with P_a = A X_a."""
+        m = len(candidates)
+        if any(len(c) != len(self.meshes) for c in candidates):
+            raise ValueError("one pointer per instance in every candidate slot")
+        ncol = int(self.num_rhs.sum())
+        G = np.empty((ncol, m, m), dtype=np.complex128)
+        h = np.empty((ncol, m), dtype=np.complex128)
+        bb = np.empty(ncol)
+        table = self._pointer_table([p for slot in candidates for p in slot])
+        _check(_lib().sbh_batch_guess_gram(self._h, {"fp64": -1, "fp16": 0, "bf16": 1}[fmt], m, table, _cptr(G),
+                                           _cptr(h), _cptr(bb)))
+        return G, h, bb
+
+    def guess_combine(self, coefficients) -> None:
+        """Initial guesses sum_a coefficients[c, a] X_a per column, from the
+        candidates of the last guess_gram."""
+        cf = np.ascontiguousarray(coefficients, dtype=np.complex128)
+        if cf.ndim != 2 or cf.shape[0] != int(self.num_rhs.sum()):
+            raise ValueError("coefficients must be (columns, candidates)")
+        _check(_lib().sbh_batch_guess_combine(self._h, cf.shape[1], _cptr(cf)))
+
+    def learned_guesses(self, net, out: Optional[DeviceVector] = None) -> None:
+        """The network's guesses for every column, computed on the device, into
+        ``out`` (packed layout) or as the batch's initial guesses."""
+        _check(_lib().sbh_batch_learned_guesses(self._h, net._h, None if out is None else out.ptr))
+
     def solutions(self):
         """Every instance's solution block, in one download."""
         out = np.empty(sum(a * b for a, b in self._packed_shapes()), dtype=np.complex128)

@@ -247,6 +247,11 @@ SIGNATURES = {
     "sbh_batch_set_guess": (_int, [_vp, _i64, _vp]),
     "sbh_batch_apply": (_int, [_vp, _int, _vp, _vp]),
     "sbh_batch_set_guesses": (_int, [_vp, _vp]),
+    "sbh_batch_set_guesses_device": (_int, [_vp, _vp]),
+    "sbh_batch_store_solutions": (_int, [_vp, _vp]),
+    "sbh_batch_guess_gram": (_int, [_vp, _int, _int, _vp, _vp, _vp, _vp]),
+    "sbh_batch_guess_combine": (_int, [_vp, _int, _vp]),
+    "sbh_batch_learned_guesses": (_int, [_vp, _vp, _vp]),
     "sbh_batch_download_lowp": (_int, [_vp, _i64, _vp, _vp, ctypes.POINTER(_i64)]),
     "sbh_batch_download_all": (_int, [_vp, _int, _vp]),
     "sbh_batch_download_solution": (_int, [_vp, _i64, _vp]),

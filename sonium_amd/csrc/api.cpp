@@ -359,11 +359,13 @@ sbh_status sbh_admission_estimate(int64_t n, int64_t rhs, int64_t freq, int64_t 
         e.scratch_bytes = prod({n, 4 * C + 8}, "assembly scratch size");
         e.rhs_bytes = prod({rhs_bytes, freq}, "frequency right-hand-side size");
         e.host_bytes = sum({e.operator_bytes, e.rhs_bytes, e.scratch_bytes}, "peak host allocation");
-        // this library: system (matrix + one right-hand side), right-hand
-        // sides and solutions, Krylov basis (shared block basis for rhs > 1)
+        // this library: system (matrix, one right-hand side, row maxima of the
+        // assembly), right-hand sides and solutions, Krylov basis (shared
+        // block basis for rhs > 1)
         const int64_t basis_vectors = rhs == 1 ? sum({restart, 3}, "basis size")
                                                : prod({sum({restart, 2}, "basis size"), rhs}, "basis size");
-        e.device_bytes = sum({e.operator_bytes, prod({n, C}, "rhs"), prod({2, rhs_bytes}, "blocks"),
+        e.device_bytes = sum({e.operator_bytes, prod({n, C}, "rhs"), prod({n, (n + 255) / 256 + 1, 8}, "row maxima"),
+                              prod({2, rhs_bytes}, "blocks"),
                               prod({basis_vectors, n, C}, "Krylov basis"),
                               prod({6, rhs_bytes}, "block scratch")},
                              "peak device allocation");
@@ -689,8 +691,10 @@ sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** ou
         SBH_REQUIRE(num_dofs >= 1, "num_dofs must be positive");
         *out = nullptr;
         use_device(ctx);
-        const int64_t bytes = (int64_t)sizeof(double2) * (num_dofs * num_dofs + num_dofs);
         SBH_REQUIRE(num_dofs <= ((int64_t)1 << 31), "num_dofs too large");
+        const int64_t stride = (num_dofs + 255) / 256 + 1;
+        const int64_t bytes = (int64_t)sizeof(double2) * (num_dofs * num_dofs + num_dofs) +
+                              (int64_t)sizeof(double) * num_dofs * stride;
         admit_device(ctx, bytes, "a dense system", false);
         std::unique_ptr<sbh_system> s(new sbh_system());
         s->ctx = ctx;
@@ -698,6 +702,8 @@ sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** ou
         s->lda = num_dofs;
         s->d_A.allocate((size_t)num_dofs * (size_t)num_dofs);
         s->d_rhs.allocate((size_t)num_dofs);
+        s->d_rowmax.allocate((size_t)num_dofs * stride);
+        s->rowmax_stride = stride;
         SBH_HIP(hipMemsetAsync(s->d_rhs.ptr, 0, s->d_rhs.bytes(), ctx->stream));
         SBH_HIP(hipStreamSynchronize(ctx->stream));
         ctx->device_reserved += bytes;
@@ -722,7 +728,12 @@ int64_t sbh_system_num_dofs(const sbh_system* sys) { return sys ? sys->n : 0; }
 sbh_status sbh_system_device_pointers(sbh_system* sys, sbh_c64** matrix, int64_t* lda, sbh_c64** rhs) {
     return guard_on(ctx_of(sys), [&] {
         SBH_REQUIRE(sys, "system is null");
-        if (matrix) *matrix = reinterpret_cast<sbh_c64*>(sys->d_A.ptr);
+        if (matrix) {
+            // the caller may now write the matrix behind the library's back
+            sys->matrix_exposed = true;
+            sys->rowmax_valid = false;
+            *matrix = reinterpret_cast<sbh_c64*>(sys->d_A.ptr);
+        }
         if (lda) *lda = sys->lda;
         if (rhs) *rhs = reinterpret_cast<sbh_c64*>(sys->d_rhs.ptr);
     });
@@ -1473,7 +1484,8 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         lp->d_colinv.allocate(64);
         lp->d_ypart.allocate((size_t)lowp_split_count() * lp->n * 128);
         SBH_HIP(hipMemsetAsync(lp->d_XT.ptr, 0, lp->d_XT.bytes(), sys->ctx->stream));
-        lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr);
+        lowp_convert(sys->ctx, sys->d_A.ptr, sys->lda, lp->n, lp->ld, format, lp->d_A.ptr, lp->d_row_inv.ptr,
+                     assembled_row_max(sys), sys->rowmax_stride);
         SBH_HIP(hipStreamSynchronize(sys->ctx->stream));
         sys->ctx->device_reserved += bytes;
         lp->admitted = bytes;
@@ -1487,7 +1499,7 @@ sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp) {
         SBH_REQUIRE(lp, "matrix is null");
         use_device(lp->ctx);
         lowp_convert(lp->ctx, lp->sys->d_A.ptr, lp->sys->lda, lp->n, lp->ld, lp->format, lp->d_A.ptr,
-                     lp->d_row_inv.ptr);
+                     lp->d_row_inv.ptr, assembled_row_max(lp->sys), lp->sys->rowmax_stride);
     });
 }
 

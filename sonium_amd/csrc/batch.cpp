@@ -76,6 +76,14 @@ struct sbh_batch {
     std::vector<sbh::VecChunk> chunk_host;
     // tuning
     sbh_batch_tuning tuning{16, 4, 4096, 0};
+    // initial guesses from device-resident vectors: gathered candidates and
+    // their products (cand_m slots of L), per-column Gram data, pointer tables
+    sbh::DeviceArray<double2> d_cand, d_cprod, d_coef, d_G, d_h;
+    sbh::DeviceArray<double> d_bb;
+    sbh::DeviceArray<const double2*> d_ptrs;
+    sbh::DeviceArray<int> d_ones;
+    int cand_m = 0;
+    int cand_format = -2;
     int cu_count = 256;
     // solver workspace
     struct Solver {
@@ -799,6 +807,131 @@ sbh_status sbh_batch_set_guesses(sbh_batch* b, const sbh_c64* X0) {
         for (size_t i = 0; i < L; ++i)
             SBH_REQUIRE(std::isfinite(X0[i].re) && std::isfinite(X0[i].im), "initial guess is not finite");
         copy_on_stream(b->ctx, b->d_x.ptr, X0, L * sizeof(double2), hipMemcpyHostToDevice);
+    });
+}
+
+namespace {
+int64_t max_block(const sbh_batch* b) {
+    int64_t m = 0;
+    for (const BatchInstance& I : b->inst) m = std::max<int64_t>(m, I.n * I.ncol);
+    return m;
+}
+
+// A device pointer of the batch's device (or null where nulls are allowed).
+void require_device_pointer(const sbh_batch* b, const void* p, bool allow_null) {
+    if (!p) {
+        SBH_REQUIRE(allow_null, "a device pointer is null");
+        return;
+    }
+    hipPointerAttribute_t attr{};
+    const hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) (void)hipGetLastError();
+    SBH_REQUIRE(e == hipSuccess && attr.type == hipMemoryTypeDevice && attr.device == b->ctx->device,
+                "a pointer is not device memory of the batch's device");
+}
+
+// Uploads a table of device pointers (checked) into b->d_ptrs.
+void upload_pointer_table(sbh_batch* b, const void* const* p, size_t count, bool allow_null) {
+    for (size_t i = 0; i < count; ++i) require_device_pointer(b, p[i], allow_null);
+    if (b->d_ptrs.count < count) b->d_ptrs.allocate(count);
+    copy_on_stream(b->ctx, b->d_ptrs.ptr, p, count * sizeof(void*), hipMemcpyHostToDevice);
+}
+}  // namespace
+
+sbh_status sbh_batch_set_guesses_device(sbh_batch* b, const sbh_c64* const* x0) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(x0, "x0 is null");
+        use_device(b->ctx);
+        upload_pointer_table(b, reinterpret_cast<const void* const*>(x0), (size_t)b->B, true);
+        batch_gather(b->ctx, b->B, b->L, max_block(b), 1, b->d_inst.ptr, b->d_ptrs.ptr, b->d_x.ptr);
+    });
+}
+
+sbh_status sbh_batch_store_solutions(sbh_batch* b, sbh_c64* const* dst) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(dst, "dst is null");
+        use_device(b->ctx);
+        upload_pointer_table(b, reinterpret_cast<const void* const*>(dst), (size_t)b->B, true);
+        batch_scatter(b->ctx, b->B, max_block(b), b->d_inst.ptr, b->d_x.ptr,
+                      const_cast<double2* const*>(reinterpret_cast<const double2* const*>(b->d_ptrs.ptr)));
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
+    });
+}
+
+sbh_status sbh_batch_guess_gram(sbh_batch* b, int format, int m, const sbh_c64* const* candidates, sbh_c64* gram,
+                                sbh_c64* projections, double* rhs_norm2) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(candidates && gram && projections && rhs_norm2, "null argument");
+        SBH_REQUIRE(m >= 1 && m <= 6, "the number of candidates must be in 1..6");
+        SBH_REQUIRE(format == SBH_BATCH_FP64 || format == 0 || format == 1, "format must be FP64 (-1), FP16 or BF16");
+        SBH_REQUIRE(format < 0 || (b->d_Alp.count == (size_t)b->lp_elems && b->lp_format == format),
+                    "the batch has no copy in that format (sbh_batch_convert)");
+        use_device(b->ctx);
+        sbh_context* ctx = b->ctx;
+        const size_t L = (size_t)b->L, C = (size_t)b->C;
+        upload_pointer_table(b, reinterpret_cast<const void* const*>(candidates), (size_t)m * b->B, true);
+        if (b->d_cand.count < (size_t)m * L) {
+            b->d_cand.allocate((size_t)m * L);
+            b->d_cprod.allocate((size_t)m * L);
+        }
+        if (b->d_ones.count != (size_t)b->B) {
+            b->d_ones.allocate((size_t)b->B);
+            fill_int(ctx, b->d_ones, 1);
+        }
+        if (b->d_G.count < C * m * m) b->d_G.allocate(C * m * m);
+        if (b->d_h.count < C * m) b->d_h.allocate(C * m);
+        if (b->d_bb.count < C) b->d_bb.allocate(C);
+        batch_gather(ctx, b->B, b->L, max_block(b), m, b->d_inst.ptr, b->d_ptrs.ptr, b->d_cand.ptr);
+        for (int a = 0; a < m; ++a)
+            batch_product(ctx, format, b->max_cols, b->ntiles, b->d_tiles.ptr, b->d_inst.ptr, b->d_ones.ptr,
+                          b->d_A.ptr, format < 0 ? nullptr : b->d_Alp.ptr, format < 0 ? nullptr : b->d_row_inv.ptr,
+                          b->d_cand.ptr + a * L, b->d_cprod.ptr + a * L);
+        batch_gram(ctx, m, (int)C, b->d_inst.ptr, b->d_col_inst.ptr, b->d_col_q.ptr, b->d_cprod.ptr, b->L,
+                   b->d_rhs.ptr, b->d_G.ptr, b->d_h.ptr, b->d_bb.ptr);
+        SBH_HIP(hipMemcpyAsync(gram, b->d_G.ptr, C * m * m * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipMemcpyAsync(projections, b->d_h.ptr, C * m * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipMemcpyAsync(rhs_norm2, b->d_bb.ptr, C * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SBH_HIP(hipStreamSynchronize(ctx->stream));
+        b->cand_m = m;
+        b->cand_format = format;
+    });
+}
+
+sbh_status sbh_batch_guess_combine(sbh_batch* b, int m, const sbh_c64* coefficients) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(coefficients, "coefficients is null");
+        SBH_REQUIRE(m >= 1 && m == b->cand_m && b->d_cand.count >= (size_t)m * b->L,
+                    "sbh_batch_guess_gram with this number of candidates has not been called");
+        const size_t C = (size_t)b->C;
+        for (size_t i = 0; i < C * m; ++i)
+            SBH_REQUIRE(std::isfinite(coefficients[i].re) && std::isfinite(coefficients[i].im),
+                        "a coefficient is not finite");
+        use_device(b->ctx);
+        if (b->d_coef.count < C * m) b->d_coef.allocate(C * m);
+        copy_on_stream(b->ctx, b->d_coef.ptr, coefficients, C * m * sizeof(double2), hipMemcpyHostToDevice);
+        batch_combine(b->ctx, b->B, max_block(b), b->d_inst.ptr, b->d_cand.ptr, b->L, m, b->d_coef.ptr, b->d_x.ptr);
+    });
+}
+
+sbh_status sbh_batch_learned_guesses(sbh_batch* b, sbh_mlp* net, sbh_c64* out) {
+    return guard_on(ctx_of(b), [&] {
+        check_problem(b);
+        SBH_REQUIRE(net, "net is null");
+        SBH_REQUIRE(mlp_context(net) == b->ctx, "the network belongs to another context");
+        SBH_REQUIRE(b->L <= (1 << 24), "the batch has more than 2^24 samples");
+        if (out) require_device_pointer(b, out, false);
+        use_device(b->ctx);
+        const int64_t mb = max_block(b);
+        float* raw = mlp_raw_workspace(net, b->L);
+        batch_learned_raw(b->ctx, b->B, mb, b->d_inst.ptr, b->d_src.ptr, b->d_dirs.ptr, raw);
+        const float* Y = mlp_forward_device(net, raw, b->L);
+        batch_learned_phase(b->ctx, b->B, mb, b->d_inst.ptr, b->d_src.ptr, b->d_dirs.ptr, Y,
+                            out ? reinterpret_cast<double2*>(out) : b->d_x.ptr);
+        SBH_HIP(hipStreamSynchronize(b->ctx->stream));
     });
 }
 

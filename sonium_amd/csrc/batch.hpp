@@ -58,4 +58,28 @@ void ragged_scale(sbh_context* ctx, const RaggedLayout& lay, const int* flag, co
 void ragged_fill(sbh_context* ctx, double2* p, int64_t n);
 void ragged_stamp(sbh_context* ctx, unsigned long long* out);
 
+// Initial guesses from device-resident vectors.  max_block is the largest
+// n * ncol of any instance; src/dst tables hold one device pointer per
+// instance (per layer for the gather).
+void batch_gather(sbh_context* ctx, int B, int64_t L, int64_t max_block, int layers, const BatchInstance* inst,
+                  const double2* const* src, double2* dst);
+void batch_scatter(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const double2* x,
+                   double2* const* dst);
+void batch_gram(sbh_context* ctx, int m, int C, const BatchInstance* inst, const int* col_inst, const int* col_q,
+                const double2* P, int64_t L, const double2* rhs, double2* G, double2* h, double* bb);
+void batch_combine(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const double2* S, int64_t L,
+                   int m, const double2* coef, double2* x);
+void batch_learned_raw(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const Source* src,
+                       const double* dirs, float* raw);
+void batch_learned_phase(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const Source* src,
+                         const double* dirs, const float* Y, double2* x);
+
+// The warm-start network (mlp_kernels.hip): its context, a raw-input
+// workspace of at least M samples (rows [M, padded M) zeroed), and the
+// forward pass over M device samples whose M x 2 outputs stay valid until the
+// network's next call.
+sbh_context* mlp_context(const sbh_mlp* net);
+float* mlp_raw_workspace(sbh_mlp* net, int64_t M);
+const float* mlp_forward_device(sbh_mlp* net, const float* raw, int64_t M);
+
 }  // namespace sbh

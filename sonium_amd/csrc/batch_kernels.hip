@@ -659,6 +659,158 @@ __global__ void rfill_kernel(double2* __restrict__ p, int64_t n) {
 
 __global__ void rstamp_kernel(unsigned long long* __restrict__ out) { *out = wall_clock64(); }
 
+// ---- initial guesses from device-resident vectors ----
+// One grid row (blockIdx.y) per instance, one grid layer (blockIdx.z) per
+// candidate slot; element e < n * ncol of an instance's block is at
+// vec_off + e of a packed vector.
+
+// dst[z * L + vec_off + e] = src[z * B + inst][e]; a null source gives zeros.
+__global__ void bgather_kernel(const BatchInstance* __restrict__ inst, int B, int64_t L,
+                               const double2* const* __restrict__ src, double2* __restrict__ dst) {
+    const BatchInstance& I = inst[blockIdx.y];
+    const int64_t cnt = I.n * I.ncol;
+    const double2* s = src[(int64_t)blockIdx.z * B + blockIdx.y];
+    double2* d = dst + (int64_t)blockIdx.z * L + I.vec_off;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * BLOCK)
+        d[e] = s ? s[e] : make_double2(0.0, 0.0);
+}
+
+// dst[inst][e] = x[vec_off + e] for every instance with a destination.
+__global__ void bscatter_kernel(const BatchInstance* __restrict__ inst, const double2* __restrict__ x,
+                                double2* const* __restrict__ dst) {
+    const BatchInstance& I = inst[blockIdx.y];
+    double2* d = dst[blockIdx.y];
+    if (!d) return;
+    const int64_t cnt = I.n * I.ncol;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * BLOCK)
+        d[e] = x[I.vec_off + e];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Per column c (one workgroup): the Gram matrix G[a][b] = sum_r conj(P_a) P_b
+// of the M candidate products, h[a] = sum_r conj(P_a) rhs and |rhs|^2, over
+// the column's rows.  Fixed order: strided per-thread sums, a butterfly per
+// wave, then the 4 wave sums in order.
+template <int M>
+__global__ __launch_bounds__(BLOCK) void bgram_kernel(const BatchInstance* __restrict__ inst,
+                                                      const int* __restrict__ col_inst, const int* __restrict__ col_q,
+                                                      const double2* __restrict__ P, int64_t L,
+                                                      const double2* __restrict__ rhs, double2* __restrict__ G,
+                                                      double2* __restrict__ h, double* __restrict__ bb) {
+    constexpr int NP = M * (M + 1) / 2;
+    constexpr int NV = 2 * NP + 2 * M + 1;
+    __shared__ double sh[BLOCK / 64][NV];
+    const int c = blockIdx.x;
+    const BatchInstance& I = inst[col_inst[c]];
+    const int q = col_q[c];
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    for (int64_t r = threadIdx.x; r < I.n; r += BLOCK) {
+        const int64_t t = I.vec_off + r * I.ncol + q;
+        double2 p[M];
+#pragma unroll
+        for (int a = 0; a < M; ++a) p[a] = P[(int64_t)a * L + t];
+        const double2 bv = rhs[t];
+        int o = 0;
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+            for (int b = a; b < M; ++b) {
+                const double2 g = cmulc(p[a], p[b]);
+                v[o++] += g.x;
+                v[o++] += g.y;
+            }
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            const double2 g = cmulc(p[a], bv);
+            v[o++] += g.x;
+            v[o++] += g.y;
+        }
+        v[o] += bv.x * bv.x + bv.y * bv.y;
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double s = wave_sum(v[i]);
+        if ((threadIdx.x & 63) == 0) sh[w][i] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NV; i += BLOCK) {
+        double s = sh[0][i];
+        for (int j = 1; j < BLOCK / 64; ++j) s += sh[j][i];
+        sh[0][i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int a = 0; a < M; ++a)
+            for (int b = a; b < M; ++b) {
+                const double re = sh[0][o++], im = sh[0][o++];
+                G[((int64_t)c * M + a) * M + b] = make_double2(re, im);
+                G[((int64_t)c * M + b) * M + a] = make_double2(re, -im);
+            }
+        for (int a = 0; a < M; ++a) {
+            const double re = sh[0][o++], im = sh[0][o++];
+            h[(int64_t)c * M + a] = make_double2(re, im);
+        }
+        bb[c] = sh[0][o];
+    }
+}
+
+// x[vec_off + e] = sum_a coef[col * m + a] S[a * L + vec_off + e], col = col_off + e % ncol.
+__global__ void bcombine_kernel(const BatchInstance* __restrict__ inst, const double2* __restrict__ S, int64_t L,
+                                int m, const double2* __restrict__ coef, double2* __restrict__ x) {
+    const BatchInstance& I = inst[blockIdx.y];
+    const int64_t cnt = I.n * I.ncol;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * BLOCK) {
+        const double2* cf = coef + (int64_t)(I.col_off + (int)(e % I.ncol)) * m;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int a = 0; a < m; ++a) acc = cadd(acc, cmul(cf[a], S[(int64_t)a * L + I.vec_off + e]));
+        x[I.vec_off + e] = acc;
+    }
+}
+
+// Raw network inputs of every (panel, column) sample, at the packed index:
+// {centroid, outward normal, incident direction, k} as FP32.
+__global__ void blearned_raw_kernel(const BatchInstance* __restrict__ inst, const Source* __restrict__ src,
+                                    const double* __restrict__ dirs, float* __restrict__ raw) {
+    const BatchInstance& I = inst[blockIdx.y];
+    const int64_t cnt = I.n * I.ncol;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * BLOCK) {
+        const Source s = src[I.panel_off + e / I.ncol];
+        const double* d = dirs + 3 * (I.dir_off + e % I.ncol);
+        float* o = raw + (I.vec_off + e) * 10;
+        for (int j = 0; j < 3; ++j) {
+            o[j] = (float)s.c[j];
+            o[3 + j] = (float)s.n[j];
+            o[6 + j] = (float)d[j];
+        }
+        o[9] = (float)I.k;
+    }
+}
+
+// Guess x0 = u exp(i k d.c) from the network's incident-phase-free density u.
+__global__ void blearned_phase_kernel(const BatchInstance* __restrict__ inst, const Source* __restrict__ src,
+                                      const double* __restrict__ dirs, const float* __restrict__ Y,
+                                      double2* __restrict__ x) {
+    const BatchInstance& I = inst[blockIdx.y];
+    const int64_t cnt = I.n * I.ncol;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * BLOCK) {
+        const Source s = src[I.panel_off + e / I.ncol];
+        const double* d = dirs + 3 * (I.dir_off + e % I.ncol);
+        const int64_t t = I.vec_off + e;
+        double sn, cs;
+        sincos(I.k * (s.c[0] * d[0] + s.c[1] * d[1] + s.c[2] * d[2]), &sn, &cs);
+        x[t] = cmul(make_double2((double)Y[2 * t], (double)Y[2 * t + 1]), make_double2(cs, sn));
+    }
+}
+
 inline unsigned nb(int64_t n, int per = BLOCK) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -786,6 +938,69 @@ void ragged_fill(sbh_context* ctx, double2* p, int64_t n) {
 
 void ragged_stamp(sbh_context* ctx, unsigned long long* out) {
     hipLaunchKernelGGL(rstamp_kernel, dim3(1), dim3(1), 0, ctx->stream, out);
+    SBH_HIP(hipGetLastError());
+}
+
+namespace {
+// Grid over the largest instance block, at most 4096 workgroups per row of
+// instances (the kernels stride over the rest).
+dim3 instance_grid(int64_t max_block, int B, int layers) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(nb(max_block), 4096)), (unsigned)B, (unsigned)layers);
+}
+}  // namespace
+
+void batch_gather(sbh_context* ctx, int B, int64_t L, int64_t max_block, int layers, const BatchInstance* inst,
+                  const double2* const* src, double2* dst) {
+    hipLaunchKernelGGL(bgather_kernel, instance_grid(max_block, B, layers), dim3(BLOCK), 0, ctx->stream, inst, B, L,
+                       src, dst);
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_scatter(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const double2* x,
+                   double2* const* dst) {
+    hipLaunchKernelGGL(bscatter_kernel, instance_grid(max_block, B, 1), dim3(BLOCK), 0, ctx->stream, inst, x, dst);
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_gram(sbh_context* ctx, int m, int C, const BatchInstance* inst, const int* col_inst, const int* col_q,
+                const double2* P, int64_t L, const double2* rhs, double2* G, double2* h, double* bb) {
+    switch (m) {
+#define SBH_GRAM_CASE(M)                                                                                           \
+    case M:                                                                                                        \
+        hipLaunchKernelGGL(bgram_kernel<M>, dim3(C), dim3(BLOCK), 0, ctx->stream, inst, col_inst, col_q, P, L, rhs, \
+                           G, h, bb);                                                                              \
+        break;
+        SBH_GRAM_CASE(1)
+        SBH_GRAM_CASE(2)
+        SBH_GRAM_CASE(3)
+        SBH_GRAM_CASE(4)
+        SBH_GRAM_CASE(5)
+        SBH_GRAM_CASE(6)
+#undef SBH_GRAM_CASE
+        default:
+            throw Error(SBH_ERR_INVALID_ARGUMENT, "the number of candidates must be in 1..6");
+    }
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_combine(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const double2* S, int64_t L,
+                   int m, const double2* coef, double2* x) {
+    hipLaunchKernelGGL(bcombine_kernel, instance_grid(max_block, B, 1), dim3(BLOCK), 0, ctx->stream, inst, S, L, m,
+                       coef, x);
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_learned_raw(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const Source* src,
+                       const double* dirs, float* raw) {
+    hipLaunchKernelGGL(blearned_raw_kernel, instance_grid(max_block, B, 1), dim3(BLOCK), 0, ctx->stream, inst, src,
+                       dirs, raw);
+    SBH_HIP(hipGetLastError());
+}
+
+void batch_learned_phase(sbh_context* ctx, int B, int64_t max_block, const BatchInstance* inst, const Source* src,
+                         const double* dirs, const float* Y, double2* x) {
+    hipLaunchKernelGGL(blearned_phase_kernel, instance_grid(max_block, B, 1), dim3(BLOCK), 0, ctx->stream, inst, src,
+                       dirs, Y, x);
     SBH_HIP(hipGetLastError());
 }
 

@@ -314,6 +314,15 @@ struct sbh_system {
     int64_t admitted = 0;  // device bytes counted against the context's budget
     sbh::DeviceArray<double2> d_A;
     sbh::DeviceArray<double2> d_rhs;
+    // Row maxima of max(|Re|, |Im|) left by the value assembly (n rows of
+    // rowmax_stride partials), from which the low-precision copy takes its
+    // row scales without reading the matrix.  Valid only right after that
+    // assembly; any other write of the matrix clears it, and a matrix whose
+    // device address was handed out never uses it.
+    sbh::DeviceArray<double> d_rowmax;
+    int64_t rowmax_stride = 0;
+    bool rowmax_valid = false;
+    bool matrix_exposed = false;
 };
 
 struct sbh_vector {
@@ -546,9 +555,15 @@ void zgemm_f64_noalloc(sbh_context* ctx, const double2* A, int64_t lda, int64_t 
 // y = A x without kernel-timing events (blas_kernels.hip)
 void zgemv_n_untimed(sbh_context* ctx, const double2* A, int64_t lda, int64_t rows, int64_t cols, const double2* x,
                      double2* y);
-// Low-precision conversion (lowp_kernels.hip)
+// Low-precision conversion (lowp_kernels.hip).  With rowmax non-null the row
+// scales come from the assembly's row maxima (n rows of stride partials)
+// instead of a pass over A.
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
-                  uint16_t* out, double* row_inv);
+                  uint16_t* out, double* row_inv, const double* rowmax, int64_t stride);
+// The row maxima of sys's matrix when the last value assembly left them valid.
+inline const double* assembled_row_max(const sbh_system* sys) {
+    return sys->rowmax_valid && !sys->matrix_exposed ? sys->d_rowmax.ptr : nullptr;
+}
 int64_t lowp_padded_ld(int64_t n);
 int64_t lowp_split_count();
 int reduction_chunks(int64_t n);

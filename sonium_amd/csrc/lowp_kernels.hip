@@ -85,6 +85,23 @@ __global__ __launch_bounds__(BLOCK) void lp_rowscale_kernel(const double2* __res
     if (lane == 0) row_inv[i] = ldexp(1.0, e - top);
 }
 
+// Row scales from the assembly's row maxima: one wave per row takes the
+// maximum of its stride partials (the same value as the pass over the row:
+// the maximum does not depend on the order) and writes row_inv as above.
+__global__ __launch_bounds__(BLOCK) void lp_rowscale_from_max_kernel(const double* __restrict__ rowmax,
+                                                                     int64_t stride, int64_t n, int top,
+                                                                     double* __restrict__ row_inv) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double m = 0.0;
+    for (int64_t c = lane; c < stride; c += 64) m = fmax(m, rowmax[i * stride + c]);
+    m = wave_max(m);
+    int e = 0;
+    if (m > 0.0 && isfinite(m)) e = ilogb(m);
+    if (lane == 0) row_inv[i] = ldexp(1.0, e - top);
+}
+
 // Packing: a block converts one 32-row tile x 64 complex entries (8 fragments).
 // Each wave-wide load reads 64 consecutive entries (1 KB) of one row; the
 // converted {re, im} words go through LDS (rows padded to 68 words so a
@@ -770,10 +787,14 @@ int64_t lowp_split_count() { return LP_SPLIT; }
 int64_t lowp_padded_rows(int64_t n) { return (n + 31) / 32 * 32; }
 
 void lowp_convert(sbh_context* ctx, const double2* A, int64_t lda, int64_t n, int64_t ld, int format,
-                  uint16_t* out, double* row_inv) {
+                  uint16_t* out, double* row_inv, const double* rowmax, int64_t stride) {
     const int top = format == 0 ? Half<0>::TOP : Half<1>::TOP;
-    hipLaunchKernelGGL(lp_rowscale_kernel, dim3(nblocks(n, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, lda, n, top,
-                       row_inv);
+    if (rowmax)
+        hipLaunchKernelGGL(lp_rowscale_from_max_kernel, dim3(nblocks(n, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream,
+                           rowmax, stride, n, top, row_inv);
+    else
+        hipLaunchKernelGGL(lp_rowscale_kernel, dim3(nblocks(n, BLOCK / 64)), dim3(BLOCK), 0, ctx->stream, A, lda, n,
+                           top, row_inv);
     SBH_HIP(hipGetLastError());
     const int64_t pack_blocks = (n + 31) / 32 * (ld / LP_PACK_COLS);
     SBH_REQUIRE(pack_blocks < ((int64_t)1 << 31), "low-precision copy too large");

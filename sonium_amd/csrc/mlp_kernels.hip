@@ -68,7 +68,8 @@ __global__ __launch_bounds__(BLOCK) void mlp_gemm_kernel(const u16* __restrict__
     }
 }
 
-// Features of samples idx[0..M) (or 0..M when idx is null) -> X (M x 64) and X^T (64 x M).
+// Features of samples idx[0..M) (or 0..M when idx is null) -> X (M x 64) and
+// X^T (64 x M; not written when XT is null, as in inference).
 __global__ void mlp_features_kernel(const float* __restrict__ raw, const int64_t* __restrict__ idx, int M,
                                     u16* __restrict__ X, u16* __restrict__ XT) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,7 +102,7 @@ __global__ void mlp_features_kernel(const float* __restrict__ raw, const int64_t
     for (int j = 0; j < FEAT; ++j) {
         const u16 h = to_bf16(f[j]);
         X[(int64_t)s * FEAT + j] = h;
-        XT[(int64_t)j * M + s] = h;
+        if (XT) XT[(int64_t)j * M + s] = h;
     }
 }
 
@@ -112,7 +113,8 @@ __device__ __forceinline__ float silu_grad(float x) {
 }
 
 // H = sum of nsplit partials + bias (FP32, kept for the backward pass);
-// A = silu(H) in BF16, row-major and transposed.
+// A = silu(H) in BF16, row-major and transposed (H and AT are null in
+// inference: only training reads them).
 __global__ void mlp_bias_act_kernel(const float* __restrict__ part, int nsplit, const float* __restrict__ bias, int M,
                                     int N, float* __restrict__ H, u16* __restrict__ A, u16* __restrict__ AT) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,10 +122,10 @@ __global__ void mlp_bias_act_kernel(const float* __restrict__ part, int nsplit, 
     const int m = (int)(t / N), j = (int)(t % N);
     float h = bias[j];
     for (int z = 0; z < nsplit; ++z) h += part[(int64_t)z * M * N + t];
-    H[t] = h;
+    if (H) H[t] = h;
     const u16 a = to_bf16(silu(h));
     A[t] = a;
-    AT[(int64_t)j * M + m] = a;
+    if (AT) AT[(int64_t)j * M + m] = a;
 }
 
 // Output layer: Y = partial sum + bias (first 2 of 64 padded columns);
@@ -149,8 +151,10 @@ __global__ void mlp_output_kernel(const float* __restrict__ part, int nsplit, co
                 g = to_bf16(e / (float)M);
             }
         }
-        dZ[t] = g;
-        dZT[(int64_t)j * M + m] = g;
+        if (dZ) {
+            dZ[t] = g;
+            dZT[(int64_t)j * M + m] = g;
+        }
     }
     sh[threadIdx.x] = l;
     __syncthreads();
@@ -272,6 +276,7 @@ struct sbh_mlp {
     sbh::DeviceArray<float> part, Y;
     sbh::DeviceArray<double> loss_part;
     sbh::DeviceArray<int64_t> idx;
+    sbh::DeviceArray<float> raw_ws;  // raw inputs of device-side predictions
     // dataset
     sbh::DeviceArray<float> data_in, data_out;
     int64_t data_count = 0;
@@ -337,8 +342,11 @@ void refresh_weights(sbh_mlp* net) {
 int forward_backward(sbh_mlp* net, const float* raw, const int64_t* idx, int M, const float* targets, bool backward) {
     sbh_context* ctx = net->ctx;
     const int W = net->width;
+    // the transposed activations, pre-activations and output gradients only
+    // feed the backward pass
+    const bool keep = targets != nullptr;
     hipLaunchKernelGGL(mlp_features_kernel, dim3(nb(M)), dim3(BLOCK), 0, ctx->stream, raw, idx, M,
-                       reinterpret_cast<u16*>(net->X.ptr), reinterpret_cast<u16*>(net->XT.ptr));
+                       reinterpret_cast<u16*>(net->X.ptr), keep ? reinterpret_cast<u16*>(net->XT.ptr) : nullptr);
     SBH_HIP(hipGetLastError());
     const uint16_t* in = net->X.ptr;
     int in_dim = FEAT;
@@ -346,8 +354,9 @@ int forward_backward(sbh_mlp* net, const float* raw, const int64_t* idx, int M, 
         const mlp::Layer& ly = net->L[l];
         gemm(ctx, in, in_dim, net->Wb.ptr + net->wb_off[l], in_dim, M, W, in_dim, 1, net->part.ptr);
         hipLaunchKernelGGL(mlp_bias_act_kernel, dim3(nb((int64_t)M * W)), dim3(BLOCK), 0, ctx->stream, net->part.ptr,
-                           1, net->params.ptr + ly.b_off, M, W, net->H[l].ptr, reinterpret_cast<u16*>(net->A[l].ptr),
-                           reinterpret_cast<u16*>(net->AT[l].ptr));
+                           1, net->params.ptr + ly.b_off, M, W, keep ? net->H[l].ptr : nullptr,
+                           reinterpret_cast<u16*>(net->A[l].ptr),
+                           keep ? reinterpret_cast<u16*>(net->AT[l].ptr) : nullptr);
         SBH_HIP(hipGetLastError());
         in = net->A[l].ptr;
         in_dim = W;
@@ -356,8 +365,9 @@ int forward_backward(sbh_mlp* net, const float* raw, const int64_t* idx, int M, 
     gemm(ctx, in, W, net->Wb.ptr + net->wb_off[net->layers - 1], W, M, OUT_PAD, W, 1, net->part.ptr);
     const int blocks = (int)nb((int64_t)M * OUT_PAD);
     hipLaunchKernelGGL(mlp_output_kernel, dim3(blocks), dim3(BLOCK), 0, ctx->stream, net->part.ptr, 1,
-                       net->params.ptr + lo.b_off, M, targets, idx, net->Y.ptr, reinterpret_cast<u16*>(net->dZ.ptr),
-                       reinterpret_cast<u16*>(net->dZT.ptr), net->loss_part.ptr);
+                       net->params.ptr + lo.b_off, M, targets, idx, net->Y.ptr,
+                       keep ? reinterpret_cast<u16*>(net->dZ.ptr) : nullptr,
+                       keep ? reinterpret_cast<u16*>(net->dZT.ptr) : nullptr, net->loss_part.ptr);
     SBH_HIP(hipGetLastError());
     if (!targets || !backward) return targets ? blocks : 0;
     SBH_REQUIRE(M % (16 * kSplitW) == 0, "training batch must be a multiple of 256 samples");
@@ -407,6 +417,30 @@ double read_loss(sbh_mlp* net, int blocks) {
 inline void use_device(sbh_context* ctx) { SBH_HIP(hipSetDevice(ctx->device)); }
 
 }  // namespace
+}  // namespace sbh
+
+namespace sbh {
+
+sbh_context* mlp_context(const sbh_mlp* net) { return net ? net->ctx : nullptr; }
+
+float* mlp_raw_workspace(sbh_mlp* net, int64_t M) {
+    SBH_REQUIRE(M > 0 && M <= (1 << 24), "sample count out of range");
+    const int64_t Mp = (M + 255) / 256 * 256;
+    if (net->raw_ws.count < (size_t)Mp * RAW) net->raw_ws.allocate((size_t)Mp * RAW);
+    if (Mp > M)
+        SBH_HIP(hipMemsetAsync(net->raw_ws.ptr + M * RAW, 0, (size_t)(Mp - M) * RAW * sizeof(float),
+                               net->ctx->stream));
+    return net->raw_ws.ptr;
+}
+
+const float* mlp_forward_device(sbh_mlp* net, const float* raw, int64_t M) {
+    SBH_REQUIRE(M > 0 && M <= (1 << 24), "sample count out of range");
+    const int Mp = (int)((M + 255) / 256 * 256);
+    ensure_batch(net, Mp);
+    forward_backward(net, raw, nullptr, Mp, nullptr, false);
+    return net->Y.ptr;
+}
+
 }  // namespace sbh
 
 using namespace sbh;

@@ -202,7 +202,7 @@ template <bool DERIV>
 __device__ inline void ref_point_t(const double pos[3], double wga, const double src[3],
                                    const double srcn[3], const double nf[3], double wavruim, double k2,
                                    double k, double hf, Acc& r, Acc& dr) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double d0 = pos[0] - src[0], d1 = pos[1] - src[1], d2 = pos[2] - src[2];
     const double dsq = d0 * d0 + d1 * d1 + d2 * d2;
     if (!(dsq > 1e-30)) return;  // |d| <= 1e-15: normalize() returns 0, then `dis < 1e-15` skips
@@ -524,12 +524,8 @@ __device__ inline void deriv_free_terms(const PanelBc& bc, const DerivParams& P,
     *rhs = rh;
 }
 
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 
-// ---------------------------------------------------------------------------
-// Far kernel.  MODE 0: every field panel has a velocity BC without
-// admittance (rigid or vibrating body), so only H and E are needed and
-// A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
 // Per-panel constants of the TR13 points y_q = v0 + s_q e1 + t_q e2: the Gram
 // entries of the two edges.  Callers compute them once per field panel.
 struct FarPoints {
@@ -615,13 +611,26 @@ __device__ __forceinline__ double2 velocity_coefficient(double hr, double hi, do
 // Far kernel.  MODE 0: every field panel has a velocity BC without
 // admittance (rigid or vibrating body), so only H and E are needed and
 // A_ij = H*gamma*tau + E*beta.  MODE 1: general per-column coefficients.
+//
+// Row maxima for the low-precision copy (rowmax non-null, rows_per_block <=
+// FAR_MAX_ROWS): the block also writes, per row i, the maximum of
+// max(|Re A_ij|, |Im A_ij|) over its 256 columns at rowmax[i * nbx +
+// blockIdx.x], leaving out the near pairs and the diagonal that the later
+// kernels overwrite (they fold their entries into rowmax[i * nbx + nbx - 1]
+// as bit patterns with an atomic max; the column-0 blocks clear that slot).
+constexpr int FAR_MAX_ROWS = 16;
+
 template <int MODE>
 __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ panels,
                                                   const Source* __restrict__ sources,
                                                   const PanelBc* __restrict__ bcs, int64_t n,
                                                   int64_t lda, int rows_per_block, double k,
                                                   double tau_r, double2 beta, double2 iwr,
-                                                  double2* __restrict__ A) {
+                                                  double2* __restrict__ A, const int64_t* __restrict__ row_ptr,
+                                                  const NearPair* __restrict__ pairs, double* __restrict__ rowmax,
+                                                  int64_t nbx) {
+    __shared__ double smax[FAR_MAX_ROWS][256];
+    __shared__ unsigned nmask[FAR_MAX_ROWS][8];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool active = j < n;
     const int64_t jj = active ? j : n - 1;
@@ -632,6 +641,20 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
     const double k2 = k * k;
     const int64_t i0 = (int64_t)blockIdx.y * rows_per_block;
     const int64_t i1 = min(n, i0 + rows_per_block);
+    const int64_t c0 = (int64_t)blockIdx.x * 256;
+    if (rowmax) {
+        // near columns of this block's rows: the rows' near pairs are one
+        // contiguous range of the row-sorted pair list, read in one pass
+        if (threadIdx.x < FAR_MAX_ROWS * 8) nmask[threadIdx.x >> 3][threadIdx.x & 7] = 0u;
+        __syncthreads();
+        const int64_t e1 = row_ptr[i1];
+        for (int64_t e = row_ptr[i0] + threadIdx.x; e < e1; e += 256) {
+            const NearPair np = pairs[e];
+            const int64_t c = (int64_t)np.j - c0;
+            if (c >= 0 && c < 256) atomicOr(&nmask[np.i - i0][c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+    }
     for (int64_t i = i0; i < i1; ++i) {
         const Source src = sources[i];
         double hr, hi, er, ei, gr, gi, tr, ti;
@@ -648,7 +671,40 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
             out = coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
         }
         if (active) A[i * lda + j] = out;
+        if (rowmax) {
+            const int r = (int)(i - i0);
+            const bool near = ((nmask[r][threadIdx.x >> 5] >> (threadIdx.x & 31)) & 1u) != 0u;
+            smax[r][threadIdx.x] = (active && !near && j != i) ? fmax(fabs(out.x), fabs(out.y)) : 0.0;
+        }
     }
+    if (rowmax) {
+        __syncthreads();
+        const int r = threadIdx.x >> 4, seg = threadIdx.x & 15;
+        double m = 0.0;
+        if (i0 + r < i1) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) m = fmax(m, smax[r][seg * 16 + c]);
+        }
+        __syncthreads();
+        smax[r][seg] = m;
+        __syncthreads();
+        if (threadIdx.x < FAR_MAX_ROWS && i0 + threadIdx.x < i1) {
+            double mm = 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) mm = fmax(mm, smax[threadIdx.x][c]);
+            const int64_t i = i0 + threadIdx.x;
+            rowmax[i * nbx + blockIdx.x] = mm;
+            if (blockIdx.x == 0) rowmax[i * nbx + nbx - 1] = 0.0;
+        }
+    }
+}
+
+// Folds max(|Re v|, |Im v|) of a near or diagonal entry into the row's slot
+// (non-negative doubles order like their bit patterns; NaN entries are
+// skipped, as fmax skips them).
+__device__ __forceinline__ void fold_row_max(double* slot, double2 v) {
+    const double m = fmax(fabs(v.x), fabs(v.y));
+    if (m == m) atomicMax(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)__double_as_longlong(m));
 }
 
 // Coefficient of one near pair, summed over its leaves by one wave (lanes
@@ -719,7 +775,8 @@ __global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ 
                                                    const PanelBc* __restrict__ bcs, int general,
                                                    int64_t lda, double k, double wavruim,
                                                    double tau_r, double2 beta, double2 iwr,
-                                                   double2* __restrict__ A) {
+                                                   double2* __restrict__ A, double* __restrict__ rowmax,
+                                                   int64_t nbx) {
     const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= npairs) return;
@@ -728,7 +785,10 @@ __global__ __launch_bounds__(256) void near_kernel(const NearPair* __restrict__ 
     const PanelGeom G = geom[np.j];  // constant indexing only: lives in registers
     const Source S = sources[np.i];
     const double2 v = near_entry(np, leaves, G, S, bc, k, wavruim, tau_r, beta, iwr, lane);
-    if (lane == 0) A[(int64_t)np.i * lda + np.j] = v;
+    if (lane == 0) {
+        A[(int64_t)np.i * lda + np.j] = v;
+        if (rowmax) fold_row_max(rowmax + (int64_t)np.i * nbx + nbx - 1, v);
+    }
 }
 
 // One wave per panel: self integrals, free terms, BC free-term RHS.
@@ -737,7 +797,8 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
                                                    const PanelBc* __restrict__ bcs, int general,
                                                    int64_t n, int64_t lda, double k, double wavruim,
                                                    double tau_r, double2 beta, double2 iwr,
-                                                   double2* __restrict__ A, double2* __restrict__ rhs) {
+                                                   double2* __restrict__ A, double2* __restrict__ rhs,
+                                                   double* __restrict__ rowmax, int64_t nbx) {
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= n) return;
@@ -747,6 +808,7 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
     if (lane == 0) {
         A[i * lda + i] = diag;
         rhs[i] = local_rhs;
+        if (rowmax) fold_row_max(rowmax + i * nbx + nbx - 1, diag);
     }
 }
 
@@ -1052,7 +1114,7 @@ __global__ void plane_wave_kernel(const Source* __restrict__ sources, int64_t n,
     rhs[t] = accumulate ? cadd(rhs[t], add) : add;
 }
 
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 
 // Far entries of a frequency family (instances on one mesh with wavenumbers
 // k_f = k0 + f dk): one lane per column, FAMILY_ROWS rows per workgroup,
@@ -1187,7 +1249,7 @@ __global__ __launch_bounds__(256) void batch_near_family_kernel(const FreqFamily
                                                                 const PanelGeom* __restrict__ geom,
                                                                 const Source* __restrict__ sources,
                                                                 double2* __restrict__ A) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const FreqFamily& fam = families[blockIdx.y];
     const int sub = threadIdx.x & (NEAR_FAMILY_LANES - 1);
     const int count = fam.count;
@@ -1612,7 +1674,7 @@ __global__ void source_rhs_kernel(const Source* __restrict__ centers, int64_t n,
     inc = cmul(inc, make_double2(sign, 0.0));
     rhs[i] = accumulate ? cadd(rhs[i], inc) : inc;
 }
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 
 // scattered[i] = sum over slices of the partials; incident[i] = plane waves at
 // x_i (incident.rs:216-260); total[i] = incident + scattered (FieldPoint::new,
@@ -1704,14 +1766,22 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, s
     int rows_per_block = 16;
     while ((n + rows_per_block - 1) / rows_per_block > 65535) rows_per_block *= 2;
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + rows_per_block - 1) / rows_per_block));
+    // row maxima for the low-precision copy, valid once every kernel below has run
+    sys->rowmax_valid = false;
+    const int64_t nbx = sys->rowmax_stride;
+    const bool row_max = rows_per_block <= FAR_MAX_ROWS && nbx == (int64_t)grid.x + 1 && sys->d_rowmax.ptr &&
+                         ws->d_row_ptr.ptr && !sys->matrix_exposed;
+    double* rm = row_max ? sys->d_rowmax.ptr : nullptr;
     hipEvent_t ev;
     ctx->begin_timed(SBH_KERNEL_ASSEMBLE_FAR, &ev);
     if (general)
         hipLaunchKernelGGL(far_kernel<1>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
-                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr);
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr,
+                           ws->d_row_ptr.ptr, ws->d_pairs.ptr, rm, nbx);
     else
         hipLaunchKernelGGL(far_kernel<0>, grid, dim3(256), 0, ctx->stream, m->d_far.ptr, m->d_src.ptr,
-                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr);
+                           m->d_bc.ptr, n, sys->lda, rows_per_block, k, ph.tau, beta, iwr, sys->d_A.ptr,
+                           ws->d_row_ptr.ptr, ws->d_pairs.ptr, rm, nbx);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_FAR, ev);
     if (ws->near_pairs > 0) {
@@ -1719,16 +1789,17 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, s
         hipLaunchKernelGGL(near_kernel, dim3((unsigned)((ws->near_pairs + 3) / 4)), dim3(256), 0,
                            ctx->stream, ws->d_pairs.ptr, ws->near_pairs, ws->d_leaves.ptr,
                            m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, general, sys->lda, k, wavruim,
-                           ph.tau, beta, iwr, sys->d_A.ptr);
+                           ph.tau, beta, iwr, sys->d_A.ptr, rm, nbx);
         SBH_HIP(hipGetLastError());
         ctx->end_timed(SBH_KERNEL_ASSEMBLE_NEAR, ev);
     }
     ctx->begin_timed(SBH_KERNEL_ASSEMBLE_SELF, &ev);
     hipLaunchKernelGGL(self_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream,
                        m->d_geom.ptr, m->d_src.ptr, m->d_bc.ptr, general, n, sys->lda, k, wavruim,
-                       ph.tau, beta, iwr, sys->d_A.ptr, sys->d_rhs.ptr);
+                       ph.tau, beta, iwr, sys->d_A.ptr, sys->d_rhs.ptr, rm, nbx);
     SBH_HIP(hipGetLastError());
     ctx->end_timed(SBH_KERNEL_ASSEMBLE_SELF, ev);
+    sys->rowmax_valid = row_max;
     const int64_t ncols = (int64_t)m->nonzero_bc.size();
     if (ncols > 0) {
         ctx->begin_timed(SBH_KERNEL_RHS, &ev);
@@ -1778,6 +1849,7 @@ void launch_derivative_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c6
     P.iwr = make_double2(0.0, ph.pressure_factor);
     P.iwr_d = make_double2(0.0, pf_derivative);
     const int general = m->general ? 1 : 0;
+    sys->rowmax_valid = false;  // the matrix now holds a derivative
     int rows_per_block = 16;
     while ((n + rows_per_block - 1) / rows_per_block > 65535) rows_per_block *= 2;
     const dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + rows_per_block - 1) / rows_per_block));

@@ -13,7 +13,10 @@ first group from the network (or zero), later groups from the minimal-residual
 combination of the solutions at the nearest solved sweep points on either
 side, taken from the two groups that bracket the group in the order (the
 reference's sweep carries the previous solution only,
-frequency_sweep.rs:718-730).
+frequency_sweep.rs:718-730).  Solved points stay in a device store; guesses
+are formed on the device (sbh_batch_learned_guesses, sbh_batch_guess_gram /
+sbh_batch_guess_combine, sbh_batch_set_guesses_device) and only the
+solutions of the whole sweep come back to the host, at the end.
 
 Tried and not kept: two contexts on the device, one host thread each, so that
 one group's assembly overlaps another's solve.  The persistent assembly
@@ -30,7 +33,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import Batch, Context, TbemMesh
+from . import Batch, Context, DeviceVector, TbemMesh
 from . import warmstart as wsm
 
 __all__ = ["group_order", "group_neighbours", "SweepResult", "SweepBatch", "batched_sweep"]
@@ -73,16 +76,37 @@ class SweepResult:
 
 
 class SweepBatch:
-    """The batch of one group (``group`` instances on ``mesh``), kept between sweeps."""
+    """The batch of one group (``group`` instances on ``mesh``), kept between
+    sweeps, with the device store of solved sweep points and a buffer for the
+    network's guesses."""
 
     def __init__(self, ctx: Context, mesh, group: int):
         self.ctx = ctx
         self.group = int(group)
+        self.n = mesh.num_elements
         self.tm = TbemMesh(ctx, mesh)
         self.c, self.nn, _ = self.tm.geometry()
         self.batch = Batch(ctx, [self.tm] * self.group)
+        self.store = None
+        self.learned = None
+
+    def ensure_store(self, points: int) -> DeviceVector:
+        if self.store is None or self.store.length < points * self.n:
+            if self.store is not None:
+                self.store.close()
+            self.store = DeviceVector(self.ctx, points * self.n)
+        if self.learned is None:
+            self.learned = DeviceVector(self.ctx, self.group * self.n)
+        return self.store
+
+    def pointer(self, j: int) -> int:
+        """Device address of sweep point j's solution in the store."""
+        return self.store.ptr + 16 * self.n * int(j)
 
     def close(self):
+        for v in (self.store, self.learned):
+            if v is not None:
+                v.close()
         self.batch.close()
         self.tm.close()
 
@@ -115,12 +139,12 @@ def batched_sweep(ctx: Context, mesh, ks, direction, *, stride: int = 4, net: Op
     n = mesh.num_elements
     d = np.asarray(direction, dtype=np.float64).reshape(3)
     neighbours = group_neighbours(stride)
-    X = np.zeros((len(ks), n), dtype=np.complex128)
     iters = np.zeros(len(ks), dtype=np.int64)
     res = np.zeros(len(ks))
     guess_res = {}
     parts = {"assemble_convert": 0.0, "guesses": 0.0, "solve": 0.0}
     batch = sb.batch
+    sb.ensure_store(len(ks))
     ctx.synchronize()
     t0 = time.perf_counter()
     try:
@@ -138,9 +162,9 @@ def batched_sweep(ctx: Context, mesh, ks, direction, *, stride: int = 4, net: Op
             if kind is not None:
                 # only the bracketing groups count
                 allowed = set() if neighbours[g] is None else {neighbours[g][0], neighbours[g][1] % stride}
-                solved = {j: X[j] for j in range(len(ks)) if j % stride in allowed}
-                _, rels = wsm.batch_guesses(batch, kind, idx, solved, net, sb.c, sb.nn, kg, d, products=products,
-                                            per_side=per_side)
+                solved = {j for j in range(len(ks)) if j % stride in allowed}
+                rels = wsm.batch_guesses_device(batch, kind, idx, solved, sb.pointer, net, products=products,
+                                                per_side=per_side, learned_buffer=sb.learned)
                 for j, r in zip(idx, rels):
                     guess_res[j] = r
             tc = time.perf_counter()
@@ -149,14 +173,15 @@ def batched_sweep(ctx: Context, mesh, ks, direction, *, stride: int = 4, net: Op
                               use_guess=kind is not None)
             if not all(q.converged for q in sts):
                 raise RuntimeError(f"sweep group {g} left a frequency unconverged")
-            for i, (j, x) in enumerate(zip(idx, batch.solutions())):
-                X[j] = x[:, 0]
+            batch.store_solutions([sb.pointer(j) for j in idx])
+            for i, j in enumerate(idx):
                 iters[j] = sts[i].iterations
                 res[j] = sts[i].residual
             td = time.perf_counter()
             parts["assemble_convert"] += tb - ta
             parts["guesses"] += tc - tb
             parts["solve"] += td - tc
+        X = sb.store.download(0, len(ks) * n).reshape(len(ks), n)
         ctx.synchronize()
         el = time.perf_counter() - t0
     finally:

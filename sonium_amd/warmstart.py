@@ -178,10 +178,61 @@ def minimal_residual_combination(b, candidates, products):
     return X @ coef, float(np.linalg.norm(r) / np.linalg.norm(b))
 
 
-def nearest_solved(solved: dict, j: int, per_side: int = 1):
-    """Indices of the nearest solved sweep points below and above j (None when
-    absent); with per_side > 1, lists of up to that many on each side, nearest
-    first."""
+def batch_guesses_device(batch, kind: str, indices, solved, store_pointer, net: Optional["WarmStartMLP"] = None,
+                         products: str = "fp64", per_side: int = 1, learned_buffer=None):
+    """batch_guesses with every vector on the device: the solved sweep points
+    live in device memory (``store_pointer(j)`` is the device address of sweep
+    point j's solution block; ``solved`` the solved indices), the network's
+    guesses come from sbh_batch_learned_guesses, and the minimal-residual
+    combinations from sbh_batch_guess_gram / sbh_batch_guess_combine; only the
+    per-column m x m normal equations are solved on the host (pseudo-inverse,
+    as batch_guesses).  ``learned_buffer`` is a DeviceVector of the batch's
+    packed length, needed for "neighbours+learned".  Returns the
+    combinations' relative residuals ||b - A x0|| / ||b|| (empty for
+    "learned" and "previous")."""
+    rels = []
+    if kind == "learned":
+        batch.learned_guesses(net)
+        return rels
+    if kind == "previous":
+        batch.set_guesses_device([None if lo is None else store_pointer(lo)
+                                  for lo, _ in (nearest_solved(solved, j) for j in indices)])
+        return rels
+    if kind not in ("neighbours", "neighbours+learned"):
+        raise ValueError(f"unknown guess kind {kind!r}")
+    near = [nearest_solved(solved, j, per_side) for j in indices]
+    if per_side == 1:
+        near = [([lo] if lo is not None else [], [hi] if hi is not None else []) for lo, hi in near]
+    slots = []
+    for r in range(per_side):
+        slots.append([store_pointer(lo[r]) if r < len(lo) else None for lo, _ in near])
+        slots.append([store_pointer(hi[r]) if r < len(hi) else None for _, hi in near])
+    if kind == "neighbours+learned":
+        if learned_buffer is None:
+            raise ValueError("neighbours+learned needs a learned_buffer")
+        batch.learned_guesses(net, out=learned_buffer)
+        offs = np.cumsum([0] + [a * b for a, b in batch._packed_shapes()])[:-1]
+        slots.append([learned_buffer.ptr + 16 * int(o) for o in offs])
+    G, h, bb = batch.guess_gram(slots, products)
+    coef = (np.linalg.pinv(G, rcond=1e-13, hermitian=True) @ h[:, :, None])[:, :, 0]
+    # ||b - P c||^2 = ||b||^2 - 2 Re(c^H h) + c^H G c
+    r2 = bb - 2.0 * np.real(np.einsum("ca,ca->c", coef.conj(), h)) + \
+        np.real(np.einsum("ca,cab,cb->c", coef.conj(), G, coef))
+    rel_all = np.sqrt(np.maximum(r2, 0.0) / bb)
+    for c in range(coef.shape[0]):
+        rel = float(rel_all[c])
+        if not (np.all(np.isfinite(coef[c])) and np.isfinite(rel) and rel < 1.0):
+            coef[c] = 0.0
+            rel = 1.0
+        rels.append(rel)
+    batch.guess_combine(coef)
+    return rels
+
+
+def nearest_solved(solved, j: int, per_side: int = 1):
+    """Indices of the nearest solved sweep points (a dict or set of indices)
+    below and above j (None when absent); with per_side > 1, lists of up to
+    that many on each side, nearest first."""
     below = sorted((i for i in solved if i < j), reverse=True)
     above = sorted(i for i in solved if i > j)
     if per_side == 1:

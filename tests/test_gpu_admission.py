@@ -16,7 +16,7 @@ def test_device_budget_refuses_before_allocating():
     ctx = sa.Context(0)
     assert ctx.device_budget() == (0, 0)
     n = 1280
-    sys_bytes = 16 * (n * n + n)
+    sys_bytes = 16 * (n * n + n) + 8 * n * ((n + 255) // 256 + 1)  # matrix, rhs, row maxima
     ctx.set_device_budget(sys_bytes + 1024)
     s1 = sa.TbemSystem(ctx, n)
     assert ctx.device_budget() == (sys_bytes + 1024, sys_bytes)

@@ -287,6 +287,82 @@ def test_batch_sweep_guesses_from_solved_neighbours(ctx):
     tm.close()
 
 
+def test_device_guesses_match_host_guesses(ctx):
+    """The device-side guess pipeline of the sweep driver gives the host
+    pipeline's guesses: solutions stored on the device and set back as guesses
+    bit for bit; previous-point guesses bit for bit; minimal-residual
+    combinations from sbh_batch_guess_gram / sbh_batch_guess_combine equal
+    warmstart.batch_guesses (host products and normal equations) to 1e-12,
+    and their residuals, formed from the Gram data, equal the host
+    recomputation b - A x0; the network's guesses (sbh_batch_learned_guesses)
+    equal its predictions on the same inputs times exp(i k d.c) to 1e-14.
+    Host addresses in a pointer table are refused before any launch."""
+    from sonium_amd import warmstart as wsm
+    from sonium_amd.mesh import icosphere
+
+    m = icosphere(1.0, 2)
+    n = m.num_elements
+    tm = sa.TbemMesh(ctx, m)
+    c, nn, _ = tm.geometry()
+    ks = np.linspace(2.0, 2.7, 8)
+    d = np.array([0.0, 0.0, 1.0])
+    b = sa.Batch(ctx, [tm] * 4)
+    store = sa.DeviceVector(ctx, len(ks) * n)
+    ptr = lambda j: store.ptr + 16 * n * j  # noqa: E731
+    idx = [0, 2, 4, 6]
+    b.set_problem([float(ks[i]) for i in idx], [d.reshape(1, 3)] * 4)
+    b.assemble()
+    st = b.solve("fp64", restart=50, max_cycles=10, tolerance=1e-10, use_graph=False)
+    assert all(s.converged for s in st)
+    solved = {j: b.solution(i)[:, 0].copy() for i, j in enumerate(idx)}
+    b.store_solutions([ptr(j) for j in idx])
+    for j in idx:
+        assert np.array_equal(store.download(j * n, n), solved[j])
+    b.set_guesses_device([ptr(j) for j in idx[::-1]])
+    assert all(np.array_equal(x[:, 0], solved[j]) for x, j in zip(b.solutions(), idx[::-1]))
+
+    idx = [1, 3, 5, 7]
+    b.set_problem([float(ks[i]) for i in idx], [d.reshape(1, 3)] * 4)
+    b.assemble()
+    b.convert("fp16")
+    wsm.batch_guesses_device(b, "previous", idx, set(solved), ptr)
+    for x, j in zip(b.solutions(), idx):
+        assert np.array_equal(x[:, 0], solved[j - 1])
+    for fmt in ("fp64", "fp16"):
+        X0, rels = wsm.batch_guesses(b, "neighbours", idx, solved, None, c, nn, ks[idx], d, products=fmt)
+        rels_dev = wsm.batch_guesses_device(b, "neighbours", idx, set(solved), ptr, products=fmt)
+        Xd = [x[:, 0] for x in b.solutions()]
+        for i in range(4):
+            assert rel_l2(Xd[i], X0[i]) < 1e-12
+            A, rhs = b.matrix(i), b.rhs(i)[:, 0]
+            r = np.linalg.norm(rhs - A @ Xd[i]) / np.linalg.norm(rhs)
+            if fmt == "fp64":
+                assert abs(rels_dev[i] - r) <= 1e-12 + 1e-8 * r
+            assert abs(rels_dev[i] - rels[i]) <= 1e-12 + 1e-8 * rels[i]
+    # a host address is refused with an error, not passed to a kernel
+    host = np.zeros(n, dtype=np.complex128)
+    with pytest.raises(sa.SbhError):
+        b.set_guesses_device([host.ctypes.data, None, None, None])
+
+    net = wsm.WarmStartMLP(ctx, width=512, seed=5)
+    b.learned_guesses(net)
+    Xl = np.stack([x[:, 0] for x in b.solutions()])
+    raw = np.concatenate([wsm.panel_inputs(c, nn, float(ks[j]), d) for j in idx])
+    y = net.predict(raw).astype(np.float64).reshape(4, n, 2)
+    ref = (y[..., 0] + 1j * y[..., 1]) * np.exp(1j * ks[idx][:, None] * (c @ d)[None, :])
+    assert np.max(np.abs(Xl - ref)) <= 1e-14 * np.max(np.abs(ref))
+    buf = sa.DeviceVector(ctx, 4 * n)
+    b.learned_guesses(net, out=buf)
+    assert np.array_equal(buf.download().reshape(4, n), Xl)
+    rels_l = wsm.batch_guesses_device(b, "neighbours+learned", idx, set(solved), ptr, net=net, learned_buffer=buf)
+    assert all(0.0 < r < 1.0 for r in rels_l)
+    net.close()
+    buf.close()
+    store.close()
+    b.close()
+    tm.close()
+
+
 def test_frequency_family_matches_single_system(ctx):
     """Instances on one mesh with equally spaced wavenumbers share one pass over
     the pair geometry (batch_far_family_kernel for the far entries,

@@ -208,6 +208,55 @@ def test_lowp_copy_is_bit_exact(ctx, c1_block, fmt):
     assert bad.shape[0] == 0, f"{bad.shape[0]} of {halves.size} halves differ"
 
 
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_row_scales_from_assembly_equal_the_matrix_pass(ctx, fmt):
+    """Right after a value assembly the copy takes its row scales from the
+    assembly kernels' row maxima (far blocks without their near pairs and
+    diagonal, near and diagonal entries folded in) instead of reading the
+    matrix twice.  The copy equals the oracle's storage of the downloaded
+    matrix bit for bit, rigid and mixed-BC meshes alike; after a derivative
+    assembly into the same system, and after the matrix's device address has
+    been handed out, the conversion reads the matrix instead and still equals
+    the oracle; a new value assembly gives the first copy again."""
+    from oracle import lowp as olp
+    from sonium_amd.mesh import ellipsoid
+
+    rng = np.random.default_rng(3)
+    cases = [(icosphere(1.0, 3), 3.0, {}),
+             (ellipsoid((1.3, 0.8, 0.6), 3), 5.0, {}),
+             (icosphere(1.0, 3), 2.0, dict(bc_type=(rng.random(1280) < 0.3).astype(np.int32),
+                                           bc_value=rng.standard_normal(1280) + 0j,
+                                           admittance=np.where(rng.random(1280) < 0.3, 0.003 - 0.001j, np.nan)))]
+    for m, k, bcs in cases:
+        tm = sa.TbemMesh(ctx, m, **bcs)
+        ws = sa.TbemAssemblyWorkspace(ctx, tm)
+        ph = sa.PhysicsParams.from_wavenumber(k)
+        beta = ph.burton_miller_beta()
+        sys_ = ws.assemble(ph, beta)
+        assert ws.stats["near_pairs"] > 0
+        lp = sa.LowpMatrix(sys_, fmt)
+        h1, inv1, ld = lp.download()
+        ref_h, ref_inv = olp.row_scaled_halves(sys_.matrix, fmt, ld)
+        assert np.array_equal(inv1, ref_inv) and np.array_equal(h1, ref_h)
+        ws.assemble_wavenumber_derivative(ph, system=sys_)
+        lp.update()
+        h2, inv2, _ = lp.download()
+        ref_h, ref_inv = olp.row_scaled_halves(sys_.matrix, fmt, ld)
+        assert np.array_equal(inv2, ref_inv) and np.array_equal(h2, ref_h)
+        ws.assemble(ph, beta, sys_)
+        lp.update()
+        h3, inv3, _ = lp.download()
+        assert np.array_equal(inv3, inv1) and np.array_equal(h3, h1)
+        sys_.device_pointers()
+        lp.update()
+        h4, inv4, _ = lp.download()
+        assert np.array_equal(inv4, inv1) and np.array_equal(h4, h1)
+        lp.close()
+        sys_.close()
+        ws.close()
+        tm.close()
+
+
 def test_device_block_solvers_equal_host_calls(ctx, c1_block):
     """gmres_block_device / gmres_ir_device on HBM-resident blocks give the host
     calls' solutions exactly; a zero start forms no FP64 product for the first
