@@ -849,6 +849,10 @@ sbh_status sbh_debug_device_copy(sbh_context* ctx, int64_t bytes, int reps);
  * launches) and of independent FP64 FMAs on every CU (TFLOP/s, FMA = 2). */
 sbh_status sbh_debug_hbm_read_rate(sbh_context* ctx, int64_t bytes, int reps, double* gbs);
 sbh_status sbh_debug_fp64_fma_rate(sbh_context* ctx, double* tflops);
+/* Test hook: sbh_mlp_predict through the layer-by-layer forward pass of
+ * training (one GEMM launch per layer, activations in HBM) instead of the
+ * fused inference kernel, to pin the two to the same bits. */
+sbh_status sbh_debug_mlp_predict_layered(sbh_mlp* net, const float* inputs, int64_t M, float* out);
 
 #ifdef __cplusplus
 }

@@ -275,6 +275,7 @@ SIGNATURES = {
     "sbh_debug_device_sincos": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sbh_debug_device_rsqrt": (_int, [_vp, _vp, _i64, _vp]),
     "sbh_debug_device_copy": (_int, [_vp, _i64, _int]),
+    "sbh_debug_mlp_predict_layered": (_int, [_vp, _vp, _i64, _vp]),
     "sbh_debug_hbm_read_rate": (_int, [_vp, _i64, _int, ctypes.POINTER(ctypes.c_double)]),
     "sbh_debug_fp64_fma_rate": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
 }

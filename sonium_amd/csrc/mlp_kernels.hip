@@ -68,14 +68,8 @@ __global__ __launch_bounds__(BLOCK) void mlp_gemm_kernel(const u16* __restrict__
     }
 }
 
-// Features of samples idx[0..M) (or 0..M when idx is null) -> X (M x 64) and
-// X^T (64 x M; not written when XT is null, as in inference).
-__global__ void mlp_features_kernel(const float* __restrict__ raw, const int64_t* __restrict__ idx, int M,
-                                    u16* __restrict__ X, u16* __restrict__ XT) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    const float* r = raw + (idx ? idx[s] : s) * RAW;
-    float f[FEAT];
+// The 64 features of one raw sample {c, n, d, k} (FEAT_USED used, zero padded).
+__device__ __forceinline__ void sample_features(const float* __restrict__ r, float* f) {
     const float c0 = r[0], c1 = r[1], c2 = r[2], n0 = r[3], n1 = r[4], n2 = r[5], d0 = r[6], d1 = r[7], d2 = r[8];
     const float kk = r[9] * (1.0f / 12.0f);
     const float dc = d0 * c0 + d1 * c1 + d2 * c2;
@@ -98,6 +92,16 @@ __global__ void mlp_features_kernel(const float* __restrict__ raw, const int64_t
     }
 #pragma unroll
     for (int j = FEAT_USED; j < FEAT; ++j) f[j] = 0.f;
+}
+
+// Features of samples idx[0..M) (or 0..M when idx is null) -> X (M x 64) and
+// X^T (64 x M; not written when XT is null, as in inference).
+__global__ void mlp_features_kernel(const float* __restrict__ raw, const int64_t* __restrict__ idx, int M,
+                                    u16* __restrict__ X, u16* __restrict__ XT) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    float f[FEAT];
+    sample_features(raw + (idx ? idx[s] : s) * RAW, f);
 #pragma unroll
     for (int j = 0; j < FEAT; ++j) {
         const u16 h = to_bf16(f[j]);
@@ -241,6 +245,91 @@ __global__ void mlp_sample_kernel(int64_t* __restrict__ idx, int M, uint64_t see
 }
 
 inline unsigned nb(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+
+// Inference through all layers: a workgroup carries INFER_ROWS samples, their
+// activations held in LDS (two row-major buffers of ldr halves per row, rows
+// padded by 8 halves); each wave computes 32-column tiles of a layer with
+// the GEMM's operand layout and accumulation order (one MFMA per 16 inputs,
+// in order), then bias + SiLU in registers, so the outputs equal the
+// layer-by-layer forward pass bit for bit.  Weights stream from L2.
+constexpr int INFER_ROWS = 32;
+constexpr int INFER_MAX_TILES = 8;  // 32-column tiles per wave (TPW <= 8): width <= 4 * 8 * 32 = 1024
+struct InferArgs {
+    const u16* w[4];
+    const float* b[4];
+    int in[4];
+    int out[4];
+};
+
+template <int TPW>
+__global__ __launch_bounds__(BLOCK) void mlp_infer_kernel(const float* __restrict__ raw, InferArgs a, int ldr,
+                                                          float* __restrict__ Y) {
+    extern __shared__ __attribute__((aligned(16))) u16 act[];
+    u16* src = act;
+    u16* dst = act + INFER_ROWS * ldr;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t s0 = (int64_t)blockIdx.x * INFER_ROWS;
+    if (threadIdx.x < INFER_ROWS) {
+        float f[FEAT];
+        sample_features(raw + (s0 + threadIdx.x) * RAW, f);
+#pragma unroll
+        for (int j = 0; j < FEAT; ++j) src[threadIdx.x * ldr + j] = to_bf16(f[j]);
+    }
+    __syncthreads();
+    for (int l = 0; l < 4; ++l) {
+        const int in = a.in[l];
+        const int ntiles = (a.out[l] + 31) / 32;
+        const u16* __restrict__ w = a.w[l];
+        f32x16 acc[TPW];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+        const int kq = (lane >> 5) * 8;
+        for (int k = 0; k < in; k += 16) {
+            const u32x4 av = *reinterpret_cast<const u32x4*>(&src[(lane & 31) * ldr + k + kq]);
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = wave + 4 * t;
+                if (tile < ntiles) {
+                    const u32x4 bv =
+                        *reinterpret_cast<const u32x4*>(w + (int64_t)(32 * tile + (lane & 31)) * in + k + kq);
+                    acc[t] = mfma32<1>(av, bv, acc[t]);
+                }
+            }
+        }
+        const float* __restrict__ bias = a.b[l];
+        if (l < 3) {
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = wave + 4 * t;
+                if (tile < ntiles) {
+                    const int col = 32 * tile + (lane & 31);
+                    const float bc = bias[col];
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                        dst[row * ldr + col] = to_bf16(silu(bc + acc[t][v]));
+                    }
+                }
+            }
+            __syncthreads();
+            u16* tmp = src;
+            src = dst;
+            dst = tmp;
+        } else if (wave == 0 && (lane & 31) < 2) {
+            const int col = lane & 31;
+            const float bc = bias[col];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                Y[(s0 + row) * 2 + col] = bc + acc[0][v];
+            }
+        }
+    }
+}
 
 }  // namespace
 
@@ -404,6 +493,45 @@ int forward_backward(sbh_mlp* net, const float* raw, const int64_t* idx, int M, 
     return blocks;
 }
 
+// Predictions of Mp samples (a multiple of 256) into net->Y: the fused
+// kernel when the width allows (every layer's tiles fit the waves' tile
+// registers), else the layer-by-layer forward pass.  Same bits either way.
+void infer(sbh_mlp* net, const float* raw, int Mp) {
+    const int W = net->width;
+    if (W > 4 * INFER_MAX_TILES * 32) {
+        ensure_batch(net, Mp);
+        forward_backward(net, raw, nullptr, Mp, nullptr, false);
+        return;
+    }
+    if ((int64_t)net->Y.count < (int64_t)Mp * 2) net->Y.allocate((size_t)Mp * 2);
+    InferArgs a;
+    for (int l = 0; l < 4; ++l) {
+        const mlp::Layer& ly = net->L[l];
+        a.w[l] = reinterpret_cast<const u16*>(net->Wb.ptr + net->wb_off[l]);
+        a.b[l] = net->params.ptr + ly.b_off;
+        a.in[l] = ly.cols;
+        a.out[l] = ly.rows;
+    }
+    const int ldr = W + 8;
+    const size_t lds = (size_t)2 * INFER_ROWS * ldr * sizeof(uint16_t);
+    const int tiles = (W / 32 + 3) / 4;  // 32-column tiles per wave in the hidden layers
+    auto launch = [&](auto kernel) {
+        SBH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        hipLaunchKernelGGL(kernel, dim3((unsigned)(Mp / INFER_ROWS)), dim3(BLOCK), lds, net->ctx->stream, raw, a, ldr,
+                           net->Y.ptr);
+    };
+    if (tiles <= 1)
+        launch(mlp_infer_kernel<1>);
+    else if (tiles <= 2)
+        launch(mlp_infer_kernel<2>);
+    else if (tiles <= 4)
+        launch(mlp_infer_kernel<4>);
+    else
+        launch(mlp_infer_kernel<8>);
+    SBH_HIP(hipGetLastError());
+}
+
 double read_loss(sbh_mlp* net, int blocks) {
     std::vector<double> h(blocks);
     SBH_HIP(hipMemcpyAsync(h.data(), net->loss_part.ptr, blocks * sizeof(double), hipMemcpyDeviceToHost,
@@ -436,8 +564,7 @@ float* mlp_raw_workspace(sbh_mlp* net, int64_t M) {
 const float* mlp_forward_device(sbh_mlp* net, const float* raw, int64_t M) {
     SBH_REQUIRE(M > 0 && M <= (1 << 24), "sample count out of range");
     const int Mp = (int)((M + 255) / 256 * 256);
-    ensure_batch(net, Mp);
-    forward_backward(net, raw, nullptr, Mp, nullptr, false);
+    infer(net, raw, Mp);
     return net->Y.ptr;
 }
 
@@ -590,7 +717,7 @@ sbh_status sbh_mlp_train(sbh_mlp* net, int64_t steps, int64_t batch, double lr_m
     });
 }
 
-sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory) {
+sbh_status sbh_debug_mlp_predict_layered(sbh_mlp* net, const float* inputs, int64_t M, float* out) {
     return guard_on(ctx_of(net), [&] {
         SBH_REQUIRE(net && inputs && out, "null argument");
         SBH_REQUIRE(M > 0 && M <= (1 << 24), "M out of range");
@@ -599,10 +726,27 @@ sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* 
         ensure_batch(net, Mp);
         DeviceArray<float> din((size_t)Mp * RAW);
         SBH_HIP(hipMemsetAsync(din.ptr, 0, din.bytes(), net->ctx->stream));
+        SBH_HIP(hipMemcpyAsync(din.ptr, inputs, (size_t)M * RAW * sizeof(float), hipMemcpyHostToDevice,
+                               net->ctx->stream));
+        forward_backward(net, din.ptr, nullptr, Mp, nullptr, false);
+        SBH_HIP(hipMemcpyAsync(out, net->Y.ptr, (size_t)M * 2 * sizeof(float), hipMemcpyDeviceToHost,
+                               net->ctx->stream));
+        SBH_HIP(hipStreamSynchronize(net->ctx->stream));
+    });
+}
+
+sbh_status sbh_mlp_predict(sbh_mlp* net, const float* inputs, int64_t M, float* out, int memory) {
+    return guard_on(ctx_of(net), [&] {
+        SBH_REQUIRE(net && inputs && out, "null argument");
+        SBH_REQUIRE(M > 0 && M <= (1 << 24), "M out of range");
+        use_device(net->ctx);
+        const int Mp = (int)((M + 255) / 256 * 256);
+        DeviceArray<float> din((size_t)Mp * RAW);
+        SBH_HIP(hipMemsetAsync(din.ptr, 0, din.bytes(), net->ctx->stream));
         SBH_HIP(hipMemcpyAsync(din.ptr, inputs, (size_t)M * RAW * sizeof(float),
                                memory == SBH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                net->ctx->stream));
-        forward_backward(net, din.ptr, nullptr, Mp, nullptr, false);
+        infer(net, din.ptr, Mp);
         SBH_HIP(hipMemcpyAsync(out, net->Y.ptr, (size_t)M * 2 * sizeof(float),
                                memory == SBH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                net->ctx->stream));

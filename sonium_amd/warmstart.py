@@ -114,6 +114,13 @@ class WarmStartMLP:
         _check(_lib().sbh_mlp_predict(self._h, _cptr(x), x.shape[0], _cptr(out), 0))
         return out
 
+    def predict_layered(self, inputs) -> np.ndarray:
+        """predict() through the layer-by-layer forward pass of training (test hook)."""
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        out = np.empty((x.shape[0], 2), dtype=np.float32)
+        _check(_lib().sbh_debug_mlp_predict_layered(self._h, _cptr(x), x.shape[0], _cptr(out)))
+        return out
+
     def guess(self, centers, normals, k: float, direction) -> np.ndarray:
         """GMRES initial guess x0 for one (k, direction) on a mesh."""
         y = self.predict(panel_inputs(centers, normals, k, direction))

@@ -65,6 +65,24 @@ def test_predictions_match_oracle(ctx):
     assert rel_fro(y, y_ref) < 1e-2
 
 
+@pytest.mark.parametrize("width", [64, 512, 1024, 2048])
+def test_fused_inference_equals_layered_forward(ctx, width):
+    """The fused inference kernel (all layers per 32-sample workgroup,
+    activations in LDS) gives the layer-by-layer forward pass of training bit
+    for bit, on ragged sample counts and after training steps; widths above
+    1024 take the layered path."""
+    net = ws.WarmStartMLP(ctx, width=width, seed=21)
+    rng = np.random.default_rng(4)
+    for m in (1, 257, 3000):
+        raw = _raw(rng, m)
+        assert np.array_equal(net.predict(raw), net.predict_layered(raw))
+    net.set_dataset(_raw(rng, 4096), rng.standard_normal((4096, 2)).astype(np.float32))
+    net.train(5, batch=1024)
+    raw = _raw(rng, 777)
+    assert np.array_equal(net.predict(raw), net.predict_layered(raw))
+    net.close()
+
+
 def test_training_lowers_loss(ctx):
     net = ws.WarmStartMLP(ctx, width=512, seed=13)
     rng = np.random.default_rng(3)
