@@ -46,14 +46,15 @@ FLOPS_PER_EVAL = 70      # FP64 flops per kernel evaluation (FMA = 2, sqrt/div/s
 FLOPS_PER_COEFF = 14     # coefficient H*gamma*tau + E*beta per pair
 FIELD_FLOPS_PER_QP = 47  # field kernel, rigid panels: FP64 flops per (point, panel, quadrature point)
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
-# this workload (copy-calibrated; profiles/zgemv_n_kernel_and_far_kernel_c2_icosphere20480_k10.md).
 # C5 batched sweep: solved points per side in the minimal-residual guesses
 C5_NEIGHBOURS_PER_SIDE = 1
 
+# HBM bytes per launch (read + written) measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE on this workload, scaled by a 1 GiB copy (FETCH_SIZE x 2 KiB,
+# WRITE_SIZE x 1 KiB); profiles/zgemv_n_kernel_and_far_kernel_c2_icosphere20480_k10.md.
 MEASURED_TRAFFIC = {
-    ("zgemv_n_kernel", 20480): 6.7134e9 + 3.2971e5,
-    ("far_kernel", 20480): 1.8935e7 + 6.7109e9,
+    ("zgemv_n_kernel", 20480): 6.7136e9 + 3.29e5,
+    ("far_kernel", 20480): 1.309e8 + 6.7643e9,
 }
 
 WORKLOADS = {
