@@ -560,7 +560,11 @@ sbh_status sbh_block_gmres(sbh_operator* op, sbh_precond* precond, const sbh_c64
  * order (32-row tiles); sbh_lowp_matrix_download returns it row-major. */
 typedef struct sbh_lowp_matrix sbh_lowp_matrix;
 typedef enum { SBH_LOWP_FP16 = 0, SBH_LOWP_BF16 = 1 } sbh_lowp_format;
-/* Converts the system's current matrix; sys must outlive the copy. */
+/* Converts the system's current matrix.  The copy holds a reference to sys
+ * (sbh_system_destroy frees it after the last copy is destroyed); while a copy
+ * exists, sbh_workspace_assemble into sys also leaves the row maxima from
+ * which sbh_lowp_matrix_update takes its scales without a pass over the
+ * matrix. */
 sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix** out);
 /* Re-converts after the system was re-assembled. */
 sbh_status sbh_lowp_matrix_update(sbh_lowp_matrix* lp);

@@ -713,14 +713,18 @@ sbh_status sbh_system_create(sbh_context* ctx, int64_t num_dofs, sbh_system** ou
     });
 }
 
-void sbh_system_destroy(sbh_system* sys) {
-    if (!sys) return;
+static void system_release(sbh_system* sys) {
+    if (sys->refs.fetch_sub(1) != 1) return;
     sbh_context* ctx = sys->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     release_device(ctx, sys->admitted);
     delete sys;
     context_release(ctx);
+}
+
+void sbh_system_destroy(sbh_system* sys) {
+    if (sys) system_release(sys);
 }
 
 int64_t sbh_system_num_dofs(const sbh_system* sys) { return sys ? sys->n : 0; }
@@ -1490,6 +1494,8 @@ sbh_status sbh_lowp_matrix_create(sbh_system* sys, int format, sbh_lowp_matrix**
         sys->ctx->device_reserved += bytes;
         lp->admitted = bytes;
         context_retain(sys->ctx);
+        sys->lowp_copies += 1;
+        sys->refs.fetch_add(1);
         *out = lp.release();
     });
 }
@@ -1509,7 +1515,10 @@ void sbh_lowp_matrix_destroy(sbh_lowp_matrix* lp) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     release_device(ctx, lp->admitted);
+    sbh_system* sys = lp->sys;
+    sys->lowp_copies -= 1;
     delete lp;
+    system_release(sys);
     context_release(ctx);
 }
 

@@ -323,6 +323,12 @@ struct sbh_system {
     int64_t rowmax_stride = 0;
     bool rowmax_valid = false;
     bool matrix_exposed = false;
+    // Live low-precision copies of this system; the assembly writes the row
+    // maxima only while there is one to use them.  Each copy also holds a
+    // reference, so the system is freed after the last of sbh_system_destroy
+    // and its copies' destruction.
+    std::atomic<int> lowp_copies{0};
+    std::atomic<int> refs{1};
 };
 
 struct sbh_vector {

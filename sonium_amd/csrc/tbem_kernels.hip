@@ -642,18 +642,23 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
     const int64_t i0 = (int64_t)blockIdx.y * rows_per_block;
     const int64_t i1 = min(n, i0 + rows_per_block);
     const int64_t c0 = (int64_t)blockIdx.x * 256;
+    // This block's rows own one contiguous range of the row-sorted near-pair
+    // list.  Its first 512 pairs are loaded into registers here, so the loads
+    // complete behind the row loop; the near mask is built after it.
+    int64_t e0 = 0, e1 = 0;
+    int near_r[2] = {-1, -1}, near_c[2] = {0, 0};
     if (rowmax) {
-        // near columns of this block's rows: the rows' near pairs are one
-        // contiguous range of the row-sorted pair list, read in one pass
         if (threadIdx.x < FAR_MAX_ROWS * 8) nmask[threadIdx.x >> 3][threadIdx.x & 7] = 0u;
-        __syncthreads();
-        const int64_t e1 = row_ptr[i1];
-        for (int64_t e = row_ptr[i0] + threadIdx.x; e < e1; e += 256) {
-            const NearPair np = pairs[e];
-            const int64_t c = (int64_t)np.j - c0;
-            if (c >= 0 && c < 256) atomicOr(&nmask[np.i - i0][c >> 5], 1u << (c & 31));
+        e0 = row_ptr[i0];
+        e1 = row_ptr[i1];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int64_t e = e0 + threadIdx.x + 256 * u;
+            if (e < e1) {
+                near_r[u] = pairs[e].i - (int)i0;
+                near_c[u] = (int)((int64_t)pairs[e].j - c0);
+            }
         }
-        __syncthreads();
     }
     for (int64_t i = i0; i < i1; ++i) {
         const Source src = sources[i];
@@ -671,19 +676,31 @@ __global__ __launch_bounds__(256) void far_kernel(const FarPanel* __restrict__ p
             out = coefficient(a, bc, make_double2(1.0, 0.0), make_double2(tau_r, 0.0), beta, iwr);
         }
         if (active) A[i * lda + j] = out;
-        if (rowmax) {
-            const int r = (int)(i - i0);
-            const bool near = ((nmask[r][threadIdx.x >> 5] >> (threadIdx.x & 31)) & 1u) != 0u;
-            smax[r][threadIdx.x] = (active && !near && j != i) ? fmax(fabs(out.x), fabs(out.y)) : 0.0;
-        }
+        if (rowmax) smax[i - i0][threadIdx.x] = active ? fmax(fabs(out.x), fabs(out.y)) : 0.0;
     }
     if (rowmax) {
+        __syncthreads();  // the mask was cleared by every wave before its row loop
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (near_r[u] >= 0 && near_c[u] >= 0 && near_c[u] < 256)
+                atomicOr(&nmask[near_r[u]][near_c[u] >> 5], 1u << (near_c[u] & 31));
+        for (int64_t e = e0 + 512 + threadIdx.x; e < e1; e += 256) {
+            const NearPair np = pairs[e];
+            const int64_t c = (int64_t)np.j - c0;
+            if (c >= 0 && c < 256) atomicOr(&nmask[np.i - i0][c >> 5], 1u << (c & 31));
+        }
         __syncthreads();
+        // per row: the maximum over the columns that are neither near pairs nor the diagonal
         const int r = threadIdx.x >> 4, seg = threadIdx.x & 15;
         double m = 0.0;
         if (i0 + r < i1) {
+            const unsigned bits = nmask[r][seg >> 1] >> (16 * (seg & 1));
 #pragma unroll
-            for (int c = 0; c < 16; ++c) m = fmax(m, smax[r][seg * 16 + c]);
+            for (int c = 0; c < 16; ++c) {
+                const int col = seg * 16 + c;
+                const bool skip = ((bits >> c) & 1u) != 0u || c0 + col == i0 + r;
+                m = fmax(m, skip ? 0.0 : smax[r][col]);
+            }
         }
         __syncthreads();
         smax[r][seg] = m;
@@ -1766,10 +1783,11 @@ void launch_assembly(sbh_workspace* ws, const sbh_physics& ph, sbh_c64 beta_c, s
     int rows_per_block = 16;
     while ((n + rows_per_block - 1) / rows_per_block > 65535) rows_per_block *= 2;
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + rows_per_block - 1) / rows_per_block));
-    // row maxima for the low-precision copy, valid once every kernel below has run
+    // row maxima for a low-precision copy of this system (only while one
+    // exists), valid once every kernel below has run
     sys->rowmax_valid = false;
     const int64_t nbx = sys->rowmax_stride;
-    const bool row_max = rows_per_block <= FAR_MAX_ROWS && nbx == (int64_t)grid.x + 1 && sys->d_rowmax.ptr &&
+    const bool row_max = sys->lowp_copies.load() > 0 && rows_per_block <= FAR_MAX_ROWS && nbx == (int64_t)grid.x + 1 && sys->d_rowmax.ptr &&
                          ws->d_row_ptr.ptr && !sys->matrix_exposed;
     double* rm = row_max ? sys->d_rowmax.ptr : nullptr;
     hipEvent_t ev;

@@ -210,14 +210,16 @@ def test_lowp_copy_is_bit_exact(ctx, c1_block, fmt):
 
 @pytest.mark.parametrize("fmt", ["fp16", "bf16"])
 def test_row_scales_from_assembly_equal_the_matrix_pass(ctx, fmt):
-    """Right after a value assembly the copy takes its row scales from the
-    assembly kernels' row maxima (far blocks without their near pairs and
-    diagonal, near and diagonal entries folded in) instead of reading the
-    matrix twice.  The copy equals the oracle's storage of the downloaded
-    matrix bit for bit, rigid and mixed-BC meshes alike; after a derivative
-    assembly into the same system, and after the matrix's device address has
-    been handed out, the conversion reads the matrix instead and still equals
-    the oracle; a new value assembly gives the first copy again."""
+    """While a system has a low-precision copy, its value assembly leaves
+    row maxima (far blocks without their near pairs and diagonal, near and
+    diagonal entries folded in), and the next conversion takes its row scales
+    from them instead of reading the matrix twice.  The first copy (made after
+    an assembly that ran without a copy: the reading pass) equals the oracle's
+    storage of the downloaded matrix bit for bit, rigid and mixed-BC meshes
+    alike; a derivative assembly into the same system invalidates the maxima
+    and the reading pass again matches the oracle; a new value assembly (now
+    with maxima) gives the first copy again bit for bit, and so does the
+    reading pass once the matrix's device address has been handed out."""
     from oracle import lowp as olp
     from sonium_amd.mesh import ellipsoid
 
