@@ -92,6 +92,26 @@ def test_truncation_caps(sphere):
     assert total < 0.5  # truncated: part of the panel is dropped, as in the reference
 
 
+@pytest.mark.parametrize("src", [[0.3, 0.3, 1e-4], [0.05, 0.05, 1e-6], [0.5, 0.25, 3e-5], [0.9, 0.05, 0.01]])
+def test_truncation_keeps_level_order_and_the_per_level_cap(src):
+    """generate_subelements walks the tree level by level (singular.rs:1408-1540):
+    a level keeps its cells in order, subdivides at most 15 of them (ndie > 15
+    stops the level: the cells after the 16th that needs subdividing are
+    dropped) and stores the rest as leaves; the result stops at 110 leaves.  No
+    reference test covers the truncation, so these invariants pin it against
+    plausible mistakes: a depth-first walk breaks the level order, a missing
+    cap puts more than 4 * 15 = 60 cells on a level."""
+    coords = np.array([0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0])
+    leaves = oracle.generate_subelements(np.array(src, dtype=float), coords, 0.5)
+    depth = np.array([int(round(np.log(0.5 / _leaf_area(l)) / np.log(4.0))) for l in leaves])
+    assert np.all(np.diff(depth) >= 0)                      # level order
+    counts = np.bincount(depth)
+    assert counts.max() <= 60                               # at most 15 cells subdivided per level
+    assert leaves.shape[0] <= 110
+    total = sum(_leaf_area(l) for l in leaves)
+    assert total <= 0.5 + 1e-15
+
+
 def test_product_route_builder_matches_oracle(sphere):
     """The library's host route builder (no GPU needed) reproduces the oracle's leaves bit for bit."""
     m, c, a, coords = sphere
