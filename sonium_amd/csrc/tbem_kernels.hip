@@ -549,6 +549,16 @@ __device__ __forceinline__ FarPoints far_points(const FarPanel& P) {
 // each normal projection, after six dot products per pair.  Off the diagonal
 // (which the self-panel rule overwrites) the point-to-point distance is at
 // least a fraction of the panel size, so the expansion loses no accuracy.
+//
+// The channel sums are grouped by powers of k, and the panel factor 0.5 J/(4 pi)
+// is applied once per pair.  With zg = w_q e^{ikr}/r, pn = r^.n_y,
+// qn = -r^.n_x, rq = pn qn and t = 3 rq + n_x.n_y:
+//   H  = sum zg (ik - 1/r) pn                       = -S1 + ik S2,
+//   Ht = sum zg (ik - 1/r) qn                       = -Q1 + ik Q2,
+//   E  = sum zg [(3/r^2 - k^2) rq + n_x.n_y/r^2 - ik t/r] = T1 - k^2 T2 - ik T3,
+// with S1 = sum zg pn/r, S2 = sum zg pn, Q1 = sum zg qn/r, Q2 = sum zg qn,
+// T1 = sum zg t/r^2, T2 = sum zg rq, T3 = sum zg t/r: per point one product
+// and two FMAs per sum instead of the complex kernel factors.
 template <int MODE>
 __device__ __forceinline__ void far_sums(const FarPanel& P, const FarPoints& F, const Source& src, double k,
                                          double k2, double& hr, double& hi, double& er, double& ei, double& gr,
@@ -564,7 +574,10 @@ __device__ __forceinline__ void far_sums(const FarPanel& P, const FarPoints& F, 
     const double g0 = -(rel0 * src.n[0] + rel1 * src.n[1] + rel2 * src.n[2]);
     const double g1 = -(P.e1[0] * src.n[0] + P.e1[1] * src.n[1] + P.e1[2] * src.n[2]);
     const double g2 = -(P.e2[0] * src.n[0] + P.e2[1] * src.n[1] + P.e2[2] * src.n[2]);
-    hr = hi = er = ei = gr = gi = tr = ti = 0.0;
+    double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+    double t1r = 0.0, t1i = 0.0, t2r = 0.0, t2i = 0.0, t3r = 0.0, t3i = 0.0;
+    double q1r = 0.0, q1i = 0.0, q2r = 0.0, q2i = 0.0;
+    gr = gi = 0.0;
     // TR13 constants are uniform: scalar loads from constant memory.
 #pragma clang loop unroll_count(2)
     for (int q = 0; q < 13; ++q) {
@@ -579,25 +592,43 @@ __device__ __forceinline__ void far_sums(const FarPanel& P, const FarPoints& F, 
         const double qn = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0)) * inv;    // -r^.n_x
         double sn, cs;
         sincos_red(k * dis, &sn, &cs);
-        const double re2 = (kTrW[q] * P.wj) * inv;
+        const double re2 = kTrW[q] * inv;
         const double zgr = cs * re2, zgi = sn * re2;
-        // zb = zg * (-1/r + i k)
-        const double zbr = -zgr * inv - zgi * k;
-        const double zbi = zgr * k - zgi * inv;
-        hr = __builtin_fma(zbr, pn, hr);
-        hi = __builtin_fma(zbi, pn, hi);
-        const double inv2 = inv * inv;
+        const double pi = pn * inv;
+        s1r = __builtin_fma(zgr, pi, s1r);
+        s1i = __builtin_fma(zgi, pi, s1i);
+        s2r = __builtin_fma(zgr, pn, s2r);
+        s2i = __builtin_fma(zgi, pn, s2i);
         const double rq = pn * qn;
-        const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
-        const double efi = -k * inv * (3.0 * rq + nxny);
-        er += zgr * efr - zgi * efi;
-        ei += zgr * efi + zgi * efr;
+        const double t = __builtin_fma(3.0, rq, nxny);
+        const double ti_ = t * inv;
+        const double ti2 = ti_ * inv;
+        t1r = __builtin_fma(zgr, ti2, t1r);
+        t1i = __builtin_fma(zgi, ti2, t1i);
+        t2r = __builtin_fma(zgr, rq, t2r);
+        t2i = __builtin_fma(zgi, rq, t2i);
+        t3r = __builtin_fma(zgr, ti_, t3r);
+        t3i = __builtin_fma(zgi, ti_, t3i);
         if (MODE == 1) {
             gr += zgr;
             gi += zgi;
-            tr = __builtin_fma(zbr, qn, tr);
-            ti = __builtin_fma(zbi, qn, ti);
+            const double qi = qn * inv;
+            q1r = __builtin_fma(zgr, qi, q1r);
+            q1i = __builtin_fma(zgi, qi, q1i);
+            q2r = __builtin_fma(zgr, qn, q2r);
+            q2i = __builtin_fma(zgi, qn, q2i);
         }
+    }
+    const double w = P.wj;
+    hr = -w * __builtin_fma(k, s2i, s1r);
+    hi = w * __builtin_fma(k, s2r, -s1i);
+    er = w * __builtin_fma(k, t3i, __builtin_fma(-k2, t2r, t1r));
+    ei = w * __builtin_fma(-k, t3r, __builtin_fma(-k2, t2i, t1i));
+    if (MODE == 1) {
+        gr *= w;
+        gi *= w;
+        tr = -w * __builtin_fma(k, q2i, q1r);
+        ti = w * __builtin_fma(k, q2r, -q1i);
     }
 }
 
