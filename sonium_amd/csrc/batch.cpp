@@ -580,6 +580,7 @@ sbh_status sbh_batch_set_problem(sbh_batch* b, const sbh_physics* physics, const
         }
         zero_on_stream(b->ctx, b->d_x.ptr, b->d_x.bytes());
         b->lp_format = -2;
+        b->cand_m = 0;  // gathered guess candidates belong to the previous problem
         build_items(b);
         build_chunks(b);  // also drops the solver workspace and its graph
         build_tiles(b);

@@ -344,6 +344,12 @@ def test_device_guesses_match_host_guesses(ctx):
     with pytest.raises(sa.SbhError):
         b.set_guesses_device([host.ctypes.data, None, None, None])
 
+    # a new problem invalidates the gathered candidates
+    b.set_problem([float(ks[i]) for i in idx], [d.reshape(1, 3)] * 4)
+    with pytest.raises(sa.SbhError):
+        b.guess_combine(np.ones((4, 2), dtype=np.complex128))
+    b.assemble()
+
     net = wsm.WarmStartMLP(ctx, width=512, seed=5)
     b.learned_guesses(net)
     Xl = np.stack([x[:, 0] for x in b.solutions()])
