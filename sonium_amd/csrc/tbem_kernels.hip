@@ -866,11 +866,19 @@ __global__ __launch_bounds__(256) void self_kernel(const PanelGeom* __restrict__
 // routes and launch structure as the value assembly, each pair producing its
 // values b and wavenumber derivatives d from one pass over its points.
 
-// Far sums with derivatives (fast arithmetic, harmonic factor +1).  With
-// dG/dk = i r G, d/dk[G (ik - 1/r)] = -k r G, so dH = -k r zg (r.n_y) and
-// dHt = -k r zg (-r.n_x); dE = dG ef + G def with def = (-2k rq, -(3rq + n.n)/r).
-// MODE 0 (velocity panels without admittance) needs E, dH, dE; MODE 1 also G,
-// Ht, dG, dHt.
+// Far sums with derivatives (fast arithmetic, harmonic factor +1), grouped by
+// powers of k as in far_sums.  With zg = w_q e^{ikr}/r, pn = r^.n_y,
+// qn = -r^.n_x, rq = pn qn, t = 3 rq + n_x.n_y, g = qn r (so that pn r = pr,
+// the pair constant (v0 - x).n_y of a flat panel), dG/dk = i r G and
+// d/dk[G (ik - 1/r)] = -k r G:
+//   E   = T1 - k^2 T2 - ik T3                 (T1 = sum zg t/r^2, T2 = sum zg rq, T3 = sum zg t/r)
+//   dH  = -k pr G                             (G = sum zg)
+//   dE  = sum [i r zg ef + zg def] = k (T2 + n_x.n_y G) - i k^2 U,  U = sum zg r rq,
+// where ef = t/r^2 - k^2 rq - ik t/r and def = -2k rq - i t/r (the i t/r
+// terms cancel); MODE 1 also G, Ht = -Q1 + ik Q2 (Q1 = sum zg qn/r, Q2 = sum zg qn),
+// dG = i D (D = sum zg r) and dHt = -k Q3 (Q3 = sum zg g).  MODE 0 (velocity
+// panels without admittance) needs E, dH, dE.  The panel factor is applied
+// once per pair.
 template <int MODE>
 // Same geometry as far_sums (distances from the edge Gram entries, normal
 // projections from per-pair dot products).
@@ -887,8 +895,8 @@ __device__ __forceinline__ void far_sums_d(const FarPanel& P, const FarPoints& F
     const double g0 = -(rel0 * src.n[0] + rel1 * src.n[1] + rel2 * src.n[2]);
     const double g1 = -(P.e1[0] * src.n[0] + P.e1[1] * src.n[1] + P.e1[2] * src.n[2]);
     const double g2 = -(P.e2[0] * src.n[0] + P.e2[1] * src.n[1] + P.e2[2] * src.n[2]);
-    b = acc_zero();
-    d = acc_zero();
+    double gsr = 0.0, gsi = 0.0, t1r = 0.0, t1i = 0.0, t2r = 0.0, t2i = 0.0, t3r = 0.0, t3i = 0.0, ur = 0.0, ui = 0.0;
+    double q1r = 0.0, q1i = 0.0, q2r = 0.0, q2i = 0.0, q3r = 0.0, q3i = 0.0, dsr = 0.0, dsi = 0.0;
 #pragma unroll 2
     for (int q = 0; q < 13; ++q) {
         const double sq = kTrS[q], tq = kTrT[q];
@@ -898,39 +906,53 @@ __device__ __forceinline__ void far_sums_d(const FarPanel& P, const FarPoints& F
         const double inv = rsqrt_nr(d2);
         const double dis = d2 * inv;
         const double pn = pr * inv;
-        const double qn = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0)) * inv;
+        const double gq = __builtin_fma(sq, g1, __builtin_fma(tq, g2, g0));  // qn r
+        const double qn = gq * inv;
         double sn, cs;
         sincos_red(k * dis, &sn, &cs);
-        const double re2 = (kTrW[q] * P.wj) * inv;
+        const double re2 = kTrW[q] * inv;
         const double zgr = cs * re2, zgi = sn * re2;
-        const double inv2 = inv * inv;
+        gsr += zgr;
+        gsi += zgi;
         const double rq = pn * qn;
-        const double t3 = 3.0 * rq + nxny;
-        const double efr = (3.0 * inv2 - k2) * rq + nxny * inv2;
-        const double defi = -inv * t3;
-        const double efi = k * defi;
-        const double defr = -2.0 * k * rq;
-        const double dzgr = -zgi * dis, dzgi = zgr * dis;
-        b.e.x += zgr * efr - zgi * efi;
-        b.e.y += zgr * efi + zgi * efr;
-        const double kd = -k * dis;
-        const double hr = kd * zgr, hi = kd * zgi;  // d/dk of zg (ik - 1/r)
-        d.h.x = __builtin_fma(hr, pn, d.h.x);
-        d.h.y = __builtin_fma(hi, pn, d.h.y);
-        d.e.x += (dzgr * efr - dzgi * efi) + (zgr * defr - zgi * defi);
-        d.e.y += (dzgr * efi + dzgi * efr) + (zgr * defi + zgi * defr);
+        const double t = __builtin_fma(3.0, rq, nxny);
+        const double ti1 = t * inv;
+        const double ti2 = ti1 * inv;
+        t1r = __builtin_fma(zgr, ti2, t1r);
+        t1i = __builtin_fma(zgi, ti2, t1i);
+        t2r = __builtin_fma(zgr, rq, t2r);
+        t2i = __builtin_fma(zgi, rq, t2i);
+        t3r = __builtin_fma(zgr, ti1, t3r);
+        t3i = __builtin_fma(zgi, ti1, t3i);
+        const double drq = dis * rq;
+        ur = __builtin_fma(zgr, drq, ur);
+        ui = __builtin_fma(zgi, drq, ui);
         if (MODE == 1) {
-            const double zbr = -zgr * inv - zgi * k;
-            const double zbi = zgr * k - zgi * inv;
-            b.g.x += zgr;
-            b.g.y += zgi;
-            b.ht.x = __builtin_fma(zbr, qn, b.ht.x);
-            b.ht.y = __builtin_fma(zbi, qn, b.ht.y);
-            d.g.x += dzgr;
-            d.g.y += dzgi;
-            d.ht.x = __builtin_fma(hr, qn, d.ht.x);
-            d.ht.y = __builtin_fma(hi, qn, d.ht.y);
+            const double qi = qn * inv;
+            q1r = __builtin_fma(zgr, qi, q1r);
+            q1i = __builtin_fma(zgi, qi, q1i);
+            q2r = __builtin_fma(zgr, qn, q2r);
+            q2i = __builtin_fma(zgi, qn, q2i);
+            q3r = __builtin_fma(zgr, gq, q3r);
+            q3i = __builtin_fma(zgi, gq, q3i);
+            dsr = __builtin_fma(zgr, dis, dsr);
+            dsi = __builtin_fma(zgi, dis, dsi);
         }
+    }
+    const double w = P.wj;
+    b = acc_zero();
+    d = acc_zero();
+    b.e = make_double2(w * __builtin_fma(k, t3i, __builtin_fma(-k2, t2r, t1r)),
+                       w * __builtin_fma(-k, t3r, __builtin_fma(-k2, t2i, t1i)));
+    const double kpw = -k * pr * w;
+    d.h = make_double2(kpw * gsr, kpw * gsi);
+    d.e = make_double2(w * __builtin_fma(k, __builtin_fma(nxny, gsr, t2r), k2 * ui),
+                       w * __builtin_fma(k, __builtin_fma(nxny, gsi, t2i), -k2 * ur));
+    if (MODE == 1) {
+        b.g = make_double2(w * gsr, w * gsi);
+        b.ht = make_double2(-w * __builtin_fma(k, q2i, q1r), w * __builtin_fma(k, q2r, -q1i));
+        d.g = make_double2(-w * dsi, w * dsr);
+        d.ht = make_double2(-k * w * q3r, -k * w * q3i);
     }
 }
 
