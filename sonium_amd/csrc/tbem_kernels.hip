@@ -1515,33 +1515,31 @@ __global__ __launch_bounds__(256) void exact_far_kernel(const FarPanel* __restri
     const int wave = threadIdx.x >> 6;
     const int64_t i0 = (int64_t)blockIdx.x * EXACT_ROWS;
     const double k2 = k * k;
-    double2 acc[EXACT_ROWS];
-#pragma unroll
-    for (int r = 0; r < EXACT_ROWS; ++r) acc[r] = make_double2(0.0, 0.0);
-    for (int64_t jc = 0; jc < n; jc += 256) {
-        const int64_t j = jc + threadIdx.x;
-        const bool active = j < n;
-        const int64_t jj = active ? j : n - 1;
-        const FarPanel P = panels[jj];
-        const FarPoints F = far_points(P);
-        PanelBc bc;
-        if (MODE == 1) bc = bcs[jj];
-        const double2 xj = x[jj];
-#pragma unroll
-        for (int r = 0; r < EXACT_ROWS; ++r) {
-            const int64_t i = min(i0 + r, n - 1);  // rows past n are computed and dropped
-            const double2 v = far_entry<MODE>(P, F, sources[i], bc, k, k2, tau_r, beta, iwr);
+    // rows one after another, each over every column chunk in order (the
+    // panels are L2-resident), so one row's point loop is live at a time
+#pragma clang loop unroll(disable)
+    for (int r = 0; r < EXACT_ROWS; ++r) {
+        const int64_t i = min(i0 + r, n - 1);  // rows past n are computed and dropped
+        const Source S = sources[i];
+        double2 acc = make_double2(0.0, 0.0);
+        for (int64_t jc = 0; jc < n; jc += 256) {
+            const int64_t j = jc + threadIdx.x;
+            const bool active = j < n;
+            const int64_t jj = active ? j : n - 1;
+            const FarPanel P = panels[jj];
+            const FarPoints F = far_points(P);
+            PanelBc bc;
+            if (MODE == 1) bc = bcs[jj];
+            const double2 xj = x[jj];
+            const double2 v = far_entry<MODE>(P, F, S, bc, k, k2, tau_r, beta, iwr);
             if (active && j != i) {
-                acc[r].x += v.x * xj.x - v.y * xj.y;
-                acc[r].y += v.x * xj.y + v.y * xj.x;
+                acc.x += v.x * xj.x - v.y * xj.y;
+                acc.y += v.x * xj.y + v.y * xj.x;
             }
         }
-    }
-#pragma unroll
-    for (int r = 0; r < EXACT_ROWS; ++r) {
-        acc[r].x = wave_sum(acc[r].x);
-        acc[r].y = wave_sum(acc[r].y);
-        if (lane == 0) red[wave][r] = acc[r];
+        acc.x = wave_sum(acc.x);
+        acc.y = wave_sum(acc.y);
+        if (lane == 0) red[wave][r] = acc;
     }
     __syncthreads();
     if (threadIdx.x < EXACT_ROWS && i0 + threadIdx.x < n) {
